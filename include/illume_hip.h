@@ -1,0 +1,113 @@
+/*
+ * illume_hip.h — C ABI of the MI355X ray-tracing library (librs_illume_amd.so).
+ *
+ * This is the drop-in boundary for rs_illume's hot path. Each entry point names
+ * the reference interface it replaces (file:line in mlunnay/rs_illume). No
+ * C++ or torch types cross it: plain pointers, sizes and an opaque hipStream_t
+ * passed as void*. Every call returns an illume_status; nothing unwinds across
+ * the ABI. Ray/hit buffers are caller-owned DEVICE pointers; the BVH handle
+ * owns its device memory and is immutable after creation, so concurrent
+ * queries on different streams are safe.
+ */
+#ifndef ILLUME_HIP_H
+#define ILLUME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (0 = success). core/primitive/mod.rs:6-17 returns Option/bool and
+ * panics on broken invariants; here a miss is data (ILLUME_MISS), errors are codes. */
+typedef enum {
+    ILLUME_OK = 0,
+    ILLUME_ERR_INVALID_ARG = 1,   /* null pointer, index out of range, bad size */
+    ILLUME_ERR_HIP = 2,           /* a HIP runtime call failed */
+    ILLUME_ERR_OUT_OF_MEMORY = 3, /* device allocation failed */
+    ILLUME_ERR_UNSUPPORTED = 4,   /* parameter outside what the library implements */
+    ILLUME_ERR_QUEUE_OVERFLOW = 5 /* a work queue was too small (renderer) */
+} illume_status;
+
+#define ILLUME_MISS 0xFFFFFFFFu
+
+/* core/geometry/ray.rs:9-31 (origin, direction, t_max, time); 32 bytes.
+ * Medium and differentials are not part of the hot path. */
+typedef struct {
+    float o[3];
+    float d[3];
+    float t_max;
+    float time;
+} illume_ray;
+
+/* Closest hit record; 16 bytes. prim_id indexes the caller's primitive order
+ * (triangles 0..n_tris-1, then spheres), not the BVH's internal order
+ * (accelerators/bvh.rs:58). ILLUME_MISS on a miss. b1, b2 are the triangle
+ * barycentrics of vertices 1 and 2 (0 for spheres). */
+typedef struct {
+    uint32_t prim_id;
+    float t;
+    float b1;
+    float b2;
+} illume_hit;
+
+typedef struct illume_bvh illume_bvh;
+
+typedef struct {
+    uint64_t n_prims;
+    uint64_t n_inner_nodes;  /* 64-byte two-child nodes */
+    uint64_t n_leaves;
+    uint64_t max_depth;
+    uint64_t device_bytes;   /* nodes + primitive records */
+    double build_seconds;
+    float bounds[6];         /* scene bounds: min xyz, max xyz */
+} illume_bvh_info_t;
+
+/* Build a BVH over triangles (world-space vertex array + index triples) and
+ * full spheres (cx, cy, cz, radius) and upload it to the current device.
+ * Replaces BVHAccel::new (accelerators/bvh.rs:32-93) with
+ * SplitMethod::SAH (bvh.rs:17-22) over GeometricPrimitive(Triangle|Sphere)
+ * (core/primitive/geometric_primitive.rs:8-30). max_prims_in_node: 1..15.
+ * Host pointers; verts/tri_idx may be null when n_tris == 0, spheres when
+ * n_spheres == 0. */
+int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                      const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out);
+
+/* Frees the handle and its device memory (Drop of BVHAccel). */
+int illume_bvh_destroy(illume_bvh *bvh);
+
+/* Size and shape of a built BVH (the reference's BVH stats counters,
+ * accelerators/bvh.rs:676-720). */
+int illume_bvh_info(const illume_bvh *bvh, illume_bvh_info_t *out);
+
+/* Closest hit for n rays: batched Primitive::intersect on the aggregate
+ * (core/primitive/mod.rs:11, accelerators/bvh.rs:563-612, reached through
+ * Scene::intersect core/scene.rs:43-47). Each ray's t_max bounds the search.
+ * rays, hits: device pointers. stream: hipStream_t (null = default stream). */
+int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
+                             void *stream);
+
+/* Any hit for n rays: batched Primitive::intersect_p (core/primitive/mod.rs:12,
+ * accelerators/bvh.rs:614-662, via Scene::intersect_p core/scene.rs:49-53).
+ * occluded[i] = 1 if any primitive is hit in (0, t_max], else 0. */
+int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
+                         void *stream);
+
+/* Instrumented closest/any traversal that also counts, per ray, node-box tests
+ * and primitive tests (the reference's "BVH/..." and "Intersections/..."
+ * counters, core/stats_accumulator.rs). counts: device uint32[2*n] (node
+ * tests, primitive tests) per ray. any != 0 selects the any-hit traversal. */
+int illume_intersect_counts(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, int any, uint32_t *counts,
+                            void *stream);
+
+/* Human-readable name of a status code. */
+const char *illume_status_string(int status);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char *illume_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ILLUME_HIP_H */

@@ -1,0 +1,181 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper around ``oracle/liboracle.so``, the CPU restatement of
+rs_illume's ray-tracing hot path. Only ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` may import this module, and only as
+the checker / baseline; the product package never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("t_max", "<f4"), ("time", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
+MISS = 0xFFFFFFFF
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        vp, u32, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64
+        L.orc_scene_create.restype = vp
+        L.orc_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, i32, i32]
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_info.argtypes = [vp, vp]
+        L.orc_scene_nodes.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_intersect.argtypes = [vp, vp, u64, vp, i32, i32, vp]
+        L.orc_intersect_p.argtypes = [vp, vp, u64, vp, i32, i32, vp]
+        L.orc_triangle_test.argtypes = [vp, vp, vp]
+        L.orc_triangle_test.restype = i32
+        L.orc_sphere_intersect.argtypes = [vp, vp, vp]
+        L.orc_sphere_intersect.restype = i32
+        L.orc_quadratic.argtypes = [vp, vp]
+        L.orc_quadratic.restype = i32
+        L.orc_gamma.argtypes = [i32]
+        L.orc_gamma.restype = ctypes.c_float
+        L.orc_next_float_up.argtypes = [ctypes.c_float]
+        L.orc_next_float_up.restype = ctypes.c_float
+        L.orc_next_float_down.argtypes = [ctypes.c_float]
+        L.orc_next_float_down.restype = ctypes.c_float
+        L.orc_offset_ray_origin.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_pcg32.argtypes = [u64, i64, u32, vp]
+        L.orc_pcg32_float.argtypes = [u64, u32, vp]
+        L.orc_hw_threads.restype = i32
+        L.orc_prim_intersect.argtypes = [vp, u32, vp, vp]
+        L.orc_prim_intersect.restype = i32
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+def n_threads_default():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class OracleScene:
+    """pbrt-v3 SAH BVH over triangles (+ spheres) built by the oracle."""
+
+    def __init__(self, verts, idx, spheres=None, max_prims_in_node=4, reverse_orientation=False):
+        self.verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        sph = np.zeros((0, 4), np.float32) if spheres is None else spheres
+        self.spheres = np.ascontiguousarray(sph, dtype=np.float32).reshape(-1, 4)
+        self.h = lib().orc_scene_create(_p(self.verts), len(self.verts), _p(self.idx), len(self.idx),
+                                        _p(self.spheres), len(self.spheres), int(max_prims_in_node),
+                                        int(bool(reverse_orientation)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().orc_scene_destroy(h)
+            self.h = None
+
+    def info(self):
+        out = np.zeros(3, np.int64)
+        lib().orc_scene_info(self.h, _p(out))
+        return {"nodes": int(out[0]), "max_depth": int(out[1]), "prims": int(out[2])}
+
+    def nodes(self):
+        n = self.info()["nodes"]
+        bounds = np.zeros((n, 6), np.float32)
+        off = np.zeros(n, np.int32)
+        npr = np.zeros(n, np.int32)
+        axis = np.zeros(n, np.int32)
+        ordered = np.zeros(self.info()["prims"], np.uint32)
+        lib().orc_scene_nodes(self.h, _p(bounds), _p(off), _p(npr), _p(axis), _p(ordered))
+        return bounds, off, npr, axis, ordered
+
+    def intersect(self, rays, brute=False, n_threads=None, with_stats=False):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        st = np.zeros(2, np.uint64)
+        lib().orc_intersect(self.h, _p(rays), len(rays), _p(hits), int(brute),
+                            n_threads or n_threads_default(), _p(st))
+        return (hits, st) if with_stats else hits
+
+    def prim_intersect(self, prim, ray):
+        """(hit, t) of one primitive against one ray (closest-hit semantics)."""
+        r = np.ascontiguousarray(ray, dtype=RAY_DTYPE).reshape(1)
+        t = np.zeros(1, np.float32)
+        ok = lib().orc_prim_intersect(self.h, int(prim), _p(r), _p(t))
+        return bool(ok), float(t[0])
+
+    def intersect_p(self, rays, brute=False, n_threads=None, with_stats=False):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint8)
+        st = np.zeros(2, np.uint64)
+        lib().orc_intersect_p(self.h, _p(rays), len(rays), _p(occ), int(brute),
+                              n_threads or n_threads_default(), _p(st))
+        return (occ, st) if with_stats else occ
+
+
+def make_ray(o, d, t_max=np.inf, time=0.0):
+    r = np.zeros(1, RAY_DTYPE)
+    r["o"] = o
+    r["d"] = d
+    r["t_max"] = t_max
+    r["time"] = time
+    return r
+
+
+def triangle_test(p0, p1, p2, ray):
+    p = np.ascontiguousarray(np.concatenate([p0, p1, p2]).astype(np.float32))
+    out = np.zeros(4, np.float32)
+    ok = lib().orc_triangle_test(_p(p), _p(ray), _p(out))
+    return (bool(ok), out)
+
+
+def sphere_intersect(center, radius, ray):
+    c = np.array([*center, radius], np.float32)
+    out = np.zeros(10, np.float32)
+    ok = lib().orc_sphere_intersect(_p(c), _p(ray), _p(out))
+    return (bool(ok), out)
+
+
+def quadratic(a, a_err, b, b_err, c, c_err):
+    v = np.array([a, a_err, b, b_err, c, c_err], np.float32)
+    out = np.zeros(6, np.float32)
+    ok = lib().orc_quadratic(_p(v), _p(out))
+    return bool(ok), out
+
+
+def pcg32(seq, n, skip=0):
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(seq, skip, n, _p(out))
+    return out
+
+
+def pcg32_float(seq, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_pcg32_float(seq, n, _p(out))
+    return out
+
+
+def gamma(n):
+    return lib().orc_gamma(n)
+
+
+def next_float_up(v):
+    return lib().orc_next_float_up(v)
+
+
+def next_float_down(v):
+    return lib().orc_next_float_down(v)
+
+
+def offset_ray_origin(p, pe, n, w):
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (p, pe, n, w)]
+    out = np.zeros(3, np.float32)
+    lib().orc_offset_ray_origin(*[_p(a) for a in arrs], _p(out))
+    return out

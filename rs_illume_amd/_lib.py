@@ -1,0 +1,72 @@
+"""ctypes binding of librs_illume_amd.so (the C ABI in include/illume_hip.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librs_illume_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "illume_hip.h")
+
+_lib = None
+
+
+class IllumeError(RuntimeError):
+    def __init__(self, status, fn, msg):
+        super().__init__("%s failed: status %d (%s): %s" % (fn, status, status_string(status), msg))
+        self.status = status
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("n_prims", ctypes.c_uint64), ("n_inner_nodes", ctypes.c_uint64), ("n_leaves", ctypes.c_uint64),
+                ("max_depth", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64), ("build_seconds", ctypes.c_double),
+                ("bounds", ctypes.c_float * 6)]
+
+
+def _declare(L):
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    sig = {
+        "illume_bvh_create": (i32, [vp, u32, vp, u32, vp, u32, i32, ctypes.POINTER(vp)]),
+        "illume_bvh_destroy": (i32, [vp]),
+        "illume_bvh_info": (i32, [vp, ctypes.POINTER(BvhInfo)]),
+        "illume_intersect_closest": (i32, [vp, vp, u32, vp, vp]),
+        "illume_intersect_any": (i32, [vp, vp, u32, vp, vp]),
+        "illume_intersect_counts": (i32, [vp, vp, u32, i32, vp, vp]),
+        "illume_status_string": (ctypes.c_char_p, [i32]),
+        "illume_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librs_illume_amd.so is not built: run `python -m rs_illume_amd.build_lib` "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def status_string(status):
+    return lib().illume_status_string(status).decode()
+
+
+def check(status, fn):
+    if status != 0:
+        raise IllumeError(status, fn, lib().illume_last_error().decode())
+
+
+def header_functions(path=HEADER):
+    """Names of the functions declared in include/illume_hip.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(illume_[a-z_0-9]+)\s*\(", text)))

@@ -1,0 +1,157 @@
+// C ABI entry points (include/illume_hip.h). Status codes, never exceptions,
+// cross the boundary; the message of the last failure is kept per host thread.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include "../../include/illume_hip.h"
+#include "bvh_builder.h"
+#include "kernels.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? ILLUME_ERR_OUT_OF_MEMORY : ILLUME_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace
+
+struct illume_bvh {
+    illume::GpuNode *d_nodes = nullptr;
+    illume::PrimRecord *d_prims = nullptr;
+    illume_bvh_info_t info;
+    int device = 0;
+};
+
+extern "C" {
+
+const char *illume_status_string(int status) {
+    switch (status) {
+        case ILLUME_OK: return "ok";
+        case ILLUME_ERR_INVALID_ARG: return "invalid argument";
+        case ILLUME_ERR_HIP: return "HIP runtime error";
+        case ILLUME_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case ILLUME_ERR_UNSUPPORTED: return "unsupported parameter";
+        case ILLUME_ERR_QUEUE_OVERFLOW: return "work queue overflow";
+        default: return "unknown status";
+    }
+}
+
+const char *illume_last_error(void) { return g_last_error.c_str(); }
+
+int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                      const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out) {
+    if (!out) return fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (max_prims_in_node < 1 || max_prims_in_node > illume::kMaxLeafPrims)
+        return fail(ILLUME_ERR_UNSUPPORTED, "max_prims_in_node must be in 1..15");
+    auto t0 = std::chrono::steady_clock::now();
+    illume::BuildInput in;
+    in.verts = verts;
+    in.n_verts = n_verts;
+    in.idx = tri_idx;
+    in.n_tris = n_tris;
+    in.spheres = spheres;
+    in.n_spheres = n_spheres;
+    in.max_prims_in_node = max_prims_in_node;
+    illume::BuildOutput bo;
+    std::string err;
+    try {
+        err = illume::build_bvh(in, &bo);
+    } catch (const std::bad_alloc &) {
+        return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
+    }
+    if (!err.empty()) return fail(ILLUME_ERR_INVALID_ARG, err);
+    if ((int)bo.max_depth >= illume::max_supported_depth())
+        return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    auto t1 = std::chrono::steady_clock::now();
+
+    illume_bvh *h = new (std::nothrow) illume_bvh();
+    if (!h) return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::memset(&h->info, 0, sizeof(h->info));
+    hipGetDevice(&h->device);
+    size_t nb = bo.nodes.size() * sizeof(illume::GpuNode);
+    size_t pb = bo.prims.size() * sizeof(illume::PrimRecord);
+    hipError_t e;
+    if (nb) {
+        if ((e = hipMalloc(&h->d_nodes, nb)) != hipSuccess) {
+            delete h;
+            return hip_fail(e, "hipMalloc(nodes)");
+        }
+        if ((e = hipMemcpy(h->d_nodes, bo.nodes.data(), nb, hipMemcpyHostToDevice)) != hipSuccess) {
+            illume_bvh_destroy(h);
+            return hip_fail(e, "hipMemcpy(nodes)");
+        }
+    }
+    if (pb) {
+        if ((e = hipMalloc(&h->d_prims, pb)) != hipSuccess) {
+            illume_bvh_destroy(h);
+            return hip_fail(e, "hipMalloc(prims)");
+        }
+        if ((e = hipMemcpy(h->d_prims, bo.prims.data(), pb, hipMemcpyHostToDevice)) != hipSuccess) {
+            illume_bvh_destroy(h);
+            return hip_fail(e, "hipMemcpy(prims)");
+        }
+    }
+    h->info.n_prims = (uint64_t)n_tris + n_spheres;
+    h->info.n_inner_nodes = bo.nodes.size();
+    h->info.n_leaves = bo.n_leaves;
+    h->info.max_depth = bo.max_depth;
+    h->info.device_bytes = nb + pb;
+    h->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    std::memcpy(h->info.bounds, bo.bounds, sizeof(bo.bounds));
+    *out = h;
+    return ILLUME_OK;
+}
+
+int illume_bvh_destroy(illume_bvh *bvh) {
+    if (!bvh) return ILLUME_OK;
+    if (bvh->d_nodes) hipFree(bvh->d_nodes);
+    if (bvh->d_prims) hipFree(bvh->d_prims);
+    delete bvh;
+    return ILLUME_OK;
+}
+
+int illume_bvh_info(const illume_bvh *bvh, illume_bvh_info_t *out) {
+    if (!bvh || !out) return fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    *out = bvh->info;
+    return ILLUME_OK;
+}
+
+static int run_traverse(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
+                        uint8_t *occluded, uint32_t *counts, bool any, void *stream) {
+    if (!bvh) return fail(ILLUME_ERR_INVALID_ARG, "bvh is null");
+    if (n == 0) return ILLUME_OK;
+    if (!rays) return fail(ILLUME_ERR_INVALID_ARG, "rays is null");
+    if (!hits && !occluded && !counts) return fail(ILLUME_ERR_INVALID_ARG, "output is null");
+    hipError_t e = illume::launch_traverse(bvh->d_nodes, bvh->d_prims, bvh->info.n_prims == 0, rays, n, hits,
+                                           occluded, counts, any, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "traversal launch");
+    return ILLUME_OK;
+}
+
+int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
+                             void *stream) {
+    if (n && !hits) return fail(ILLUME_ERR_INVALID_ARG, "hits is null");
+    return run_traverse(bvh, rays, n, hits, nullptr, nullptr, false, stream);
+}
+
+int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
+                         void *stream) {
+    if (n && !occluded) return fail(ILLUME_ERR_INVALID_ARG, "occluded is null");
+    return run_traverse(bvh, rays, n, nullptr, occluded, nullptr, true, stream);
+}
+
+int illume_intersect_counts(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, int any, uint32_t *counts,
+                            void *stream) {
+    if (n && !counts) return fail(ILLUME_ERR_INVALID_ARG, "counts is null");
+    return run_traverse(bvh, rays, n, nullptr, nullptr, counts, any != 0, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+// Binned SAH builder (replaces BVHAccel::new / recursive_build,
+// accelerators/bvh.rs:32-247, for SplitMethod::SAH).
+//
+// Differences from the reference's build that do not change hit results:
+//  * 16 bins on each of the 3 axes (the reference: 12 buckets on the longest
+//    centroid axis); the cost model is the reference's: split cost
+//    1 + (n0*SA0 + n1*SA1)/SA against leaf cost n (bvh.rs:203-226).
+//  * A node whose primitives all share one centroid is split by count rather
+//    than made into an oversized leaf (bvh.rs:134-142), so every leaf holds at
+//    most max_prims_in_node primitives.
+//  * Below depth kMedianDepth splits are object medians, which bounds the
+//    tree depth and so the traversal stack.
+//  * Sphere boxes are widened by one ulp outwards so box culling stays
+//    conservative for hits computed with the sphere's interval arithmetic.
+// Closest-hit ids do not depend on the tree shape (the closest hit is unique
+// up to exact ties), which is why the tree may differ from the reference's.
+#include "bvh_builder.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace illume {
+namespace {
+
+constexpr int kBins = 16;
+constexpr int kMedianDepth = 40;
+
+struct Box {
+    float lo[3], hi[3];
+    Box() {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::numeric_limits<float>::infinity();
+            hi[i] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box &b) {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], b.lo[i]);
+            hi[i] = std::max(hi[i], b.hi[i]);
+        }
+    }
+    void grow(const float *p) {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], p[i]);
+            hi[i] = std::max(hi[i], p[i]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.f;
+        return 2.f * (dx * dy + dx * dz + dy * dz);
+    }
+};
+
+struct TmpNode {
+    Box box;
+    int32_t left = -1, right = -1;  // children (TmpNode indices) or -1 for a leaf
+    uint32_t start = 0, count = 0;  // leaf range in `ids`
+};
+
+struct Builder {
+    const BuildInput &in;
+    std::vector<Box> pbox;
+    std::vector<float> cen;  // 3 per prim
+    std::vector<uint32_t> ids;
+    std::vector<TmpNode> tmp;
+    uint32_t max_depth = 0;
+    int max_prims;
+
+    explicit Builder(const BuildInput &i) : in(i), max_prims(i.max_prims_in_node) {}
+
+    int leaf(const Box &b, uint32_t s, uint32_t e) {
+        TmpNode n;
+        n.box = b;
+        n.start = s;
+        n.count = e - s;
+        tmp.push_back(n);
+        return (int)tmp.size() - 1;
+    }
+
+    // Object median along the longest centroid axis.
+    uint32_t median_split(uint32_t s, uint32_t e, const Box &cb) {
+        int axis = 0;
+        float ext = -1;
+        for (int a = 0; a < 3; ++a) {
+            float x = cb.hi[a] - cb.lo[a];
+            if (x > ext) {
+                ext = x;
+                axis = a;
+            }
+        }
+        uint32_t mid = (s + e) / 2;
+        std::nth_element(ids.begin() + s, ids.begin() + mid, ids.begin() + e,
+                         [&](uint32_t a, uint32_t b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
+        return mid;
+    }
+
+    int build(uint32_t s, uint32_t e, uint32_t depth) {
+        max_depth = std::max(max_depth, depth);
+        Box b, cb;
+        for (uint32_t i = s; i < e; ++i) {
+            b.grow(pbox[ids[i]]);
+            cb.grow(&cen[3 * ids[i]]);
+        }
+        uint32_t n = e - s;
+        if (n == 1) return leaf(b, s, e);
+        uint32_t mid = 0;
+        bool split = false;
+        if (depth >= (uint32_t)kMedianDepth) {
+            if (n <= (uint32_t)max_prims) return leaf(b, s, e);
+            mid = median_split(s, e, cb);
+            split = true;
+        } else {
+            float best_cost = std::numeric_limits<float>::infinity();
+            int best_axis = -1, best_bin = -1;
+            float parent_area = b.area();
+            for (int a = 0; a < 3; ++a) {
+                float ext = cb.hi[a] - cb.lo[a];
+                if (!(ext > 0)) continue;
+                float k = kBins / ext;
+                Box bb[kBins];
+                uint32_t bc[kBins] = {0};
+                for (uint32_t i = s; i < e; ++i) {
+                    uint32_t p = ids[i];
+                    int bi = (int)((cen[3 * p + a] - cb.lo[a]) * k);
+                    bi = std::min(std::max(bi, 0), kBins - 1);
+                    bc[bi]++;
+                    bb[bi].grow(pbox[p]);
+                }
+                float right_area[kBins];
+                uint32_t right_cnt[kBins];
+                Box acc;
+                uint32_t cnt = 0;
+                for (int i = kBins - 1; i > 0; --i) {
+                    acc.grow(bb[i]);
+                    cnt += bc[i];
+                    right_area[i] = acc.area();
+                    right_cnt[i] = cnt;
+                }
+                Box lacc;
+                uint32_t lcnt = 0;
+                for (int i = 0; i < kBins - 1; ++i) {
+                    lacc.grow(bb[i]);
+                    lcnt += bc[i];
+                    if (lcnt == 0 || right_cnt[i + 1] == 0) continue;
+                    float cost = 1.f + ((float)lcnt * lacc.area() + (float)right_cnt[i + 1] * right_area[i + 1]) /
+                                           std::max(parent_area, 1e-30f);
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = a;
+                        best_bin = i;
+                    }
+                }
+            }
+            if (best_axis < 0) {
+                // all centroids coincide
+                if (n <= (uint32_t)max_prims) return leaf(b, s, e);
+                mid = s + n / 2;
+                split = true;
+            } else {
+                if (n <= (uint32_t)max_prims && best_cost >= (float)n) return leaf(b, s, e);
+                int a = best_axis;
+                float k = kBins / (cb.hi[a] - cb.lo[a]);
+                float lo = cb.lo[a];
+                auto it = std::partition(ids.begin() + s, ids.begin() + e, [&](uint32_t p) {
+                    int bi = (int)((cen[3 * p + a] - lo) * k);
+                    bi = std::min(std::max(bi, 0), kBins - 1);
+                    return bi <= best_bin;
+                });
+                mid = (uint32_t)(it - ids.begin());
+                if (mid == s || mid == e) mid = median_split(s, e, cb);
+                split = true;
+            }
+        }
+        (void)split;
+        int self = (int)tmp.size();
+        tmp.push_back(TmpNode());
+        tmp[self].box = b;
+        int l = build(s, mid, depth + 1);
+        int r = build(mid, e, depth + 1);
+        tmp[self].left = l;
+        tmp[self].right = r;
+        return self;
+    }
+};
+
+inline bool finite3(const float *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+}  // namespace
+
+std::string build_bvh(const BuildInput &in, BuildOutput *out) {
+    out->nodes.clear();
+    out->prims.clear();
+    out->n_leaves = 0;
+    out->max_depth = 0;
+    if (in.max_prims_in_node < 1 || in.max_prims_in_node > kMaxLeafPrims)
+        return "max_prims_in_node must be in 1..15";
+    uint64_t n_prims = (uint64_t)in.n_tris + in.n_spheres;
+    if (n_prims >= kMaxPrimSlots) return "too many primitives (limit 2^27)";
+    if (in.n_tris && (!in.verts || !in.idx)) return "null vertex or index array";
+    if (in.n_spheres && !in.spheres) return "null sphere array";
+
+    Builder B(in);
+    B.pbox.resize(n_prims);
+    B.cen.resize(3 * n_prims);
+    B.ids.resize(n_prims);
+    for (uint32_t v = 0; v < in.n_verts && in.n_tris; ++v)
+        if (!finite3(in.verts + 3 * (size_t)v)) return "non-finite vertex " + std::to_string(v);
+    for (uint32_t t = 0; t < in.n_tris; ++t) {
+        Box b;
+        for (int k = 0; k < 3; ++k) {
+            uint32_t vi = in.idx[3 * (size_t)t + k];
+            if (vi >= in.n_verts) return "triangle " + std::to_string(t) + " indexes vertex out of range";
+            b.grow(in.verts + 3 * (size_t)vi);
+        }
+        B.pbox[t] = b;
+    }
+    for (uint32_t i = 0; i < in.n_spheres; ++i) {
+        const float *sp = in.spheres + 4 * (size_t)i;
+        if (!finite3(sp) || !std::isfinite(sp[3]) || !(sp[3] > 0)) return "invalid sphere " + std::to_string(i);
+        Box b;
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = std::nextafter(sp[a] + (-sp[3]), -std::numeric_limits<float>::infinity());
+            b.hi[a] = std::nextafter(sp[a] + sp[3], std::numeric_limits<float>::infinity());
+        }
+        B.pbox[in.n_tris + i] = b;
+    }
+    for (uint64_t p = 0; p < n_prims; ++p) {
+        for (int a = 0; a < 3; ++a) B.cen[3 * p + a] = B.pbox[p].lo[a] * 0.5f + B.pbox[p].hi[a] * 0.5f;
+        B.ids[p] = (uint32_t)p;
+    }
+    if (n_prims == 0) return "";
+    B.tmp.reserve(2 * n_prims);
+    int root = B.build(0, (uint32_t)n_prims, 0);
+    out->max_depth = B.max_depth;
+    const Box &rb = B.tmp[root].box;
+    for (int a = 0; a < 3; ++a) {
+        out->bounds[a] = rb.lo[a];
+        out->bounds[3 + a] = rb.hi[a];
+    }
+
+    // Emit: depth-first over inner nodes; leaves append their primitive
+    // records in visit order.
+    out->prims.reserve(n_prims);
+    auto emit_prims = [&](const TmpNode &t) -> int32_t {
+        uint32_t first = (uint32_t)out->prims.size();
+        for (uint32_t i = 0; i < t.count; ++i) {
+            uint32_t p = B.ids[t.start + i];
+            PrimRecord r;
+            std::memset(&r, 0, sizeof(r));
+            if (p < in.n_tris) {
+                for (int k = 0; k < 3; ++k) {
+                    const float *v = in.verts + 3 * (size_t)in.idx[3 * (size_t)p + k];
+                    float *dst = k == 0 ? r.a : (k == 1 ? r.b : r.c);
+                    dst[0] = v[0];
+                    dst[1] = v[1];
+                    dst[2] = v[2];
+                }
+                std::memcpy(&r.a[3], &p, 4);
+            } else {
+                const float *sp = in.spheres + 4 * (size_t)(p - in.n_tris);
+                r.a[0] = sp[0];
+                r.a[1] = sp[1];
+                r.a[2] = sp[2];
+                uint32_t tag = p | kSphereBit;
+                std::memcpy(&r.a[3], &tag, 4);
+                r.b[0] = sp[3];
+            }
+            out->prims.push_back(r);
+        }
+        out->n_leaves++;
+        return make_leaf_ref(first, t.count);
+    };
+    auto set_box = [](GpuNode &g, int which, const Box &b) {
+        if (which == 0) {
+            g.c0_lo_x = b.lo[0]; g.c0_hi_x = b.hi[0]; g.c0_lo_y = b.lo[1]; g.c0_hi_y = b.hi[1];
+            g.c0_lo_z = b.lo[2]; g.c0_hi_z = b.hi[2];
+        } else {
+            g.c1_lo_x = b.lo[0]; g.c1_hi_x = b.hi[0]; g.c1_lo_y = b.lo[1]; g.c1_hi_y = b.hi[1];
+            g.c1_lo_z = b.lo[2]; g.c1_hi_z = b.hi[2];
+        }
+    };
+    const float inf = std::numeric_limits<float>::infinity();
+    Box never;  // a box no ray enters: all planes at +inf
+    for (int a = 0; a < 3; ++a) never.lo[a] = never.hi[a] = inf;
+
+    if (B.tmp[root].left < 0) {
+        GpuNode g;
+        std::memset(&g, 0, sizeof(g));
+        set_box(g, 0, B.tmp[root].box);
+        set_box(g, 1, never);
+        g.child0 = emit_prims(B.tmp[root]);
+        g.child1 = make_leaf_ref(0, 0);  // unreachable: its box is never entered
+        out->nodes.push_back(g);
+        return "";
+    }
+    // iterative DFS: (tmp index, gpu slot)
+    std::vector<std::pair<int, int>> stack;
+    out->nodes.reserve(n_prims);
+    out->nodes.push_back(GpuNode());
+    stack.push_back({root, 0});
+    while (!stack.empty()) {
+        auto [ti, gi] = stack.back();
+        stack.pop_back();
+        const TmpNode t = B.tmp[ti];
+        GpuNode g;
+        std::memset(&g, 0, sizeof(g));
+        int kids[2] = {t.left, t.right};
+        int32_t refs[2];
+        for (int c = 0; c < 2; ++c) {
+            const TmpNode &k = B.tmp[kids[c]];
+            set_box(g, c, k.box);
+            if (k.left < 0) {
+                refs[c] = emit_prims(k);
+            } else {
+                int slot = (int)out->nodes.size();
+                out->nodes.push_back(GpuNode());
+                refs[c] = slot;
+            }
+        }
+        g.child0 = refs[0];
+        g.child1 = refs[1];
+        out->nodes[gi] = g;
+        // push right first so the left subtree is laid out next
+        if (B.tmp[t.right].left >= 0) stack.push_back({t.right, refs[1]});
+        if (B.tmp[t.left].left >= 0) stack.push_back({t.left, refs[0]});
+    }
+    return "";
+}
+
+}  // namespace illume

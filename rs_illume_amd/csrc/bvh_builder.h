@@ -1,0 +1,31 @@
+// Host-side BVH builder: binned SAH over triangles and spheres, emitted in the
+// 64-byte two-child node layout of layout.h.
+#pragma once
+#include <string>
+#include <vector>
+#include "layout.h"
+
+namespace illume {
+
+struct BuildInput {
+    const float *verts = nullptr;  // 3 floats per vertex
+    uint32_t n_verts = 0;
+    const uint32_t *idx = nullptr;  // 3 per triangle
+    uint32_t n_tris = 0;
+    const float *spheres = nullptr;  // cx, cy, cz, r
+    uint32_t n_spheres = 0;
+    int max_prims_in_node = 4;
+};
+
+struct BuildOutput {
+    std::vector<GpuNode> nodes;
+    std::vector<PrimRecord> prims;  // leaf order
+    uint32_t n_leaves = 0;
+    uint32_t max_depth = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// Returns an empty string on success, else a description of the invalid input.
+std::string build_bvh(const BuildInput &in, BuildOutput *out);
+
+}  // namespace illume

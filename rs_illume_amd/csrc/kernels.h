@@ -1,0 +1,18 @@
+// Internal launchers shared between the C ABI and the kernel sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/illume_hip.h"
+#include "layout.h"
+
+namespace illume {
+
+// Deepest tree the traversal stack supports (LDS ring + private spill).
+int max_supported_depth();
+
+// Closest hit (hits), any hit (occluded) or counting traversal (counts != null).
+// `empty` = the BVH has no primitives: outputs are filled with misses.
+hipError_t launch_traverse(const GpuNode *nodes, const PrimRecord *prims, bool empty, const illume_ray *rays,
+                           uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
+                           hipStream_t stream);
+
+}  // namespace illume

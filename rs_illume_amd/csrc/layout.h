@@ -1,0 +1,45 @@
+// Device memory layout of a built BVH, shared by the host builder and the
+// HIP kernels.
+//
+// Inner nodes hold the boxes of BOTH children (64 bytes, one cache-line-sized
+// record per visit), so a visit tests two boxes with one 64-byte read and
+// never reads the box of a node it does not descend into. Leaves are not
+// nodes: a child reference either names an inner node (>= 0) or a contiguous
+// run of 1..15 primitive records (< 0).
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace illume {
+
+// 64-byte two-child node. Field order groups the x/y slabs of both children,
+// then both z slabs, then the links, so one float4 per slab pair is loaded.
+struct alignas(64) GpuNode {
+    float c0_lo_x, c0_hi_x, c0_lo_y, c0_hi_y;
+    float c1_lo_x, c1_hi_x, c1_lo_y, c1_hi_y;
+    float c0_lo_z, c0_hi_z, c1_lo_z, c1_hi_z;
+    int32_t child0, child1, pad0, pad1;
+};
+static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
+
+// Leaf reference: ~((first_slot << 4) | count), count in 1..15.
+constexpr int kLeafCountBits = 4;
+constexpr int kMaxLeafPrims = (1 << kLeafCountBits) - 1;
+constexpr uint32_t kMaxPrimSlots = 1u << 27;
+__host__ __device__ inline int32_t make_leaf_ref(uint32_t first_slot, uint32_t count) {
+    return (int32_t)~((first_slot << kLeafCountBits) | count);
+}
+__host__ __device__ inline uint32_t leaf_first(int32_t ref) { return ((uint32_t)~ref) >> kLeafCountBits; }
+__host__ __device__ inline uint32_t leaf_count(int32_t ref) { return ((uint32_t)~ref) & kMaxLeafPrims; }
+
+// Primitive record: 48 bytes = 3 x float4.
+//   triangle: (p0.xyz, id) (p1.xyz, 0) (p2.xyz, 0)   vertices in mesh index order
+//   sphere:   (center.xyz, id | kSphereBit) (radius, 0, 0, 0) (0, 0, 0, 0)
+// id is the caller's primitive id stored as raw bits.
+constexpr uint32_t kSphereBit = 0x80000000u;
+struct PrimRecord {
+    float a[4], b[4], c[4];
+};
+static_assert(sizeof(PrimRecord) == 48, "PrimRecord must be 48 bytes");
+
+}  // namespace illume

@@ -1,0 +1,63 @@
+"""CPU checks of the C ABI library: it loads, exports every function that
+include/illume_hip.h declares, and rejects invalid input before touching a GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def test_library_exports_every_declared_symbol():
+    from rs_illume_amd import _lib
+    L = _lib.lib()
+    names = _lib.header_functions()
+    assert len(names) >= 8
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    from rs_illume_amd import _lib
+    assert _lib.status_string(0) == "ok"
+    assert "invalid" in _lib.status_string(1)
+    assert _lib.status_string(99) == "unknown status"
+
+
+def _create(verts, idx, spheres=None, leaf=4):
+    from rs_illume_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    sph = np.zeros((0, 4), np.float32) if spheres is None else spheres
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a.size else None
+    st = L.illume_bvh_create(p(verts), len(verts), p(idx), len(idx), p(sph), len(sph), leaf, ctypes.byref(h))
+    return st, L.illume_last_error().decode()
+
+
+@pytest.mark.parametrize("case", ["index_out_of_range", "nan_vertex", "bad_sphere", "leaf_too_big", "leaf_zero"])
+def test_invalid_scene_rejected_before_device_work(case):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    idx = np.array([[0, 1, 2]], np.uint32)
+    sph = None
+    leaf = 4
+    expect = 1
+    if case == "index_out_of_range":
+        idx = np.array([[0, 1, 3]], np.uint32)
+    elif case == "nan_vertex":
+        v[1, 2] = np.nan
+    elif case == "bad_sphere":
+        sph = np.array([[0, 0, 0, -1.0]], np.float32)
+    elif case == "leaf_too_big":
+        leaf, expect = 16, 4
+    elif case == "leaf_zero":
+        leaf, expect = 0, 4
+    st, msg = _create(v, idx, sph, leaf)
+    assert st == expect, (st, msg)
+    assert msg
+
+
+def test_null_arguments():
+    from rs_illume_amd import _lib
+    L = _lib.lib()
+    assert L.illume_bvh_create(None, 0, None, 0, None, 0, 4, None) == 1
+    assert L.illume_intersect_closest(None, None, 1, None, None) == 1
+    assert L.illume_intersect_any(None, None, 1, None, None) == 1
+    assert L.illume_bvh_destroy(None) == 0
