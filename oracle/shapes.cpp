@@ -476,6 +476,8 @@ bool sphere_intersect(const SphereShape &s, const Ray &r, SurfaceInteraction *si
     si->shading_dpdu = si->dpdu;
     si->time = r.time;
     si->t = tsh.v;
+    si->b1 = 0;
+    si->b2 = 0;
     *t_hit = tsh.v;
     return true;
 }
