@@ -100,6 +100,79 @@ int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t
 int illume_intersect_counts(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, int any, uint32_t *counts,
                             void *stream);
 
+/* ------------------------------------------------------------------ path tracing */
+
+typedef struct illume_scene illume_scene;
+
+/* Materials with constant parameters (core/material.rs:14-16 is a trait only;
+ * these are pbrt-v3's Matte, Metal and Glass): */
+typedef enum {
+    ILLUME_MATTE = 0, /* LambertianReflection(kd) (reflection/lambertian_reflection.rs:17-45) */
+    ILLUME_METAL = 1, /* MicrofacetReflection(TrowbridgeReitz(alpha), FresnelConductor(eta, k))
+                         (reflection/microfacet_reflection.rs:24-87, microfacet/trowbridge_reitz_distribution.rs) */
+    ILLUME_GLASS = 2  /* FresnelSpecular(R=1, T=1, 1, ior) (reflection/fresnel_specular.rs:29-86) */
+} illume_material_type;
+
+/* 64 bytes. alpha is the Trowbridge-Reitz alpha itself (no roughness remap). */
+typedef struct {
+    int32_t type;
+    float kd[3];
+    float eta[3];
+    float k[3];
+    float alpha;
+    float ior;
+    float pad[4];
+} illume_material;
+
+/* PerspectiveCamera (cameras/perspective.rs:35-80) from a look-at frame
+ * (core/transform.rs:181-208); fov in degrees spans the shorter image axis. */
+typedef struct {
+    float pos[3];
+    float look[3];
+    float up[3];
+    float fov;
+    uint32_t width;
+    uint32_t height;
+} illume_camera;
+
+/* Per-call render parameters: sample indices [sample_offset, sample_offset+spp)
+ * of every pixel are traced (counter-based sampler streams keyed by seed,
+ * pixel and sample index, so splitting spp over calls gives the same image). */
+typedef struct {
+    uint32_t spp;
+    uint32_t max_depth;
+    uint32_t sample_offset;
+    uint32_t flags; /* reserved, 0 */
+    uint64_t seed;
+} illume_render_cfg;
+
+/* Build a renderable scene: the BVH of illume_bvh_create (max 4 primitives per
+ * leaf) plus per-primitive material index and emitted radiance (3 floats per
+ * primitive; non-zero makes that TRIANGLE a one-sided DiffuseAreaLight, which
+ * the reference lacks; spheres may not emit). Lights are chosen by power
+ * (core/lightdistrib.rs, integrator.rs:496-505). Host pointers.
+ * Replaces Scene::new (core/scene.rs:20-37). */
+int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                        const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
+                        const float *prim_emission, const illume_material *materials, uint32_t n_materials,
+                        illume_scene **out);
+
+int illume_scene_destroy(illume_scene *scene);
+
+/* The scene's aggregate, for Scene::intersect / intersect_p queries through
+ * illume_intersect_closest / illume_intersect_any (core/scene.rs:43-53). The
+ * handle stays owned by the scene. */
+int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out);
+
+/* Integrator::render (core/integrator.rs:17-29, SamplerIntegratorBase::render
+ * 48-145) with PathIntegrator::li: for every pixel adds the sum of the
+ * radiance of cfg->spp camera paths to rgb_sum[3*(y*width+x) + c] (a device
+ * float buffer of width*height*3 the caller zeroes and divides by the total
+ * sample count; box filter of radius 0.5). ray_counts (device, 2 x uint64, may
+ * be null) accumulates the number of closest-hit and any-hit rays traced. */
+int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                  float *rgb_sum, uint64_t *ray_counts, void *stream);
+
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);
 

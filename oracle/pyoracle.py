@@ -179,3 +179,61 @@ def offset_ray_origin(p, pe, n, w):
     out = np.zeros(3, np.float32)
     lib().orc_offset_ray_origin(*[_p(a) for a in arrs], _p(out))
     return out
+
+
+# ---------------------------------------------------------------- path tracer
+def _render_decl():
+    L = lib()
+    if getattr(L, "_rdecl", False):
+        return L
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    L.orc_render_scene_create.restype = vp
+    L.orc_render_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32]
+    L.orc_render_scene_destroy.argtypes = [vp]
+    L.orc_render.argtypes = [vp, vp, u32, u32, u64, u32, vp, vp, i32]
+    L.orc_camera_rays.argtypes = [vp, vp, u32, vp]
+    L.orc_sampler_floats.argtypes = [u64, u64, u64, u32, vp]
+    L._rdecl = True
+    return L
+
+
+class OracleRenderScene:
+    """The oracle's PathIntegrator over a scene given as the flat arrays of
+    rs_illume_amd.scenes.SceneDesc.arrays()."""
+
+    def __init__(self, a):
+        self.a = {k: np.ascontiguousarray(v) for k, v in a.items()}
+        a = self.a
+        self.h = _render_decl().orc_render_scene_create(
+            _p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]), _p(a["spheres"]), len(a["spheres"]),
+            _p(a["prim_material"]), _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            _render_decl().orc_render_scene_destroy(h)
+            self.h = None
+
+    def render_sum(self, camera, spp, max_depth, seed=0, sample_offset=0, n_threads=None):
+        """Per-pixel radiance sums (H, W, 3) float32 and (closest, any) ray counts."""
+        cam = np.ascontiguousarray(camera).reshape(1)
+        w, h = int(cam["width"][0]), int(cam["height"][0])
+        out = np.zeros((h, w, 3), np.float32)
+        st = np.zeros(2, np.uint64)
+        _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _p(out), _p(st),
+                                  n_threads or n_threads_default())
+        return out, st
+
+
+def camera_rays(camera, film_xy_time):
+    cam = np.ascontiguousarray(camera).reshape(1)
+    f = np.ascontiguousarray(film_xy_time, np.float32).reshape(-1, 3)
+    out = np.zeros((len(f), 8), np.float32)
+    _render_decl().orc_camera_rays(_p(cam), _p(f), len(f), _p(out))
+    return out
+
+
+def sampler_floats(seed, pixel, sample, n):
+    out = np.zeros(n, np.float32)
+    _render_decl().orc_sampler_floats(seed, pixel, sample, n, _p(out))
+    return out

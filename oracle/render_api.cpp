@@ -1,0 +1,156 @@
+// ORACLE — test infrastructure only (see pbrt.h). C entry points of the path
+// tracer for the tests and the CPU baseline.
+#include <atomic>
+#include <thread>
+#include "render.h"
+
+using namespace orc;
+
+namespace {
+// 64-byte material record: type, kd[3], eta[3], k[3], alpha, ior, pad[4]
+struct OrcMaterial {
+    int32_t type;
+    float kd[3], eta[3], k[3];
+    float alpha, ior;
+    float pad[4];
+};
+// camera record: pos[3], look[3], up[3], fov, width, height
+struct OrcCamera {
+    float pos[3], look[3], up[3], fov;
+    uint32_t width, height;
+};
+}  // namespace
+
+extern "C" {
+
+// prim_emission: 3 floats per primitive (zero = not emissive; only triangles
+// may emit). Every emissive triangle becomes one one-sided DiffuseAreaLight.
+void *orc_render_scene_create(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
+                              const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
+                              const float *prim_emission, const void *materials, uint32_t n_materials) {
+    RenderScene *s = new RenderScene();
+    s->ps.verts.resize(n_verts);
+    for (uint32_t i = 0; i < n_verts; ++i) s->ps.verts[i] = V3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    s->ps.idx.assign(idx, idx + 3 * (size_t)n_tris);
+    s->ps.spheres.resize(n_spheres);
+    for (uint32_t i = 0; i < n_spheres; ++i)
+        s->ps.spheres[i].init(V3(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3]);
+    s->bvh.build(&s->ps, 4);
+    uint32_t np = n_tris + n_spheres;
+    s->prim_material.assign(prim_material, prim_material + np);
+    const OrcMaterial *m = (const OrcMaterial *)materials;
+    for (uint32_t i = 0; i < n_materials; ++i) {
+        Material x;
+        x.type = m[i].type;
+        x.kd = RGB(m[i].kd[0], m[i].kd[1], m[i].kd[2]);
+        x.eta = RGB(m[i].eta[0], m[i].eta[1], m[i].eta[2]);
+        x.k = RGB(m[i].k[0], m[i].k[1], m[i].k[2]);
+        x.alpha = m[i].alpha;
+        x.ior = m[i].ior;
+        s->materials.push_back(x);
+    }
+    s->prim_light.assign(np, -1);
+    std::vector<float> power;
+    for (uint32_t p = 0; p < n_tris; ++p) {
+        RGB L(prim_emission[3 * p], prim_emission[3 * p + 1], prim_emission[3 * p + 2]);
+        if (is_black(L)) continue;
+        AreaLight l;
+        l.prim = p;
+        l.p0 = s->ps.verts[idx[3 * p]];
+        l.p1 = s->ps.verts[idx[3 * p + 1]];
+        l.p2 = s->ps.verts[idx[3 * p + 2]];
+        l.L = L;
+        l.area = triangle_area(l.p0, l.p1, l.p2);
+        s->prim_light[p] = (int)s->lights.size();
+        s->lights.push_back(l);
+        // DiffuseAreaLight::power = L * area * pi (one-sided), by luminance
+        power.push_back(luminance(L * l.area * Pi));
+    }
+    if (!power.empty()) s->light_distrib.init(power);
+    return s;
+}
+
+void orc_render_scene_destroy(void *h) { delete (RenderScene *)h; }
+
+// SamplerIntegratorBase::render (integrator.rs:48-145): for every pixel and
+// sample index in [sample_offset, sample_offset + spp) trace one camera path,
+// apply the NaN/negative/inf radiance guard (113-122) and add L to
+// rgb_sum[pixel] (box filter of radius 0.5: weight 1 for the pixel the
+// sample lands in). stats: [0] closest-hit rays, [1] any-hit rays.
+void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t sample_offset,
+                float *rgb_sum, uint64_t *stats, int n_threads) {
+    RenderScene *s = (RenderScene *)h;
+    const OrcCamera *c = (const OrcCamera *)camera;
+    Camera cam;
+    cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
+             V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
+    uint64_t n_pix = (uint64_t)c->width * c->height;
+    int nt = std::max(1, n_threads);
+    std::vector<RenderStats> st(nt);
+    std::atomic<uint64_t> next(0);
+    auto work = [&](int t) {
+        while (true) {
+            uint64_t b = next.fetch_add(64);
+            if (b >= n_pix) break;
+            uint64_t e = std::min(n_pix, b + 64);
+            for (uint64_t pix = b; pix < e; ++pix) {
+                int x = (int)(pix % c->width), y = (int)(pix / c->width);
+                RGB sum;
+                for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
+                    Sampler sampler;
+                    sampler.start(seed, pix, si);
+                    // sampler.rs:78-84 get_camera_sample: p_film, time, p_lens
+                    float uf[2], ul[2];
+                    sampler.get_2d(uf);
+                    float time = sampler.get_1d();
+                    sampler.get_2d(ul);
+                    Ray ray = cam.generate_ray((float)x + uf[0], (float)y + uf[1], time);
+                    RGB L = path_li(*s, ray, sampler, (int)max_depth, &st[t]);
+                    if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = RGB();
+                    else if (luminance(L) < -1e-5f) L = RGB();
+                    else if (std::isinf(luminance(L))) L = RGB();
+                    sum += L;
+                }
+                rgb_sum[3 * pix] += sum.x;
+                rgb_sum[3 * pix + 1] += sum.y;
+                rgb_sum[3 * pix + 2] += sum.z;
+            }
+        }
+    };
+    if (nt == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    if (stats) {
+        stats[0] = stats[1] = 0;
+        for (auto &x : st) {
+            stats[0] += x.closest_rays;
+            stats[1] += x.any_rays;
+        }
+    }
+}
+
+// Camera ray probe: rays (8 floats: o, d, t_max, time) for film points.
+void orc_camera_rays(const void *camera, const float *film_xy_time, uint32_t n, float *rays) {
+    const OrcCamera *c = (const OrcCamera *)camera;
+    Camera cam;
+    cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
+             V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r = cam.generate_ray(film_xy_time[3 * i], film_xy_time[3 * i + 1], film_xy_time[3 * i + 2]);
+        float *o = rays + 8 * i;
+        o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;
+        o[6] = r.t_max; o[7] = r.time;
+    }
+}
+
+// Stream probe: n uniform floats of the sampler stream for (seed, pixel, sample).
+void orc_sampler_floats(uint64_t seed, uint64_t pixel, uint64_t sample, uint32_t n, float *out) {
+    Sampler s;
+    s.start(seed, pixel, sample);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.get_1d();
+}
+
+}  // extern "C"

@@ -29,6 +29,15 @@ struct illume_bvh {
     int device = 0;
 };
 
+namespace illume {
+int abi_fail(int status, const std::string &msg) { return fail(status, msg); }
+int abi_hip_fail(hipError_t e, const char *what) { return hip_fail(e, what); }
+void bvh_device_arrays(const illume_bvh *bvh, const GpuNode **nodes, const PrimRecord **prims) {
+    *nodes = bvh->d_nodes;
+    *prims = bvh->d_prims;
+}
+}  // namespace illume
+
 extern "C" {
 
 const char *illume_status_string(int status) {

@@ -3,8 +3,13 @@
 #include <hip/hip_runtime.h>
 #include "../../include/illume_hip.h"
 #include "layout.h"
+#include "scene_types.h"
 
 namespace illume {
+
+// Path tracer: adds the radiance sums of spp paths per pixel to rgb.
+hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
+                         uint32_t sample_offset, float *rgb, unsigned long long *counters, hipStream_t stream);
 
 // Deepest tree the traversal stack supports (LDS ring + private spill).
 int max_supported_depth();

@@ -1,0 +1,341 @@
+// Scene, light table and camera set-up on the host for the path tracer, and
+// the illume_scene_* / illume_render entry points.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#include "../../include/illume_hip.h"
+#include "kernels.h"
+#include "scene_types.h"
+
+namespace illume {
+int abi_fail(int status, const std::string &msg);
+int abi_hip_fail(hipError_t e, const char *what);
+void bvh_device_arrays(const illume_bvh *bvh, const GpuNode **nodes, const PrimRecord **prims);
+}  // namespace illume
+
+using namespace illume;
+
+namespace {
+
+// 4x4 row-major matrices for the camera transforms (core/matrix.rs,
+// core/transform.rs), composed and inverted as pbrt-v3 does.
+struct M4 {
+    float m[4][4];
+};
+M4 m_identity() {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = i == j ? 1.f : 0.f;
+    return r;
+}
+M4 m_mul(const M4 &a, const M4 &b) {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j] + a.m[i][3] * b.m[3][j];
+    return r;
+}
+// Gauss-Jordan elimination with full pivoting (matrix.rs:84-162).
+bool m_inverse(const M4 &a, M4 *out) {
+    int indxc[4], indxr[4], ipiv[4] = {0, 0, 0, 0};
+    float minv[4][4];
+    std::memcpy(minv, a.m, sizeof(minv));
+    for (int i = 0; i < 4; i++) {
+        int irow = 0, icol = 0;
+        float big = 0.f;
+        for (int j = 0; j < 4; j++) {
+            if (ipiv[j] == 1) continue;
+            for (int k = 0; k < 4; k++) {
+                if (ipiv[k] == 0) {
+                    if (std::fabs(minv[j][k]) >= big) {
+                        big = std::fabs(minv[j][k]);
+                        irow = j;
+                        icol = k;
+                    }
+                } else if (ipiv[k] > 1) {
+                    return false;
+                }
+            }
+        }
+        ++ipiv[icol];
+        if (irow != icol)
+            for (int k = 0; k < 4; ++k) std::swap(minv[irow][k], minv[icol][k]);
+        indxr[i] = irow;
+        indxc[i] = icol;
+        if (minv[icol][icol] == 0.f) return false;
+        float pivinv = 1.f / minv[icol][icol];
+        minv[icol][icol] = 1.f;
+        for (int j = 0; j < 4; j++) minv[icol][j] *= pivinv;
+        for (int j = 0; j < 4; j++) {
+            if (j == icol) continue;
+            float save = minv[j][icol];
+            minv[j][icol] = 0;
+            for (int k = 0; k < 4; k++) minv[j][k] -= minv[icol][k] * save;
+        }
+    }
+    for (int j = 3; j >= 0; j--)
+        if (indxr[j] != indxc[j])
+            for (int k = 0; k < 4; k++) std::swap(minv[k][indxr[j]], minv[k][indxc[j]]);
+    std::memcpy(out->m, minv, sizeof(minv));
+    return true;
+}
+struct Xf {
+    M4 m, mi;
+};
+Xf xf_compose(const Xf &a, const Xf &b) { return Xf{m_mul(a.m, b.m), m_mul(b.mi, a.mi)}; }
+Xf xf_inverse(const Xf &a) { return Xf{a.mi, a.m}; }
+Xf xf_scale(float x, float y, float z) {
+    Xf t{m_identity(), m_identity()};
+    t.m.m[0][0] = x; t.m.m[1][1] = y; t.m.m[2][2] = z;
+    t.mi.m[0][0] = 1.f / x; t.mi.m[1][1] = 1.f / y; t.mi.m[2][2] = 1.f / z;
+    return t;
+}
+Xf xf_translate(float x, float y, float z) {
+    Xf t{m_identity(), m_identity()};
+    t.m.m[0][3] = x; t.m.m[1][3] = y; t.m.m[2][3] = z;
+    t.mi.m[0][3] = -x; t.mi.m[1][3] = -y; t.mi.m[2][3] = -z;
+    return t;
+}
+struct V {
+    float x, y, z;
+};
+V vsub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+V vnorm(V a) {
+    float l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return V{a.x / l, a.y / l, a.z / l};
+}
+V vcross(V a, V b) {  // evaluated in f64 (vector3.rs:66-82)
+    double ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+    return V{(float)((ay * bz) - (az * by)), (float)((az * bx) - (ax * bz)), (float)((ax * by) - (ay * bx))};
+}
+// transform.rs:181-208: world-to-camera of a look-at frame
+Xf xf_look_at(V pos, V look, V up) {
+    M4 c2w;
+    std::memset(&c2w, 0, sizeof(c2w));
+    c2w.m[0][3] = pos.x; c2w.m[1][3] = pos.y; c2w.m[2][3] = pos.z; c2w.m[3][3] = 1;
+    V dir = vnorm(vsub(look, pos));
+    V right = vnorm(vcross(vnorm(up), dir));
+    V new_up = vcross(dir, right);
+    c2w.m[0][0] = right.x; c2w.m[1][0] = right.y; c2w.m[2][0] = right.z;
+    c2w.m[0][1] = new_up.x; c2w.m[1][1] = new_up.y; c2w.m[2][1] = new_up.z;
+    c2w.m[0][2] = dir.x; c2w.m[1][2] = dir.y; c2w.m[2][2] = dir.z;
+    Xf t;
+    m_inverse(c2w, &t.m);
+    t.mi = c2w;
+    return t;
+}
+// pbrt-v3 Perspective(fov, n, f) = Scale(1/tan(fov/2), 1/tan(fov/2), 1) * persp
+Xf xf_perspective(float fov, float n, float f) {
+    M4 p = m_identity();
+    p.m[2][2] = f / (f - n);
+    p.m[2][3] = -f * n / (f - n);
+    p.m[3][2] = 1;
+    p.m[3][3] = 0;
+    Xf pt;
+    pt.m = p;
+    m_inverse(p, &pt.mi);
+    const float pi = 3.14159265358979323846f;
+    float inv_tan = 1.f / std::tan((pi / 180.f * fov) / 2.f);
+    return xf_compose(xf_scale(inv_tan, inv_tan, 1), pt);
+}
+
+bool make_camera(const illume_camera &c, DevCamera *out) {
+    if (c.width == 0 || c.height == 0 || !(c.fov > 0.f && c.fov < 180.f)) return false;
+    Xf c2w = xf_inverse(xf_look_at(V{c.pos[0], c.pos[1], c.pos[2]}, V{c.look[0], c.look[1], c.look[2]},
+                                   V{c.up[0], c.up[1], c.up[2]}));
+    float aspect = (float)c.width / (float)c.height;
+    float sx0, sx1, sy0, sy1;
+    if (aspect > 1.f) {
+        sx0 = -aspect; sx1 = aspect; sy0 = -1.f; sy1 = 1.f;
+    } else {
+        sx0 = -1.f; sx1 = 1.f; sy0 = -1.f / aspect; sy1 = 1.f / aspect;
+    }
+    // perspective.rs:46-53
+    Xf screen_to_raster = xf_compose(
+        xf_compose(xf_scale((float)c.width, (float)c.height, 1), xf_scale(1 / (sx1 - sx0), 1 / (sy0 - sy1), 1)),
+        xf_translate(-sx0, -sy1, 0));
+    Xf raster_to_screen = xf_inverse(screen_to_raster);
+    Xf camera_to_screen = xf_perspective(c.fov, 1e-2f, 1000.f);
+    Xf raster_to_camera = xf_compose(xf_inverse(camera_to_screen), raster_to_screen);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out->raster_to_camera[4 * i + j] = raster_to_camera.m.m[i][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) out->camera_to_world[4 * i + j] = c2w.m.m[i][j];
+    out->width = c.width;
+    out->height = c.height;
+    return true;
+}
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+    *dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+struct illume_scene {
+    illume_bvh *bvh = nullptr;
+    DevScene dev;
+    std::vector<void *> allocs;
+    uint32_t n_prims = 0;
+};
+
+extern "C" {
+
+int illume_scene_destroy(illume_scene *s) {
+    if (!s) return ILLUME_OK;
+    for (void *p : s->allocs)
+        if (p) hipFree(p);
+    illume_bvh_destroy(s->bvh);
+    delete s;
+    return ILLUME_OK;
+}
+
+int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                        const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
+                        const float *prim_emission, const illume_material *materials, uint32_t n_materials,
+                        illume_scene **out) {
+    if (!out) return abi_fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    uint64_t np = (uint64_t)n_tris + n_spheres;
+    if (np && (!prim_material || !prim_emission)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null per-primitive array");
+    if (n_materials == 0 || !materials) return abi_fail(ILLUME_ERR_INVALID_ARG, "no materials");
+    for (uint64_t p = 0; p < np; ++p)
+        if (prim_material[p] >= n_materials)
+            return abi_fail(ILLUME_ERR_INVALID_ARG, "primitive " + std::to_string(p) + " has no valid material");
+    for (uint32_t i = 0; i < n_materials; ++i) {
+        const illume_material &m = materials[i];
+        if (m.type < ILLUME_MATTE || m.type > ILLUME_GLASS)
+            return abi_fail(ILLUME_ERR_INVALID_ARG, "material " + std::to_string(i) + " has an unknown type");
+        if (m.type == ILLUME_METAL && !(m.alpha > 0.f)) return abi_fail(ILLUME_ERR_INVALID_ARG, "metal alpha must be > 0");
+        if (m.type == ILLUME_GLASS && !(m.ior > 0.f)) return abi_fail(ILLUME_ERR_INVALID_ARG, "glass ior must be > 0");
+    }
+    for (uint32_t s = 0; s < n_spheres; ++s) {
+        const float *e = prim_emission + 3 * ((size_t)n_tris + s);
+        if (e[0] != 0.f || e[1] != 0.f || e[2] != 0.f)
+            return abi_fail(ILLUME_ERR_UNSUPPORTED, "only triangles may emit");
+    }
+    illume_scene *s = new (std::nothrow) illume_scene();
+    if (!s) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::memset(&s->dev, 0, sizeof(s->dev));
+    int st = illume_bvh_create(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, &s->bvh);
+    if (st != ILLUME_OK) {
+        delete s;
+        return st;
+    }
+    s->n_prims = (uint32_t)np;
+    bvh_device_arrays(s->bvh, &s->dev.nodes, &s->dev.prims);
+
+    std::vector<float4> tv(3 * (size_t)n_tris);
+    for (uint32_t t = 0; t < n_tris; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const float *v = verts + 3 * (size_t)tri_idx[3 * (size_t)t + k];
+            tv[3 * (size_t)t + k] = make_float4(v[0], v[1], v[2], 0.f);
+        }
+    std::vector<float4> sv(n_spheres);
+    for (uint32_t i = 0; i < n_spheres; ++i)
+        sv[i] = make_float4(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2], spheres[4 * i + 3]);
+    std::vector<uint32_t> pm(prim_material, prim_material + np);
+    std::vector<int32_t> pl(np, -1);
+    std::vector<DevMaterial> mats(n_materials);
+    std::memcpy(mats.data(), materials, n_materials * sizeof(DevMaterial));
+    std::vector<DevLight> lights;
+    std::vector<float> power;
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        const float *e = prim_emission + 3 * (size_t)t;
+        if (e[0] == 0.f && e[1] == 0.f && e[2] == 0.f) continue;
+        DevLight l;
+        std::memset(&l, 0, sizeof(l));
+        const float4 &a = tv[3 * (size_t)t], &b = tv[3 * (size_t)t + 1], &c = tv[3 * (size_t)t + 2];
+        l.p0[0] = a.x; l.p0[1] = a.y; l.p0[2] = a.z;
+        l.p1[0] = b.x; l.p1[1] = b.y; l.p1[2] = b.z;
+        l.p2[0] = c.x; l.p2[1] = c.y; l.p2[2] = c.z;
+        l.L[0] = e[0]; l.L[1] = e[1]; l.L[2] = e[2];
+        // Triangle::area (triangle.rs:609-614)
+        V cr = vcross(V{b.x - a.x, b.y - a.y, b.z - a.z}, V{c.x - a.x, c.y - a.y, c.z - a.z});
+        l.area = 0.5f * std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+        l.prim = t;
+        pl[t] = (int32_t)lights.size();
+        lights.push_back(l);
+        // DiffuseAreaLight power L * area * pi, by luminance (spectrum.rs:498-501)
+        const float pi = 3.14159265358979323846f;
+        float r = e[0] * l.area * pi, g = e[1] * l.area * pi, bb = e[2] * l.area * pi;
+        power.push_back(0.212671f * r + 0.715160f * g + 0.072169f * bb);
+    }
+    // Distribution1D (sampling.rs:20-42)
+    std::vector<float> cdf(power.size() + 1, 0.f);
+    float func_int = 0.f;
+    if (!power.empty()) {
+        size_t n = power.size();
+        for (size_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + power[i - 1] / (float)n;
+        func_int = cdf[n];
+        if (func_int == 0)
+            for (size_t i = 1; i < n + 1; ++i) cdf[i] = (float)i / (float)n;
+        else
+            for (size_t i = 1; i < n + 1; ++i) cdf[i] /= func_int;
+    }
+    hipError_t e = hipSuccess;
+    float4 *d_tv = nullptr, *d_sv = nullptr;
+    uint32_t *d_pm = nullptr;
+    int32_t *d_pl = nullptr;
+    DevMaterial *d_m = nullptr;
+    DevLight *d_l = nullptr;
+    float *d_f = nullptr, *d_c = nullptr;
+    if ((e = upload(&d_tv, tv)) == hipSuccess) s->allocs.push_back(d_tv);
+    if (e == hipSuccess && (e = upload(&d_sv, sv)) == hipSuccess) s->allocs.push_back(d_sv);
+    if (e == hipSuccess && (e = upload(&d_pm, pm)) == hipSuccess) s->allocs.push_back(d_pm);
+    if (e == hipSuccess && (e = upload(&d_pl, pl)) == hipSuccess) s->allocs.push_back(d_pl);
+    if (e == hipSuccess && (e = upload(&d_m, mats)) == hipSuccess) s->allocs.push_back(d_m);
+    if (e == hipSuccess && (e = upload(&d_l, lights)) == hipSuccess) s->allocs.push_back(d_l);
+    if (e == hipSuccess && (e = upload(&d_f, power)) == hipSuccess) s->allocs.push_back(d_f);
+    if (e == hipSuccess && (e = upload(&d_c, cdf)) == hipSuccess) s->allocs.push_back(d_c);
+    if (e != hipSuccess) {
+        illume_scene_destroy(s);
+        return abi_hip_fail(e, "scene upload");
+    }
+    s->dev.tri_verts = d_tv;
+    s->dev.spheres = d_sv;
+    s->dev.prim_material = d_pm;
+    s->dev.prim_light = d_pl;
+    s->dev.materials = d_m;
+    s->dev.lights = d_l;
+    s->dev.light_func = d_f;
+    s->dev.light_cdf = d_c;
+    s->dev.light_func_int = func_int;
+    s->dev.n_lights = (uint32_t)lights.size();
+    s->dev.n_tris = n_tris;
+    *out = s;
+    return ILLUME_OK;
+}
+
+int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out) {
+    if (!scene || !out) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    *out = scene->bvh;
+    return ILLUME_OK;
+}
+
+int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                  float *rgb_sum, uint64_t *ray_counts, void *stream) {
+    if (!scene || !camera || !cfg || !rgb_sum) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    DevCamera cam;
+    if (!make_camera(*camera, &cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
+    if (cfg->spp == 0) return ILLUME_OK;
+    if ((uint64_t)cfg->sample_offset + cfg->spp > (1u << 24))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+    if ((uint64_t)camera->width * camera->height >= (1ull << 40))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+    if (scene->n_prims == 0) return ILLUME_OK;  // nothing to hit, no lights: black
+    hipError_t e = launch_render(scene->dev, cam, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset, rgb_sum,
+                                 (unsigned long long *)ray_counts, (hipStream_t)stream);
+    if (e != hipSuccess) return abi_hip_fail(e, "render launch");
+    return ILLUME_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Device-side BVH traversal shared by the intersection kernels and the path
+// tracer: closest hit (BVHAccel::intersect, accelerators/bvh.rs:563-612) and
+// any hit (BVHAccel::intersect_p, bvh.rs:614-662) over the 64-byte
+// two-child layout of layout.h.
+//
+// Each visit of an inner node reads both child boxes (one 64-byte record),
+// tests them with the conservative slab test, descends into the nearer hit
+// child and pushes the farther. The stack is a ring in LDS (kStackLds entries
+// per lane, laid out [slot][lane] so a wave's pushes and pops hit 64 distinct
+// banks); entries deeper than the ring spill to a private array that only
+// rays with unusually deep stacks touch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "geom.h"
+#include "layout.h"
+
+#pragma clang fp contract(off)
+
+namespace illume {
+
+constexpr int kStackLds = 16;  // power of two
+constexpr int kSpill = 64;
+constexpr int kMaxTraceDepth = kStackLds + kSpill;
+
+template <int BLOCK>
+struct TraceStack {
+    int *lds;  // this lane's column: lds[slot * BLOCK]
+    int sp;
+    int spill[kSpill];
+    __device__ __forceinline__ void reset(int *column) {
+        lds = column;
+        sp = 0;
+    }
+    __device__ __forceinline__ void push(int v) {
+        int slot = sp & (kStackLds - 1);
+        if (sp >= kStackLds) spill[sp - kStackLds] = lds[slot * BLOCK];
+        lds[slot * BLOCK] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        int slot = sp & (kStackLds - 1);
+        int v = lds[slot * BLOCK];
+        if (sp >= kStackLds) lds[slot * BLOCK] = spill[sp - kStackLds];
+        return v;
+    }
+};
+
+struct TraceHit {
+    uint32_t prim;  // caller's primitive id, ILLUME_MISS-equivalent 0xFFFFFFFF on a miss
+    uint32_t slot;  // primitive record index (leaf order)
+    float t, b0, b1, b2;
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true) of the ray (o, d) in
+// (0, t_max]. Returns true on a hit. COUNT adds box / primitive tests to the
+// counters.
+template <bool ANY, bool COUNT, int BLOCK>
+__device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, f3 o,
+                                          f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st, uint32_t &n_box,
+                                          uint32_t &n_prim) {
+    const f3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    hit.prim = 0xFFFFFFFFu;
+    hit.slot = 0;
+    hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
+    bool found = false;
+    int cur = 0;
+    st.sp = 0;
+    while (true) {
+        if (cur >= 0) {
+            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
+            const float4 a = np[0], b = np[1], c = np[2];
+            const int4 l = reinterpret_cast<const int4 *>(np)[3];
+            if (COUNT) n_box += 2;
+            float t0, t1;
+            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, o, inv, t_max, t0);
+            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, o, inv, t_max, t1);
+            if (h0 && h1) {
+                int near = l.x, far = l.y;
+                if (t1 < t0) {
+                    near = l.y;
+                    far = l.x;
+                }
+                st.push(far);
+                cur = near;
+                continue;
+            }
+            if (h0) {
+                cur = l.x;
+                continue;
+            }
+            if (h1) {
+                cur = l.y;
+                continue;
+            }
+        } else {
+            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *pp = reinterpret_cast<const float4 *>(prims + first + k);
+                const float4 pa = pp[0];
+                const uint32_t tag = __float_as_uint(pa.w);
+                if (COUNT) ++n_prim;
+                if (tag & kSphereBit) {
+                    const float4 pb = pp[1];
+                    float t;
+                    if (sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t)) {
+                        found = true;
+                        if (ANY) return true;
+                        t_max = t;
+                        hit.prim = tag & ~kSphereBit;
+                        hit.slot = first + k;
+                        hit.t = t;
+                        hit.b0 = hit.b1 = hit.b2 = 0.f;
+                    }
+                } else {
+                    const float4 pb = pp[1], pc = pp[2];
+                    const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
+                    float t, b0, b1, b2;
+                    if (triangle_test(p0, p1, p2, o, d, t_max, t, b0, b1, b2)) {
+                        if (ANY) return true;
+                        if (!triangle_degenerate(p0, p1, p2)) {
+                            found = true;
+                            t_max = t;
+                            hit.prim = tag;
+                            hit.slot = first + k;
+                            hit.t = t;
+                            hit.b0 = b0;
+                            hit.b1 = b1;
+                            hit.b2 = b2;
+                        }
+                    }
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    return found;
+}
+
+}  // namespace illume

@@ -1,0 +1,164 @@
+"""Scene descriptions for the path tracer: flat arrays handed to
+``illume_scene_create`` (include/illume_hip.h).
+
+A scene is triangles (vertex array + index triples) and full spheres, a
+material index and an emitted radiance per primitive (emissive triangles are
+one-sided area lights), and a material table. Builders here produce the
+Cornell-box workloads: the plain Lambertian box (256x256, 16 spp, depth 5) and
+its glossy small-light variant (Trowbridge-Reitz conductor walls, a light of 1%
+of the ceiling area, 512x512, depth 8).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MATTE, METAL, GLASS = 0, 1, 2
+MATERIAL_DTYPE = np.dtype([("type", "<i4"), ("kd", "<f4", 3), ("eta", "<f4", 3), ("k", "<f4", 3), ("alpha", "<f4"),
+                           ("ior", "<f4"), ("pad", "<f4", 4)])
+CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("look", "<f4", 3), ("up", "<f4", 3), ("fov", "<f4"), ("width", "<u4"),
+                         ("height", "<u4")])
+assert MATERIAL_DTYPE.itemsize == 64 and CAMERA_DTYPE.itemsize == 48
+
+# RGB conductor constants of copper (eta, k) at roughly 650/550/450 nm.
+COPPER_ETA = (0.200438, 0.924033, 1.10221)
+COPPER_K = (3.91295, 2.45285, 2.14219)
+
+
+def matte(kd):
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["type"] = MATTE
+    m["kd"] = kd
+    return m
+
+
+def metal(alpha, eta=COPPER_ETA, k=COPPER_K):
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["type"] = METAL
+    m["eta"] = eta
+    m["k"] = k
+    m["alpha"] = alpha
+    return m
+
+
+def glass(ior=1.5):
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["type"] = GLASS
+    m["ior"] = ior
+    return m
+
+
+def camera(pos, look, up, fov, width, height):
+    c = np.zeros(1, CAMERA_DTYPE)
+    c["pos"], c["look"], c["up"], c["fov"], c["width"], c["height"] = pos, look, up, fov, width, height
+    return c
+
+
+@dataclass
+class SceneDesc:
+    verts: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    idx: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.uint32))
+    spheres: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    tri_material: list = field(default_factory=list)
+    tri_emission: list = field(default_factory=list)
+    sphere_material: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+
+    def add_material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_triangle(self, a, b, c, mat, emission=(0, 0, 0), normal=None):
+        """Triangle a, b, c; if `normal` is given the vertex order is chosen so
+        that the geometric normal normalize((a-c) x (b-c)) points along it."""
+        a, b, c = (np.asarray(x, np.float32) for x in (a, b, c))
+        if normal is not None and np.dot(np.cross(a - c, b - c), normal) < 0:
+            b, c = c, b
+        base = len(self.verts)
+        self.verts = np.concatenate([self.verts, np.stack([a, b, c])]).astype(np.float32)
+        self.idx = np.concatenate([self.idx, np.array([[base, base + 1, base + 2]], np.uint32)])
+        self.tri_material.append(mat)
+        self.tri_emission.append(emission)
+
+    def add_quad(self, p0, p1, p2, p3, mat, emission=(0, 0, 0), normal=None):
+        """Planar quad p0 p1 p2 p3 (in order around the boundary) as two triangles."""
+        self.add_triangle(p0, p1, p2, mat, emission, normal)
+        self.add_triangle(p0, p2, p3, mat, emission, normal)
+
+    def add_sphere(self, center, radius, mat):
+        self.spheres = np.concatenate([self.spheres, np.array([[*center, radius]], np.float32)])
+        self.sphere_material.append(mat)
+
+    def arrays(self):
+        n_t, n_s = len(self.idx), len(self.spheres)
+        prim_material = np.array(list(self.tri_material) + list(self.sphere_material), np.uint32).reshape(n_t + n_s)
+        prim_emission = np.zeros((n_t + n_s, 3), np.float32)
+        if n_t:
+            prim_emission[:n_t] = np.asarray(self.tri_emission, np.float32).reshape(n_t, 3)
+        mats = np.concatenate(self.materials) if self.materials else np.zeros(0, MATERIAL_DTYPE)
+        return {
+            "verts": np.ascontiguousarray(self.verts, np.float32).reshape(-1, 3),
+            "idx": np.ascontiguousarray(self.idx, np.uint32).reshape(-1, 3),
+            "spheres": np.ascontiguousarray(self.spheres, np.float32).reshape(-1, 4),
+            "prim_material": prim_material,
+            "prim_emission": prim_emission,
+            "materials": np.ascontiguousarray(mats, MATERIAL_DTYPE),
+        }
+
+
+def cornell_box(seed=0, glossy=False, light_size=0.25, light_L=17.0, wall_alpha_range=(0.05, 0.5)):
+    """Box [-1,1]^3 open at +z: floor, ceiling, back grey 0.75, left (x=-1)
+    red, right (x=1) green; two spheres r=0.3 (albedo 0.75) with centres
+    uniform in [-0.7,0.7]^3; a light_size^2 one-sided quad light just below
+    the ceiling facing -y. glossy=True makes the five walls Trowbridge-Reitz
+    copper with alpha uniform in wall_alpha_range (one alpha per wall)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = SceneDesc()
+    grey = s.add_material(matte((0.75, 0.75, 0.75)))
+    red = s.add_material(matte((0.63, 0.06, 0.05)))
+    green = s.add_material(matte((0.14, 0.45, 0.09)))
+    centres = rng.uniform(-0.7, 0.7, (2, 3))
+    if glossy:
+        alphas = rng.uniform(*wall_alpha_range, 5)
+        walls = [s.add_material(metal(float(a))) for a in alphas]
+    else:
+        walls = [grey, grey, grey, red, green]
+    floor_m, ceil_m, back_m, left_m, right_m = walls
+    s.add_quad((-1, -1, -1), (1, -1, -1), (1, -1, 1), (-1, -1, 1), floor_m, normal=(0, 1, 0))
+    s.add_quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), ceil_m, normal=(0, -1, 0))
+    s.add_quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), back_m, normal=(0, 0, 1))
+    s.add_quad((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1), left_m, normal=(1, 0, 0))
+    s.add_quad((1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1), right_m, normal=(-1, 0, 0))
+    h = light_size / 2
+    y = 1 - 1e-4
+    light_m = s.add_material(matte((0.0, 0.0, 0.0)))
+    s.add_quad((-h, y, -h), (h, y, -h), (h, y, h), (-h, y, h), light_m, emission=(light_L,) * 3, normal=(0, -1, 0))
+    for c in centres:
+        s.add_sphere(c, 0.3, grey)
+    return s
+
+
+def cornell_camera(width=256, height=256):
+    return camera((0, 0, 3.75), (0, 0, 0), (0, 1, 0), 40.0, width, height)
+
+
+def furnace_box(albedo=0.5, emission=1.0):
+    """A closed box whose every wall is matte(albedo) AND emits `emission`
+    (white furnace): without Russian roulette the radiance along any camera
+    ray is emission * sum_{k=0..max_depth} albedo^k."""
+    s = SceneDesc()
+    m = s.add_material(matte((albedo,) * 3))
+    e = (emission,) * 3
+    s.add_quad((-1, -1, -1), (1, -1, -1), (1, -1, 1), (-1, -1, 1), m, e, normal=(0, 1, 0))
+    s.add_quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), m, e, normal=(0, -1, 0))
+    s.add_quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), m, e, normal=(0, 0, 1))
+    s.add_quad((-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1), m, e, normal=(0, 0, -1))
+    s.add_quad((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1), m, e, normal=(1, 0, 0))
+    s.add_quad((1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1), m, e, normal=(-1, 0, 0))
+    return s
+
+
+def rel_mse(img, ref, eps=0.01):
+    """Mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))

@@ -1,0 +1,102 @@
+"""Pins the oracle path tracer (PathIntegrator, estimate_direct, camera,
+sampler, film) with closed forms: the white furnace, the direct light of a
+small quad, camera geometry and sampler-stream identities."""
+import numpy as np
+import pytest
+
+from rs_illume_amd import scenes as S
+
+
+@pytest.mark.parametrize("albedo,depth", [(0.5, 0), (0.5, 1), (0.5, 3), (0.8, 5)])
+def test_white_furnace(oracle, albedo, depth):
+    """Closed box, every wall matte(albedo) and emitting 1: E[L] = sum_{k<=depth} albedo^k
+    (with MIS next-event estimation, Russian roulette and the BSDF-sampled term)."""
+    sc = oracle.OracleRenderScene(S.furnace_box(albedo, 1.0).arrays())
+    cam = S.camera((0.1, -0.2, 0.05), (0.3, 0.1, -1), (0, 1, 0), 70, 24, 24)
+    spp = 256
+    img, st = sc.render_sum(cam, spp, depth, seed=3)
+    est = img.astype(np.float64) / spp
+    expect = sum(albedo ** k for k in range(depth + 1))
+    # per-pixel means over 256 spp, 576 pixels: the image mean is within ~3 sigma
+    assert abs(est.mean() - expect) < 0.01 * expect + 1e-6, (est.mean(), expect)
+    if depth == 0:
+        assert np.allclose(est, 1.0)   # emitted radiance only, exactly
+    assert st[0] > 0
+
+
+def test_direct_light_of_small_quad(oracle):
+    """A camera looking straight down at a large matte floor (albedo a) under
+    a small one-sided square light of side s at height H: the floor radiance
+    straight below is a/pi * L * (solid-angle-weighted cos) ~= a * L * s^2 / (pi H^2)."""
+    a, L, side, H = 0.6, 40.0, 0.1, 1.0
+    s = S.SceneDesc()
+    fm = s.add_material(S.matte((a, a, a)))
+    lm = s.add_material(S.matte((0, 0, 0)))
+    s.add_quad((-50, 0, -50), (50, 0, -50), (50, 0, 50), (-50, 0, 50), fm, normal=(0, 1, 0))
+    h = side / 2
+    s.add_quad((-h, H, -h), (h, H, -h), (h, H, h), (-h, H, h), lm, emission=(L, L, L), normal=(0, -1, 0))
+    sc = oracle.OracleRenderScene(s.arrays())
+    # narrow camera below the light looking down at the floor point (0,0,0)
+    cam = S.camera((0.0, 0.5, 0.0), (0, 0, 0), (0, 0, 1), 0.5, 4, 4)
+    spp = 2048
+    img, _ = sc.render_sum(cam, spp, 1, seed=5)
+    est = img.astype(np.float64).mean() / spp
+    # exact irradiance of a square light above a point: E = L * integral cos cos / r^2 dA
+    n = 400
+    g = (np.arange(n) + 0.5) / n * side - h
+    X, Z = np.meshgrid(g, g)
+    r2 = X ** 2 + Z ** 2 + H ** 2
+    E = L * np.sum((H / np.sqrt(r2)) ** 2 / r2) * (side / n) ** 2
+    expect = a / np.pi * E
+    assert abs(est - expect) < 0.02 * expect, (est, expect)
+
+
+def test_camera_geometry(oracle):
+    cam = S.camera((0, 0, 3.75), (0, 0, 0), (0, 1, 0), 40.0, 256, 256)
+    rays = oracle.camera_rays(cam, np.array([[128, 128, 0], [0, 128, 0], [128, 0, 0]], np.float32))
+    o, d = rays[:, :3], rays[:, 3:6]
+    assert np.allclose(o, [0, 0, 3.75], atol=1e-5)
+    assert np.allclose(d[0], [0, 0, -1], atol=1e-6)          # image centre looks at the target
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-6)
+    # the edge pixel of a 40 degree field of view is 20 degrees off axis
+    ang = np.degrees(np.arccos(np.clip(d[1] @ d[0], -1, 1)))
+    assert abs(ang - 20.0) < 1e-3
+    ang = np.degrees(np.arccos(np.clip(d[2] @ d[0], -1, 1)))
+    assert abs(ang - 20.0) < 1e-3
+    assert d[2][1] > 0   # raster y = 0 is the top of the image (+up)
+
+
+def test_sampler_stream_is_pcg32(oracle):
+    """The per-(pixel, sample) stream is pbrt's RNG(seq) with seq a splitmix64
+    hash; the floats are uniform_float of its outputs."""
+    def splitmix(x):
+        M = (1 << 64) - 1
+        x = (x + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+    seed, pix, smp = 7, 12345, 9
+    seq = splitmix(seed ^ splitmix((pix << 24) | smp))
+    ints = oracle.pcg32(seq, 16)
+    expect = np.minimum(ints.astype(np.float32) * np.float32(2.0 ** -32), np.float32(float.fromhex("0x1.fffffep-1")))
+    assert np.array_equal(oracle.sampler_floats(seed, pix, smp, 16), expect.astype(np.float32))
+
+
+def test_render_is_additive_over_sample_ranges(oracle):
+    """Counter-based streams: splitting spp over calls gives the same sums."""
+    sc = oracle.OracleRenderScene(S.cornell_box(0).arrays())
+    cam = S.cornell_camera(16, 16)
+    full, _ = sc.render_sum(cam, 8, 5, seed=2)
+    a, _ = sc.render_sum(cam, 3, 5, seed=2)
+    b, _ = sc.render_sum(cam, 5, 5, seed=2, sample_offset=3)
+    assert np.allclose(full, a + b, rtol=1e-6, atol=1e-6)
+
+
+def test_glossy_box_energy_is_bounded(oracle):
+    """Copper walls reflect less than they receive: the glossy box renders
+    finite, non-negative and darker than a white-walled furnace bound."""
+    sc = oracle.OracleRenderScene(S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625).arrays())
+    cam = S.cornell_camera(32, 32)
+    img, st = sc.render_sum(cam, 16, 8, seed=1)
+    assert np.isfinite(img).all() and (img >= 0).all()
+    assert st[1] > 0 and st[0] > st[1] / 4
