@@ -173,6 +173,30 @@ int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out);
 int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                   float *rgb_sum, uint64_t *ray_counts, void *stream);
 
+/* ------------------------------------------------------------------ learned proposal
+ * No counterpart in the reference: an MLP (144-64-64-64-40, ReLU) from the
+ * frequency encoding of a hit's normalised position and normal to an 8-lobe
+ * von Mises-Fisher mixture over incident directions, BF16 weights on MFMA with
+ * FP32 accumulation, trained online with Adam (beta1 0.9, beta2 0.999,
+ * eps 1e-8) on the weighted negative log-likelihood of path-traced radiance
+ * samples. Parameters: 20,200 floats, per layer W[in][out] row-major then b. */
+
+typedef struct illume_guide illume_guide;
+
+int illume_guide_create(const float *init_params, float lr, illume_guide **out);
+int illume_guide_destroy(illume_guide *guide);
+/* Copies the fp32 master parameters to host memory (20,200 floats). */
+int illume_guide_get_params(const illume_guide *guide, float *host_out);
+/* inputs: device [n][6] (position normalised to [0,1]^3 by the scene bounds,
+ * unit normal); out: device [n][40] raw network outputs. */
+int illume_guide_infer(const illume_guide *guide, const float *inputs, uint32_t n, float *out, void *stream);
+/* records: device [n][10] (position (3), normal (3), unit direction (3),
+ * weight); grad_out: device [20200], the gradient of
+ * -sum_r (w_r / sum w) log q(dir_r | pos_r, normal_r). */
+int illume_guide_grad(const illume_guide *guide, const float *records, uint32_t n, float *grad_out, void *stream);
+/* One Adam step on the gradient of that loss over the batch. */
+int illume_guide_train_step(illume_guide *guide, const float *records, uint32_t n, void *stream);
+
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);
 

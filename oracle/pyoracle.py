@@ -237,3 +237,81 @@ def sampler_floats(seed, pixel, sample, n):
     out = np.zeros(n, np.float32)
     _render_decl().orc_sampler_floats(seed, pixel, sample, n, _p(out))
     return out
+
+
+# ---------------------------------------------------------------- learned proposal
+GUIDE_PARAMS = 20200
+GUIDE_OUT = 40
+
+
+def _guide_decl():
+    L = lib()
+    if getattr(L, "_gdecl", False):
+        return L
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    L.orc_guide_num_params.restype = i32
+    L.orc_guide_forward.argtypes = [vp, i32, vp, vp, vp, i32, vp]
+    L.orc_guide_encode.argtypes = [i32, vp, vp, vp, vp]
+    L.orc_guide_loss_grad.restype = ctypes.c_double
+    L.orc_guide_loss_grad.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, i32, vp]
+    L.orc_guide_loss_f64.restype = ctypes.c_double
+    L.orc_guide_loss_f64.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.orc_guide_pdf.argtypes = [vp, i32, vp, vp]
+    L.orc_guide_sample.argtypes = [vp, i32, vp, vp, vp]
+    L.orc_adam.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L._gdecl = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def guide_forward(params, x, nrm, bounds, bf16=False):
+    x, nrm = _f32(x).reshape(-1, 3), _f32(nrm).reshape(-1, 3)
+    out = np.zeros((len(x), GUIDE_OUT), np.float32)
+    _guide_decl().orc_guide_forward(_p(_f32(params)), len(x), _p(x), _p(nrm), _p(_f32(bounds)), int(bf16), _p(out))
+    return out
+
+
+def guide_encode(x, nrm, bounds):
+    x, nrm = _f32(x).reshape(-1, 3), _f32(nrm).reshape(-1, 3)
+    out = np.zeros((len(x), 144), np.float32)
+    _guide_decl().orc_guide_encode(len(x), _p(x), _p(nrm), _p(_f32(bounds)), _p(out))
+    return out
+
+
+def guide_loss_grad(params, x, nrm, wi, weight, bounds, bf16=False, f64=False):
+    x, nrm, wi = (_f32(a).reshape(-1, 3) for a in (x, nrm, wi))
+    weight = _f32(weight).reshape(-1)
+    g = np.zeros(GUIDE_PARAMS, np.float32)
+    loss = _guide_decl().orc_guide_loss_grad(_p(_f32(params)), len(x), _p(x), _p(nrm), _p(wi), _p(weight),
+                                             _p(_f32(bounds)), int(bf16), int(f64), _p(g))
+    return loss, g
+
+
+def guide_loss_f64(params64, x, nrm, wi, weight, bounds):
+    x, nrm, wi = (_f32(a).reshape(-1, 3) for a in (x, nrm, wi))
+    weight = _f32(weight).reshape(-1)
+    p = np.ascontiguousarray(params64, np.float64)
+    return _guide_decl().orc_guide_loss_f64(_p(p), len(x), _p(x), _p(nrm), _p(wi), _p(weight), _p(_f32(bounds)))
+
+
+def guide_pdf(outs, wi):
+    outs, wi = _f32(outs).reshape(-1, GUIDE_OUT), _f32(wi).reshape(-1, 3)
+    pdf = np.zeros(len(outs), np.float32)
+    _guide_decl().orc_guide_pdf(_p(outs), len(outs), _p(wi), _p(pdf))
+    return pdf
+
+
+def guide_sample(outs, u):
+    outs, u = _f32(outs).reshape(-1, GUIDE_OUT), _f32(u).reshape(-1, 3)
+    wi = np.zeros((len(outs), 3), np.float32)
+    pdf = np.zeros(len(outs), np.float32)
+    _guide_decl().orc_guide_sample(_p(outs), len(outs), _p(u), _p(wi), _p(pdf))
+    return wi, pdf
+
+
+def adam(p, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """In-place Adam step on float32 arrays p, m, v."""
+    _guide_decl().orc_adam(_p(p), _p(m), _p(v), _p(_f32(g)), len(p), int(t), lr, b1, b2, eps)

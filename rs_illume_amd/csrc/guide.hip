@@ -1,0 +1,347 @@
+// Learned sampling proposal: 144-64-64-64-40 ReLU MLP from the frequency
+// encoding of (normalised hit position, normal) to an 8-lobe vMF mixture.
+// Inference and training (forward, backward, Adam) run on MFMA with bf16
+// weights and activations and fp32 accumulation
+// (v_mfma_f32_16x16x32_bf16).
+//
+// One wave owns a tile of 32 rows (two 16-row MFMA tiles). Activations of
+// every layer stay in the wave's LDS; weights are read as MFMA B operands
+// straight from HBM/L2 (40 KB per copy, reused by every wave). Weight
+// gradients are the tile's A^T * dZ products, added to one fp32 gradient
+// vector with float atomics; Adam then updates the fp32 master weights and
+// rewrites both bf16 copies (n-major for forward B operands, k-major for the
+// backward dA = dZ * W^T products).
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include "guide_kernels.h"
+
+namespace illume {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRows = 32;    // rows per wave
+constexpr int kWave = 64;
+constexpr int kEnc = 144, kEncPad = 160, kHid = 64, kOut = 40, kOutPad = 48, kLobes = 8;
+
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// A operand of a row-major [rows][ld] bf16 matrix in LDS: rows r0..r0+15,
+// k0..k0+31. Lane l holds row r0 + (l & 15), k = k0 + (l >> 4) * 8 + 0..7.
+__device__ __forceinline__ bf16x8 lds_a(const __bf16 *m, int ld, int r0, int k0, int lane) {
+    return *reinterpret_cast<const bf16x8 *>(m + (r0 + (lane & 15)) * ld + k0 + (lane >> 4) * 8);
+}
+// B operand from an n-major [n][ld] bf16 matrix (global): columns n0..n0+15,
+// k0..k0+31. Lane l holds column n0 + (l & 15), k = k0 + (l >> 4) * 8 + 0..7.
+__device__ __forceinline__ bf16x8 glb_b(const __bf16 *__restrict__ w, int ld, int n0, int k0, int lane) {
+    return *reinterpret_cast<const bf16x8 *>(w + (n0 + (lane & 15)) * ld + k0 + (lane >> 4) * 8);
+}
+// Transposed operand from a row-major [rows][ld] LDS matrix: element (i, k) =
+// m[k][i]; i = i0 + (lane & 15), k = k0 + (lane >> 4) * 8 + 0..7.
+__device__ __forceinline__ bf16x8 lds_t(const __bf16 *m, int ld, int i0, int k0, int lane) {
+    bf16x8 v;
+    const int i = i0 + (lane & 15), k = k0 + (lane >> 4) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = m[(k + e) * ld + i];
+    return v;
+}
+
+struct WaveLds {
+    __bf16 a0[kRows * kEncPad];
+    __bf16 a1[kRows * kHid], a2[kRows * kHid], a3[kRows * kHid];
+    float y[kRows * kOutPad];
+    __bf16 dz[2][kRows * kHid];
+};
+
+// Frequency encoding: input i (position xyz in [0,1], normal xyz), octave j:
+// sin(2^j pi v) at 24 i + 2 j, cos at 24 i + 2 j + 1; columns 144..159 zero.
+__device__ void encode_tile(const float *__restrict__ in, int in_stride, uint32_t row0, uint32_t n, __bf16 *a0,
+                            int lane) {
+    for (int idx = lane; idx < kRows * kEncPad; idx += kWave) {
+        const int r = idx / kEncPad, f = idx % kEncPad;
+        float val = 0.f;
+        const uint32_t row = row0 + r;
+        if (f < kEnc && row < n) {
+            const int i = f / 24, j = (f % 24) / 2;
+            const float v = in[(size_t)row * in_stride + i];
+            const float arg = (float)((double)(1 << j) * 3.14159265358979323846) * v;
+            val = (f & 1) ? cosf(arg) : sinf(arg);
+        }
+        a0[idx] = to_bf16(val);
+    }
+}
+
+// out_tile = act(A[32 x K] * W[K x N] + b): W given n-major bf16 [N][K].
+template <int K, int N, bool RELU>
+__device__ void dense(const __bf16 *a, const __bf16 *__restrict__ wt, const float *__restrict__ b, __bf16 *out_bf,
+                      float *out_f, int lane) {
+#pragma unroll
+    for (int rt = 0; rt < kRows / 16; ++rt) {
+#pragma unroll
+        for (int ct = 0; ct < N / 16; ++ct) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k0 = 0; k0 < K; k0 += 32)
+                acc = mfma(lds_a(a, K, rt * 16, k0, lane), glb_b(wt, K, ct * 16, k0, lane), acc);
+            const int col = ct * 16 + (lane & 15);
+            const float bias = b[col];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = rt * 16 + (lane >> 4) * 4 + e;
+                float v = acc[e] + bias;
+                if (RELU) {
+                    v = v > 0.f ? v : 0.f;
+                    out_bf[row * N + col] = to_bf16(v);
+                } else {
+                    out_f[row * N + col] = v;
+                }
+            }
+        }
+    }
+}
+
+__device__ void forward_tile(const GuideDev &g, const float *__restrict__ in, int in_stride, uint32_t row0,
+                             uint32_t n, WaveLds &s, int lane) {
+    encode_tile(in, in_stride, row0, n, s.a0, lane);
+    __syncthreads();
+    dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.a1, nullptr, lane);
+    __syncthreads();
+    dense<kHid, kHid, true>(s.a1, g.wt[1], g.bias[1], s.a2, nullptr, lane);
+    __syncthreads();
+    dense<kHid, kHid, true>(s.a2, g.wt[2], g.bias[2], s.a3, nullptr, lane);
+    __syncthreads();
+    dense<kHid, kOutPad, false>(s.a3, g.wt[3], g.bias[3], nullptr, s.y, lane);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in, uint32_t n,
+                                                            float *__restrict__ out) {
+    __shared__ WaveLds s;
+    const int lane = threadIdx.x;
+    const uint32_t row0 = blockIdx.x * kRows;
+    forward_tile(g, in, 6, row0, n, s, lane);
+    for (int idx = lane; idx < kRows * kOut; idx += kWave) {
+        const int r = idx / kOut, c = idx % kOut;
+        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = s.y[r * kOutPad + c];
+    }
+}
+
+// d(-log q)/d outputs for one row (mixture decode as in the sampler).
+__device__ void nll_grad_row(const float *y, const float *w, float scale, float *gy, float *loss) {
+    float mx = y[0];
+    for (int j = 1; j < kLobes; ++j) mx = fmaxf(mx, y[5 * j]);
+    float pi[kLobes], s = 0.f;
+    for (int j = 0; j < kLobes; ++j) {
+        pi[j] = expf(y[5 * j] - mx);
+        s += pi[j];
+    }
+    float v[kLobes], mu[kLobes][3], kap[kLobes], mn[kLobes], q = 0.f;
+    bool clamped[kLobes];
+    for (int j = 0; j < kLobes; ++j) {
+        pi[j] /= s;
+        const float a = y[5 * j + 1], b = y[5 * j + 2], c = y[5 * j + 3];
+        const float len = sqrtf(a * a + b * b + c * c);
+        mn[j] = len;
+        const float inv = len > 0.f ? 1.f / len : 0.f;
+        mu[j][0] = a * inv;
+        mu[j][1] = b * inv;
+        mu[j][2] = len > 0.f ? c * inv : 1.f;
+        const float r = y[5 * j + 4];
+        clamped[j] = r < -4.6f || r > 6.9f;
+        kap[j] = expf(fminf(fmaxf(r, -4.6f), 6.9f));
+        const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        v[j] = kap[j] / (6.28318530717958647692f * (-expm1f(-2.f * kap[j]))) * expf(kap[j] * (cdot - 1.f));
+        q += pi[j] * v[j];
+    }
+    *loss = -scale * logf(q);
+    for (int j = 0; j < kLobes; ++j) {
+        const float gam = pi[j] * v[j] / q;
+        gy[5 * j] = scale * (pi[j] - gam);
+        const float k = kap[j];
+        const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        const float dlk = 1.f / k - 2.f * expf(-2.f * k) / (-expm1f(-2.f * k)) + (cdot - 1.f);
+        gy[5 * j + 4] = clamped[j] ? 0.f : scale * (-gam * dlk * k);
+        const float inv = mn[j] > 0.f ? 1.f / mn[j] : 0.f;
+        for (int a = 0; a < 3; ++a) gy[5 * j + 1 + a] = scale * (-gam * k * (w[a] - cdot * mu[j][a]) * inv);
+    }
+}
+
+// dW[in][out] += A^T dZ over the tile's rows; A [32][KA] (bf16, LDS),
+// dZ [32][NZ] (bf16, LDS); only in < in_valid and out < out_valid are stored.
+template <int KA, int NZ>
+__device__ void accumulate_dw(const __bf16 *a, const __bf16 *dz, float *__restrict__ gw, int in_valid, int out_valid,
+                              int lane) {
+    for (int it = 0; it < (in_valid + 15) / 16; ++it) {
+        for (int ot = 0; ot < (out_valid + 15) / 16; ++ot) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(lds_t(a, KA, it * 16, 0, lane), lds_t(dz, NZ, ot * 16, 0, lane), acc);
+            const int o = ot * 16 + (lane & 15);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = it * 16 + (lane >> 4) * 4 + e;
+                if (i < in_valid && o < out_valid) atomicAdd(gw + i * out_valid + o, acc[e]);
+            }
+        }
+    }
+}
+
+// dA[32][64] = dZ[32][KZ] * W^T with W k-major bf16 [64(in)][KZ(out)];
+// dZ_prev = dA * (A_prev > 0) -> bf16; db_prev accumulated by the caller's layer.
+template <int KZ>
+__device__ void backprop(const __bf16 *dz, const __bf16 *__restrict__ wk, const __bf16 *a_prev, __bf16 *dz_prev,
+                         float *__restrict__ gb_prev, int lane) {
+#pragma unroll
+    for (int rt = 0; rt < kRows / 16; ++rt) {
+#pragma unroll
+        for (int ct = 0; ct < kHid / 16; ++ct) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k0 = 0; k0 < KZ; k0 += 32)
+                acc = mfma(lds_a(dz, KZ, rt * 16, k0, lane), glb_b(wk, KZ, ct * 16, k0, lane), acc);
+            const int col = ct * 16 + (lane & 15);
+            float colsum = 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = rt * 16 + (lane >> 4) * 4 + e;
+                const float v = (float)a_prev[row * kHid + col] > 0.f ? acc[e] : 0.f;
+                dz_prev[row * kHid + col] = to_bf16(v);
+                colsum += v;
+            }
+            atomicAdd(gb_prev + col, colsum);
+        }
+    }
+}
+
+// records: [n][10] = position (3, normalised), normal (3), direction (3), weight
+__global__ __launch_bounds__(kWave) void guide_grad_kernel(GuideDev g, const float *__restrict__ rec, uint32_t n,
+                                                           const float *__restrict__ wsum, float *__restrict__ grad,
+                                                           float *__restrict__ loss) {
+    __shared__ WaveLds s;
+    const int lane = threadIdx.x;
+    const uint32_t row0 = blockIdx.x * kRows;
+    forward_tile(g, rec, 10, row0, n, s, lane);
+    const float ws = *wsum;
+    // output gradient: lane r < 32 owns row r
+    float row_loss = 0.f;
+    if (lane < kRows) {
+        float gy[kOut];
+        const uint32_t row = row0 + lane;
+        const float wgt = row < n ? rec[(size_t)row * 10 + 9] : 0.f;
+        if (row < n && wgt > 0.f && ws > 0.f) {
+            const float *r = rec + (size_t)row * 10;
+            const float w3[3] = {r[6], r[7], r[8]};
+            nll_grad_row(s.y + lane * kOutPad, w3, wgt / ws, gy, &row_loss);
+        } else {
+            for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
+        }
+        for (int o = 0; o < kHid; ++o) s.dz[0][lane * kHid + o] = to_bf16(o < kOut ? gy[o] : 0.f);
+        float *gb4 = grad + g.b_off[3];
+        for (int o = 0; o < kOut; ++o)
+            if (gy[o] != 0.f) atomicAdd(gb4 + o, gy[o]);
+    }
+    if (loss) {
+        float l = row_loss;
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+        if (lane == 0 && l != 0.f) atomicAdd(loss, l);
+    }
+    __syncthreads();
+    // layer 4: dW4 = A3^T dY, dZ3 = (dY W4^T) * (A3 > 0)
+    accumulate_dw<kHid, kHid>(s.a3, s.dz[0], grad + g.w_off[3], kHid, kOut, lane);
+    backprop<kHid>(s.dz[0], g.wk[3], s.a3, s.dz[1], grad + g.b_off[2], lane);
+    __syncthreads();
+    accumulate_dw<kHid, kHid>(s.a2, s.dz[1], grad + g.w_off[2], kHid, kHid, lane);
+    backprop<kHid>(s.dz[1], g.wk[2], s.a2, s.dz[0], grad + g.b_off[1], lane);
+    __syncthreads();
+    accumulate_dw<kHid, kHid>(s.a1, s.dz[0], grad + g.w_off[1], kHid, kHid, lane);
+    backprop<kHid>(s.dz[0], g.wk[1], s.a1, s.dz[1], grad + g.b_off[0], lane);
+    __syncthreads();
+    accumulate_dw<kEncPad, kHid>(s.a0, s.dz[1], grad + g.w_off[0], kEnc, kHid, lane);
+}
+
+__global__ void guide_wsum_kernel(const float *__restrict__ rec, uint32_t n, float *__restrict__ wsum) {
+    float s = 0.f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float w = rec[(size_t)i * 10 + 9];
+        if (w > 0.f) s += w;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0 && s != 0.f) atomicAdd(wsum, s);
+}
+
+__global__ void zero_kernel(float *__restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master weights, then the bf16 copies.
+__global__ void guide_adam_kernel(GuideDev g, const float *__restrict__ grad, float lr, float c1, float c2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)kGuideParams) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float gi = grad[i];
+    const float m = b1 * g.m[i] + (1.f - b1) * gi;
+    const float v = b2 * g.v[i] + (1.f - b2) * gi * gi;
+    g.m[i] = m;
+    g.v[i] = v;
+    const float p = g.params[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    g.params[i] = p;
+    write_bf16_copy(g, i, p);
+}
+
+__global__ void guide_sync_kernel(GuideDev g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (uint32_t)kGuideParams) write_bf16_copy(g, i, g.params[i]);
+}
+
+}  // namespace
+
+__device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p) {
+    for (int l = 0; l < 4; ++l) {
+        const uint32_t wo = g.w_off[l], bo = g.b_off[l];
+        const int din = g.din[l], dout = g.dout[l];
+        if (i >= wo && i < bo) {
+            const uint32_t k = (i - wo) / dout, o = (i - wo) % dout;
+            g.wt[l][o * g.wt_ld[l] + k] = (__bf16)p;   // n-major [out][in_pad]
+            g.wk[l][k * g.wk_ld[l] + o] = (__bf16)p;   // k-major [in][out_pad]
+            return;
+        }
+        if (i >= bo && i < bo + (uint32_t)dout) {
+            g.bias[l][i - bo] = p;
+            return;
+        }
+    }
+}
+
+hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, float *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    guide_infer_kernel<<<(n + kRows - 1) / kRows, kWave, 0, st>>>(g, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, float *grad, float *wsum, float *loss,
+                             hipStream_t st) {
+    zero_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(grad, kGuideParams);
+    zero_kernel<<<1, 64, 0, st>>>(wsum, 1);
+    if (loss) zero_kernel<<<1, 64, 0, st>>>(loss, 1);
+    if (n == 0) return hipGetLastError();
+    guide_wsum_kernel<<<256, 256, 0, st>>>(rec, n, wsum);
+    guide_grad_kernel<<<(n + kRows - 1) / kRows, kWave, 0, st>>>(g, rec, n, wsum, grad, loss);
+    return hipGetLastError();
+}
+
+hipError_t guide_launch_adam(const GuideDev &g, const float *grad, float lr, int step, hipStream_t st) {
+    const float c1 = 1.f - powf(0.9f, (float)step), c2 = 1.f - powf(0.999f, (float)step);
+    guide_adam_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(g, grad, lr, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t guide_launch_sync(const GuideDev &g, hipStream_t st) {
+    guide_sync_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(g);
+    return hipGetLastError();
+}
+
+}  // namespace illume
