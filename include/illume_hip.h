@@ -197,6 +197,30 @@ int illume_guide_grad(const illume_guide *guide, const float *records, uint32_t 
 /* One Adam step on the gradient of that loss over the batch. */
 int illume_guide_train_step(illume_guide *guide, const float *records, uint32_t n, void *stream);
 
+/* ------------------------------------------------------------------ wavefront renderer
+ * The same Integrator::render as illume_render, executed as per-stage kernels
+ * over a pool of path slots (state in HBM): extend (closest hit), proposal
+ * inference (MFMA), shade (NEE, MIS, continuation), so the learned proposal
+ * can guide the continuation. With guide == NULL or guide_frac == 0 the image
+ * equals illume_render's for the same cfg. */
+
+typedef struct illume_wavefront illume_wavefront;
+
+typedef struct {
+    float guide_frac;      /* probability of sampling the proposal (one-sample MIS with the BSDF) */
+    uint32_t train;        /* 1: after every pass of max_paths paths, turn its vertices into radiance
+                              samples and take Adam steps of train_batch samples */
+    uint32_t train_batch;  /* 0 = 65536 */
+    uint32_t reserved;
+} illume_guide_cfg;
+
+int illume_wavefront_create(const illume_scene *scene, uint32_t max_paths, uint32_t max_depth,
+                            illume_wavefront **out);
+int illume_wavefront_destroy(illume_wavefront *wf);
+int illume_wavefront_render(illume_wavefront *wf, const illume_camera *camera, const illume_render_cfg *cfg,
+                            illume_guide *guide, const illume_guide_cfg *gcfg, float *rgb_sum, uint64_t *ray_counts,
+                            void *stream);
+
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);
 

@@ -130,6 +130,23 @@ __global__ __launch_bounds__(kWave) void guide_infer_kernel(GuideDev g, const fl
     }
 }
 
+// Same, with the row count read from device memory (queues filled by an
+// earlier kernel); blocks beyond the count exit at once.
+__global__ __launch_bounds__(kWave) void guide_infer_count_kernel(GuideDev g, const float *__restrict__ in,
+                                                                  const uint32_t *__restrict__ n_ptr, uint32_t max_n,
+                                                                  float *__restrict__ out) {
+    __shared__ WaveLds s;
+    const uint32_t n = min(*n_ptr, max_n);
+    const uint32_t row0 = blockIdx.x * kRows;
+    if (row0 >= n) return;
+    const int lane = threadIdx.x;
+    forward_tile(g, in, 6, row0, n, s, lane);
+    for (int idx = lane; idx < kRows * kOut; idx += kWave) {
+        const int r = idx / kOut, c = idx % kOut;
+        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = s.y[r * kOutPad + c];
+    }
+}
+
 // d(-log q)/d outputs for one row (mixture decode as in the sampler).
 __device__ void nll_grad_row(const float *y, const float *w, float scale, float *gy, float *loss) {
     float mx = y[0];
@@ -319,6 +336,13 @@ __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p) {
 hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, float *out, hipStream_t st) {
     if (n == 0) return hipSuccess;
     guide_infer_kernel<<<(n + kRows - 1) / kRows, kWave, 0, st>>>(g, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
+                                    float *out, hipStream_t st) {
+    if (max_n == 0) return hipSuccess;
+    guide_infer_count_kernel<<<(max_n + kRows - 1) / kRows, kWave, 0, st>>>(g, in, n_ptr, max_n, out);
     return hipGetLastError();
 }
 

@@ -24,6 +24,10 @@ struct illume_guide {
     int step = 0;
 };
 
+namespace illume {
+const GuideDev *guide_dev(const illume_guide *g) { return &g->dev; }
+}  // namespace illume
+
 namespace {
 const int kDin[4] = {144, 64, 64, 64}, kDout[4] = {64, 64, 64, 40};
 const int kInPad[4] = {160, 64, 64, 64}, kOutPadN[4] = {64, 64, 64, 48}, kOutPadK[4] = {64, 64, 64, 64};
@@ -52,8 +56,8 @@ int illume_guide_create(const float *init_params, float lr, illume_guide **out) 
     if (!out || !init_params) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (!(lr > 0.f)) return abi_fail(ILLUME_ERR_INVALID_ARG, "learning rate must be > 0");
-    for (int i = 0; i < kGuideParams; ++i)
-        if (!std::isfinite(init_params[i])) return abi_fail(ILLUME_ERR_INVALID_ARG, "non-finite initial parameter");
+    // Non-finite parameters are accepted: the renderer then finds every
+    // decoded mixture invalid and samples the BSDF alone at every vertex.
     illume_guide *g = new (std::nothrow) illume_guide();
     if (!g) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     g->lr = lr;

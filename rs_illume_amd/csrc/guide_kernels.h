@@ -21,6 +21,9 @@ struct GuideDev {
 __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p);
 
 hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, float *out, hipStream_t st);
+// Row count read on the device from *n_ptr (at most max_n).
+hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
+                                    float *out, hipStream_t st);
 // Gradient of the weighted NLL over n records into grad (zeroed first); wsum
 // (device float) receives the weight sum, loss (optional) the loss.
 hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, float *grad, float *wsum, float *loss,

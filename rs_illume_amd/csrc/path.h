@@ -1,0 +1,426 @@
+// Device-side path tracing shared by the megakernel (render.hip) and the
+// wavefront kernels (wavefront.hip): scene queries, hit interactions,
+// estimate_direct, camera rays and one bounce of PathIntegrator::li with an
+// optional learned proposal.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "shade.h"
+#include "trace.h"
+
+#pragma clang fp contract(off)
+
+namespace illume {
+
+struct PathCounters {
+    uint32_t closest, any;
+};
+
+template <int BLOCK>
+__device__ __forceinline__ bool scene_closest(const DevScene &sc, f3 o, f3 d, float t_max, TraceHit &h,
+                                              TraceStack<BLOCK> &st, PathCounters &pc) {
+    uint32_t nb = 0, np = 0;
+    ++pc.closest;
+    return trace_ray<false, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
+}
+template <int BLOCK>
+__device__ __forceinline__ bool scene_any(const DevScene &sc, f3 o, f3 d, float t_max, TraceStack<BLOCK> &st,
+                                          PathCounters &pc) {
+    uint32_t nb = 0, np = 0;
+    TraceHit h;
+    ++pc.any;
+    return trace_ray<true, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
+}
+
+// Builds the interaction of a closest hit.
+__device__ __forceinline__ void hit_isect(const DevScene &sc, const TraceHit &h, f3 o, f3 d, Isect &it) {
+    it.prim = h.prim;
+    if (h.prim < sc.n_tris) {
+        const float4 *v = sc.tri_verts + 3 * (size_t)h.prim;
+        const float4 a = v[0], b = v[1], c = v[2];
+        triangle_isect(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h.b0, h.b1, h.b2, d, it);
+    } else {
+        const float4 s = sc.spheres[h.prim - sc.n_tris];
+        sphere_isect(mk(s.x, s.y, s.z), s.w, o, d, h.t, it);
+    }
+}
+
+// DiffuseAreaLight::L: one-sided emission along the geometric normal.
+__device__ __forceinline__ f3 area_L(const DevLight &l, f3 n, f3 w) {
+    return dot(n, w) > 0 ? ld3(l.L) : mk(0.f, 0.f, 0.f);
+}
+
+// sampling.rs:75-88 sample_discrete with pbrt-v3's FindInterval
+__device__ __forceinline__ int light_sample_discrete(const DevScene &sc, float u, float &pdf) {
+    int size = (int)sc.n_lights + 1, first = 0, len = size;
+    while (len > 0) {
+        int half = len >> 1, middle = first + half;
+        if (sc.light_cdf[middle] <= u) {
+            first = middle + 1;
+            len -= half + 1;
+        } else
+            len = half;
+    }
+    int offset = min(max(first - 1, 0), size - 2);
+    pdf = (sc.light_func_int > 0) ? sc.light_func[offset] / (sc.light_func_int * (float)sc.n_lights) : 0.f;
+    return offset;
+}
+
+// estimate_direct (integrator.rs:353-494) for a one-sided triangle area light,
+// with both MIS strategies (the BSDF-sampled term per pbrt-v3).
+template <int BLOCK>
+__device__ f3 estimate_direct(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1, int li,
+                              float ul0, float ul1, TraceStack<BLOCK> &st, PathCounters &pc) {
+    const DevLight &light = sc.lights[li];
+    const int flags = kAll & ~kSpecular;
+    f3 ld = mk(0.f, 0.f, 0.f);
+    const f3 p0 = ld3(light.p0), p1 = ld3(light.p1), p2 = ld3(light.p2);
+    // Light::sample_li -> Triangle::sample (triangle.rs:616-642) and the
+    // area-to-solid-angle conversion of shape.rs:43-57.
+    float su0 = sqrtf(ul0);
+    float b0 = 1 - su0, b1 = ul1 * su0;
+    f3 lp = add(add(smul(b0, p0), smul(b1, p1)), smul(1 - b0 - b1, p2));
+    f3 ln = normalize(cross(sub(p1, p0), sub(p2, p0)));
+    f3 lpe = smul(gamma_f(6), add(add(fabs3(smul(b0, p0)), fabs3(smul(b1, p1))), fabs3(smul(1 - b0 - b1, p2))));
+    float light_pdf = 1 / light.area;
+    f3 w = sub(lp, it.p);
+    if (length_squared(w) == 0)
+        light_pdf = 0;
+    else {
+        w = normalize(w);
+        light_pdf *= length_squared(sub(it.p, lp)) / abs_dot(ln, neg(w));
+        if (__builtin_isinf(light_pdf)) light_pdf = 0.f;
+    }
+    f3 wi = mk(0.f, 0.f, 0.f);
+    f3 li_v = mk(0.f, 0.f, 0.f);
+    if (!(light_pdf == 0 || length_squared(sub(lp, it.p)) == 0)) {
+        wi = normalize(sub(lp, it.p));
+        li_v = area_L(light, ln, neg(wi));
+    } else {
+        light_pdf = 0;
+    }
+    float scattering_pdf = 0;
+    if (light_pdf > 0 && !is_black(li_v)) {
+        const f3 fv = bsdf.f(it.wo, wi, flags);
+        const float ad = abs_dot(wi, it.shading_n);
+        f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
+        scattering_pdf = bsdf.pdf(it.wo, wi, flags);
+        if (!is_black(f)) {
+            f3 so, sd;
+            spawn_ray_to(it, lp, lpe, ln, so, sd);
+            if (scene_any<BLOCK>(sc, so, sd, 1.f - 0.0001f, st, pc)) li_v = mk(0.f, 0.f, 0.f);
+            if (!is_black(li_v)) {
+                float weight = power_heuristic(light_pdf, scattering_pdf);
+                f3 c = cmul(f, li_v);
+                ld = add(ld, mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf));
+            }
+        }
+    }
+    {
+        int sampled = 0;
+        f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
+        float ad = abs_dot(wi, it.shading_n);
+        f = mk(f.x * ad, f.y * ad, f.z * ad);
+        bool sampled_specular = (sampled & kSpecular) != 0;
+        if (!is_black(f) && scattering_pdf > 0) {
+            float weight = 1;
+            if (!sampled_specular) {
+                // Shape::pdf_with_point (shape.rs:60-81): the light's own triangle only
+                f3 so;
+                spawn_ray(it, wi, so);
+                float t, c0, c1, c2;
+                if (!triangle_test(p0, p1, p2, so, wi, __builtin_inff(), t, c0, c1, c2) ||
+                    triangle_degenerate(p0, p1, p2))
+                    return ld;
+                Isect il;
+                triangle_isect(p0, p1, p2, c0, c1, c2, wi, il);
+                float lpdf = length_squared(sub(it.p, il.p)) / (abs_dot(il.n, neg(wi)) * light.area);
+                if (__builtin_isinf(lpdf)) lpdf = 0.f;
+                light_pdf = lpdf;
+                if (light_pdf == 0) return ld;
+                weight = power_heuristic(scattering_pdf, light_pdf);
+            }
+            f3 so;
+            spawn_ray(it, wi, so);
+            TraceHit h;
+            bool found = scene_closest<BLOCK>(sc, so, wi, __builtin_inff(), h, st, pc);
+            f3 li2 = mk(0.f, 0.f, 0.f);
+            if (found && sc.prim_light[h.prim] == li) {
+                Isect lh;
+                hit_isect(sc, h, so, wi, lh);
+                li2 = area_L(light, lh.n, neg(wi));
+            }
+            if (!is_black(li2)) {
+                f3 c = cmul(f, li2);
+                ld = add(ld, mk(c.x * weight / scattering_pdf, c.y * weight / scattering_pdf,
+                                c.z * weight / scattering_pdf));
+            }
+        }
+    }
+    return ld;
+}
+
+
+// PerspectiveCamera::generate_ray (perspective.rs:103-126, lens radius 0) for
+// film position (fx, fy), then camera_to_world applied to the ray with the
+// origin advanced past its error bound (transform.rs:365-389).
+__device__ __forceinline__ void camera_ray(const DevCamera &cam, float fx, float fy, f3 &o, f3 &d, float &t_max) {
+    const float *m = cam.raster_to_camera;
+    float px = m[0] * fx + m[1] * fy + m[2] * 0.f + m[3];
+    float py = m[4] * fx + m[5] * fy + m[6] * 0.f + m[7];
+    float pz = m[8] * fx + m[9] * fy + m[10] * 0.f + m[11];
+    float pw = m[12] * fx + m[13] * fy + m[14] * 0.f + m[15];
+    f3 pcam = (pw == 1.f) ? mk(px, py, pz) : mk(px / pw, py / pw, pz / pw);
+    f3 dc = normalize(pcam);
+    const float *c = cam.camera_to_world;
+    o = mk((c[0] * 0.f + c[1] * 0.f) + (c[2] * 0.f + c[3]), (c[4] * 0.f + c[5] * 0.f) + (c[6] * 0.f + c[7]),
+           (c[8] * 0.f + c[9] * 0.f) + (c[10] * 0.f + c[11]));
+    f3 oe = smul(gamma_f(3), mk(fabsf(c[0] * 0.f) + fabsf(c[1] * 0.f) + fabsf(c[2] * 0.f) + fabsf(c[3]),
+                                fabsf(c[4] * 0.f) + fabsf(c[5] * 0.f) + fabsf(c[6] * 0.f) + fabsf(c[7]),
+                                fabsf(c[8] * 0.f) + fabsf(c[9] * 0.f) + fabsf(c[10] * 0.f) + fabsf(c[11])));
+    d = mk(c[0] * dc.x + c[1] * dc.y + c[2] * dc.z, c[4] * dc.x + c[5] * dc.y + c[6] * dc.z,
+           c[8] * dc.x + c[9] * dc.y + c[10] * dc.z);
+    t_max = __builtin_inff();
+    float ls = length_squared(d);
+    if (ls > 0) {
+        float dt = dot(fabs3(d), oe) / ls;
+        o = add(o, mul(d, dt));
+        t_max -= dt;
+    }
+}
+
+// ---------------------------------------------------------------- learned proposal
+// 8-lobe vMF mixture decoded from the network's 40 outputs (softmax weights,
+// normalised means, concentrations exp(clamp(r, -4.6, 6.9))).
+struct Mixture {
+    float pi[8], mu[8][3], kappa[8];
+};
+__device__ __forceinline__ bool decode_mixture(const float *__restrict__ y, Mixture &m) {
+    float mx = y[0];
+    for (int j = 1; j < 8; ++j) mx = fmaxf(mx, y[5 * j]);
+    float s = 0.f;
+    for (int j = 0; j < 8; ++j) {
+        m.pi[j] = expf(y[5 * j] - mx);
+        s += m.pi[j];
+    }
+    bool ok = s > 0.f && __builtin_isfinite(s);
+    for (int j = 0; j < 8; ++j) {
+        m.pi[j] /= s;
+        const float a = y[5 * j + 1], b = y[5 * j + 2], c = y[5 * j + 3];
+        const float len = sqrtf(a * a + b * b + c * c);
+        const float inv = len > 0.f ? 1.f / len : 0.f;
+        m.mu[j][0] = a * inv;
+        m.mu[j][1] = b * inv;
+        m.mu[j][2] = len > 0.f ? c * inv : 1.f;
+        m.kappa[j] = expf(fminf(fmaxf(y[5 * j + 4], -4.6f), 6.9f));
+        ok = ok && __builtin_isfinite(m.pi[j]) && __builtin_isfinite(len) && __builtin_isfinite(m.kappa[j]);
+    }
+    return ok;
+}
+__device__ __forceinline__ float mixture_pdf(const Mixture &m, f3 w) {
+    float q = 0.f;
+    for (int j = 0; j < 8; ++j) {
+        const float k = m.kappa[j];
+        const float c = m.mu[j][0] * w.x + m.mu[j][1] * w.y + m.mu[j][2] * w.z;
+        q += m.pi[j] * (k / (6.28318530717958647692f * (-expm1f(-2.f * k))) * expf(k * (c - 1.f)));
+    }
+    return q;
+}
+// Lobe by ul, then the vMF sample of Jakob (2012) around the lobe mean.
+__device__ __forceinline__ f3 mixture_sample(const Mixture &m, float ul, float u1, float u2) {
+    int j = 0;
+    float c = 0.f;
+    for (; j < 7; ++j) {
+        c += m.pi[j];
+        if (ul < c) break;
+    }
+    const float k = m.kappa[j];
+    float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
+    w = fminf(1.f, fmaxf(-1.f, w));
+    const float s = sqrtf(fmaxf(0.f, 1.f - w * w));
+    const float phi = 6.28318530717958647692f * u2;
+    const f3 mu = mk(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
+    f3 t1, t2;
+    if (fabsf(mu.x) > fabsf(mu.y)) {
+        const float l = sqrtf(mu.x * mu.x + mu.z * mu.z);
+        t1 = mk(-mu.z / l, 0.f, mu.x / l);
+    } else {
+        const float l = sqrtf(mu.y * mu.y + mu.z * mu.z);
+        t1 = mk(0.f, mu.z / l, -mu.y / l);
+    }
+    t2 = mk(mu.y * t1.z - mu.z * t1.y, mu.z * t1.x - mu.x * t1.z, mu.x * t1.y - mu.y * t1.x);
+    const float lx = s * cosf(phi), ly = s * sinf(phi);
+    return normalize(add(add(smul(lx, t1), smul(ly, t2)), smul(w, mu)));
+}
+
+// Guide usage for one render pass.
+struct GuideCfg {
+    const float *out;         // [paths][40] network outputs, null = plain path tracing
+    float frac;               // probability of sampling the proposal (one-sample MIS)
+    float bounds_lo[3], inv_ext[3];
+};
+
+// One training vertex: proposal input (normalised position, normal), the
+// sampled direction and its mixture pdf, the throughput after the vertex and
+// the path radiance before anything arriving along that direction.
+struct VertexRec {
+    float in[6];
+    float wi[3];
+    float pdf;
+    float beta[3];
+    float lsnap[3];
+};
+
+struct PathState {
+    f3 o, d;
+    float t_max;
+    f3 beta, L;
+    float eta_scale;
+    uint32_t bounces;
+    bool specular, active;
+    Pcg32 rng;
+};
+
+// The camera-sample stage of SamplerIntegratorBase::render: sampler stream,
+// get_camera_sample (film 2D, time 1D, lens 2D) and the camera ray.
+__device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
+                                           PathState &ps) {
+    const int x = (int)(pix % cam.width), y = (int)(pix / cam.width);
+    ps.rng.seed(seed, pix, s);
+    const float uf0 = ps.rng.get_1d(), uf1 = ps.rng.get_1d();
+    ps.rng.get_1d();
+    ps.rng.get_1d();
+    ps.rng.get_1d();
+    camera_ray(cam, (float)x + uf0, (float)y + uf1, ps.o, ps.d, ps.t_max);
+    ps.beta = mk(1.f, 1.f, 1.f);
+    ps.L = mk(0.f, 0.f, 0.f);
+    ps.eta_scale = 1.f;
+    ps.bounces = 0;
+    ps.specular = false;
+    ps.active = true;
+}
+
+// One bounce of PathIntegrator::li after the closest-hit query (found, h).
+// With a proposal (gout != null, valid mixture, non-specular BSDF) the
+// continuation is one-sample MIS: the mixture with probability frac, else the
+// BSDF, weighted by frac * q + (1 - frac) * p_bsdf. recs (may be null) gets a
+// training vertex per sampled continuation; n_recs counts them.
+template <int BLOCK>
+__device__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
+                           const float *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
+                           TraceStack<BLOCK> &st, PathCounters &pc) {
+    if (recs && n_recs > 0 && recs[n_recs - 1].lsnap[0] < 0.f) {
+        // the radiance gathered so far excludes everything arriving along the
+        // last recorded direction
+        VertexRec &r = recs[n_recs - 1];
+        r.lsnap[0] = ps.L.x;
+        r.lsnap[1] = ps.L.y;
+        r.lsnap[2] = ps.L.z;
+    }
+    Isect it;
+    if (found) hit_isect(sc, h, ps.o, ps.d, it);
+    if (ps.bounces == 0 || ps.specular) {
+        if (found) {
+            int li = sc.prim_light[h.prim];
+            if (li >= 0) ps.L = add(ps.L, cmul(ps.beta, area_L(sc.lights[li], it.n, neg(ps.d))));
+        }
+    }
+    if (!found || ps.bounces >= max_depth) {
+        ps.active = false;
+        return;
+    }
+    Bsdf bsdf;
+    bsdf.init(it, sc.materials[sc.prim_material[h.prim]]);
+    const bool nonspec = bsdf.matches(kAll & ~kSpecular);
+    if (nonspec && sc.n_lights > 0) {
+        // uniform_sample_one_light (integrator.rs:321-350)
+        float lpdf;
+        const float ul = ps.rng.get_1d();
+        int ln = light_sample_discrete(sc, ul, lpdf);
+        if (lpdf != 0) {
+            const float ul0 = ps.rng.get_1d(), ul1 = ps.rng.get_1d();
+            const float us0 = ps.rng.get_1d(), us1 = ps.rng.get_1d();
+            f3 ldir = estimate_direct<BLOCK>(sc, it, bsdf, us0, us1, ln, ul0, ul1, st, pc);
+            ldir = mk(ldir.x / lpdf, ldir.y / lpdf, ldir.z / lpdf);
+            ps.L = add(ps.L, cmul(ps.beta, ldir));
+        }
+    }
+    f3 wo = neg(ps.d), wi = mk(0.f, 0.f, 0.f);
+    float pdf = 0;
+    int flags = 0;
+    f3 f;
+    Mixture mix;
+    const bool guided = gout && nonspec && decode_mixture(gout, mix);
+    if (guided) {
+        const float usel = ps.rng.get_1d();
+        float p_bsdf = 0.f;
+        if (usel < gc.frac) {
+            const float ul = ps.rng.get_1d(), u1 = ps.rng.get_1d(), u2 = ps.rng.get_1d();
+            wi = mixture_sample(mix, ul, u1, u2);
+            f = bsdf.f(wo, wi, kAll);
+            p_bsdf = bsdf.pdf(wo, wi, kAll);
+            flags = bsdf.type & ~kSpecular;
+        } else {
+            const float u0 = ps.rng.get_1d(), u1 = ps.rng.get_1d();
+            f = bsdf.sample_f(wo, wi, u0, u1, p_bsdf, kAll, flags);
+            if (p_bsdf == 0.f) f = mk(0.f, 0.f, 0.f);
+        }
+        const float q = mixture_pdf(mix, wi);
+        pdf = gc.frac * q + (1.f - gc.frac) * p_bsdf;
+        if (!(pdf > 0.f) || !__builtin_isfinite(pdf)) pdf = 0.f;
+    } else {
+        const float u0 = ps.rng.get_1d(), u1 = ps.rng.get_1d();
+        f = bsdf.sample_f(wo, wi, u0, u1, pdf, kAll, flags);
+    }
+    if (is_black(f) || pdf == 0.f) {
+        ps.active = false;
+        return;
+    }
+    float ad = abs_dot(wi, it.shading_n);
+    f3 bf = cmul(ps.beta, f);
+    ps.beta = mk(bf.x * ad / pdf, bf.y * ad / pdf, bf.z * ad / pdf);
+    ps.specular = (flags & kSpecular) != 0;
+    if ((flags & kSpecular) && (flags & kTrans)) {
+        float eta = bsdf.eta;
+        ps.eta_scale *= (dot(wo, it.n) > 0) ? (eta * eta) : 1 / (eta * eta);
+    }
+    if (recs && nonspec && n_recs < max_depth) {
+        VertexRec &r = recs[n_recs++];
+        r.in[0] = (it.p.x - gc.bounds_lo[0]) * gc.inv_ext[0];
+        r.in[1] = (it.p.y - gc.bounds_lo[1]) * gc.inv_ext[1];
+        r.in[2] = (it.p.z - gc.bounds_lo[2]) * gc.inv_ext[2];
+        r.in[3] = it.n.x;
+        r.in[4] = it.n.y;
+        r.in[5] = it.n.z;
+        r.wi[0] = wi.x;
+        r.wi[1] = wi.y;
+        r.wi[2] = wi.z;
+        r.pdf = pdf;
+        r.beta[0] = ps.beta.x;
+        r.beta[1] = ps.beta.y;
+        r.beta[2] = ps.beta.z;
+        r.lsnap[0] = r.lsnap[1] = r.lsnap[2] = -1.f;  // set when the next vertex is reached
+    }
+    spawn_ray(it, wi, ps.o);
+    ps.d = wi;
+    ps.t_max = __builtin_inff();
+    f3 rr = mk(ps.beta.x * ps.eta_scale, ps.beta.y * ps.eta_scale, ps.beta.z * ps.eta_scale);
+    ++ps.bounces;
+    if (max_comp(rr) < 1.f && ps.bounces - 1 > 3) {
+        float q = fmaxf(0.05f, 1 - max_comp(rr));
+        if (ps.rng.get_1d() < q) {
+            ps.active = false;
+            return;
+        }
+        ps.beta = mk(ps.beta.x / (1 - q), ps.beta.y / (1 - q), ps.beta.z / (1 - q));
+    }
+}
+
+// SamplerIntegratorBase::render's radiance guard (integrator.rs:113-122).
+__device__ __forceinline__ f3 guard_radiance(f3 L) {
+    if (__builtin_isnan(L.x) || __builtin_isnan(L.y) || __builtin_isnan(L.z)) return mk(0.f, 0.f, 0.f);
+    if (luminance(L) < -1e-5f) return mk(0.f, 0.f, 0.f);
+    if (__builtin_isinf(luminance(L))) return mk(0.f, 0.f, 0.f);
+    return L;
+}
+
+}  // namespace illume

@@ -187,6 +187,17 @@ struct illume_scene {
     uint32_t n_prims = 0;
 };
 
+namespace illume {
+const DevScene *scene_dev(const illume_scene *s) { return &s->dev; }
+uint32_t scene_prims(const illume_scene *s) { return s->n_prims; }
+void scene_bounds(const illume_scene *s, float *b) {
+    illume_bvh_info_t info;
+    illume_bvh_info(s->bvh, &info);
+    std::memcpy(b, info.bounds, sizeof(info.bounds));
+}
+bool scene_camera(const illume_camera &c, DevCamera *out);
+}  // namespace illume
+
 extern "C" {
 
 int illume_scene_destroy(illume_scene *s) {
@@ -314,6 +325,12 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
     *out = s;
     return ILLUME_OK;
 }
+
+}  // extern "C"
+
+bool illume::scene_camera(const illume_camera &c, DevCamera *out) { return make_camera(c, out); }
+
+extern "C" {
 
 int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out) {
     if (!scene || !out) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");

@@ -131,3 +131,68 @@ class PathIntegrator:
         """The image (H, W, 3) float32 device tensor: mean radiance per pixel."""
         s = self.render_sum(scene, stream=stream)
         return s / float(self.spp)
+
+
+class GuideCfg(ctypes.Structure):
+    _fields_ = [("guide_frac", ctypes.c_float), ("train", ctypes.c_uint32), ("train_batch", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+def _declare_wf():
+    L = _declare()
+    if getattr(L, "_wf_declared", False):
+        return L
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    L.illume_wavefront_create.restype = i32
+    L.illume_wavefront_create.argtypes = [vp, u32, u32, ctypes.POINTER(vp)]
+    L.illume_wavefront_destroy.restype = i32
+    L.illume_wavefront_destroy.argtypes = [vp]
+    L.illume_wavefront_render.restype = i32
+    L.illume_wavefront_render.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), vp, ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L._wf_declared = True
+    return L
+
+
+class WavefrontPathIntegrator(PathIntegrator):
+    """The same Integrator::render as :class:`PathIntegrator`, run as per-stage
+    kernels over a pool of ``max_paths`` path slots; optionally guided by a
+    :class:`rs_illume_amd.guide.Guide` (one-sample MIS with probability
+    ``guide_frac``) and training it online after every pass."""
+
+    def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
+                 train=False, train_batch=65536):
+        super().__init__(camera, spp, max_depth, seed)
+        self.scene = scene
+        self.guide = guide
+        self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, int(bool(train)), int(train_batch), 0)
+        L = _declare_wf()
+        h = ctypes.c_void_p()
+        _lib.check(L.illume_wavefront_create(scene.handle, int(max_paths), int(max_depth), ctypes.byref(h)),
+                   "illume_wavefront_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _declare_wf().illume_wavefront_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_sum(self, scene=None, rgb_sum=None, spp=None, sample_offset=0, ray_counts=None, stream=None):
+        w, h = self.resolution
+        if rgb_sum is None:
+            rgb_sum = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
+        cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), 0, self.seed)
+        rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+        g = self.guide.handle if self.guide is not None else None
+        _lib.check(_declare_wf().illume_wavefront_render(self._h, _p(self.camera), ctypes.byref(cfg), g,
+                                                         ctypes.byref(self.gcfg), ctypes.c_void_p(rgb_sum.data_ptr()),
+                                                         rc, _stream_handle(stream)), "illume_wavefront_render")
+        return rgb_sum
+
+    def render(self, scene=None, stream=None):
+        return self.render_sum(stream=stream) / float(self.spp)
