@@ -199,27 +199,46 @@ int illume_guide_train_step(illume_guide *guide, const float *records, uint32_t 
 
 /* ------------------------------------------------------------------ wavefront renderer
  * The same Integrator::render as illume_render, executed as per-stage kernels
- * over a pool of path slots (state in HBM): extend (closest hit), proposal
- * inference (MFMA), shade (NEE, MIS, continuation), so the learned proposal
- * can guide the continuation. With guide == NULL or guide_frac == 0 the image
- * equals illume_render's for the same cfg. */
+ * over a pool of path slots whose state lives in HBM: regenerate (camera
+ * paths into free slots), extend (closest hit), proposal inference (MFMA),
+ * shade (NEE, MIS, continuation; survivors compacted into the next queue).
+ * Iterations are captured in a hipGraph and replayed; one render call can
+ * batch many scenes and cameras of uneven size (jobs) through one pool.
+ * With guide == NULL or guide_frac == 0 the image equals illume_render's. */
 
 typedef struct illume_wavefront illume_wavefront;
 
 typedef struct {
     float guide_frac;      /* probability of sampling the proposal (one-sample MIS with the BSDF) */
-    uint32_t train;        /* 1: after every pass of max_paths paths, turn its vertices into radiance
-                              samples and take Adam steps of train_batch samples */
+    uint32_t train;        /* 1: finished paths emit radiance samples; Adam steps run whenever
+                              train_batch samples are ready */
     uint32_t train_batch;  /* 0 = 65536 */
     uint32_t reserved;
 } illume_guide_cfg;
 
-int illume_wavefront_create(const illume_scene *scene, uint32_t max_paths, uint32_t max_depth,
-                            illume_wavefront **out);
+/* One render of a batch: its image occupies width*height*3 floats of the
+ * film buffer starting at film_offset. */
+typedef struct {
+    const illume_scene *scene;
+    illume_camera camera;
+    illume_render_cfg cfg;
+    uint64_t film_offset;
+} illume_job;
+
+int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavefront **out);
 int illume_wavefront_destroy(illume_wavefront *wf);
-int illume_wavefront_render(illume_wavefront *wf, const illume_camera *camera, const illume_render_cfg *cfg,
-                            illume_guide *guide, const illume_guide_cfg *gcfg, float *rgb_sum, uint64_t *ray_counts,
-                            void *stream);
+/* Launch shape: threads per block of the extend/shade stages (64/128/256),
+ * regenerate only while fewer than regen_min slots are live (max_paths+1 =
+ * always), iterations per captured graph (0 = no graph). Chosen by the
+ * autotuner (rs_illume_amd/autotune.py). */
+int illume_wavefront_set_tuning(illume_wavefront *wf, uint32_t block, uint32_t regen_min, uint32_t graph_iters);
+/* Iterations the last render call ran. */
+uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf);
+int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
+                            const illume_render_cfg *cfg, illume_guide *guide, const illume_guide_cfg *gcfg,
+                            float *rgb_sum, uint64_t *ray_counts, void *stream);
+int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
+                                  const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream);
 
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);

@@ -183,12 +183,14 @@ hipError_t upload(T **dst, const std::vector<T> &src) {
 struct illume_scene {
     illume_bvh *bvh = nullptr;
     DevScene dev;
+    DevScene *d_dev = nullptr;  // the same record in device memory (job tables point at it)
     std::vector<void *> allocs;
     uint32_t n_prims = 0;
 };
 
 namespace illume {
 const DevScene *scene_dev(const illume_scene *s) { return &s->dev; }
+const DevScene *scene_dev_device(const illume_scene *s) { return s->d_dev; }
 uint32_t scene_prims(const illume_scene *s) { return s->n_prims; }
 void scene_bounds(const illume_scene *s, float *b) {
     illume_bvh_info_t info;
@@ -322,6 +324,15 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
     s->dev.light_func_int = func_int;
     s->dev.n_lights = (uint32_t)lights.size();
     s->dev.n_tris = n_tris;
+    e = hipMalloc(&s->d_dev, sizeof(DevScene));
+    if (e == hipSuccess) {
+        s->allocs.push_back(s->d_dev);
+        e = hipMemcpy(s->d_dev, &s->dev, sizeof(DevScene), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        illume_scene_destroy(s);
+        return abi_hip_fail(e, "scene upload");
+    }
     *out = s;
     return ILLUME_OK;
 }

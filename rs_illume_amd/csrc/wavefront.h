@@ -1,31 +1,60 @@
-// Path-slot pool of the wavefront path tracer (structure of arrays in HBM).
+// Path-slot pool of the wavefront path tracer (structure of arrays in HBM)
+// and its device-side work counters.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "path.h"
 
 namespace illume {
 
+// One render job: a scene, a camera and a range of sample indices, writing to
+// its own region of the film buffer. The paths of all jobs form one global
+// index space: job j owns [path_begin, path_begin + width*height*spp).
+struct WfJob {
+    const DevScene *scene;  // device pointer
+    DevCamera cam;
+    unsigned long long path_begin;
+    unsigned long long film_offset;  // float offset of the job's image in the film buffer
+    uint64_t seed;
+    uint32_t spp, sample_offset, max_depth, pad;
+    GuideCfg gc;
+};
+
+struct WfCounters {
+    uint32_t n_q[2];      // lengths of the two slot queues
+    uint32_t n_free;      // free slots
+    uint32_t parity;      // queue read this iteration
+    uint32_t guide_count; // rows in the proposal queue
+    uint32_t take;        // slots (re)started by the last regenerate stage
+    unsigned long long next_path, total_paths, done_paths;
+    unsigned long long rec_count;  // training samples written
+};
+
 struct WfState {
     float4 *o_tmax, *d_eta, *beta, *L;
     ulonglong2 *rng;   // PCG32 state, increment
     uint2 *flags;      // bounces, bit0 active | bit1 specular bounce
-    uint32_t *pix;
+    uint32_t *pix, *job;
     float4 *hit0, *hit1;
     int32_t *guide_slot;     // row of the proposal queue, -1 if none
     float *guide_in;         // [capacity][6]
-    uint32_t *guide_count;
+    float *guide_out;        // [capacity][40]
+    uint32_t *q[2];          // slot queues
+    uint32_t *free_q;        // free slots
     VertexRec *recs;         // [capacity][max_depth], null when not training
     uint32_t *n_recs;
-    uint32_t max_depth;
+    float *train_rec;        // [train_capacity][10]
+    uint32_t train_capacity;
+    uint32_t capacity, max_depth;
+    uint32_t regen_min;      // regenerate only when fewer than this many slots are active
+    const WfJob *jobs;
+    uint32_t n_jobs;
+    WfCounters *ctr;
 };
 
-hipError_t wf_launch_start(const WfState &w, const DevCamera &cam, uint32_t n_paths, uint64_t seed,
-                           uint64_t first_path, hipStream_t st);
-hipError_t wf_launch_extend(const WfState &w, const DevScene &sc, uint32_t n_paths, bool guided, const GuideCfg &gc,
-                            unsigned long long *counters, hipStream_t st);
-hipError_t wf_launch_shade(const WfState &w, const DevScene &sc, uint32_t n_paths, const GuideCfg &gc, float *rgb,
-                           unsigned long long *counters, hipStream_t st);
-hipError_t wf_launch_records(const WfState &w, uint32_t n_paths, float *out, uint32_t *out_count, uint32_t capacity,
-                             hipStream_t st);
+// Resets the counters for a new batch of total_paths paths: all slots free.
+hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hipStream_t st);
+// One iteration: regenerate, extend, [infer], shade, bookkeeping.
+hipError_t wf_launch_iteration(const WfState &w, bool guided, const void *guide_dev, float *film,
+                               unsigned long long *ray_counts, int block, hipStream_t st);
 
 }  // namespace illume

@@ -1,20 +1,19 @@
 // Wavefront path tracer: the bounce loop of PathIntegrator::li split into
 // per-stage kernels over a pool of path slots whose state lives in HBM as
-// structure-of-arrays, so a batched MFMA stage (the learned proposal's
-// inference) can run between the closest-hit query and shading.
+// structure-of-arrays, with compacted slot queues and path regeneration
+// (Laine, Karras, Aila 2013), so that
+//   * lanes only ever hold live paths (the shade stage compacts survivors
+//     into the next queue),
+//   * finished slots are refilled with new camera paths from any job of the
+//     batch (scenes and cameras of very different sizes share one pool),
+//   * a batched MFMA stage (the learned proposal's inference) runs between
+//     the closest-hit query and shading.
 //
-//   start   camera samples -> path slots
-//   extend  closest hit of every active slot; slots whose hit needs the
-//           proposal append (slot, normalised position, normal) to a queue
-//   infer   MLP over the queue (guide.hip), count read on the device
-//   shade   one bounce of path_shade (NEE + MIS rays traced inline); finished
-//           slots add their radiance to the film
-//   records after the last bounce: per-vertex training samples
-//           (incident radiance / pdf) appended to the training buffer
-//
-// Every kernel reads its work count from device memory and exits early, so
-// the host can launch fixed grids (and capture the loop in a graph).
+// One iteration = regenerate -> extend -> [infer] -> shade -> bookkeeping.
+// Every kernel reads its work counts from device memory and exits early, so
+// the host launches fixed grids and can capture iterations in a hipGraph.
 #include <hip/hip_runtime.h>
+#include "guide_kernels.h"
 #include "kernels.h"
 #include "path.h"
 #include "wavefront.h"
@@ -24,8 +23,6 @@
 namespace illume {
 
 namespace {
-
-constexpr int kBlock = 128;
 
 __device__ __forceinline__ void store_state(const WfState &w, uint32_t p, const PathState &ps) {
     w.o_tmax[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.t_max);
@@ -52,49 +49,112 @@ __device__ __forceinline__ void load_state(const WfState &w, uint32_t p, PathSta
     ps.specular = (f.y & 2u) != 0;
 }
 
-__global__ __launch_bounds__(kBlock) void wf_start_kernel(WfState w, DevCamera cam, uint32_t n_paths, uint64_t seed,
-                                                          uint64_t first_path) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n_paths) return;
-    const uint64_t n_pix = (uint64_t)cam.width * cam.height;
-    const uint64_t g = first_path + p;
-    PathState ps;
-    path_start(cam, seed, g % n_pix, (uint32_t)(g / n_pix), ps);
-    store_state(w, p, ps);
-    w.pix[p] = (uint32_t)(g % n_pix);
-    w.n_recs[p] = 0;
-    if (p == 0) *w.guide_count = 0;
+// Slots to (re)start this iteration: all free slots, up to the paths left,
+// once fewer than regen_min slots are live.
+__device__ __forceinline__ uint32_t regen_take(const WfState &w, const WfCounters &c) {
+    if (c.n_q[c.parity] >= w.regen_min) return 0;
+    const unsigned long long rem = c.total_paths - c.next_path;
+    return (uint32_t)min((unsigned long long)c.n_free, rem);
 }
 
-__global__ __launch_bounds__(kBlock) void wf_extend_kernel(WfState w, DevScene sc, uint32_t n_paths, int guided,
-                                                           GuideCfg gc, unsigned long long *__restrict__ counters) {
-    __shared__ int lds_stack[kStackLds * kBlock];
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ uint32_t find_job(const WfState &w, unsigned long long g) {
+    uint32_t lo = 0, hi = w.n_jobs;  // last job with path_begin <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (w.jobs[mid].path_begin <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void wf_reset_kernel(WfState w, unsigned long long total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.capacity) w.free_q[i] = i;
+    if (i == 0) {
+        WfCounters &c = *w.ctr;
+        c.n_q[0] = c.n_q[1] = 0;
+        c.n_free = w.capacity;
+        c.parity = 0;
+        c.guide_count = 0;
+        c.take = 0;
+        c.next_path = 0;
+        c.total_paths = total;
+        c.done_paths = 0;
+    }
+}
+
+__global__ void wf_regen_kernel(WfState w) {
+    const WfCounters c = *w.ctr;
+    const uint32_t take = regen_take(w, c);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= take) return;
+    const uint32_t slot = w.free_q[c.n_free - take + i];
+    const unsigned long long g = c.next_path + i;
+    const uint32_t j = find_job(w, g);
+    const WfJob &job = w.jobs[j];
+    const unsigned long long n_pix = (unsigned long long)job.cam.width * job.cam.height;
+    const unsigned long long local = g - job.path_begin;
+    const uint64_t pix = local % n_pix;
+    PathState ps;
+    path_start(job.cam, job.seed, pix, job.sample_offset + (uint32_t)(local / n_pix), ps);
+    store_state(w, slot, ps);
+    w.pix[slot] = (uint32_t)pix;
+    w.job[slot] = j;
+    w.n_recs[slot] = 0;
+    w.q[c.parity][c.n_q[c.parity] + i] = slot;
+}
+
+__global__ void wf_book_regen_kernel(WfState w) {
+    WfCounters &c = *w.ctr;
+    const uint32_t take = regen_take(w, c);
+    c.n_free -= take;
+    c.next_path += take;
+    c.n_q[c.parity] += take;
+    c.take = take;
+}
+
+__global__ void wf_book_end_kernel(WfState w) {
+    WfCounters &c = *w.ctr;
+    c.n_q[c.parity] = 0;
+    c.parity ^= 1u;
+    c.guide_count = 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
+                                                          unsigned long long *__restrict__ counters) {
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    const uint32_t parity = w.ctr->parity;
+    const uint32_t n_in = w.ctr->n_q[parity];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t n_closest = 0;
-    if (p < n_paths && (w.flags[p].y & 1u)) {
+    if (i < n_in) {
+        const uint32_t p = w.q[parity][i];
+        const WfJob &job = w.jobs[w.job[p]];
+        const DevScene &sc = *job.scene;
         const float4 a = w.o_tmax[p], b = w.d_eta[p];
-        TraceStack<kBlock> st;
+        TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x);
         TraceHit h;
         uint32_t nb = 0, np = 0;
         const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-        const bool found = trace_ray<false, false, kBlock>(sc.nodes, sc.prims, o, d, a.w, h, st, nb, np);
+        const bool found = trace_ray<false, false, BLOCK>(sc.nodes, sc.prims, o, d, a.w, h, st, nb, np);
         ++n_closest;
         w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
         w.hit1[p] = make_float4(h.b1, h.b2, 0.f, 0.f);
         int gslot = -1;
-        if (guided && found) {
+        if (guided && found && job.gc.out) {
             const DevMaterial &m = sc.materials[sc.prim_material[h.prim]];
             const bool nonspec = (m.type == kMetal) ||
                                  (m.type == kMatte && !(m.kd[0] == 0.f && m.kd[1] == 0.f && m.kd[2] == 0.f));
-            if (nonspec && (w.flags[p].x < w.max_depth)) {
+            if (nonspec && w.flags[p].x < job.max_depth) {
                 Isect it;
                 hit_isect(sc, h, o, d, it);
-                gslot = (int)atomicAdd(w.guide_count, 1u);
+                gslot = (int)atomicAdd(&w.ctr->guide_count, 1u);
                 float *in = w.guide_in + 6 * (size_t)gslot;
-                in[0] = (it.p.x - gc.bounds_lo[0]) * gc.inv_ext[0];
-                in[1] = (it.p.y - gc.bounds_lo[1]) * gc.inv_ext[1];
-                in[2] = (it.p.z - gc.bounds_lo[2]) * gc.inv_ext[2];
+                in[0] = (it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0];
+                in[1] = (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1];
+                in[2] = (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2];
                 in[3] = it.n.x;
                 in[4] = it.n.y;
                 in[5] = it.n.z;
@@ -108,15 +168,41 @@ __global__ __launch_bounds__(kBlock) void wf_extend_kernel(WfState w, DevScene s
     }
 }
 
-__global__ __launch_bounds__(kBlock) void wf_shade_kernel(WfState w, DevScene sc, uint32_t n_paths, GuideCfg gc,
-                                                          float *__restrict__ rgb,
-                                                          unsigned long long *__restrict__ counters) {
-    __shared__ int lds_stack[kStackLds * kBlock];
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) *w.guide_count = 0;  // the next extend stage refills the proposal queue
+// Training samples of a finished path: for vertex k with a snapshot, the
+// radiance that arrived along its sampled direction is
+// (L_final - L_snap) / beta_after; the weight is its luminance over the pdf.
+__device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf) {
+    const float lf[3] = {Lf.x, Lf.y, Lf.z};
+    for (uint32_t k = 0; k < n; ++k) {
+        const VertexRec &r = w.recs[(size_t)p * w.max_depth + k];
+        if (r.lsnap[0] < 0.f) continue;
+        float li[3];
+        for (int c = 0; c < 3; ++c) li[c] = r.beta[c] > 0.f ? fmaxf(0.f, lf[c] - r.lsnap[c]) / r.beta[c] : 0.f;
+        const float wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r.pdf;
+        if (!(wgt > 0.f) || !__builtin_isfinite(wgt)) continue;
+        const unsigned long long s = atomicAdd(&w.ctr->rec_count, 1ull);
+        if (s >= w.train_capacity) continue;
+        float *o = w.train_rec + 10 * (size_t)s;
+        for (int c = 0; c < 6; ++c) o[c] = r.in[c];
+        o[6] = r.wi[0];
+        o[7] = r.wi[1];
+        o[8] = r.wi[2];
+        o[9] = wgt;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__restrict__ film,
+                                                         unsigned long long *__restrict__ counters) {
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    const uint32_t parity = w.ctr->parity;
+    const uint32_t n_in = w.ctr->n_q[parity];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     PathCounters pc{0, 0};
-    if (p < n_paths && (w.flags[p].y & 1u)) {
-        TraceStack<kBlock> st;
+    if (i < n_in) {
+        const uint32_t p = w.q[parity][i];
+        const WfJob &job = w.jobs[w.job[p]];
+        TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x);
         PathState ps;
         load_state(w, p, ps);
@@ -130,19 +216,24 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(WfState w, DevScene sc
         h.b2 = h1.y;
         const bool found = h.prim != 0xFFFFFFFFu;
         const int gslot = w.guide_slot[p];
-        const float *gout = (gc.out && gslot >= 0) ? gc.out + 40 * (size_t)gslot : nullptr;
+        const float *gout = (job.gc.out && gslot >= 0) ? job.gc.out + 40 * (size_t)gslot : nullptr;
         uint32_t n_recs = w.n_recs[p];
         VertexRec *recs = w.recs ? w.recs + (size_t)p * w.max_depth : nullptr;
-        path_shade<kBlock>(sc, w.max_depth, ps, found, h, gout, gc, recs, n_recs, st, pc);
+        path_shade<BLOCK>(*job.scene, job.max_depth, ps, found, h, gout, job.gc, recs, n_recs, st, pc);
         w.n_recs[p] = n_recs;
-        store_state(w, p, ps);
-        if (!ps.active) {
+        if (ps.active) {
+            store_state(w, p, ps);
+            w.q[parity ^ 1u][atomicAdd(&w.ctr->n_q[parity ^ 1u], 1u)] = p;
+        } else {
             const f3 L = guard_radiance(ps.L);
-            const uint32_t pix = w.pix[p];
-            atomicAdd(rgb + 3 * (size_t)pix, L.x);
-            atomicAdd(rgb + 3 * (size_t)pix + 1, L.y);
-            atomicAdd(rgb + 3 * (size_t)pix + 2, L.z);
-            w.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+            float *px = film + job.film_offset + 3 * (size_t)w.pix[p];
+            atomicAdd(px, L.x);
+            atomicAdd(px + 1, L.y);
+            atomicAdd(px + 2, L.z);
+            if (recs) emit_records(w, p, n_recs, L);
+            w.flags[p] = make_uint2(ps.bounces, 0u);
+            w.free_q[atomicAdd(&w.ctr->n_free, 1u)] = p;
+            atomicAdd(&w.ctr->done_paths, 1ull);
         }
     }
     if (counters) {
@@ -158,56 +249,37 @@ __global__ __launch_bounds__(kBlock) void wf_shade_kernel(WfState w, DevScene sc
     }
 }
 
-// Training samples: for vertex k with a snapshot, the radiance that arrived
-// along its sampled direction is (L_final - L_snap) / beta_after; the sample
-// weight is its luminance over the sampling pdf.
-__global__ __launch_bounds__(kBlock) void wf_records_kernel(WfState w, uint32_t n_paths, float *__restrict__ out,
-                                                            uint32_t *__restrict__ out_count, uint32_t capacity) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n_paths || !w.recs) return;
-    const float4 Lf = w.L[p];
-    const uint32_t n = w.n_recs[p];
-    for (uint32_t k = 0; k < n; ++k) {
-        const VertexRec &r = w.recs[(size_t)p * w.max_depth + k];
-        if (r.lsnap[0] < 0.f) continue;
-        float li[3];
-        const float lf[3] = {Lf.x, Lf.y, Lf.z};
-        for (int c = 0; c < 3; ++c) li[c] = r.beta[c] > 0.f ? fmaxf(0.f, lf[c] - r.lsnap[c]) / r.beta[c] : 0.f;
-        const float wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r.pdf;
-        if (!(wgt > 0.f) || !__builtin_isfinite(wgt)) continue;
-        const uint32_t slot = atomicAdd(out_count, 1u);
-        if (slot >= capacity) continue;
-        float *o = out + 10 * (size_t)slot;
-        for (int c = 0; c < 6; ++c) o[c] = r.in[c];
-        o[6] = r.wi[0];
-        o[7] = r.wi[1];
-        o[8] = r.wi[2];
-        o[9] = wgt;
+template <int BLOCK>
+hipError_t iteration(const WfState &w, bool guided, const GuideDev *g, float *film, unsigned long long *cnt,
+                     hipStream_t st) {
+    const uint32_t grid = (w.capacity + BLOCK - 1) / BLOCK;
+    wf_regen_kernel<<<(w.capacity + 255) / 256, 256, 0, st>>>(w);
+    wf_book_regen_kernel<<<1, 1, 0, st>>>(w);
+    wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, guided ? 1 : 0, cnt);
+    if (guided) {
+        hipError_t e = guide_launch_infer_count(*g, w.guide_in, &w.ctr->guide_count, w.capacity, w.guide_out, st);
+        if (e != hipSuccess) return e;
     }
+    wf_shade_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, film, cnt);
+    wf_book_end_kernel<<<1, 1, 0, st>>>(w);
+    return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t wf_launch_start(const WfState &w, const DevCamera &cam, uint32_t n_paths, uint64_t seed,
-                           uint64_t first_path, hipStream_t st) {
-    wf_start_kernel<<<(n_paths + kBlock - 1) / kBlock, kBlock, 0, st>>>(w, cam, n_paths, seed, first_path);
+hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hipStream_t st) {
+    wf_reset_kernel<<<(w.capacity + 255) / 256, 256, 0, st>>>(w, total_paths);
     return hipGetLastError();
 }
-hipError_t wf_launch_extend(const WfState &w, const DevScene &sc, uint32_t n_paths, bool guided, const GuideCfg &gc,
-                            unsigned long long *counters, hipStream_t st) {
-    wf_extend_kernel<<<(n_paths + kBlock - 1) / kBlock, kBlock, 0, st>>>(w, sc, n_paths, guided ? 1 : 0, gc,
-                                                                         counters);
-    return hipGetLastError();
-}
-hipError_t wf_launch_shade(const WfState &w, const DevScene &sc, uint32_t n_paths, const GuideCfg &gc, float *rgb,
-                           unsigned long long *counters, hipStream_t st) {
-    wf_shade_kernel<<<(n_paths + kBlock - 1) / kBlock, kBlock, 0, st>>>(w, sc, n_paths, gc, rgb, counters);
-    return hipGetLastError();
-}
-hipError_t wf_launch_records(const WfState &w, uint32_t n_paths, float *out, uint32_t *out_count, uint32_t capacity,
-                             hipStream_t st) {
-    wf_records_kernel<<<(n_paths + kBlock - 1) / kBlock, kBlock, 0, st>>>(w, n_paths, out, out_count, capacity);
-    return hipGetLastError();
+
+hipError_t wf_launch_iteration(const WfState &w, bool guided, const void *guide_dev, float *film,
+                               unsigned long long *ray_counts, int block, hipStream_t st) {
+    const GuideDev *g = (const GuideDev *)guide_dev;
+    switch (block) {
+        case 64: return iteration<64>(w, guided, g, film, ray_counts, st);
+        case 256: return iteration<256>(w, guided, g, film, ray_counts, st);
+        default: return iteration<128>(w, guided, g, film, ray_counts, st);
+    }
 }
 
 }  // namespace illume

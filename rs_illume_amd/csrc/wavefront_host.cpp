@@ -1,11 +1,13 @@
-// illume_wavefront_* entry points: the path-slot pool and the host loop of
-// the wavefront path tracer (start, then per bounce extend / infer / shade,
-// then training records and Adam steps of the learned proposal).
+// illume_wavefront_* entry points: the path-slot pool, the job table of a
+// (batched) render, the host loop that replays hipGraph-captured iterations
+// of the wavefront path tracer, and the online training of the proposal.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "../../include/illume_hip.h"
 #include "guide_kernels.h"
@@ -15,6 +17,7 @@ namespace illume {
 int abi_fail(int status, const std::string &msg);
 int abi_hip_fail(hipError_t e, const char *what);
 const DevScene *scene_dev(const illume_scene *s);
+const DevScene *scene_dev_device(const illume_scene *s);
 uint32_t scene_prims(const illume_scene *s);
 void scene_bounds(const illume_scene *s, float *b);
 bool scene_camera(const illume_camera &c, DevCamera *out);
@@ -23,15 +26,20 @@ const GuideDev *guide_dev(const illume_guide *g);
 
 using namespace illume;
 
+namespace {
+typedef std::tuple<bool, const void *, float *, unsigned long long *, int, uint32_t> GraphKey;
+}
+
 struct illume_wavefront {
-    const illume_scene *scene = nullptr;
     uint32_t capacity = 0, max_depth = 0;
+    uint32_t block = 128, regen_min = 0, graph_iters = 8;
     WfState w;
-    float *guide_out = nullptr;
-    float *train_rec = nullptr;
-    uint32_t *train_count = nullptr;
-    uint32_t train_capacity = 0;
+    WfJob *d_jobs = nullptr;
+    uint32_t jobs_capacity = 0;
     std::vector<void *> allocs;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    hipStream_t own_stream = nullptr;  // used when the caller passes the null stream (not capturable)
+    uint64_t iterations = 0;  // of the last render call
 };
 
 namespace {
@@ -41,28 +49,35 @@ hipError_t walloc(illume_wavefront *wf, T **p, size_t count) {
     if (e == hipSuccess) wf->allocs.push_back(*p);
     return e;
 }
+void drop_graphs(illume_wavefront *wf) {
+    for (auto &kv : wf->graphs) hipGraphExecDestroy(kv.second);
+    wf->graphs.clear();
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 }  // namespace
 
 extern "C" {
 
 int illume_wavefront_destroy(illume_wavefront *wf) {
     if (!wf) return ILLUME_OK;
+    drop_graphs(wf);
+    if (wf->own_stream) hipStreamDestroy(wf->own_stream);
     for (void *p : wf->allocs) hipFree(p);
     delete wf;
     return ILLUME_OK;
 }
 
-int illume_wavefront_create(const illume_scene *scene, uint32_t max_paths, uint32_t max_depth,
-                            illume_wavefront **out) {
-    if (!scene || !out) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavefront **out) {
+    if (!out) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (max_paths == 0 || max_paths > (1u << 26)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_paths in 1..2^26");
+    if (max_paths < 64 || max_paths > (1u << 26)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_paths in 64..2^26");
     if (max_depth == 0 || max_depth > 64) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth in 1..64");
     illume_wavefront *wf = new (std::nothrow) illume_wavefront();
     if (!wf) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
-    wf->scene = scene;
     wf->capacity = max_paths;
     wf->max_depth = max_depth;
+    wf->regen_min = max_paths + 1;  // regenerate every iteration
     WfState &w = wf->w;
     std::memset(&w, 0, sizeof(w));
     const size_t P = max_paths;
@@ -73,89 +88,181 @@ int illume_wavefront_create(const illume_scene *scene, uint32_t max_paths, uint3
     if (e == hipSuccess) e = walloc(wf, &w.rng, P);
     if (e == hipSuccess) e = walloc(wf, &w.flags, P);
     if (e == hipSuccess) e = walloc(wf, &w.pix, P);
+    if (e == hipSuccess) e = walloc(wf, &w.job, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit0, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit1, P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_slot, P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_in, 6 * P);
-    if (e == hipSuccess) e = walloc(wf, &w.guide_count, 64);
+    if (e == hipSuccess) e = walloc(wf, &w.guide_out, 40 * P);
+    if (e == hipSuccess) e = walloc(wf, &w.q[0], P);
+    if (e == hipSuccess) e = walloc(wf, &w.q[1], P);
+    if (e == hipSuccess) e = walloc(wf, &w.free_q, P);
     if (e == hipSuccess) e = walloc(wf, &w.recs, P * max_depth);
     if (e == hipSuccess) e = walloc(wf, &w.n_recs, P);
-    if (e == hipSuccess) e = walloc(wf, &wf->guide_out, 40 * P);
-    wf->train_capacity = (uint32_t)std::min<size_t>(P * max_depth, 1u << 26);
-    if (e == hipSuccess) e = walloc(wf, &wf->train_rec, 10 * (size_t)wf->train_capacity);
-    if (e == hipSuccess) e = walloc(wf, &wf->train_count, 64);
-    if (e == hipSuccess) e = hipMemset(w.guide_count, 0, 64 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(wf->train_count, 0, 64 * sizeof(uint32_t));
+    w.train_capacity = (uint32_t)std::min<size_t>(P * max_depth, 1u << 25);
+    if (e == hipSuccess) e = walloc(wf, &w.train_rec, 10 * (size_t)w.train_capacity);
+    if (e == hipSuccess) e = walloc(wf, &w.ctr, 1);
+    if (e == hipSuccess) e = hipMemset(w.ctr, 0, sizeof(WfCounters));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&wf->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         illume_wavefront_destroy(wf);
         return abi_hip_fail(e, "wavefront allocation");
     }
+    w.capacity = max_paths;
     w.max_depth = max_depth;
     *out = wf;
     return ILLUME_OK;
 }
 
-int illume_wavefront_render(illume_wavefront *wf, const illume_camera *camera, const illume_render_cfg *cfg,
-                            illume_guide *guide, const illume_guide_cfg *gcfg, float *rgb_sum, uint64_t *ray_counts,
-                            void *stream) {
-    if (!wf || !camera || !cfg || !rgb_sum) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
-    if (cfg->max_depth > wf->max_depth) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth above the pool's");
-    if ((uint64_t)cfg->sample_offset + cfg->spp > (1u << 24))
-        return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
-    DevCamera cam;
-    if (!scene_camera(*camera, &cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
-    if (cfg->spp == 0 || scene_prims(wf->scene) == 0) return ILLUME_OK;
-    const bool guided = guide && gcfg && gcfg->guide_frac > 0.f;
-    const bool train = guide && gcfg && gcfg->train;
+int illume_wavefront_set_tuning(illume_wavefront *wf, uint32_t block, uint32_t regen_min, uint32_t graph_iters) {
+    if (!wf) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    if (block != 64 && block != 128 && block != 256) return abi_fail(ILLUME_ERR_UNSUPPORTED, "block in {64,128,256}");
+    if (graph_iters > 64) return abi_fail(ILLUME_ERR_UNSUPPORTED, "graph_iters in 0..64");
+    wf->block = block;
+    wf->regen_min = std::max(1u, std::min(regen_min, wf->capacity + 1));
+    wf->graph_iters = graph_iters;
+    drop_graphs(wf);
+    return ILLUME_OK;
+}
+
+uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf) { return wf ? wf->iterations : 0; }
+
+int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
+                                  const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
+    if (!wf || !film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (gcfg && !(gcfg->guide_frac >= 0.f && gcfg->guide_frac <= 1.f))
         return abi_fail(ILLUME_ERR_INVALID_ARG, "guide_frac must be in [0, 1]");
+    const bool guided = guide && gcfg && gcfg->guide_frac > 0.f;
+    const bool train = guide && gcfg && gcfg->train;
     const uint32_t batch = (gcfg && gcfg->train_batch) ? gcfg->train_batch : 65536u;
-    hipStream_t st = (hipStream_t)stream;
-    const DevScene &sc = *scene_dev(wf->scene);
-    WfState w = wf->w;
-    w.max_depth = cfg->max_depth;
-    if (!train) w.recs = nullptr;
-    GuideCfg gc;
-    std::memset(&gc, 0, sizeof(gc));
-    float b[6];
-    scene_bounds(wf->scene, b);
-    for (int a = 0; a < 3; ++a) {
-        gc.bounds_lo[a] = b[a];
-        float ext = b[3 + a] - b[a];
-        gc.inv_ext[a] = ext > 0.f ? 1.f / ext : 1.f;
-    }
-    gc.frac = guided ? gcfg->guide_frac : 0.f;
-    gc.out = guided ? wf->guide_out : nullptr;
-    unsigned long long *cnt = (unsigned long long *)ray_counts;
-    const uint64_t n_pix = (uint64_t)cam.width * cam.height;
-    const uint64_t first = (uint64_t)cfg->sample_offset * n_pix;
-    const uint64_t total = (uint64_t)cfg->spp * n_pix;
-    hipError_t e = hipSuccess;
-    for (uint64_t done = 0; done < total && e == hipSuccess;) {
-        const uint32_t n = (uint32_t)std::min<uint64_t>(wf->capacity, total - done);
-        e = wf_launch_start(w, cam, n, cfg->seed, first + done, st);
-        for (uint32_t bounce = 0; bounce <= cfg->max_depth && e == hipSuccess; ++bounce) {
-            e = wf_launch_extend(w, sc, n, guided, gc, cnt, st);
-            if (e == hipSuccess && guided)
-                e = guide_launch_infer_count(*guide_dev(guide), w.guide_in, w.guide_count, n, wf->guide_out, st);
-            if (e == hipSuccess) e = wf_launch_shade(w, sc, n, gc, rgb_sum, cnt, st);
+    // Work already queued by the caller (e.g. zeroing the film) completes
+    // first; the pipeline then runs on the caller's stream, or on the pool's
+    // own stream when the caller passes the (uncapturable) null stream.
+    hipStream_t caller = (hipStream_t)stream;
+    hipStream_t st = caller ? caller : wf->own_stream;
+    std::vector<WfJob> hj;
+    unsigned long long total = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const illume_job &in = jobs[j];
+        if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
+        if (in.cfg.max_depth > wf->max_depth) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth above the pool's");
+        if ((uint64_t)in.cfg.sample_offset + in.cfg.spp > (1u << 24))
+            return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+        WfJob job;
+        std::memset(&job, 0, sizeof(job));
+        if (!scene_camera(in.camera, &job.cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
+        const unsigned long long n = scene_prims(in.scene) ? (unsigned long long)job.cam.width * job.cam.height *
+                                                                 in.cfg.spp
+                                                           : 0ull;
+        if (n == 0) continue;
+        job.scene = scene_dev_device(in.scene);
+        job.path_begin = total;
+        job.film_offset = in.film_offset;
+        job.seed = in.cfg.seed;
+        job.spp = in.cfg.spp;
+        job.sample_offset = in.cfg.sample_offset;
+        job.max_depth = in.cfg.max_depth;
+        float b[6];
+        scene_bounds(in.scene, b);
+        for (int a = 0; a < 3; ++a) {
+            job.gc.bounds_lo[a] = b[a];
+            const float ext = b[3 + a] - b[a];
+            job.gc.inv_ext[a] = ext > 0.f ? 1.f / ext : 1.f;
         }
-        if (e == hipSuccess && train) {
-            e = wf_launch_records(w, n, wf->train_rec, wf->train_count, wf->train_capacity, st);
-            uint32_t n_rec = 0;
-            if (e == hipSuccess) e = hipMemcpyAsync(&n_rec, wf->train_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            n_rec = std::min(n_rec, wf->train_capacity);
-            for (uint32_t s = 0; s < n_rec && e == hipSuccess; s += batch) {
-                int r = illume_guide_train_step(guide, wf->train_rec + 10 * (size_t)s, std::min(batch, n_rec - s), st);
+        job.gc.frac = guided ? gcfg->guide_frac : 0.f;
+        job.gc.out = guided ? wf->w.guide_out : nullptr;
+        hj.push_back(job);
+        total += n;
+    }
+    if (total == 0) return ILLUME_OK;
+    hipError_t e = hipSuccess;
+    if (hj.size() > wf->jobs_capacity) {
+        if (wf->d_jobs) {
+            hipFree(wf->d_jobs);
+            wf->allocs.erase(std::remove(wf->allocs.begin(), wf->allocs.end(), (void *)wf->d_jobs), wf->allocs.end());
+        }
+        wf->d_jobs = nullptr;
+        wf->jobs_capacity = 0;
+        e = walloc(wf, &wf->d_jobs, hj.size());
+        if (e != hipSuccess) return abi_hip_fail(e, "job table allocation");
+        wf->jobs_capacity = (uint32_t)hj.size();
+        drop_graphs(wf);
+    }
+    e = caller ? hipStreamSynchronize(caller) : hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(wf->d_jobs, hj.data(), hj.size() * sizeof(WfJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return abi_hip_fail(e, "job table upload");
+    WfState w = wf->w;
+    w.jobs = wf->d_jobs;
+    w.n_jobs = (uint32_t)hj.size();
+    w.regen_min = wf->regen_min;
+    if (!train) w.recs = nullptr;
+    unsigned long long *cnt = (unsigned long long *)ray_counts;
+    const void *gdev = guided ? (const void *)guide_dev(guide) : nullptr;
+    e = wf_launch_reset(w, total, st);
+    if (e == hipSuccess && train) zero_u64_kernel<<<1, 1, 0, st>>>(&w.ctr->rec_count);
+    // Every path finishes within max_depth + 1 iterations of its start; the
+    // pool restarts at least one path per iteration whenever it is idle.
+    const uint64_t max_iters = (total / wf->capacity + 2) * (uint64_t)(wf->max_depth + 2) + total + 16;
+    const uint32_t k = std::max(1u, wf->graph_iters);
+    GraphKey key(guided, gdev, film, cnt, wf->block, w.n_jobs);
+    hipGraphExec_t exec = nullptr;
+    if (wf->graph_iters > 0 && e == hipSuccess) {
+        auto it = wf->graphs.find(key);
+        if (it != wf->graphs.end()) {
+            exec = it->second;
+        } else {
+            hipGraph_t graph;
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, gdev, film, cnt, wf->block, st);
+            hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e2 == hipSuccess) hipGraphDestroy(graph);
+            if (e == hipSuccess) wf->graphs[key] = exec;
+        }
+    }
+    WfCounters c;
+    uint64_t iters = 0;
+    while (e == hipSuccess) {
+        if (exec) e = hipGraphLaunch(exec, st);
+        else
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, gdev, film, cnt, wf->block, st);
+        iters += k;
+        if (e == hipSuccess) e = hipMemcpyAsync(&c, w.ctr, sizeof(c), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) break;
+        const bool finished = c.done_paths >= c.total_paths;
+        if (train && (c.rec_count >= batch || finished) && c.rec_count > 0) {
+            const uint32_t n_rec = (uint32_t)std::min<unsigned long long>(c.rec_count, w.train_capacity);
+            for (uint32_t s = 0; s < n_rec; s += batch) {
+                int r = illume_guide_train_step(guide, w.train_rec + 10 * (size_t)s, std::min(batch, n_rec - s), st);
                 if (r != ILLUME_OK) return r;
             }
-            if (e == hipSuccess) e = hipMemsetAsync(wf->train_count, 0, sizeof(uint32_t), st);
+            zero_u64_kernel<<<1, 1, 0, st>>>(&w.ctr->rec_count);
+            e = hipGetLastError();
         }
-        done += n;
+        if (finished) break;
+        if (iters > max_iters) {
+            wf->iterations = iters;
+            return abi_fail(ILLUME_ERR_QUEUE_OVERFLOW, "wavefront made no progress");
+        }
     }
+    wf->iterations = iters;
     if (e != hipSuccess) return abi_hip_fail(e, "wavefront render");
     return ILLUME_OK;
+}
+
+int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
+                            const illume_render_cfg *cfg, illume_guide *guide, const illume_guide_cfg *gcfg,
+                            float *rgb_sum, uint64_t *ray_counts, void *stream) {
+    if (!scene || !camera || !cfg) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    illume_job job;
+    std::memset(&job, 0, sizeof(job));
+    job.scene = scene;
+    job.camera = *camera;
+    job.cfg = *cfg;
+    job.film_offset = 0;
+    return illume_wavefront_render_batch(wf, &job, 1, guide, gcfg, rgb_sum, ray_counts, stream);
 }
 
 }  // extern "C"

@@ -138,38 +138,76 @@ class GuideCfg(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+class CameraC(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("fov", ctypes.c_float), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+class JobC(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("camera", CameraC), ("cfg", RenderCfg), ("film_offset", ctypes.c_uint64)]
+
+
+def _camera_c(cam):
+    cam = np.ascontiguousarray(cam, CAMERA_DTYPE).reshape(1)
+    c = CameraC()
+    c.pos[:] = [float(x) for x in cam["pos"][0]]
+    c.look[:] = [float(x) for x in cam["look"][0]]
+    c.up[:] = [float(x) for x in cam["up"][0]]
+    c.fov = float(cam["fov"][0])
+    c.width = int(cam["width"][0])
+    c.height = int(cam["height"][0])
+    return c
+
+
 def _declare_wf():
     L = _declare()
     if getattr(L, "_wf_declared", False):
         return L
     vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
     L.illume_wavefront_create.restype = i32
-    L.illume_wavefront_create.argtypes = [vp, u32, u32, ctypes.POINTER(vp)]
+    L.illume_wavefront_create.argtypes = [u32, u32, ctypes.POINTER(vp)]
     L.illume_wavefront_destroy.restype = i32
     L.illume_wavefront_destroy.argtypes = [vp]
+    L.illume_wavefront_set_tuning.restype = i32
+    L.illume_wavefront_set_tuning.argtypes = [vp, u32, u32, u32]
+    L.illume_wavefront_last_iterations.restype = ctypes.c_uint64
+    L.illume_wavefront_last_iterations.argtypes = [vp]
     L.illume_wavefront_render.restype = i32
-    L.illume_wavefront_render.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), vp, ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L.illume_wavefront_render.argtypes = [vp, vp, vp, ctypes.POINTER(RenderCfg), vp, ctypes.POINTER(GuideCfg), vp, vp,
+                                          vp]
+    L.illume_wavefront_render_batch.restype = i32
+    L.illume_wavefront_render_batch.argtypes = [vp, ctypes.POINTER(JobC), u32, vp, ctypes.POINTER(GuideCfg), vp, vp, vp]
     L._wf_declared = True
     return L
 
 
-class WavefrontPathIntegrator(PathIntegrator):
-    """The same Integrator::render as :class:`PathIntegrator`, run as per-stage
-    kernels over a pool of ``max_paths`` path slots; optionally guided by a
-    :class:`rs_illume_amd.guide.Guide` (one-sample MIS with probability
-    ``guide_frac``) and training it online after every pass."""
+class Wavefront:
+    """Pool of ``max_paths`` path slots in HBM with the wavefront pipeline
+    (regenerate, extend, infer, shade) replayed from hipGraphs."""
 
-    def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
-                 train=False, train_batch=65536):
-        super().__init__(camera, spp, max_depth, seed)
-        self.scene = scene
-        self.guide = guide
-        self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, int(bool(train)), int(train_batch), 0)
+    def __init__(self, max_paths=1 << 20, max_depth=8, block=128, regen_min=None, graph_iters=8):
         L = _declare_wf()
+        torch.cuda.init()
         h = ctypes.c_void_p()
-        _lib.check(L.illume_wavefront_create(scene.handle, int(max_paths), int(max_depth), ctypes.byref(h)),
+        _lib.check(L.illume_wavefront_create(int(max_paths), int(max_depth), ctypes.byref(h)),
                    "illume_wavefront_create")
         self._h = h
+        self.max_paths = int(max_paths)
+        self.max_depth = int(max_depth)
+        self.tune(block, self.max_paths + 1 if regen_min is None else regen_min, graph_iters)
+
+    def tune(self, block=128, regen_min=None, graph_iters=8):
+        rm = self.max_paths + 1 if regen_min is None else int(regen_min)
+        _lib.check(_declare_wf().illume_wavefront_set_tuning(self._h, int(block), rm, int(graph_iters)),
+                   "illume_wavefront_set_tuning")
+        self.params = {"block": int(block), "regen_min": rm, "graph_iters": int(graph_iters)}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_iterations(self):
+        return int(_declare_wf().illume_wavefront_last_iterations(self._h))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -182,17 +220,60 @@ class WavefrontPathIntegrator(PathIntegrator):
         except Exception:
             pass
 
+    def render_batch(self, jobs, film=None, guide=None, gcfg=None, ray_counts=None, stream=None):
+        """jobs: list of (Scene, camera, spp, max_depth, seed[, sample_offset]).
+        Returns (film, offsets): film is a flat float32 device tensor holding
+        every job's radiance sums (H*W*3 each) at its offset."""
+        arr = (JobC * len(jobs))()
+        offsets, off = [], 0
+        for i, jb in enumerate(jobs):
+            sc, cam, spp, depth, seed = jb[:5]
+            so = jb[5] if len(jb) > 5 else 0
+            c = _camera_c(cam)
+            arr[i].scene = sc.handle
+            arr[i].camera = c
+            arr[i].cfg = RenderCfg(int(spp), int(depth), int(so), 0, int(seed))
+            arr[i].film_offset = off
+            offsets.append(off)
+            off += c.width * c.height * 3
+        if film is None:
+            film = torch.zeros(off, dtype=torch.float32, device="cuda")
+        g = guide.handle if guide is not None else None
+        gc = gcfg if gcfg is not None else GuideCfg(0.0, 0, 0, 0)
+        rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+        _lib.check(_declare_wf().illume_wavefront_render_batch(self._h, arr, len(jobs), g, ctypes.byref(gc),
+                                                               ctypes.c_void_p(film.data_ptr()), rc,
+                                                               _stream_handle(stream)), "illume_wavefront_render_batch")
+        return film, offsets
+
+
+class WavefrontPathIntegrator(PathIntegrator):
+    """The same Integrator::render as :class:`PathIntegrator`, run on a
+    :class:`Wavefront` pool; optionally guided by a
+    :class:`rs_illume_amd.guide.Guide` (one-sample MIS with probability
+    ``guide_frac``) and training it online from finished paths."""
+
+    def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
+                 train=False, train_batch=65536, wavefront=None):
+        super().__init__(camera, spp, max_depth, seed)
+        self.scene = scene
+        self.guide = guide
+        self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, int(bool(train)), int(train_batch), 0)
+        self.wf = wavefront if wavefront is not None else Wavefront(max_paths, max_depth)
+
     def render_sum(self, scene=None, rgb_sum=None, spp=None, sample_offset=0, ray_counts=None, stream=None):
         w, h = self.resolution
         if rgb_sum is None:
             rgb_sum = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
+        sc = scene if scene is not None else self.scene
         cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), 0, self.seed)
         rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
         g = self.guide.handle if self.guide is not None else None
-        _lib.check(_declare_wf().illume_wavefront_render(self._h, _p(self.camera), ctypes.byref(cfg), g,
-                                                         ctypes.byref(self.gcfg), ctypes.c_void_p(rgb_sum.data_ptr()),
-                                                         rc, _stream_handle(stream)), "illume_wavefront_render")
+        _lib.check(_declare_wf().illume_wavefront_render(self.wf.handle, sc.handle, _p(self.camera), ctypes.byref(cfg),
+                                                         g, ctypes.byref(self.gcfg),
+                                                         ctypes.c_void_p(rgb_sum.data_ptr()), rc,
+                                                         _stream_handle(stream)), "illume_wavefront_render")
         return rgb_sum
 
     def render(self, scene=None, stream=None):
-        return self.render_sum(stream=stream) / float(self.spp)
+        return self.render_sum(scene, stream=stream) / float(self.spp)

@@ -162,3 +162,90 @@ def rel_mse(img, ref, eps=0.01):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+# ---------------------------------------------------------------- batched procedural scenes
+_ICO_CACHE = {}
+
+
+def icosphere(level):
+    """Unit icosphere: (verts float64 [V,3], faces int [F,3]) with 20*4^level faces."""
+    if level in _ICO_CACHE:
+        return _ICO_CACHE[level]
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = np.array(v, np.float64)
+    verts /= np.linalg.norm(verts, axis=1, keepdims=True)
+    faces = np.array(f, np.int64)
+    for _ in range(level):
+        edges = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+        edges.sort(axis=1)
+        uniq, inv = np.unique(edges, axis=0, return_inverse=True)
+        mid = verts[uniq[:, 0]] + verts[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(verts)
+        verts = np.concatenate([verts, mid])
+        nf = len(faces)
+        a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
+        ab, bc, ca = base + inv[:nf], base + inv[nf:2 * nf], base + inv[2 * nf:]
+        faces = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                                np.stack([ab, bc, ca], 1)])
+    _ICO_CACHE[level] = (verts, faces)
+    return verts, faces
+
+
+def procedural_scene(seed):
+    """One scene of the batched workload: triangle count log-uniform in
+    [1e3, 2e6] from randomly displaced icospheres on a ground quad, Lambertian
+    or dielectric (IOR 1.5) objects, 1-4 quad lights, a square image of side
+    log-uniform in [64, 1024], spp log-uniform in [8, 64].
+    Returns (SceneDesc, camera, spp)."""
+    rng = np.random.Generator(np.random.PCG64(1000 + seed))
+    target = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+    s = SceneDesc()
+    ground = s.add_material(matte((0.6, 0.6, 0.6)))
+    s.add_quad((-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, 4), ground, normal=(0, 1, 0))
+    verts, idx = [s.verts], [s.idx]
+    tri_mat = list(s.tri_material)
+    n = 2
+    base_v = len(s.verts)
+    while n < target:
+        remaining = target - n
+        level = int(np.clip(np.floor(np.log(max(remaining, 20) / 20) / np.log(4)), 0, 6))
+        level = min(level, int(rng.integers(3, 7)))
+        v, f = icosphere(level)
+        r = np.exp(rng.uniform(np.log(0.15), np.log(0.6)))
+        c = np.array([rng.uniform(-3, 3), r * rng.uniform(0.8, 1.5), rng.uniform(-3, 3)])
+        amp = rng.uniform(0.0, 0.15)
+        disp = 1 + amp * np.sin(3 * v[:, :1] + rng.uniform(0, 6)) * np.cos(4 * v[:, 1:2] + rng.uniform(0, 6))
+        vv = (c + r * v * disp).astype(np.float32)
+        if rng.random() < 0.3:
+            m = s.add_material(glass(1.5))
+        else:
+            m = s.add_material(matte(tuple(rng.uniform(0.1, 0.8, 3))))
+        verts.append(vv)
+        idx.append((f + base_v).astype(np.uint32))
+        tri_mat.extend([m] * len(f))
+        base_v += len(vv)
+        n += len(f)
+    s.verts = np.concatenate(verts).astype(np.float32)
+    s.idx = np.concatenate(idx).astype(np.uint32)
+    s.tri_material = tri_mat
+    s.tri_emission = [(0, 0, 0)] * len(s.idx)
+    n_lights = int(rng.integers(1, 5))
+    lm = s.add_material(matte((0, 0, 0)))
+    for _ in range(n_lights):
+        h = rng.uniform(0.2, 0.6)
+        cx, cz, y = rng.uniform(-2.5, 2.5), rng.uniform(-2.5, 2.5), rng.uniform(3, 5)
+        L = rng.uniform(5, 20) / (h * h)
+        s.add_quad((cx - h, y, cz - h), (cx + h, y, cz - h), (cx + h, y, cz + h), (cx - h, y, cz + h), lm,
+                   emission=(L, L, L), normal=(0, -1, 0))
+    side = int(np.exp(rng.uniform(np.log(64), np.log(1024))))
+    spp = int(np.exp(rng.uniform(np.log(8), np.log(64))))
+    ang = rng.uniform(0, 2 * np.pi)
+    cam = camera((7 * np.cos(ang), rng.uniform(2, 4), 7 * np.sin(ang)), (0, 0.5, 0), (0, 1, 0), 45.0, side, side)
+    return s, cam, spp
