@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--rays", type=int, default=N_RAYS)
+    ap.add_argument("--width", type=int, default=2, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard with no exchange; "
@@ -78,7 +79,7 @@ def main():
     v, idx = W.triangle_soup(N_TRIS, seed=1)
     rays = W.sphere_to_cube_rays(args.rays, seed=2)
     log("building BVH")
-    bvh = BVHAccel(v, idx, None, 4)
+    bvh = BVHAccel(v, idx, None, 4, width=args.width)
     info = bvh.info()
     log("bvh", info)
     rd = rays_to_device(rays)
@@ -131,7 +132,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "incoherent closest-hit, 1M-triangle scene (2^20 tris, lognormal sigma median "
                                "0.005), 2^24 rays sphere->cube, SAH BVH <=4 prims/leaf",
-                   "n_tris": N_TRIS, "n_rays": n, "max_prims_in_node": 4, "parallelism": "single-gpu"},
+                   "n_tris": N_TRIS, "n_rays": n, "max_prims_in_node": 4, "bvh_width": args.width,
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_ray": round(bytes_per_ray, 1), "box_tests_per_ray": round(box_tests, 2),

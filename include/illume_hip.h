@@ -73,6 +73,13 @@ typedef struct {
 int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                       const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out);
 
+/* Same build with a chosen node width: 2 = 64-byte two-child nodes, 4 =
+ * 128-byte four-child nodes (the binary SAH tree collapsed, half the depth).
+ * Hit results do not depend on the width. illume_bvh_create uses width 2. */
+int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                         const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
+                         illume_bvh **out);
+
 /* Frees the handle and its device memory (Drop of BVHAccel). */
 int illume_bvh_destroy(illume_bvh *bvh);
 

@@ -41,10 +41,11 @@ class BVHAccel:
     Arguments follow ``BVHAccel::new(prims, max_prims_in_node, split_method)``
     (bvh.rs:32): the primitives are given as arrays — ``verts`` float32 [V,3],
     ``idx`` uint32 [T,3] and optional ``spheres`` float32 [S,4] (cx, cy, cz, r).
-    Primitive ids are triangles 0..T-1 then spheres T..T+S-1.
+    Primitive ids are triangles 0..T-1 then spheres T..T+S-1. ``width``
+    selects 64-byte two-child (2) or 128-byte four-child (4) nodes.
     """
 
-    def __init__(self, verts, idx, spheres=None, max_prims_in_node=4, split_method="SAH"):
+    def __init__(self, verts, idx, spheres=None, max_prims_in_node=4, split_method="SAH", width=2):
         if split_method not in SPLIT_METHODS:
             raise ValueError("split_method must be one of %s" % (SPLIT_METHODS,))
         if not torch.cuda.is_available():
@@ -59,9 +60,10 @@ class BVHAccel:
         torch.cuda.init()
         h = ctypes.c_void_p()
         p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a.size else None
-        _lib.check(L.illume_bvh_create(p(self.verts), len(self.verts), p(self.idx), self.n_tris, p(self.spheres),
-                                       self.n_spheres, int(max_prims_in_node), ctypes.byref(h)),
-                   "illume_bvh_create")
+        _lib.check(L.illume_bvh_create_ex(p(self.verts), len(self.verts), p(self.idx), self.n_tris, p(self.spheres),
+                                          self.n_spheres, int(max_prims_in_node), int(width), ctypes.byref(h)),
+                   "illume_bvh_create_ex")
+        self.width = int(width)
         self._h = h
 
     def close(self):

@@ -23,8 +23,9 @@ int hip_fail(hipError_t e, const char *what) {
 }  // namespace
 
 struct illume_bvh {
-    illume::GpuNode *d_nodes = nullptr;
+    void *d_nodes = nullptr;  // GpuNode[] (width 2) or GpuNode4[] (width 4)
     illume::PrimRecord *d_prims = nullptr;
+    int width = 2;
     illume_bvh_info_t info;
     int device = 0;
 };
@@ -33,7 +34,7 @@ namespace illume {
 int abi_fail(int status, const std::string &msg) { return fail(status, msg); }
 int abi_hip_fail(hipError_t e, const char *what) { return hip_fail(e, what); }
 void bvh_device_arrays(const illume_bvh *bvh, const GpuNode **nodes, const PrimRecord **prims) {
-    *nodes = bvh->d_nodes;
+    *nodes = bvh->width == 2 ? (const GpuNode *)bvh->d_nodes : nullptr;
     *prims = bvh->d_prims;
 }
 }  // namespace illume
@@ -54,12 +55,14 @@ const char *illume_status_string(int status) {
 
 const char *illume_last_error(void) { return g_last_error.c_str(); }
 
-int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
-                      const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out) {
+int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                         const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
+                         illume_bvh **out) {
     if (!out) return fail(ILLUME_ERR_INVALID_ARG, "out is null");
     *out = nullptr;
     if (max_prims_in_node < 1 || max_prims_in_node > illume::kMaxLeafPrims)
         return fail(ILLUME_ERR_UNSUPPORTED, "max_prims_in_node must be in 1..15");
+    if (width != 2 && width != 4) return fail(ILLUME_ERR_UNSUPPORTED, "width must be 2 or 4");
     auto t0 = std::chrono::steady_clock::now();
     illume::BuildInput in;
     in.verts = verts;
@@ -69,6 +72,7 @@ int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_
     in.spheres = spheres;
     in.n_spheres = n_spheres;
     in.max_prims_in_node = max_prims_in_node;
+    in.width = width;
     illume::BuildOutput bo;
     std::string err;
     try {
@@ -77,15 +81,19 @@ int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_
         return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed during BVH build");
     }
     if (!err.empty()) return fail(ILLUME_ERR_INVALID_ARG, err);
-    if ((int)bo.max_depth >= illume::max_supported_depth())
+    // stack entries: up to width-1 pending siblings per level
+    if ((int)bo.max_depth * (width - 1) >= illume::max_supported_depth())
         return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     auto t1 = std::chrono::steady_clock::now();
 
     illume_bvh *h = new (std::nothrow) illume_bvh();
     if (!h) return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(&h->info, 0, sizeof(h->info));
+    h->width = width;
     hipGetDevice(&h->device);
-    size_t nb = bo.nodes.size() * sizeof(illume::GpuNode);
+    const void *nsrc = width == 4 ? (const void *)bo.nodes4.data() : (const void *)bo.nodes.data();
+    size_t n_nodes = width == 4 ? bo.nodes4.size() : bo.nodes.size();
+    size_t nb = width == 4 ? n_nodes * sizeof(illume::GpuNode4) : n_nodes * sizeof(illume::GpuNode);
     size_t pb = bo.prims.size() * sizeof(illume::PrimRecord);
     hipError_t e;
     if (nb) {
@@ -93,7 +101,7 @@ int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_
             delete h;
             return hip_fail(e, "hipMalloc(nodes)");
         }
-        if ((e = hipMemcpy(h->d_nodes, bo.nodes.data(), nb, hipMemcpyHostToDevice)) != hipSuccess) {
+        if ((e = hipMemcpy(h->d_nodes, nsrc, nb, hipMemcpyHostToDevice)) != hipSuccess) {
             illume_bvh_destroy(h);
             return hip_fail(e, "hipMemcpy(nodes)");
         }
@@ -109,7 +117,7 @@ int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_
         }
     }
     h->info.n_prims = (uint64_t)n_tris + n_spheres;
-    h->info.n_inner_nodes = bo.nodes.size();
+    h->info.n_inner_nodes = n_nodes;
     h->info.n_leaves = bo.n_leaves;
     h->info.max_depth = bo.max_depth;
     h->info.device_bytes = nb + pb;
@@ -117,6 +125,11 @@ int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_
     std::memcpy(h->info.bounds, bo.bounds, sizeof(bo.bounds));
     *out = h;
     return ILLUME_OK;
+}
+
+int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                      const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out) {
+    return illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node, 2, out);
 }
 
 int illume_bvh_destroy(illume_bvh *bvh) {
@@ -139,7 +152,7 @@ static int run_traverse(const illume_bvh *bvh, const illume_ray *rays, uint32_t 
     if (n == 0) return ILLUME_OK;
     if (!rays) return fail(ILLUME_ERR_INVALID_ARG, "rays is null");
     if (!hits && !occluded && !counts) return fail(ILLUME_ERR_INVALID_ARG, "output is null");
-    hipError_t e = illume::launch_traverse(bvh->d_nodes, bvh->d_prims, bvh->info.n_prims == 0, rays, n, hits,
+    hipError_t e = illume::launch_traverse(bvh->d_nodes, bvh->width, bvh->d_prims, bvh->info.n_prims == 0, rays, n, hits,
                                            occluded, counts, any, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "traversal launch");
     return ILLUME_OK;

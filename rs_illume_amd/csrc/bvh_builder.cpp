@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <tuple>
 
 namespace illume {
 namespace {
@@ -284,6 +285,69 @@ std::string build_bvh(const BuildInput &in, BuildOutput *out) {
     const float inf = std::numeric_limits<float>::infinity();
     Box never;  // a box no ray enters: all planes at +inf
     for (int a = 0; a < 3; ++a) never.lo[a] = never.hi[a] = inf;
+
+    if (in.width == 4) {
+        // Collapse: each 4-wide node takes its binary node's two children and
+        // repeatedly opens the inner child of largest area until it has four.
+        auto set4 = [](GpuNode4 &g, int c, const Box &b) {
+            g.lo_x[c] = b.lo[0]; g.hi_x[c] = b.hi[0];
+            g.lo_y[c] = b.lo[1]; g.hi_y[c] = b.hi[1];
+            g.lo_z[c] = b.lo[2]; g.hi_z[c] = b.hi[2];
+        };
+        out->nodes4.reserve(n_prims / 2 + 1);
+        out->nodes4.push_back(GpuNode4());
+        std::vector<std::tuple<int, int, uint32_t>> st4;  // (tmp index, node slot, depth)
+        st4.push_back({root, 0, 1});
+        out->max_depth = 0;
+        while (!st4.empty()) {
+            auto [ti, gi, dep] = st4.back();
+            st4.pop_back();
+            out->max_depth = std::max(out->max_depth, dep);
+            int kids[4], nk = 0;
+            if (B.tmp[ti].left < 0) {
+                kids[nk++] = ti;  // the root is a leaf
+            } else {
+                kids[nk++] = B.tmp[ti].left;
+                kids[nk++] = B.tmp[ti].right;
+                while (nk < 4) {
+                    int best = -1;
+                    float best_area = -1.f;
+                    for (int c = 0; c < nk; ++c) {
+                        const TmpNode &k = B.tmp[kids[c]];
+                        if (k.left >= 0 && k.box.area() > best_area) {
+                            best_area = k.box.area();
+                            best = c;
+                        }
+                    }
+                    if (best < 0) break;
+                    const TmpNode &k = B.tmp[kids[best]];
+                    kids[best] = k.left;
+                    kids[nk++] = k.right;
+                }
+            }
+            GpuNode4 g;
+            std::memset(&g, 0, sizeof(g));
+            for (int c = 0; c < 4; ++c) {
+                if (c >= nk) {
+                    set4(g, c, never);
+                    g.child[c] = make_leaf_ref(0, 0);
+                    continue;
+                }
+                const TmpNode &k = B.tmp[kids[c]];
+                set4(g, c, k.box);
+                if (k.left < 0) {
+                    g.child[c] = emit_prims(k);
+                } else {
+                    int slot = (int)out->nodes4.size();
+                    out->nodes4.push_back(GpuNode4());
+                    g.child[c] = slot;
+                    st4.push_back({kids[c], slot, dep + 1});
+                }
+            }
+            out->nodes4[gi] = g;
+        }
+        return "";
+    }
 
     if (B.tmp[root].left < 0) {
         GpuNode g;

@@ -15,10 +15,12 @@ struct BuildInput {
     const float *spheres = nullptr;  // cx, cy, cz, r
     uint32_t n_spheres = 0;
     int max_prims_in_node = 4;
+    int width = 2;  // 2: GpuNode, 4: GpuNode4 (binary tree collapsed)
 };
 
 struct BuildOutput {
     std::vector<GpuNode> nodes;
+    std::vector<GpuNode4> nodes4;
     std::vector<PrimRecord> prims;  // leaf order
     uint32_t n_leaves = 0;
     uint32_t max_depth = 0;

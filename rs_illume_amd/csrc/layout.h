@@ -22,6 +22,18 @@ struct alignas(64) GpuNode {
 };
 static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
 
+// 128-byte four-child node: the six slabs of the four children as float4
+// rows (lo_x, hi_x, lo_y, hi_y, lo_z, hi_z), then the four child references.
+// Unused child slots have all planes at +inf (never entered) and an empty
+// leaf reference. Half the depth of the binary tree: half the dependent
+// node fetches per ray.
+struct alignas(128) GpuNode4 {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
+
 // Leaf reference: ~((first_slot << 4) | count), count in 1..15.
 constexpr int kLeafCountBits = 4;
 constexpr int kMaxLeafPrims = (1 << kLeafCountBits) - 1;

@@ -138,4 +138,114 @@ __device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, con
     return found;
 }
 
+// Leaf primitives of one leaf reference (shared by both node widths).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace_leaf(const PrimRecord *__restrict__ prims, int ref, f3 o, f3 d, float &t_max,
+                                           TraceHit &hit, bool &found, uint32_t &n_prim) {
+    const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 *pp = reinterpret_cast<const float4 *>(prims + first + k);
+        const float4 pa = pp[0];
+        const uint32_t tag = __float_as_uint(pa.w);
+        if (COUNT) ++n_prim;
+        if (tag & kSphereBit) {
+            const float4 pb = pp[1];
+            float t;
+            if (sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t)) {
+                found = true;
+                if (ANY) return true;
+                t_max = t;
+                hit.prim = tag & ~kSphereBit;
+                hit.slot = first + k;
+                hit.t = t;
+                hit.b0 = hit.b1 = hit.b2 = 0.f;
+            }
+        } else {
+            const float4 pb = pp[1], pc = pp[2];
+            const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
+            float t, b0, b1, b2;
+            if (triangle_test(p0, p1, p2, o, d, t_max, t, b0, b1, b2)) {
+                if (ANY) {
+                    found = true;
+                    return true;
+                }
+                if (!triangle_degenerate(p0, p1, p2)) {
+                    found = true;
+                    t_max = t;
+                    hit.prim = tag;
+                    hit.slot = first + k;
+                    hit.t = t;
+                    hit.b0 = b0;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+// Compare-exchange of (distance, reference) pairs for the 4-child order.
+__device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    tb = s ? ta : tb;
+    ta = t;
+    const int r = s ? rb : ra;
+    rb = s ? ra : rb;
+    ra = r;
+}
+
+// Traversal of the 4-wide tree: each visit tests four child boxes from one
+// 128-byte node, sorts the entered children by entry distance, descends into
+// the nearest and pushes the rest farthest-first. A stack of
+// kMaxTraceDepth entries holds 3 per level (the builder caps the depth).
+template <bool ANY, bool COUNT, int BLOCK>
+__device__ __forceinline__ bool trace_ray4(const GpuNode4 *__restrict__ nodes, const PrimRecord *__restrict__ prims,
+                                           f3 o, f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st,
+                                           uint32_t &n_box, uint32_t &n_prim) {
+    const f3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    hit.prim = 0xFFFFFFFFu;
+    hit.slot = 0;
+    hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
+    bool found = false;
+    int cur = 0;
+    st.sp = 0;
+    const float inf = __builtin_inff();
+    while (true) {
+        if (cur >= 0) {
+            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const int4 ch = reinterpret_cast<const int4 *>(np)[6];
+            if (COUNT) n_box += 4;
+            float t0, t1, t2, t3;
+            const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, t_max, t0);
+            const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, t_max, t1);
+            const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, t_max, t2);
+            const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, t_max, t3);
+            float ta = h0 ? t0 : inf, tb = h1 ? t1 : inf, tc = h2 ? t2 : inf, td = h3 ? t3 : inf;
+            int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
+            const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+            if (n_hit > 0) {
+                // sorting network for four keys; misses (inf) sink to the end
+                cswap(ta, ra, tb, rb);
+                cswap(tc, rc, td, rd);
+                cswap(ta, ra, tc, rc);
+                cswap(tb, rb, td, rd);
+                cswap(tb, rb, tc, rc);
+                if (n_hit > 3) st.push(rd);
+                if (n_hit > 2) st.push(rc);
+                if (n_hit > 1) st.push(rb);
+                cur = ra;
+                continue;
+            }
+        } else {
+            if (trace_leaf<ANY, COUNT>(prims, cur, o, d, t_max, hit, found, n_prim)) return true;
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    return found;
+}
+
 }  // namespace illume

@@ -12,8 +12,8 @@ namespace {
 
 constexpr int kBlock = 256;
 
-template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void traverse_kernel(const GpuNode *__restrict__ nodes,
+template <bool ANY, bool COUNT, int WIDTH>
+__global__ __launch_bounds__(kBlock) void traverse_kernel(const void *__restrict__ nodes,
                                                           const PrimRecord *__restrict__ prims,
                                                           const illume_ray *__restrict__ rays, uint32_t n,
                                                           illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
@@ -27,8 +27,11 @@ __global__ __launch_bounds__(kBlock) void traverse_kernel(const GpuNode *__restr
     st.reset(lds_stack + threadIdx.x);
     TraceHit h;
     uint32_t n_box = 0, n_prim = 0;
+    const f3 o = mk(r0.x, r0.y, r0.z), d = mk(r0.w, r1.x, r1.y);
     const bool found =
-        trace_ray<ANY, COUNT, kBlock>(nodes, prims, mk(r0.x, r0.y, r0.z), mk(r0.w, r1.x, r1.y), r1.z, h, st, n_box, n_prim);
+        WIDTH == 4
+            ? trace_ray4<ANY, COUNT, kBlock>((const GpuNode4 *)nodes, prims, o, d, r1.z, h, st, n_box, n_prim)
+            : trace_ray<ANY, COUNT, kBlock>((const GpuNode *)nodes, prims, o, d, r1.z, h, st, n_box, n_prim);
     if (COUNT) {
         counts[2 * (size_t)i] = n_box;
         counts[2 * (size_t)i + 1] = n_prim;
@@ -54,7 +57,22 @@ __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded
 
 int max_supported_depth() { return kMaxTraceDepth; }
 
-hipError_t launch_traverse(const GpuNode *nodes, const PrimRecord *prims, bool empty, const illume_ray *rays,
+template <int W>
+void launch_w(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n, illume_hit *hits,
+              uint8_t *occluded, uint32_t *counts, bool any, uint32_t grid, hipStream_t stream) {
+    if (counts) {
+        if (any)
+            traverse_kernel<true, true, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, nullptr, counts);
+        else
+            traverse_kernel<false, true, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, nullptr, counts);
+    } else if (any) {
+        traverse_kernel<true, false, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, occluded, nullptr);
+    } else {
+        traverse_kernel<false, false, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, hits, nullptr, nullptr);
+    }
+}
+
+hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims, bool empty, const illume_ray *rays,
                            uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
                            hipStream_t stream) {
     if (n == 0) return hipSuccess;
@@ -64,16 +82,10 @@ hipError_t launch_traverse(const GpuNode *nodes, const PrimRecord *prims, bool e
                                                       counts ? nullptr : (any ? occluded : nullptr), counts);
         return hipGetLastError();
     }
-    if (counts) {
-        if (any)
-            traverse_kernel<true, true><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, nullptr, counts);
-        else
-            traverse_kernel<false, true><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, nullptr, counts);
-    } else if (any) {
-        traverse_kernel<true, false><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, occluded, nullptr);
-    } else {
-        traverse_kernel<false, false><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, hits, nullptr, nullptr);
-    }
+    if (width == 4)
+        launch_w<4>(nodes, prims, rays, n, hits, occluded, counts, any, grid, stream);
+    else
+        launch_w<2>(nodes, prims, rays, n, hits, occluded, counts, any, grid, stream);
     return hipGetLastError();
 }
 

@@ -36,16 +36,17 @@ def _assert_parity(res, occ_gpu, occ_ref):
     assert np.array_equal(occ_gpu, occ_ref), "any-hit mismatch at %s" % np.nonzero(occ_gpu != occ_ref)[0][:20]
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("name", ["isect_tris_spheres", "isect_tris_only"])
-def test_golden_fixture(gpu, oracle, name):
+def test_golden_fixture(gpu, oracle, name, width):
     from rs_illume_amd.accel import BVHAccel
-    _say("case golden", name)
+    _say("case golden", name, "width", width)
     z = np.load(os.path.join(HERE, "golden", name + ".npz"))
     rays = np.ascontiguousarray(z["rays"]).view(oracle.RAY_DTYPE).reshape(-1)
     ref = np.zeros(len(rays), oracle.HIT_DTYPE)
     for k in ("prim", "t", "b1", "b2"):
         ref[k] = z[k]
-    bvh = BVHAccel(z["verts"], z["idx"], z["spheres"], 4)
+    bvh = BVHAccel(z["verts"], z["idx"], z["spheres"], 4, width=width)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(z["verts"], z["idx"], z["spheres"], 4)
     res = compare_closest(h, ref, osc, rays)
@@ -54,15 +55,16 @@ def test_golden_fixture(gpu, oracle, name):
     assert res["t_exact_frac"] == 1.0
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("seed,n_tris,n_spheres,leaf", [(101, 16384, 64, 4), (102, 30000, 0, 1), (103, 5000, 500, 8),
                                                         (104, 20000, 16, 15)])
-def test_seeded_scene(gpu, oracle, seed, n_tris, n_spheres, leaf):
+def test_seeded_scene(gpu, oracle, seed, n_tris, n_spheres, leaf, width):
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel
-    _say("case seeded", seed, n_tris, n_spheres, leaf)
+    _say("case seeded", seed, n_tris, n_spheres, leaf, "width", width)
     v, idx, sph = small_scene(n_tris, n_spheres, seed, median_sigma=0.02)
     rays = np.concatenate([W.sphere_to_cube_rays(50000, seed=seed + 1), edge_case_rays(v, idx, sph, seed + 2)])
-    bvh = BVHAccel(v, idx, sph, leaf)
+    bvh = BVHAccel(v, idx, sph, leaf, width=width)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(v, idx, sph, 4)
     ref = osc.intersect(rays)
@@ -91,7 +93,8 @@ def test_degenerate_scenes(gpu, oracle):
     _assert_parity(res, o, osc.intersect_p(rays))
 
 
-def test_tiny_and_empty(gpu, oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_tiny_and_empty(gpu, oracle, width):
     from rs_illume_amd.accel import BVHAccel
     from rs_illume_amd import workloads as W
     _say("case tiny/empty")
@@ -99,7 +102,7 @@ def test_tiny_and_empty(gpu, oracle):
     # single triangle (root is a leaf)
     v = np.array([[0, 0, 0.5], [1, 0, 0.5], [0, 1, 0.5]], np.float32)
     idx = np.array([[0, 1, 2]], np.uint32)
-    bvh = BVHAccel(v, idx, None, 4)
+    bvh = BVHAccel(v, idx, None, 4, width=width)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(v, idx, None, 4)
     res = compare_closest(h, osc.intersect(rays), osc, rays)
@@ -107,7 +110,7 @@ def test_tiny_and_empty(gpu, oracle):
     assert res["hits"] > 0
     # spheres only
     sph = np.array([[0.5, 0.5, 0.5, 0.25], [0.2, 0.8, 0.5, 0.1]], np.float32)
-    bvh = BVHAccel(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), sph, 4)
+    bvh = BVHAccel(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), sph, 4, width=width)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), sph, 4)
     res = compare_closest(h, osc.intersect(rays), osc, rays)
@@ -136,17 +139,18 @@ def test_invalid_arguments_gpu(gpu):
         BVHAccel(np.array([[0, 0, np.nan], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
 
 
-def test_full_size_1m_tris_16m_rays(gpu, oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_full_size_1m_tris_16m_rays(gpu, oracle, width):
     """The 1M-triangle, 16M-incoherent-ray workload at full size: oracle parity
     on a strided 1/64 sample, plus size-independent properties on all rays."""
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, hits_to_numpy, rays_to_device, split_hits
     n_tris, n_rays = 1 << 20, 1 << 24
-    _say("case full-size", n_tris, n_rays)
+    _say("case full-size", n_tris, n_rays, "width", width)
     v, idx = W.triangle_soup(n_tris, seed=1)
     rays = W.sphere_to_cube_rays(n_rays, seed=2)
-    bvh = BVHAccel(v, idx, None, 4)
+    bvh = BVHAccel(v, idx, None, 4, width=width)
     _say("bvh", bvh.info())
     rd = rays_to_device(rays)
     hd = bvh.intersect(rd)
