@@ -34,19 +34,28 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-    if src.endswith(".hip"):
-        cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + ["-c", src, "-o", obj]
-    else:
-        cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", src, "-o", obj]
+    # Every translation unit is compiled as HIP for gfx950: hipcc would give a
+    # .cpp file its default device target (not gfx950), so a kernel defined
+    # there would have no code object on the device. Host-side files must not
+    # define kernels at all; check_host_sources enforces that.
+    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     return obj
 
 
+def check_host_sources(srcs):
+    """Kernels live in .hip files only; a __global__ in a .cpp file is an error."""
+    for src in srcs:
+        if src.endswith(".cpp") and "__global__" in open(src).read():
+            raise RuntimeError("%s defines a kernel: move it to a .hip file" % src)
+
+
 def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
+    check_host_sources(srcs)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

@@ -53,6 +53,8 @@ struct WfState {
 
 // Resets the counters for a new batch of total_paths paths: all slots free.
 hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hipStream_t st);
+// Clears the training-record counter (a kernel, so it may sit between graph replays).
+hipError_t wf_launch_clear_records(const WfState &w, hipStream_t st);
 // One iteration: regenerate, extend, [infer], shade, bookkeeping.
 hipError_t wf_launch_iteration(const WfState &w, bool guided, const void *guide_dev, float *film,
                                unsigned long long *ray_counts, int block, hipStream_t st);

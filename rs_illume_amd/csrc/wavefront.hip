@@ -272,6 +272,15 @@ hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hip
     return hipGetLastError();
 }
 
+namespace {
+__global__ void wf_clear_records_kernel(WfCounters *c) { c->rec_count = 0ull; }
+}  // namespace
+
+hipError_t wf_launch_clear_records(const WfState &w, hipStream_t st) {
+    wf_clear_records_kernel<<<1, 1, 0, st>>>(w.ctr);
+    return hipGetLastError();
+}
+
 hipError_t wf_launch_iteration(const WfState &w, bool guided, const void *guide_dev, float *film,
                                unsigned long long *ray_counts, int block, hipStream_t st) {
     const GuideDev *g = (const GuideDev *)guide_dev;

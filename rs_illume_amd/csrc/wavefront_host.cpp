@@ -27,7 +27,11 @@ const GuideDev *guide_dev(const illume_guide *g);
 using namespace illume;
 
 namespace {
-typedef std::tuple<bool, const void *, float *, unsigned long long *, int, uint32_t> GraphKey;
+// A captured graph bakes in its kernels' arguments: the film and counter
+// pointers, the record buffer (training on or off) and the proposal's device
+// arrays (the whole GuideDev by value, so a new guide that reuses an old one's
+// host address never replays a graph holding freed weights).
+typedef std::tuple<bool, bool, std::string, float *, unsigned long long *, int, uint32_t> GraphKey;
 }
 
 struct illume_wavefront {
@@ -53,8 +57,6 @@ void drop_graphs(illume_wavefront *wf) {
     for (auto &kv : wf->graphs) hipGraphExecDestroy(kv.second);
     wf->graphs.clear();
 }
-
-__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 }  // namespace
 
 extern "C" {
@@ -199,12 +201,13 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
     unsigned long long *cnt = (unsigned long long *)ray_counts;
     const void *gdev = guided ? (const void *)guide_dev(guide) : nullptr;
     e = wf_launch_reset(w, total, st);
-    if (e == hipSuccess && train) zero_u64_kernel<<<1, 1, 0, st>>>(&w.ctr->rec_count);
+    if (e == hipSuccess && train) e = wf_launch_clear_records(w, st);
     // Every path finishes within max_depth + 1 iterations of its start; the
     // pool restarts at least one path per iteration whenever it is idle.
     const uint64_t max_iters = (total / wf->capacity + 2) * (uint64_t)(wf->max_depth + 2) + total + 16;
     const uint32_t k = std::max(1u, wf->graph_iters);
-    GraphKey key(guided, gdev, film, cnt, wf->block, w.n_jobs);
+    const std::string gbytes = guided ? std::string((const char *)gdev, sizeof(GuideDev)) : std::string();
+    GraphKey key(guided, train, gbytes, film, cnt, wf->block, w.n_jobs);
     hipGraphExec_t exec = nullptr;
     if (wf->graph_iters > 0 && e == hipSuccess) {
         auto it = wf->graphs.find(key);
@@ -238,8 +241,7 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
                 int r = illume_guide_train_step(guide, w.train_rec + 10 * (size_t)s, std::min(batch, n_rec - s), st);
                 if (r != ILLUME_OK) return r;
             }
-            zero_u64_kernel<<<1, 1, 0, st>>>(&w.ctr->rec_count);
-            e = hipGetLastError();
+            e = wf_launch_clear_records(w, st);
         }
         if (finished) break;
         if (iters > max_iters) {

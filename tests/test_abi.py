@@ -61,3 +61,12 @@ def test_null_arguments():
     assert L.illume_intersect_closest(None, None, 1, None, None) == 1
     assert L.illume_intersect_any(None, None, 1, None, None) == 1
     assert L.illume_bvh_destroy(None) == 0
+
+
+def test_kernels_only_in_hip_sources():
+    """Every kernel is compiled for gfx950: kernels may only be defined in .hip
+    files, and every source is compiled with the gfx950 device flags."""
+    from rs_illume_amd import build_lib
+    srcs = build_lib.sources()
+    build_lib.check_host_sources(srcs)
+    assert "--offload-arch=gfx950" in build_lib.DEVICE
