@@ -1,14 +1,24 @@
 #!/usr/bin/env python3
 """Headline benchmark: incoherent closest-hit throughput on the 1M-triangle
-scene (Mrays/s), one MI355X.
+scene (Mrays/s), one MI355X, plus the path-tracing measurements that share the
+metric line.
 
-Workload: 2^20 triangles (centroids uniform in the unit cube, vertex offsets
-Gaussian with lognormal sigma, median 0.005), 2^24 rays from a radius-2 sphere
-towards uniform points of the cube, SAH BVH with <= 4 primitives per leaf,
-FP32. A step = one closest-hit launch over all 2^24 rays with rays and BVH
-already resident in HBM. Prints one JSON line.
+Headline workload: 2^20 triangles (centroids uniform in the unit cube, vertex
+offsets Gaussian with lognormal sigma, median 0.005), 2^24 rays from a
+radius-2 sphere towards uniform points of the cube, SAH BVH with <= 4
+primitives per leaf, FP32. A step = one closest-hit launch over all 2^24 rays
+with rays and BVH already resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Also measured (keys of the same JSON line):
+  cornell  - Cornell box 256x256, 16 spp, depth 5: GPU samples/s and rays/s
+             (megakernel), and the oracle's CPU path tracer on host cores;
+  guided   - glossy small-light Cornell box 512x512, depth 8: wall time until
+             relMSE <= 0.01 against a 16384-spp plain render, plain GPU path
+             tracer vs the guided one (online training included in its time);
+  mlp      - the proposal network's MFMA kernels: inference and training
+             throughput as a fraction of the bf16 dense peak.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-render]
 """
 import argparse
 import json
@@ -22,39 +32,167 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 N_TRIS = 1 << 20
 N_RAYS = 1 << 24
+MLP_INFER_FLOP = 2 * (144 * 64 + 64 * 64 + 64 * 64 + 64 * 40)  # 39,936 per sample
+MLP_TRAIN_FLOP = 3 * MLP_INFER_FLOP                              # forward + two backward products
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(v, idx, rays, n_nodes_budget_s=15.0):
-    """The oracle's pbrt-v3 BVH timed on host cores over a bounded ray sample
-    of the same workload (BVH build excluded)."""
+def _oracle():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    return pyoracle
+
+
+def cpu_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(v, idx, rays, budget_s=15.0):
+    """The oracle's pbrt-v3 BVH timed on host cores over a bounded ray sample
+    of the same workload (BVH build excluded). Also returns the oracle's mean
+    node and primitive tests per ray on that sample: the per-ray work figure
+    of the algorithmic-bytes roofline."""
+    pyoracle = _oracle()
+    threads = cpu_threads()
     t0 = time.time()
     sc = pyoracle.OracleScene(v, idx, None, 4)
     build_s = time.time() - t0
     n = 1 << 16
     sample = rays[:: len(rays) // n][:n]
     t0 = time.time()
-    sc.intersect(sample, n_threads=threads)
+    _, st = sc.intersect(sample, n_threads=threads, with_stats=True)
     dt = time.time() - t0
     # scale the sample so that the timed CPU work is ~10-20 s
-    target = min(len(rays), max(n, int(n * n_nodes_budget_s / max(dt, 1e-3))))
+    target = min(len(rays), max(n, int(n * budget_s / max(dt, 1e-3))))
     target = 1 << int(np.log2(target))
     if target > n:
         sample = rays[:: len(rays) // target][:target]
         t0 = time.time()
-        sc.intersect(sample, n_threads=threads)
+        _, st = sc.intersect(sample, n_threads=threads, with_stats=True)
         dt = time.time() - t0
-    return {"value": len(sample) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+    counts = (float(st[0]) / len(sample), float(st[1]) / len(sample))
+    return {"value": round(len(sample) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "%d of the %d rays (strided), closest hit, oracle pbrt-v3 SAH BVH (build %.1f s excluded)"
-                      % (len(sample), len(rays), build_s)}
+                      % (len(sample), len(rays), build_s)}, counts
+
+
+def bench_cornell(no_cpu):
+    """Config-0 Cornell box on the megakernel path tracer; the oracle renders
+    a bounded number of samples per pixel on host cores."""
+    import torch
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.render import PathIntegrator, Scene
+    desc = S.cornell_box(0)
+    sc = Scene(desc)
+    cam = S.cornell_camera(256, 256)
+    spp = 16
+    pi = PathIntegrator(cam, spp=spp, max_depth=5, seed=0)
+    pi.render_sum(sc)
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    reps = 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pi.render_sum(sc, ray_counts=counts)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    rays = counts.cpu().numpy().astype(np.float64) / reps
+    out = {"workload": "Cornell box 256x256, 16 spp, max depth 5 (10 triangles, 2 spheres, 0.25^2 light)",
+           "samples_per_s": round(256 * 256 * spp / dt, 1), "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1),
+           "ms_per_frame": round(dt * 1e3, 3), "closest_rays": int(rays[0]), "any_rays": int(rays[1])}
+    if not no_cpu:
+        pyoracle = _oracle()
+        osc = pyoracle.OracleRenderScene(desc.arrays())
+        threads = cpu_threads()
+        t0 = time.time()
+        _, st = osc.render_sum(cam, 1, 5, seed=0, n_threads=threads)
+        dt1 = time.time() - t0
+        cspp = int(max(1, min(spp, 10.0 / max(dt1, 1e-3))))
+        t0 = time.time()
+        _, st = osc.render_sum(cam, cspp, 5, seed=0, n_threads=threads)
+        dtc = time.time() - t0
+        out["cpu_baseline"] = {"samples_per_s": round(256 * 256 * cspp / dtc, 1),
+                               "mrays_per_s": round(float(st.sum()) / dtc / 1e6, 3), "cores": threads,
+                               "kind": "port", "sample": "%d spp of the 256x256 frame" % cspp}
+    sc.close()
+    return out
+
+
+def bench_guided(ref_spp=16384, res=512, log=log):
+    """Config-3 equal-quality comparison (rs_illume_amd.converge)."""
+    import torch
+    from rs_illume_amd import converge as C
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.render import PathIntegrator, Scene
+    sc = Scene(S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625))
+    cam = S.cornell_camera(res, res)
+    pi = PathIntegrator(cam, spp=256, max_depth=8, seed=987654321)
+    ref = torch.zeros((res, res, 3), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(ref_spp // 256):
+        pi.render_sum(sc, rgb_sum=ref, spp=256, sample_offset=256 * k)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    ref /= ref_spp
+    log("reference %d spp in %.1f s" % (ref_spp, ref_s))
+    r = C.guided_vs_plain(sc, cam, ref, target=0.01, spp_per_pass=4, max_spp=4096, max_depth=8, seed=1)
+    out = {"workload": "glossy small-light Cornell box %dx%d, max depth 8, TR copper walls alpha~U[0.05,0.5], "
+                       "light 1%% of the ceiling; relMSE vs %d-spp plain render" % (res, res, ref_spp),
+           "target_rel_mse": 0.01, "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
+           "spp_per_pass": r["spp_per_pass"], "plain_renderer": r["plain_renderer"]}
+    for k in ("plain", "guided"):
+        rr = r[k]["reached"]
+        out[k] = {"seconds": round(rr["seconds"], 5), "spp": rr["spp"], "rel_mse": round(rr["rel_mse"], 5)} \
+            if rr else {"seconds": None, "spp": r[k]["spp"], "rel_mse": round(r[k]["curve"][-1][2], 5)}
+    out["time_ratio_guided_over_plain"] = round(r["time_ratio"], 3) if "time_ratio" in r else None
+    sc.close()
+    return out
+
+
+def bench_mlp():
+    """MFMA kernels of the proposal: inference over 2^20 samples and one Adam
+    step over 65,536 records, timed with HIP events on the launch stream."""
+    import torch
+    from rs_illume_amd.guide import Guide
+    g = Guide(seed=0)
+    n = 1 << 20
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.rand((n, 6), device="cuda", generator=gen)
+    x[:, 3:] = torch.nn.functional.normalize(x[:, 3:] * 2 - 1, dim=1)
+    out = torch.empty((n, 40), device="cuda")
+    recs = torch.rand((65536, 10), device="cuda", generator=gen)
+    recs[:, 3:6] = torch.nn.functional.normalize(recs[:, 3:6] * 2 - 1, dim=1)
+    recs[:, 6:9] = torch.nn.functional.normalize(recs[:, 6:9] * 2 - 1, dim=1)
+    for _ in range(3):
+        g.infer(x, out)
+        g.train_step(recs)
+    st = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        g.infer(x, out)
+    e1.record(st)
+    for _ in range(reps):
+        g.train_step(recs)
+    e2.record(st)
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    trn_ms = e1.elapsed_time(e2) / reps
+    inf_tf = MLP_INFER_FLOP * n / (inf_ms * 1e-3) / 1e12
+    trn_tf = MLP_TRAIN_FLOP * 65536 / (trn_ms * 1e-3) / 1e12
+    return {"infer": {"samples": n, "ms": round(inf_ms, 4), "tflops": round(inf_tf, 2),
+                      "frac_bf16_peak": round(inf_tf / BF16_PEAK_TFLOPS, 5)},
+            "train_step": {"samples": 65536, "ms": round(trn_ms, 4), "tflops": round(trn_tf, 2),
+                           "frac_bf16_peak": round(trn_tf / BF16_PEAK_TFLOPS, 5)},
+            "flop_per_sample": {"infer": MLP_INFER_FLOP, "train": MLP_TRAIN_FLOP}, "peak_tflops": BF16_PEAK_TFLOPS}
 
 
 def main():
@@ -62,7 +200,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing measurements")
     ap.add_argument("--rays", type=int, default=N_RAYS)
     ap.add_argument("--width", type=int, default=2, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
@@ -90,7 +229,6 @@ def main():
     # algorithmic work per ray from the instrumented traversal (outside the timed region)
     counts = bvh.intersect_counts(rd).double().mean(0).cpu().numpy()
     box_tests, prim_tests = float(counts[0]), float(counts[1])
-    bytes_per_ray = 32 + 8 + 32 * box_tests + 36 * prim_tests
 
     for _ in range(args.warmup):
         bvh.intersect(rd, hits)
@@ -109,13 +247,34 @@ def main():
     gpu_ms = ev0.elapsed_time(ev1)
     ms_per_step = gpu_ms / args.steps
     mrays = n / (ms_per_step * 1e-3) / 1e6
-    achieved_gbs = bytes_per_ray * n / (ms_per_step * 1e-3) / 1e9
 
     hit_frac = float(((hits[:, 0].to(torch.int64) & 0xFFFFFFFF) != 0xFFFFFFFF).float().mean())
-    cpu = None
+    cpu, oracle_counts = None, None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(v, idx, rays)
+        cpu, oracle_counts = cpu_baseline(v, idx, rays)
+    # algorithmic bytes per ray: the oracle's pbrt-layout BVH counts (32-byte
+    # nodes, 36-byte triangles) when the CPU leg ran, else the GPU's own counts
+    if oracle_counts:
+        n_node, n_tri = oracle_counts
+        src = "oracle pbrt BVH (node tests, triangle tests) on the cpu_baseline sample"
+    else:
+        n_node, n_tri = box_tests / 2.0, prim_tests
+        src = "GPU counting kernel (box tests / 2, primitive tests)"
+    alg_bytes = 32 + 8 + 32 * n_node + 36 * n_tri
+    achieved_gbs = alg_bytes * n / (ms_per_step * 1e-3) / 1e9
+    del rd, hits
+    bvh.close()
+    torch.cuda.empty_cache()
+
+    extra = {}
+    if not args.no_render:
+        log("cornell")
+        extra["cornell"] = bench_cornell(args.no_cpu)
+        log("mlp")
+        extra["mlp"] = bench_mlp()
+        log("guided vs plain")
+        extra["guided"] = bench_guided()
 
     out = {
         "metric": "Mrays/s closest-hit; time-to-relMSE 0.01, guided vs plain GPU path tracer",
@@ -136,14 +295,16 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_ray": round(bytes_per_ray, 1), "box_tests_per_ray": round(box_tests, 2),
-                     "prim_tests_per_ray": round(prim_tests, 2)},
+                     "bytes_per_ray": round(alg_bytes, 1), "bytes_source": src,
+                     "node_tests_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2),
+                     "gpu_box_tests_per_ray": round(box_tests, 2), "gpu_prim_tests_per_ray": round(prim_tests, 2)},
         "cpu_baseline": cpu,
         "hit_fraction": round(hit_frac, 4),
         "bvh": {"inner_nodes": info["n_inner_nodes"], "max_depth": info["max_depth"],
                 "device_mb": round(info["device_bytes"] / 2 ** 20, 1), "build_s": round(info["build_seconds"], 2)},
         "wall_s_timed": round(wall, 4),
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

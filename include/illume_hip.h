@@ -203,6 +203,8 @@ int illume_guide_infer(const illume_guide *guide, const float *inputs, uint32_t 
 int illume_guide_grad(const illume_guide *guide, const float *records, uint32_t n, float *grad_out, void *stream);
 /* One Adam step on the gradient of that loss over the batch. */
 int illume_guide_train_step(illume_guide *guide, const float *records, uint32_t n, void *stream);
+/* Adam steps taken so far and the loss of the last one (synchronises the device). */
+int illume_guide_stats(const illume_guide *guide, uint64_t *steps, float *last_loss);
 
 /* ------------------------------------------------------------------ wavefront renderer
  * The same Integrator::render as illume_render, executed as per-stage kernels
@@ -217,10 +219,14 @@ typedef struct illume_wavefront illume_wavefront;
 
 typedef struct {
     float guide_frac;      /* probability of sampling the proposal (one-sample MIS with the BSDF) */
-    uint32_t train;        /* 1: finished paths emit radiance samples; Adam steps run whenever
-                              train_batch samples are ready */
+    uint32_t train;        /* 0: no training. N > 0: finished paths emit radiance samples and,
+                              after every replay of the captured iterations that gathered at
+                              least train_batch samples, up to N Adam steps run, each on an
+                              interleaved subset of train_batch samples */
     uint32_t train_batch;  /* 0 = 65536 */
-    uint32_t reserved;
+    float min_alpha;       /* the proposal is used (and trained) only at diffuse vertices and at
+                              microfacet vertices with alpha >= min_alpha; near-specular lobes
+                              keep plain BSDF sampling. 0 = every non-specular vertex */
 } illume_guide_cfg;
 
 /* One render of a batch: its image occupies width*height*3 floats of the

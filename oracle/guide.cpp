@@ -45,7 +45,9 @@ T rnd(T x, bool bf16) {
 
 // Position normalised to [0,1]^3 by the scene bounds, normal in [-1,1]^3;
 // feature (6*i + ... ) order: input i, octave j -> sin(2^j pi v), cos(2^j pi v)
-// at index i*24 + 2j, i*24 + 2j + 1.
+// at index i*24 + 2j, i*24 + 2j + 1. The angle is reduced exactly first:
+// 2^j pi v = 2 pi (2^(j-1) v), and 2^(j-1) v and its fractional part are
+// exact in binary floating point, so sin/cos take 2 pi fract(2^(j-1) v).
 template <typename T>
 void encode(const float *x, const float *n, const float *bounds, T *f) {
     T v[kIn];
@@ -56,9 +58,10 @@ void encode(const float *x, const float *n, const float *bounds, T *f) {
     }
     for (int i = 0; i < kIn; ++i)
         for (int j = 0; j < kOct; ++j) {
-            T arg = (T)((double)(1 << j) * 3.14159265358979323846) * v[i];
-            f[i * 2 * kOct + 2 * j] = std::sin(arg);
-            f[i * 2 * kOct + 2 * j + 1] = std::cos(arg);
+            const double r = std::ldexp((double)v[i], j - 1);
+            const double arg = 6.28318530717958647692 * (r - std::floor(r));
+            f[i * 2 * kOct + 2 * j] = (T)std::sin(arg);
+            f[i * 2 * kOct + 2 * j + 1] = (T)std::cos(arg);
         }
 }
 

@@ -1,0 +1,83 @@
+"""Equal-quality comparison of the guided and the plain GPU path tracer:
+wall time until the running image reaches a relMSE target against a
+high-spp plain reference.
+
+Both estimators are progressive: passes of ``spp_per_pass`` samples per pixel
+are accumulated into one radiance sum (every pass uses its own sample
+indices, so the passes are independent samples of the same estimator), and
+after each pass the relMSE of the running mean against the reference is
+evaluated on the device. Only the render passes are timed (device
+synchronised before and after each one); the error evaluation is not. The
+guided tracer trains its proposal online inside its passes, and that
+training time is part of its time.
+"""
+import time
+
+import numpy as np
+import torch
+
+
+def rel_mse_dev(img, ref, eps=0.01):
+    """relMSE = mean over pixels and channels of (I - R)^2 / (R^2 + eps), on
+    the device in float64."""
+    d = img.double() - ref.double()
+    return float((d * d / (ref.double() ** 2 + eps)).mean())
+
+
+def progressive(render_pass, ref, target, spp_per_pass, max_spp, eps=0.01):
+    """render_pass(film, sample_offset) adds spp_per_pass samples per pixel to
+    film. Returns a dict with the time/spp at which relMSE <= target was first
+    reached (None if not within max_spp) and the convergence curve."""
+    film = torch.zeros_like(ref)
+    spp, elapsed = 0, 0.0
+    curve = []
+    reached = None
+    while spp < max_spp:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        render_pass(film, spp)
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        spp += spp_per_pass
+        e = rel_mse_dev(film / spp, ref, eps)
+        curve.append((spp, round(elapsed, 5), e))
+        if e <= target:
+            reached = {"seconds": elapsed, "spp": spp, "rel_mse": e}
+            break
+    return {"reached": reached, "curve": curve, "spp": spp, "seconds": elapsed}
+
+
+def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=4096, max_depth=8, guide=None,
+                    guide_frac=0.5, train_batch=65536, plain="megakernel", seed=1, max_paths=1 << 21, log=None):
+    """Time-to-relMSE of the plain and the guided tracer on one scene.
+    ``plain`` selects the plain renderer ("megakernel" or "wavefront")."""
+    from .guide import Guide
+    from .render import GuideCfg, PathIntegrator, Wavefront, WavefrontPathIntegrator
+    w, h = int(np.asarray(camera)["width"].reshape(-1)[0]), int(np.asarray(camera)["height"].reshape(-1)[0])
+    if plain == "megakernel":
+        pi = PathIntegrator(camera, spp=spp_per_pass, max_depth=max_depth, seed=seed)
+        plain_pass = lambda film, off: pi.render_sum(scene, rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+    else:
+        wpi = WavefrontPathIntegrator(scene, camera, spp=spp_per_pass, max_depth=max_depth, seed=seed,
+                                      max_paths=max_paths)
+        plain_pass = lambda film, off: wpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+    plain_pass(torch.zeros((h, w, 3), dtype=torch.float32, device="cuda"), 0)  # warm-up
+    rp = progressive(plain_pass, ref, target, spp_per_pass, max_spp)
+    if log:
+        log("plain", rp["reached"], rp["curve"][-1])
+    g = guide if guide is not None else Guide(seed=seed)
+    wf = Wavefront(max_paths=max_paths, max_depth=max_depth)
+    gpi = WavefrontPathIntegrator(scene, camera, spp=spp_per_pass, max_depth=max_depth, seed=seed + 1000, guide=g,
+                                  guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf)
+    # warm-up on a throw-away guide so graph capture is not timed
+    wg = WavefrontPathIntegrator(scene, camera, spp=1, max_depth=max_depth, seed=seed + 2000, guide=Guide(seed=seed + 1),
+                                 guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf)
+    wg.render_sum(rgb_sum=torch.zeros((h, w, 3), dtype=torch.float32, device="cuda"))
+    guided_pass = lambda film, off: gpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+    rg = progressive(guided_pass, ref, target, spp_per_pass, max_spp)
+    if log:
+        log("guided", rg["reached"], rg["curve"][-1])
+    out = {"plain": rp, "guided": rg, "target": target, "spp_per_pass": spp_per_pass, "plain_renderer": plain}
+    if rp["reached"] and rg["reached"]:
+        out["time_ratio"] = rg["reached"]["seconds"] / rp["reached"]["seconds"]
+    return out

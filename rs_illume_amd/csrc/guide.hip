@@ -24,6 +24,9 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kRows = 32;    // rows per wave
 constexpr int kWave = 64;
 constexpr int kEnc = 144, kEncPad = 160, kHid = 64, kOut = 40, kOutPad = 48, kLobes = 8;
+constexpr uint32_t kGradBlocks = 256;  // target grid of the gradient kernel
+constexpr int kGradWaves = 2;          // waves per gradient block (LDS: 2 x 36 KB + 79 KB)
+static_assert(kGradPartialFloats == (size_t)kGradBlocks * kGuideParams, "one partial row per gradient block");
 
 __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -59,6 +62,9 @@ struct WaveLds {
 
 // Frequency encoding: input i (position xyz in [0,1], normal xyz), octave j:
 // sin(2^j pi v) at 24 i + 2 j, cos at 24 i + 2 j + 1; columns 144..159 zero.
+// The angle is reduced exactly (2^j pi v = 2 pi fract(2^(j-1) v): scaling by
+// a power of two and taking the fraction are exact) and the hardware
+// v_sin/v_cos, which take their argument in revolutions, do the rest.
 __device__ void encode_tile(const float *__restrict__ in, int in_stride, uint32_t row0, uint32_t n, __bf16 *a0,
                             int lane) {
     for (int idx = lane; idx < kRows * kEncPad; idx += kWave) {
@@ -68,31 +74,45 @@ __device__ void encode_tile(const float *__restrict__ in, int in_stride, uint32_
         if (f < kEnc && row < n) {
             const int i = f / 24, j = (f % 24) / 2;
             const float v = in[(size_t)row * in_stride + i];
-            const float arg = (float)((double)(1 << j) * 3.14159265358979323846) * v;
-            val = (f & 1) ? cosf(arg) : sinf(arg);
+            const float r = ldexpf(v, j - 1);
+            const float rev = r - floorf(r);
+            val = (f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
         }
         a0[idx] = to_bf16(val);
     }
 }
 
 // out_tile = act(A[32 x K] * W[K x N] + b): W given n-major bf16 [N][K].
+// Every B fragment of the layer (N/16 x K/32 of them, 4 VGPRs each) is loaded
+// before the first MFMA, so the wave waits for global memory once per layer
+// rather than once per product.
 template <int K, int N, bool RELU>
 __device__ void dense(const __bf16 *a, const __bf16 *__restrict__ wt, const float *__restrict__ b, __bf16 *out_bf,
                       float *out_f, int lane) {
+    constexpr int KS = K / 32, NT = N / 16;
+    bf16x8 bw[NT][KS];
+    float bias[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) bw[ct][ks] = glb_b(wt, K, ct * 16, ks * 32, lane);
+        bias[ct] = b[ct * 16 + (lane & 15)];
+    }
 #pragma unroll
     for (int rt = 0; rt < kRows / 16; ++rt) {
+        bf16x8 af[KS];
 #pragma unroll
-        for (int ct = 0; ct < N / 16; ++ct) {
+        for (int ks = 0; ks < KS; ++ks) af[ks] = lds_a(a, K, rt * 16, ks * 32, lane);
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k0 = 0; k0 < K; k0 += 32)
-                acc = mfma(lds_a(a, K, rt * 16, k0, lane), glb_b(wt, K, ct * 16, k0, lane), acc);
+            for (int ks = 0; ks < KS; ++ks) acc = mfma(af[ks], bw[ct][ks], acc);
             const int col = ct * 16 + (lane & 15);
-            const float bias = b[col];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int row = rt * 16 + (lane >> 4) * 4 + e;
-                float v = acc[e] + bias;
+                float v = acc[e] + bias[ct];
                 if (RELU) {
                     v = v > 0.f ? v : 0.f;
                     out_bf[row * N + col] = to_bf16(v);
@@ -104,29 +124,58 @@ __device__ void dense(const __bf16 *a, const __bf16 *__restrict__ wt, const floa
     }
 }
 
+// Forward pass of one tile keeping every layer's activations (for the
+// backward pass). A wave with active == false only takes part in the block's
+// barriers.
 __device__ void forward_tile(const GuideDev &g, const float *__restrict__ in, int in_stride, uint32_t row0,
-                             uint32_t n, WaveLds &s, int lane) {
-    encode_tile(in, in_stride, row0, n, s.a0, lane);
+                             uint32_t n, WaveLds &s, int lane, bool active) {
+    if (active) encode_tile(in, in_stride, row0, n, s.a0, lane);
     __syncthreads();
-    dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.a1, nullptr, lane);
+    if (active) dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.a1, nullptr, lane);
     __syncthreads();
-    dense<kHid, kHid, true>(s.a1, g.wt[1], g.bias[1], s.a2, nullptr, lane);
+    if (active) dense<kHid, kHid, true>(s.a1, g.wt[1], g.bias[1], s.a2, nullptr, lane);
     __syncthreads();
-    dense<kHid, kHid, true>(s.a2, g.wt[2], g.bias[2], s.a3, nullptr, lane);
+    if (active) dense<kHid, kHid, true>(s.a2, g.wt[2], g.bias[2], s.a3, nullptr, lane);
     __syncthreads();
-    dense<kHid, kOutPad, false>(s.a3, g.wt[3], g.bias[3], nullptr, s.y, lane);
+    if (active) dense<kHid, kOutPad, false>(s.a3, g.wt[3], g.bias[3], nullptr, s.y, lane);
     __syncthreads();
+}
+
+// Inference keeps only what the next layer reads: the encoding buffer (10 KB)
+// and one hidden buffer (4 KB); hidden layers alternate between them and the
+// fp32 outputs (6 KB) reuse the encoding buffer. 14 KB per wave lets 11 waves
+// share a CU's LDS.
+struct alignas(16) InferLds {
+    __bf16 a0[kRows * kEncPad];
+    __bf16 h[kRows * kHid];
+};
+static_assert(kRows * kOutPad * sizeof(float) <= sizeof(InferLds::a0), "outputs fit the encoding buffer");
+
+__device__ const float *forward_infer(const GuideDev &g, const float *__restrict__ in, uint32_t row0, uint32_t n,
+                                      InferLds &s, int lane) {
+    encode_tile(in, 6, row0, n, s.a0, lane);
+    __syncthreads();
+    dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.h, nullptr, lane);
+    __syncthreads();
+    dense<kHid, kHid, true>(s.h, g.wt[1], g.bias[1], s.a0, nullptr, lane);
+    __syncthreads();
+    dense<kHid, kHid, true>(s.a0, g.wt[2], g.bias[2], s.h, nullptr, lane);
+    __syncthreads();
+    float *y = reinterpret_cast<float *>(s.a0);
+    dense<kHid, kOutPad, false>(s.h, g.wt[3], g.bias[3], nullptr, y, lane);
+    __syncthreads();
+    return y;
 }
 
 __global__ __launch_bounds__(kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in, uint32_t n,
                                                             float *__restrict__ out) {
-    __shared__ WaveLds s;
+    __shared__ InferLds s;
     const int lane = threadIdx.x;
     const uint32_t row0 = blockIdx.x * kRows;
-    forward_tile(g, in, 6, row0, n, s, lane);
+    const float *y = forward_infer(g, in, row0, n, s, lane);
     for (int idx = lane; idx < kRows * kOut; idx += kWave) {
         const int r = idx / kOut, c = idx % kOut;
-        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = s.y[r * kOutPad + c];
+        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = y[r * kOutPad + c];
     }
 }
 
@@ -135,15 +184,15 @@ __global__ __launch_bounds__(kWave) void guide_infer_kernel(GuideDev g, const fl
 __global__ __launch_bounds__(kWave) void guide_infer_count_kernel(GuideDev g, const float *__restrict__ in,
                                                                   const uint32_t *__restrict__ n_ptr, uint32_t max_n,
                                                                   float *__restrict__ out) {
-    __shared__ WaveLds s;
+    __shared__ InferLds s;
     const uint32_t n = min(*n_ptr, max_n);
     const uint32_t row0 = blockIdx.x * kRows;
     if (row0 >= n) return;
     const int lane = threadIdx.x;
-    forward_tile(g, in, 6, row0, n, s, lane);
+    const float *y = forward_infer(g, in, row0, n, s, lane);
     for (int idx = lane; idx < kRows * kOut; idx += kWave) {
         const int r = idx / kOut, c = idx % kOut;
-        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = s.y[r * kOutPad + c];
+        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = y[r * kOutPad + c];
     }
 }
 
@@ -189,9 +238,10 @@ __device__ void nll_grad_row(const float *y, const float *w, float scale, float 
 
 // dW[in][out] += A^T dZ over the tile's rows; A [32][KA] (bf16, LDS),
 // dZ [32][NZ] (bf16, LDS); only in < in_valid and out < out_valid are stored.
+// gw is the block's fp32 accumulator in LDS, shared by the block's waves
+// (LDS float atomics).
 template <int KA, int NZ>
-__device__ void accumulate_dw(const __bf16 *a, const __bf16 *dz, float *__restrict__ gw, int in_valid, int out_valid,
-                              int lane) {
+__device__ void accumulate_dw(const __bf16 *a, const __bf16 *dz, float *gw, int in_valid, int out_valid, int lane) {
     for (int it = 0; it < (in_valid + 15) / 16; ++it) {
         for (int ot = 0; ot < (out_valid + 15) / 16; ++ot) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -207,18 +257,28 @@ __device__ void accumulate_dw(const __bf16 *a, const __bf16 *dz, float *__restri
 }
 
 // dA[32][64] = dZ[32][KZ] * W^T with W k-major bf16 [64(in)][KZ(out)];
-// dZ_prev = dA * (A_prev > 0) -> bf16; db_prev accumulated by the caller's layer.
+// dZ_prev = dA * (A_prev > 0) -> bf16; the bias gradient of the previous
+// layer (column sums of dZ_prev) is added to gb_prev (LDS). As in dense(),
+// all B fragments are loaded first.
 template <int KZ>
 __device__ void backprop(const __bf16 *dz, const __bf16 *__restrict__ wk, const __bf16 *a_prev, __bf16 *dz_prev,
-                         float *__restrict__ gb_prev, int lane) {
+                         float *gb_prev, int lane) {
+    constexpr int KS = KZ / 32, NT = kHid / 16;
+    bf16x8 bw[NT][KS];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) bw[ct][ks] = glb_b(wk, KZ, ct * 16, ks * 32, lane);
 #pragma unroll
     for (int rt = 0; rt < kRows / 16; ++rt) {
+        bf16x8 af[KS];
 #pragma unroll
-        for (int ct = 0; ct < kHid / 16; ++ct) {
+        for (int ks = 0; ks < KS; ++ks) af[ks] = lds_a(dz, KZ, rt * 16, ks * 32, lane);
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k0 = 0; k0 < KZ; k0 += 32)
-                acc = mfma(lds_a(dz, KZ, rt * 16, k0, lane), glb_b(wk, KZ, ct * 16, k0, lane), acc);
+            for (int ks = 0; ks < KS; ++ks) acc = mfma(af[ks], bw[ct][ks], acc);
             const int col = ct * 16 + (lane & 15);
             float colsum = 0.f;
 #pragma unroll
@@ -234,55 +294,101 @@ __device__ void backprop(const __bf16 *dz, const __bf16 *__restrict__ wk, const 
 }
 
 // records: [n][10] = position (3, normalised), normal (3), direction (3), weight
-__global__ __launch_bounds__(kWave) void guide_grad_kernel(GuideDev g, const float *__restrict__ rec, uint32_t n,
-                                                           const float *__restrict__ wsum, float *__restrict__ grad,
-                                                           float *__restrict__ loss) {
-    __shared__ WaveLds s;
-    const int lane = threadIdx.x;
-    const uint32_t row0 = blockIdx.x * kRows;
-    forward_tile(g, rec, 10, row0, n, s, lane);
+// (row r at rec + stride_f * r). kGradWaves waves per block; each wave walks
+// `tiles` 32-row tiles with its own activations in LDS, and the block
+// accumulates the gradient of every parameter in one fp32 LDS array
+// (parameter order, LDS float atomics), stored once at the end as the block's
+// row of `partial` ([blocks][params]); guide_reduce_kernel sums the rows.
+// No global atomics: device-scope float atomics on 20k addresses from every
+// block serialise at the memory side. Every wave runs the same number of
+// tile iterations (tiles past n are skipped between the barriers), so each
+// __syncthreads is reached by all waves.
+struct GradLds {
+    WaveLds s[kGradWaves];
+    float dw[kGuideParams];
+};
+static_assert(sizeof(GradLds) <= 160 * 1024, "gradient kernel LDS within one CU's 160 KB");
+
+__global__ __launch_bounds__(kGradWaves *kWave) void guide_grad_kernel(
+    GuideDev g, const float *__restrict__ rec, uint32_t n, uint32_t stride_f, uint32_t tiles,
+    const float *__restrict__ wsum, float *__restrict__ partial, float *__restrict__ loss) {
+    __shared__ GradLds L;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    WaveLds &s = L.s[wave];
+    for (int i = threadIdx.x; i < kGuideParams; i += kGradWaves * kWave) L.dw[i] = 0.f;
     const float ws = *wsum;
-    // output gradient: lane r < 32 owns row r
-    float row_loss = 0.f;
-    if (lane < kRows) {
-        float gy[kOut];
-        const uint32_t row = row0 + lane;
-        const float wgt = row < n ? rec[(size_t)row * 10 + 9] : 0.f;
-        if (row < n && wgt > 0.f && ws > 0.f) {
-            const float *r = rec + (size_t)row * 10;
-            const float w3[3] = {r[6], r[7], r[8]};
-            nll_grad_row(s.y + lane * kOutPad, w3, wgt / ws, gy, &row_loss);
-        } else {
-            for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
+    float block_loss = 0.f;
+    for (uint32_t t = 0; t < tiles; ++t) {
+        const uint32_t row0 = ((blockIdx.x * tiles + t) * kGradWaves + wave) * kRows;
+        const bool active = row0 < n;
+        __syncthreads();
+        forward_tile(g, rec, (int)stride_f, row0, n, s, lane, active);
+        // output gradient: lane r < 32 owns row r
+        if (active) {
+            float gy[kOut], row_loss = 0.f;
+            const uint32_t row = row0 + lane;
+            const float wgt = (lane < kRows && row < n) ? rec[(size_t)row * stride_f + 9] : 0.f;
+            if (lane < kRows && row < n && wgt > 0.f && ws > 0.f) {
+                const float *r = rec + (size_t)row * stride_f;
+                const float w3[3] = {r[6], r[7], r[8]};
+                nll_grad_row(s.y + lane * kOutPad, w3, wgt / ws, gy, &row_loss);
+            } else {
+                for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
+            }
+            block_loss += row_loss;
+            if (lane < kRows)
+                for (int o = 0; o < kHid; ++o) s.dz[0][lane * kHid + o] = to_bf16(o < kOut ? gy[o] : 0.f);
+            // bias of the output layer: column sums over the tile's rows
+            float *gb4 = L.dw + g.b_off[3];
+            for (int o = 0; o < kOut; ++o) {
+                float v = gy[o];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0 && v != 0.f) atomicAdd(gb4 + o, v);
+            }
         }
-        for (int o = 0; o < kHid; ++o) s.dz[0][lane * kHid + o] = to_bf16(o < kOut ? gy[o] : 0.f);
-        float *gb4 = grad + g.b_off[3];
-        for (int o = 0; o < kOut; ++o)
-            if (gy[o] != 0.f) atomicAdd(gb4 + o, gy[o]);
+        __syncthreads();
+        // layer 4: dW4 = A3^T dY, dZ3 = (dY W4^T) * (A3 > 0)
+        if (active) {
+            accumulate_dw<kHid, kHid>(s.a3, s.dz[0], L.dw + g.w_off[3], kHid, kOut, lane);
+            backprop<kHid>(s.dz[0], g.wk[3], s.a3, s.dz[1], L.dw + g.b_off[2], lane);
+        }
+        __syncthreads();
+        if (active) {
+            accumulate_dw<kHid, kHid>(s.a2, s.dz[1], L.dw + g.w_off[2], kHid, kHid, lane);
+            backprop<kHid>(s.dz[1], g.wk[2], s.a2, s.dz[0], L.dw + g.b_off[1], lane);
+        }
+        __syncthreads();
+        if (active) {
+            accumulate_dw<kHid, kHid>(s.a1, s.dz[0], L.dw + g.w_off[1], kHid, kHid, lane);
+            backprop<kHid>(s.dz[0], g.wk[1], s.a1, s.dz[1], L.dw + g.b_off[0], lane);
+        }
+        __syncthreads();
+        if (active) accumulate_dw<kEncPad, kHid>(s.a0, s.dz[1], L.dw + g.w_off[0], kEnc, kHid, lane);
     }
     if (loss) {
-        float l = row_loss;
+        float l = block_loss;
         for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
         if (lane == 0 && l != 0.f) atomicAdd(loss, l);
     }
     __syncthreads();
-    // layer 4: dW4 = A3^T dY, dZ3 = (dY W4^T) * (A3 > 0)
-    accumulate_dw<kHid, kHid>(s.a3, s.dz[0], grad + g.w_off[3], kHid, kOut, lane);
-    backprop<kHid>(s.dz[0], g.wk[3], s.a3, s.dz[1], grad + g.b_off[2], lane);
-    __syncthreads();
-    accumulate_dw<kHid, kHid>(s.a2, s.dz[1], grad + g.w_off[2], kHid, kHid, lane);
-    backprop<kHid>(s.dz[1], g.wk[2], s.a2, s.dz[0], grad + g.b_off[1], lane);
-    __syncthreads();
-    accumulate_dw<kHid, kHid>(s.a1, s.dz[0], grad + g.w_off[1], kHid, kHid, lane);
-    backprop<kHid>(s.dz[0], g.wk[1], s.a1, s.dz[1], grad + g.b_off[0], lane);
-    __syncthreads();
-    accumulate_dw<kEncPad, kHid>(s.a0, s.dz[1], grad + g.w_off[0], kEnc, kHid, lane);
+    float *dst = partial + (size_t)blockIdx.x * kGuideParams;
+    for (int i = threadIdx.x; i < kGuideParams; i += kGradWaves * kWave) dst[i] = L.dw[i];
 }
 
-__global__ void guide_wsum_kernel(const float *__restrict__ rec, uint32_t n, float *__restrict__ wsum) {
+// grad[i] = sum over the blocks' rows of partial[b][i] (coalesced over i).
+__global__ void guide_reduce_kernel(const float *__restrict__ partial, uint32_t blocks, float *__restrict__ grad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)kGuideParams) return;
+    float s = 0.f;
+    for (uint32_t b = 0; b < blocks; ++b) s += partial[(size_t)b * kGuideParams + i];
+    grad[i] = s;
+}
+
+__global__ void guide_wsum_kernel(const float *__restrict__ rec, uint32_t n, uint32_t stride_f,
+                                  float *__restrict__ wsum) {
     float s = 0.f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float w = rec[(size_t)i * 10 + 9];
+        const float w = rec[(size_t)i * stride_f + 9];
         if (w > 0.f) s += w;
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -346,14 +452,22 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
     return hipGetLastError();
 }
 
-hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, float *grad, float *wsum, float *loss,
-                             hipStream_t st) {
-    zero_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(grad, kGuideParams);
+hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
+                             float *wsum, float *loss, float *partial, hipStream_t st) {
     zero_kernel<<<1, 64, 0, st>>>(wsum, 1);
     if (loss) zero_kernel<<<1, 64, 0, st>>>(loss, 1);
-    if (n == 0) return hipGetLastError();
-    guide_wsum_kernel<<<256, 256, 0, st>>>(rec, n, wsum);
-    guide_grad_kernel<<<(n + kRows - 1) / kRows, kWave, 0, st>>>(g, rec, n, wsum, grad, loss);
+    if (n == 0) {
+        zero_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(grad, kGuideParams);
+        return hipGetLastError();
+    }
+    const uint32_t stride_f = 10u * (stride ? stride : 1u);
+    guide_wsum_kernel<<<256, 256, 0, st>>>(rec, n, stride_f, wsum);
+    // about one block per CU: each block holds 117 KB of LDS
+    const uint32_t n_tiles = (n + kRows - 1) / kRows;
+    const uint32_t tiles = (n_tiles + kGradBlocks * kGradWaves - 1) / (kGradBlocks * kGradWaves);  // per wave
+    const uint32_t blocks = (n_tiles + tiles * kGradWaves - 1) / (tiles * kGradWaves);
+    guide_grad_kernel<<<blocks, kGradWaves * kWave, 0, st>>>(g, rec, n, stride_f, tiles, wsum, partial, loss);
+    guide_reduce_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(partial, blocks, grad);
     return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ struct illume_guide {
     GuideDev dev;
     std::vector<void *> allocs;
     float *grad = nullptr, *wsum = nullptr, *loss = nullptr;
+    float *partial = nullptr;  // per-block gradients of a training step
     float lr = 1e-3f;
     int step = 0;
 };
@@ -83,6 +84,7 @@ int illume_guide_create(const float *init_params, float lr, illume_guide **out) 
     if (e == hipSuccess) e = dalloc(g, &g->grad, kGuideParams);
     if (e == hipSuccess) e = dalloc(g, &g->wsum, 64);
     if (e == hipSuccess) e = dalloc(g, &g->loss, 64);
+    if (e == hipSuccess) e = dalloc(g, &g->partial, kGradPartialFloats);
     if (e == hipSuccess) e = hipMemcpy(d.params, init_params, kGuideParams * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = guide_launch_sync(d, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -108,17 +110,40 @@ int illume_guide_infer(const illume_guide *g, const float *inputs, uint32_t n, f
 
 int illume_guide_grad(const illume_guide *g, const float *records, uint32_t n, float *grad_out, void *stream) {
     if (!g || !grad_out || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
-    hipError_t e = guide_launch_grad(g->dev, records, n, grad_out, g->wsum, nullptr, (hipStream_t)stream);
+    hipError_t e = guide_launch_grad(g->dev, records, n, 1, grad_out, g->wsum, nullptr, g->partial, (hipStream_t)stream);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide grad launch");
 }
+
+}  // extern "C"
+
+namespace illume {
+// One Adam step on n records read with the given stride (in records).
+int guide_train_strided(illume_guide *g, const float *records, uint32_t n, uint32_t stride, hipStream_t st) {
+    if (n == 0) return ILLUME_OK;
+    hipError_t e = guide_launch_grad(g->dev, records, n, stride, g->grad, g->wsum, g->loss, g->partial, st);
+    if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
+    return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
+}
+}  // namespace illume
+
+extern "C" {
 
 int illume_guide_train_step(illume_guide *g, const float *records, uint32_t n, void *stream) {
     if (!g || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (n == 0) return ILLUME_OK;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = guide_launch_grad(g->dev, records, n, g->grad, g->wsum, g->loss, st);
+    hipError_t e = guide_launch_grad(g->dev, records, n, 1, g->grad, g->wsum, g->loss, g->partial, st);
     if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
+}
+
+int illume_guide_stats(const illume_guide *g, uint64_t *steps, float *last_loss) {
+    if (!g || !steps || !last_loss) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    *steps = (uint64_t)g->step;
+    *last_loss = 0.f;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && g->step > 0) e = hipMemcpy(last_loss, g->loss, sizeof(float), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide stats");
 }
 
 }  // extern "C"

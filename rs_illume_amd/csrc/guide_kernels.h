@@ -25,9 +25,12 @@ hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, fl
 hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
                                     float *out, hipStream_t st);
 // Gradient of the weighted NLL over n records into grad (zeroed first); wsum
-// (device float) receives the weight sum, loss (optional) the loss.
-hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, float *grad, float *wsum, float *loss,
-                             hipStream_t st);
+// (device float) receives the weight sum, loss (optional) the loss. Record r
+// is read at rec + 10 * stride * r (stride >= 1 trains on an even subset).
+// partial: scratch of kGradPartialFloats floats (per-block gradients).
+constexpr size_t kGradPartialFloats = (size_t)256 * kGuideParams;
+hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
+                             float *wsum, float *loss, float *partial, hipStream_t st);
 hipError_t guide_launch_adam(const GuideDev &g, const float *grad, float lr, int step, hipStream_t st);
 // Rewrites the bf16 copies and biases from the fp32 master weights.
 hipError_t guide_launch_sync(const GuideDev &g, hipStream_t st);

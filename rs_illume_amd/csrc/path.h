@@ -257,7 +257,15 @@ struct GuideCfg {
     const float *out;         // [paths][40] network outputs, null = plain path tracing
     float frac;               // probability of sampling the proposal (one-sample MIS)
     float bounds_lo[3], inv_ext[3];
+    float min_alpha;          // microfacet vertices below this roughness are not guided
 };
+
+// Vertices where the proposal is queried and trained: diffuse, or microfacet
+// with alpha >= min_alpha.
+__device__ __forceinline__ bool guidable(const DevMaterial &m, float min_alpha) {
+    if (m.type == kMetal) return m.alpha >= min_alpha;
+    return m.type == kMatte && !(m.kd[0] == 0.f && m.kd[1] == 0.f && m.kd[2] == 0.f);
+}
 
 // One training vertex: proposal input (normalised position, normal), the
 // sampled direction and its mixture pdf, the throughput after the vertex and
@@ -383,7 +391,7 @@ __device__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps
         float eta = bsdf.eta;
         ps.eta_scale *= (dot(wo, it.n) > 0) ? (eta * eta) : 1 / (eta * eta);
     }
-    if (recs && nonspec && n_recs < max_depth) {
+    if (recs && nonspec && n_recs < max_depth && guidable(bsdf.m, gc.min_alpha)) {
         VertexRec &r = recs[n_recs++];
         r.in[0] = (it.p.x - gc.bounds_lo[0]) * gc.inv_ext[0];
         r.in[1] = (it.p.y - gc.bounds_lo[1]) * gc.inv_ext[1];

@@ -145,9 +145,7 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
         int gslot = -1;
         if (guided && found && job.gc.out) {
             const DevMaterial &m = sc.materials[sc.prim_material[h.prim]];
-            const bool nonspec = (m.type == kMetal) ||
-                                 (m.type == kMatte && !(m.kd[0] == 0.f && m.kd[1] == 0.f && m.kd[2] == 0.f));
-            if (nonspec && w.flags[p].x < job.max_depth) {
+            if (guidable(m, job.gc.min_alpha) && w.flags[p].x < job.max_depth) {
                 Isect it;
                 hit_isect(sc, h, o, d, it);
                 gslot = (int)atomicAdd(&w.ctr->guide_count, 1u);

@@ -22,6 +22,7 @@ uint32_t scene_prims(const illume_scene *s);
 void scene_bounds(const illume_scene *s, float *b);
 bool scene_camera(const illume_camera &c, DevCamera *out);
 const GuideDev *guide_dev(const illume_guide *g);
+int guide_train_strided(illume_guide *g, const float *records, uint32_t n, uint32_t stride, hipStream_t st);
 }  // namespace illume
 
 using namespace illume;
@@ -134,6 +135,7 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
     if (!wf || !film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (gcfg && !(gcfg->guide_frac >= 0.f && gcfg->guide_frac <= 1.f))
         return abi_fail(ILLUME_ERR_INVALID_ARG, "guide_frac must be in [0, 1]");
+    if (gcfg && !(gcfg->min_alpha >= 0.f)) return abi_fail(ILLUME_ERR_INVALID_ARG, "min_alpha must be >= 0");
     const bool guided = guide && gcfg && gcfg->guide_frac > 0.f;
     const bool train = guide && gcfg && gcfg->train;
     const uint32_t batch = (gcfg && gcfg->train_batch) ? gcfg->train_batch : 65536u;
@@ -172,6 +174,7 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
             job.gc.inv_ext[a] = ext > 0.f ? 1.f / ext : 1.f;
         }
         job.gc.frac = guided ? gcfg->guide_frac : 0.f;
+        job.gc.min_alpha = gcfg ? gcfg->min_alpha : 0.f;
         job.gc.out = guided ? wf->w.guide_out : nullptr;
         hj.push_back(job);
         total += n;
@@ -236,9 +239,15 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
         if (e != hipSuccess) break;
         const bool finished = c.done_paths >= c.total_paths;
         if (train && (c.rec_count >= batch || finished) && c.rec_count > 0) {
+            // up to gcfg->train Adam steps per replay; step s reads the
+            // records s, s + S*q, s + 2*S*q, ... (S steps, q = spacing), so
+            // the steps see disjoint, evenly spread subsets
             const uint32_t n_rec = (uint32_t)std::min<unsigned long long>(c.rec_count, w.train_capacity);
-            for (uint32_t s = 0; s < n_rec; s += batch) {
-                int r = illume_guide_train_step(guide, w.train_rec + 10 * (size_t)s, std::min(batch, n_rec - s), st);
+            const uint32_t rows = std::min(batch, n_rec);
+            const uint32_t steps = std::max(1u, std::min(gcfg->train, n_rec / rows));
+            const uint32_t q = n_rec / (rows * steps);
+            for (uint32_t s = 0; s < steps; ++s) {
+                int r = guide_train_strided(guide, w.train_rec + 10 * (size_t)s * q, rows, steps * q, st);
                 if (r != ILLUME_OK) return r;
             }
             e = wf_launch_clear_records(w, st);

@@ -101,6 +101,13 @@ class Guide:
                                                           _stream_handle(stream)), "illume_guide_grad")
         return g
 
+    def stats(self):
+        """(Adam steps taken, loss of the last step)."""
+        steps, loss = ctypes.c_uint64(), ctypes.c_float()
+        self._lib.check(self._lib.lib().illume_guide_stats(self._h, ctypes.byref(steps), ctypes.byref(loss)),
+                        "illume_guide_stats")
+        return int(steps.value), float(loss.value)
+
     def train_step(self, records, stream=None):
         """One Adam step on the batch of records (n, 10)."""
         from .accel import _stream_handle
@@ -123,6 +130,8 @@ def _declare(L):
     L.illume_guide_infer.argtypes = [vp, vp, u32, vp, vp]
     L.illume_guide_grad.restype = i32
     L.illume_guide_grad.argtypes = [vp, vp, u32, vp, vp]
+    L.illume_guide_stats.restype = i32
+    L.illume_guide_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)]
     L.illume_guide_train_step.restype = i32
     L.illume_guide_train_step.argtypes = [vp, vp, u32, vp]
     L._guide_declared = True

@@ -135,7 +135,7 @@ class PathIntegrator:
 
 class GuideCfg(ctypes.Structure):
     _fields_ = [("guide_frac", ctypes.c_float), ("train", ctypes.c_uint32), ("train_batch", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("min_alpha", ctypes.c_float)]
 
 
 class CameraC(ctypes.Structure):
@@ -239,7 +239,7 @@ class Wavefront:
         if film is None:
             film = torch.zeros(off, dtype=torch.float32, device="cuda")
         g = guide.handle if guide is not None else None
-        gc = gcfg if gcfg is not None else GuideCfg(0.0, 0, 0, 0)
+        gc = gcfg if gcfg is not None else GuideCfg(0.0, 0, 0, 0.0)
         rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
         _lib.check(_declare_wf().illume_wavefront_render_batch(self._h, arr, len(jobs), g, ctypes.byref(gc),
                                                                ctypes.c_void_p(film.data_ptr()), rc,
@@ -251,14 +251,19 @@ class WavefrontPathIntegrator(PathIntegrator):
     """The same Integrator::render as :class:`PathIntegrator`, run on a
     :class:`Wavefront` pool; optionally guided by a
     :class:`rs_illume_amd.guide.Guide` (one-sample MIS with probability
-    ``guide_frac``) and training it online from finished paths."""
+    ``guide_frac``) and training it online from finished paths (``train`` =
+    the most Adam steps per graph replay, True = 16, 0/False = frozen). The proposal is used
+    only at diffuse vertices and at microfacet vertices with alpha >=
+    ``min_alpha``."""
 
     def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
-                 train=False, train_batch=65536, wavefront=None):
+                 train=False, train_batch=65536, wavefront=None, min_alpha=0.0):
         super().__init__(camera, spp, max_depth, seed)
         self.scene = scene
         self.guide = guide
-        self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, int(bool(train)), int(train_batch), 0)
+        steps = 16 if train is True else int(train)
+        self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, steps, int(train_batch),
+                             float(min_alpha))
         self.wf = wavefront if wavefront is not None else Wavefront(max_paths, max_depth)
 
     def render_sum(self, scene=None, rgb_sum=None, spp=None, sample_offset=0, ray_counts=None, stream=None):

@@ -105,12 +105,14 @@ class SceneDesc:
         }
 
 
-def cornell_box(seed=0, glossy=False, light_size=0.25, light_L=17.0, wall_alpha_range=(0.05, 0.5)):
+def cornell_box(seed=0, glossy=False, light_size=0.25, light_L=17.0, wall_alpha_range=(0.05, 0.5), light_up=False):
     """Box [-1,1]^3 open at +z: floor, ceiling, back grey 0.75, left (x=-1)
     red, right (x=1) green; two spheres r=0.3 (albedo 0.75) with centres
     uniform in [-0.7,0.7]^3; a light_size^2 one-sided quad light just below
     the ceiling facing -y. glossy=True makes the five walls Trowbridge-Reitz
-    copper with alpha uniform in wall_alpha_range (one alpha per wall)."""
+    copper with alpha uniform in wall_alpha_range (one alpha per wall).
+    light_up=True turns the light over (at y = 0.9, facing +y): the room is
+    then lit only by the patch of ceiling above it (all light indirect)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     s = SceneDesc()
     grey = s.add_material(matte((0.75, 0.75, 0.75)))
@@ -129,9 +131,10 @@ def cornell_box(seed=0, glossy=False, light_size=0.25, light_L=17.0, wall_alpha_
     s.add_quad((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1), left_m, normal=(1, 0, 0))
     s.add_quad((1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1), right_m, normal=(-1, 0, 0))
     h = light_size / 2
-    y = 1 - 1e-4
+    y = 0.9 if light_up else 1 - 1e-4
     light_m = s.add_material(matte((0.0, 0.0, 0.0)))
-    s.add_quad((-h, y, -h), (h, y, -h), (h, y, h), (-h, y, h), light_m, emission=(light_L,) * 3, normal=(0, -1, 0))
+    s.add_quad((-h, y, -h), (h, y, -h), (h, y, h), (-h, y, h), light_m, emission=(light_L,) * 3,
+               normal=(0, 1, 0) if light_up else (0, -1, 0))
     for c in centres:
         s.add_sphere(c, 0.3, grey)
     return s
@@ -249,3 +252,98 @@ def procedural_scene(seed):
     ang = rng.uniform(0, 2 * np.pi)
     cam = camera((7 * np.cos(ang), rng.uniform(2, 4), 7 * np.sin(ang)), (0, 0.5, 0), (0, 1, 0), 45.0, side, side)
     return s, cam, spp
+
+
+# ---------------------------------------------------------------- terrain + spheres + sky (combined workload)
+def fbm_heightfield(n=1024, octaves=6, lacunarity=2.0, gain=0.5, base_cells=4, seed=0):
+    """fBm of value noise on an n x n grid over [0,1]^2: octave k has
+    base_cells * lacunarity^k lattice cells per side, amplitude gain^k, and
+    smoothstep-interpolated uniform lattice values in [-1, 1]. Returns float64
+    [n, n] normalised to [0, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.linspace(0.0, 1.0, n)
+    h = np.zeros((n, n))
+    amp, cells = 1.0, float(base_cells)
+    for _ in range(octaves):
+        m = int(np.ceil(cells)) + 1
+        lat = rng.uniform(-1.0, 1.0, (m + 1, m + 1))
+        g = x * cells
+        i = np.minimum(np.floor(g).astype(np.int64), m - 1)
+        f = g - i
+        s = f * f * (3 - 2 * f)
+        i0, i1 = i[:, None], i[None, :]
+        sy, sx = s[:, None], s[None, :]
+        v00, v01 = lat[i0, i1], lat[i0, i1 + 1]
+        v10, v11 = lat[i0 + 1, i1], lat[i0 + 1, i1 + 1]
+        h += amp * ((v00 * (1 - sx) + v01 * sx) * (1 - sy) + (v10 * (1 - sx) + v11 * sx) * sy)
+        amp *= gain
+        cells *= lacunarity
+    h -= h.min()
+    return h / max(h.max(), 1e-12)
+
+
+def terrain_scene(seed=0, n=1024, n_spheres=10000, height=0.35, sky=True):
+    """Heightfield terrain over [-1,1]^2 (xz), 2 (n-1)^2 triangles, fBm of 6
+    octaves (lacunarity 2, gain 0.5), with n_spheres spheres resting on it
+    (radius lognormal, median 0.02, sigma_log 0.5); Lambertian or
+    Trowbridge-Reitz conductor materials; lit by the analytic sky.
+    Returns SceneDesc (with .sky set)."""
+    rng = np.random.Generator(np.random.PCG64(5000 + seed))
+    hf = fbm_heightfield(n, seed=seed) * height
+    xs = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    verts = np.stack([X, hf, Z], -1).reshape(-1, 3).astype(np.float32)
+    r, c = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a = (r * n + c).reshape(-1)
+    b, d, e = a + 1, a + n, a + n + 1
+    # counter-clockwise seen from +y
+    idx = np.concatenate([np.stack([a, d, b], 1), np.stack([b, d, e], 1)]).astype(np.uint32)
+    s = SceneDesc()
+    mats = [s.add_material(matte((0.45, 0.42, 0.35))), s.add_material(matte((0.3, 0.45, 0.25))),
+            s.add_material(metal(0.3)), s.add_material(metal(0.1))]
+    s.verts, s.idx = verts, idx
+    # terrain: Lambertian, two albedos by height band
+    band = (hf.reshape(-1)[a] > 0.5 * height)
+    tri_m = np.concatenate([np.where(band, mats[0], mats[1]), np.where(band, mats[0], mats[1])])
+    s.tri_material = [int(m) for m in tri_m]
+    s.tri_emission = [(0, 0, 0)] * len(idx)
+    rad = np.exp(np.log(0.02) + 0.5 * rng.standard_normal(n_spheres))
+    px = rng.uniform(-0.95, 0.95, n_spheres)
+    pz = rng.uniform(-0.95, 0.95, n_spheres)
+    gi = np.clip(((pz + 1) / 2 * (n - 1)).round().astype(np.int64), 0, n - 1)
+    gj = np.clip(((px + 1) / 2 * (n - 1)).round().astype(np.int64), 0, n - 1)
+    py = hf[gi, gj] + rad
+    kind = rng.random(n_spheres)
+    sph_mats = []
+    for k in range(n_spheres):
+        if kind[k] < 0.5:
+            sph_mats.append(s.add_material(matte(tuple(rng.uniform(0.1, 0.85, 3)))))
+        else:
+            sph_mats.append(mats[2] if kind[k] < 0.75 else mats[3])
+    for k in range(n_spheres):
+        s.spheres.append((px[k], py[k], pz[k], rad[k]))
+        s.sphere_material.append(sph_mats[k])
+    s.sky = SKY_DEFAULT if sky else None
+    return s
+
+
+# zenith RGB, horizon RGB (radiance); the sky emits from the upper hemisphere (+y)
+SKY_DEFAULT = ((0.35, 0.55, 1.0), (1.1, 1.05, 0.95))
+
+
+def terrain_views(seed=0, n_views=16):
+    """n_views cameras around and above the terrain; pixel counts
+    log-uniform in [256^2, 1920*1080], aspect log-uniform in [1, 16/9]."""
+    rng = np.random.Generator(np.random.PCG64(7000 + seed))
+    cams = []
+    for _ in range(n_views):
+        npx = np.exp(rng.uniform(np.log(256 * 256), np.log(1920 * 1080)))
+        asp = np.exp(rng.uniform(0.0, np.log(16 / 9)))
+        w = int(round(np.sqrt(npx * asp)))
+        h = int(round(w / asp))
+        ang = rng.uniform(0, 2 * np.pi)
+        dist = rng.uniform(1.6, 2.6)
+        pos = (dist * np.cos(ang), rng.uniform(0.6, 1.4), dist * np.sin(ang))
+        look = (rng.uniform(-0.3, 0.3), 0.1, rng.uniform(-0.3, 0.3))
+        cams.append(camera(pos, look, (0, 1, 0), 45.0, w, h))
+    return cams

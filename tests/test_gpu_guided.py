@@ -98,27 +98,31 @@ def test_guided_render_is_unbiased(gpu):
 
 
 def test_training_gains_over_plain_at_equal_spp(gpu):
-    import torch
+    """What the trained proposal buys: a room lit only indirectly (the light
+    faces the ceiling), equal spp, relMSE against an 8192-spp plain render.
+    The proposal is trained online by the renderer itself (16 Adam steps per
+    graph replay, batch 16384), then frozen for the measured render."""
     from rs_illume_amd.guide import Guide
     from rs_illume_amd.render import Scene, WavefrontPathIntegrator
     _say("case guided gain")
-    sc = Scene(_glossy())
+    sc = Scene(S.cornell_box(0, light_up=True, light_L=60.0))
     cam = S.cornell_camera(64, 64)
     ref = WavefrontPathIntegrator(sc, cam, spp=8192, max_depth=8, seed=100, max_paths=1 << 22).render()
     ref = ref.cpu().numpy()
     spp = 64
     plain = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=7).render().cpu().numpy()
     g = Guide(seed=1, lr=1e-3)
-    # warm-up training passes (samples discarded), then a guided render at equal spp
-    trainer = WavefrontPathIntegrator(sc, cam, spp=16, max_depth=8, seed=50, max_paths=64 * 64 * 4, guide=g,
-                                      guide_frac=0.5, train=True, train_batch=16384)
-    for k in range(8):
+    # training passes (samples discarded), then a guided render at equal spp
+    trainer = WavefrontPathIntegrator(sc, cam, spp=16, max_depth=8, seed=50, max_paths=1 << 16, guide=g,
+                                      guide_frac=0.5, train=16, train_batch=16384)
+    for k in range(64):
         trainer.render_sum(sample_offset=16 * k)
-    torch.cuda.synchronize()
+    steps, loss = g.stats()
     guided = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=7, guide=g, guide_frac=0.5).render()
     guided = guided.cpu().numpy()
     e_p, e_g = S.rel_mse(plain, ref), S.rel_mse(guided, ref)
-    _say("relMSE plain", e_p, "guided (trained)", e_g, "ratio", e_g / e_p)
+    _say("adam steps", steps, "last loss", loss, "relMSE plain", e_p, "guided (trained)", e_g, "ratio", e_g / e_p)
+    assert steps > 100
     assert np.isfinite(guided).all()
     assert abs(guided.mean() - ref.mean()) < 0.03 * ref.mean()
-    assert e_g < e_p
+    assert e_g < 0.5 * e_p

@@ -27,6 +27,23 @@ def test_param_count(oracle):
     assert oracle._guide_decl().orc_guide_num_params() == 20200
 
 
+def test_encoding_is_sin_cos_of_power_of_two_octaves(oracle):
+    """Feature 24 i + 2 j = sin(2^j pi v_i), + 1 = cos(2^j pi v_i), with v the
+    position normalised by the bounds and the raw normal: checked against the
+    f64 closed form (the oracle reduces the angle exactly, so the only error
+    is the final rounding to f32)."""
+    x, nrm, _, _ = records(500, seed=3)
+    f = oracle.guide_encode(x, nrm, BOUNDS)
+    # the normalised position as the oracle rounds it (f32), then f64 angles
+    vp = (x - np.float32(-1)) / np.float32(2)
+    v = np.concatenate([vp.astype(np.float64), nrm.astype(np.float64)], 1)
+    for i in range(6):
+        for j in range(12):
+            ang = (2.0 ** j) * np.pi * v[:, i]
+            assert np.abs(f[:, 24 * i + 2 * j] - np.sin(ang)).max() < 6e-8
+            assert np.abs(f[:, 24 * i + 2 * j + 1] - np.cos(ang)).max() < 6e-8
+
+
 def test_gradient_matches_finite_differences(oracle):
     p = init_params(3, out_scale=1.0)
     x, nrm, wi, w = records(16, 2)
