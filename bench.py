@@ -16,7 +16,9 @@ Also measured (keys of the same JSON line):
              relMSE <= 0.01 against a 16384-spp plain render, plain GPU path
              tracer vs the guided one (online training included in its time);
   mlp      - the proposal network's MFMA kernels: inference and training
-             throughput as a fraction of the bf16 dense peak.
+             throughput as a fraction of the bf16 dense peak;
+  batched  - 64 procedural scenes of uneven size in one wavefront render_batch
+             call with the autotuned launch shape.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-render]
 """
@@ -156,6 +158,64 @@ def bench_guided(ref_spp=16384, res=512, log=log):
     return out
 
 
+def bench_batched(no_cpu, log=log):
+    """Config 2: 64 procedural scenes (seeds 1-64) through one wavefront pool
+    in one render_batch call; launch shape chosen by the autotuner on a 2-spp
+    probe of the same batch. The oracle renders scene 1 at 1 spp on host cores."""
+    import torch
+    from rs_illume_amd import autotune
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.render import Scene
+    t0 = time.time()
+    descs = [S.procedural_scene(seed) for seed in range(1, 65)]
+    scenes = [Scene(d) for d, _, _ in descs]
+    setup_s = time.time() - t0
+    jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
+    n_tris = sum(len(d.idx) for d, _, _ in descs)
+    n_px = sum(int(c["width"][0]) * int(c["height"][0]) for _, c, _ in descs)
+    n_paths = sum(int(c["width"][0]) * int(c["height"][0]) * spp for _, c, spp in descs)
+    log("batched: %d scenes, %.1fM tris, %.1fM px, %.0fM paths, setup %.1f s" % (len(jobs), n_tris / 1e6, n_px / 1e6,
+                                                                               n_paths / 1e6, setup_s))
+    t0 = time.time()
+    best, table = autotune.tune(jobs, max_depth=8, sample_spp=2)
+    tune_s = time.time() - t0
+    log("autotune %.1f s: %s" % (tune_s, best))
+    wf = autotune.make_wavefront(best, max_depth=8)
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    wf.render_batch(jobs[:1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    film, _ = wf.render_batch(jobs, ray_counts=counts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rays = counts.cpu().numpy().astype(np.float64)
+    out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
+                       "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one render_batch call",
+           "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths,
+           "seconds": round(dt, 4), "samples_per_s": round(n_paths / dt, 1),
+           "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1), "iterations": wf.last_iterations(),
+           "autotune": {"best": {k: best[k] for k in ("max_paths", "block", "regen_frac", "graph_iters")},
+                        "candidates": len(table), "seconds": round(tune_s, 1)},
+           "film_finite": bool(torch.isfinite(film).all())}
+    wf.close()
+    if not no_cpu:
+        pyoracle = _oracle()
+        d, cam, _ = descs[0]
+        osc = pyoracle.OracleRenderScene(d.arrays())
+        threads = cpu_threads()
+        t0 = time.time()
+        _, st = osc.render_sum(cam, 1, 8, seed=1, n_threads=threads)
+        dtc = time.time() - t0
+        px = int(cam["width"][0]) * int(cam["height"][0])
+        out["cpu_baseline"] = {"samples_per_s": round(px / dtc, 1), "mrays_per_s": round(float(st.sum()) / dtc / 1e6, 3),
+                               "cores": threads, "kind": "port",
+                               "sample": "scene 1 (%d tris, %dx%d) at 1 spp; not the whole batch"
+                                         % (len(d.idx), int(cam["width"][0]), int(cam["height"][0]))}
+    for sc in scenes:
+        sc.close()
+    return out
+
+
 def bench_mlp():
     """MFMA kernels of the proposal: inference over 2^20 samples and one Adam
     step over 65,536 records, timed with HIP events on the launch stream."""
@@ -275,6 +335,8 @@ def main():
         extra["mlp"] = bench_mlp()
         log("guided vs plain")
         extra["guided"] = bench_guided()
+        log("batched")
+        extra["batched"] = bench_batched(args.no_cpu)
 
     out = {
         "metric": "Mrays/s closest-hit; time-to-relMSE 0.01, guided vs plain GPU path tracer",

@@ -81,3 +81,33 @@ def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=409
     if rp["reached"] and rg["reached"]:
         out["time_ratio"] = rg["reached"]["seconds"] / rp["reached"]["seconds"]
     return out
+
+
+def _train_passes(res=512, passes=3, spp=16, train=4, batch=65536):
+    """Guided training passes on the glossy small-light Cornell box (the
+    workload profiled for the proposal kernels)."""
+    from . import scenes as S
+    from .guide import Guide
+    from .render import Scene, Wavefront, WavefrontPathIntegrator
+    sc = Scene(S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625))
+    cam = S.cornell_camera(res, res)
+    g = Guide(seed=1)
+    wf = Wavefront(max_paths=1 << 21, max_depth=8)
+    gpi = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=0, guide=g, guide_frac=0.5, train=train,
+                                  train_batch=batch, wavefront=wf)
+    for k in range(passes):
+        gpi.render_sum(sample_offset=spp * k)
+    torch.cuda.synchronize()
+    print("adam steps, last loss:", g.stats(), flush=True)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="guided training passes (profiling workload)")
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--passes", type=int, default=3)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--train", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=65536)
+    a = ap.parse_args()
+    _train_passes(a.res, a.passes, a.spp, a.train, a.batch)
