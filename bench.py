@@ -18,7 +18,9 @@ Also measured (keys of the same JSON line):
   mlp      - the proposal network's MFMA kernels: inference and training
              throughput as a fraction of the bf16 dense peak;
   batched  - 64 procedural scenes of uneven size in one wavefront render_batch
-             call with the autotuned launch shape.
+             call with the autotuned launch shape;
+  combined - terrain + spheres under the analytic sky, 16 views batched,
+             plain vs guided time-to-relMSE per view and samples/s.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-render]
 """
@@ -216,6 +218,82 @@ def bench_batched(no_cpu, log=log):
     return out
 
 
+def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log):
+    """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
+    sky, 16 views of uneven size in one render_batch per pass, max depth 10.
+    Plain and guided (one proposal network for the scene, trained online;
+    the radiance field it learns is view-independent) run progressive passes
+    of all views; a view's time-to-relMSE is the batch time at which its image
+    first reaches the target against a ref_spp plain reference."""
+    import torch
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.guide import Guide
+    from rs_illume_amd.render import GuideCfg, Scene, Wavefront
+    t0 = time.time()
+    desc = S.terrain_scene(0)
+    sc = Scene(desc)
+    cams = S.terrain_views(0)
+    setup_s = time.time() - t0
+    depth = 10
+    sizes = [int(c["width"][0]) * int(c["height"][0]) for c in cams]
+    n_px = sum(sizes)
+    wf = Wavefront(max_paths=1 << 22, max_depth=depth, block=64, graph_iters=4)
+
+    def jobs(spp, off, seed):
+        return [(sc, c, spp, depth, seed + v, off) for v, c in enumerate(cams)]
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = torch.zeros(3 * n_px, dtype=torch.float32, device="cuda")
+    for k in range(ref_spp // 32):
+        wf.render_batch(jobs(32, 32 * k, 5000), film=ref)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    ref /= ref_spp
+    offs = np.concatenate([[0], np.cumsum(sizes)]) * 3
+    log("combined: %.1fM tris, %d spheres, %d views, %.2f Mpx; reference %d spp in %.1f s"
+        % (len(desc.idx) / 1e6, len(desc.spheres), len(cams), n_px / 1e6, ref_spp, ref_s))
+
+    def run(guide):
+        film = torch.zeros_like(ref)
+        gc = GuideCfg(0.5, 16, 65536, 0.0) if guide is not None else None
+        reached = [None] * len(cams)
+        elapsed, spp = 0.0, 0
+        wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99) if guide is not None else None, gcfg=gc)
+        while spp < max_spp and any(r is None for r in reached):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            wf.render_batch(jobs(spp_pass, spp, 100 if guide is None else 200), film=film, guide=guide, gcfg=gc)
+            torch.cuda.synchronize()
+            elapsed += time.perf_counter() - t1
+            spp += spp_pass
+            for v in range(len(cams)):
+                if reached[v] is None:
+                    a, b = int(offs[v]), int(offs[v + 1])
+                    from rs_illume_amd.converge import rel_mse_dev
+                    if rel_mse_dev(film[a:b] / spp, ref[a:b]) <= target:
+                        reached[v] = (round(elapsed, 4), spp)
+        return {"seconds": round(elapsed, 4), "spp": spp, "samples_per_s": round(n_px * spp / elapsed, 1),
+                "views_reached": sum(r is not None for r in reached),
+                "time_to_target_per_view": [r[0] if r else None for r in reached],
+                "spp_to_target_per_view": [r[1] if r else None for r in reached]}
+
+    plain = run(None)
+    guided = run(Guide(seed=7))
+    wf.close()
+    sc.close()
+    both = [(p, g) for p, g in zip(plain["time_to_target_per_view"], guided["time_to_target_per_view"])
+            if p is not None and g is not None]
+    return {"workload": "fBm terrain 1024^2 (2.09M tris, 6 octaves, lacunarity 2, gain 0.5) + 10k spheres "
+                        "(radius lognormal, median 0.02), Lambertian/microfacet, analytic sky; 16 views "
+                        "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
+                        % (n_px / 1e6, spp_pass),
+            "setup_s": round(setup_s, 1), "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
+            "target_rel_mse": target, "plain": plain, "guided": guided,
+            "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both
+            else None}
+
+
 def bench_mlp():
     """MFMA kernels of the proposal: inference over 2^20 samples and one Adam
     step over 65,536 records, timed with HIP events on the launch stream."""
@@ -262,6 +340,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing measurements")
+    ap.add_argument("--only", choices=("cornell", "mlp", "guided", "batched", "combined"),
+                    help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
     ap.add_argument("--width", type=int, default=2, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
@@ -273,6 +353,12 @@ def main():
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, rays_to_device
+
+    if args.only:
+        fn = {"cornell": lambda: bench_cornell(args.no_cpu), "mlp": bench_mlp, "guided": bench_guided,
+              "batched": lambda: bench_batched(args.no_cpu), "combined": bench_combined}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
 
     log("generating workload")
     v, idx = W.triangle_soup(N_TRIS, seed=1)
@@ -337,6 +423,8 @@ def main():
         extra["guided"] = bench_guided()
         log("batched")
         extra["batched"] = bench_batched(args.no_cpu)
+        log("combined")
+        extra["combined"] = bench_combined()
 
     out = {
         "metric": "Mrays/s closest-hit; time-to-relMSE 0.01, guided vs plain GPU path tracer",

@@ -164,6 +164,24 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
                         const float *prim_emission, const illume_material *materials, uint32_t n_materials,
                         illume_scene **out);
 
+/* Analytic sky: an infinite area light (the role of pbrt-v3's
+ * InfiniteAreaLight, which the reference lacks) with radiance
+ * L(w) = zenith * t + horizon * (1 - t), t = sqrt(w.y), above the horizon
+ * (w.y > 0) and 0 below; sampled cosine-weighted about +y. In the power
+ * distribution its power is pi r^2 times its sphere-averaged radiance, r the
+ * radius of the scene's bounding sphere. Escaping camera and specular rays
+ * see it. */
+typedef struct illume_sky {
+    float zenith[3];
+    float horizon[3];
+} illume_sky;
+
+/* illume_scene_create plus an optional analytic sky (sky == NULL: none). */
+int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                           const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
+                           const float *prim_emission, const illume_material *materials, uint32_t n_materials,
+                           const illume_sky *sky, illume_scene **out);
+
 int illume_scene_destroy(illume_scene *scene);
 
 /* The scene's aggregate, for Scene::intersect / intersect_p queries through

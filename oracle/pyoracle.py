@@ -188,7 +188,7 @@ def _render_decl():
         return L
     vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     L.orc_render_scene_create.restype = vp
-    L.orc_render_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32]
+    L.orc_render_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, vp]
     L.orc_render_scene_destroy.argtypes = [vp]
     L.orc_render.argtypes = [vp, vp, u32, u32, u64, u32, vp, vp, i32]
     L.orc_camera_rays.argtypes = [vp, vp, u32, vp]
@@ -202,11 +202,13 @@ class OracleRenderScene:
     rs_illume_amd.scenes.SceneDesc.arrays()."""
 
     def __init__(self, a):
-        self.a = {k: np.ascontiguousarray(v) for k, v in a.items()}
+        self.a = {k: np.ascontiguousarray(v) for k, v in a.items() if v is not None}
         a = self.a
+        sky = a.get("sky")
         self.h = _render_decl().orc_render_scene_create(
             _p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]), _p(a["spheres"]), len(a["spheres"]),
-            _p(a["prim_material"]), _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]))
+            _p(a["prim_material"]), _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]),
+            _p(np.ascontiguousarray(sky, np.float32)) if sky is not None else None)
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -430,6 +430,23 @@ static float pdf_li(const AreaLight &l, const SurfaceInteraction &ref, const V3 
     return pdf;
 }
 
+RGB Sky::L(const V3 &w_in) const {
+    const V3 w = normalize(w_in);
+    if (!(w.y > 0)) return RGB();
+    const float t = std::sqrt(w.y);
+    return zenith * t + horizon * (1 - t);
+}
+
+// Sky sample: cosine-weighted direction about +y (the local frame's z is the
+// world's y); pdf = w.y / pi.
+static RGB sky_sample_li(const Sky &sky, const float *u, V3 *wi, float *pdf) {
+    const V3 d = cosine_sample_hemisphere(u);
+    *wi = V3(d.x, d.z, d.y);
+    *pdf = wi->y > 0 ? wi->y * InvPi : 0.f;
+    return *pdf > 0 ? sky.L(*wi) : RGB();
+}
+static float sky_pdf_li(const V3 &wi) { return wi.y > 0 ? wi.y * InvPi : 0.f; }
+
 // Scene::intersect with the surface interaction of the closest hit.
 static bool scene_intersect(const RenderScene &sc, Ray &ray, SurfaceInteraction *si, RenderStats *st) {
     if (st) st->closest_rays++;
@@ -443,8 +460,12 @@ static bool scene_intersect_p(const RenderScene &sc, const Ray &ray, RenderStats
 
 // integrator.rs:353-494 estimate_direct with pbrt-v3's BSDF-sampled MIS term
 // (divergence 34).
+static RGB estimate_direct_sky(const RenderScene &sc, const SurfaceInteraction &it, const BSDF &bsdf,
+                               const float *u_scat, const float *u_light, RenderStats *st);
+
 static RGB estimate_direct(const RenderScene &sc, const SurfaceInteraction &it, const BSDF &bsdf, const float *u_scat,
                            int light_idx, const float *u_light, RenderStats *st) {
+    if (light_idx == (int)sc.lights.size()) return estimate_direct_sky(sc, it, bsdf, u_scat, u_light, st);
     const AreaLight &light = sc.lights[light_idx];
     int bsdf_flags = BSDF_ALL & ~BSDF_SPECULAR;
     RGB ld;
@@ -486,10 +507,54 @@ static RGB estimate_direct(const RenderScene &sc, const SurfaceInteraction &it, 
     return ld;
 }
 
+// estimate_direct for the sky: the light sample's shadow ray leaves the scene
+// (any hit over (0, inf)); the BSDF sample counts only if its ray escapes.
+static RGB estimate_direct_sky(const RenderScene &sc, const SurfaceInteraction &it, const BSDF &bsdf,
+                               const float *u_scat, const float *u_light, RenderStats *st) {
+    int bsdf_flags = BSDF_ALL & ~BSDF_SPECULAR;
+    RGB ld;
+    V3 wi;
+    float light_pdf = 0, scattering_pdf = 0;
+    RGB li = sky_sample_li(sc.sky, u_light, &wi, &light_pdf);
+    if (light_pdf > 0 && !is_black(li)) {
+        RGB f = bsdf.f(it.wo, wi, bsdf_flags) * abs_dot(wi, it.shading_n);
+        scattering_pdf = bsdf.pdf(it.wo, wi, bsdf_flags);
+        if (!is_black(f)) {
+            if (scene_intersect_p(sc, it.spawn_ray(wi), st)) li = RGB();
+            if (!is_black(li)) {
+                float weight = power_heuristic(1, light_pdf, 1, scattering_pdf);
+                ld += cmul(f, li) * weight / light_pdf;
+            }
+        }
+    }
+    {
+        int sampled = 0;
+        RGB f = bsdf.sample_f(it.wo, &wi, u_scat, &scattering_pdf, bsdf_flags, &sampled);
+        f *= abs_dot(wi, it.shading_n);
+        bool sampled_specular = (sampled & BSDF_SPECULAR) != 0;
+        if (!is_black(f) && scattering_pdf > 0) {
+            float weight = 1;
+            if (!sampled_specular) {
+                light_pdf = sky_pdf_li(wi);
+                if (light_pdf == 0) return ld;
+                weight = power_heuristic(1, scattering_pdf, 1, light_pdf);
+            }
+            SurfaceInteraction lisect;
+            Ray ray = it.spawn_ray(wi);
+            bool found = scene_intersect(sc, ray, &lisect, st);
+            if (!found) {
+                RGB li2 = sc.sky.L(wi);
+                if (!is_black(li2)) ld += cmul(f, li2) * weight / scattering_pdf;
+            }
+        }
+    }
+    return ld;
+}
+
 // integrator.rs:321-350 uniform_sample_one_light with the power distribution.
 static RGB sample_one_light(const RenderScene &sc, const SurfaceInteraction &it, const BSDF &bsdf, Sampler &sampler,
                             RenderStats *st) {
-    int n = (int)sc.lights.size();
+    int n = sc.n_lights();
     if (n == 0) return RGB();
     float light_pdf;
     int light_num = sc.light_distrib.sample_discrete(sampler.get_1d(), &light_pdf);
@@ -513,6 +578,8 @@ RGB path_li(const RenderScene &sc, Ray ray, Sampler &sampler, int max_depth, Ren
             if (found) {
                 int li = sc.prim_light[isect.prim];
                 if (li >= 0) L += cmul(beta, area_L(sc.lights[li], isect.n, -ray.d));
+            } else if (sc.sky.on) {
+                L += cmul(beta, sc.sky.L(ray.d));  // InfiniteAreaLight::Le of the escaped ray
             }
         }
         if (!found || bounces >= max_depth) break;

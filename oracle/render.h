@@ -34,6 +34,16 @@ struct AreaLight {
     float area;
 };
 
+// Analytic sky (an infinite area light; pbrt-v3 InfiniteAreaLight with an
+// analytic radiance instead of an image map): L(w) = zenith * t +
+// horizon * (1 - t), t = sqrt(w.y), for w.y > 0 and 0 below the horizon.
+// Sampled cosine-weighted about +y (pdf w.y / pi).
+struct Sky {
+    bool on = false;
+    RGB zenith, horizon;
+    RGB L(const V3 &w) const;
+};
+
 // sampling.rs:8-93 Distribution1D (discrete part)
 struct Distribution1D {
     std::vector<float> func, cdf;
@@ -49,7 +59,9 @@ struct RenderScene {
     std::vector<int> prim_light;  // light index or -1
     std::vector<Material> materials;
     std::vector<AreaLight> lights;
+    Sky sky;                       // light index lights.size() in the distribution when on
     Distribution1D light_distrib;  // power distribution (integrator.rs:496-505)
+    int n_lights() const { return (int)lights.size() + (sky.on ? 1 : 0); }
 };
 
 // PerspectiveCamera (cameras/perspective.rs:35-126) with pbrt-v3's

@@ -25,9 +25,11 @@ extern "C" {
 
 // prim_emission: 3 floats per primitive (zero = not emissive; only triangles
 // may emit). Every emissive triangle becomes one one-sided DiffuseAreaLight.
+// sky: zenith RGB, horizon RGB (6 floats) or null for no sky.
 void *orc_render_scene_create(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
                               const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
-                              const float *prim_emission, const void *materials, uint32_t n_materials) {
+                              const float *prim_emission, const void *materials, uint32_t n_materials,
+                              const float *sky) {
     RenderScene *s = new RenderScene();
     s->ps.verts.resize(n_verts);
     for (uint32_t i = 0; i < n_verts; ++i) s->ps.verts[i] = V3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
@@ -65,6 +67,30 @@ void *orc_render_scene_create(const float *verts, uint32_t n_verts, const uint32
         s->lights.push_back(l);
         // DiffuseAreaLight::power = L * area * pi (one-sided), by luminance
         power.push_back(luminance(L * l.area * Pi));
+    }
+    if (sky) {
+        // InfiniteAreaLight::power = pi r^2 * (average radiance over the
+        // sphere), r the radius of the scene's bounding sphere; the analytic
+        // sky's average is (2/3 zenith + 1/3 horizon) / 2.
+        s->sky.on = true;
+        s->sky.zenith = RGB(sky[0], sky[1], sky[2]);
+        s->sky.horizon = RGB(sky[3], sky[4], sky[5]);
+        // bounds of the triangles' vertices and the spheres' centre +- radius
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t t = 0; t < n_tris; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const float *v = verts + 3 * (size_t)idx[3 * (size_t)t + k];
+                for (int a = 0; a < 3; ++a) b[a] = std::min(b[a], v[a]), b[3 + a] = std::max(b[3 + a], v[a]);
+            }
+        for (uint32_t i = 0; i < n_spheres; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b[a] = std::min(b[a], spheres[4 * i + a] - spheres[4 * i + 3]);
+                b[3 + a] = std::max(b[3 + a], spheres[4 * i + a] + spheres[4 * i + 3]);
+            }
+        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        const float r = np > 0 ? 0.5f * std::sqrt(dx * dx + dy * dy + dz * dz) : 0.f;
+        const RGB avg = (s->sky.zenith * (2.f / 3.f) + s->sky.horizon * (1.f / 3.f)) * 0.5f;
+        power.push_back(luminance(avg * (Pi * r * r)));
     }
     if (!power.empty()) s->light_distrib.init(power);
     return s;

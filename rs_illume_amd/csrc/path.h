@@ -65,11 +65,81 @@ __device__ __forceinline__ int light_sample_discrete(const DevScene &sc, float u
     return offset;
 }
 
+// Analytic sky (an infinite area light with an analytic radiance):
+// L(w) = zenith * t + horizon * (1 - t), t = sqrt(w.y), for w.y > 0 and 0
+// below the horizon; sampled cosine-weighted about +y (pdf w.y / pi).
+__device__ __forceinline__ f3 sky_L(const DevScene &sc, f3 w_in) {
+    const f3 w = normalize(w_in);
+    if (!(w.y > 0.f)) return mk(0.f, 0.f, 0.f);
+    const float t = sqrtf(w.y);
+    return mk(sc.sky_zenith[0] * t + sc.sky_horizon[0] * (1 - t), sc.sky_zenith[1] * t + sc.sky_horizon[1] * (1 - t),
+              sc.sky_zenith[2] * t + sc.sky_horizon[2] * (1 - t));
+}
+__device__ __forceinline__ float sky_pdf(f3 wi) { return wi.y > 0.f ? wi.y * kInvPi : 0.f; }
+
+// estimate_direct for the sky: the light sample's shadow ray leaves the scene
+// (any hit over (0, inf)); the BSDF sample counts only if its ray escapes.
+template <int BLOCK>
+__device__ f3 estimate_direct_sky(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1,
+                                  float ul0, float ul1, TraceStack<BLOCK> &st, PathCounters &pc) {
+    const int flags = kAll & ~kSpecular;
+    f3 ld = mk(0.f, 0.f, 0.f);
+    const f3 dl = cosine_sample_hemisphere(ul0, ul1);
+    f3 wi = mk(dl.x, dl.z, dl.y);
+    float light_pdf = sky_pdf(wi);
+    float scattering_pdf = 0.f;
+    f3 li_v = light_pdf > 0.f ? sky_L(sc, wi) : mk(0.f, 0.f, 0.f);
+    if (light_pdf > 0 && !is_black(li_v)) {
+        const f3 fv = bsdf.f(it.wo, wi, flags);
+        const float ad = abs_dot(wi, it.shading_n);
+        f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
+        scattering_pdf = bsdf.pdf(it.wo, wi, flags);
+        if (!is_black(f)) {
+            f3 so;
+            spawn_ray(it, wi, so);
+            if (scene_any<BLOCK>(sc, so, wi, __builtin_inff(), st, pc)) li_v = mk(0.f, 0.f, 0.f);
+            if (!is_black(li_v)) {
+                float weight = power_heuristic(light_pdf, scattering_pdf);
+                f3 c = cmul(f, li_v);
+                ld = add(ld, mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf));
+            }
+        }
+    }
+    {
+        int sampled = 0;
+        f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
+        float ad = abs_dot(wi, it.shading_n);
+        f = mk(f.x * ad, f.y * ad, f.z * ad);
+        bool sampled_specular = (sampled & kSpecular) != 0;
+        if (!is_black(f) && scattering_pdf > 0) {
+            float weight = 1;
+            if (!sampled_specular) {
+                light_pdf = sky_pdf(wi);
+                if (light_pdf == 0) return ld;
+                weight = power_heuristic(scattering_pdf, light_pdf);
+            }
+            f3 so;
+            spawn_ray(it, wi, so);
+            TraceHit h;
+            if (!scene_closest<BLOCK>(sc, so, wi, __builtin_inff(), h, st, pc)) {
+                const f3 li2 = sky_L(sc, wi);
+                if (!is_black(li2)) {
+                    f3 c = cmul(f, li2);
+                    ld = add(ld, mk(c.x * weight / scattering_pdf, c.y * weight / scattering_pdf,
+                                    c.z * weight / scattering_pdf));
+                }
+            }
+        }
+    }
+    return ld;
+}
+
 // estimate_direct (integrator.rs:353-494) for a one-sided triangle area light,
 // with both MIS strategies (the BSDF-sampled term per pbrt-v3).
 template <int BLOCK>
 __device__ f3 estimate_direct(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1, int li,
                               float ul0, float ul1, TraceStack<BLOCK> &st, PathCounters &pc) {
+    if (li == sc.sky_index) return estimate_direct_sky<BLOCK>(sc, it, bsdf, us0, us1, ul0, ul1, st, pc);
     const DevLight &light = sc.lights[li];
     const int flags = kAll & ~kSpecular;
     f3 ld = mk(0.f, 0.f, 0.f);
@@ -330,6 +400,8 @@ __device__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps
         if (found) {
             int li = sc.prim_light[h.prim];
             if (li >= 0) ps.L = add(ps.L, cmul(ps.beta, area_L(sc.lights[li], it.n, neg(ps.d))));
+        } else if (sc.sky_index >= 0) {
+            ps.L = add(ps.L, cmul(ps.beta, sky_L(sc, ps.d)));  // Le of the escaped ray
         }
     }
     if (!found || ps.bounces >= max_depth) {

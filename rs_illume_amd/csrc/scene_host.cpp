@@ -215,7 +215,20 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
                         const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
                         const float *prim_emission, const illume_material *materials, uint32_t n_materials,
                         illume_scene **out) {
+    return illume_scene_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, prim_material, prim_emission,
+                                  materials, n_materials, nullptr, out);
+}
+
+int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                           const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
+                           const float *prim_emission, const illume_material *materials, uint32_t n_materials,
+                           const illume_sky *sky, illume_scene **out) {
     if (!out) return abi_fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    if (sky)
+        for (int c = 0; c < 3; ++c)
+            if (!(sky->zenith[c] >= 0.f && sky->horizon[c] >= 0.f && std::isfinite(sky->zenith[c]) &&
+                  std::isfinite(sky->horizon[c])))
+                return abi_fail(ILLUME_ERR_INVALID_ARG, "sky radiance must be finite and >= 0");
     *out = nullptr;
     uint64_t np = (uint64_t)n_tris + n_spheres;
     if (np && (!prim_material || !prim_emission)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null per-primitive array");
@@ -282,6 +295,31 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
         float r = e[0] * l.area * pi, g = e[1] * l.area * pi, bb = e[2] * l.area * pi;
         power.push_back(0.212671f * r + 0.715160f * g + 0.072169f * bb);
     }
+    int32_t sky_index = -1;
+    if (sky) {
+        // the sky's power: pi r^2 * sphere-averaged radiance ((2/3 zenith +
+        // 1/3 horizon) / 2), r the radius of the scene's bounding sphere
+        // bounds of the triangles' vertices and the spheres' centre +- radius
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t t = 0; t < n_tris; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const float *v = verts + 3 * (size_t)tri_idx[3 * (size_t)t + k];
+                for (int a = 0; a < 3; ++a) b[a] = std::min(b[a], v[a]), b[3 + a] = std::max(b[3 + a], v[a]);
+            }
+        for (uint32_t i = 0; i < n_spheres; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b[a] = std::min(b[a], spheres[4 * i + a] - spheres[4 * i + 3]);
+                b[3 + a] = std::max(b[3 + a], spheres[4 * i + a] + spheres[4 * i + 3]);
+            }
+        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        const float r = np > 0 ? 0.5f * std::sqrt(dx * dx + dy * dy + dz * dz) : 0.f;
+        const float pi = 3.14159265358979323846f;
+        float avg[3];
+        for (int c = 0; c < 3; ++c) avg[c] = (sky->zenith[c] * (2.f / 3.f) + sky->horizon[c] * (1.f / 3.f)) * 0.5f;
+        sky_index = (int32_t)lights.size();
+        power.push_back(0.212671f * (avg[0] * (pi * r * r)) + 0.715160f * (avg[1] * (pi * r * r)) +
+                        0.072169f * (avg[2] * (pi * r * r)));
+    }
     // Distribution1D (sampling.rs:20-42)
     std::vector<float> cdf(power.size() + 1, 0.f);
     float func_int = 0.f;
@@ -322,8 +360,14 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
     s->dev.light_func = d_f;
     s->dev.light_cdf = d_c;
     s->dev.light_func_int = func_int;
-    s->dev.n_lights = (uint32_t)lights.size();
+    s->dev.n_lights = (uint32_t)power.size();
     s->dev.n_tris = n_tris;
+    s->dev.sky_index = sky_index;
+    if (sky)
+        for (int c = 0; c < 3; ++c) {
+            s->dev.sky_zenith[c] = sky->zenith[c];
+            s->dev.sky_horizon[c] = sky->horizon[c];
+        }
     e = hipMalloc(&s->d_dev, sizeof(DevScene));
     if (e == hipSuccess) {
         s->allocs.push_back(s->d_dev);

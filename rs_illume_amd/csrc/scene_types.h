@@ -35,8 +35,10 @@ struct DevScene {
     const float *light_func;    // power distribution (sampling.rs:8-93)
     const float *light_cdf;     // n_lights + 1
     float light_func_int;
-    uint32_t n_lights;
+    uint32_t n_lights;          // area lights + the sky (if any)
     uint32_t n_tris;
+    int32_t sky_index;          // light index of the analytic sky, -1 if none
+    float sky_zenith[3], sky_horizon[3];
 };
 
 struct DevCamera {

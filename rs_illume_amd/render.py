@@ -28,6 +28,8 @@ def _declare():
     vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
     L.illume_scene_create.restype = i32
     L.illume_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, ctypes.POINTER(vp)]
+    L.illume_scene_create_ex.restype = i32
+    L.illume_scene_create_ex.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, vp, ctypes.POINTER(vp)]
     L.illume_scene_destroy.restype = i32
     L.illume_scene_destroy.argtypes = [vp]
     L.illume_scene_bvh.restype = i32
@@ -43,7 +45,8 @@ def _p(a):
 
 
 class Scene:
-    """Scene::new(aggregate, lights) over a :class:`scenes.SceneDesc`."""
+    """Scene::new(aggregate, lights) over a :class:`scenes.SceneDesc` (with its
+    analytic sky, if it has one)."""
 
     def __init__(self, desc):
         if not torch.cuda.is_available():
@@ -53,10 +56,13 @@ class Scene:
         L = _declare()
         torch.cuda.init()
         h = ctypes.c_void_p()
-        _lib.check(L.illume_scene_create(_p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]),
-                                         _p(a["spheres"]), len(a["spheres"]), _p(a["prim_material"]),
-                                         _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]),
-                                         ctypes.byref(h)), "illume_scene_create")
+        sky = a.get("sky")
+        sky_p = _p(np.ascontiguousarray(sky, np.float32)) if sky is not None else None
+        self._sky = sky
+        _lib.check(L.illume_scene_create_ex(_p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]),
+                                            _p(a["spheres"]), len(a["spheres"]), _p(a["prim_material"]),
+                                            _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]),
+                                            sky_p, ctypes.byref(h)), "illume_scene_create_ex")
         self._h = h
 
     @property

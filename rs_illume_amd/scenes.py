@@ -62,6 +62,7 @@ class SceneDesc:
     tri_emission: list = field(default_factory=list)
     sphere_material: list = field(default_factory=list)
     materials: list = field(default_factory=list)
+    sky: tuple = None  # ((zenith rgb), (horizon rgb)) of the analytic sky, or None
 
     def add_material(self, m):
         self.materials.append(m)
@@ -102,6 +103,7 @@ class SceneDesc:
             "prim_material": prim_material,
             "prim_emission": prim_emission,
             "materials": np.ascontiguousarray(mats, MATERIAL_DTYPE),
+            "sky": None if self.sky is None else np.asarray(self.sky, np.float32).reshape(6),
         }
 
 
@@ -314,15 +316,15 @@ def terrain_scene(seed=0, n=1024, n_spheres=10000, height=0.35, sky=True):
     gj = np.clip(((px + 1) / 2 * (n - 1)).round().astype(np.int64), 0, n - 1)
     py = hf[gi, gj] + rad
     kind = rng.random(n_spheres)
+    albedo = rng.uniform(0.1, 0.85, (n_spheres, 3))
     sph_mats = []
     for k in range(n_spheres):
         if kind[k] < 0.5:
-            sph_mats.append(s.add_material(matte(tuple(rng.uniform(0.1, 0.85, 3)))))
+            sph_mats.append(s.add_material(matte(tuple(albedo[k]))))
         else:
             sph_mats.append(mats[2] if kind[k] < 0.75 else mats[3])
-    for k in range(n_spheres):
-        s.spheres.append((px[k], py[k], pz[k], rad[k]))
-        s.sphere_material.append(sph_mats[k])
+    s.spheres = np.stack([px, py, pz, rad], 1).astype(np.float32)
+    s.sphere_material = sph_mats
     s.sky = SKY_DEFAULT if sky else None
     return s
 

@@ -30,7 +30,7 @@ def test_wavefront_equals_megakernel(gpu):
     import torch
     from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
     _say("case wavefront == megakernel")
-    for desc, depth in ((S.cornell_box(0), 5), (_glossy(), 8)):
+    for desc, depth in ((S.cornell_box(0), 5), (_glossy(), 8), (S.terrain_scene(0, n=24, n_spheres=40), 6)):
         sc = Scene(desc)
         cam = S.cornell_camera(48, 48)
         mk_sum = PathIntegrator(cam, spp=8, max_depth=depth, seed=5).render_sum(sc)

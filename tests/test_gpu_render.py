@@ -61,6 +61,29 @@ def test_glass_spheres_match_oracle(gpu, oracle):
     _compare("glass", desc, S.cornell_camera(48, 48), 16, 8, 4, oracle)
 
 
+def test_sky_terrain_matches_oracle(gpu, oracle):
+    """Small fBm terrain with spheres (Lambertian and microfacet) under the
+    analytic sky: light sampling of the sky, both MIS terms, escaping rays."""
+    desc = S.terrain_scene(0, n=24, n_spheres=40, height=0.35)
+    cam = S.camera((1.6, 1.0, 1.2), (0, 0.1, 0), (0, 1, 0), 45.0, 48, 40)
+    _compare("sky terrain", desc, cam, 8, 6, 2, oracle)
+
+
+def test_sky_uniform_plane_exact_gpu(gpu):
+    """Uniform sky over an upward Lambertian plane, direct light only: every
+    sample is a * L0 (closed form)."""
+    _say("case sky plane")
+    L0, a = (1.0, 2.0, 3.0), 0.5
+    s = S.SceneDesc()
+    m = s.add_material(S.matte((a,) * 3))
+    s.add_quad((-100, 0, -100), (100, 0, -100), (100, 0, 100), (-100, 0, 100), m, normal=(0, 1, 0))
+    s.sky = (L0, L0)
+    cam = S.camera((0, 1, 0), (0, 0, 0.001), (0, 0, 1), 60.0, 16, 16)
+    g, gc = _gpu_sum(s, cam, 4, 1, 3)
+    assert np.allclose(g / 4, np.array(L0) * a, rtol=2e-6), (g.min(), g.max())
+    assert gc[1] > 0
+
+
 @pytest.mark.parametrize("albedo,depth", [(0.5, 2), (0.8, 5)])
 def test_white_furnace_gpu(gpu, albedo, depth):
     _say("case furnace", albedo, depth)

@@ -100,3 +100,61 @@ def test_glossy_box_energy_is_bounded(oracle):
     img, st = sc.render_sum(cam, 16, 8, seed=1)
     assert np.isfinite(img).all() and (img >= 0).all()
     assert st[1] > 0 and st[0] > st[1] / 4
+
+
+def _sky_plane(albedo, zenith, horizon, size=100.0):
+    s = S.SceneDesc()
+    m = s.add_material(S.matte((albedo,) * 3))
+    s.add_quad((-size, 0, -size), (size, 0, -size), (size, 0, size), (-size, 0, size), m, normal=(0, 1, 0))
+    s.sky = (zenith, horizon)
+    return s
+
+
+def test_sky_uniform_plane_direct_is_exact(oracle):
+    """Uniform sky L0 over an upward Lambertian plane (albedo a), direct light
+    only: both MIS strategies have the cosine pdf, so every sample is exactly
+    a * L0 (closed form, zero variance)."""
+    L0, a = (1.0, 2.0, 3.0), 0.5
+    desc = _sky_plane(a, L0, L0)
+    osc = oracle.OracleRenderScene(desc.arrays())
+    cam = S.camera((0, 1, 0), (0, 0, 0.001), (0, 0, 1), 60.0, 16, 16)
+    img, st = osc.render_sum(cam, 4, 1, seed=3)
+    img = img / 4
+    assert np.allclose(img, np.array(L0) * a, rtol=2e-6), (img.min(), img.max())
+    assert st[1] > 0  # the sky's shadow rays were traced
+
+
+def test_sky_gradient_plane_direct_matches_closed_form(oracle):
+    """Gradient sky over an upward Lambertian plane, direct light only:
+    E[L] = (a/pi) int L(w) cos dw = a (0.8 zenith + 0.2 horizon)."""
+    Z, H, a = np.array([0.4, 0.6, 1.0]), np.array([1.0, 1.0, 1.0]), 0.7
+    desc = _sky_plane(a, tuple(Z), tuple(H))
+    osc = oracle.OracleRenderScene(desc.arrays())
+    cam = S.camera((0, 1, 0), (0, 0, 0.001), (0, 0, 1), 60.0, 32, 32)
+    spp = 64
+    img, _ = osc.render_sum(cam, spp, 1, seed=4)
+    mean = (img / spp).reshape(-1, 3).mean(0)
+    expect = a * (0.8 * Z + 0.2 * H)
+    assert np.allclose(mean, expect, rtol=0.01), (mean, expect)
+
+
+def test_sky_seen_directly(oracle):
+    """Camera rays that escape see L(w): a 1-degree camera looking straight up
+    sees the zenith radiance (t = sqrt(w.y) >= 0.99998)."""
+    Z, H = np.array([0.2, 0.5, 2.0]), np.array([3.0, 1.0, 0.5])
+    desc = _sky_plane(0.5, tuple(Z), tuple(H), size=1.0)
+    osc = oracle.OracleRenderScene(desc.arrays())
+    cam = S.camera((0, 1, 0), (0, 2, 0), (0, 0, 1), 1.0, 8, 8)
+    img, _ = osc.render_sum(cam, 2, 3, seed=5)
+    assert np.allclose(img / 2, Z, rtol=1e-4, atol=1e-4)
+
+
+def test_sky_below_horizon_is_black(oracle):
+    """Looking straight down through an opening: the sky emits nothing below
+    the horizon."""
+    desc = _sky_plane(0.5, (1, 1, 1), (1, 1, 1), size=1.0)
+    desc.verts = desc.verts + np.float32([0, 5, 0])  # plane above the camera, out of view
+    osc = oracle.OracleRenderScene(desc.arrays())
+    cam = S.camera((0, 1, 0), (0, 0, 0), (0, 0, 1), 10.0, 8, 8)
+    img, _ = osc.render_sum(cam, 2, 3, seed=6)
+    assert (img == 0).all()
