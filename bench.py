@@ -47,6 +47,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(width):
+    """HBM bytes per closest-hit launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(REPO, "profiles", "traverse_pmc_1M_tris_16M_rays.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    w = d["width"].get(str(width))
+    if not w:
+        return None
+    cal = d["calibration"]
+    return (w["fetch_kib"] * cal["FETCH_SIZE"] + w["write_kib"] * cal["WRITE_SIZE"]) * 1024.0
+
+
 def _oracle():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
@@ -343,7 +358,7 @@ def main():
     ap.add_argument("--only", choices=("cornell", "mlp", "guided", "batched", "combined"),
                     help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
-    ap.add_argument("--width", type=int, default=2, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--width", type=int, default=4, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard with no exchange; "
@@ -408,6 +423,9 @@ def main():
         n_node, n_tri = box_tests / 2.0, prim_tests
         src = "GPU counting kernel (box tests / 2, primitive tests)"
     alg_bytes = 32 + 8 + 32 * n_node + 36 * n_tri
+    traffic = pmc_traffic(args.width) if n == N_RAYS else None
+    if traffic:
+        traffic = round(traffic)
     achieved_gbs = alg_bytes * n / (ms_per_step * 1e-3) / 1e9
     del rd, hits
     bvh.close()
@@ -444,7 +462,10 @@ def main():
                    "n_tris": N_TRIS, "n_rays": n, "max_prims_in_node": 4, "bvh_width": args.width,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/traverse_pmc_1M_tris_16M_rays.json: rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE bytes per launch (calibrated)" if traffic else None,
+                     "algorithmic_bytes_per_launch": round(alg_bytes * n),
                      "bytes_per_ray": round(alg_bytes, 1), "bytes_source": src,
                      "node_tests_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2),
                      "gpu_box_tests_per_ray": round(box_tests, 2), "gpu_prim_tests_per_ray": round(prim_tests, 2)},

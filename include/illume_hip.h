@@ -271,6 +271,15 @@ int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, con
 int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
                                   const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream);
 
+/* ------------------------------------------------------------------ measurement
+ * No counterpart in the reference: a calibration probe for hardware read
+ * counters. n_lanes lanes each read `steps` random 64-byte lines of buf (device,
+ * buf_bytes bytes, 16-byte aligned) with four float4 loads per line, the way
+ * the traversal fetches nodes; out: device [n_lanes] floats (sums, so the
+ * loads are kept). Bytes read = n_lanes * steps * 64. */
+int illume_probe_random_reads(const void *buf, uint64_t buf_bytes, uint32_t n_lanes, uint32_t steps, float *out,
+                              void *stream);
+
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);
 

@@ -10,14 +10,23 @@ namespace illume {
 
 namespace {
 
-constexpr int kBlock = 256;
+// Launch shape per node width (measured on the 2^20-triangle / 2^24-ray
+// workload): two-child nodes run best in 128-lane blocks at the natural
+// register count; four-child nodes in 64-lane blocks limited to 64 VGPRs
+// (8 waves per SIMD), which keeps more node fetches in flight.
+template <int WIDTH>
+struct TraverseShape {
+    static constexpr int kBlock = 128, kWaves = 1;
+};
+template <>
+struct TraverseShape<4> {
+    static constexpr int kBlock = 64, kWaves = 8;
+};
 
-template <bool ANY, bool COUNT, int WIDTH>
-__global__ __launch_bounds__(kBlock) void traverse_kernel(const void *__restrict__ nodes,
-                                                          const PrimRecord *__restrict__ prims,
-                                                          const illume_ray *__restrict__ rays, uint32_t n,
-                                                          illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
-                                                          uint32_t *__restrict__ counts) {
+template <bool ANY, bool COUNT, int WIDTH, int kBlock = TraverseShape<WIDTH>::kBlock>
+__global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse_kernel(
+    const void *__restrict__ nodes, const PrimRecord *__restrict__ prims, const illume_ray *__restrict__ rays,
+    uint32_t n, illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded, uint32_t *__restrict__ counts) {
     __shared__ int lds_stack[kStackLds * kBlock];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -59,7 +68,9 @@ int max_supported_depth() { return kMaxTraceDepth; }
 
 template <int W>
 void launch_w(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n, illume_hit *hits,
-              uint8_t *occluded, uint32_t *counts, bool any, uint32_t grid, hipStream_t stream) {
+              uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {
+    constexpr int kBlock = TraverseShape<W>::kBlock;
+    const uint32_t grid = (n + kBlock - 1) / kBlock;
     if (counts) {
         if (any)
             traverse_kernel<true, true, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, nullptr, nullptr, counts);
@@ -76,16 +87,15 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
                            uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
                            hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t grid = (n + kBlock - 1) / kBlock;
     if (empty) {
-        fill_miss_kernel<<<grid, kBlock, 0, stream>>>(n, counts ? nullptr : (any ? nullptr : hits),
-                                                      counts ? nullptr : (any ? occluded : nullptr), counts);
+        fill_miss_kernel<<<(n + 255) / 256, 256, 0, stream>>>(n, counts ? nullptr : (any ? nullptr : hits),
+                                                              counts ? nullptr : (any ? occluded : nullptr), counts);
         return hipGetLastError();
     }
     if (width == 4)
-        launch_w<4>(nodes, prims, rays, n, hits, occluded, counts, any, grid, stream);
+        launch_w<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
     else
-        launch_w<2>(nodes, prims, rays, n, hits, occluded, counts, any, grid, stream);
+        launch_w<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
     return hipGetLastError();
 }
 

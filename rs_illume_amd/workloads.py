@@ -51,3 +51,82 @@ def spheres_in_cube(n, seed=4, median_radius=0.02, log_sigma=0.5):
     c = rng.random((n, 3))
     r = np.exp(np.log(median_radius) + log_sigma * rng.standard_normal(n))
     return np.concatenate([c, r[:, None]], axis=1).astype(np.float32)
+
+
+def _profile_launches(width=2, launches=2, copy_mib=1024):
+    """The profiling workload for hardware counters: the calibrations of
+    FETCH_SIZE / WRITE_SIZE against known bytes (a plain device copy of
+    copy_mib MiB; a torch gather of 2^22 random 64-byte rows; and
+    illume_probe_random_reads: 2^20 lanes x 64 random 64-byte lines read lane
+    by lane with float4 loads, the access pattern of node fetches), then
+    `launches` closest-hit launches of the 2^20-triangle / 2^24-ray workload."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .accel import BVHAccel, rays_to_device
+    n = copy_mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    rows = src.view(-1, 16)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    pick = torch.randint(0, rows.shape[0], (1 << 22,), device="cuda", generator=gen)
+    gathered = rows.index_select(0, pick)
+    probe_out = torch.empty(1 << 20, dtype=torch.float32, device="cuda")
+    L = _lib.lib()
+    L.illume_probe_random_reads.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+    _lib.check(L.illume_probe_random_reads(ctypes.c_void_p(src.data_ptr()), src.numel() * 4, 1 << 20, 64,
+                                           ctypes.c_void_p(probe_out.data_ptr()), None), "illume_probe_random_reads")
+    torch.cuda.synchronize()
+    del gathered, probe_out
+    v, idx = triangle_soup(1 << 20, seed=1)
+    rays = sphere_to_cube_rays(1 << 24, seed=2)
+    bvh = BVHAccel(v, idx, None, 4, width=width)
+    rd = rays_to_device(rays)
+    hits = torch.empty((rd.shape[0], 4), dtype=torch.int32, device="cuda")
+    for _ in range(launches):
+        bvh.intersect(rd, hits)
+    torch.cuda.synchronize()
+    print("copy %d MiB, %d closest-hit launches (width %d) done" % (copy_mib, launches, width), flush=True)
+
+
+def _probe_ceiling(sizes_mib=(16, 100, 1024, 8192), lanes=1 << 22, steps=64):
+    """Random-read ceiling of the device for the node-fetch access pattern:
+    illume_probe_random_reads over buffers of each size, timed with HIP events."""
+    import ctypes
+    import torch
+    from . import _lib
+    L = _lib.lib()
+    L.illume_probe_random_reads.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+    out = torch.empty(lanes, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    for mib in sizes_mib:
+        buf = torch.zeros(mib << 18, dtype=torch.float32, device="cuda")
+        args = (ctypes.c_void_p(buf.data_ptr()), buf.numel() * 4, lanes, steps, ctypes.c_void_p(out.data_ptr()),
+                ctypes.c_void_p(st.cuda_stream))
+        _lib.check(L.illume_probe_random_reads(*args), "illume_probe_random_reads")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(3):
+            _lib.check(L.illume_probe_random_reads(*args), "illume_probe_random_reads")
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        print("buffer %5d MiB: %.3f ms, %.2f TB/s of random 64-byte lines" % (mib, ms, lanes * steps * 64 / ms / 1e9),
+              flush=True)
+        del buf
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="closest-hit profiling workload")
+    ap.add_argument("--width", type=int, default=2, choices=(2, 4))
+    ap.add_argument("--launches", type=int, default=2)
+    ap.add_argument("--probe", action="store_true", help="time the random-read probe over several buffer sizes")
+    a = ap.parse_args()
+    if a.probe:
+        _probe_ceiling()
+    else:
+        _profile_launches(a.width, a.launches)
