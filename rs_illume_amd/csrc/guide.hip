@@ -1,16 +1,17 @@
 // Learned sampling proposal: 144-64-64-64-40 ReLU MLP from the frequency
 // encoding of (normalised hit position, normal) to an 8-lobe vMF mixture.
 // Inference and training (forward, backward, Adam) run on MFMA with bf16
-// weights and activations and fp32 accumulation
-// (v_mfma_f32_16x16x32_bf16).
+// weights and activations and fp32 accumulation (v_mfma_f32_16x16x32_bf16).
 //
-// One wave owns a tile of 32 rows (two 16-row MFMA tiles). Activations of
-// every layer stay in the wave's LDS; weights are read as MFMA B operands
-// straight from HBM/L2 (40 KB per copy, reused by every wave). Weight
-// gradients are the tile's A^T * dZ products, added to one fp32 gradient
-// vector with float atomics; Adam then updates the fp32 master weights and
-// rewrites both bf16 copies (n-major for forward B operands, k-major for the
-// backward dA = dZ * W^T products).
+// Every product is computed transposed (weights as the A operand, rows of the
+// batch as the B operand), with the hidden units ordered so that an output
+// tile's registers are already the next product's B operand: activations and
+// their gradients stay in registers through all layers. Weights are staged in
+// LDS once per block. Training runs in two passes: per 32-row tile the
+// forward and backward products (storing activations and gradients
+// transposed), then the weight gradients as MFMA products with K = rows over
+// chunks of tiles, then one reduction; Adam updates the fp32 master weights
+// and rewrites the bf16 copies.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include "guide_kernels.h"
@@ -21,178 +22,157 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRows = 32;    // rows per wave
 constexpr int kWave = 64;
-constexpr int kEnc = 144, kEncPad = 160, kHid = 64, kOut = 40, kOutPad = 48, kLobes = 8;
-constexpr uint32_t kGradBlocks = 256;  // target grid of the gradient kernel
-constexpr int kGradWaves = 2;          // waves per gradient block (LDS: 2 x 36 KB + 79 KB)
-static_assert(kGradPartialFloats == (size_t)kGradBlocks * kGuideParams, "one partial row per gradient block");
+constexpr int kEnc = 144, kOut = 40, kLobes = 8;
+constexpr uint32_t kInferGrid = 768;   // inference blocks at most (3 per CU: 47 KB of LDS each)
+static_assert(kGradPartialFloats >= (size_t)128 * kGuideParams, "one partial row per gradient split");
 
 __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
-// A operand of a row-major [rows][ld] bf16 matrix in LDS: rows r0..r0+15,
-// k0..k0+31. Lane l holds row r0 + (l & 15), k = k0 + (l >> 4) * 8 + 0..7.
-__device__ __forceinline__ bf16x8 lds_a(const __bf16 *m, int ld, int r0, int k0, int lane) {
-    return *reinterpret_cast<const bf16x8 *>(m + (r0 + (lane & 15)) * ld + k0 + (lane >> 4) * 8);
-}
-// B operand from an n-major [n][ld] bf16 matrix (global): columns n0..n0+15,
-// k0..k0+31. Lane l holds column n0 + (l & 15), k = k0 + (l >> 4) * 8 + 0..7.
-__device__ __forceinline__ bf16x8 glb_b(const __bf16 *__restrict__ w, int ld, int n0, int k0, int lane) {
-    return *reinterpret_cast<const bf16x8 *>(w + (n0 + (lane & 15)) * ld + k0 + (lane >> 4) * 8);
-}
-// Transposed operand from a row-major [rows][ld] LDS matrix: element (i, k) =
-// m[k][i]; i = i0 + (lane & 15), k = k0 + (lane >> 4) * 8 + 0..7.
-__device__ __forceinline__ bf16x8 lds_t(const __bf16 *m, int ld, int i0, int k0, int lane) {
-    bf16x8 v;
-    const int i = i0 + (lane & 15), k = k0 + (lane >> 4) * 8;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = m[(k + e) * ld + i];
-    return v;
-}
-
-struct WaveLds {
-    __bf16 a0[kRows * kEncPad];
-    __bf16 a1[kRows * kHid], a2[kRows * kHid], a3[kRows * kHid];
-    float y[kRows * kOutPad];
-    __bf16 dz[2][kRows * kHid];
-};
-
 // Frequency encoding: input i (position xyz in [0,1], normal xyz), octave j:
-// sin(2^j pi v) at 24 i + 2 j, cos at 24 i + 2 j + 1; columns 144..159 zero.
-// The angle is reduced exactly (2^j pi v = 2 pi fract(2^(j-1) v): scaling by
-// a power of two and taking the fraction are exact) and the hardware
-// v_sin/v_cos, which take their argument in revolutions, do the rest.
-__device__ void encode_tile(const float *__restrict__ in, int in_stride, uint32_t row0, uint32_t n, __bf16 *a0,
-                            int lane) {
-    for (int idx = lane; idx < kRows * kEncPad; idx += kWave) {
-        const int r = idx / kEncPad, f = idx % kEncPad;
-        float val = 0.f;
-        const uint32_t row = row0 + r;
-        if (f < kEnc && row < n) {
-            const int i = f / 24, j = (f % 24) / 2;
-            const float v = in[(size_t)row * in_stride + i];
-            const float r = ldexpf(v, j - 1);
+// sin(2^j pi v) at feature 24 i + 2 j, cos at 24 i + 2 j + 1; features
+// 144..159 are zero. The angle is reduced exactly (2^j pi v =
+// 2 pi fract(2^(j-1) v): scaling by a power of two and taking the fraction
+// are exact) and the hardware v_sin/v_cos, which take their argument in
+// revolutions, do the rest.
+
+// Inference computes the transposed products Y^T = W^T X^T: the weights are
+// the MFMA A operand (16 output units x 32 inputs) and the rows of the batch
+// the B operand (32 inputs x 16 rows). An output tile then sits in the lanes
+// exactly as the next layer's B operand needs it (lane = row, lane group =
+// a run of 4 output units), given the input order of guide_wa_pos, so the
+// activations never leave registers; the encoding is computed straight into
+// B-operand registers. Each block stages all weights (47 KB, padded rows) in
+// LDS once; its waves then loop over 32-row tiles (two 16-row MFMA columns
+// share every A fragment) without further barriers.
+constexpr int kInfWaves = 4;
+
+struct alignas(16) InferWeights {
+    __bf16 w0[64 * 168];
+    __bf16 w1[64 * 72];
+    __bf16 w2[64 * 72];
+    __bf16 w3[48 * 72];
+    float bias[4][64];
+};
+static_assert(sizeof(InferWeights) % 16 == 0, "16-byte staging");
+
+__device__ __forceinline__ bf16x8 lds_wa(const __bf16 *w, int ld, int m, int j, int lane) {
+    return *reinterpret_cast<const bf16x8 *>(w + (16 * m + (lane & 15)) * ld + 32 * j + (lane >> 4) * 8);
+}
+
+// Encoding of one row into layer 0's B operand: lane group g computes
+// features 40 g .. 40 g + 39 (input i = f / 24, octave o = (f % 24) / 2,
+// sin at even f, cos at odd f); features 144.. are zero.
+__device__ __forceinline__ void encode_b(const float *__restrict__ in, uint32_t stride, uint32_t row, uint32_t n, int g,
+                                         bf16x8 *b) {
+    float v[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] = row < n ? in[(size_t)row * stride + i] : 0.f;
+#pragma unroll
+    for (int s = 0; s < 40; s += 2) {
+        const int f = 40 * g + s;
+        float sv = 0.f, cv = 0.f;
+        if (f < kEnc) {
+            const int i = f / 24, o = (f % 24) / 2;
+            const float x = i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : i == 3 ? v[3] : i == 4 ? v[4] : v[5];
+            const float r = ldexpf(x, o - 1);
             const float rev = r - floorf(r);
-            val = (f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+            sv = __builtin_amdgcn_sinf(rev);
+            cv = __builtin_amdgcn_cosf(rev);
         }
-        a0[idx] = to_bf16(val);
+        b[s / 8][s % 8] = to_bf16(sv);
+        b[s / 8][s % 8 + 1] = to_bf16(cv);
     }
 }
 
-// out_tile = act(A[32 x K] * W[K x N] + b): W given n-major bf16 [N][K].
-// Every B fragment of the layer (N/16 x K/32 of them, 4 VGPRs each) is loaded
-// before the first MFMA, so the wave waits for global memory once per layer
-// rather than once per product.
-template <int K, int N, bool RELU>
-__device__ void dense(const __bf16 *a, const __bf16 *__restrict__ wt, const float *__restrict__ b, __bf16 *out_bf,
-                      float *out_f, int lane) {
-    constexpr int KS = K / 32, NT = N / 16;
-    bf16x8 bw[NT][KS];
-    float bias[NT];
+// acc (output tiles of one 16-row column) -> bias, ReLU, bf16 -> the next
+// layer's B operand (tile m fills k-block m / 2, elements 4 (m % 2) + e).
+template <int MT>
+__device__ __forceinline__ void relu_to_b(const floatx4 *acc, const float *bias, int g, bf16x8 *b) {
 #pragma unroll
-    for (int ct = 0; ct < NT; ++ct) {
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) bw[ct][ks] = glb_b(wt, K, ct * 16, ks * 32, lane);
-        bias[ct] = b[ct * 16 + (lane & 15)];
+        for (int e = 0; e < 4; ++e) {
+            const float h = acc[m][e] + bias[16 * m + 4 * g + e];
+            b[m / 2][4 * (m % 2) + e] = to_bf16(h > 0.f ? h : 0.f);
+        }
+}
+
+template <int KB, int MT>
+__device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*b)[KB], floatx4 (*acc)[4], int lane) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        acc[0][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc[1][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const bf16x8 a = lds_wa(w, ld, m, j, lane);
+            acc[0][m] = mfma(a, b[0][j], acc[0][m]);
+            acc[1][m] = mfma(a, b[1][j], acc[1][m]);
+        }
     }
+}
+
+// n rows of in [n][6] -> out [n][40]; n from n_ptr (at most max_n) when
+// n_ptr is non-null.
+__global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in,
+                                                                       const uint32_t *__restrict__ n_ptr,
+                                                                       uint32_t n_arg, float *__restrict__ out) {
+    __shared__ InferWeights W;
+    const uint32_t n = n_ptr ? min(*n_ptr, n_arg) : n_arg;
+    const uint32_t n_iter = (n + 31) / 32;
+    if (blockIdx.x * kInfWaves >= n_iter) return;  // the whole block has no rows
+    {
+        const __bf16 *src[4] = {g.wa[0], g.wa[1], g.wa[2], g.wa[3]};
+        __bf16 *dst[4] = {W.w0, W.w1, W.w2, W.w3};
 #pragma unroll
-    for (int rt = 0; rt < kRows / 16; ++rt) {
-        bf16x8 af[KS];
+        for (int l = 0; l < 4; ++l) {
+            const int n16 = kGuideWaRows[l] * kGuideWaLd[l] / 8;
+            for (int i = threadIdx.x; i < n16; i += kInfWaves * kWave)
+                reinterpret_cast<uint4 *>(dst[l])[i] = reinterpret_cast<const uint4 *>(src[l])[i];
+        }
+        for (int i = threadIdx.x; i < 4 * 64; i += kInfWaves * kWave) {
+            const int l = i / 64, o = i % 64;
+            W.bias[l][o] = o < kGuideWaRows[l] ? g.bias[l][o] : 0.f;
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, grp = lane >> 4;
+    for (uint32_t it = blockIdx.x * kInfWaves + wave; it < n_iter; it += gridDim.x * kInfWaves) {
+        // Re-read the weight fragments from LDS in every iteration: kept in
+        // registers across iterations they would take 168 VGPRs.
+        __asm__ volatile("" ::: "memory");
+        const uint32_t base = it * 32;
+        bf16x8 b0[2][5], b1[2][2];
+        floatx4 acc[2][4];
+        encode_b(in, 6, base + (lane & 15), n, grp, b0[0]);
+        encode_b(in, 6, base + 16 + (lane & 15), n, grp, b0[1]);
+        layer_t<5, 4>(W.w0, 168, b0, acc, lane);
+        relu_to_b<4>(acc[0], W.bias[0], grp, b1[0]);
+        relu_to_b<4>(acc[1], W.bias[0], grp, b1[1]);
+        layer_t<2, 4>(W.w1, 72, b1, acc, lane);
+        relu_to_b<4>(acc[0], W.bias[1], grp, b1[0]);
+        relu_to_b<4>(acc[1], W.bias[1], grp, b1[1]);
+        layer_t<2, 4>(W.w2, 72, b1, acc, lane);
+        relu_to_b<4>(acc[0], W.bias[2], grp, b1[0]);
+        relu_to_b<4>(acc[1], W.bias[2], grp, b1[1]);
+        layer_t<2, 3>(W.w3, 72, b1, acc, lane);
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) af[ks] = lds_a(a, K, rt * 16, ks * 32, lane);
+        for (int nn = 0; nn < 2; ++nn) {
+            const uint32_t row = base + 16 * nn + (lane & 15);
+            if (row >= n) continue;
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) acc = mfma(af[ks], bw[ct][ks], acc);
-            const int col = ct * 16 + (lane & 15);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int row = rt * 16 + (lane >> 4) * 4 + e;
-                float v = acc[e] + bias[ct];
-                if (RELU) {
-                    v = v > 0.f ? v : 0.f;
-                    out_bf[row * N + col] = to_bf16(v);
-                } else {
-                    out_f[row * N + col] = v;
-                }
+            for (int m = 0; m < 3; ++m) {
+                const int o = 16 * m + 4 * grp;
+                if (o >= kOut) continue;
+                const float4 y = make_float4(acc[nn][m][0] + W.bias[3][o], acc[nn][m][1] + W.bias[3][o + 1],
+                                             acc[nn][m][2] + W.bias[3][o + 2], acc[nn][m][3] + W.bias[3][o + 3]);
+                *reinterpret_cast<float4 *>(out + (size_t)row * kOut + o) = y;
             }
         }
-    }
-}
-
-// Forward pass of one tile keeping every layer's activations (for the
-// backward pass). A wave with active == false only takes part in the block's
-// barriers.
-__device__ void forward_tile(const GuideDev &g, const float *__restrict__ in, int in_stride, uint32_t row0,
-                             uint32_t n, WaveLds &s, int lane, bool active) {
-    if (active) encode_tile(in, in_stride, row0, n, s.a0, lane);
-    __syncthreads();
-    if (active) dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.a1, nullptr, lane);
-    __syncthreads();
-    if (active) dense<kHid, kHid, true>(s.a1, g.wt[1], g.bias[1], s.a2, nullptr, lane);
-    __syncthreads();
-    if (active) dense<kHid, kHid, true>(s.a2, g.wt[2], g.bias[2], s.a3, nullptr, lane);
-    __syncthreads();
-    if (active) dense<kHid, kOutPad, false>(s.a3, g.wt[3], g.bias[3], nullptr, s.y, lane);
-    __syncthreads();
-}
-
-// Inference keeps only what the next layer reads: the encoding buffer (10 KB)
-// and one hidden buffer (4 KB); hidden layers alternate between them and the
-// fp32 outputs (6 KB) reuse the encoding buffer. 14 KB per wave lets 11 waves
-// share a CU's LDS.
-struct alignas(16) InferLds {
-    __bf16 a0[kRows * kEncPad];
-    __bf16 h[kRows * kHid];
-};
-static_assert(kRows * kOutPad * sizeof(float) <= sizeof(InferLds::a0), "outputs fit the encoding buffer");
-
-__device__ const float *forward_infer(const GuideDev &g, const float *__restrict__ in, uint32_t row0, uint32_t n,
-                                      InferLds &s, int lane) {
-    encode_tile(in, 6, row0, n, s.a0, lane);
-    __syncthreads();
-    dense<kEncPad, kHid, true>(s.a0, g.wt[0], g.bias[0], s.h, nullptr, lane);
-    __syncthreads();
-    dense<kHid, kHid, true>(s.h, g.wt[1], g.bias[1], s.a0, nullptr, lane);
-    __syncthreads();
-    dense<kHid, kHid, true>(s.a0, g.wt[2], g.bias[2], s.h, nullptr, lane);
-    __syncthreads();
-    float *y = reinterpret_cast<float *>(s.a0);
-    dense<kHid, kOutPad, false>(s.h, g.wt[3], g.bias[3], nullptr, y, lane);
-    __syncthreads();
-    return y;
-}
-
-__global__ __launch_bounds__(kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in, uint32_t n,
-                                                            float *__restrict__ out) {
-    __shared__ InferLds s;
-    const int lane = threadIdx.x;
-    const uint32_t row0 = blockIdx.x * kRows;
-    const float *y = forward_infer(g, in, row0, n, s, lane);
-    for (int idx = lane; idx < kRows * kOut; idx += kWave) {
-        const int r = idx / kOut, c = idx % kOut;
-        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = y[r * kOutPad + c];
-    }
-}
-
-// Same, with the row count read from device memory (queues filled by an
-// earlier kernel); blocks beyond the count exit at once.
-__global__ __launch_bounds__(kWave) void guide_infer_count_kernel(GuideDev g, const float *__restrict__ in,
-                                                                  const uint32_t *__restrict__ n_ptr, uint32_t max_n,
-                                                                  float *__restrict__ out) {
-    __shared__ InferLds s;
-    const uint32_t n = min(*n_ptr, max_n);
-    const uint32_t row0 = blockIdx.x * kRows;
-    if (row0 >= n) return;
-    const int lane = threadIdx.x;
-    const float *y = forward_infer(g, in, row0, n, s, lane);
-    for (int idx = lane; idx < kRows * kOut; idx += kWave) {
-        const int r = idx / kOut, c = idx % kOut;
-        if (row0 + r < n) out[(size_t)(row0 + r) * kOut + c] = y[r * kOutPad + c];
     }
 }
 
@@ -236,164 +216,312 @@ __device__ void nll_grad_row(const float *y, const float *w, float scale, float 
     }
 }
 
-// dW[in][out] += A^T dZ over the tile's rows; A [32][KA] (bf16, LDS),
-// dZ [32][NZ] (bf16, LDS); only in < in_valid and out < out_valid are stored.
-// gw is the block's fp32 accumulator in LDS, shared by the block's waves
-// (LDS float atomics).
-template <int KA, int NZ>
-__device__ void accumulate_dw(const __bf16 *a, const __bf16 *dz, float *gw, int in_valid, int out_valid, int lane) {
-    for (int it = 0; it < (in_valid + 15) / 16; ++it) {
-        for (int ot = 0; ot < (out_valid + 15) / 16; ++ot) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma(lds_t(a, KA, it * 16, 0, lane), lds_t(dz, NZ, ot * 16, 0, lane), acc);
-            const int o = ot * 16 + (lane & 15);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = it * 16 + (lane >> 4) * 4 + e;
-                if (i < in_valid && o < out_valid) atomicAdd(gw + i * out_valid + o, acc[e]);
-            }
-        }
-    }
-}
+// ------------------------------------------------------------------ training
+// Phase A (guide_train_tile_kernel), per 32-row tile and per wave: the
+// forward pass as in inference, keeping the activations in registers; the
+// outputs go to the wave's LDS, where lanes 0..31 turn their row into the
+// NLL gradient; the backward products dX^T = W G^T then run with the weights
+// as A operands (wb copies, K in register order) so that every gradient again
+// lands in the registers that the next backward product reads. Activations
+// X_1..X_3 and pre-activation gradients G_0..G_3 are stored transposed,
+// [unit][32 rows] bf16, for phase B.
+// Phase B (guide_train_dw_kernel): dW_l = X_l^T G_l as MFMA products with
+// K = rows, split over chunks of tiles (one partial row per chunk), X_0 (the
+// encoding) recomputed from the records; the bias gradients are the column
+// sums of G_l. guide_reduce_kernel adds the partial rows.
+constexpr int kTrainWaves = 6;
+constexpr uint32_t kTrainGrid = 256;   // phase A blocks at most (one per CU: 140 KB of LDS each)
+constexpr uint32_t kTrainSplits = 128;  // phase B row chunks
+constexpr int kDwItems = 9 + 4 + 4 + 4;  // (layer, 16-unit input tile) pairs
 
-// dA[32][64] = dZ[32][KZ] * W^T with W k-major bf16 [64(in)][KZ(out)];
-// dZ_prev = dA * (A_prev > 0) -> bf16; the bias gradient of the previous
-// layer (column sums of dZ_prev) is added to gb_prev (LDS). As in dense(),
-// all B fragments are loaded first.
-template <int KZ>
-__device__ void backprop(const __bf16 *dz, const __bf16 *__restrict__ wk, const __bf16 *a_prev, __bf16 *dz_prev,
-                         float *gb_prev, int lane) {
-    constexpr int KS = KZ / 32, NT = kHid / 16;
-    bf16x8 bw[NT][KS];
-#pragma unroll
-    for (int ct = 0; ct < NT; ++ct)
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) bw[ct][ks] = glb_b(wk, KZ, ct * 16, ks * 32, lane);
-#pragma unroll
-    for (int rt = 0; rt < kRows / 16; ++rt) {
-        bf16x8 af[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) af[ks] = lds_a(dz, KZ, rt * 16, ks * 32, lane);
-#pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) acc = mfma(af[ks], bw[ct][ks], acc);
-            const int col = ct * 16 + (lane & 15);
-            float colsum = 0.f;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int row = rt * 16 + (lane >> 4) * 4 + e;
-                const float v = (float)a_prev[row * kHid + col] > 0.f ? acc[e] : 0.f;
-                dz_prev[row * kHid + col] = to_bf16(v);
-                colsum += v;
-            }
-            atomicAdd(gb_prev + col, colsum);
-        }
-    }
-}
-
-// records: [n][10] = position (3, normalised), normal (3), direction (3), weight
-// (row r at rec + stride_f * r). kGradWaves waves per block; each wave walks
-// `tiles` 32-row tiles with its own activations in LDS, and the block
-// accumulates the gradient of every parameter in one fp32 LDS array
-// (parameter order, LDS float atomics), stored once at the end as the block's
-// row of `partial` ([blocks][params]); guide_reduce_kernel sums the rows.
-// No global atomics: device-scope float atomics on 20k addresses from every
-// block serialise at the memory side. Every wave runs the same number of
-// tile iterations (tiles past n are skipped between the barriers), so each
-// __syncthreads is reached by all waves.
-struct GradLds {
-    WaveLds s[kGradWaves];
-    float dw[kGuideParams];
+struct alignas(16) TrainWeights {
+    __bf16 w0[64 * 168];
+    __bf16 w1[64 * 72];
+    __bf16 w2[64 * 72];
+    __bf16 w3[48 * 72];
+    __bf16 b1[64 * 72];
+    __bf16 b2[64 * 72];
+    __bf16 b3[64 * 72];
+    float bias[4][64];
 };
-static_assert(sizeof(GradLds) <= 160 * 1024, "gradient kernel LDS within one CU's 160 KB");
+struct alignas(16) TrainWaveLds {
+    float y[32][48];     // outputs of the tile, then their gradients
+    __bf16 tr[64 * 40];  // transpose staging, [unit][rows] with rows padded to 40
+};
+struct TrainLds {
+    TrainWeights w;
+    TrainWaveLds wave[kTrainWaves];
+};
+static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS within one CU's 160 KB");
 
-__global__ __launch_bounds__(kGradWaves *kWave) void guide_grad_kernel(
-    GuideDev g, const float *__restrict__ rec, uint32_t n, uint32_t stride_f, uint32_t tiles,
-    const float *__restrict__ wsum, float *__restrict__ partial, float *__restrict__ loss) {
-    __shared__ GradLds L;
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    WaveLds &s = L.s[wave];
-    for (int i = threadIdx.x; i < kGuideParams; i += kGradWaves * kWave) L.dw[i] = 0.f;
-    const float ws = *wsum;
-    float block_loss = 0.f;
-    for (uint32_t t = 0; t < tiles; ++t) {
-        const uint32_t row0 = ((blockIdx.x * tiles + t) * kGradWaves + wave) * kRows;
-        const bool active = row0 < n;
-        __syncthreads();
-        forward_tile(g, rec, (int)stride_f, row0, n, s, lane, active);
-        // output gradient: lane r < 32 owns row r
-        if (active) {
-            float gy[kOut], row_loss = 0.f;
-            const uint32_t row = row0 + lane;
-            const float wgt = (lane < kRows && row < n) ? rec[(size_t)row * stride_f + 9] : 0.f;
-            if (lane < kRows && row < n && wgt > 0.f && ws > 0.f) {
-                const float *r = rec + (size_t)row * stride_f;
-                const float w3[3] = {r[6], r[7], r[8]};
-                nll_grad_row(s.y + lane * kOutPad, w3, wgt / ws, gy, &row_loss);
-            } else {
-                for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
-            }
-            block_loss += row_loss;
-            if (lane < kRows)
-                for (int o = 0; o < kHid; ++o) s.dz[0][lane * kHid + o] = to_bf16(o < kOut ? gy[o] : 0.f);
-            // bias of the output layer: column sums over the tile's rows
-            float *gb4 = L.dw + g.b_off[3];
-            for (int o = 0; o < kOut; ++o) {
-                float v = gy[o];
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == 0 && v != 0.f) atomicAdd(gb4 + o, v);
-            }
+// LDS written by some lanes of a wave and read by others of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ReLU pattern of a hidden layer for this lane's row of column nn: bit
+// 16 nn + 4 m + e set when unit 16 m + 4 g + e is positive.
+__device__ __forceinline__ uint32_t relu_bits(const bf16x8 *h, int nn) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if ((float)h[m / 2][4 * (m % 2) + e] > 0.f) bits |= 1u << (16 * nn + 4 * m + e);
+    return bits;
+}
+
+// Gradient of a hidden layer's output (acc, units 16 m + 4 g + e of this
+// lane's row in column nn) masked by that layer's ReLU -> the next backward
+// product's B operand.
+__device__ __forceinline__ void mask_to_b(const floatx4 *acc, uint32_t bits, int nn, bf16x8 *b) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float v = ((bits >> (16 * nn + 4 * m + e)) & 1u) ? acc[m][e] : 0.f;
+            b[m / 2][4 * (m % 2) + e] = to_bf16(v);
         }
-        __syncthreads();
-        // layer 4: dW4 = A3^T dY, dZ3 = (dY W4^T) * (A3 > 0)
-        if (active) {
-            accumulate_dw<kHid, kHid>(s.a3, s.dz[0], L.dw + g.w_off[3], kHid, kOut, lane);
-            backprop<kHid>(s.dz[0], g.wk[3], s.a3, s.dz[1], L.dw + g.b_off[2], lane);
-        }
-        __syncthreads();
-        if (active) {
-            accumulate_dw<kHid, kHid>(s.a2, s.dz[1], L.dw + g.w_off[2], kHid, kHid, lane);
-            backprop<kHid>(s.dz[1], g.wk[2], s.a2, s.dz[0], L.dw + g.b_off[1], lane);
-        }
-        __syncthreads();
-        if (active) {
-            accumulate_dw<kHid, kHid>(s.a1, s.dz[0], L.dw + g.w_off[1], kHid, kHid, lane);
-            backprop<kHid>(s.dz[0], g.wk[1], s.a1, s.dz[1], L.dw + g.b_off[0], lane);
-        }
-        __syncthreads();
-        if (active) accumulate_dw<kEncPad, kHid>(s.a0, s.dz[1], L.dw + g.w_off[0], kEnc, kHid, lane);
+}
+
+// Stores a 64-unit tensor held in register order (v[n] for the two 16-row
+// columns) as [unit][32 rows] bf16 at dst, through the wave's LDS.
+__device__ __forceinline__ void store_units(TrainWaveLds &S, const bf16x8 (*v)[2], __bf16 *dst, int lane) {
+    const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int nn = 0; nn < 2; ++nn)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S.tr[(16 * m + 4 * g + e) * 40 + 16 * nn + r] = v[nn][m / 2][4 * (m % 2) + e];
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = lane + 64 * q, unit = c / 4, part = c % 4;
+        reinterpret_cast<uint4 *>(dst)[c] = *reinterpret_cast<const uint4 *>(S.tr + unit * 40 + part * 8);
     }
-    if (loss) {
-        float l = block_loss;
-        for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-        if (lane == 0 && l != 0.f) atomicAdd(loss, l);
+    wave_lds_sync();
+}
+
+__global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
+    GuideDev g, const float *__restrict__ rec, uint32_t n, uint32_t stride_f, float *__restrict__ wsum,
+    __bf16 *__restrict__ tiles, float *__restrict__ loss) {
+    __shared__ TrainLds L;
+    const uint32_t n_tiles = (n + 31) / 32;
+    if (blockIdx.x * kTrainWaves >= n_tiles) return;  // the whole block has no rows
+    {
+        const __bf16 *src[7] = {g.wa[0], g.wa[1], g.wa[2], g.wa[3], g.wb[1], g.wb[2], g.wb[3]};
+        __bf16 *dst[7] = {L.w.w0, L.w.w1, L.w.w2, L.w.w3, L.w.b1, L.w.b2, L.w.b3};
+        const int n16[7] = {64 * 168 / 8, 64 * 72 / 8, 64 * 72 / 8, 48 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8};
+#pragma unroll
+        for (int l = 0; l < 7; ++l)
+            for (int i = threadIdx.x; i < n16[l]; i += kTrainWaves * kWave)
+                reinterpret_cast<uint4 *>(dst[l])[i] = reinterpret_cast<const uint4 *>(src[l])[i];
+        for (int i = threadIdx.x; i < 4 * 64; i += kTrainWaves * kWave) {
+            const int l = i / 64, o = i % 64;
+            L.w.bias[l][o] = o < kGuideWaRows[l] ? g.bias[l][o] : 0.f;
+        }
     }
     __syncthreads();
-    float *dst = partial + (size_t)blockIdx.x * kGuideParams;
-    for (int i = threadIdx.x; i < kGuideParams; i += kGradWaves * kWave) dst[i] = L.dw[i];
-}
-
-// grad[i] = sum over the blocks' rows of partial[b][i] (coalesced over i).
-__global__ void guide_reduce_kernel(const float *__restrict__ partial, uint32_t blocks, float *__restrict__ grad) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)kGuideParams) return;
-    float s = 0.f;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[(size_t)b * kGuideParams + i];
-    grad[i] = s;
-}
-
-__global__ void guide_wsum_kernel(const float *__restrict__ rec, uint32_t n, uint32_t stride_f,
-                                  float *__restrict__ wsum) {
-    float s = 0.f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float w = rec[(size_t)i * stride_f + 9];
-        if (w > 0.f) s += w;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, grp = lane >> 4, r16 = lane & 15;
+    TrainWaveLds &S = L.wave[wave];
+    float wave_loss = 0.f, wave_wsum = 0.f;
+    for (uint32_t t = blockIdx.x * kTrainWaves + wave; t < n_tiles; t += gridDim.x * kTrainWaves) {
+        __asm__ volatile("" ::: "memory");
+        const uint32_t base = t * 32;
+        __bf16 *tile = tiles + (size_t)t * (kGuideTileBytes / 2);
+        uint32_t mask1, mask2, mask3;
+        {
+            bf16x8 x0[2][5], h[2][2];
+            floatx4 acc[2][4];
+            encode_b(rec, stride_f, base + r16, n, grp, x0[0]);
+            encode_b(rec, stride_f, base + 16 + r16, n, grp, x0[1]);
+            layer_t<5, 4>(L.w.w0, 168, x0, acc, lane);
+            relu_to_b<4>(acc[0], L.w.bias[0], grp, h[0]);
+            relu_to_b<4>(acc[1], L.w.bias[0], grp, h[1]);
+            mask1 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
+            store_units(S, h, tile + 0 * 64 * 32, lane);
+            layer_t<2, 4>(L.w.w1, 72, h, acc, lane);
+            relu_to_b<4>(acc[0], L.w.bias[1], grp, h[0]);
+            relu_to_b<4>(acc[1], L.w.bias[1], grp, h[1]);
+            mask2 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
+            store_units(S, h, tile + 1 * 64 * 32, lane);
+            layer_t<2, 4>(L.w.w2, 72, h, acc, lane);
+            relu_to_b<4>(acc[0], L.w.bias[2], grp, h[0]);
+            relu_to_b<4>(acc[1], L.w.bias[2], grp, h[1]);
+            mask3 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
+            store_units(S, h, tile + 2 * 64 * 32, lane);
+            layer_t<2, 3>(L.w.w3, 72, h, acc, lane);
+#pragma unroll
+            for (int nn = 0; nn < 2; ++nn)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int o = 16 * m + 4 * grp;
+                    *reinterpret_cast<float4 *>(&S.y[16 * nn + r16][o]) =
+                        make_float4(acc[nn][m][0] + L.w.bias[3][o], acc[nn][m][1] + L.w.bias[3][o + 1],
+                                    acc[nn][m][2] + L.w.bias[3][o + 2], acc[nn][m][3] + L.w.bias[3][o + 3]);
+                }
+        }
+        wave_lds_sync();
+        // output gradient: lane r < 32 owns row base + r
+        if (lane < 32) {
+            float gy[kOut], row_loss = 0.f;
+            for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
+            const uint32_t row = base + lane;
+            if (row < n) {
+                const float *r = rec + (size_t)row * stride_f;
+                const float wgt = r[9];
+                if (wgt > 0.f) {
+                    const float w3[3] = {r[6], r[7], r[8]};
+                    nll_grad_row(S.y[lane], w3, wgt, gy, &row_loss);
+                    wave_wsum += wgt;
+                }
+            }
+            wave_loss += row_loss;
+            for (int o = 0; o < 48; ++o) S.y[lane][o] = o < kOut ? gy[o] : 0.f;
+        }
+        wave_lds_sync();
+        // G_3: B operand (natural output order, K padded to 64) and its tile
+        bf16x8 g3[2][2], gz[2][2];
+        floatx4 acc[2][4];
+#pragma unroll
+        for (int nn = 0; nn < 2; ++nn)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 32 * j + 8 * grp + e;
+                    g3[nn][j][e] = to_bf16(k < 48 ? S.y[16 * nn + r16][k] : 0.f);
+                }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = lane + 64 * q, unit = c / 4, part = c % 4;
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = to_bf16(unit < 48 ? S.y[part * 8 + e][unit] : 0.f);
+            reinterpret_cast<bf16x8 *>(tile + 6 * 64 * 32)[c] = v;
+        }
+        // backward: G_2 = (W_3 G_3^T) * relu'(h3), then G_1, G_0 likewise
+        layer_t<2, 4>(L.w.b3, 72, g3, acc, lane);
+        mask_to_b(acc[0], mask3, 0, gz[0]);
+        mask_to_b(acc[1], mask3, 1, gz[1]);
+        store_units(S, gz, tile + 5 * 64 * 32, lane);
+        layer_t<2, 4>(L.w.b2, 72, gz, acc, lane);
+        mask_to_b(acc[0], mask2, 0, gz[0]);
+        mask_to_b(acc[1], mask2, 1, gz[1]);
+        store_units(S, gz, tile + 4 * 64 * 32, lane);
+        layer_t<2, 4>(L.w.b1, 72, gz, acc, lane);
+        mask_to_b(acc[0], mask1, 0, gz[0]);
+        mask_to_b(acc[1], mask1, 1, gz[1]);
+        store_units(S, gz, tile + 3 * 64 * 32, lane);
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s != 0.f) atomicAdd(wsum, s);
+    float l = wave_loss, w = wave_wsum;
+    for (int off = 32; off > 0; off >>= 1) {
+        l += __shfl_xor(l, off);
+        w += __shfl_xor(w, off);
+    }
+    if (lane == 0) {
+        if (loss && l != 0.f) atomicAdd(loss, l);
+        if (w != 0.f) atomicAdd(wsum, w);
+    }
 }
+
+// One wave per (split, layer, input tile): dW over the split's tiles for all
+// of the layer's output tiles, written to partial row `split`; input tile 0
+// also writes the bias gradients (column sums of G).
+__global__ __launch_bounds__(kWave) void guide_train_dw_kernel(GuideDev g, const float *__restrict__ rec, uint32_t n,
+                                                               uint32_t stride_f, const __bf16 *__restrict__ tiles,
+                                                               uint32_t n_tiles, uint32_t chunk,
+                                                               float *__restrict__ partial) {
+    const int item = blockIdx.x % kDwItems;
+    const uint32_t split = blockIdx.x / kDwItems;
+    const int l = item < 9 ? 0 : 1 + (item - 9) / 4;
+    const int it = item < 9 ? item : (item - 9) % 4;
+    const int lane = threadIdx.x, grp = lane >> 4, r16 = lane & 15;
+    const int n_out = l == 3 ? 3 : 4;
+    const uint32_t t0 = split * chunk, t1 = min(n_tiles, t0 + chunk);
+    floatx4 acc[4];
+    float colsum[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        colsum[u] = 0.f;
+    }
+#pragma unroll 4
+    for (uint32_t t = t0; t < t1; ++t) {
+        const __bf16 *tile = tiles + (size_t)t * (kGuideTileBytes / 2);
+        bf16x8 a;
+        if (l == 0) {
+            // encoding feature f of rows 8 grp .. 8 grp + 7 of the tile
+            const int f = 16 * it + r16, i = f / 24, o = (f % 24) / 2;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t row = t * 32 + 8 * grp + e;
+                const float v = row < n ? rec[(size_t)row * stride_f + i] : 0.f;
+                const float r = ldexpf(v, o - 1);
+                const float rev = r - floorf(r);
+                a[e] = to_bf16((f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev));
+            }
+        } else {
+            a = reinterpret_cast<const bf16x8 *>(tile + ((l - 1) * 64 + 16 * it + r16) * 32)[grp];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= n_out) break;
+            const bf16x8 b = reinterpret_cast<const bf16x8 *>(tile + ((3 + l) * 64 + 16 * u + r16) * 32)[grp];
+            acc[u] = mfma(a, b, acc[u]);
+            if (it == 0)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) colsum[u] += (float)b[e];
+        }
+    }
+    float *prow = partial + (size_t)split * kGuideParams;
+    const int din = g.din[l], dout = g.dout[l];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (u >= n_out) break;
+        const int out = 16 * u + r16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int in = 16 * it + 4 * grp + e;
+            if (in < din && out < dout) prow[g.w_off[l] + in * dout + out] = acc[u][e];
+        }
+        if (it == 0) {
+            float c = colsum[u];
+            c += __shfl_xor(c, 16);
+            c += __shfl_xor(c, 32);
+            if (grp == 0 && out < dout) prow[g.b_off[l] + out] = c;
+        }
+    }
+}
+
+// grad[i] = (sum over the splits' rows of partial[s][i]) / wsum: 32
+// parameters per block, 8 threads per parameter each summing every eighth
+// row, then an LDS reduction. The tile pass weighted rows by w, not w / sum w;
+// the normalisation (of the loss too) happens here.
+__global__ __launch_bounds__(256) void guide_reduce_kernel(const float *__restrict__ partial, uint32_t rows,
+                                                           const float *__restrict__ wsum, float *__restrict__ grad,
+                                                           float *__restrict__ loss) {
+    __shared__ float part[8][32];
+    const int p = threadIdx.x % 32, q = threadIdx.x / 32;
+    const uint32_t i = blockIdx.x * 32 + p;
+    float s = 0.f;
+    if (i < (uint32_t)kGuideParams)
+        for (uint32_t r = q; r < rows; r += 8) s += partial[(size_t)r * kGuideParams + i];
+    part[q][p] = s;
+    __syncthreads();
+    const float ws = *wsum;
+    const float inv = ws > 0.f ? 1.f / ws : 0.f;
+    if (q == 0 && i < (uint32_t)kGuideParams) {
+        float t = 0.f;
+        for (int k = 0; k < 8; ++k) t += part[k][p];
+        grad[i] = t * inv;
+    }
+    if (loss && blockIdx.x == 0 && threadIdx.x == 0) *loss *= inv;
+}
+
 
 __global__ void zero_kernel(float *__restrict__ p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -430,6 +558,8 @@ __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p) {
             const uint32_t k = (i - wo) / dout, o = (i - wo) % dout;
             g.wt[l][o * g.wt_ld[l] + k] = (__bf16)p;   // n-major [out][in_pad]
             g.wk[l][k * g.wk_ld[l] + o] = (__bf16)p;   // k-major [in][out_pad]
+            g.wa[l][o * kGuideWaLd[l] + guide_wa_pos(l, (int)k)] = (__bf16)p;
+            if (l > 0) g.wb[l][k * 72 + guide_wb_pos(l, (int)o)] = (__bf16)p;
             return;
         }
         if (i >= bo && i < bo + (uint32_t)dout) {
@@ -441,19 +571,21 @@ __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p) {
 
 hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, float *out, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    guide_infer_kernel<<<(n + kRows - 1) / kRows, kWave, 0, st>>>(g, in, n, out);
+    const uint32_t blocks = min((n + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
+    guide_infer_kernel<<<blocks, kInfWaves * kWave, 0, st>>>(g, in, nullptr, n, out);
     return hipGetLastError();
 }
 
 hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
                                     float *out, hipStream_t st) {
     if (max_n == 0) return hipSuccess;
-    guide_infer_count_kernel<<<(max_n + kRows - 1) / kRows, kWave, 0, st>>>(g, in, n_ptr, max_n, out);
+    const uint32_t blocks = min((max_n + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
+    guide_infer_kernel<<<blocks, kInfWaves * kWave, 0, st>>>(g, in, n_ptr, max_n, out);
     return hipGetLastError();
 }
 
 hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
-                             float *wsum, float *loss, float *partial, hipStream_t st) {
+                             float *wsum, float *loss, float *partial, void *tiles, hipStream_t st) {
     zero_kernel<<<1, 64, 0, st>>>(wsum, 1);
     if (loss) zero_kernel<<<1, 64, 0, st>>>(loss, 1);
     if (n == 0) {
@@ -461,13 +593,15 @@ hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, ui
         return hipGetLastError();
     }
     const uint32_t stride_f = 10u * (stride ? stride : 1u);
-    guide_wsum_kernel<<<256, 256, 0, st>>>(rec, n, stride_f, wsum);
-    // about one block per CU: each block holds 117 KB of LDS
-    const uint32_t n_tiles = (n + kRows - 1) / kRows;
-    const uint32_t tiles = (n_tiles + kGradBlocks * kGradWaves - 1) / (kGradBlocks * kGradWaves);  // per wave
-    const uint32_t blocks = (n_tiles + tiles * kGradWaves - 1) / (tiles * kGradWaves);
-    guide_grad_kernel<<<blocks, kGradWaves * kWave, 0, st>>>(g, rec, n, stride_f, tiles, wsum, partial, loss);
-    guide_reduce_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(partial, blocks, grad);
+    const uint32_t n_tiles = (n + 31) / 32;
+    const uint32_t blocks = min((n_tiles + kTrainWaves - 1) / kTrainWaves, kTrainGrid);
+    guide_train_tile_kernel<<<blocks, kTrainWaves * kWave, 0, st>>>(g, rec, n, stride_f, wsum, (__bf16 *)tiles,
+                                                                    loss);
+    const uint32_t chunk = (n_tiles + kTrainSplits - 1) / kTrainSplits;
+    const uint32_t splits = (n_tiles + chunk - 1) / chunk;
+    guide_train_dw_kernel<<<splits * kDwItems, kWave, 0, st>>>(g, rec, n, stride_f, (const __bf16 *)tiles, n_tiles,
+                                                                chunk, partial);
+    guide_reduce_kernel<<<(kGuideParams + 31) / 32, 256, 0, st>>>(partial, splits, wsum, grad, loss);
     return hipGetLastError();
 }
 

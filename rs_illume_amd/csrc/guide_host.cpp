@@ -20,7 +20,9 @@ struct illume_guide {
     GuideDev dev;
     std::vector<void *> allocs;
     float *grad = nullptr, *wsum = nullptr, *loss = nullptr;
-    float *partial = nullptr;  // per-block gradients of a training step
+    float *partial = nullptr;  // per-split gradients of a training step
+    void *tiles = nullptr;     // per-tile activations and gradients of a training step
+    size_t tiles_bytes = 0;
     float lr = 1e-3f;
     int step = 0;
 };
@@ -44,10 +46,26 @@ hipError_t dalloc(illume_guide *g, T **p, size_t count) {
 }
 }  // namespace
 
+namespace {
+// The training pass's tile scratch for n records, grown on demand (never
+// inside graph capture: training runs between replays).
+hipError_t ensure_tiles(illume_guide *g, uint32_t n) {
+    const size_t need = (size_t)((n + 31) / 32) * kGuideTileBytes;
+    if (need <= g->tiles_bytes) return hipSuccess;
+    if (g->tiles) hipFree(g->tiles);
+    g->tiles = nullptr;
+    g->tiles_bytes = 0;
+    hipError_t e = hipMalloc(&g->tiles, need);
+    if (e == hipSuccess) g->tiles_bytes = need;
+    return e;
+}
+}  // namespace
+
 extern "C" {
 
 int illume_guide_destroy(illume_guide *g) {
     if (!g) return ILLUME_OK;
+    if (g->tiles) hipFree(g->tiles);
     for (void *p : g->allocs) hipFree(p);
     delete g;
     return ILLUME_OK;
@@ -76,6 +94,8 @@ int illume_guide_create(const float *init_params, float lr, illume_guide **out) 
         off += kDout[l];
         e = dalloc(g, &d.wt[l], (size_t)kOutPadN[l] * kInPad[l]);
         if (e == hipSuccess) e = dalloc(g, &d.wk[l], (size_t)kDin[l] * kOutPadK[l]);
+        if (e == hipSuccess) e = dalloc(g, &d.wa[l], (size_t)kGuideWaRows[l] * kGuideWaLd[l]);
+        if (e == hipSuccess && l > 0) e = dalloc(g, &d.wb[l], (size_t)64 * 72);
         if (e == hipSuccess) e = dalloc(g, &d.bias[l], (size_t)kOutPadN[l]);
     }
     if (e == hipSuccess) e = dalloc(g, &d.params, kGuideParams);
@@ -110,7 +130,9 @@ int illume_guide_infer(const illume_guide *g, const float *inputs, uint32_t n, f
 
 int illume_guide_grad(const illume_guide *g, const float *records, uint32_t n, float *grad_out, void *stream) {
     if (!g || !grad_out || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
-    hipError_t e = guide_launch_grad(g->dev, records, n, 1, grad_out, g->wsum, nullptr, g->partial, (hipStream_t)stream);
+    hipError_t e = ensure_tiles((illume_guide *)g, n);
+    if (e == hipSuccess)
+        e = guide_launch_grad(g->dev, records, n, 1, grad_out, g->wsum, nullptr, g->partial, g->tiles, (hipStream_t)stream);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide grad launch");
 }
 
@@ -120,7 +142,8 @@ namespace illume {
 // One Adam step on n records read with the given stride (in records).
 int guide_train_strided(illume_guide *g, const float *records, uint32_t n, uint32_t stride, hipStream_t st) {
     if (n == 0) return ILLUME_OK;
-    hipError_t e = guide_launch_grad(g->dev, records, n, stride, g->grad, g->wsum, g->loss, g->partial, st);
+    hipError_t e = ensure_tiles(g, n);
+    if (e == hipSuccess) e = guide_launch_grad(g->dev, records, n, stride, g->grad, g->wsum, g->loss, g->partial, g->tiles, st);
     if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
 }
@@ -132,7 +155,8 @@ int illume_guide_train_step(illume_guide *g, const float *records, uint32_t n, v
     if (!g || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (n == 0) return ILLUME_OK;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = guide_launch_grad(g->dev, records, n, 1, g->grad, g->wsum, g->loss, g->partial, st);
+    hipError_t e = ensure_tiles(g, n);
+    if (e == hipSuccess) e = guide_launch_grad(g->dev, records, n, 1, g->grad, g->wsum, g->loss, g->partial, g->tiles, st);
     if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
 }

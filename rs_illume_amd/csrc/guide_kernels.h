@@ -13,10 +13,44 @@ struct GuideDev {
     float *m, *v;   // Adam moments
     __bf16 *wt[4];  // n-major copies [out_pad][in_pad] (forward B operands)
     __bf16 *wk[4];  // k-major copies [in][out_pad] (backward B operands)
+    __bf16 *wa[4];  // inference A operands [out_pad][kGuideWaLd[l]], inputs in register order (guide_wa_pos)
+    __bf16 *wb[4];  // backward A operands, l = 1..3: W_l [in][72], outputs in register order (guide_wb_pos)
     float *bias[4]; // biases, fp32, out_pad entries (zero padded)
     uint32_t w_off[4], b_off[4];
     int din[4], dout[4], wt_ld[4], wk_ld[4];
 };
+
+// Inference copies: row strides (input slots padded by 8 so that the 16 rows
+// an A fragment reads fall in different LDS banks) and rows (outputs, padded
+// to 16).
+constexpr int kGuideWaLd[4] = {168, 72, 72, 72};
+constexpr int kGuideWaRows[4] = {64, 64, 64, 48};
+
+// Slot of input k of layer l in the inference kernel's register order.
+// Layer 0 (encoding feature f, 0..143): lane group g = f / 40 computes
+// features 40g..40g+39 of its row; slot s = f % 40 lands in k-block s / 8,
+// element s % 8, i.e. k = 32 (s / 8) + 8 g + s % 8. Layers 1..3 (hidden unit
+// d, the output row of the previous layer's MFMA tile m = d / 16, row
+// r = d % 16): the lane that holds it is in group r / 4, element r % 4, and
+// tiles 2j, 2j+1 fill k-block j, so k = 32 (m / 2) + 8 (r / 4) + 4 (m % 2) + r % 4.
+__host__ __device__ inline int guide_wa_pos(int l, int k) {
+    if (l == 0) {
+        const int g = k / 40, s = k % 40;
+        return 32 * (s / 8) + 8 * g + s % 8;
+    }
+    const int m = k / 16, r = k % 16;
+    return 32 * (m / 2) + 8 * (r / 4) + 4 * (m % 2) + r % 4;
+}
+
+// Slot of output o of layer l (1..3) in the backward products' K order: the
+// output layer's gradient is read in natural order, hidden-layer gradients
+// arrive in the register order of guide_wa_pos.
+__host__ __device__ inline int guide_wb_pos(int l, int o) { return l == 3 ? o : guide_wa_pos(l + 1, o); }
+
+// Per 32-row tile, the training pass stores the activations X_1..X_3 and the
+// pre-activation gradients G_0..G_3 transposed ([unit][32 rows] bf16) for
+// the weight-gradient products: 3 x 64 + 4 x 64 units (G_3 padded to 64).
+constexpr size_t kGuideTileBytes = (size_t)(3 * 64 + 4 * 64) * 32 * 2;
 
 __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p);
 
@@ -27,10 +61,11 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
 // Gradient of the weighted NLL over n records into grad (zeroed first); wsum
 // (device float) receives the weight sum, loss (optional) the loss. Record r
 // is read at rec + 10 * stride * r (stride >= 1 trains on an even subset).
-// partial: scratch of kGradPartialFloats floats (per-block gradients).
+// partial: scratch of kGradPartialFloats floats (per-split gradients);
+// tiles: scratch of ceil(n / 32) * kGuideTileBytes bytes.
 constexpr size_t kGradPartialFloats = (size_t)256 * kGuideParams;
 hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
-                             float *wsum, float *loss, float *partial, hipStream_t st);
+                             float *wsum, float *loss, float *partial, void *tiles, hipStream_t st);
 hipError_t guide_launch_adam(const GuideDev &g, const float *grad, float lr, int step, hipStream_t st);
 // Rewrites the bf16 copies and biases from the fp32 master weights.
 hipError_t guide_launch_sync(const GuideDev &g, hipStream_t st);
