@@ -143,7 +143,7 @@ def bench_cornell(no_cpu):
     return out
 
 
-def bench_guided(ref_spp=16384, res=512, log=log):
+def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32, plain="megakernel"):
     """Config-3 equal-quality comparison (rs_illume_amd.converge)."""
     import torch
     from rs_illume_amd import converge as C
@@ -161,11 +161,13 @@ def bench_guided(ref_spp=16384, res=512, log=log):
     ref_s = time.perf_counter() - t0
     ref /= ref_spp
     log("reference %d spp in %.1f s" % (ref_spp, ref_s))
-    r = C.guided_vs_plain(sc, cam, ref, target=0.01, spp_per_pass=4, max_spp=4096, max_depth=8, seed=1)
+    r = C.guided_vs_plain(sc, cam, ref, target=0.01, spp_per_pass=4, max_spp=4096, max_depth=8, seed=1,
+                          train_steps=train_steps, warmup_steps=warmup_steps, plain=plain)
     out = {"workload": "glossy small-light Cornell box %dx%d, max depth 8, TR copper walls alpha~U[0.05,0.5], "
                        "light 1%% of the ceiling; relMSE vs %d-spp plain render" % (res, res, ref_spp),
            "target_rel_mse": 0.01, "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
-           "spp_per_pass": r["spp_per_pass"], "plain_renderer": r["plain_renderer"]}
+           "spp_per_pass": r["spp_per_pass"], "plain_renderer": r["plain_renderer"], "train_steps": train_steps,
+           "warmup_steps": warmup_steps}
     for k in ("plain", "guided"):
         rr = r[k]["reached"]
         out[k] = {"seconds": round(rr["seconds"], 5), "spp": rr["spp"], "rel_mse": round(rr["rel_mse"], 5)} \
@@ -233,7 +235,7 @@ def bench_batched(no_cpu, log=log):
     return out
 
 
-def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log):
+def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one render_batch per pass, max depth 10.
     Plain and guided (one proposal network for the scene, trained online;
@@ -271,14 +273,21 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log):
 
     def run(guide):
         film = torch.zeros_like(ref)
-        gc = GuideCfg(0.5, 16, 65536, 0.0) if guide is not None else None
+        gc = GuideCfg(0.5, train_steps, 65536, 0.0) if guide is not None else None
+        warm = guide is not None and warmup_steps > 0
         reached = [None] * len(cams)
         elapsed, spp = 0.0, 0
         wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99) if guide is not None else None, gcfg=gc)
         while spp < max_spp and any(r is None for r in reached):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
+            if warm:
+                gc.guide_frac = 0.0
             wf.render_batch(jobs(spp_pass, spp, 100 if guide is None else 200), film=film, guide=guide, gcfg=gc)
+            if warm and guide.stats()[0] >= warmup_steps:
+                warm = False
+            if guide is not None and not warm:
+                gc.guide_frac = 0.5
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t1
             spp += spp_pass
@@ -304,7 +313,8 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log):
                         "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
                         % (n_px / 1e6, spp_pass),
             "setup_s": round(setup_s, 1), "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
-            "target_rel_mse": target, "plain": plain, "guided": guided,
+            "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
+            "plain": plain, "guided": guided,
             "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both
             else None}
 

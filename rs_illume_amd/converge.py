@@ -48,9 +48,13 @@ def progressive(render_pass, ref, target, spp_per_pass, max_spp, eps=0.01):
 
 
 def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=4096, max_depth=8, guide=None,
-                    guide_frac=0.5, train_batch=65536, plain="megakernel", seed=1, max_paths=1 << 21, log=None):
+                    guide_frac=0.5, train_batch=65536, plain="megakernel", seed=1, max_paths=1 << 21, log=None,
+                    train_steps=16, warmup_steps=0):
     """Time-to-relMSE of the plain and the guided tracer on one scene.
-    ``plain`` selects the plain renderer ("megakernel" or "wavefront")."""
+    ``plain`` selects the plain renderer ("megakernel" or "wavefront"). The
+    guided tracer samples the BSDF alone (training on those paths) until the
+    proposal has taken ``warmup_steps`` Adam steps, then uses one-sample MIS
+    with probability ``guide_frac``; every pass is an unbiased estimate."""
     from .guide import Guide
     from .render import GuideCfg, PathIntegrator, Wavefront, WavefrontPathIntegrator
     w, h = int(np.asarray(camera)["width"].reshape(-1)[0]), int(np.asarray(camera)["height"].reshape(-1)[0])
@@ -68,16 +72,28 @@ def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=409
     g = guide if guide is not None else Guide(seed=seed)
     wf = Wavefront(max_paths=max_paths, max_depth=max_depth)
     gpi = WavefrontPathIntegrator(scene, camera, spp=spp_per_pass, max_depth=max_depth, seed=seed + 1000, guide=g,
-                                  guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf)
+                                  guide_frac=guide_frac, train=train_steps, train_batch=train_batch, wavefront=wf)
     # warm-up on a throw-away guide so graph capture is not timed
     wg = WavefrontPathIntegrator(scene, camera, spp=1, max_depth=max_depth, seed=seed + 2000, guide=Guide(seed=seed + 1),
                                  guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf)
     wg.render_sum(rgb_sum=torch.zeros((h, w, 3), dtype=torch.float32, device="cuda"))
-    guided_pass = lambda film, off: gpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+    state = {"warm": warmup_steps > 0}
+
+    def guided_pass(film, off):
+        if state["warm"]:
+            gpi.gcfg.guide_frac = 0.0
+            gpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+            if g.stats()[0] >= warmup_steps:
+                state["warm"] = False
+                gpi.gcfg.guide_frac = guide_frac
+        else:
+            gpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+
     rg = progressive(guided_pass, ref, target, spp_per_pass, max_spp)
     if log:
         log("guided", rg["reached"], rg["curve"][-1])
-    out = {"plain": rp, "guided": rg, "target": target, "spp_per_pass": spp_per_pass, "plain_renderer": plain}
+    out = {"plain": rp, "guided": rg, "target": target, "spp_per_pass": spp_per_pass, "plain_renderer": plain,
+           "train_steps": train_steps, "warmup_steps": warmup_steps}
     if rp["reached"] and rg["reached"]:
         out["time_ratio"] = rg["reached"]["seconds"] / rp["reached"]["seconds"]
     return out
