@@ -369,6 +369,7 @@ def main():
                     help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
     ap.add_argument("--width", type=int, default=4, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--builder", default="SAH", choices=("SAH", "LBVH"), help="BVH builder (host SAH or device LBVH)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard with no exchange; "
@@ -389,7 +390,7 @@ def main():
     v, idx = W.triangle_soup(N_TRIS, seed=1)
     rays = W.sphere_to_cube_rays(args.rays, seed=2)
     log("building BVH")
-    bvh = BVHAccel(v, idx, None, 4, width=args.width)
+    bvh = BVHAccel(v, idx, None, 4, split_method=args.builder, width=args.width)
     info = bvh.info()
     log("bvh", info)
     rd = rays_to_device(rays)
@@ -470,6 +471,7 @@ def main():
         "config": {"workload": "incoherent closest-hit, 1M-triangle scene (2^20 tris, lognormal sigma median "
                                "0.005), 2^24 rays sphere->cube, SAH BVH <=4 prims/leaf",
                    "n_tris": N_TRIS, "n_rays": n, "max_prims_in_node": 4, "bvh_width": args.width,
+                   "bvh_builder": args.builder,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -80,6 +80,17 @@ int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *t
                          const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
                          illume_bvh **out);
 
+/* The same aggregate built on the device: a linear BVH over Morton codes of
+ * the primitive centroids (the role of BVHAccel::new's HLBVH branch,
+ * accelerators/bvh.rs:249-502), subtrees of at most max_prims_in_node
+ * primitives as leaves; width 4 collapses it on the host. Builds in
+ * milliseconds where the SAH build takes seconds; traversal is somewhat
+ * slower (a lower-quality tree), hit results are identical. Host input
+ * arrays; the build runs on `stream` and is complete on return. */
+int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                          const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width, void *stream,
+                          illume_bvh **out);
+
 /* Frees the handle and its device memory (Drop of BVHAccel). */
 int illume_bvh_destroy(illume_bvh *bvh);
 
@@ -176,11 +187,19 @@ typedef struct illume_sky {
     float horizon[3];
 } illume_sky;
 
-/* illume_scene_create plus an optional analytic sky (sky == NULL: none). */
+/* Options of illume_scene_create_ex. */
+typedef struct illume_scene_opts {
+    const illume_sky *sky; /* NULL: no sky */
+    int32_t bvh_builder;   /* 0: binned SAH on the host (illume_bvh_create), 1: linear BVH on the device
+                              (illume_bvh_create_gpu) */
+    int32_t reserved;
+} illume_scene_opts;
+
+/* illume_scene_create with options (opts == NULL: the defaults, no sky). */
 int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                            const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
                            const float *prim_emission, const illume_material *materials, uint32_t n_materials,
-                           const illume_sky *sky, illume_scene **out);
+                           const illume_scene_opts *opts, illume_scene **out);
 
 int illume_scene_destroy(illume_scene *scene);
 

@@ -8,6 +8,7 @@
 #include "../../include/illume_hip.h"
 #include "bvh_builder.h"
 #include "kernels.h"
+#include "lbvh.h"
 
 namespace {
 thread_local std::string g_last_error;
@@ -123,6 +124,76 @@ int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *t
     h->info.device_bytes = nb + pb;
     h->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     std::memcpy(h->info.bounds, bo.bounds, sizeof(bo.bounds));
+    *out = h;
+    return ILLUME_OK;
+}
+
+int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                          const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width, void *stream,
+                          illume_bvh **out) {
+    if (!out) return fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (max_prims_in_node < 1 || max_prims_in_node > illume::kMaxLeafPrims)
+        return fail(ILLUME_ERR_UNSUPPORTED, "max_prims_in_node must be in 1..15");
+    if (width != 2 && width != 4) return fail(ILLUME_ERR_UNSUPPORTED, "width must be 2 or 4");
+    const uint64_t n = (uint64_t)n_tris + n_spheres;
+    // a handful of primitives: the host build is as fast and handles a root leaf
+    if (n < 2 * (uint64_t)max_prims_in_node)
+        return illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node, width,
+                                    out);
+    auto t0 = std::chrono::steady_clock::now();
+    std::string err = illume::validate_prims(verts, n_verts, tri_idx, n_tris, spheres, n_spheres);
+    if (!err.empty()) return fail(ILLUME_ERR_INVALID_ARG, err);
+    illume::LbvhResult r;
+    hipError_t e = illume::lbvh_build(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node,
+                                      (hipStream_t)stream, &r);
+    if (e != hipSuccess) return hip_fail(e, "device BVH build");
+    if ((int)r.max_depth * (width - 1) >= illume::max_supported_depth()) {
+        hipFree(r.nodes);
+        hipFree(r.prims);
+        return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    illume_bvh *h = new (std::nothrow) illume_bvh();
+    if (!h) {
+        hipFree(r.nodes);
+        hipFree(r.prims);
+        return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    std::memset(&h->info, 0, sizeof(h->info));
+    h->width = width;
+    hipGetDevice(&h->device);
+    h->d_prims = r.prims;
+    size_t nb = (size_t)r.n_nodes * sizeof(illume::GpuNode);
+    uint32_t depth = r.max_depth;
+    if (width == 2) {
+        h->d_nodes = r.nodes;
+    } else {
+        std::vector<illume::GpuNode> n2(r.n_nodes);
+        std::vector<illume::GpuNode4> n4;
+        e = hipMemcpy(n2.data(), r.nodes, nb, hipMemcpyDeviceToHost);
+        hipFree(r.nodes);
+        if (e != hipSuccess) {
+            illume_bvh_destroy(h);
+            return hip_fail(e, "hipMemcpy(binary nodes)");
+        }
+        depth = illume::collapse_to_width4(n2, &n4);
+        nb = n4.size() * sizeof(illume::GpuNode4);
+        if ((e = hipMalloc(&h->d_nodes, nb)) == hipSuccess)
+            e = hipMemcpy(h->d_nodes, n4.data(), nb, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            illume_bvh_destroy(h);
+            return hip_fail(e, "four-child nodes");
+        }
+        r.n_nodes = (uint32_t)n4.size();
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    h->info.n_prims = n;
+    h->info.n_inner_nodes = r.n_nodes;
+    h->info.n_leaves = r.n_leaves;
+    h->info.max_depth = depth;
+    h->info.device_bytes = nb + n * sizeof(illume::PrimRecord);
+    h->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    std::memcpy(h->info.bounds, r.bounds, sizeof(r.bounds));
     *out = h;
     return ILLUME_OK;
 }

@@ -15,6 +15,7 @@
 // Closest-hit ids do not depend on the tree shape (the closest hit is unique
 // up to exact ties), which is why the tree may differ from the reference's.
 #include "bvh_builder.h"
+#include "lbvh.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -391,6 +392,101 @@ std::string build_bvh(const BuildInput &in, BuildOutput *out) {
         if (B.tmp[t.left].left >= 0) stack.push_back({t.left, refs[0]});
     }
     return "";
+}
+
+}  // namespace illume
+
+namespace illume {
+
+std::string validate_prims(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
+                           const float *spheres, uint32_t n_spheres) {
+    if ((uint64_t)n_tris + n_spheres >= kMaxPrimSlots) return "too many primitives (limit 2^27)";
+    if (n_tris && (!verts || !idx)) return "null vertex or index array";
+    if (n_spheres && !spheres) return "null sphere array";
+    for (uint32_t v = 0; v < n_verts && n_tris; ++v)
+        if (!finite3(verts + 3 * (size_t)v)) return "non-finite vertex " + std::to_string(v);
+    for (uint64_t t = 0; t < 3 * (uint64_t)n_tris; ++t)
+        if (idx[t] >= n_verts) return "triangle " + std::to_string(t / 3) + " indexes vertex out of range";
+    for (uint32_t i = 0; i < n_spheres; ++i) {
+        const float *sp = spheres + 4 * (size_t)i;
+        if (!finite3(sp) || !std::isfinite(sp[3]) || !(sp[3] > 0)) return "invalid sphere " + std::to_string(i);
+    }
+    return "";
+}
+
+uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4> *n4) {
+    struct Kid {
+        float b[6];
+        int32_t ref;
+    };
+    auto kid = [&](const GpuNode &g, int c) {
+        Kid k;
+        if (c == 0) {
+            k.b[0] = g.c0_lo_x; k.b[1] = g.c0_lo_y; k.b[2] = g.c0_lo_z;
+            k.b[3] = g.c0_hi_x; k.b[4] = g.c0_hi_y; k.b[5] = g.c0_hi_z;
+            k.ref = g.child0;
+        } else {
+            k.b[0] = g.c1_lo_x; k.b[1] = g.c1_lo_y; k.b[2] = g.c1_lo_z;
+            k.b[3] = g.c1_hi_x; k.b[4] = g.c1_hi_y; k.b[5] = g.c1_hi_z;
+            k.ref = g.child1;
+        }
+        return k;
+    };
+    auto area = [](const Kid &k) {
+        const float dx = k.b[3] - k.b[0], dy = k.b[4] - k.b[1], dz = k.b[5] - k.b[2];
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    };
+    const float inf = std::numeric_limits<float>::infinity();
+    n4->clear();
+    n4->reserve(n2.size() / 2 + 1);
+    n4->push_back(GpuNode4());
+    std::vector<std::tuple<int32_t, int, uint32_t>> st;  // (binary node, slot, depth)
+    st.push_back({0, 0, 1});
+    uint32_t max_depth = 0;
+    while (!st.empty()) {
+        auto [bi, slot, dep] = st.back();
+        st.pop_back();
+        max_depth = std::max(max_depth, dep);
+        Kid kids[4];
+        int nk = 0;
+        kids[nk++] = kid(n2[bi], 0);
+        kids[nk++] = kid(n2[bi], 1);
+        while (nk < 4) {
+            int best = -1;
+            float best_area = -1.f;
+            for (int c = 0; c < nk; ++c)
+                if (kids[c].ref >= 0 && area(kids[c]) > best_area) {
+                    best_area = area(kids[c]);
+                    best = c;
+                }
+            if (best < 0) break;
+            const GpuNode &g = n2[kids[best].ref];
+            kids[best] = kid(g, 0);
+            kids[nk++] = kid(g, 1);
+        }
+        GpuNode4 g4;
+        std::memset(&g4, 0, sizeof(g4));
+        for (int c = 0; c < 4; ++c) {
+            if (c >= nk) {
+                g4.lo_x[c] = g4.hi_x[c] = g4.lo_y[c] = g4.hi_y[c] = g4.lo_z[c] = g4.hi_z[c] = inf;
+                g4.child[c] = make_leaf_ref(0, 0);
+                continue;
+            }
+            const Kid &k = kids[c];
+            g4.lo_x[c] = k.b[0]; g4.lo_y[c] = k.b[1]; g4.lo_z[c] = k.b[2];
+            g4.hi_x[c] = k.b[3]; g4.hi_y[c] = k.b[4]; g4.hi_z[c] = k.b[5];
+            if (k.ref < 0) {
+                g4.child[c] = k.ref;
+            } else {
+                const int s = (int)n4->size();
+                n4->push_back(GpuNode4());
+                g4.child[c] = s;
+                st.push_back({k.ref, s, dep + 1});
+            }
+        }
+        (*n4)[slot] = g4;
+    }
+    return max_depth;
 }
 
 }  // namespace illume

@@ -1,0 +1,33 @@
+// Device BVH builder (lbvh.hip) and the host collapse of a two-child BVH into
+// the four-child layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "layout.h"
+
+namespace illume {
+
+struct LbvhResult {
+    GpuNode *nodes = nullptr;      // device, n_nodes two-child nodes, root at 0
+    PrimRecord *prims = nullptr;   // device, leaf (Morton) order
+    uint32_t n_nodes = 0, n_leaves = 0, max_depth = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// Host input arrays (as for build_bvh), already validated; n_tris + n_spheres
+// >= 2. The caller owns res->nodes and res->prims (hipFree).
+hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
+                      const float *spheres, uint32_t n_spheres, int max_prims, hipStream_t st, LbvhResult *res);
+
+// Four-child nodes from a two-child BVH (any node order, root at 0): each
+// four-child node takes its binary node's two children and opens the inner
+// child of largest surface area until it has four. Depth-first order; leaf
+// references are kept. Returns the four-child depth.
+uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4> *n4);
+
+// Validation shared by both builders: "" or a description of the problem.
+std::string validate_prims(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
+                           const float *spheres, uint32_t n_spheres);
+
+}  // namespace illume

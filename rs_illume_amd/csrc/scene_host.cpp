@@ -222,8 +222,11 @@ int illume_scene_create(const float *verts, uint32_t n_verts, const uint32_t *tr
 int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                            const float *spheres, uint32_t n_spheres, const uint32_t *prim_material,
                            const float *prim_emission, const illume_material *materials, uint32_t n_materials,
-                           const illume_sky *sky, illume_scene **out) {
+                           const illume_scene_opts *opts, illume_scene **out) {
     if (!out) return abi_fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    const illume_sky *sky = opts ? opts->sky : nullptr;
+    const int builder = opts ? opts->bvh_builder : 0;
+    if (builder != 0 && builder != 1) return abi_fail(ILLUME_ERR_UNSUPPORTED, "bvh_builder must be 0 or 1");
     if (sky)
         for (int c = 0; c < 3; ++c)
             if (!(sky->zenith[c] >= 0.f && sky->horizon[c] >= 0.f && std::isfinite(sky->zenith[c]) &&
@@ -251,7 +254,9 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     illume_scene *s = new (std::nothrow) illume_scene();
     if (!s) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(&s->dev, 0, sizeof(s->dev));
-    int st = illume_bvh_create(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, &s->bvh);
+    int st = builder == 1
+                 ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 2, nullptr, &s->bvh)
+                 : illume_bvh_create(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, &s->bvh);
     if (st != ILLUME_OK) {
         delete s;
         return st;

@@ -21,6 +21,10 @@ class RenderCfg(ctypes.Structure):
                 ("flags", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
 
 
+class SceneOpts(ctypes.Structure):
+    _fields_ = [("sky", ctypes.c_void_p), ("bvh_builder", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 def _declare():
     L = _lib.lib()
     if getattr(L, "_render_declared", False):
@@ -29,7 +33,8 @@ def _declare():
     L.illume_scene_create.restype = i32
     L.illume_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, ctypes.POINTER(vp)]
     L.illume_scene_create_ex.restype = i32
-    L.illume_scene_create_ex.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, vp, ctypes.POINTER(vp)]
+    L.illume_scene_create_ex.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, ctypes.POINTER(SceneOpts),
+                                         ctypes.POINTER(vp)]
     L.illume_scene_destroy.restype = i32
     L.illume_scene_destroy.argtypes = [vp]
     L.illume_scene_bvh.restype = i32
@@ -46,9 +51,9 @@ def _p(a):
 
 class Scene:
     """Scene::new(aggregate, lights) over a :class:`scenes.SceneDesc` (with its
-    analytic sky, if it has one)."""
+    analytic sky, if it has one). ``builder``: "SAH" (host) or "LBVH" (device)."""
 
-    def __init__(self, desc):
+    def __init__(self, desc, builder="SAH"):
         if not torch.cuda.is_available():
             raise RuntimeError("Scene needs a ROCm GPU")
         a = desc.arrays() if hasattr(desc, "arrays") else desc
@@ -56,13 +61,15 @@ class Scene:
         L = _declare()
         torch.cuda.init()
         h = ctypes.c_void_p()
+        if builder not in ("SAH", "LBVH"):
+            raise ValueError("builder must be 'SAH' or 'LBVH'")
         sky = a.get("sky")
-        sky_p = _p(np.ascontiguousarray(sky, np.float32)) if sky is not None else None
-        self._sky = sky
+        self._sky = None if sky is None else np.ascontiguousarray(sky, np.float32)
+        opts = SceneOpts(self._sky.ctypes.data if self._sky is not None else None, 1 if builder == "LBVH" else 0, 0)
         _lib.check(L.illume_scene_create_ex(_p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]),
                                             _p(a["spheres"]), len(a["spheres"]), _p(a["prim_material"]),
                                             _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]),
-                                            sky_p, ctypes.byref(h)), "illume_scene_create_ex")
+                                            ctypes.byref(opts), ctypes.byref(h)), "illume_scene_create_ex")
         self._h = h
 
     @property

@@ -55,16 +55,17 @@ def test_golden_fixture(gpu, oracle, name, width):
     assert res["t_exact_frac"] == 1.0
 
 
+@pytest.mark.parametrize("builder", ["SAH", "LBVH"])
 @pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("seed,n_tris,n_spheres,leaf", [(101, 16384, 64, 4), (102, 30000, 0, 1), (103, 5000, 500, 8),
                                                         (104, 20000, 16, 15)])
-def test_seeded_scene(gpu, oracle, seed, n_tris, n_spheres, leaf, width):
+def test_seeded_scene(gpu, oracle, seed, n_tris, n_spheres, leaf, width, builder):
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel
-    _say("case seeded", seed, n_tris, n_spheres, leaf, "width", width)
+    _say("case seeded", seed, n_tris, n_spheres, leaf, "width", width, "builder", builder)
     v, idx, sph = small_scene(n_tris, n_spheres, seed, median_sigma=0.02)
     rays = np.concatenate([W.sphere_to_cube_rays(50000, seed=seed + 1), edge_case_rays(v, idx, sph, seed + 2)])
-    bvh = BVHAccel(v, idx, sph, leaf, width=width)
+    bvh = BVHAccel(v, idx, sph, leaf, split_method=builder, width=width)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(v, idx, sph, 4)
     ref = osc.intersect(rays)
@@ -74,9 +75,10 @@ def test_seeded_scene(gpu, oracle, seed, n_tris, n_spheres, leaf, width):
     _assert_parity(res, o, occ)
 
 
-def test_degenerate_scenes(gpu, oracle):
+@pytest.mark.parametrize("builder", ["SAH", "LBVH"])
+def test_degenerate_scenes(gpu, oracle, builder):
     from rs_illume_amd.accel import BVHAccel
-    _say("case degenerate")
+    _say("case degenerate", builder)
     rng = np.random.default_rng(7)
     # many triangles sharing one centroid, zero-area triangles, duplicates
     base = rng.random((3, 3)).astype(np.float32)
@@ -85,7 +87,7 @@ def test_degenerate_scenes(gpu, oracle):
     idx = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
     from rs_illume_amd import workloads as W
     rays = W.sphere_to_cube_rays(20000, seed=8)
-    bvh = BVHAccel(v, idx, None, 4)
+    bvh = BVHAccel(v, idx, None, 4, split_method=builder)
     h, o = _gpu_run(bvh, rays)
     osc = oracle.OracleScene(v, idx, None, 4)
     res = compare_closest(h, osc.intersect(rays), osc, rays)
@@ -139,18 +141,19 @@ def test_invalid_arguments_gpu(gpu):
         BVHAccel(np.array([[0, 0, np.nan], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
 
 
+@pytest.mark.parametrize("builder", ["SAH", "LBVH"])
 @pytest.mark.parametrize("width", [2, 4])
-def test_full_size_1m_tris_16m_rays(gpu, oracle, width):
+def test_full_size_1m_tris_16m_rays(gpu, oracle, width, builder):
     """The 1M-triangle, 16M-incoherent-ray workload at full size: oracle parity
     on a strided 1/64 sample, plus size-independent properties on all rays."""
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, hits_to_numpy, rays_to_device, split_hits
     n_tris, n_rays = 1 << 20, 1 << 24
-    _say("case full-size", n_tris, n_rays, "width", width)
+    _say("case full-size", n_tris, n_rays, "width", width, "builder", builder)
     v, idx = W.triangle_soup(n_tris, seed=1)
     rays = W.sphere_to_cube_rays(n_rays, seed=2)
-    bvh = BVHAccel(v, idx, None, 4, width=width)
+    bvh = BVHAccel(v, idx, None, 4, split_method=builder, width=width)
     _say("bvh", bvh.info())
     rd = rays_to_device(rays)
     hd = bvh.intersect(rd)
