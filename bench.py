@@ -177,7 +177,7 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
     return out
 
 
-def bench_batched(no_cpu, log=log):
+def bench_batched(no_cpu, log=log, builder="SAH"):
     """Config 2: 64 procedural scenes (seeds 1-64) through one wavefront pool
     in one render_batch call; launch shape chosen by the autotuner on a 2-spp
     probe of the same batch. The oracle renders scene 1 at 1 spp on host cores."""
@@ -187,14 +187,16 @@ def bench_batched(no_cpu, log=log):
     from rs_illume_amd.render import Scene
     t0 = time.time()
     descs = [S.procedural_scene(seed) for seed in range(1, 65)]
-    scenes = [Scene(d) for d, _, _ in descs]
+    gen_s = time.time() - t0
+    t0 = time.time()
+    scenes = [Scene(d, builder=builder) for d, _, _ in descs]
     setup_s = time.time() - t0
     jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
     n_tris = sum(len(d.idx) for d, _, _ in descs)
     n_px = sum(int(c["width"][0]) * int(c["height"][0]) for _, c, _ in descs)
     n_paths = sum(int(c["width"][0]) * int(c["height"][0]) * spp for _, c, spp in descs)
-    log("batched: %d scenes, %.1fM tris, %.1fM px, %.0fM paths, setup %.1f s" % (len(jobs), n_tris / 1e6, n_px / 1e6,
-                                                                               n_paths / 1e6, setup_s))
+    log("batched: %d scenes, %.1fM tris, %.1fM px, %.0fM paths, generation %.1f s, scene set-up (%s) %.1f s"
+        % (len(jobs), n_tris / 1e6, n_px / 1e6, n_paths / 1e6, gen_s, builder, setup_s))
     t0 = time.time()
     best, table = autotune.tune(jobs, max_depth=8, sample_spp=2)
     tune_s = time.time() - t0
@@ -210,7 +212,8 @@ def bench_batched(no_cpu, log=log):
     rays = counts.cpu().numpy().astype(np.float64)
     out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
                        "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one render_batch call",
-           "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths,
+           "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths, "bvh_builder": builder,
+           "scene_setup_s": round(setup_s, 2),
            "seconds": round(dt, 4), "samples_per_s": round(n_paths / dt, 1),
            "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1), "iterations": wf.last_iterations(),
            "autotune": {"best": {k: best[k] for k in ("max_paths", "block", "regen_frac", "graph_iters")},
@@ -235,7 +238,8 @@ def bench_batched(no_cpu, log=log):
     return out
 
 
-def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32):
+def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
+                   builder="SAH"):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one render_batch per pass, max depth 10.
     Plain and guided (one proposal network for the scene, trained online;
@@ -248,9 +252,10 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
     from rs_illume_amd.render import GuideCfg, Scene, Wavefront
     t0 = time.time()
     desc = S.terrain_scene(0)
-    sc = Scene(desc)
+    t1 = time.time()
+    sc = Scene(desc, builder=builder)
+    setup_s = time.time() - t1
     cams = S.terrain_views(0)
-    setup_s = time.time() - t0
     depth = 10
     sizes = [int(c["width"][0]) * int(c["height"][0]) for c in cams]
     n_px = sum(sizes)
@@ -312,7 +317,8 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
                         "(radius lognormal, median 0.02), Lambertian/microfacet, analytic sky; 16 views "
                         "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
                         % (n_px / 1e6, spp_pass),
-            "setup_s": round(setup_s, 1), "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
+            "bvh_builder": builder, "scene_setup_s": round(setup_s, 2), "reference_spp": ref_spp,
+            "reference_s": round(ref_s, 2),
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
             "plain": plain, "guided": guided,
             "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both

@@ -113,9 +113,11 @@ __global__ void morton_kernel(const float *__restrict__ cen, uint32_t n, const f
                               uint64_t *__restrict__ keys, uint32_t *__restrict__ ids) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    // one scale for all axes (the largest centroid extent): per-axis scales
+    // would stretch flat scenes and split their thin axis too early
+    const float ext = fmaxf(fmaxf(cb[3] - cb[0], cb[4] - cb[1]), cb[5] - cb[2]);
     uint32_t q[3];
     for (int a = 0; a < 3; ++a) {
-        const float ext = cb[3 + a] - cb[a];
         const float u = ext > 0.f ? (cen[3 * (size_t)p + a] - cb[a]) / ext : 0.f;
         q[a] = (uint32_t)fminf(fmaxf(u * 2097152.f, 0.f), 2097151.f);
     }

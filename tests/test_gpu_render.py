@@ -19,10 +19,10 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _gpu_sum(desc, cam, spp, depth, seed, sample_offset=0):
+def _gpu_sum(desc, cam, spp, depth, seed, sample_offset=0, builder="SAH"):
     import torch
     from rs_illume_amd.render import PathIntegrator, Scene
-    sc = Scene(desc)
+    sc = Scene(desc, builder=builder)
     pi = PathIntegrator(cam, spp=spp, max_depth=depth, seed=seed)
     counts = torch.zeros(2, dtype=torch.int64, device="cuda")
     s = pi.render_sum(sc, sample_offset=sample_offset, ray_counts=counts)
@@ -30,9 +30,9 @@ def _gpu_sum(desc, cam, spp, depth, seed, sample_offset=0):
     return s.cpu().numpy(), counts.cpu().numpy()
 
 
-def _compare(name, desc, cam, spp, depth, seed, oracle):
-    _say("case", name)
-    g, gc = _gpu_sum(desc, cam, spp, depth, seed)
+def _compare(name, desc, cam, spp, depth, seed, oracle, builder="SAH"):
+    _say("case", name, builder)
+    g, gc = _gpu_sum(desc, cam, spp, depth, seed, builder=builder)
     o, oc = oracle.OracleRenderScene(desc.arrays()).render_sum(cam, spp, depth, seed=seed)
     rm = S.rel_mse(g / spp, o / spp)
     exact = float(np.mean(np.isclose(g, o, rtol=1e-4, atol=1e-5)))
@@ -61,12 +61,14 @@ def test_glass_spheres_match_oracle(gpu, oracle):
     _compare("glass", desc, S.cornell_camera(48, 48), 16, 8, 4, oracle)
 
 
-def test_sky_terrain_matches_oracle(gpu, oracle):
+@pytest.mark.parametrize("builder", ["SAH", "LBVH"])
+def test_sky_terrain_matches_oracle(gpu, oracle, builder):
     """Small fBm terrain with spheres (Lambertian and microfacet) under the
-    analytic sky: light sampling of the sky, both MIS terms, escaping rays."""
+    analytic sky: light sampling of the sky, both MIS terms, escaping rays;
+    the scene's BVH from either builder."""
     desc = S.terrain_scene(0, n=24, n_spheres=40, height=0.35)
     cam = S.camera((1.6, 1.0, 1.2), (0, 0.1, 0), (0, 1, 0), 45.0, 48, 40)
-    _compare("sky terrain", desc, cam, 8, 6, 2, oracle)
+    _compare("sky terrain", desc, cam, 8, 6, 2, oracle, builder=builder)
 
 
 def test_sky_uniform_plane_exact_gpu(gpu):
