@@ -125,6 +125,18 @@ def bench_cornell(no_cpu):
     out = {"workload": "Cornell box 256x256, 16 spp, max depth 5 (10 triangles, 2 spheres, 0.25^2 light)",
            "samples_per_s": round(256 * 256 * spp / dt, 1), "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1),
            "ms_per_frame": round(dt * 1e3, 3), "closest_rays": int(rays[0]), "any_rays": int(rays[1])}
+    # the same frame through pbrt-v3's default Gaussian filter (radius 2): samples over the
+    # 260x260 sample bounds, each splatted into up to 16 pixels
+    from rs_illume_amd.film import Filter, render_filtered
+    filt = Filter("gaussian")
+    rgbw = render_filtered(pi, sc, filt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        render_filtered(pi, sc, filt, rgbw=rgbw)
+    torch.cuda.synchronize()
+    dtf = (time.perf_counter() - t0) / reps
+    out["gaussian_filter"] = {"samples_per_s": round(260 * 260 * spp / dtf, 1), "ms_per_frame": round(dtf * 1e3, 3)}
     if not no_cpu:
         pyoracle = _oracle()
         osc = pyoracle.OracleRenderScene(desc.arrays())

@@ -217,6 +217,52 @@ int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out);
 int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                   float *rgb_sum, uint64_t *ray_counts, void *stream);
 
+/* ------------------------------------------------------------------ reconstruction filters
+ * The Filter trait (core/filter.rs) and its implementations: BoxFilter
+ * (filters/box_filter.rs), TriangleFilter (triangle.rs), GaussianFilter
+ * (gaussian.rs), MitchellFilter (mitchell.rs), LanczosSincFilter (sinc.rs),
+ * tabulated as Film::new does (core/film.rs:69-81: 16 x 16 values at
+ * ((i + 0.5) r / 16) over the positive quadrant). */
+
+typedef enum illume_filter_type {
+    ILLUME_FILTER_BOX = 0,
+    ILLUME_FILTER_TRIANGLE = 1,
+    ILLUME_FILTER_GAUSSIAN = 2,
+    ILLUME_FILTER_MITCHELL = 3,
+    ILLUME_FILTER_LANCZOS = 4
+} illume_filter_type;
+
+typedef struct illume_filter_desc {
+    int32_t type;     /* illume_filter_type */
+    float radius[2];  /* x, y radius in pixels, each in (0, 32] */
+    float alpha;      /* Gaussian falloff */
+    float b, c;       /* Mitchell B, C */
+    float tau;        /* Lanczos window: sinc(x) sinc(x / tau) */
+} illume_filter_desc;
+
+typedef struct illume_filter illume_filter;
+
+int illume_filter_create(const illume_filter_desc *desc, illume_filter **out);
+int illume_filter_destroy(illume_filter *filter);
+/* Host copy of the 16 x 16 weight table (row y, column x). */
+int illume_filter_table(const illume_filter *filter, float *host_table256);
+
+/* illume_render through a reconstruction filter: camera samples are drawn for
+ * every pixel of the sample bounds (Film::get_sample_bounds, film.rs:97-103:
+ * the image grown by the filter radius) and each is splatted as
+ * FilmTile::add_sample does (film.rs:288-333, with pbrt-v3's upper bound
+ * floor(p - 0.5 + r) + 1 and full x loop): rgbw_sum[4*(y*width+x) + c] gets
+ * w * L (c < 3) and w (c = 3), w the table weight. The caller zeroes rgbw_sum
+ * (device float, width*height*4) and resolves it with illume_film_resolve.
+ * With a box filter of radius 0.5 the sample positions, sampler streams and
+ * radiance equal illume_render's and every weight is 1. */
+int illume_render_filtered(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                           const illume_filter *filter, float *rgbw_sum, uint64_t *ray_counts, void *stream);
+
+/* Film::write_image's per-pixel step (film.rs:187-226, without XYZ): rgb_out[3p + c] =
+ * scale * (w != 0 ? max(0, rgbw[4p + c] / w) : rgbw[4p + c]). Device buffers. */
+int illume_film_resolve(const float *rgbw_sum, uint64_t n_pixels, float scale, float *rgb_out, void *stream);
+
 /* ------------------------------------------------------------------ learned proposal
  * No counterpart in the reference: an MLP (144-64-64-64-40, ReLU) from the
  * frequency encoding of a hit's normalised position and normal to an 8-lobe

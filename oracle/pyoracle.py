@@ -193,6 +193,13 @@ def _render_decl():
     L.orc_render.argtypes = [vp, vp, u32, u32, u64, u32, vp, vp, i32]
     L.orc_camera_rays.argtypes = [vp, vp, u32, vp]
     L.orc_sampler_floats.argtypes = [u64, u64, u64, u32, vp]
+    L.orc_render_filtered.argtypes = [vp, vp, vp, u32, u32, u64, u32, vp, vp, i32]
+    L.orc_render_filtered.restype = i32
+    for fn, at in (("orc_filter_table", [vp, vp]), ("orc_filter_eval", [vp, vp, u32, vp]),
+                   ("orc_film_sample_bounds", [vp, i32, i32, vp]),
+                   ("orc_film_add_samples", [vp, vp, u32, i32, i32, vp])):
+        getattr(L, fn).argtypes = at
+        getattr(L, fn).restype = i32
     L._rdecl = True
     return L
 
@@ -225,6 +232,58 @@ class OracleRenderScene:
         _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _p(out), _p(st),
                                   n_threads or n_threads_default())
         return out, st
+
+    def render_filtered(self, camera, filter_desc, spp, max_depth, seed=0, sample_offset=0, n_threads=None):
+        """Filtered film sums (H, W, 4): weighted RGB and weight, and ray counts."""
+        cam = np.ascontiguousarray(camera).reshape(1)
+        w, h = int(cam["width"][0]), int(cam["height"][0])
+        out = np.zeros((h, w, 4), np.float32)
+        st = np.zeros(2, np.uint64)
+        _ok(_render_decl().orc_render_filtered(self.h, _p(cam), _p(_filter(filter_desc)), spp, max_depth, seed,
+                                               sample_offset, _p(out), _p(st), n_threads or n_threads_default()),
+            "render_filtered")
+        return out, st
+
+
+def _filter(desc):
+    d = np.ascontiguousarray(desc).reshape(1)
+    if d.dtype.itemsize != 28:
+        raise ValueError("filter description must be a 28-byte record (illume_filter_desc)")
+    return d
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise ValueError("oracle %s: invalid filter" % what)
+
+
+def filter_table(desc):
+    out = np.zeros(256, np.float32)
+    _ok(_render_decl().orc_filter_table(_p(_filter(desc)), _p(out)), "filter_table")
+    return out.reshape(16, 16)
+
+
+def filter_eval(desc, xy):
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    out = np.zeros(len(xy), np.float32)
+    _ok(_render_decl().orc_filter_eval(_p(_filter(desc)), _p(xy), len(xy), _p(out)), "filter_eval")
+    return out
+
+
+def film_sample_bounds(desc, width, height):
+    out = np.zeros(4, np.int32)
+    _ok(_render_decl().orc_film_sample_bounds(_p(_filter(desc)), width, height, _p(out)), "sample_bounds")
+    return tuple(int(v) for v in out)
+
+
+def film_add_samples(desc, samples, width, height, rgbw=None):
+    """samples: (n, 5) film x, film y, r, g, b -> rgbw (height, width, 4)."""
+    s = np.ascontiguousarray(samples, np.float32).reshape(-1, 5)
+    if rgbw is None:
+        rgbw = np.zeros((height, width, 4), np.float32)
+    _ok(_render_decl().orc_film_add_samples(_p(_filter(desc)), _p(s), len(s), width, height, _p(rgbw)),
+        "film_add_samples")
+    return rgbw
 
 
 def camera_rays(camera, film_xy_time):

@@ -2,6 +2,7 @@
 // tracer for the tests and the CPU baseline.
 #include <atomic>
 #include <thread>
+#include "film.h"
 #include "render.h"
 
 using namespace orc;
@@ -156,6 +157,111 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
             stats[1] += x.any_rays;
         }
     }
+}
+
+// SamplerIntegratorBase::render through a reconstruction filter: camera
+// samples for every pixel of Film::get_sample_bounds (film.rs:97-103), the
+// sampler stream of sample pixel (x, y) numbered (y - y0) * (x1 - x0) + (x - x0)
+// (so a box filter of radius 0.5 uses orc_render's streams), each sample
+// splatted by film_add_sample into rgbw (4 floats per image pixel). Threads
+// take sample rows t, t + n, ... into private films summed in thread order, so
+// the result does not depend on timing.
+int orc_render_filtered(void *h, const void *camera, const void *filter_desc, uint32_t spp, uint32_t max_depth,
+                        uint64_t seed, uint32_t sample_offset, float *rgbw, uint64_t *stats, int n_threads) {
+    RenderScene *s = (RenderScene *)h;
+    const OrcCamera *c = (const OrcCamera *)camera;
+    Filter f;
+    if (!f.init(*(const FilterDesc *)filter_desc)) return -1;
+    FilmFilter ff;
+    ff.init(f);
+    Camera cam;
+    cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
+             V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
+    const int W = (int)c->width, H = (int)c->height;
+    int x0, y0, x1, y1;
+    film_sample_bounds(ff, W, H, &x0, &y0, &x1, &y1);
+    const uint64_t wext = (uint64_t)(x1 - x0);
+    const int nt = std::max(1, n_threads);
+    std::vector<RenderStats> st(nt);
+    std::vector<std::vector<float>> films(nt, std::vector<float>((size_t)W * H * 4, 0.f));
+    auto work = [&](int t) {
+        for (int y = y0 + t; y < y1; y += nt)
+            for (int x = x0; x < x1; ++x) {
+                const uint64_t pix = (uint64_t)(y - y0) * wext + (uint64_t)(x - x0);
+                for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
+                    Sampler sampler;
+                    sampler.start(seed, pix, si);
+                    float uf[2], ul[2];
+                    sampler.get_2d(uf);
+                    float time = sampler.get_1d();
+                    sampler.get_2d(ul);
+                    const float fx = (float)x + uf[0], fy = (float)y + uf[1];
+                    Ray ray = cam.generate_ray(fx, fy, time);
+                    RGB L = path_li(*s, ray, sampler, (int)max_depth, &st[t]);
+                    if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = RGB();
+                    else if (luminance(L) < -1e-5f) L = RGB();
+                    else if (std::isinf(luminance(L))) L = RGB();
+                    const float l3[3] = {L.x, L.y, L.z};
+                    film_add_sample(ff, films[t].data(), W, H, fx, fy, l3);
+                }
+            }
+    };
+    if (nt == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    for (int t = 0; t < nt; ++t)
+        for (size_t i = 0; i < films[t].size(); ++i) rgbw[i] += films[t][i];
+    if (stats) {
+        stats[0] = stats[1] = 0;
+        for (auto &x : st) {
+            stats[0] += x.closest_rays;
+            stats[1] += x.any_rays;
+        }
+    }
+    return 0;
+}
+
+// The 16 x 16 filter table of a filter description (0 on success).
+int orc_filter_table(const void *filter_desc, float *table) {
+    Filter f;
+    if (!f.init(*(const FilterDesc *)filter_desc)) return -1;
+    FilmFilter ff;
+    ff.init(f);
+    for (int i = 0; i < 256; ++i) table[i] = ff.table[i];
+    return 0;
+}
+
+// Filter::evaluate at n points (xy pairs).
+int orc_filter_eval(const void *filter_desc, const float *xy, uint32_t n, float *out) {
+    Filter f;
+    if (!f.init(*(const FilterDesc *)filter_desc)) return -1;
+    for (uint32_t i = 0; i < n; ++i) out[i] = f.eval(xy[2 * i], xy[2 * i + 1]);
+    return 0;
+}
+
+// Film::get_sample_bounds: out = x0, y0, x1, y1.
+int orc_film_sample_bounds(const void *filter_desc, int width, int height, int *out) {
+    Filter f;
+    if (!f.init(*(const FilterDesc *)filter_desc)) return -1;
+    FilmFilter ff;
+    ff.init(f);
+    film_sample_bounds(ff, width, height, out, out + 1, out + 2, out + 3);
+    return 0;
+}
+
+// n samples (film x, film y, r, g, b) splatted into rgbw (width x height x 4).
+int orc_film_add_samples(const void *filter_desc, const float *samples, uint32_t n, int width, int height,
+                         float *rgbw) {
+    Filter f;
+    if (!f.init(*(const FilterDesc *)filter_desc)) return -1;
+    FilmFilter ff;
+    ff.init(f);
+    for (uint32_t i = 0; i < n; ++i)
+        film_add_sample(ff, rgbw, width, height, samples[5 * i], samples[5 * i + 1], samples + 5 * i + 2);
+    return 0;
 }
 
 // Camera ray probe: rays (8 floats: o, d, t_max, time) for film points.

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../../include/illume_hip.h"
+#include "film.h"
 #include "layout.h"
 #include "scene_types.h"
 
@@ -10,6 +11,15 @@ namespace illume {
 // Path tracer: adds the radiance sums of spp paths per pixel to rgb.
 hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
                          uint32_t sample_offset, float *rgb, unsigned long long *counters, hipStream_t stream);
+
+// Path tracer through a reconstruction filter: samples over fs's sample
+// bounds, splatted into rgbw (4 floats per image pixel).
+hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
+                                  uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
+                                  unsigned long long *counters, hipStream_t stream);
+
+// rgb[3p + c] = scale * (w != 0 ? max(0, rgbw[4p + c] * (1 / w)) : rgbw[4p + c]).
+hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream);
 
 // Deepest tree the traversal stack supports (LDS ring + private spill).
 int max_supported_depth();

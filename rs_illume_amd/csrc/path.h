@@ -359,22 +359,32 @@ struct PathState {
 };
 
 // The camera-sample stage of SamplerIntegratorBase::render: sampler stream,
-// get_camera_sample (film 2D, time 1D, lens 2D) and the camera ray.
-__device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
-                                           PathState &ps) {
-    const int x = (int)(pix % cam.width), y = (int)(pix / cam.width);
-    ps.rng.seed(seed, pix, s);
+// get_camera_sample (film 2D, time 1D, lens 2D) and the camera ray, for
+// sample s of the sample pixel (x, y) whose sampler stream is numbered
+// `stream`; (fx, fy) gets the film position of the camera sample.
+__device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t seed, uint64_t stream, int x, int y,
+                                              uint32_t s, PathState &ps, float &fx, float &fy) {
+    ps.rng.seed(seed, stream, s);
     const float uf0 = ps.rng.get_1d(), uf1 = ps.rng.get_1d();
     ps.rng.get_1d();
     ps.rng.get_1d();
     ps.rng.get_1d();
-    camera_ray(cam, (float)x + uf0, (float)y + uf1, ps.o, ps.d, ps.t_max);
+    fx = (float)x + uf0;
+    fy = (float)y + uf1;
+    camera_ray(cam, fx, fy, ps.o, ps.d, ps.t_max);
     ps.beta = mk(1.f, 1.f, 1.f);
     ps.L = mk(0.f, 0.f, 0.f);
     ps.eta_scale = 1.f;
     ps.bounces = 0;
     ps.specular = false;
     ps.active = true;
+}
+
+// Camera path of sample s of image pixel `pix` (stream = pixel index).
+__device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
+                                           PathState &ps) {
+    float fx, fy;
+    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy);
 }
 
 // One bounce of PathIntegrator::li after the closest-hit query (found, h).

@@ -12,6 +12,7 @@
 // to the caller's float RGB sum buffer with float atomics; ray counts are
 // reduced per block in LDS before one 64-bit atomic per block.
 #include <hip/hip_runtime.h>
+#include "film.h"
 #include "kernels.h"
 #include "path.h"
 
@@ -23,15 +24,18 @@ namespace {
 
 constexpr int kRenderBlock = 128;
 
-template <int BLOCK>
+// FILTERED: camera samples over the sample bounds of `fs` (stream = index in
+// those bounds) splatted through the filter into 4-float pixels; otherwise
+// one sample pixel per image pixel, added to 3-float pixels.
+template <int BLOCK, bool FILTERED>
 __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
-                                                        uint64_t seed, uint32_t sample_offset, float *__restrict__ rgb,
-                                                        unsigned long long *__restrict__ counters) {
+                                                        uint64_t seed, uint32_t sample_offset, float *__restrict__ film,
+                                                        unsigned long long *__restrict__ counters, FilmSplat fs) {
     __shared__ int lds_stack[kStackLds * BLOCK];
     __shared__ uint32_t blk_counts[2];
     if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t n_pix = (uint64_t)cam.width * cam.height;
+    const uint64_t n_pix = FILTERED ? (uint64_t)fs.wext * fs.hext : (uint64_t)cam.width * cam.height;
     const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     PathCounters pc{0, 0};
     if (gid < n_pix * spp) {
@@ -40,7 +44,11 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
         TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x);
         PathState ps;
-        path_start(cam, seed, pix, s, ps);
+        float fx, fy;
+        if (FILTERED)
+            path_start_at(cam, seed, pix, fs.x0 + (int)(pix % fs.wext), fs.y0 + (int)(pix / fs.wext), s, ps, fx, fy);
+        else
+            path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy);
         GuideCfg gc{};
         uint32_t n_recs = 0;
         while (ps.active) {
@@ -49,9 +57,13 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
             path_shade<BLOCK>(sc, max_depth, ps, found, h, nullptr, gc, nullptr, n_recs, st, pc);
         }
         const f3 L = guard_radiance(ps.L);
-        atomicAdd(rgb + 3 * pix, L.x);
-        atomicAdd(rgb + 3 * pix + 1, L.y);
-        atomicAdd(rgb + 3 * pix + 2, L.z);
+        if (FILTERED) {
+            film_splat(fs, film, (int)cam.width, (int)cam.height, fx, fy, L.x, L.y, L.z);
+        } else {
+            atomicAdd(film + 3 * pix, L.x);
+            atomicAdd(film + 3 * pix + 1, L.y);
+            atomicAdd(film + 3 * pix + 2, L.z);
+        }
     }
     if (counters) {
         atomicAdd(&blk_counts[0], pc.closest);
@@ -64,6 +76,24 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
     }
 }
 
+// Film::write_image's per-pixel normalisation (film.rs:187-226).
+__global__ __launch_bounds__(256) void resolve_kernel(const float *__restrict__ rgbw, uint64_t n, float scale,
+                                                      float *__restrict__ rgb) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float r = rgbw[4 * i], g = rgbw[4 * i + 1], b = rgbw[4 * i + 2];
+    const float w = rgbw[4 * i + 3];
+    if (w != 0.f) {
+        const float inv = 1.f / w;
+        r = fmaxf(r * inv, 0.f);
+        g = fmaxf(g * inv, 0.f);
+        b = fmaxf(b * inv, 0.f);
+    }
+    rgb[3 * i] = r * scale;
+    rgb[3 * i + 1] = g * scale;
+    rgb[3 * i + 2] = b * scale;
+}
+
 }  // namespace
 
 hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
@@ -72,8 +102,28 @@ hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp,
     if (n == 0) return hipSuccess;
     const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
     if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
-    render_kernel<kRenderBlock><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(sc, cam, spp, max_depth, seed,
-                                                                             sample_offset, rgb, counters);
+    render_kernel<kRenderBlock, false><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
+        sc, cam, spp, max_depth, seed, sample_offset, rgb, counters, FilmSplat{});
+    return hipGetLastError();
+}
+
+hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
+                                  uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
+                                  unsigned long long *counters, hipStream_t stream) {
+    const uint64_t n = (uint64_t)fs.wext * fs.hext * spp;
+    if (n == 0) return hipSuccess;
+    const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
+    if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
+    render_kernel<kRenderBlock, true><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
+        sc, cam, spp, max_depth, seed, sample_offset, rgbw, counters, fs);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t grid = (n + 255) / 256;
+    if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
+    resolve_kernel<<<(uint32_t)grid, 256, 0, stream>>>(rgbw, n, scale, rgb);
     return hipGetLastError();
 }
 

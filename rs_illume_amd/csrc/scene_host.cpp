@@ -198,6 +198,7 @@ void scene_bounds(const illume_scene *s, float *b) {
     std::memcpy(b, info.bounds, sizeof(info.bounds));
 }
 bool scene_camera(const illume_camera &c, DevCamera *out);
+void filter_splat(const illume_filter *f, uint32_t width, uint32_t height, FilmSplat *out);
 }  // namespace illume
 
 extern "C" {
@@ -412,6 +413,26 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
     hipError_t e = launch_render(scene->dev, cam, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset, rgb_sum,
                                  (unsigned long long *)ray_counts, (hipStream_t)stream);
     if (e != hipSuccess) return abi_hip_fail(e, "render launch");
+    return ILLUME_OK;
+}
+
+int illume_render_filtered(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                           const illume_filter *filter, float *rgbw_sum, uint64_t *ray_counts, void *stream) {
+    if (!scene || !camera || !cfg || !filter || !rgbw_sum) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    DevCamera cam;
+    if (!make_camera(*camera, &cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
+    if (cfg->spp == 0) return ILLUME_OK;
+    if ((uint64_t)cfg->sample_offset + cfg->spp > (1u << 24))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+    if ((uint64_t)camera->width + 64 >= (1u << 24) || (uint64_t)camera->height + 64 >= (1u << 24))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "image side beyond 2^24 - 64");
+    FilmSplat fs;
+    filter_splat(filter, camera->width, camera->height, &fs);
+    if ((uint64_t)fs.wext * fs.hext >= (1ull << 40)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+    if (scene->n_prims == 0) return ILLUME_OK;
+    hipError_t e = launch_render_filtered(scene->dev, cam, fs, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset,
+                                          rgbw_sum, (unsigned long long *)ray_counts, (hipStream_t)stream);
+    if (e != hipSuccess) return abi_hip_fail(e, "filtered render launch");
     return ILLUME_OK;
 }
 

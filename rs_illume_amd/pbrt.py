@@ -22,7 +22,8 @@ Two layers, as in the reference's core/parser:
   pbrt-v3's scene API (api.cpp) for the subset the GPU path tracer renders.
 
 Loaded: LookAt + perspective camera (fov), image film (resolution,
-filename), any sampler's pixelsamples, the path integrator's maxdepth,
+filename), box/triangle/gaussian/mitchell/sinc pixel filters, any sampler's
+pixelsamples, the path integrator's maxdepth,
 Identity/Translate/Scale/Rotate/Transform/ConcatTransform,
 CoordinateSystem/CoordSysTransform, ReverseOrientation, matte/metal/glass
 materials (constant textures allowed), named materials, diffuse area lights on
@@ -447,8 +448,9 @@ def write_ply(path, verts, tris):
 # ------------------------------------------------------------------------------------ loader
 class Loaded:
     """Result of :func:`load`: ``desc`` (scenes.SceneDesc), ``camera``
-    (scenes.camera record), ``spp``, ``max_depth``, the film ``filename`` and
-    the warnings for directives and parameters that were skipped."""
+    (scenes.camera record), ``spp``, ``max_depth``, the film ``filename``, the
+    pixel ``filter`` (film.filter_desc record, None for pbrt-v3's default box)
+    and the warnings for directives and parameters that were skipped."""
 
     def __init__(self):
         self.desc = S.SceneDesc()
@@ -456,6 +458,7 @@ class Loaded:
         self.spp = 16
         self.max_depth = 5
         self.filename = None
+        self.filter = None  # film.filter_desc record; None: box of radius 0.5
         self.warnings = []
 
 
@@ -748,9 +751,10 @@ class _Loader:
             self.ctm = self.coords[d.args[0]].copy()
         elif n == "ReverseOrientation":
             self.gs.reverse = not self.gs.reverse
-        elif n in ("ActiveTransform", "TransformTimes", "Accelerator", "PixelFilter", "MakeNamedMedium",
-                   "MediumInterface"):
+        elif n in ("ActiveTransform", "TransformTimes", "Accelerator", "MakeNamedMedium", "MediumInterface"):
             self.warn(d, "ignored")
+        elif n == "PixelFilter":
+            self.pixel_filter(d)
         elif n == "Camera":
             self.camera(d)
         elif n == "Film":
@@ -850,6 +854,30 @@ class _Loader:
         else:
             raise PbrtError("%s: not supported" % d.where())
 
+    def pixel_filter(self, d):
+        """pbrt-v3 filter parameters: xwidth/ywidth (radius), Gaussian alpha,
+        Mitchell B and C, Lanczos tau; defaults per filter as pbrt-v3's."""
+        from . import film
+        kind = d.args[0]
+        if kind not in film.NAMES:
+            raise PbrtError("%s: filter %r is not supported (box, triangle, gaussian, mitchell, sinc)"
+                            % (d.where(), kind))
+        t = film.NAMES[kind]
+        used = set()
+        r = film.DEFAULTS[t]["radius"]
+        rx, ry = self.one(d, "xwidth", r, used), self.one(d, "ywidth", r, used)
+        kw = {}
+        if t == film.GAUSSIAN:
+            kw["alpha"] = self.one(d, "alpha", 2.0, used)
+        elif t == film.MITCHELL:
+            kw["b"], kw["c"] = self.one(d, "B", 1.0 / 3.0, used), self.one(d, "C", 1.0 / 3.0, used)
+        elif t == film.LANCZOS:
+            kw["tau"] = self.one(d, "tau", 3.0, used)
+        if not (0 < rx <= 32 and 0 < ry <= 32):
+            raise PbrtError("%s: filter radius must be in (0, 32]" % d.where())
+        self.check_unused(d, used)
+        self.out.filter = film.filter_desc(t, radius=(rx, ry), **kw)
+
     def camera(self, d):
         if self.in_world:
             raise PbrtError("%s: Camera inside the world block" % d.where())
@@ -937,7 +965,24 @@ def _material_text(m):
     return 'Material "glass" "float index" [ %s ]' % _f(m["ior"])
 
 
-def write_text(desc, camera, spp=16, max_depth=5, filename=None):
+def _filter_text(f):
+    from . import film
+    f = np.asarray(f).reshape(-1)[0]
+    t = int(f["type"])
+    name = {film.BOX: "box", film.TRIANGLE: "triangle", film.GAUSSIAN: "gaussian", film.MITCHELL: "mitchell",
+            film.LANCZOS: "sinc"}[t]
+    out = 'PixelFilter "%s" "float xwidth" [ %s ] "float ywidth" [ %s ]' % (name, _f(f["radius"][0]),
+                                                                          _f(f["radius"][1]))
+    if t == film.GAUSSIAN:
+        out += ' "float alpha" [ %s ]' % _f(f["alpha"])
+    elif t == film.MITCHELL:
+        out += ' "float B" [ %s ] "float C" [ %s ]' % (_f(f["b"]), _f(f["c"]))
+    elif t == film.LANCZOS:
+        out += ' "float tau" [ %s ]' % _f(f["tau"])
+    return out
+
+
+def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=None):
     """pbrt-v3 text of a flat scene. Triangles keep their order: each run of
     triangles with one material and emission becomes one trianglemesh (its
     own vertices, in first-use order); spheres become translated spheres; the
@@ -952,8 +997,10 @@ def write_text(desc, camera, spp=16, max_depth=5, filename=None):
                                     " ".join(_f(v) for v in c["up"])),
              'Camera "perspective" "float fov" [ %s ]' % _f(c["fov"]), film,
              'Sampler "random" "integer pixelsamples" [ %d ]' % spp,
-             'Integrator "path" "integer maxdepth" [ %d ]' % max_depth,
-             "WorldBegin"]
+             'Integrator "path" "integer maxdepth" [ %d ]' % max_depth]
+    if pixel_filter is not None:
+        lines.append(_filter_text(pixel_filter))
+    lines.append("WorldBegin")
     a = desc.arrays()
     mats = a["materials"]
     if a["sky"] is not None:
@@ -990,11 +1037,20 @@ def write_text(desc, camera, spp=16, max_depth=5, filename=None):
 
 
 def render(loaded, spp=None, wavefront=False, seed=0, builder="SAH"):
-    """Render a loaded scene on the GPU: (H, W, 3) float32 tensor of the mean
-    radiance per pixel."""
+    """Render a loaded scene on the GPU: (H, W, 3) float32 tensor of the
+    pixel values (mean radiance through the scene's pixel filter). The
+    wavefront renderer films with the default box filter only."""
     from .render import PathIntegrator, Scene, WavefrontPathIntegrator
     spp = loaded.spp if spp is None else spp
     sc = Scene(loaded.desc, builder=builder)
+    if loaded.filter is not None:
+        if wavefront:
+            raise PbrtError("the wavefront renderer films with the box filter of radius 0.5 only")
+        from .film import Filter, render_filtered, resolve
+        pi = PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed)
+        f = loaded.filter.reshape(-1)[0]
+        return resolve(render_filtered(pi, sc, Filter(int(f["type"]), radius=tuple(f["radius"]), alpha=f["alpha"],
+                                                      b=f["b"], c=f["c"], tau=f["tau"])))
     if wavefront:
         return WavefrontPathIntegrator(sc, loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed).render()
     return PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed).render(sc)

@@ -140,3 +140,21 @@ def test_command_line_writes_pfm(gpu, tmp_path):
     ref = _gpu(desc, cam, 4, 5, 0) / 4.0
     assert img.shape == (30, 40, 3)
     assert np.allclose(img, ref.astype(np.float32), rtol=SUM_RTOL, atol=0)
+
+
+def test_command_line_filtered(gpu, tmp_path, oracle):
+    """A scene file with a Gaussian pixel filter renders through the filtered
+    film; the PFM matches the oracle's filtered film at equal streams."""
+    from rs_illume_amd.film import filter_desc
+    from rs_illume_amd.imageio import read_pfm
+    desc, cam = S.cornell_box(0), S.cornell_camera(36, 28)
+    f = filter_desc("gaussian", radius=1.5, alpha=2.0)
+    (tmp_path / "g.pbrt").write_text(P.write_text(desc, cam, spp=4, max_depth=5, pixel_filter=f))
+    _say("case pbrt main gaussian")
+    P.main([str(tmp_path / "g.pbrt"), "-o", str(tmp_path / "g.pfm")])
+    img = read_pfm(str(tmp_path / "g.pfm"))
+    o, _ = oracle.OracleRenderScene(desc.arrays()).render_filtered(cam, f, 4, 5, seed=0)
+    ref = np.maximum(o[..., :3] / o[..., 3:4], 0)
+    rm = S.rel_mse(img, ref)
+    _say("gaussian relMSE", rm)
+    assert rm <= REL_MSE_BAR

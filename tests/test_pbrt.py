@@ -411,9 +411,33 @@ def test_camera_errors():
 def test_warnings_for_ignored_parameters():
     ld = world(TRI.replace('"point P"', '"float st" [0 0 1 0 0 1] "point P"'),
                HEAD + 'PixelFilter "gaussian"\nSampler "sobol" "integer pixelsamples" 8 "integer dimension" 5\n')
-    assert any("PixelFilter" in w for w in ld.warnings)
+    assert not any("PixelFilter" in w for w in ld.warnings)
     assert any("'dimension' ignored" in w for w in ld.warnings)
     assert any("'st' ignored" in w for w in ld.warnings)
+
+
+@pytest.mark.parametrize("text,expect", [
+    ('PixelFilter "box"', dict(type=0, radius=(0.5, 0.5))),
+    ('PixelFilter "gaussian"', dict(type=2, radius=(2, 2), alpha=2)),
+    ('PixelFilter "gaussian" "float xwidth" 1.5 "float ywidth" 1 "float alpha" 3', dict(type=2, radius=(1.5, 1), alpha=3)),
+    ('PixelFilter "mitchell"', dict(type=3, radius=(2, 2), b=np.float32(1 / 3), c=np.float32(1 / 3))),
+    ('PixelFilter "mitchell" "float B" 0.5 "float C" 0.25', dict(type=3, b=0.5, c=0.25)),
+    ('PixelFilter "sinc"', dict(type=4, radius=(4, 4), tau=3)),
+    ('PixelFilter "triangle" "float xwidth" 1', dict(type=1, radius=(1, 2))),
+])
+def test_pixel_filters(text, expect):
+    """pbrt-v3 filter parameters and defaults (xwidth/ywidth, alpha, B, C, tau)."""
+    ld = world(TRI, HEAD + text + "\n")
+    f = ld.filter[0]
+    for k, v in expect.items():
+        assert np.allclose(f[k], v), (k, f[k], v)
+
+
+def test_pixel_filter_errors():
+    with pytest.raises(P.PbrtError, match="filter 'bspline'"):
+        world(TRI, HEAD + 'PixelFilter "bspline"\n')
+    with pytest.raises(P.PbrtError, match="radius"):
+        world(TRI, HEAD + 'PixelFilter "box" "float xwidth" 0\n')
 
 
 # ------------------------------------------------------------------------ round trips
@@ -449,6 +473,14 @@ def test_write_load_round_trip(name, tmp_path):
     assert ld.warnings == []
     # and the written text is a fixed point
     assert P.write_text(ld.desc, ld.camera, 8, 6, "out.pfm") == text
+
+
+def test_round_trip_keeps_the_filter():
+    from rs_illume_amd.film import filter_desc
+    desc, cam = S.cornell_box(0), S.cornell_camera(8, 8)
+    f = filter_desc("mitchell", radius=(1.5, 2.5), b=0.25, c=0.375)
+    ld = P.load_text(P.write_text(desc, cam, pixel_filter=f))
+    assert ld.filter.tobytes() == f.tobytes()
 
 
 def test_round_trip_renders_identically_in_the_oracle(oracle):
