@@ -137,6 +137,16 @@ def bench_cornell(no_cpu):
     torch.cuda.synchronize()
     dtf = (time.perf_counter() - t0) / reps
     out["gaussian_filter"] = {"samples_per_s": round(260 * 260 * spp / dtf, 1), "ms_per_frame": round(dtf * 1e3, 3)}
+    # the same frame with pbrt-v3's Halton sampler (scrambled radical inverses per dimension)
+    ph = PathIntegrator(cam, spp=spp, max_depth=5, seed=0, sampler="halton")
+    acc = ph.render_sum(sc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ph.render_sum(sc, rgb_sum=acc)
+    torch.cuda.synchronize()
+    dth = (time.perf_counter() - t0) / reps
+    out["halton_sampler"] = {"samples_per_s": round(256 * 256 * spp / dth, 1), "ms_per_frame": round(dth * 1e3, 3)}
     if not no_cpu:
         pyoracle = _oracle()
         osc = pyoracle.OracleRenderScene(desc.arrays())

@@ -160,9 +160,20 @@ typedef struct {
     uint32_t spp;
     uint32_t max_depth;
     uint32_t sample_offset;
-    uint32_t flags; /* reserved, 0 */
+    uint32_t flags; /* 0 or ILLUME_RENDER_HALTON */
     uint64_t seed;
 } illume_render_cfg;
+
+/* flags: the Halton sampler instead of the random streams (illume_render and
+ * illume_render_filtered; the seed is then unused). pbrt-v3's HaltonSampler
+ * (PBR book 3e 7.4) over the reference's radical-inverse primitives
+ * (core/low_discrepancy.rs:196-205, 422-466, 1505-1531): sample s of pixel
+ * (x, y) is the global index that the pixel map of the sample bounds gives;
+ * dimensions 0-1 are the film offset, 2-4 time and lens, then the path's
+ * dimensions in order, each a radical inverse in the next prime base with
+ * that base's digit permutation (1000 bases; past them a hash of (index,
+ * dimension)). */
+#define ILLUME_RENDER_HALTON 1u
 
 /* Build a renderable scene: the BVH of illume_bvh_create (max 4 primitives per
  * leaf) plus per-primitive material index and emitted radiance (3 floats per

@@ -190,10 +190,17 @@ def _render_decl():
     L.orc_render_scene_create.restype = vp
     L.orc_render_scene_create.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, vp]
     L.orc_render_scene_destroy.argtypes = [vp]
-    L.orc_render.argtypes = [vp, vp, u32, u32, u64, u32, vp, vp, i32]
+    L.orc_render.argtypes = [vp, vp, u32, u32, u64, u32, u32, vp, vp, i32]
     L.orc_camera_rays.argtypes = [vp, vp, u32, vp]
     L.orc_sampler_floats.argtypes = [u64, u64, u64, u32, vp]
-    L.orc_render_filtered.argtypes = [vp, vp, vp, u32, u32, u64, u32, vp, vp, i32]
+    L.orc_render_filtered.argtypes = [vp, vp, vp, u32, u32, u64, u32, u32, vp, vp, i32]
+    L.orc_radical_inverse.argtypes = [i32, vp, u32, i32, vp]
+    L.orc_halton_perm.argtypes = [i32, vp, vp]
+    L.orc_inverse_radical_inverse.argtypes = [u32, u64, i32]
+    L.orc_inverse_radical_inverse.restype = u64
+    L.orc_halton_sample.argtypes = [i32, i32, i32, i32, ctypes.c_int64, u32, vp]
+    L.orc_halton_sample.restype = ctypes.c_int64
+    L.orc_halton_map.argtypes = [i32, i32, vp]
     L.orc_render_filtered.restype = i32
     for fn, at in (("orc_filter_table", [vp, vp]), ("orc_filter_eval", [vp, vp, u32, vp]),
                    ("orc_film_sample_bounds", [vp, i32, i32, vp]),
@@ -223,26 +230,67 @@ class OracleRenderScene:
             _render_decl().orc_render_scene_destroy(h)
             self.h = None
 
-    def render_sum(self, camera, spp, max_depth, seed=0, sample_offset=0, n_threads=None):
-        """Per-pixel radiance sums (H, W, 3) float32 and (closest, any) ray counts."""
+    def render_sum(self, camera, spp, max_depth, seed=0, sample_offset=0, n_threads=None, sampler="random"):
+        """Per-pixel radiance sums (H, W, 3) float32 and (closest, any) ray counts.
+        sampler: "random" (PCG32 streams) or "halton"."""
         cam = np.ascontiguousarray(camera).reshape(1)
         w, h = int(cam["width"][0]), int(cam["height"][0])
         out = np.zeros((h, w, 3), np.float32)
         st = np.zeros(2, np.uint64)
-        _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _p(out), _p(st),
-                                  n_threads or n_threads_default())
+        _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _sampler_flags(sampler),
+                                  _p(out), _p(st), n_threads or n_threads_default())
         return out, st
 
-    def render_filtered(self, camera, filter_desc, spp, max_depth, seed=0, sample_offset=0, n_threads=None):
+    def render_filtered(self, camera, filter_desc, spp, max_depth, seed=0, sample_offset=0, n_threads=None,
+                        sampler="random"):
         """Filtered film sums (H, W, 4): weighted RGB and weight, and ray counts."""
         cam = np.ascontiguousarray(camera).reshape(1)
         w, h = int(cam["width"][0]), int(cam["height"][0])
         out = np.zeros((h, w, 4), np.float32)
         st = np.zeros(2, np.uint64)
         _ok(_render_decl().orc_render_filtered(self.h, _p(cam), _p(_filter(filter_desc)), spp, max_depth, seed,
-                                               sample_offset, _p(out), _p(st), n_threads or n_threads_default()),
+                                               sample_offset, _sampler_flags(sampler), _p(out), _p(st),
+                                               n_threads or n_threads_default()),
             "render_filtered")
         return out, st
+
+
+def _sampler_flags(sampler):
+    if sampler not in ("random", "halton"):
+        raise ValueError("sampler must be 'random' or 'halton'")
+    return 1 if sampler == "halton" else 0
+
+
+def radical_inverse(base_index, a, scrambled=False):
+    a = np.ascontiguousarray(a, np.uint64).reshape(-1)
+    out = np.zeros(len(a), np.float32)
+    _render_decl().orc_radical_inverse(base_index, _p(a), len(a), int(scrambled), _p(out))
+    return out
+
+
+def halton_perm(base_index):
+    """(prime, permutation of 0..prime-1) of a Halton dimension."""
+    out = np.zeros(8192, np.uint16)
+    prime = np.zeros(1, np.uint32)
+    _render_decl().orc_halton_perm(base_index, _p(out), _p(prime))
+    return int(prime[0]), out[:int(prime[0])].copy()
+
+
+def inverse_radical_inverse(base, inverse, n_digits):
+    return int(_render_decl().orc_inverse_radical_inverse(base, inverse, n_digits))
+
+
+def halton_sample(res, pixel, sample, n):
+    """(global index, first n dimensions) of a Halton sample."""
+    out = np.zeros(n, np.float32)
+    idx = _render_decl().orc_halton_sample(res[0], res[1], pixel[0], pixel[1], sample, n, _p(out))
+    return int(idx), out
+
+
+def halton_map(res):
+    out = np.zeros(7, np.int64)
+    _render_decl().orc_halton_map(res[0], res[1], _p(out))
+    return dict(zip(("exp0", "exp1", "scale0", "scale1", "stride", "inv0", "inv1"), (int(v) for v in out)))
 
 
 def _filter(desc):

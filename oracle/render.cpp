@@ -42,7 +42,17 @@ static uint64_t splitmix64(uint64_t x) {
 uint64_t sampler_stream(uint64_t seed, uint64_t pixel, uint64_t sample) {
     return splitmix64(seed ^ splitmix64((pixel << 24) | (sample & 0xffffff)));
 }
-void Sampler::start(uint64_t seed, uint64_t pixel, uint64_t sample) { rng.set_sequence(sampler_stream(seed, pixel, sample)); }
+void Sampler::start(uint64_t seed, uint64_t pixel, uint64_t sample) {
+    halton = nullptr;
+    rng.set_sequence(sampler_stream(seed, pixel, sample));
+}
+
+float Sampler::halton_next() {
+    const int d = dim++;
+    if (d < kHaltonDims) return halton->sample_dimension(index, d);
+    const uint64_t h = splitmix64((uint64_t)index * 0x9E3779B97F4A7C15ull + (uint64_t)d);
+    return std::min(OneMinusEpsilon, (float)(uint32_t)(h >> 32) * 0x1p-32f);
+}
 
 // sampling.rs:210-230
 static void concentric_sample_disk(const float *u, float *dx, float *dy) {

@@ -5,6 +5,7 @@
 #pragma once
 #include <vector>
 #include "bvh.h"
+#include "halton.h"
 
 namespace orc {
 
@@ -73,16 +74,28 @@ struct Camera {
     Ray generate_ray(float fx, float fy, float time) const;  // perspective.rs:103-126
 };
 
-// Counter-based random sampler: one PCG32 stream per (pixel, sample index),
-// consumed sequentially by get_1d/get_2d in pbrt's RandomSampler order.
+// The sampler of a camera path. Random (default): one PCG32 stream per
+// (pixel, sample index), consumed sequentially by get_1d/get_2d in pbrt's
+// RandomSampler order. Halton (start_halton): dimension d of the path's
+// global Halton index, d counting up from 0 (pbrt-v3 GlobalSampler); past the
+// 1000 prime bases, a hash of (index, d).
 struct Sampler {
     Rng rng;
+    const HaltonPixelMap *halton = nullptr;
+    int64_t index = 0;
+    int dim = 0;
     void start(uint64_t seed, uint64_t pixel, uint64_t sample);
-    float get_1d() { return rng.uniform_float(); }
-    void get_2d(float *u) {
-        u[0] = rng.uniform_float();
-        u[1] = rng.uniform_float();
+    void start_halton(const HaltonPixelMap *h, int px, int py, int64_t sample) {
+        halton = h;
+        index = h->index_for_sample(px, py, sample);
+        dim = 0;
     }
+    float get_1d() { return halton ? halton_next() : rng.uniform_float(); }
+    void get_2d(float *u) {
+        u[0] = get_1d();
+        u[1] = get_1d();
+    }
+    float halton_next();
 };
 uint64_t sampler_stream(uint64_t seed, uint64_t pixel, uint64_t sample);
 

@@ -103,15 +103,20 @@ void orc_render_scene_destroy(void *h) { delete (RenderScene *)h; }
 // sample index in [sample_offset, sample_offset + spp) trace one camera path,
 // apply the NaN/negative/inf radiance guard (113-122) and add L to
 // rgb_sum[pixel] (box filter of radius 0.5: weight 1 for the pixel the
-// sample lands in). stats: [0] closest-hit rays, [1] any-hit rays.
+// sample lands in). flags bit 0: the Halton sampler (pbrt-v3 HaltonSampler over
+// the image) instead of the random streams. stats: [0] closest-hit rays, [1]
+// any-hit rays.
 void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t sample_offset,
-                float *rgb_sum, uint64_t *stats, int n_threads) {
+                uint32_t flags, float *rgb_sum, uint64_t *stats, int n_threads) {
     RenderScene *s = (RenderScene *)h;
     const OrcCamera *c = (const OrcCamera *)camera;
     Camera cam;
     cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
              V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
     uint64_t n_pix = (uint64_t)c->width * c->height;
+    HaltonPixelMap hmap;
+    hmap.init((int)c->width, (int)c->height);
+    const bool halton = (flags & 1u) != 0;
     int nt = std::max(1, n_threads);
     std::vector<RenderStats> st(nt);
     std::atomic<uint64_t> next(0);
@@ -125,7 +130,8 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
                 RGB sum;
                 for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
                     Sampler sampler;
-                    sampler.start(seed, pix, si);
+                    if (halton) sampler.start_halton(&hmap, x, y, si);
+                    else sampler.start(seed, pix, si);
                     // sampler.rs:78-84 get_camera_sample: p_film, time, p_lens
                     float uf[2], ul[2];
                     sampler.get_2d(uf);
@@ -167,7 +173,8 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
 // take sample rows t, t + n, ... into private films summed in thread order, so
 // the result does not depend on timing.
 int orc_render_filtered(void *h, const void *camera, const void *filter_desc, uint32_t spp, uint32_t max_depth,
-                        uint64_t seed, uint32_t sample_offset, float *rgbw, uint64_t *stats, int n_threads) {
+                        uint64_t seed, uint32_t sample_offset, uint32_t flags, float *rgbw, uint64_t *stats,
+                        int n_threads) {
     RenderScene *s = (RenderScene *)h;
     const OrcCamera *c = (const OrcCamera *)camera;
     Filter f;
@@ -181,6 +188,9 @@ int orc_render_filtered(void *h, const void *camera, const void *filter_desc, ui
     int x0, y0, x1, y1;
     film_sample_bounds(ff, W, H, &x0, &y0, &x1, &y1);
     const uint64_t wext = (uint64_t)(x1 - x0);
+    HaltonPixelMap hmap;
+    hmap.init(x1 - x0, y1 - y0);
+    const bool halton = (flags & 1u) != 0;
     const int nt = std::max(1, n_threads);
     std::vector<RenderStats> st(nt);
     std::vector<std::vector<float>> films(nt, std::vector<float>((size_t)W * H * 4, 0.f));
@@ -190,7 +200,8 @@ int orc_render_filtered(void *h, const void *camera, const void *filter_desc, ui
                 const uint64_t pix = (uint64_t)(y - y0) * wext + (uint64_t)(x - x0);
                 for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
                     Sampler sampler;
-                    sampler.start(seed, pix, si);
+                    if (halton) sampler.start_halton(&hmap, x, y, si);
+                    else sampler.start(seed, pix, si);
                     float uf[2], ul[2];
                     sampler.get_2d(uf);
                     float time = sampler.get_1d();
@@ -276,6 +287,44 @@ void orc_camera_rays(const void *camera, const float *film_xy_time, uint32_t n, 
         o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;
         o[6] = r.t_max; o[7] = r.time;
     }
+}
+
+// Halton probes: radical inverse (scrambled with base_index's permutation when
+// scrambled != 0) of n indices; the permutation of a base; the global index
+// and first n dimensions of sample `sample` of pixel (px, py) for a sample
+// bounds of res_x x res_y.
+void orc_radical_inverse(int base_index, const uint64_t *a, uint32_t n, int scrambled, float *out) {
+    const HaltonTables &t = halton_tables();
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = scrambled ? scrambled_radical_inverse(base_index, a[i], t.perms.data() + t.prime_sums[base_index])
+                           : radical_inverse(base_index, a[i]);
+}
+void orc_halton_perm(int base_index, uint16_t *out, uint32_t *prime) {
+    const HaltonTables &t = halton_tables();
+    *prime = t.primes[base_index];
+    for (uint32_t j = 0; j < t.primes[base_index]; ++j) out[j] = t.perms[t.prime_sums[base_index] + j];
+}
+uint64_t orc_inverse_radical_inverse(uint32_t base, uint64_t inverse, int n_digits) {
+    return inverse_radical_inverse(base, inverse, n_digits);
+}
+int64_t orc_halton_sample(int res_x, int res_y, int px, int py, int64_t sample, uint32_t n, float *out) {
+    HaltonPixelMap m;
+    m.init(res_x, res_y);
+    Sampler s;
+    s.start_halton(&m, px, py, sample);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.get_1d();
+    return m.index_for_sample(px, py, sample);
+}
+void orc_halton_map(int res_x, int res_y, int64_t *out) {
+    HaltonPixelMap m;
+    m.init(res_x, res_y);
+    out[0] = m.exp[0];
+    out[1] = m.exp[1];
+    out[2] = m.scale[0];
+    out[3] = m.scale[1];
+    out[4] = m.stride;
+    out[5] = m.mult_inv[0];
+    out[6] = m.mult_inv[1];
 }
 
 // Stream probe: n uniform floats of the sampler stream for (seed, pixel, sample).

@@ -1,0 +1,118 @@
+// Halton sampler on the device: the radical-inverse primitives of
+// core/low_discrepancy.rs (radical_inverse 422-439/466-, scrambled 441-463,
+// inverse_radical_inverse 196-205) and pbrt-v3's HaltonSampler pixel-to-index
+// map (GetIndexForSample / SampleDimension, PBR book 3e 7.4), in the
+// reference's float32 operation order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace illume {
+
+constexpr uint32_t kHaltonDims = 1000;  // PRIME_TABLE_SIZE (low_discrepancy.rs:18)
+constexpr int kHaltonMaxRes = 128;      // pbrt-v3 HaltonSampler kMaxResolution
+
+// Device table: [prime, prefix sum of primes] for the 1000 bases, then the
+// digit permutations (uint16) of every base back to back.
+__device__ __forceinline__ const uint16_t *halton_perms(const uint32_t *table) {
+    return reinterpret_cast<const uint16_t *>(table + 2 * kHaltonDims);
+}
+
+// Per-launch pixel map of a sample-bounds resolution (HaltonSampler::new).
+struct HaltonLaunch {
+    const uint32_t *table;  // null: random sampler
+    int32_t exp0, exp1;
+    int64_t scale0, scale1, stride, inv0, inv1;
+};
+
+// a / base and a % base, with a 32-bit path for indices below 2^32.
+__device__ __forceinline__ void divmod(uint64_t a, uint32_t base, uint64_t &q, uint32_t &r) {
+    if ((a >> 32) == 0) {
+        const uint32_t a32 = (uint32_t)a, q32 = a32 / base;
+        q = q32;
+        r = a32 - q32 * base;
+    } else {
+        q = a / base;
+        r = (uint32_t)(a - q * base);
+    }
+}
+
+__device__ __forceinline__ float radical_inverse_base2(uint64_t a) {
+    return (float)__brevll(a) * 0x1p-64f;  // low_discrepancy.rs:468-474
+}
+
+__device__ __forceinline__ float radical_inverse_prime(uint32_t base, uint64_t a) {
+    const float inv_base = 1.f / (float)base;
+    uint64_t reversed = 0;
+    float inv_base_n = 1.f;
+    while (a != 0) {
+        uint64_t next;
+        uint32_t digit;
+        divmod(a, base, next, digit);
+        reversed = reversed * base + digit;
+        inv_base_n *= inv_base;
+        a = next;
+    }
+    return fminf((float)reversed * inv_base_n, 0x1.fffffep-1f);
+}
+
+__device__ __forceinline__ float scrambled_radical_inverse(uint32_t base, const uint16_t *__restrict__ perm,
+                                                           uint64_t a) {
+    const float inv_base = 1.f / (float)base;
+    uint64_t reversed = 0;
+    float inv_base_n = 1.f;
+    while (a != 0) {
+        uint64_t next;
+        uint32_t digit;
+        divmod(a, base, next, digit);
+        reversed = reversed * base + perm[digit];
+        inv_base_n *= inv_base;
+        a = next;
+    }
+    return fminf(inv_base_n * ((float)reversed + inv_base * (float)perm[0] / (1.f - inv_base)), 0x1.fffffep-1f);
+}
+
+__device__ __forceinline__ uint64_t inverse_radical_inverse(uint32_t base, uint64_t inverse, int n_digits) {
+    uint64_t index = 0;
+    for (int i = 0; i < n_digits; ++i) {
+        const uint64_t digit = inverse % base;
+        inverse /= base;
+        index = index * base + digit;
+    }
+    return index;
+}
+
+__device__ __forceinline__ int64_t mod_pos(int64_t a, int64_t b) {
+    const int64_t r = a - (a / b) * b;
+    return r < 0 ? r + b : r;
+}
+
+// GetIndexForSample: the global index of sample `sample_num` of pixel (px, py).
+__device__ __forceinline__ int64_t halton_index(const HaltonLaunch &h, int px, int py, int64_t sample_num) {
+    int64_t offset = 0;
+    if (h.stride > 1) {
+        const uint64_t d0 = inverse_radical_inverse(2, (uint64_t)mod_pos(px, kHaltonMaxRes), h.exp0);
+        const uint64_t d1 = inverse_radical_inverse(3, (uint64_t)mod_pos(py, kHaltonMaxRes), h.exp1);
+        offset = (int64_t)d0 * (h.stride / h.scale0) * h.inv0;
+        offset += (int64_t)d1 * (h.stride / h.scale1) * h.inv1;
+        offset %= h.stride;
+    }
+    return offset + sample_num * h.stride;
+}
+
+// SampleDimension for dimensions >= 2 (scrambled); past the prime table a
+// hash of (index, dim).
+__device__ __forceinline__ float halton_dim(const uint32_t *__restrict__ table, uint64_t index, uint32_t dim) {
+    if (dim < kHaltonDims) {
+        const uint32_t base = table[2 * dim], off = table[2 * dim + 1];
+        return scrambled_radical_inverse(base, halton_perms(table) + off, index);
+    }
+    uint64_t z = index * 0x9E3779B97F4A7C15ull + dim;  // splitmix64
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return fminf((float)(uint32_t)(z >> 32) * 0x1p-32f, 0x1.fffffep-1f);
+}
+
+}  // namespace illume

@@ -355,17 +355,31 @@ struct PathState {
     float eta_scale;
     uint32_t bounces;
     bool specular, active;
-    Pcg32 rng;
+    PathSampler rng;
 };
 
 // The camera-sample stage of SamplerIntegratorBase::render: sampler stream,
 // get_camera_sample (film 2D, time 1D, lens 2D) and the camera ray, for
 // sample s of the sample pixel (x, y) whose sampler stream is numbered
-// `stream`; (fx, fy) gets the film position of the camera sample.
+// `stream`; (fx, fy) gets the film position of the camera sample. With a
+// Halton table the film offset is dimensions 0 and 1 of the pixel's global
+// index (pbrt-v3 HaltonSampler::SampleDimension) and later dimensions follow.
 __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t seed, uint64_t stream, int x, int y,
-                                              uint32_t s, PathState &ps, float &fx, float &fy) {
-    ps.rng.seed(seed, stream, s);
-    const float uf0 = ps.rng.get_1d(), uf1 = ps.rng.get_1d();
+                                              uint32_t s, PathState &ps, float &fx, float &fy,
+                                              const HaltonLaunch &hl) {
+    float uf0, uf1;
+    if (hl.table) {
+        const int64_t idx = halton_index(hl, x, y, (int64_t)s);
+        uf0 = radical_inverse_base2((uint64_t)idx >> hl.exp0);
+        uf1 = radical_inverse_prime(3, (uint64_t)idx / (uint64_t)hl.scale1);
+        ps.rng.halton = hl.table;
+        ps.rng.state = (uint64_t)idx;
+        ps.rng.inc = 2;
+    } else {
+        ps.rng.seed(seed, stream, s);
+        uf0 = ps.rng.get_1d();
+        uf1 = ps.rng.get_1d();
+    }
     ps.rng.get_1d();
     ps.rng.get_1d();
     ps.rng.get_1d();
@@ -384,7 +398,7 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
 __device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
                                            PathState &ps) {
     float fx, fy;
-    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy);
+    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, HaltonLaunch{});
 }
 
 // One bounce of PathIntegrator::li after the closest-hit query (found, h).

@@ -30,7 +30,8 @@ constexpr int kRenderBlock = 128;
 template <int BLOCK, bool FILTERED>
 __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
                                                         uint64_t seed, uint32_t sample_offset, float *__restrict__ film,
-                                                        unsigned long long *__restrict__ counters, FilmSplat fs) {
+                                                        unsigned long long *__restrict__ counters, FilmSplat fs,
+                                                        HaltonLaunch hl) {
     __shared__ int lds_stack[kStackLds * BLOCK];
     __shared__ uint32_t blk_counts[2];
     if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
@@ -46,9 +47,10 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
         PathState ps;
         float fx, fy;
         if (FILTERED)
-            path_start_at(cam, seed, pix, fs.x0 + (int)(pix % fs.wext), fs.y0 + (int)(pix / fs.wext), s, ps, fx, fy);
+            path_start_at(cam, seed, pix, fs.x0 + (int)(pix % fs.wext), fs.y0 + (int)(pix / fs.wext), s, ps, fx, fy,
+                          hl);
         else
-            path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy);
+            path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, hl);
         GuideCfg gc{};
         uint32_t n_recs = 0;
         while (ps.active) {
@@ -97,25 +99,26 @@ __global__ __launch_bounds__(256) void resolve_kernel(const float *__restrict__ 
 }  // namespace
 
 hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
-                         uint32_t sample_offset, float *rgb, unsigned long long *counters, hipStream_t stream) {
+                         uint32_t sample_offset, float *rgb, unsigned long long *counters, const HaltonLaunch &hl,
+                         hipStream_t stream) {
     const uint64_t n = (uint64_t)cam.width * cam.height * spp;
     if (n == 0) return hipSuccess;
     const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
     if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
     render_kernel<kRenderBlock, false><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
-        sc, cam, spp, max_depth, seed, sample_offset, rgb, counters, FilmSplat{});
+        sc, cam, spp, max_depth, seed, sample_offset, rgb, counters, FilmSplat{}, hl);
     return hipGetLastError();
 }
 
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
                                   uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
-                                  unsigned long long *counters, hipStream_t stream) {
+                                  unsigned long long *counters, const HaltonLaunch &hl, hipStream_t stream) {
     const uint64_t n = (uint64_t)fs.wext * fs.hext * spp;
     if (n == 0) return hipSuccess;
     const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
     if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
     render_kernel<kRenderBlock, true><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
-        sc, cam, spp, max_depth, seed, sample_offset, rgbw, counters, fs);
+        sc, cam, spp, max_depth, seed, sample_offset, rgbw, counters, fs, hl);
     return hipGetLastError();
 }
 

@@ -409,9 +409,13 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
         return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
     if ((uint64_t)camera->width * camera->height >= (1ull << 40))
         return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+    if (cfg->flags & ~ILLUME_RENDER_HALTON) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
     if (scene->n_prims == 0) return ILLUME_OK;  // nothing to hit, no lights: black
+    HaltonLaunch hl{};
+    if ((cfg->flags & ILLUME_RENDER_HALTON) && !halton_launch((int)camera->width, (int)camera->height, &hl))
+        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "Halton tables");
     hipError_t e = launch_render(scene->dev, cam, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset, rgb_sum,
-                                 (unsigned long long *)ray_counts, (hipStream_t)stream);
+                                 (unsigned long long *)ray_counts, hl, (hipStream_t)stream);
     if (e != hipSuccess) return abi_hip_fail(e, "render launch");
     return ILLUME_OK;
 }
@@ -429,9 +433,13 @@ int illume_render_filtered(const illume_scene *scene, const illume_camera *camer
     FilmSplat fs;
     filter_splat(filter, camera->width, camera->height, &fs);
     if ((uint64_t)fs.wext * fs.hext >= (1ull << 40)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+    if (cfg->flags & ~ILLUME_RENDER_HALTON) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
     if (scene->n_prims == 0) return ILLUME_OK;
+    HaltonLaunch hl{};
+    if ((cfg->flags & ILLUME_RENDER_HALTON) && !halton_launch((int)fs.wext, (int)fs.hext, &hl))
+        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "Halton tables");
     hipError_t e = launch_render_filtered(scene->dev, cam, fs, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset,
-                                          rgbw_sum, (unsigned long long *)ray_counts, (hipStream_t)stream);
+                                          rgbw_sum, (unsigned long long *)ray_counts, hl, (hipStream_t)stream);
     if (e != hipSuccess) return abi_hip_fail(e, "filtered render launch");
     return ILLUME_OK;
 }

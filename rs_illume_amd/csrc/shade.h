@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "geom.h"
+#include "halton.h"
 #include "layout.h"
 #include "scene_types.h"
 
@@ -18,17 +19,20 @@ constexpr float kInvPi = 0.31830988618379067154f;
 constexpr float kOneMinusEpsilon = 0x1.fffffep-1f;
 
 // ---------------------------------------------------------------- sampler
-// PCG32 (core/rng.rs:9-63): one stream per (pixel, sample index), selected
-// by a splitmix64 hash of (seed, pixel, sample); dimensions are consumed
-// sequentially in pbrt's RandomSampler order.
+// The sampler of one camera path. Random (halton == null): PCG32
+// (core/rng.rs:9-63), one stream per (pixel, sample index) selected by a
+// splitmix64 hash of (seed, pixel, sample); dimensions are consumed
+// sequentially in pbrt's RandomSampler order. Halton: state is the path's
+// global Halton index, inc the next dimension (halton.h).
 __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ULL;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
     x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
     return x ^ (x >> 31);
 }
-struct Pcg32 {
+struct PathSampler {
     uint64_t state, inc;
+    const uint32_t *halton;
     __device__ __forceinline__ uint32_t next() {
         uint64_t old = state;
         state = old * 0x5851f42d4c957f2dULL + inc;
@@ -37,6 +41,7 @@ struct Pcg32 {
         return (xs >> rot) | (xs << ((~rot + 1u) & 31));
     }
     __device__ __forceinline__ void seed(uint64_t seed, uint64_t pixel, uint64_t sample) {
+        halton = nullptr;
         uint64_t seq = splitmix64(seed ^ splitmix64((pixel << 24) | (sample & 0xffffff)));
         state = 0u;
         inc = (seq << 1u) | 1u;
@@ -44,7 +49,10 @@ struct Pcg32 {
         state += 0x853c49e6748fea9bULL;
         next();
     }
-    __device__ __forceinline__ float get_1d() { return fminf(kOneMinusEpsilon, (float)next() * 0x1p-32f); }
+    __device__ __forceinline__ float get_1d() {
+        if (halton) return halton_dim(halton, state, (uint32_t)inc++);
+        return fminf(kOneMinusEpsilon, (float)next() * 0x1p-32f);
+    }
 };
 
 // ---------------------------------------------------------------- helpers

@@ -150,6 +150,8 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
         const illume_job &in = jobs[j];
         if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
         if (in.cfg.max_depth > wf->max_depth) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth above the pool's");
+        if (in.cfg.flags != 0)
+            return abi_fail(ILLUME_ERR_UNSUPPORTED, "the wavefront renderer uses the random sampler (flags 0)");
         if ((uint64_t)in.cfg.sample_offset + in.cfg.spp > (1u << 24))
             return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
         WfJob job;

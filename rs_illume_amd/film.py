@@ -110,8 +110,8 @@ def render_filtered(integrator, scene, filt, rgbw=None, spp=None, sample_offset=
         rgbw = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
     if rgbw.shape != (h, w, 4) or rgbw.dtype != torch.float32 or not rgbw.is_cuda or not rgbw.is_contiguous():
         raise ValueError("rgbw must be a contiguous cuda float32 tensor of shape (%d, %d, 4)" % (h, w))
-    cfg = RenderCfg(integrator.spp if spp is None else int(spp), integrator.max_depth, int(sample_offset), 0,
-                    integrator.seed)
+    cfg = RenderCfg(integrator.spp if spp is None else int(spp), integrator.max_depth, int(sample_offset),
+                    integrator.flags, integrator.seed)
     rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
     _lib.check(_declare().illume_render_filtered(scene.handle, _p(integrator.camera), ctypes.byref(cfg), filt.handle,
                                                  ctypes.c_void_p(rgbw.data_ptr()), rc, _stream_handle(stream)),

@@ -22,8 +22,9 @@ Two layers, as in the reference's core/parser:
   pbrt-v3's scene API (api.cpp) for the subset the GPU path tracer renders.
 
 Loaded: LookAt + perspective camera (fov), image film (resolution,
-filename), box/triangle/gaussian/mitchell/sinc pixel filters, any sampler's
-pixelsamples, the path integrator's maxdepth,
+filename), box/triangle/gaussian/mitchell/sinc pixel filters, the halton and
+random samplers (other samplers render with Halton) and pixelsamples, the path
+integrator's maxdepth,
 Identity/Translate/Scale/Rotate/Transform/ConcatTransform,
 CoordinateSystem/CoordSysTransform, ReverseOrientation, matte/metal/glass
 materials (constant textures allowed), named materials, diffuse area lights on
@@ -449,8 +450,9 @@ def write_ply(path, verts, tris):
 class Loaded:
     """Result of :func:`load`: ``desc`` (scenes.SceneDesc), ``camera``
     (scenes.camera record), ``spp``, ``max_depth``, the film ``filename``, the
-    pixel ``filter`` (film.filter_desc record, None for pbrt-v3's default box)
-    and the warnings for directives and parameters that were skipped."""
+    pixel ``filter`` (film.filter_desc record, None for pbrt-v3's default box),
+    the ``sampler`` ("halton", pbrt-v3's default, or "random") and the
+    warnings for directives and parameters that were skipped."""
 
     def __init__(self):
         self.desc = S.SceneDesc()
@@ -459,6 +461,7 @@ class Loaded:
         self.max_depth = 5
         self.filename = None
         self.filter = None  # film.filter_desc record; None: box of radius 0.5
+        self.sampler = "halton"  # pbrt-v3's default sampler
         self.warnings = []
 
 
@@ -767,6 +770,12 @@ class _Loader:
             self.check_unused(d, used)
         elif n == "Sampler":
             used = set()
+            kind = d.args[0]
+            if kind in ("halton", "random"):
+                self.out.sampler = kind
+            else:
+                self.out.sampler = "halton"
+                self.warn(d, "sampler %r rendered with the Halton sampler" % kind)
             self.out.spp = self.one(d, "pixelsamples", 16, used, int)
             if self.out.spp <= 0:
                 raise PbrtError("%s: pixelsamples must be positive" % d.where())
@@ -982,7 +991,7 @@ def _filter_text(f):
     return out
 
 
-def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=None):
+def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=None, sampler="random"):
     """pbrt-v3 text of a flat scene. Triangles keep their order: each run of
     triangles with one material and emission becomes one trianglemesh (its
     own vertices, in first-use order); spheres become translated spheres; the
@@ -996,7 +1005,7 @@ def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=No
     lines = ["LookAt %s  %s  %s" % (" ".join(_f(v) for v in c["pos"]), " ".join(_f(v) for v in c["look"]),
                                     " ".join(_f(v) for v in c["up"])),
              'Camera "perspective" "float fov" [ %s ]' % _f(c["fov"]), film,
-             'Sampler "random" "integer pixelsamples" [ %d ]' % spp,
+             'Sampler "%s" "integer pixelsamples" [ %d ]' % (sampler, spp),
              'Integrator "path" "integer maxdepth" [ %d ]' % max_depth]
     if pixel_filter is not None:
         lines.append(_filter_text(pixel_filter))
@@ -1038,8 +1047,9 @@ def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=No
 
 def render(loaded, spp=None, wavefront=False, seed=0, builder="SAH"):
     """Render a loaded scene on the GPU: (H, W, 3) float32 tensor of the
-    pixel values (mean radiance through the scene's pixel filter). The
-    wavefront renderer films with the default box filter only."""
+    pixel values (mean radiance through the scene's pixel filter, with the
+    scene's sampler). The wavefront renderer films with the default box
+    filter and the random sampler only."""
     from .render import PathIntegrator, Scene, WavefrontPathIntegrator
     spp = loaded.spp if spp is None else spp
     sc = Scene(loaded.desc, builder=builder)
@@ -1047,13 +1057,14 @@ def render(loaded, spp=None, wavefront=False, seed=0, builder="SAH"):
         if wavefront:
             raise PbrtError("the wavefront renderer films with the box filter of radius 0.5 only")
         from .film import Filter, render_filtered, resolve
-        pi = PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed)
+        pi = PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed, sampler=loaded.sampler)
         f = loaded.filter.reshape(-1)[0]
         return resolve(render_filtered(pi, sc, Filter(int(f["type"]), radius=tuple(f["radius"]), alpha=f["alpha"],
                                                       b=f["b"], c=f["c"], tau=f["tau"])))
     if wavefront:
         return WavefrontPathIntegrator(sc, loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed).render()
-    return PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed).render(sc)
+    return PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed,
+                          sampler=loaded.sampler).render(sc)
 
 
 def main(argv=None):

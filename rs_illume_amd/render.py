@@ -16,6 +16,10 @@ from .accel import _stream_handle
 from .scenes import CAMERA_DTYPE, MATERIAL_DTYPE
 
 
+# illume_render_cfg.flags per sampler: PCG32 streams, or pbrt-v3's Halton sampler
+SAMPLERS = {"random": 0, "halton": 1}
+
+
 class RenderCfg(ctypes.Structure):
     _fields_ = [("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("sample_offset", ctypes.c_uint32),
                 ("flags", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
@@ -117,11 +121,18 @@ class PathIntegrator:
     """Integrator::render with pbrt-v3's PathIntegrator::li (max_depth,
     Russian roulette after 3 bounces, one light sample per vertex with MIS)."""
 
-    def __init__(self, camera, spp=16, max_depth=5, seed=0):
+    def __init__(self, camera, spp=16, max_depth=5, seed=0, sampler="random"):
         self.camera = np.ascontiguousarray(camera, CAMERA_DTYPE).reshape(1)
         self.spp = int(spp)
         self.max_depth = int(max_depth)
         self.seed = int(seed)
+        if sampler not in SAMPLERS:
+            raise ValueError("sampler must be one of %s" % (tuple(SAMPLERS),))
+        self.sampler = sampler
+
+    @property
+    def flags(self):
+        return SAMPLERS[self.sampler]
 
     @property
     def resolution(self):
@@ -133,7 +144,8 @@ class PathIntegrator:
         w, h = self.resolution
         if rgb_sum is None:
             rgb_sum = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
-        cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), 0, self.seed)
+        cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), self.flags,
+                        self.seed)
         rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
         _lib.check(_declare().illume_render(scene.handle, _p(self.camera), ctypes.byref(cfg),
                                             ctypes.c_void_p(rgb_sum.data_ptr()), rc, _stream_handle(stream)),
@@ -284,7 +296,8 @@ class WavefrontPathIntegrator(PathIntegrator):
         if rgb_sum is None:
             rgb_sum = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
         sc = scene if scene is not None else self.scene
-        cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), 0, self.seed)
+        cfg = RenderCfg(self.spp if spp is None else int(spp), self.max_depth, int(sample_offset), self.flags,
+                        self.seed)
         rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
         g = self.guide.handle if self.guide is not None else None
         _lib.check(_declare_wf().illume_wavefront_render(self.wf.handle, sc.handle, _p(self.camera), ctypes.byref(cfg),
