@@ -24,7 +24,7 @@ int hip_fail(hipError_t e, const char *what) {
 }  // namespace
 
 struct illume_bvh {
-    void *d_nodes = nullptr;  // GpuNode[] (width 2) or GpuNode4[] (width 4)
+    void *d_nodes = nullptr;  // GpuNode[] (width 2) or GpuNode4Q[] (width 4)
     illume::PrimRecord *d_prims = nullptr;
     int width = 2;
     illume_bvh_info_t info;
@@ -92,9 +92,14 @@ int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *t
     std::memset(&h->info, 0, sizeof(h->info));
     h->width = width;
     hipGetDevice(&h->device);
-    const void *nsrc = width == 4 ? (const void *)bo.nodes4.data() : (const void *)bo.nodes.data();
-    size_t n_nodes = width == 4 ? bo.nodes4.size() : bo.nodes.size();
-    size_t nb = width == 4 ? n_nodes * sizeof(illume::GpuNode4) : n_nodes * sizeof(illume::GpuNode);
+    std::vector<illume::GpuNode4Q> nq;
+    if (width == 4 && !illume::quantize_width4(bo.nodes4, &nq)) {
+        delete h;
+        return fail(ILLUME_ERR_INVALID_ARG, "node boxes cannot be quantized (non-finite bounds)");
+    }
+    const void *nsrc = width == 4 ? (const void *)nq.data() : (const void *)bo.nodes.data();
+    size_t n_nodes = width == 4 ? nq.size() : bo.nodes.size();
+    size_t nb = width == 4 ? n_nodes * sizeof(illume::GpuNode4Q) : n_nodes * sizeof(illume::GpuNode);
     size_t pb = bo.prims.size() * sizeof(illume::PrimRecord);
     hipError_t e;
     if (nb) {
@@ -177,9 +182,14 @@ int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *
             return hip_fail(e, "hipMemcpy(binary nodes)");
         }
         depth = illume::collapse_to_width4(n2, &n4);
-        nb = n4.size() * sizeof(illume::GpuNode4);
+        std::vector<illume::GpuNode4Q> nq;
+        if (!illume::quantize_width4(n4, &nq)) {
+            illume_bvh_destroy(h);
+            return fail(ILLUME_ERR_INVALID_ARG, "node boxes cannot be quantized (non-finite bounds)");
+        }
+        nb = nq.size() * sizeof(illume::GpuNode4Q);
         if ((e = hipMalloc(&h->d_nodes, nb)) == hipSuccess)
-            e = hipMemcpy(h->d_nodes, n4.data(), nb, hipMemcpyHostToDevice);
+            e = hipMemcpy(h->d_nodes, nq.data(), nb, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             illume_bvh_destroy(h);
             return hip_fail(e, "four-child nodes");

@@ -489,4 +489,70 @@ uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4
     return max_depth;
 }
 
+namespace {
+// plane of byte q: origin + q * step, float32, product exact, sum rounded
+inline float q_plane(float origin, float step, int q) { return origin + (float)q * step; }
+inline float pow2f(int e) { return std::ldexp(1.f, e); }
+}  // namespace
+
+bool quantize_width4(const std::vector<GpuNode4> &n4, std::vector<GpuNode4Q> *out) {
+    out->resize(n4.size());
+    for (size_t i = 0; i < n4.size(); ++i) {
+        const GpuNode4 &g = n4[i];
+        GpuNode4Q q;
+        std::memset(&q, 0, sizeof(q));
+        const float *lo[3] = {g.lo_x, g.lo_y, g.lo_z}, *hi[3] = {g.hi_x, g.hi_y, g.hi_z};
+        uint8_t *qlo[3] = {q.lo_x, q.lo_y, q.lo_z}, *qhi[3] = {q.hi_x, q.hi_y, q.hi_z};
+        float *origin[3] = {&q.ox, &q.oy, &q.oz};
+        uint8_t *expo[3] = {&q.ex, &q.ey, &q.ez};
+        bool used[4];
+        for (int c = 0; c < 4; ++c) {
+            used[c] = g.child[c] != kEmptyChild;
+            q.child[c] = g.child[c];
+        }
+        for (int a = 0; a < 3; ++a) {
+            float mn = std::numeric_limits<float>::infinity(), mx = -mn;
+            for (int c = 0; c < 4; ++c)
+                if (used[c]) {
+                    if (!std::isfinite(lo[a][c]) || !std::isfinite(hi[a][c])) return false;
+                    mn = std::min(mn, lo[a][c]);
+                    mx = std::max(mx, hi[a][c]);
+                }
+            if (!(mn <= mx)) {  // no used child: any valid encoding
+                mn = mx = 0.f;
+            }
+            *origin[a] = mn;
+            // smallest power-of-two step whose 255th plane reaches mx
+            int e = -126;
+            const double ext = (double)mx - (double)mn;
+            if (ext > 0) e = std::max(-126, (int)std::ceil(std::log2(ext / 255.0)));
+            while (e < 127 && q_plane(mn, pow2f(e), 255) < mx) ++e;
+            if (q_plane(mn, pow2f(e), 255) < mx) return false;
+            *expo[a] = (uint8_t)(e + 127);
+            const float step = pow2f(e);
+            for (int c = 0; c < 4; ++c) {
+                if (!used[c]) {
+                    qlo[a][c] = 255;
+                    qhi[a][c] = 0;
+                    continue;
+                }
+                // largest q with plane(q) <= lo, smallest q with plane(q) >= hi
+                int ql = (int)std::floor(((double)lo[a][c] - (double)mn) / (double)step);
+                ql = std::min(255, std::max(0, ql));
+                while (ql > 0 && q_plane(mn, step, ql) > lo[a][c]) --ql;
+                while (ql < 255 && q_plane(mn, step, ql + 1) <= lo[a][c]) ++ql;
+                int qh = (int)std::ceil(((double)hi[a][c] - (double)mn) / (double)step);
+                qh = std::min(255, std::max(0, qh));
+                while (qh < 255 && q_plane(mn, step, qh) < hi[a][c]) ++qh;
+                while (qh > 0 && q_plane(mn, step, qh - 1) >= hi[a][c]) --qh;
+                if (q_plane(mn, step, ql) > lo[a][c] || q_plane(mn, step, qh) < hi[a][c]) return false;
+                qlo[a][c] = (uint8_t)ql;
+                qhi[a][c] = (uint8_t)qh;
+            }
+        }
+        (*out)[i] = q;
+    }
+    return true;
+}
+
 }  // namespace illume

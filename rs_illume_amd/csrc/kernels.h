@@ -33,7 +33,7 @@ int max_supported_depth();
 
 // Closest hit (hits), any hit (occluded) or counting traversal (counts != null).
 // `empty` = the BVH has no primitives: outputs are filled with misses.
-// nodes: GpuNode (width 2) or GpuNode4 (width 4) array.
+// nodes: GpuNode (width 2) or GpuNode4Q (width 4) array.
 hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims, bool empty, const illume_ray *rays,
                            uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
                            hipStream_t stream);

@@ -22,17 +22,34 @@ struct alignas(64) GpuNode {
 };
 static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
 
-// 128-byte four-child node: the six slabs of the four children as float4
-// rows (lo_x, hi_x, lo_y, hi_y, lo_z, hi_z), then the four child references.
-// Unused child slots have all planes at +inf (never entered) and an empty
-// leaf reference. Half the depth of the binary tree: half the dependent
-// node fetches per ray.
+// 128-byte four-child node, the builder's form (the device gets GpuNode4Q):
+// the six slabs of the four children as float4 rows (lo_x, hi_x, lo_y, hi_y,
+// lo_z, hi_z), then the four child references. Unused child slots have all
+// planes at +inf and an empty leaf reference. Half the depth of the binary
+// tree: half the dependent node fetches per ray.
 struct alignas(128) GpuNode4 {
     float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
     int32_t child[4];
     uint32_t pad[4];
 };
 static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
+
+// 64-byte four-child node with quantized child boxes (one cache line per
+// visit instead of two). A child plane is origin + q * 2^(e - 127) for a byte
+// q, evaluated in float32 (the product is exact, the sum rounds to nearest);
+// the builder picks q so that every plane evaluated this way lies on or
+// outside the child's float box, so the dequantized box contains it and the
+// slab test on it enters every box the unquantized test enters. Unused slots
+// hold kEmptyChild and are never entered.
+struct alignas(64) GpuNode4Q {
+    float ox, oy, oz;
+    uint8_t ex, ey, ez, pad0;  // biased exponents of the three quantization steps
+    uint8_t lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    uint32_t pad1[2];
+};
+static_assert(sizeof(GpuNode4Q) == 64, "GpuNode4Q must be 64 bytes");
+constexpr int32_t kEmptyChild = -1;  // ~(0 << 4 | 0): a leaf of no primitives
 
 // Leaf reference: ~((first_slot << 4) | count), count in 1..15.
 constexpr int kLeafCountBits = 4;

@@ -26,6 +26,10 @@ hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx,
 // references are kept. Returns the four-child depth.
 uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4> *n4);
 
+// The 64-byte quantized form of four-child nodes (same topology and child
+// references). False if a box is not finite or cannot be covered.
+bool quantize_width4(const std::vector<GpuNode4> &n4, std::vector<GpuNode4Q> *out);
+
 // Validation shared by both builders: "" or a description of the problem.
 std::string validate_prims(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
                            const float *spheres, uint32_t n_spheres);

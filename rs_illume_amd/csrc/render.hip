@@ -42,8 +42,9 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
     if (gid < n_pix * spp) {
         const uint64_t pix = gid % n_pix;
         const uint32_t s = (uint32_t)(gid / n_pix) + sample_offset;
+        int spill[kSpill];
         TraceStack<BLOCK> st;
-        st.reset(lds_stack + threadIdx.x);
+        st.reset(lds_stack + threadIdx.x, spill);
         PathState ps;
         float fx, fy;
         if (FILTERED)

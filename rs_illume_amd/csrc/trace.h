@@ -22,13 +22,18 @@ constexpr int kStackLds = 16;  // power of two
 constexpr int kSpill = 64;
 constexpr int kMaxTraceDepth = kStackLds + kSpill;
 
+// The overflow array is the caller's own local array, not a member: an
+// array indexed by a run-time value lives in scratch memory, and as a member
+// it would take the stack pointer and the LDS pointer with it (a scratch load
+// and store of sp on every push and pop).
 template <int BLOCK>
 struct TraceStack {
-    int *lds;  // this lane's column: lds[slot * BLOCK]
+    int *lds;    // this lane's column: lds[slot * BLOCK]
+    int *spill;  // this lane's kSpill-entry overflow
     int sp;
-    int spill[kSpill];
-    __device__ __forceinline__ void reset(int *column) {
+    __device__ __forceinline__ void reset(int *column, int *overflow) {
         lds = column;
+        spill = overflow;
         sp = 0;
     }
     __device__ __forceinline__ void push(int v) {
@@ -196,12 +201,25 @@ __device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
     ra = r;
 }
 
-// Traversal of the 4-wide tree: each visit tests four child boxes from one
-// 128-byte node, sorts the entered children by entry distance, descends into
-// the nearest and pushes the rest farthest-first. A stack of
-// kMaxTraceDepth entries holds 3 per level (the builder caps the depth).
+// Box of child C of a quantized node (words w1 = lo_x hi_x lo_y hi_y, w2.x/y =
+// lo_z hi_z; byte C of each) entered by the ray? Planes are origin + q * step.
+template <int C>
+__device__ __forceinline__ bool qslab(const uint4 &w1, const uint4 &w2, float ox, float oy, float oz, float sx,
+                                      float sy, float sz, f3 o, f3 inv, float t_max, float &t) {
+    constexpr int sh = 8 * C;
+    const float lx = ox + (float)((w1.x >> sh) & 0xffu) * sx, hx = ox + (float)((w1.y >> sh) & 0xffu) * sx;
+    const float ly = oy + (float)((w1.z >> sh) & 0xffu) * sy, hy = oy + (float)((w1.w >> sh) & 0xffu) * sy;
+    const float lz = oz + (float)((w2.x >> sh) & 0xffu) * sz, hz = oz + (float)((w2.y >> sh) & 0xffu) * sz;
+    return slab(lx, hx, ly, hy, lz, hz, o, inv, t_max, t);
+}
+
+// Traversal of the 4-wide tree: each visit reads one 64-byte quantized node,
+// tests the four dequantized child boxes, sorts the entered children by entry
+// distance, descends into the nearest and pushes the rest farthest-first. A
+// stack of kMaxTraceDepth entries holds 3 per level (the builder caps the
+// depth).
 template <bool ANY, bool COUNT, int BLOCK>
-__device__ __forceinline__ bool trace_ray4(const GpuNode4 *__restrict__ nodes, const PrimRecord *__restrict__ prims,
+__device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, const PrimRecord *__restrict__ prims,
                                            f3 o, f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st,
                                            uint32_t &n_box, uint32_t &n_prim) {
     const f3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
@@ -214,15 +232,19 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4 *__restrict__ nodes, c
     const float inf = __builtin_inff();
     while (true) {
         if (cur >= 0) {
-            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 ch = reinterpret_cast<const int4 *>(np)[6];
+            const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
             if (COUNT) n_box += 4;
+            const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+            const float sx = __uint_as_float((w0.w & 0xffu) << 23),
+                        sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
+                        sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+            const int4 ch = make_int4((int)w2.z, (int)w2.w, (int)w3.x, (int)w3.y);
             float t0, t1, t2, t3;
-            const bool h0 = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, t_max, t0);
-            const bool h1 = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, t_max, t1);
-            const bool h2 = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, t_max, t2);
-            const bool h3 = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, t_max, t3);
+            const bool h0 = qslab<0>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t0) && ch.x != kEmptyChild;
+            const bool h1 = qslab<1>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t1) && ch.y != kEmptyChild;
+            const bool h2 = qslab<2>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t2) && ch.z != kEmptyChild;
+            const bool h3 = qslab<3>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t3) && ch.w != kEmptyChild;
             float ta = h0 ? t0 : inf, tb = h1 ? t1 : inf, tc = h2 ? t2 : inf, td = h3 ? t3 : inf;
             int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
             const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;

@@ -32,14 +32,15 @@ __global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse
     if (i >= n) return;
     const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
     const float4 r0 = rp[0], r1 = rp[1];
+    int spill[kSpill];
     TraceStack<kBlock> st;
-    st.reset(lds_stack + threadIdx.x);
+    st.reset(lds_stack + threadIdx.x, spill);
     TraceHit h;
     uint32_t n_box = 0, n_prim = 0;
     const f3 o = mk(r0.x, r0.y, r0.z), d = mk(r0.w, r1.x, r1.y);
     const bool found =
         WIDTH == 4
-            ? trace_ray4<ANY, COUNT, kBlock>((const GpuNode4 *)nodes, prims, o, d, r1.z, h, st, n_box, n_prim)
+            ? trace_ray4<ANY, COUNT, kBlock>((const GpuNode4Q *)nodes, prims, o, d, r1.z, h, st, n_box, n_prim)
             : trace_ray<ANY, COUNT, kBlock>((const GpuNode *)nodes, prims, o, d, r1.z, h, st, n_box, n_prim);
     if (COUNT) {
         counts[2 * (size_t)i] = n_box;

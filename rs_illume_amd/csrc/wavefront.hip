@@ -134,8 +134,9 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
         const WfJob &job = w.jobs[w.job[p]];
         const DevScene &sc = *job.scene;
         const float4 a = w.o_tmax[p], b = w.d_eta[p];
+        int spill[kSpill];
         TraceStack<BLOCK> st;
-        st.reset(lds_stack + threadIdx.x);
+        st.reset(lds_stack + threadIdx.x, spill);
         TraceHit h;
         uint32_t nb = 0, np = 0;
         const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
@@ -201,8 +202,9 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
     if (i < n_in) {
         const uint32_t p = w.q[parity][i];
         const WfJob &job = w.jobs[w.job[p]];
+        int spill[kSpill];
         TraceStack<BLOCK> st;
-        st.reset(lds_stack + threadIdx.x);
+        st.reset(lds_stack + threadIdx.x, spill);
         PathState ps;
         load_state(w, p, ps);
         const float4 h0 = w.hit0[p], h1 = w.hit1[p];
