@@ -34,8 +34,8 @@ struct illume_bvh {
 namespace illume {
 int abi_fail(int status, const std::string &msg) { return fail(status, msg); }
 int abi_hip_fail(hipError_t e, const char *what) { return hip_fail(e, what); }
-void bvh_device_arrays(const illume_bvh *bvh, const GpuNode **nodes, const PrimRecord **prims) {
-    *nodes = bvh->width == 2 ? (const GpuNode *)bvh->d_nodes : nullptr;
+void bvh_device_arrays(const illume_bvh *bvh, const GpuNode4Q **nodes, const PrimRecord **prims) {
+    *nodes = bvh->width == 4 ? (const GpuNode4Q *)bvh->d_nodes : nullptr;
     *prims = bvh->d_prims;
 }
 }  // namespace illume

@@ -101,8 +101,10 @@ __device__ __forceinline__ int64_t halton_index(const HaltonLaunch &h, int px, i
 }
 
 // SampleDimension for dimensions >= 2 (scrambled); past the prime table a
-// hash of (index, dim).
-__device__ __forceinline__ float halton_dim(const uint32_t *__restrict__ table, uint64_t index, uint32_t dim) {
+// hash of (index, dim). Not inlined: the path code that draws samples is
+// shared with renderers that only use the random sampler, whose register
+// allocation must not carry the digit loops.
+__device__ __noinline__ float halton_dim(const uint32_t *__restrict__ table, uint64_t index, uint32_t dim) {
     if (dim < kHaltonDims) {
         const uint32_t base = table[2 * dim], off = table[2 * dim + 1];
         return scrambled_radical_inverse(base, halton_perms(table) + off, index);

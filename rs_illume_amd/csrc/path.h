@@ -20,7 +20,7 @@ __device__ __forceinline__ bool scene_closest(const DevScene &sc, f3 o, f3 d, fl
                                               TraceStack<BLOCK> &st, PathCounters &pc) {
     uint32_t nb = 0, np = 0;
     ++pc.closest;
-    return trace_ray<false, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
+    return trace_ray4<false, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
 }
 template <int BLOCK>
 __device__ __forceinline__ bool scene_any(const DevScene &sc, f3 o, f3 d, float t_max, TraceStack<BLOCK> &st,
@@ -28,7 +28,7 @@ __device__ __forceinline__ bool scene_any(const DevScene &sc, f3 o, f3 d, float 
     uint32_t nb = 0, np = 0;
     TraceHit h;
     ++pc.any;
-    return trace_ray<true, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
+    return trace_ray4<true, false, BLOCK>(sc.nodes, sc.prims, o, d, t_max, h, st, nb, np);
 }
 
 // Builds the interaction of a closest hit.
@@ -407,7 +407,7 @@ __device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, 
 // BSDF, weighted by frac * q + (1 - frac) * p_bsdf. recs (may be null) gets a
 // training vertex per sampled continuation; n_recs counts them.
 template <int BLOCK>
-__device__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
+__device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
                            const float *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
                            TraceStack<BLOCK> &st, PathCounters &pc) {
     if (recs && n_recs > 0 && recs[n_recs - 1].lsnap[0] < 0.f) {

@@ -13,7 +13,7 @@
 namespace illume {
 int abi_fail(int status, const std::string &msg);
 int abi_hip_fail(hipError_t e, const char *what);
-void bvh_device_arrays(const illume_bvh *bvh, const GpuNode **nodes, const PrimRecord **prims);
+void bvh_device_arrays(const illume_bvh *bvh, const GpuNode4Q **nodes, const PrimRecord **prims);
 }  // namespace illume
 
 using namespace illume;
@@ -256,8 +256,8 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     if (!s) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(&s->dev, 0, sizeof(s->dev));
     int st = builder == 1
-                 ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 2, nullptr, &s->bvh)
-                 : illume_bvh_create(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, &s->bvh);
+                 ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, nullptr, &s->bvh)
+                 : illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, &s->bvh);
     if (st != ILLUME_OK) {
         delete s;
         return st;

@@ -24,7 +24,7 @@ struct DevLight {  // one-sided DiffuseAreaLight on a triangle
 };
 
 struct DevScene {
-    const GpuNode *nodes;
+    const GpuNode4Q *nodes;     // four-child quantized BVH
     const PrimRecord *prims;    // leaf order
     const float4 *tri_verts;    // by triangle id: p0, p1, p2 (xyz)
     const float4 *spheres;      // by sphere index: cx, cy, cz, r

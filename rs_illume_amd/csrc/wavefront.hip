@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
         TraceHit h;
         uint32_t nb = 0, np = 0;
         const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
-        const bool found = trace_ray<false, false, BLOCK>(sc.nodes, sc.prims, o, d, a.w, h, st, nb, np);
+        const bool found = trace_ray4<false, false, BLOCK>(sc.nodes, sc.prims, o, d, a.w, h, st, nb, np);
         ++n_closest;
         w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
         w.hit1[p] = make_float4(h.b1, h.b2, 0.f, 0.f);
