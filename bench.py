@@ -147,6 +147,16 @@ def bench_cornell(no_cpu):
     torch.cuda.synchronize()
     dth = (time.perf_counter() - t0) / reps
     out["halton_sampler"] = {"samples_per_s": round(256 * 256 * spp / dth, 1), "ms_per_frame": round(dth * 1e3, 3)}
+    # and with pbrt-v3's Sobol' sampler
+    ps = PathIntegrator(cam, spp=spp, max_depth=5, seed=0, sampler="sobol")
+    acc = ps.render_sum(sc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ps.render_sum(sc, rgb_sum=acc)
+    torch.cuda.synchronize()
+    dts = (time.perf_counter() - t0) / reps
+    out["sobol_sampler"] = {"samples_per_s": round(256 * 256 * spp / dts, 1), "ms_per_frame": round(dts * 1e3, 3)}
     if not no_cpu:
         pyoracle = _oracle()
         osc = pyoracle.OracleRenderScene(desc.arrays())

@@ -160,7 +160,7 @@ typedef struct {
     uint32_t spp;
     uint32_t max_depth;
     uint32_t sample_offset;
-    uint32_t flags; /* 0 or ILLUME_RENDER_HALTON */
+    uint32_t flags; /* 0, ILLUME_RENDER_HALTON or ILLUME_RENDER_SOBOL */
     uint64_t seed;
 } illume_render_cfg;
 
@@ -174,6 +174,18 @@ typedef struct {
  * that base's digit permutation (1000 bases; past them a hash of (index,
  * dimension)). */
 #define ILLUME_RENDER_HALTON 1u
+
+/* flags: the Sobol' sampler instead of the random streams (illume_render and
+ * illume_render_filtered; the seed is then unused). pbrt-v3's SobolSampler
+ * (PBR book 3e 7.7) over the reference's sobol_sample_float and
+ * sobol_interval_to_index (core/low_discrepancy.rs:354-420) and generator
+ * matrices equal to its SOBOL_MATRICES_32 / VD_C_SOBOL_MATRICES(_INV)
+ * (core/sobolmatrices.rs), built from the Joe-Kuo direction numbers: sample s
+ * of pixel (x, y) is the index whose first two dimensions fall in that pixel of
+ * the power-of-two grid over the sample bounds; later dimensions in order
+ * (1024; past them a hash of (index, dimension)). The sample bounds' larger
+ * side may not exceed 2^25. */
+#define ILLUME_RENDER_SOBOL 2u
 
 /* Build a renderable scene: the BVH of illume_bvh_create (max 4 primitives per
  * leaf) plus per-primitive material index and emitted radiance (3 floats per

@@ -201,6 +201,11 @@ def _render_decl():
     L.orc_halton_sample.argtypes = [i32, i32, i32, i32, ctypes.c_int64, u32, vp]
     L.orc_halton_sample.restype = ctypes.c_int64
     L.orc_halton_map.argtypes = [i32, i32, vp]
+    L.orc_sobol_tables.argtypes = [vp, vp, vp]
+    L.orc_sobol_sample.argtypes = [i32, i32, i32, i32, i32, i32, ctypes.c_int64, u32, vp]
+    L.orc_sobol_sample.restype = ctypes.c_int64
+    L.orc_sobol_value.argtypes = [ctypes.c_int64, i32]
+    L.orc_sobol_value.restype = ctypes.c_float
     L.orc_render_filtered.restype = i32
     for fn, at in (("orc_filter_table", [vp, vp]), ("orc_filter_eval", [vp, vp, u32, vp]),
                    ("orc_film_sample_bounds", [vp, i32, i32, vp]),
@@ -256,9 +261,31 @@ class OracleRenderScene:
 
 
 def _sampler_flags(sampler):
-    if sampler not in ("random", "halton"):
-        raise ValueError("sampler must be 'random' or 'halton'")
-    return 1 if sampler == "halton" else 0
+    flags = {"random": 0, "halton": 1, "sobol": 2}
+    if sampler not in flags:
+        raise ValueError("sampler must be 'random', 'halton' or 'sobol'")
+    return flags[sampler]
+
+
+def sobol_tables():
+    """(matrices uint32 [1024, 52], vdc uint64 [25, 52], vdc_inv uint64 [26, 52])."""
+    c = np.zeros(1024 * 52, np.uint32)
+    v = np.zeros(25 * 52, np.uint64)
+    vi = np.zeros(26 * 52, np.uint64)
+    _render_decl().orc_sobol_tables(_p(c), _p(v), _p(vi))
+    return c.reshape(1024, 52), v.reshape(25, 52), vi.reshape(26, 52)
+
+
+def sobol_sample(bounds, pixel, sample, n):
+    """(global index, first n dimensions) of a Sobol' sample; bounds = (x0, y0, w, h)."""
+    out = np.zeros(n, np.float32)
+    idx = _render_decl().orc_sobol_sample(bounds[0], bounds[1], bounds[2], bounds[3], pixel[0], pixel[1], sample,
+                                          n, _p(out))
+    return int(idx), out
+
+
+def sobol_value(index, dim):
+    return float(_render_decl().orc_sobol_value(index, dim))
 
 
 def radical_inverse(base_index, a, scrambled=False):

@@ -44,14 +44,25 @@ uint64_t sampler_stream(uint64_t seed, uint64_t pixel, uint64_t sample) {
 }
 void Sampler::start(uint64_t seed, uint64_t pixel, uint64_t sample) {
     halton = nullptr;
+    sobol = nullptr;
     rng.set_sequence(sampler_stream(seed, pixel, sample));
+}
+
+static float hashed_dimension(int64_t index, int d) {
+    const uint64_t h = splitmix64((uint64_t)index * 0x9E3779B97F4A7C15ull + (uint64_t)d);
+    return std::min(OneMinusEpsilon, (float)(uint32_t)(h >> 32) * 0x1p-32f);
 }
 
 float Sampler::halton_next() {
     const int d = dim++;
     if (d < kHaltonDims) return halton->sample_dimension(index, d);
-    const uint64_t h = splitmix64((uint64_t)index * 0x9E3779B97F4A7C15ull + (uint64_t)d);
-    return std::min(OneMinusEpsilon, (float)(uint32_t)(h >> 32) * 0x1p-32f);
+    return hashed_dimension(index, d);
+}
+
+float Sampler::sobol_next() {
+    const int d = dim++;
+    if (d < kSobolDims) return sobol->sample_dimension(index, d, px, py);
+    return hashed_dimension(index, d);
 }
 
 // sampling.rs:210-230

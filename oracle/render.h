@@ -6,6 +6,7 @@
 #include <vector>
 #include "bvh.h"
 #include "halton.h"
+#include "sobol.h"
 
 namespace orc {
 
@@ -76,26 +77,38 @@ struct Camera {
 
 // The sampler of a camera path. Random (default): one PCG32 stream per
 // (pixel, sample index), consumed sequentially by get_1d/get_2d in pbrt's
-// RandomSampler order. Halton (start_halton): dimension d of the path's
-// global Halton index, d counting up from 0 (pbrt-v3 GlobalSampler); past the
-// 1000 prime bases, a hash of (index, d).
+// RandomSampler order. Halton (start_halton) and Sobol' (start_sobol):
+// dimension d of the path's global index, d counting up from 0 (pbrt-v3
+// GlobalSampler); past the 1000 prime bases or 1024 Sobol' dimensions, a hash
+// of (index, d).
 struct Sampler {
     Rng rng;
     const HaltonPixelMap *halton = nullptr;
+    const SobolPixelMap *sobol = nullptr;
     int64_t index = 0;
-    int dim = 0;
+    int dim = 0, px = 0, py = 0;
     void start(uint64_t seed, uint64_t pixel, uint64_t sample);
-    void start_halton(const HaltonPixelMap *h, int px, int py, int64_t sample) {
+    void start_halton(const HaltonPixelMap *h, int x, int y, int64_t sample) {
         halton = h;
-        index = h->index_for_sample(px, py, sample);
+        sobol = nullptr;
+        index = h->index_for_sample(x, y, sample);
         dim = 0;
     }
-    float get_1d() { return halton ? halton_next() : rng.uniform_float(); }
+    void start_sobol(const SobolPixelMap *s, int x, int y, int64_t sample) {
+        halton = nullptr;
+        sobol = s;
+        px = x;
+        py = y;
+        index = s->index_for_sample(x, y, sample);
+        dim = 0;
+    }
+    float get_1d() { return halton ? halton_next() : (sobol ? sobol_next() : rng.uniform_float()); }
     void get_2d(float *u) {
         u[0] = get_1d();
         u[1] = get_1d();
     }
     float halton_next();
+    float sobol_next();
 };
 uint64_t sampler_stream(uint64_t seed, uint64_t pixel, uint64_t sample);
 

@@ -104,7 +104,8 @@ void orc_render_scene_destroy(void *h) { delete (RenderScene *)h; }
 // apply the NaN/negative/inf radiance guard (113-122) and add L to
 // rgb_sum[pixel] (box filter of radius 0.5: weight 1 for the pixel the
 // sample lands in). flags bit 0: the Halton sampler (pbrt-v3 HaltonSampler over
-// the image) instead of the random streams. stats: [0] closest-hit rays, [1]
+// the image), bit 1: the Sobol' sampler (pbrt-v3 SobolSampler), instead of the
+// random streams. stats: [0] closest-hit rays, [1]
 // any-hit rays.
 void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t sample_offset,
                 uint32_t flags, float *rgb_sum, uint64_t *stats, int n_threads) {
@@ -116,7 +117,9 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
     uint64_t n_pix = (uint64_t)c->width * c->height;
     HaltonPixelMap hmap;
     hmap.init((int)c->width, (int)c->height);
-    const bool halton = (flags & 1u) != 0;
+    SobolPixelMap smap;
+    smap.init(0, 0, (int)c->width, (int)c->height);
+    const bool halton = (flags & 1u) != 0, sobol = (flags & 2u) != 0;
     int nt = std::max(1, n_threads);
     std::vector<RenderStats> st(nt);
     std::atomic<uint64_t> next(0);
@@ -131,6 +134,7 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
                 for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
                     Sampler sampler;
                     if (halton) sampler.start_halton(&hmap, x, y, si);
+                    else if (sobol) sampler.start_sobol(&smap, x, y, si);
                     else sampler.start(seed, pix, si);
                     // sampler.rs:78-84 get_camera_sample: p_film, time, p_lens
                     float uf[2], ul[2];
@@ -190,7 +194,9 @@ int orc_render_filtered(void *h, const void *camera, const void *filter_desc, ui
     const uint64_t wext = (uint64_t)(x1 - x0);
     HaltonPixelMap hmap;
     hmap.init(x1 - x0, y1 - y0);
-    const bool halton = (flags & 1u) != 0;
+    SobolPixelMap smap;
+    smap.init(x0, y0, x1 - x0, y1 - y0);
+    const bool halton = (flags & 1u) != 0, sobol = (flags & 2u) != 0;
     const int nt = std::max(1, n_threads);
     std::vector<RenderStats> st(nt);
     std::vector<std::vector<float>> films(nt, std::vector<float>((size_t)W * H * 4, 0.f));
@@ -201,6 +207,7 @@ int orc_render_filtered(void *h, const void *camera, const void *filter_desc, ui
                 for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
                     Sampler sampler;
                     if (halton) sampler.start_halton(&hmap, x, y, si);
+                    else if (sobol) sampler.start_sobol(&smap, x, y, si);
                     else sampler.start(seed, pix, si);
                     float uf[2], ul[2];
                     sampler.get_2d(uf);
@@ -326,6 +333,26 @@ void orc_halton_map(int res_x, int res_y, int64_t *out) {
     out[5] = m.mult_inv[0];
     out[6] = m.mult_inv[1];
 }
+
+// Sobol' probes: the generator matrices and interval-to-index tables; one
+// sample's global index and first n dimensions for a sample bounds.
+void orc_sobol_tables(uint32_t *c, uint64_t *vdc, uint64_t *vdc_inv) {
+    const SobolTables &t = sobol_tables();
+    for (int i = 0; i < kSobolDims * kSobolMatrixSize; ++i) c[i] = t.c[i];
+    for (int m = 0; m < 25; ++m)
+        for (int k = 0; k < kSobolMatrixSize; ++k) vdc[m * kSobolMatrixSize + k] = t.vdc[m][k];
+    for (int m = 0; m < 26; ++m)
+        for (int k = 0; k < kSobolMatrixSize; ++k) vdc_inv[m * kSobolMatrixSize + k] = t.vdc_inv[m][k];
+}
+int64_t orc_sobol_sample(int x0, int y0, int w, int h, int px, int py, int64_t sample, uint32_t n, float *out) {
+    SobolPixelMap m;
+    m.init(x0, y0, w, h);
+    Sampler s;
+    s.start_sobol(&m, px, py, sample);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.get_1d();
+    return m.index_for_sample(px, py, sample);
+}
+float orc_sobol_value(int64_t index, int dim) { return sobol_sample(index, dim, 0); }
 
 // Stream probe: n uniform floats of the sampler stream for (seed, pixel, sample).
 void orc_sampler_floats(uint64_t seed, uint64_t pixel, uint64_t sample, uint32_t n, float *out) {

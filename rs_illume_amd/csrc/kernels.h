@@ -3,27 +3,28 @@
 #include <hip/hip_runtime.h>
 #include "../../include/illume_hip.h"
 #include "film.h"
-#include "halton.h"
+#include "lowdisc.h"
 #include "layout.h"
 #include "scene_types.h"
 
 namespace illume {
 
 // Path tracer: adds the radiance sums of spp paths per pixel to rgb.
-// hl.table null: random sampler streams; else the Halton sampler.
+// hl.table null: random sampler streams; else the Halton or Sobol' sampler.
 hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
-                         uint32_t sample_offset, float *rgb, unsigned long long *counters, const HaltonLaunch &hl,
+                         uint32_t sample_offset, float *rgb, unsigned long long *counters, const SamplerLaunch &hl,
                          hipStream_t stream);
 
 // Path tracer through a reconstruction filter: samples over fs's sample
 // bounds, splatted into rgbw (4 floats per image pixel).
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
                                   uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
-                                  unsigned long long *counters, const HaltonLaunch &hl, hipStream_t stream);
+                                  unsigned long long *counters, const SamplerLaunch &hl, hipStream_t stream);
 
-// The Halton pixel map of a res_x x res_y sample bounds over this device's
-// table (uploaded on first use); false if the table could not be made.
-bool halton_launch(int res_x, int res_y, HaltonLaunch *out);
+// The sampler parameters of kind kSamplerHalton or kSamplerSobol for the
+// sample bounds [x0, x0 + w) x [y0, y0 + h), over this device's table
+// (uploaded on first use); false if the table could not be made.
+bool sampler_launch(uint32_t kind, int x0, int y0, int w, int h, SamplerLaunch *out);
 
 // rgb[3p + c] = scale * (w != 0 ? max(0, rgbw[4p + c] * (1 / w)) : rgbw[4p + c]).
 hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream);

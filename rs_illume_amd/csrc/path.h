@@ -362,18 +362,28 @@ struct PathState {
 // get_camera_sample (film 2D, time 1D, lens 2D) and the camera ray, for
 // sample s of the sample pixel (x, y) whose sampler stream is numbered
 // `stream`; (fx, fy) gets the film position of the camera sample. With a
-// Halton table the film offset is dimensions 0 and 1 of the pixel's global
-// index (pbrt-v3 HaltonSampler::SampleDimension) and later dimensions follow.
+// low-discrepancy table the film offset is dimensions 0 and 1 of the pixel's
+// global index (pbrt-v3 HaltonSampler / SobolSampler::SampleDimension) and
+// later dimensions follow.
 __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t seed, uint64_t stream, int x, int y,
                                               uint32_t s, PathState &ps, float &fx, float &fy,
-                                              const HaltonLaunch &hl) {
+                                              const SamplerLaunch &sl) {
     float uf0, uf1;
-    if (hl.table) {
-        const int64_t idx = halton_index(hl, x, y, (int64_t)s);
-        uf0 = radical_inverse_base2((uint64_t)idx >> hl.exp0);
-        uf1 = radical_inverse_prime(3, (uint64_t)idx / (uint64_t)hl.scale1);
-        ps.rng.halton = hl.table;
-        ps.rng.state = (uint64_t)idx;
+    if (sl.table) {
+        uint64_t idx;
+        if (sl.kind == kSamplerHalton) {
+            idx = (uint64_t)halton_index(sl, x, y, (int64_t)s);
+            uf0 = radical_inverse_base2(idx >> sl.exp0);
+            uf1 = radical_inverse_prime(3, idx / (uint64_t)sl.scale1);
+        } else {
+            idx = sobol_index(sl.table, sl.log2_res, s, x - sl.x0, y - sl.y0);
+            const float sx = sobol_value(sl.table, idx, 0) * sl.res + (float)sl.x0;
+            const float sy = sobol_value(sl.table, idx, 1) * sl.res + (float)sl.y0;
+            uf0 = fminf(fmaxf(sx - (float)x, 0.f), kOneMinusEpsilon);
+            uf1 = fminf(fmaxf(sy - (float)y, 0.f), kOneMinusEpsilon);
+        }
+        ps.rng.lds = sl.table;
+        ps.rng.state = idx;
         ps.rng.inc = 2;
     } else {
         ps.rng.seed(seed, stream, s);
@@ -398,7 +408,7 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
 __device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
                                            PathState &ps) {
     float fx, fy;
-    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, HaltonLaunch{});
+    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, SamplerLaunch{});
 }
 
 // One bounce of PathIntegrator::li after the closest-hit query (found, h).

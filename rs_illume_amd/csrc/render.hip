@@ -31,7 +31,7 @@ template <int BLOCK, bool FILTERED>
 __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
                                                         uint64_t seed, uint32_t sample_offset, float *__restrict__ film,
                                                         unsigned long long *__restrict__ counters, FilmSplat fs,
-                                                        HaltonLaunch hl) {
+                                                        SamplerLaunch hl) {
     __shared__ int lds_stack[kStackLds * BLOCK];
     __shared__ uint32_t blk_counts[2];
     if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(const float *__restrict__ 
 }  // namespace
 
 hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t max_depth, uint64_t seed,
-                         uint32_t sample_offset, float *rgb, unsigned long long *counters, const HaltonLaunch &hl,
+                         uint32_t sample_offset, float *rgb, unsigned long long *counters, const SamplerLaunch &hl,
                          hipStream_t stream) {
     const uint64_t n = (uint64_t)cam.width * cam.height * spp;
     if (n == 0) return hipSuccess;
@@ -113,7 +113,7 @@ hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp,
 
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
                                   uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
-                                  unsigned long long *counters, const HaltonLaunch &hl, hipStream_t stream) {
+                                  unsigned long long *counters, const SamplerLaunch &hl, hipStream_t stream) {
     const uint64_t n = (uint64_t)fs.wext * fs.hext * spp;
     if (n == 0) return hipSuccess;
     const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;

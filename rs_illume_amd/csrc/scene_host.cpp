@@ -201,6 +201,17 @@ bool scene_camera(const illume_camera &c, DevCamera *out);
 void filter_splat(const illume_filter *f, uint32_t width, uint32_t height, FilmSplat *out);
 }  // namespace illume
 
+namespace {
+// Sampler kind of illume_render_cfg.flags: 0 (random streams), kSamplerHalton,
+// kSamplerSobol, or ~0u for an invalid combination.
+uint32_t sampler_kind(uint32_t flags) {
+    if (flags == 0) return 0;
+    if (flags == ILLUME_RENDER_HALTON) return kSamplerHalton;
+    if (flags == ILLUME_RENDER_SOBOL) return kSamplerSobol;
+    return ~0u;
+}
+}  // namespace
+
 extern "C" {
 
 int illume_scene_destroy(illume_scene *s) {
@@ -409,11 +420,12 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
         return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
     if ((uint64_t)camera->width * camera->height >= (1ull << 40))
         return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
-    if (cfg->flags & ~ILLUME_RENDER_HALTON) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
+    const uint32_t kind = sampler_kind(cfg->flags);
+    if (kind == ~0u) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
     if (scene->n_prims == 0) return ILLUME_OK;  // nothing to hit, no lights: black
-    HaltonLaunch hl{};
-    if ((cfg->flags & ILLUME_RENDER_HALTON) && !halton_launch((int)camera->width, (int)camera->height, &hl))
-        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "Halton tables");
+    SamplerLaunch hl{};
+    if (kind && !sampler_launch(kind, 0, 0, (int)camera->width, (int)camera->height, &hl))
+        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
     hipError_t e = launch_render(scene->dev, cam, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset, rgb_sum,
                                  (unsigned long long *)ray_counts, hl, (hipStream_t)stream);
     if (e != hipSuccess) return abi_hip_fail(e, "render launch");
@@ -433,11 +445,12 @@ int illume_render_filtered(const illume_scene *scene, const illume_camera *camer
     FilmSplat fs;
     filter_splat(filter, camera->width, camera->height, &fs);
     if ((uint64_t)fs.wext * fs.hext >= (1ull << 40)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
-    if (cfg->flags & ~ILLUME_RENDER_HALTON) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
+    const uint32_t kind = sampler_kind(cfg->flags);
+    if (kind == ~0u) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
     if (scene->n_prims == 0) return ILLUME_OK;
-    HaltonLaunch hl{};
-    if ((cfg->flags & ILLUME_RENDER_HALTON) && !halton_launch((int)fs.wext, (int)fs.hext, &hl))
-        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "Halton tables");
+    SamplerLaunch hl{};
+    if (kind && !sampler_launch(kind, fs.x0, fs.y0, (int)fs.wext, (int)fs.hext, &hl))
+        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
     hipError_t e = launch_render_filtered(scene->dev, cam, fs, cfg->spp, cfg->max_depth, cfg->seed, cfg->sample_offset,
                                           rgbw_sum, (unsigned long long *)ray_counts, hl, (hipStream_t)stream);
     if (e != hipSuccess) return abi_hip_fail(e, "filtered render launch");

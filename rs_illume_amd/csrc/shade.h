@@ -6,7 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "geom.h"
-#include "halton.h"
+#include "lowdisc.h"
 #include "layout.h"
 #include "scene_types.h"
 
@@ -19,11 +19,12 @@ constexpr float kInvPi = 0.31830988618379067154f;
 constexpr float kOneMinusEpsilon = 0x1.fffffep-1f;
 
 // ---------------------------------------------------------------- sampler
-// The sampler of one camera path. Random (halton == null): PCG32
+// The sampler of one camera path. Random (lds == null): PCG32
 // (core/rng.rs:9-63), one stream per (pixel, sample index) selected by a
 // splitmix64 hash of (seed, pixel, sample); dimensions are consumed
-// sequentially in pbrt's RandomSampler order. Halton: state is the path's
-// global Halton index, inc the next dimension (halton.h).
+// sequentially in pbrt's RandomSampler order. Low-discrepancy (lds = the
+// Halton or Sobol' table): state is the path's global index, inc the next
+// dimension (lowdisc.h).
 __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ULL;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -32,7 +33,7 @@ __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
 }
 struct PathSampler {
     uint64_t state, inc;
-    const uint32_t *halton;
+    const uint32_t *lds;
     __device__ __forceinline__ uint32_t next() {
         uint64_t old = state;
         state = old * 0x5851f42d4c957f2dULL + inc;
@@ -41,7 +42,7 @@ struct PathSampler {
         return (xs >> rot) | (xs << ((~rot + 1u) & 31));
     }
     __device__ __forceinline__ void seed(uint64_t seed, uint64_t pixel, uint64_t sample) {
-        halton = nullptr;
+        lds = nullptr;
         uint64_t seq = splitmix64(seed ^ splitmix64((pixel << 24) | (sample & 0xffffff)));
         state = 0u;
         inc = (seq << 1u) | 1u;
@@ -50,7 +51,7 @@ struct PathSampler {
         next();
     }
     __device__ __forceinline__ float get_1d() {
-        if (halton) return halton_dim(halton, state, (uint32_t)inc++);
+        if (lds) return lowdisc_dim(lds, state, (uint32_t)inc++);
         return fminf(kOneMinusEpsilon, (float)next() * 0x1p-32f);
     }
 };

@@ -44,7 +44,7 @@ __device__ __forceinline__ void load_state(const WfState &w, uint32_t p, PathSta
     ps.L = mk(l.x, l.y, l.z);
     ps.rng.state = r.x;
     ps.rng.inc = r.y;
-    ps.rng.halton = nullptr;
+    ps.rng.lds = nullptr;
     ps.bounces = f.x;
     ps.active = (f.y & 1u) != 0;
     ps.specular = (f.y & 2u) != 0;

@@ -22,9 +22,9 @@ Two layers, as in the reference's core/parser:
   pbrt-v3's scene API (api.cpp) for the subset the GPU path tracer renders.
 
 Loaded: LookAt + perspective camera (fov), image film (resolution,
-filename), box/triangle/gaussian/mitchell/sinc pixel filters, the halton and
-random samplers (other samplers render with Halton) and pixelsamples, the path
-integrator's maxdepth,
+filename), box/triangle/gaussian/mitchell/sinc pixel filters, the halton,
+sobol and random samplers (other samplers render with Halton) and
+pixelsamples, the path integrator's maxdepth,
 Identity/Translate/Scale/Rotate/Transform/ConcatTransform,
 CoordinateSystem/CoordSysTransform, ReverseOrientation, matte/metal/glass
 materials (constant textures allowed), named materials, diffuse area lights on
@@ -451,7 +451,7 @@ class Loaded:
     """Result of :func:`load`: ``desc`` (scenes.SceneDesc), ``camera``
     (scenes.camera record), ``spp``, ``max_depth``, the film ``filename``, the
     pixel ``filter`` (film.filter_desc record, None for pbrt-v3's default box),
-    the ``sampler`` ("halton", pbrt-v3's default, or "random") and the
+    the ``sampler`` ("halton", pbrt-v3's default, "sobol" or "random") and the
     warnings for directives and parameters that were skipped."""
 
     def __init__(self):
@@ -771,7 +771,7 @@ class _Loader:
         elif n == "Sampler":
             used = set()
             kind = d.args[0]
-            if kind in ("halton", "random"):
+            if kind in ("halton", "random", "sobol"):
                 self.out.sampler = kind
             else:
                 self.out.sampler = "halton"

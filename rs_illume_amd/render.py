@@ -16,8 +16,8 @@ from .accel import _stream_handle
 from .scenes import CAMERA_DTYPE, MATERIAL_DTYPE
 
 
-# illume_render_cfg.flags per sampler: PCG32 streams, or pbrt-v3's Halton sampler
-SAMPLERS = {"random": 0, "halton": 1}
+# illume_render_cfg.flags per sampler: PCG32 streams, or pbrt-v3's Halton or Sobol' sampler
+SAMPLERS = {"random": 0, "halton": 1, "sobol": 2}
 
 
 class RenderCfg(ctypes.Structure):

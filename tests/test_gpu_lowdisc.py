@@ -1,7 +1,7 @@
-"""The Halton sampler on the GPU (illume_render_cfg.flags = ILLUME_RENDER_HALTON)
-against the oracle's at equal samples, through the plain and the filtered
-film, with global indices below and above 2^32; and its lower error than the
-random sampler at equal spp."""
+"""The Halton and Sobol' samplers on the GPU (illume_render_cfg.flags =
+ILLUME_RENDER_HALTON / ILLUME_RENDER_SOBOL) against the oracle's at equal
+samples, through the plain and the filtered film, with global indices below
+and above 2^32; and their lower error than the random sampler at equal spp."""
 import sys
 
 import numpy as np
@@ -29,42 +29,48 @@ def _gpu_sum(desc, cam, spp, depth, sampler, sample_offset=0, seed=0, scene=None
     return s.cpu().numpy(), counts.cpu().numpy()
 
 
+@pytest.mark.parametrize("sampler", ["halton", "sobol"])
 @pytest.mark.parametrize("sample_offset", [0, 1 << 20])
-def test_halton_matches_oracle(gpu, oracle, sample_offset):
-    """sample_offset 2^20 puts every global index above 2^32 (stride 64 x 81)."""
+def test_matches_oracle(gpu, oracle, sampler, sample_offset):
+    """sample_offset 2^20 puts every global index above 2^32 (Halton stride
+    64 x 81; Sobol' frame << 12)."""
     desc, cam = S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625), S.cornell_camera(48, 40)
-    _say("case halton offset", sample_offset)
-    g, gc = _gpu_sum(desc, cam, 8, 6, "halton", sample_offset)
+    _say("case", sampler, "offset", sample_offset)
+    g, gc = _gpu_sum(desc, cam, 8, 6, sampler, sample_offset)
     o, oc = oracle.OracleRenderScene(desc.arrays()).render_sum(cam, 8, 6, sample_offset=sample_offset,
-                                                               sampler="halton")
+                                                               sampler=sampler)
     rm = S.rel_mse(g / 8, o / 8)
-    _say("halton relMSE", rm, "rays", gc, oc)
+    _say(sampler, "relMSE", rm, "rays", gc, oc)
     assert np.isfinite(g).all() and rm <= REL_MSE_BAR, rm
     assert abs(int(gc[0]) - int(oc[0])) <= 0.002 * oc[0] + 10
 
 
-def test_halton_filtered_matches_oracle(gpu, oracle):
+@pytest.mark.parametrize("sampler", ["halton", "sobol"])
+def test_filtered_matches_oracle(gpu, oracle, sampler):
+    """The filter's sample bounds start at (-2, -2): pixel indices relative to
+    them (Sobol') and negative pixels (Halton's mod 128)."""
     from rs_illume_amd.film import Filter, filter_desc, render_filtered, resolve
     from rs_illume_amd.render import PathIntegrator, Scene
     desc, cam = S.cornell_box(0), S.cornell_camera(40, 30)
-    _say("case halton filtered")
-    pi = PathIntegrator(cam, spp=4, max_depth=5, sampler="halton")
+    _say("case filtered", sampler)
+    pi = PathIntegrator(cam, spp=4, max_depth=5, sampler=sampler)
     g = resolve(render_filtered(pi, Scene(desc), Filter("mitchell"))).cpu().numpy()
     o, _ = oracle.OracleRenderScene(desc.arrays()).render_filtered(cam, filter_desc("mitchell"), 4, 5,
-                                                                   sampler="halton")
+                                                                   sampler=sampler)
     oi = np.maximum(o[..., :3] / o[..., 3:4], 0)
     rm = S.rel_mse(g, oi)
-    _say("halton filtered relMSE", rm)
+    _say(sampler, "filtered relMSE", rm)
     assert rm <= REL_MSE_BAR, rm
 
 
-def test_halton_passes_add_up(gpu):
+@pytest.mark.parametrize("sampler", ["halton", "sobol"])
+def test_passes_add_up(gpu, sampler):
     """Sample indices are global: 2 + 6 spp in two calls equal 8 spp in one."""
     import torch
     from rs_illume_amd.render import PathIntegrator, Scene
     desc, cam = S.cornell_box(0), S.cornell_camera(32, 32)
     sc = Scene(desc)
-    pi = PathIntegrator(cam, spp=8, max_depth=5, sampler="halton")
+    pi = PathIntegrator(cam, spp=8, max_depth=5, sampler=sampler)
     one = pi.render_sum(sc)
     two = pi.render_sum(sc, spp=2)
     pi.render_sum(sc, rgb_sum=two, spp=6, sample_offset=2)
@@ -72,24 +78,32 @@ def test_halton_passes_add_up(gpu):
     assert np.allclose(one.cpu().numpy(), two.cpu().numpy(), rtol=2e-6, atol=1e-7)
 
 
-def test_halton_lowers_error(gpu):
+def test_low_discrepancy_lowers_error(gpu):
     from rs_illume_amd.render import Scene
     desc, cam = S.cornell_box(0), S.cornell_camera(128, 128)
     sc = Scene(desc)
     ref, _ = _gpu_sum(desc, cam, 4096, 5, "random", seed=77, scene=sc)
     ref /= 4096
     errs = {}
-    for smp in ("random", "halton"):
+    for smp in ("random", "halton", "sobol"):
         img, _ = _gpu_sum(desc, cam, 64, 5, smp, seed=1, scene=sc)
         errs[smp] = S.rel_mse(img / 64, ref)
     _say("relMSE at 64 spp", errs)
     assert errs["halton"] < 0.75 * errs["random"], errs
+    assert errs["sobol"] < 0.75 * errs["random"], errs
 
 
-def test_wavefront_rejects_halton(gpu):
+def test_wavefront_rejects_low_discrepancy_and_bad_flags(gpu):
     from rs_illume_amd._lib import IllumeError
-    from rs_illume_amd.render import Scene, WavefrontPathIntegrator
-    wpi = WavefrontPathIntegrator(Scene(S.cornell_box(0)), S.cornell_camera(16, 16), spp=1)
-    wpi.sampler = "halton"
+    from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
+    sc = Scene(S.cornell_box(0))
+    wpi = WavefrontPathIntegrator(sc, S.cornell_camera(16, 16), spp=1)
+    for smp in ("halton", "sobol"):
+        wpi.sampler = smp
+        with pytest.raises(IllumeError):
+            wpi.render()
+    class BothFlags(PathIntegrator):
+        flags = 3  # ILLUME_RENDER_HALTON | ILLUME_RENDER_SOBOL: not a sampler
+
     with pytest.raises(IllumeError):
-        wpi.render()
+        BothFlags(S.cornell_camera(16, 16), spp=1).render(sc)

@@ -413,8 +413,8 @@ def test_warnings_for_ignored_parameters():
                HEAD + 'PixelFilter "gaussian"\nSampler "sobol" "integer pixelsamples" 8 "integer dimension" 5\n')
     assert not any("PixelFilter" in w for w in ld.warnings)
     assert any("'dimension' ignored" in w for w in ld.warnings)
-    assert any("sampler 'sobol' rendered with the Halton sampler" in w for w in ld.warnings)
-    assert ld.sampler == "halton"
+    assert ld.sampler == "sobol"
+    assert P.load_text(HEAD + 'Sampler "02sequence" WorldBegin %s WorldEnd' % TRI).sampler == "halton"
     assert P.load_text(HEAD + 'Sampler "random" WorldBegin %s WorldEnd' % TRI).sampler == "random"
     assert P.load_text(HEAD + 'WorldBegin %s WorldEnd' % TRI).sampler == "halton"  # pbrt-v3's default
     assert any("'st' ignored" in w for w in ld.warnings)
