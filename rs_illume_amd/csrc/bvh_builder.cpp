@@ -491,7 +491,8 @@ uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4
 
 namespace {
 // plane of byte q: origin + q * step, float32, product exact, sum rounded
-inline float q_plane(float origin, float step, int q) { return origin + (float)q * step; }
+// the device evaluates a plane with the same fused multiply-add
+inline float q_plane(float origin, float step, int q) { return std::fma((float)q, step, origin); }
 inline float pow2f(int e) { return std::ldexp(1.f, e); }
 }  // namespace
 

@@ -60,12 +60,14 @@ __device__ __forceinline__ float next_float_down(float v) {
     return __uint_as_float(ui);
 }
 
-// Watertight ray-triangle test, shapes/triangle.rs:142-256 (pbrt-v3; the
-// delta_t bound uses the true max component, divergence 6 of the reference).
-// On a hit returns true with t and barycentrics b0..b2.
-__device__ __forceinline__ bool triangle_test(f3 p0, f3 p1, f3 p2, f3 o, f3 d, float t_max, float &t_out,
-                                              float &b0_out, float &b1_out, float &b2_out) {
-    f3 p0t = sub(p0, o), p1t = sub(p1, o), p2t = sub(p2, o);
+// The per-ray part of the watertight test (triangle.rs:156-176): the axis
+// permutation that makes the largest |d| component z, and the shear that
+// maps the ray onto +z.
+struct Shear {
+    int k;  // kx | ky << 2 | kz << 4
+    float sx, sy, sz;
+};
+__device__ __forceinline__ Shear make_shear(f3 d) {
     f3 ad = fabs3(d);
     int kz = (ad.x > ad.y) ? ((ad.x > ad.z) ? 0 : 2) : ((ad.y > ad.z) ? 1 : 2);
     int kx = kz + 1;
@@ -73,12 +75,27 @@ __device__ __forceinline__ bool triangle_test(f3 p0, f3 p1, f3 p2, f3 o, f3 d, f
     int ky = kx + 1;
     if (ky == 3) ky = 0;
     float dx = comp(d, kx), dy = comp(d, ky), dz = comp(d, kz);
+    Shear s;
+    s.k = kx | (ky << 2) | (kz << 4);
+    s.sx = -dx / dz;
+    s.sy = -dy / dz;
+    s.sz = 1.f / dz;
+    return s;
+}
+
+// Watertight ray-triangle test, shapes/triangle.rs:142-256 (pbrt-v3; the
+// delta_t bound uses the true max component, divergence 6 of the reference),
+// with the ray's shear computed once by the caller. On a hit returns true
+// with t and barycentrics b0..b2. Apart from the accept/reject comparisons
+// the results do not depend on t_max.
+__device__ __forceinline__ bool triangle_test_sheared(const Shear &sh, f3 p0, f3 p1, f3 p2, f3 o, float t_max,
+                                                      float &t_out, float &b0_out, float &b1_out, float &b2_out) {
+    f3 p0t = sub(p0, o), p1t = sub(p1, o), p2t = sub(p2, o);
+    const int kx = sh.k & 3, ky = (sh.k >> 2) & 3, kz = sh.k >> 4;
     f3 q0 = mk(comp(p0t, kx), comp(p0t, ky), comp(p0t, kz));
     f3 q1 = mk(comp(p1t, kx), comp(p1t, ky), comp(p1t, kz));
     f3 q2 = mk(comp(p2t, kx), comp(p2t, ky), comp(p2t, kz));
-    float sx = -dx / dz;
-    float sy = -dy / dz;
-    float sz = 1.f / dz;
+    const float sx = sh.sx, sy = sh.sy, sz = sh.sz;
     q0.x += sx * q0.z;
     q0.y += sy * q0.z;
     q1.x += sx * q1.z;
@@ -128,6 +145,11 @@ __device__ __forceinline__ bool triangle_test(f3 p0, f3 p1, f3 p2, f3 o, f3 d, f
     b1_out = b1;
     b2_out = b2;
     return true;
+}
+
+__device__ __forceinline__ bool triangle_test(f3 p0, f3 p1, f3 p2, f3 o, f3 d, float t_max, float &t_out,
+                                              float &b0_out, float &b1_out, float &b2_out) {
+    return triangle_test_sheared(make_shear(d), p0, p1, p2, o, t_max, t_out, b0_out, b1_out, b2_out);
 }
 
 // triangle.rs:258-296 for the default uv parameterisation: a closest hit on a
@@ -240,6 +262,8 @@ __device__ __forceinline__ bool sphere_test(f3 c, float radius, f3 ro, f3 rd, fl
 // choosing planes by dir_is_neg (bounds3.rs:284-318) — with exits scaled by
 // 1+2*gamma(3). fminf/fmaxf drop NaNs (0*inf), so a box the reference's test
 // enters is always entered here: culling is at least as conservative.
+// Rounding is monotone, so scaling the smallest exit once gives the same
+// value as scaling each axis's exit before taking the minimum.
 __device__ __forceinline__ bool slab(float lo_x, float hi_x, float lo_y, float hi_y, float lo_z, float hi_z, f3 o,
                                      f3 inv, float t_max, float &t_enter) {
     constexpr float k = 1.0f + 2.0f * gamma_f(3);
@@ -247,7 +271,7 @@ __device__ __forceinline__ bool slab(float lo_x, float hi_x, float lo_y, float h
     float ay = (lo_y - o.y) * inv.y, by = (hi_y - o.y) * inv.y;
     float az = (lo_z - o.z) * inv.z, bz = (hi_z - o.z) * inv.z;
     float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-    float t1 = fminf(fminf(fmaxf(ax, bx) * k, fmaxf(ay, by) * k), fmaxf(az, bz) * k);
+    float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * k;
     t_enter = t0;
     return t0 <= t1 && t0 < t_max && t1 > 0.f;
 }

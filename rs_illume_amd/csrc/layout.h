@@ -35,8 +35,8 @@ struct alignas(128) GpuNode4 {
 static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
 
 // 64-byte four-child node with quantized child boxes (one cache line per
-// visit instead of two). A child plane is origin + q * 2^(e - 127) for a byte
-// q, evaluated in float32 (the product is exact, the sum rounds to nearest);
+// visit instead of two). A child plane is fma(q, 2^(e - 127), origin) for a
+// byte q, evaluated in float32 (one rounding, the product being exact);
 // the builder picks q so that every plane evaluated this way lies on or
 // outside the child's float box, so the dequantized box contains it and the
 // slab test on it enters every box the unquantized test enters. Unused slots

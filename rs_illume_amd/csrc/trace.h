@@ -143,10 +143,13 @@ __device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, con
     return found;
 }
 
-// Leaf primitives of one leaf reference (shared by both node widths).
+// Primitives of one leaf reference (four-child traversal). A nearer hit only
+// moves t_max and the record index; the caller fills in the rest of the hit
+// record once, after the traversal, which keeps four values fewer live
+// across the loop. Returns true when an any-hit query is answered.
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trace_leaf(const PrimRecord *__restrict__ prims, int ref, f3 o, f3 d, float &t_max,
-                                           TraceHit &hit, bool &found, uint32_t &n_prim) {
+__device__ __forceinline__ bool trace_leaf(const PrimRecord *__restrict__ prims, int ref, f3 o, f3 d, const Shear &sh,
+                                           float &t_max, uint32_t &slot, uint32_t &n_prim) {
     const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
     for (uint32_t k = 0; k < cnt; ++k) {
         const float4 *pp = reinterpret_cast<const float4 *>(prims + first + k);
@@ -157,32 +160,22 @@ __device__ __forceinline__ bool trace_leaf(const PrimRecord *__restrict__ prims,
             const float4 pb = pp[1];
             float t;
             if (sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t)) {
-                found = true;
+                slot = first + k;
                 if (ANY) return true;
                 t_max = t;
-                hit.prim = tag & ~kSphereBit;
-                hit.slot = first + k;
-                hit.t = t;
-                hit.b0 = hit.b1 = hit.b2 = 0.f;
             }
         } else {
             const float4 pb = pp[1], pc = pp[2];
             const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
             float t, b0, b1, b2;
-            if (triangle_test(p0, p1, p2, o, d, t_max, t, b0, b1, b2)) {
+            if (triangle_test_sheared(sh, p0, p1, p2, o, t_max, t, b0, b1, b2)) {
                 if (ANY) {
-                    found = true;
+                    slot = first + k;
                     return true;
                 }
                 if (!triangle_degenerate(p0, p1, p2)) {
-                    found = true;
+                    slot = first + k;
                     t_max = t;
-                    hit.prim = tag;
-                    hit.slot = first + k;
-                    hit.t = t;
-                    hit.b0 = b0;
-                    hit.b1 = b1;
-                    hit.b2 = b2;
                 }
             }
         }
@@ -202,14 +195,14 @@ __device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
 }
 
 // Box of child C of a quantized node (words w1 = lo_x hi_x lo_y hi_y, w2.x/y =
-// lo_z hi_z; byte C of each) entered by the ray? Planes are origin + q * step.
+// lo_z hi_z; byte C of each) entered by the ray? Planes are fma(q, step, origin).
 template <int C>
 __device__ __forceinline__ bool qslab(const uint4 &w1, const uint4 &w2, float ox, float oy, float oz, float sx,
                                       float sy, float sz, f3 o, f3 inv, float t_max, float &t) {
     constexpr int sh = 8 * C;
-    const float lx = ox + (float)((w1.x >> sh) & 0xffu) * sx, hx = ox + (float)((w1.y >> sh) & 0xffu) * sx;
-    const float ly = oy + (float)((w1.z >> sh) & 0xffu) * sy, hy = oy + (float)((w1.w >> sh) & 0xffu) * sy;
-    const float lz = oz + (float)((w2.x >> sh) & 0xffu) * sz, hz = oz + (float)((w2.y >> sh) & 0xffu) * sz;
+    const float lx = fmaf((float)((w1.x >> sh) & 0xffu), sx, ox), hx = fmaf((float)((w1.y >> sh) & 0xffu), sx, ox);
+    const float ly = fmaf((float)((w1.z >> sh) & 0xffu), sy, oy), hy = fmaf((float)((w1.w >> sh) & 0xffu), sy, oy);
+    const float lz = fmaf((float)((w2.x >> sh) & 0xffu), sz, oz), hz = fmaf((float)((w2.y >> sh) & 0xffu), sz, oz);
     return slab(lx, hx, ly, hy, lz, hz, o, inv, t_max, t);
 }
 
@@ -223,10 +216,12 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
                                            f3 o, f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st,
                                            uint32_t &n_box, uint32_t &n_prim) {
     const f3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const Shear sh = make_shear(d);
     hit.prim = 0xFFFFFFFFu;
     hit.slot = 0;
     hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
-    bool found = false;
+    constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+    uint32_t slot = kNoSlot;
     int cur = 0;
     st.sp = 0;
     const float inf = __builtin_inff();
@@ -262,12 +257,29 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
                 continue;
             }
         } else {
-            if (trace_leaf<ANY, COUNT>(prims, cur, o, d, t_max, hit, found, n_prim)) return true;
+            if (trace_leaf<ANY, COUNT>(prims, cur, o, d, sh, t_max, slot, n_prim)) break;
         }
         if (st.sp == 0) break;
         cur = st.pop();
     }
-    return found;
+    if (slot == kNoSlot) return false;
+    hit.slot = slot;
+    if (ANY) return true;
+    const float4 *pp = reinterpret_cast<const float4 *>(prims + slot);
+    const float4 pa = pp[0];
+    const uint32_t tag = __float_as_uint(pa.w);
+    hit.t = t_max;
+    if (tag & kSphereBit) {
+        hit.prim = tag & ~kSphereBit;
+    } else {
+        // the accepted test again, on the same operands: the same t and barycentrics
+        const float4 pb = pp[1], pc = pp[2];
+        float t;
+        triangle_test_sheared(sh, mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), mk(pc.x, pc.y, pc.z), o,
+                              __builtin_inff(), t, hit.b0, hit.b1, hit.b2);
+        hit.prim = tag;
+    }
+    return true;
 }
 
 }  // namespace illume
