@@ -57,22 +57,83 @@ struct TraceHit {
     float t, b0, b1, b2;
 };
 
+// One primitive test of the traversal loop: on a hit nearer than t_max
+// (any hit for ANY; closest hit ignores degenerate triangles, as
+// Triangle::intersect does), returns true with its distance.
+template <bool ANY>
+__device__ __forceinline__ bool prim_step(const PrimRecord *__restrict__ prims, uint32_t pf, f3 o, f3 d,
+                                          const Shear &sh, float t_max, float &t) {
+    const float4 *pp = reinterpret_cast<const float4 *>(prims + pf);
+    const float4 pa = pp[0];
+    const uint32_t tag = __float_as_uint(pa.w);
+    if (tag & kSphereBit) {
+        const float4 pb = pp[1];
+        return sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t);
+    }
+    const float4 pb = pp[1], pc = pp[2];
+    const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
+    float b0, b1, b2;
+    return triangle_test_sheared(sh, p0, p1, p2, o, t_max, t, b0, b1, b2) &&
+           (ANY || !triangle_degenerate(p0, p1, p2));
+}
+
+// The hit record of primitive record `slot` hit at distance t: the triangle
+// test again on the same operands gives the same barycentrics (nothing in it
+// but the accept/reject comparisons depends on t_max).
+__device__ __forceinline__ void finish_hit(const PrimRecord *__restrict__ prims, uint32_t slot, f3 o, const Shear &sh,
+                                           float t, TraceHit &hit) {
+    const float4 *pp = reinterpret_cast<const float4 *>(prims + slot);
+    const float4 pa = pp[0];
+    const uint32_t tag = __float_as_uint(pa.w);
+    hit.slot = slot;
+    hit.t = t;
+    if (tag & kSphereBit) {
+        hit.prim = tag & ~kSphereBit;
+        return;
+    }
+    const float4 pb = pp[1], pc = pp[2];
+    float tt;
+    triangle_test_sheared(sh, mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), mk(pc.x, pc.y, pc.z), o, __builtin_inff(), tt,
+                          hit.b0, hit.b1, hit.b2);
+    hit.prim = tag;
+}
+
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr int kTraceDone = (int)0x80000000;  // neither a node (>= 0) nor a leaf reference
+
 // Closest hit (ANY = false) or any hit (ANY = true) of the ray (o, d) in
-// (0, t_max]. Returns true on a hit. COUNT adds box / primitive tests to the
-// counters.
+// (0, t_max] over the two-child tree. Returns true on a hit. COUNT adds box /
+// primitive tests to the counters. One step per iteration, as in trace_ray4
+// below.
 template <bool ANY, bool COUNT, int BLOCK>
 __device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, const PrimRecord *__restrict__ prims, f3 o,
                                           f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st, uint32_t &n_box,
                                           uint32_t &n_prim) {
     const f3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const Shear sh = make_shear(d);
     hit.prim = 0xFFFFFFFFu;
     hit.slot = 0;
     hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
-    bool found = false;
+    uint32_t slot = kNoSlot;
     int cur = 0;
+    uint32_t pf = 0, pn = 0;
     st.sp = 0;
     while (true) {
-        if (cur >= 0) {
+        if (pn != 0) {
+            if (COUNT) ++n_prim;
+            float t;
+            if (prim_step<ANY>(prims, pf, o, d, sh, t_max, t)) {
+                slot = pf;
+                t_max = t;
+                if (ANY) {
+                    // answered: leave through the loop's one exit
+                    pn = 1;
+                    cur = kTraceDone;
+                }
+            }
+            ++pf;
+            --pn;
+        } else {
             const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
             const float4 a = np[0], b = np[1], c = np[2];
             const int4 l = reinterpret_cast<const int4 *>(np)[3];
@@ -81,106 +142,28 @@ __device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, con
             const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, o, inv, t_max, t0);
             const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, o, inv, t_max, t1);
             if (h0 && h1) {
-                int near = l.x, far = l.y;
-                if (t1 < t0) {
-                    near = l.y;
-                    far = l.x;
-                }
-                st.push(far);
-                cur = near;
-                continue;
-            }
-            if (h0) {
+                const bool swap = t1 < t0;
+                st.push(swap ? l.x : l.y);
+                cur = swap ? l.y : l.x;
+            } else if (h0) {
                 cur = l.x;
-                continue;
-            }
-            if (h1) {
+            } else if (h1) {
                 cur = l.y;
-                continue;
-            }
-        } else {
-            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *pp = reinterpret_cast<const float4 *>(prims + first + k);
-                const float4 pa = pp[0];
-                const uint32_t tag = __float_as_uint(pa.w);
-                if (COUNT) ++n_prim;
-                if (tag & kSphereBit) {
-                    const float4 pb = pp[1];
-                    float t;
-                    if (sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t)) {
-                        found = true;
-                        if (ANY) return true;
-                        t_max = t;
-                        hit.prim = tag & ~kSphereBit;
-                        hit.slot = first + k;
-                        hit.t = t;
-                        hit.b0 = hit.b1 = hit.b2 = 0.f;
-                    }
-                } else {
-                    const float4 pb = pp[1], pc = pp[2];
-                    const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
-                    float t, b0, b1, b2;
-                    if (triangle_test(p0, p1, p2, o, d, t_max, t, b0, b1, b2)) {
-                        if (ANY) return true;
-                        if (!triangle_degenerate(p0, p1, p2)) {
-                            found = true;
-                            t_max = t;
-                            hit.prim = tag;
-                            hit.slot = first + k;
-                            hit.t = t;
-                            hit.b0 = b0;
-                            hit.b1 = b1;
-                            hit.b2 = b2;
-                        }
-                    }
-                }
+            } else {
+                cur = st.sp == 0 ? kTraceDone : st.pop();
             }
         }
-        if (st.sp == 0) break;
-        cur = st.pop();
-    }
-    return found;
-}
-
-// Primitives of one leaf reference (four-child traversal). A nearer hit only
-// moves t_max and the record index; the caller fills in the rest of the hit
-// record once, after the traversal, which keeps four values fewer live
-// across the loop. Returns true when an any-hit query is answered.
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trace_leaf(const PrimRecord *__restrict__ prims, int ref, f3 o, f3 d, const Shear &sh,
-                                           float &t_max, uint32_t &slot, uint32_t &n_prim) {
-    const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const float4 *pp = reinterpret_cast<const float4 *>(prims + first + k);
-        const float4 pa = pp[0];
-        const uint32_t tag = __float_as_uint(pa.w);
-        if (COUNT) ++n_prim;
-        if (tag & kSphereBit) {
-            const float4 pb = pp[1];
-            float t;
-            if (sphere_test(mk(pa.x, pa.y, pa.z), pb.x, o, d, t_max, t)) {
-                slot = first + k;
-                if (ANY) return true;
-                t_max = t;
-            }
-        } else {
-            const float4 pb = pp[1], pc = pp[2];
-            const f3 p0 = mk(pa.x, pa.y, pa.z), p1 = mk(pb.x, pb.y, pb.z), p2 = mk(pc.x, pc.y, pc.z);
-            float t, b0, b1, b2;
-            if (triangle_test_sheared(sh, p0, p1, p2, o, t_max, t, b0, b1, b2)) {
-                if (ANY) {
-                    slot = first + k;
-                    return true;
-                }
-                if (!triangle_degenerate(p0, p1, p2)) {
-                    slot = first + k;
-                    t_max = t;
-                }
-            }
+        if (pn == 0 && cur < 0 && cur != kTraceDone) {
+            pf = leaf_first(cur);
+            pn = leaf_count(cur);
+            cur = st.sp == 0 ? kTraceDone : st.pop();
         }
+        if (pn == 0 && cur == kTraceDone) break;
     }
-    return false;
+    if (slot == kNoSlot) return false;
+    hit.slot = slot;
+    if (!ANY) finish_hit(prims, slot, o, sh, t_max, hit);
+    return true;
 }
 
 // Compare-exchange of (distance, reference) pairs for the 4-child order.
@@ -211,6 +194,18 @@ __device__ __forceinline__ bool qslab(const uint4 &w1, const uint4 &w2, float ox
 // distance, descends into the nearest and pushes the rest farthest-first. A
 // stack of kMaxTraceDepth entries holds 3 per level (the builder caps the
 // depth).
+//
+// The loop body is one step: a lane either visits an inner node or tests one
+// primitive of its current leaf, so a wave pays for at most one node visit and
+// one primitive test per iteration whatever mix of the two its lanes are in.
+// (Nesting a leaf loop inside the node loop makes every pop a wave-wide
+// rendezvous that runs the whole leaf loop whenever any lane holds a leaf; on
+// incoherent rays that left a sixth of the lanes active.) A nearer hit only
+// moves t_max and the record index; the rest of the hit record is filled in
+// once, after the loop. An answered any-hit query leaves through the loop's
+// one exit: a second exit from inside the primitive branch made the
+// shadow-ray loops of the wavefront shading kernel twenty times slower for the
+// same node and primitive tests.
 template <bool ANY, bool COUNT, int BLOCK>
 __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, const PrimRecord *__restrict__ prims,
                                            f3 o, f3 d, float t_max, TraceHit &hit, TraceStack<BLOCK> &st,
@@ -220,13 +215,28 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
     hit.prim = 0xFFFFFFFFu;
     hit.slot = 0;
     hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
-    constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
     uint32_t slot = kNoSlot;
-    int cur = 0;
+    int cur = 0;                // next node: inner node (>= 0), leaf (< 0) or kTraceDone
+    uint32_t pf = 0, pn = 0;    // primitives left in the current leaf: slots pf .. pf + pn - 1
     st.sp = 0;
     const float inf = __builtin_inff();
+    // one step per iteration: a lane either visits an inner node or tests one primitive
     while (true) {
-        if (cur >= 0) {
+        if (pn != 0) {
+            if (COUNT) ++n_prim;
+            float t;
+            if (prim_step<ANY>(prims, pf, o, d, sh, t_max, t)) {
+                slot = pf;
+                t_max = t;
+                if (ANY) {
+                    // answered: leave through the loop's one exit
+                    pn = 1;
+                    cur = kTraceDone;
+                }
+            }
+            ++pf;
+            --pn;
+        } else {
             const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
             const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
             if (COUNT) n_box += 4;
@@ -244,7 +254,6 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
             int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
             const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;
             if (n_hit > 0) {
-                // sorting network for four keys; misses (inf) sink to the end
                 cswap(ta, ra, tb, rb);
                 cswap(tc, rc, td, rd);
                 cswap(ta, ra, tc, rc);
@@ -254,31 +263,21 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
                 if (n_hit > 2) st.push(rc);
                 if (n_hit > 1) st.push(rb);
                 cur = ra;
-                continue;
+            } else {
+                cur = st.sp == 0 ? kTraceDone : st.pop();
             }
-        } else {
-            if (trace_leaf<ANY, COUNT>(prims, cur, o, d, sh, t_max, slot, n_prim)) break;
         }
-        if (st.sp == 0) break;
-        cur = st.pop();
+        // a leaf reached (or popped) becomes the current leaf; the node after it is popped now
+        if (pn == 0 && cur < 0 && cur != kTraceDone) {
+            pf = leaf_first(cur);
+            pn = leaf_count(cur);
+            cur = st.sp == 0 ? kTraceDone : st.pop();
+        }
+        if (pn == 0 && cur == kTraceDone) break;
     }
     if (slot == kNoSlot) return false;
     hit.slot = slot;
-    if (ANY) return true;
-    const float4 *pp = reinterpret_cast<const float4 *>(prims + slot);
-    const float4 pa = pp[0];
-    const uint32_t tag = __float_as_uint(pa.w);
-    hit.t = t_max;
-    if (tag & kSphereBit) {
-        hit.prim = tag & ~kSphereBit;
-    } else {
-        // the accepted test again, on the same operands: the same t and barycentrics
-        const float4 pb = pp[1], pc = pp[2];
-        float t;
-        triangle_test_sheared(sh, mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), mk(pc.x, pc.y, pc.z), o,
-                              __builtin_inff(), t, hit.b0, hit.b1, hit.b2);
-        hit.prim = tag;
-    }
+    if (!ANY) finish_hit(prims, slot, o, sh, t_max, hit);
     return true;
 }
 
