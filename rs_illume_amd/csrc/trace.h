@@ -101,6 +101,40 @@ __device__ __forceinline__ void finish_hit(const PrimRecord *__restrict__ prims,
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr int kTraceDone = (int)0x80000000;  // neither a node (>= 0) nor a leaf reference
 
+// A leaf reached (or popped) becomes the current leaf: slots pf .. pf + pn - 1;
+// the node after it is popped at once (the stack order is unchanged).
+template <int BLOCK>
+__device__ __forceinline__ void take_leaf(int &cur, uint32_t &pf, uint32_t &pn, TraceStack<BLOCK> &st) {
+    if (pn == 0 && cur < 0 && cur != kTraceDone) {
+        pf = leaf_first(cur);
+        pn = leaf_count(cur);
+        cur = st.sp == 0 ? kTraceDone : st.pop();
+    }
+}
+
+// One visit of a two-child node: tests both child boxes, pushes the farther
+// of two entered children and returns the nearer; with none entered, returns
+// the top of the stack or kTraceDone.
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ int node_step2(const GpuNode *__restrict__ nodes, int cur, f3 o, f3 inv, float t_max,
+                                          TraceStack<BLOCK> &st, uint32_t &n_box) {
+    const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
+    const float4 a = np[0], b = np[1], c = np[2];
+    const int4 l = reinterpret_cast<const int4 *>(np)[3];
+    if (COUNT) n_box += 2;
+    float t0, t1;
+    const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, o, inv, t_max, t0);
+    const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, o, inv, t_max, t1);
+    if (h0 && h1) {
+        const bool swap = t1 < t0;
+        st.push(swap ? l.x : l.y);
+        return swap ? l.y : l.x;
+    }
+    if (h0) return l.x;
+    if (h1) return l.y;
+    return st.sp == 0 ? kTraceDone : st.pop();
+}
+
 // Closest hit (ANY = false) or any hit (ANY = true) of the ray (o, d) in
 // (0, t_max] over the two-child tree. Returns true on a hit. COUNT adds box /
 // primitive tests to the counters. One step per iteration, as in trace_ray4
@@ -134,30 +168,9 @@ __device__ __forceinline__ bool trace_ray(const GpuNode *__restrict__ nodes, con
             ++pf;
             --pn;
         } else {
-            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
-            const float4 a = np[0], b = np[1], c = np[2];
-            const int4 l = reinterpret_cast<const int4 *>(np)[3];
-            if (COUNT) n_box += 2;
-            float t0, t1;
-            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, o, inv, t_max, t0);
-            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, o, inv, t_max, t1);
-            if (h0 && h1) {
-                const bool swap = t1 < t0;
-                st.push(swap ? l.x : l.y);
-                cur = swap ? l.y : l.x;
-            } else if (h0) {
-                cur = l.x;
-            } else if (h1) {
-                cur = l.y;
-            } else {
-                cur = st.sp == 0 ? kTraceDone : st.pop();
-            }
+            cur = node_step2<COUNT, BLOCK>(nodes, cur, o, inv, t_max, st, n_box);
         }
-        if (pn == 0 && cur < 0 && cur != kTraceDone) {
-            pf = leaf_first(cur);
-            pn = leaf_count(cur);
-            cur = st.sp == 0 ? kTraceDone : st.pop();
-        }
+        take_leaf(cur, pf, pn, st);
         if (pn == 0 && cur == kTraceDone) break;
     }
     if (slot == kNoSlot) return false;
@@ -189,11 +202,45 @@ __device__ __forceinline__ bool qslab(const uint4 &w1, const uint4 &w2, float ox
     return slab(lx, hx, ly, hy, lz, hz, o, inv, t_max, t);
 }
 
-// Traversal of the 4-wide tree: each visit reads one 64-byte quantized node,
-// tests the four dequantized child boxes, sorts the entered children by entry
-// distance, descends into the nearest and pushes the rest farthest-first. A
-// stack of kMaxTraceDepth entries holds 3 per level (the builder caps the
-// depth).
+// One visit of a four-child node: tests the four dequantized child boxes,
+// sorts the entered children by entry distance, pushes all but the nearest
+// (farthest first) and returns the nearest; with none entered, returns the
+// top of the stack or kTraceDone.
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ int node_step4(const GpuNode4Q *__restrict__ nodes, int cur, f3 o, f3 inv, float t_max,
+                                          TraceStack<BLOCK> &st, uint32_t &n_box) {
+    const float inf = __builtin_inff();
+    const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
+    const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+    if (COUNT) n_box += 4;
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
+                sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+    const int4 ch = make_int4((int)w2.z, (int)w2.w, (int)w3.x, (int)w3.y);
+    float t0, t1, t2, t3;
+    const bool h0 = qslab<0>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t0) && ch.x != kEmptyChild;
+    const bool h1 = qslab<1>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t1) && ch.y != kEmptyChild;
+    const bool h2 = qslab<2>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t2) && ch.z != kEmptyChild;
+    const bool h3 = qslab<3>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t3) && ch.w != kEmptyChild;
+    float ta = h0 ? t0 : inf, tb = h1 ? t1 : inf, tc = h2 ? t2 : inf, td = h3 ? t3 : inf;
+    int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
+    const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+    if (n_hit == 0) return st.sp == 0 ? kTraceDone : st.pop();
+    // sorting network for four keys; misses (inf) sink to the end
+    cswap(ta, ra, tb, rb);
+    cswap(tc, rc, td, rd);
+    cswap(ta, ra, tc, rc);
+    cswap(tb, rb, td, rd);
+    cswap(tb, rb, tc, rc);
+    if (n_hit > 3) st.push(rd);
+    if (n_hit > 2) st.push(rc);
+    if (n_hit > 1) st.push(rb);
+    return ra;
+}
+
+// Traversal of the 4-wide tree: each visit reads one 64-byte quantized node
+// (node_step4). A stack of kMaxTraceDepth entries holds 3 per level (the
+// builder caps the depth).
 //
 // The loop body is one step: a lane either visits an inner node or tests one
 // primitive of its current leaf, so a wave pays for at most one node visit and
@@ -216,11 +263,9 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
     hit.slot = 0;
     hit.t = hit.b0 = hit.b1 = hit.b2 = 0.f;
     uint32_t slot = kNoSlot;
-    int cur = 0;                // next node: inner node (>= 0), leaf (< 0) or kTraceDone
-    uint32_t pf = 0, pn = 0;    // primitives left in the current leaf: slots pf .. pf + pn - 1
+    int cur = 0;              // next node: inner node (>= 0), leaf (< 0) or kTraceDone
+    uint32_t pf = 0, pn = 0;  // primitives left in the current leaf
     st.sp = 0;
-    const float inf = __builtin_inff();
-    // one step per iteration: a lane either visits an inner node or tests one primitive
     while (true) {
         if (pn != 0) {
             if (COUNT) ++n_prim;
@@ -237,42 +282,9 @@ __device__ __forceinline__ bool trace_ray4(const GpuNode4Q *__restrict__ nodes, 
             ++pf;
             --pn;
         } else {
-            const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
-            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-            if (COUNT) n_box += 4;
-            const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
-            const float sx = __uint_as_float((w0.w & 0xffu) << 23),
-                        sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
-                        sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
-            const int4 ch = make_int4((int)w2.z, (int)w2.w, (int)w3.x, (int)w3.y);
-            float t0, t1, t2, t3;
-            const bool h0 = qslab<0>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t0) && ch.x != kEmptyChild;
-            const bool h1 = qslab<1>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t1) && ch.y != kEmptyChild;
-            const bool h2 = qslab<2>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t2) && ch.z != kEmptyChild;
-            const bool h3 = qslab<3>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t3) && ch.w != kEmptyChild;
-            float ta = h0 ? t0 : inf, tb = h1 ? t1 : inf, tc = h2 ? t2 : inf, td = h3 ? t3 : inf;
-            int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
-            const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-            if (n_hit > 0) {
-                cswap(ta, ra, tb, rb);
-                cswap(tc, rc, td, rd);
-                cswap(ta, ra, tc, rc);
-                cswap(tb, rb, td, rd);
-                cswap(tb, rb, tc, rc);
-                if (n_hit > 3) st.push(rd);
-                if (n_hit > 2) st.push(rc);
-                if (n_hit > 1) st.push(rb);
-                cur = ra;
-            } else {
-                cur = st.sp == 0 ? kTraceDone : st.pop();
-            }
+            cur = node_step4<COUNT, BLOCK>(nodes, cur, o, inv, t_max, st, n_box);
         }
-        // a leaf reached (or popped) becomes the current leaf; the node after it is popped now
-        if (pn == 0 && cur < 0 && cur != kTraceDone) {
-            pf = leaf_first(cur);
-            pn = leaf_count(cur);
-            cur = st.sp == 0 ? kTraceDone : st.pop();
-        }
+        take_leaf(cur, pf, pn, st);
         if (pn == 0 && cur == kTraceDone) break;
     }
     if (slot == kNoSlot) return false;

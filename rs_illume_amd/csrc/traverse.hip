@@ -1,6 +1,9 @@
-// Batched ray queries: closest hit, any hit and the instrumented (counting)
-// traversal, one ray per lane over the traversal in trace.h.
+// Batched ray queries over the traversal steps of trace.h: closest and any
+// hit on the four-child tree by persistent waves that refill their lanes from
+// a chunk of the batch; the two-child tree and the instrumented (counting)
+// traversal one ray per lane.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "kernels.h"
 #include "trace.h"
 
@@ -10,10 +13,11 @@ namespace illume {
 
 namespace {
 
-// Launch shape per node width (measured on the 2^20-triangle / 2^24-ray
-// workload): two-child nodes run best in 128-lane blocks at the natural
-// register count; four-child nodes in 64-lane blocks limited to 64 VGPRs
-// (8 waves per SIMD), which keeps more node fetches in flight.
+// Launch shape of the one-ray-per-lane kernel per node width (measured on the
+// 2^20-triangle / 2^24-ray workload): two-child nodes run best in 128-lane
+// blocks at the natural register count; four-child nodes in 64-lane blocks
+// limited to 64 VGPRs (8 waves per SIMD), which keeps more node fetches in
+// flight.
 template <int WIDTH>
 struct TraverseShape {
     static constexpr int kBlock = 128, kWaves = 1;
@@ -52,6 +56,101 @@ __global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse
     }
 }
 
+// Persistent batch traversal: each wave (one 64-lane block) owns a contiguous
+// chunk of rays. A lane that finishes its ray keeps its result until at least
+// kRefillMin lanes of the wave are idle; then those lanes write their results
+// and take the chunk's next rays (in lane order), so a wave's trip count
+// follows its chunk's total work instead of each group of 64 rays' slowest
+// ray. The grid is one wave per wave slot of the device. Waiting for a
+// quarter of the wave to be idle amortises the refill (ray load, reciprocal
+// direction, shear, result write) over many lanes: refilling at 4 idle lanes
+// or at 32 was slower on the 2^20-triangle / 2^24-ray workload.
+constexpr int kRefillMin = 16;
+
+constexpr int kRefillWaves = 8;  // waves per SIMD: at most 64 VGPRs, no spills
+
+template <bool ANY, int WIDTH>
+__global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
+    const void *__restrict__ nodes, const PrimRecord *__restrict__ prims, const illume_ray *__restrict__ rays,
+    uint32_t n, uint32_t per_wave, illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded) {
+    constexpr int kBlock = 64;
+    constexpr uint32_t kNoRay = 0xFFFFFFFFu;
+    __shared__ int lds_stack[kStackLds * kBlock];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t begin = blockIdx.x * per_wave;
+    if (begin >= n) return;
+    const uint32_t end = min(n, begin + per_wave);
+    uint32_t next = begin;  // the chunk's next ray (wave-uniform)
+    int spill[kSpill];
+    TraceStack<kBlock> st;
+    st.reset(lds_stack + lane, spill);
+    uint32_t ray = kNoRay, slot = kNoSlot, pf = 0, pn = 0, n_box = 0;
+    bool busy = false;
+    f3 o = mk(0.f, 0.f, 0.f), d = o, inv = o;
+    Shear sh{0, 0.f, 0.f, 0.f};
+    float t_max = 0.f;
+    int cur = kTraceDone;
+    while (true) {
+        const uint64_t idle = __ballot(!busy);
+        const int n_idle = __popcll(idle);
+        if (n_idle >= kRefillMin || n_idle == kBlock) {
+            if (!busy) {
+                if (ray != kNoRay) {
+                    if (ANY) {
+                        occluded[ray] = slot != kNoSlot ? 1 : 0;
+                    } else {
+                        TraceHit h;
+                        h.prim = 0xFFFFFFFFu;
+                        h.t = h.b0 = h.b1 = h.b2 = 0.f;
+                        if (slot != kNoSlot) finish_hit(prims, slot, o, sh, t_max, h);
+                        reinterpret_cast<float4 *>(hits)[ray] = make_float4(__uint_as_float(h.prim), h.t, h.b1, h.b2);
+                    }
+                    ray = kNoRay;
+                }
+                const uint32_t r = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (r < end) {
+                    const float4 *rp = reinterpret_cast<const float4 *>(rays + r);
+                    const float4 r0 = rp[0], r1 = rp[1];
+                    o = mk(r0.x, r0.y, r0.z);
+                    d = mk(r0.w, r1.x, r1.y);
+                    t_max = r1.z;
+                    inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+                    sh = make_shear(d);
+                    slot = kNoSlot;
+                    cur = 0;
+                    pf = pn = 0;
+                    st.sp = 0;
+                    ray = r;
+                    busy = true;
+                }
+            }
+            next = min(next + (uint32_t)n_idle, end);
+            if (__ballot(busy) == 0) break;  // chunk done, every result written
+        }
+        if (busy) {
+            if (pn != 0) {
+                float t;
+                if (prim_step<ANY>(prims, pf, o, d, sh, t_max, t)) {
+                    slot = pf;
+                    t_max = t;
+                    if (ANY) {
+                        pn = 1;
+                        cur = kTraceDone;
+                    }
+                }
+                ++pf;
+                --pn;
+            } else if (WIDTH == 4) {
+                cur = node_step4<false, kBlock>((const GpuNode4Q *)nodes, cur, o, inv, t_max, st, n_box);
+            } else {
+                cur = node_step2<false, kBlock>((const GpuNode *)nodes, cur, o, inv, t_max, st, n_box);
+            }
+            take_leaf(cur, pf, pn, st);
+            if (pn == 0 && cur == kTraceDone) busy = false;
+        }
+    }
+}
+
 __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -67,9 +166,30 @@ __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded
 
 int max_supported_depth() { return kMaxTraceDepth; }
 
+// Persistent launch on the four-child tree: one 64-lane wave per wave slot
+// (compute units x 4 SIMDs x waves per SIMD), each owning a chunk of the batch.
+hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
+                         illume_hit *hits, uint8_t *occluded, bool any, hipStream_t stream) {
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const uint32_t slots = (uint32_t)std::max(cus, 1) * 4u * (uint32_t)kRefillWaves;
+    const uint32_t waves = std::min((n + 63u) / 64u, slots);
+    const uint32_t per_wave = (n + waves - 1) / waves;
+    if (any)
+        traverse_refill_kernel<true, 4><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, nullptr, occluded);
+    else
+        traverse_refill_kernel<false, 4><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, hits, nullptr);
+    return hipGetLastError();
+}
+
+// One ray per lane: counting queries, and every query on the two-child tree
+// (whose cheaper steps do not repay the refill: 14.4 against 12.8 ms per
+// launch on the 2^20-triangle / 2^24-ray workload).
 template <int W>
-void launch_w(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n, illume_hit *hits,
-              uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {
+hipError_t launch_per_ray(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
+                          illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {
     constexpr int kBlock = TraverseShape<W>::kBlock;
     const uint32_t grid = (n + kBlock - 1) / kBlock;
     if (counts) {
@@ -82,6 +202,7 @@ void launch_w(const void *nodes, const PrimRecord *prims, const illume_ray *rays
     } else {
         traverse_kernel<false, false, W><<<grid, kBlock, 0, stream>>>(nodes, prims, rays, n, hits, nullptr, nullptr);
     }
+    return hipGetLastError();
 }
 
 hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims, bool empty, const illume_ray *rays,
@@ -93,11 +214,9 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
                                                               counts ? nullptr : (any ? occluded : nullptr), counts);
         return hipGetLastError();
     }
-    if (width == 4)
-        launch_w<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
-    else
-        launch_w<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
-    return hipGetLastError();
+    if (width == 4 && !counts) return launch_refill(nodes, prims, rays, n, hits, occluded, any, stream);
+    if (width == 4) return launch_per_ray<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
+    return launch_per_ray<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
 }
 
 }  // namespace illume
