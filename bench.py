@@ -515,6 +515,8 @@ def main():
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/traverse_pmc_1M_tris_16M_rays.json: rocprofv3 FETCH_SIZE + "
                                        "WRITE_SIZE bytes per launch (calibrated)" if traffic else None,
+                     "traffic_gbs": round(traffic / (ms_per_step * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                      "algorithmic_bytes_per_launch": round(alg_bytes * n),
                      "bytes_per_ray": round(alg_bytes, 1), "bytes_source": src,
                      "node_tests_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2),

@@ -179,3 +179,40 @@ def test_full_size_1m_tris_16m_rays(gpu, oracle, width, builder):
     res = compare_closest(h, ref, osc, rs)
     _say(res)
     _assert_parity(res, o, occ)
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_batch_sizes_across_wave_chunks(gpu, oracle, width):
+    """Batch queries on the four-child tree run as persistent waves, each
+    owning a chunk of the batch and refilling its lanes from it; the chunk size
+    depends on the batch size and the device's wave slots (64 lanes x 4 SIMDs
+    x 8 waves per compute unit). Every batch size — below one wave, exact
+    multiples of 64, one ray either side of filling every wave slot — gives each
+    ray exactly the hit and occlusion it gets in one large batch, and the large
+    batch matches the oracle."""
+    import torch
+    from rs_illume_amd import workloads as W
+    from rs_illume_amd.accel import BVHAccel, hits_to_numpy, rays_to_device
+    v, idx, sph = small_scene(16384, 64, 111, median_sigma=0.02)
+    bvh = BVHAccel(v, idx, sph, 4, width=width)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    full = 64 * 4 * 8 * cus
+    n_max = full + 1
+    _say("case batch sizes, width", width, "wave slots x 64 =", full)
+    rays = W.sphere_to_cube_rays(n_max, seed=112)
+    rd = rays_to_device(rays)
+    h_all = bvh.intersect(rd)
+    o_all = bvh.intersect_p(rd)
+    sample = np.arange(0, n_max, 7)
+    osc = oracle.OracleScene(v, idx, sph, 4)
+    rs = rays[sample]
+    res = compare_closest(hits_to_numpy(h_all)[sample], osc.intersect(rs), osc, rs)
+    _say(res)
+    _assert_parity(res, o_all.cpu().numpy()[sample], osc.intersect_p(rs))
+    for n in (1, 2, 63, 64, 65, 1000, 4097, full // 2 + 3, full - 1, full):
+        for off in (0, n_max - n):
+            sub = rd[off:off + n].contiguous()
+            h = bvh.intersect(sub)
+            o = bvh.intersect_p(sub)
+            assert torch.equal(h, h_all[off:off + n]), (n, off)
+            assert torch.equal(o, o_all[off:off + n]), (n, off)
