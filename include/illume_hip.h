@@ -164,8 +164,8 @@ typedef struct {
     uint64_t seed;
 } illume_render_cfg;
 
-/* flags: the Halton sampler instead of the random streams (illume_render and
- * illume_render_filtered; the seed is then unused). pbrt-v3's HaltonSampler
+/* flags: the Halton sampler instead of the random streams (illume_render,
+ * illume_render_filtered and the wavefront renderer; the seed is then unused). pbrt-v3's HaltonSampler
  * (PBR book 3e 7.4) over the reference's radical-inverse primitives
  * (core/low_discrepancy.rs:196-205, 422-466, 1505-1531): sample s of pixel
  * (x, y) is the global index that the pixel map of the sample bounds gives;
@@ -175,8 +175,8 @@ typedef struct {
  * dimension)). */
 #define ILLUME_RENDER_HALTON 1u
 
-/* flags: the Sobol' sampler instead of the random streams (illume_render and
- * illume_render_filtered; the seed is then unused). pbrt-v3's SobolSampler
+/* flags: the Sobol' sampler instead of the random streams (illume_render,
+ * illume_render_filtered and the wavefront renderer; the seed is then unused). pbrt-v3's SobolSampler
  * (PBR book 3e 7.7) over the reference's sobol_sample_float and
  * sobol_interval_to_index (core/low_discrepancy.rs:354-420) and generator
  * matrices equal to its SOBOL_MATRICES_32 / VD_C_SOBOL_MATRICES(_INV)
@@ -358,6 +358,20 @@ int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, con
                             float *rgb_sum, uint64_t *ray_counts, void *stream);
 int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
                                   const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream);
+/* The filtered film of illume_render_filtered on the wavefront pool: job j's
+ * camera samples cover filters[j]'s sample bounds and are splatted into its
+ * image of width*height*4 floats (r, g, b, weight sums) at film_offset;
+ * illume_film_resolve turns each image into RGB. Samplers as in
+ * illume_render_filtered (cfg.flags). Film::add_sample through
+ * Film::get_sample_bounds (core/film.rs:83-105, 125-185). */
+int illume_wavefront_render_batch_filtered(illume_wavefront *wf, const illume_job *jobs,
+                                           const illume_filter *const *filters, uint32_t n_jobs, illume_guide *guide,
+                                           const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts,
+                                           void *stream);
+int illume_wavefront_render_filtered(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
+                                     const illume_render_cfg *cfg, const illume_filter *filter, illume_guide *guide,
+                                     const illume_guide_cfg *gcfg, float *rgbw_sum, uint64_t *ray_counts,
+                                     void *stream);
 
 /* ------------------------------------------------------------------ measurement
  * No counterpart in the reference: a calibration probe for hardware read

@@ -25,6 +25,9 @@ hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, cons
 // sample bounds [x0, x0 + w) x [y0, y0 + h), over this device's table
 // (uploaded on first use); false if the table could not be made.
 bool sampler_launch(uint32_t kind, int x0, int y0, int w, int h, SamplerLaunch *out);
+// Sampler kind of illume_render_cfg.flags: 0 (random streams), kSamplerHalton,
+// kSamplerSobol, or ~0u for an invalid combination.
+uint32_t sampler_kind(uint32_t flags);
 
 // rgb[3p + c] = scale * (w != 0 ? max(0, rgbw[4p + c] * (1 / w)) : rgbw[4p + c]).
 hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream);

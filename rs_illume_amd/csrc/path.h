@@ -404,13 +404,6 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
     ps.active = true;
 }
 
-// Camera path of sample s of image pixel `pix` (stream = pixel index).
-__device__ __forceinline__ void path_start(const DevCamera &cam, uint64_t seed, uint64_t pix, uint32_t s,
-                                           PathState &ps) {
-    float fx, fy;
-    path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, SamplerLaunch{});
-}
-
 // One bounce of PathIntegrator::li after the closest-hit query (found, h).
 // With a proposal (gout != null, valid mixture, non-specular BSDF) the
 // continuation is one-sample MIS: the mixture with probability frac, else the

@@ -199,18 +199,13 @@ void scene_bounds(const illume_scene *s, float *b) {
 }
 bool scene_camera(const illume_camera &c, DevCamera *out);
 void filter_splat(const illume_filter *f, uint32_t width, uint32_t height, FilmSplat *out);
-}  // namespace illume
-
-namespace {
-// Sampler kind of illume_render_cfg.flags: 0 (random streams), kSamplerHalton,
-// kSamplerSobol, or ~0u for an invalid combination.
 uint32_t sampler_kind(uint32_t flags) {
     if (flags == 0) return 0;
     if (flags == ILLUME_RENDER_HALTON) return kSamplerHalton;
     if (flags == ILLUME_RENDER_SOBOL) return kSamplerSobol;
     return ~0u;
 }
-}  // namespace
+}  // namespace illume
 
 extern "C" {
 

@@ -2,6 +2,7 @@
 // and its device-side work counters.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "film.h"
 #include "path.h"
 
 namespace illume {
@@ -15,8 +16,11 @@ struct WfJob {
     unsigned long long path_begin;
     unsigned long long film_offset;  // float offset of the job's image in the film buffer
     uint64_t seed;
-    uint32_t spp, sample_offset, max_depth, pad;
+    uint32_t spp, sample_offset, max_depth;
+    uint32_t filtered;  // 1: camera samples over fs's sample bounds, splatted into 4-float pixels
     GuideCfg gc;
+    SamplerLaunch sl;   // low-discrepancy sampler of the job (table null: random streams)
+    FilmSplat fs;
 };
 
 struct WfCounters {
@@ -34,6 +38,7 @@ struct WfState {
     ulonglong2 *rng;   // PCG32 state, increment
     uint2 *flags;      // bounces, bit0 active | bit1 specular bounce
     uint32_t *pix, *job;
+    float2 *film_xy;   // film position of the path's camera sample (filtered jobs)
     float4 *hit0, *hit1;
     int32_t *guide_slot;     // row of the proposal queue, -1 if none
     float *guide_in;         // [capacity][6]

@@ -32,7 +32,9 @@ __device__ __forceinline__ void store_state(const WfState &w, uint32_t p, const 
     w.rng[p] = make_ulonglong2(ps.rng.state, ps.rng.inc);
     w.flags[p] = make_uint2(ps.bounces, (ps.active ? 1u : 0u) | (ps.specular ? 2u : 0u));
 }
-__device__ __forceinline__ void load_state(const WfState &w, uint32_t p, PathState &ps) {
+// table: the job's low-discrepancy table (null for the random streams); the
+// sampler's index and next dimension are the stored state and increment.
+__device__ __forceinline__ void load_state(const WfState &w, uint32_t p, const uint32_t *table, PathState &ps) {
     const float4 a = w.o_tmax[p], b = w.d_eta[p], c = w.beta[p], l = w.L[p];
     const ulonglong2 r = w.rng[p];
     const uint2 f = w.flags[p];
@@ -44,7 +46,7 @@ __device__ __forceinline__ void load_state(const WfState &w, uint32_t p, PathSta
     ps.L = mk(l.x, l.y, l.z);
     ps.rng.state = r.x;
     ps.rng.inc = r.y;
-    ps.rng.lds = nullptr;
+    ps.rng.lds = table;
     ps.bounces = f.x;
     ps.active = (f.y & 1u) != 0;
     ps.specular = (f.y & 2u) != 0;
@@ -93,12 +95,20 @@ __global__ void wf_regen_kernel(WfState w) {
     const unsigned long long g = c.next_path + i;
     const uint32_t j = find_job(w, g);
     const WfJob &job = w.jobs[j];
-    const unsigned long long n_pix = (unsigned long long)job.cam.width * job.cam.height;
+    // filtered jobs sample the filter's sample bounds (stream = index in them),
+    // the others one sample pixel per image pixel
+    const bool filt = job.filtered != 0;
+    const unsigned long long n_pix =
+        filt ? (unsigned long long)job.fs.wext * job.fs.hext : (unsigned long long)job.cam.width * job.cam.height;
     const unsigned long long local = g - job.path_begin;
     const uint64_t pix = local % n_pix;
+    const int x = filt ? job.fs.x0 + (int)(pix % job.fs.wext) : (int)(pix % job.cam.width);
+    const int y = filt ? job.fs.y0 + (int)(pix / job.fs.wext) : (int)(pix / job.cam.width);
     PathState ps;
-    path_start(job.cam, job.seed, pix, job.sample_offset + (uint32_t)(local / n_pix), ps);
+    float fx, fy;
+    path_start_at(job.cam, job.seed, pix, x, y, job.sample_offset + (uint32_t)(local / n_pix), ps, fx, fy, job.sl);
     store_state(w, slot, ps);
+    w.film_xy[slot] = make_float2(fx, fy);
     w.pix[slot] = (uint32_t)pix;
     w.job[slot] = j;
     w.n_recs[slot] = 0;
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
         TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x, spill);
         PathState ps;
-        load_state(w, p, ps);
+        load_state(w, p, job.sl.table, ps);
         const float4 h0 = w.hit0[p], h1 = w.hit1[p];
         TraceHit h;
         h.prim = __float_as_uint(h0.x);
@@ -227,10 +237,16 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
             w.q[parity ^ 1u][atomicAdd(&w.ctr->n_q[parity ^ 1u], 1u)] = p;
         } else {
             const f3 L = guard_radiance(ps.L);
-            float *px = film + job.film_offset + 3 * (size_t)w.pix[p];
-            atomicAdd(px, L.x);
-            atomicAdd(px + 1, L.y);
-            atomicAdd(px + 2, L.z);
+            if (job.filtered) {
+                const float2 fxy = w.film_xy[p];
+                film_splat(job.fs, film + job.film_offset, (int)job.cam.width, (int)job.cam.height, fxy.x, fxy.y, L.x,
+                           L.y, L.z);
+            } else {
+                float *px = film + job.film_offset + 3 * (size_t)w.pix[p];
+                atomicAdd(px, L.x);
+                atomicAdd(px + 1, L.y);
+                atomicAdd(px + 2, L.z);
+            }
             if (recs) emit_records(w, p, n_recs, L);
             w.flags[p] = make_uint2(ps.bounces, 0u);
             w.free_q[atomicAdd(&w.ctr->n_free, 1u)] = p;

@@ -11,6 +11,7 @@
 #include <vector>
 #include "../../include/illume_hip.h"
 #include "guide_kernels.h"
+#include "kernels.h"
 #include "wavefront.h"
 
 namespace illume {
@@ -21,6 +22,7 @@ const DevScene *scene_dev_device(const illume_scene *s);
 uint32_t scene_prims(const illume_scene *s);
 void scene_bounds(const illume_scene *s, float *b);
 bool scene_camera(const illume_camera &c, DevCamera *out);
+void filter_splat(const illume_filter *f, uint32_t width, uint32_t height, FilmSplat *out);
 const GuideDev *guide_dev(const illume_guide *g);
 int guide_train_strided(illume_guide *g, const float *records, uint32_t n, uint32_t stride, hipStream_t st);
 }  // namespace illume
@@ -92,6 +94,7 @@ int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavef
     if (e == hipSuccess) e = walloc(wf, &w.flags, P);
     if (e == hipSuccess) e = walloc(wf, &w.pix, P);
     if (e == hipSuccess) e = walloc(wf, &w.job, P);
+    if (e == hipSuccess) e = walloc(wf, &w.film_xy, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit0, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit1, P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_slot, P);
@@ -130,8 +133,15 @@ int illume_wavefront_set_tuning(illume_wavefront *wf, uint32_t block, uint32_t r
 
 uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf) { return wf ? wf->iterations : 0; }
 
-int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
-                                  const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
+}  // extern "C"
+
+namespace {
+
+// The batch render behind the entry points; filters == null: every job
+// unfiltered (3 floats per pixel), else filters[j] (non-null) for job j
+// (4 floats per pixel, splatted).
+int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filter *const *filters, uint32_t n_jobs,
+                 illume_guide *guide, const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
     if (!wf || !film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (gcfg && !(gcfg->guide_frac >= 0.f && gcfg->guide_frac <= 1.f))
         return abi_fail(ILLUME_ERR_INVALID_ARG, "guide_frac must be in [0, 1]");
@@ -150,17 +160,31 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
         const illume_job &in = jobs[j];
         if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
         if (in.cfg.max_depth > wf->max_depth) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth above the pool's");
-        if (in.cfg.flags != 0)
-            return abi_fail(ILLUME_ERR_UNSUPPORTED, "the wavefront renderer uses the random sampler (flags 0)");
+        const uint32_t kind = sampler_kind(in.cfg.flags);
+        if (kind == ~0u) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
         if ((uint64_t)in.cfg.sample_offset + in.cfg.spp > (1u << 24))
             return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+        if (filters && !filters[j]) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a filter");
         WfJob job;
         std::memset(&job, 0, sizeof(job));
         if (!scene_camera(in.camera, &job.cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
-        const unsigned long long n = scene_prims(in.scene) ? (unsigned long long)job.cam.width * job.cam.height *
-                                                                 in.cfg.spp
-                                                           : 0ull;
+        unsigned long long n_pix = (unsigned long long)job.cam.width * job.cam.height;
+        int x0 = 0, y0 = 0, sw = (int)job.cam.width, sh = (int)job.cam.height;
+        if (filters) {
+            if ((uint64_t)job.cam.width + 64 >= (1u << 24) || (uint64_t)job.cam.height + 64 >= (1u << 24))
+                return abi_fail(ILLUME_ERR_UNSUPPORTED, "image side beyond 2^24 - 64");
+            filter_splat(filters[j], job.cam.width, job.cam.height, &job.fs);
+            job.filtered = 1;
+            n_pix = (unsigned long long)job.fs.wext * job.fs.hext;
+            x0 = job.fs.x0;
+            y0 = job.fs.y0;
+            sw = (int)job.fs.wext;
+            sh = (int)job.fs.hext;
+        }
+        const unsigned long long n = scene_prims(in.scene) ? n_pix * in.cfg.spp : 0ull;
         if (n == 0) continue;
+        if (kind && !sampler_launch(kind, x0, y0, sw, sh, &job.sl))
+            return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
         job.scene = scene_dev_device(in.scene);
         job.path_begin = total;
         job.film_offset = in.film_offset;
@@ -265,6 +289,23 @@ int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, 
     return ILLUME_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
+                                  const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
+    return render_batch(wf, jobs, nullptr, n_jobs, guide, gcfg, film, ray_counts, stream);
+}
+
+int illume_wavefront_render_batch_filtered(illume_wavefront *wf, const illume_job *jobs,
+                                           const illume_filter *const *filters, uint32_t n_jobs, illume_guide *guide,
+                                           const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts,
+                                           void *stream) {
+    if (n_jobs && !filters) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    return render_batch(wf, jobs, filters, n_jobs, guide, gcfg, film, ray_counts, stream);
+}
+
 int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
                             const illume_render_cfg *cfg, illume_guide *guide, const illume_guide_cfg *gcfg,
                             float *rgb_sum, uint64_t *ray_counts, void *stream) {
@@ -275,7 +316,21 @@ int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, con
     job.camera = *camera;
     job.cfg = *cfg;
     job.film_offset = 0;
-    return illume_wavefront_render_batch(wf, &job, 1, guide, gcfg, rgb_sum, ray_counts, stream);
+    return render_batch(wf, &job, nullptr, 1, guide, gcfg, rgb_sum, ray_counts, stream);
+}
+
+int illume_wavefront_render_filtered(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
+                                     const illume_render_cfg *cfg, const illume_filter *filter, illume_guide *guide,
+                                     const illume_guide_cfg *gcfg, float *rgbw_sum, uint64_t *ray_counts,
+                                     void *stream) {
+    if (!scene || !camera || !cfg || !filter) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    illume_job job;
+    std::memset(&job, 0, sizeof(job));
+    job.scene = scene;
+    job.camera = *camera;
+    job.cfg = *cfg;
+    job.film_offset = 0;
+    return render_batch(wf, &job, &filter, 1, guide, gcfg, rgbw_sum, ray_counts, stream);
 }
 
 }  // extern "C"

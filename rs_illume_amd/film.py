@@ -103,8 +103,10 @@ class Filter:
 def render_filtered(integrator, scene, filt, rgbw=None, spp=None, sample_offset=0, ray_counts=None, stream=None):
     """Adds the filtered film sums of ``spp`` samples per sample pixel to rgbw
     (device float32 (H, W, 4): weighted RGB and weight) and returns it.
-    ``integrator`` supplies camera, max_depth and seed (a PathIntegrator)."""
-    from .render import RenderCfg, _p
+    ``integrator`` supplies camera, max_depth, seed and sampler: a
+    PathIntegrator (megakernel) or a WavefrontPathIntegrator (its pool, scene
+    default and proposal)."""
+    from .render import RenderCfg, WavefrontPathIntegrator, _declare_wf, _p
     w, h = integrator.resolution
     if rgbw is None:
         rgbw = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
@@ -113,7 +115,16 @@ def render_filtered(integrator, scene, filt, rgbw=None, spp=None, sample_offset=
     cfg = RenderCfg(integrator.spp if spp is None else int(spp), integrator.max_depth, int(sample_offset),
                     integrator.flags, integrator.seed)
     rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
-    _lib.check(_declare().illume_render_filtered(scene.handle, _p(integrator.camera), ctypes.byref(cfg), filt.handle,
+    if isinstance(integrator, WavefrontPathIntegrator):
+        sc = scene if scene is not None else integrator.scene
+        g = integrator.guide.handle if integrator.guide is not None else None
+        _lib.check(_declare_wf().illume_wavefront_render_filtered(
+            integrator.wf.handle, sc.handle, _p(integrator.camera), ctypes.byref(cfg), filt.handle, g,
+            ctypes.byref(integrator.gcfg), ctypes.c_void_p(rgbw.data_ptr()), rc, _stream_handle(stream)),
+            "illume_wavefront_render_filtered")
+        return rgbw
+    _declare()
+    _lib.check(_lib.lib().illume_render_filtered(scene.handle, _p(integrator.camera), ctypes.byref(cfg), filt.handle,
                                                  ctypes.c_void_p(rgbw.data_ptr()), rc, _stream_handle(stream)),
                "illume_render_filtered")
     return rgbw

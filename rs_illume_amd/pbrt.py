@@ -1048,23 +1048,21 @@ def write_text(desc, camera, spp=16, max_depth=5, filename=None, pixel_filter=No
 def render(loaded, spp=None, wavefront=False, seed=0, builder="SAH"):
     """Render a loaded scene on the GPU: (H, W, 3) float32 tensor of the
     pixel values (mean radiance through the scene's pixel filter, with the
-    scene's sampler). The wavefront renderer films with the default box
-    filter and the random sampler only."""
+    scene's sampler), by the megakernel or the wavefront renderer."""
     from .render import PathIntegrator, Scene, WavefrontPathIntegrator
     spp = loaded.spp if spp is None else spp
     sc = Scene(loaded.desc, builder=builder)
-    if loaded.filter is not None:
-        if wavefront:
-            raise PbrtError("the wavefront renderer films with the box filter of radius 0.5 only")
-        from .film import Filter, render_filtered, resolve
+    if wavefront:
+        pi = WavefrontPathIntegrator(sc, loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed,
+                                     max_paths=1 << 20, sampler=loaded.sampler)
+    else:
         pi = PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed, sampler=loaded.sampler)
+    if loaded.filter is not None:
+        from .film import Filter, render_filtered, resolve
         f = loaded.filter.reshape(-1)[0]
         return resolve(render_filtered(pi, sc, Filter(int(f["type"]), radius=tuple(f["radius"]), alpha=f["alpha"],
                                                       b=f["b"], c=f["c"], tau=f["tau"])))
-    if wavefront:
-        return WavefrontPathIntegrator(sc, loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed).render()
-    return PathIntegrator(loaded.camera, spp=spp, max_depth=loaded.max_depth, seed=seed,
-                          sampler=loaded.sampler).render(sc)
+    return pi.render(sc)
 
 
 def main(argv=None):

@@ -202,6 +202,12 @@ def _declare_wf():
                                           vp]
     L.illume_wavefront_render_batch.restype = i32
     L.illume_wavefront_render_batch.argtypes = [vp, ctypes.POINTER(JobC), u32, vp, ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L.illume_wavefront_render_batch_filtered.restype = i32
+    L.illume_wavefront_render_batch_filtered.argtypes = [vp, ctypes.POINTER(JobC), ctypes.POINTER(vp), u32, vp,
+                                                         ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L.illume_wavefront_render_filtered.restype = i32
+    L.illume_wavefront_render_filtered.argtypes = [vp, vp, vp, ctypes.POINTER(RenderCfg), vp, vp,
+                                                   ctypes.POINTER(GuideCfg), vp, vp, vp]
     L._wf_declared = True
     return L
 
@@ -245,30 +251,47 @@ class Wavefront:
         except Exception:
             pass
 
-    def render_batch(self, jobs, film=None, guide=None, gcfg=None, ray_counts=None, stream=None):
-        """jobs: list of (Scene, camera, spp, max_depth, seed[, sample_offset]).
+    def render_batch(self, jobs, film=None, guide=None, gcfg=None, ray_counts=None, stream=None, filters=None):
+        """jobs: list of (Scene, camera, spp, max_depth, seed[, sample_offset[,
+        sampler]]), sampler one of ``SAMPLERS`` (default "random").
         Returns (film, offsets): film is a flat float32 device tensor holding
-        every job's radiance sums (H*W*3 each) at its offset."""
+        every job's radiance sums (H*W*3 each) at its offset. With
+        ``filters`` (one :class:`rs_illume_amd.film.Filter` per job) each
+        job's image is its filtered film, H*W*4 floats (weighted RGB and
+        weight; :func:`rs_illume_amd.film.resolve` gives the pixels)."""
+        if filters is not None and len(filters) != len(jobs):
+            raise ValueError("one filter per job")
+        per_px = 3 if filters is None else 4
         arr = (JobC * len(jobs))()
         offsets, off = [], 0
         for i, jb in enumerate(jobs):
             sc, cam, spp, depth, seed = jb[:5]
             so = jb[5] if len(jb) > 5 else 0
+            smp = jb[6] if len(jb) > 6 else "random"
+            if smp not in SAMPLERS:
+                raise ValueError("sampler must be one of %s" % (tuple(SAMPLERS),))
             c = _camera_c(cam)
             arr[i].scene = sc.handle
             arr[i].camera = c
-            arr[i].cfg = RenderCfg(int(spp), int(depth), int(so), 0, int(seed))
+            arr[i].cfg = RenderCfg(int(spp), int(depth), int(so), SAMPLERS[smp], int(seed))
             arr[i].film_offset = off
             offsets.append(off)
-            off += c.width * c.height * 3
+            off += c.width * c.height * per_px
         if film is None:
             film = torch.zeros(off, dtype=torch.float32, device="cuda")
         g = guide.handle if guide is not None else None
         gc = gcfg if gcfg is not None else GuideCfg(0.0, 0, 0, 0.0)
         rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
-        _lib.check(_declare_wf().illume_wavefront_render_batch(self._h, arr, len(jobs), g, ctypes.byref(gc),
-                                                               ctypes.c_void_p(film.data_ptr()), rc,
-                                                               _stream_handle(stream)), "illume_wavefront_render_batch")
+        if filters is None:
+            _lib.check(_declare_wf().illume_wavefront_render_batch(self._h, arr, len(jobs), g, ctypes.byref(gc),
+                                                                   ctypes.c_void_p(film.data_ptr()), rc,
+                                                                   _stream_handle(stream)),
+                       "illume_wavefront_render_batch")
+        else:
+            fh = (ctypes.c_void_p * len(jobs))(*[f.handle.value for f in filters])
+            _lib.check(_declare_wf().illume_wavefront_render_batch_filtered(
+                self._h, arr, fh, len(jobs), g, ctypes.byref(gc), ctypes.c_void_p(film.data_ptr()), rc,
+                _stream_handle(stream)), "illume_wavefront_render_batch_filtered")
         return film, offsets
 
 
@@ -282,8 +305,8 @@ class WavefrontPathIntegrator(PathIntegrator):
     ``min_alpha``."""
 
     def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
-                 train=False, train_batch=65536, wavefront=None, min_alpha=0.0):
-        super().__init__(camera, spp, max_depth, seed)
+                 train=False, train_batch=65536, wavefront=None, min_alpha=0.0, sampler="random"):
+        super().__init__(camera, spp, max_depth, seed, sampler)
         self.scene = scene
         self.guide = guide
         steps = 16 if train is True else int(train)

@@ -1,7 +1,9 @@
 """The Halton and Sobol' samplers on the GPU (illume_render_cfg.flags =
 ILLUME_RENDER_HALTON / ILLUME_RENDER_SOBOL) against the oracle's at equal
 samples, through the plain and the filtered film, with global indices below
-and above 2^32; and their lower error than the random sampler at equal spp."""
+and above 2^32; their lower error than the random sampler at equal spp; and
+the wavefront renderer (samplers, filtered film, mixed batches) equal to the
+megakernel."""
 import sys
 
 import numpy as np
@@ -93,17 +95,90 @@ def test_low_discrepancy_lowers_error(gpu):
     assert errs["sobol"] < 0.75 * errs["random"], errs
 
 
-def test_wavefront_rejects_low_discrepancy_and_bad_flags(gpu):
+@pytest.mark.parametrize("sampler", ["halton", "sobol"])
+def test_wavefront_equals_megakernel(gpu, sampler):
+    """The wavefront renderer keeps the sampler's index and next dimension in
+    each path slot and restarts paths in any order: the image and the ray
+    counts equal the megakernel's at the same samples."""
+    import torch
+    from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
+    _say("case wavefront", sampler)
+    for desc, depth in ((S.cornell_box(0), 5), (S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625), 8)):
+        sc = Scene(desc)
+        cam = S.cornell_camera(40, 36)
+        c1 = torch.zeros(2, dtype=torch.int64, device="cuda")
+        c2 = torch.zeros(2, dtype=torch.int64, device="cuda")
+        mk = PathIntegrator(cam, spp=6, max_depth=depth, sampler=sampler).render_sum(sc, sample_offset=3,
+                                                                                   ray_counts=c1)
+        wf = WavefrontPathIntegrator(sc, cam, spp=6, max_depth=depth, max_paths=3000, sampler=sampler)
+        ws = wf.render_sum(sample_offset=3, ray_counts=c2)
+        torch.cuda.synchronize()
+        a, b = mk.cpu().numpy(), ws.cpu().numpy()
+        _say("max abs diff", np.abs(a - b).max(), "rays", c1.cpu().numpy(), c2.cpu().numpy())
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-5)
+        assert torch.equal(c1, c2)
+
+
+@pytest.mark.parametrize("sampler", ["random", "halton", "sobol"])
+def test_wavefront_filtered_equals_megakernel(gpu, sampler):
+    """Filtered film on the wavefront pool: camera samples over the filter's
+    sample bounds (from (-2, -2) for the Mitchell filter of radius 2), each
+    splatted at its film position when its path ends."""
+    import torch
+    from rs_illume_amd.film import Filter, render_filtered
+    from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
+    desc, cam = S.cornell_box(0), S.cornell_camera(40, 30)
+    _say("case wavefront filtered", sampler)
+    sc = Scene(desc)
+    f = Filter("mitchell")
+    mk = render_filtered(PathIntegrator(cam, spp=4, max_depth=5, seed=9, sampler=sampler), sc, f)
+    wf = render_filtered(WavefrontPathIntegrator(sc, cam, spp=4, max_depth=5, seed=9, max_paths=2000,
+                                                 sampler=sampler), sc, f)
+    torch.cuda.synchronize()
+    a, b = mk.cpu().numpy(), wf.cpu().numpy()
+    _say("max abs diff", np.abs(a - b).max())
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+def test_wavefront_batch_mixes_samplers_and_filters(gpu):
+    """One pool, jobs of different sizes, samplers and filters: each job's
+    image is the one its own megakernel render gives."""
+    from rs_illume_amd.film import Filter, render_filtered
+    from rs_illume_amd.render import PathIntegrator, Scene, Wavefront
+    _say("case wavefront batch, samplers and filters")
+    scenes = [Scene(S.cornell_box(0)), Scene(S.cornell_box(1, glossy=True, light_size=0.2, light_L=26.5625))]
+    cams = [S.cornell_camera(36, 28), S.cornell_camera(20, 44)]
+    jobs = [(scenes[0], cams[0], 5, 5, 1, 0, "halton"), (scenes[1], cams[1], 3, 8, 2, 4, "sobol"),
+            (scenes[0], cams[1], 4, 5, 3, 0, "random")]
+    filters = [Filter("gaussian"), Filter("box"), Filter("triangle", radius=(1.5, 1.0))]
+    wf = Wavefront(max_paths=2048, max_depth=8)
+    film, offs = wf.render_batch(jobs)
+    film = film.cpu().numpy()
+    filt, foffs = wf.render_batch(jobs, filters=filters)
+    filt = filt.cpu().numpy()
+    for (sc, cam, spp, depth, seed, so, smp), off, foff, f in zip(jobs, offs, foffs, filters):
+        w, h = int(cam["width"][0]), int(cam["height"][0])
+        pi = PathIntegrator(cam, spp=spp, max_depth=depth, seed=seed, sampler=smp)
+        ref = pi.render_sum(sc, sample_offset=so).cpu().numpy()
+        got = film[off:off + w * h * 3].reshape(h, w, 3)
+        assert np.allclose(ref, got, rtol=1e-5, atol=1e-5), (smp, np.abs(ref - got).max())
+        fref = render_filtered(pi, sc, f, sample_offset=so).cpu().numpy()
+        fgot = filt[foff:foff + w * h * 4].reshape(h, w, 4)
+        assert np.allclose(fref, fgot, rtol=1e-5, atol=1e-5), (smp, np.abs(fref - fgot).max())
+
+
+def test_bad_flags_rejected(gpu):
     from rs_illume_amd._lib import IllumeError
     from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
     sc = Scene(S.cornell_box(0))
-    wpi = WavefrontPathIntegrator(sc, S.cornell_camera(16, 16), spp=1)
-    for smp in ("halton", "sobol"):
-        wpi.sampler = smp
-        with pytest.raises(IllumeError):
-            wpi.render()
+
     class BothFlags(PathIntegrator):
         flags = 3  # ILLUME_RENDER_HALTON | ILLUME_RENDER_SOBOL: not a sampler
 
+    class BothFlagsWf(WavefrontPathIntegrator):
+        flags = 3
+
     with pytest.raises(IllumeError):
         BothFlags(S.cornell_camera(16, 16), spp=1).render(sc)
+    with pytest.raises(IllumeError):
+        BothFlagsWf(sc, S.cornell_camera(16, 16), spp=1, max_paths=1024).render()

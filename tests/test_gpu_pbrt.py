@@ -158,3 +158,20 @@ def test_command_line_filtered(gpu, tmp_path, oracle):
     rm = S.rel_mse(img, ref)
     _say("gaussian relMSE", rm)
     assert rm <= REL_MSE_BAR
+
+
+@pytest.mark.parametrize("sampler", ["random", "halton"])
+def test_command_line_wavefront_filtered(gpu, tmp_path, sampler):
+    """--wavefront renders a scene's pixel filter and sampler on the wavefront
+    pool: the same image as the megakernel (float-atomic order aside)."""
+    from rs_illume_amd.film import filter_desc
+    from rs_illume_amd.imageio import read_pfm
+    desc, cam = S.cornell_box(0), S.cornell_camera(36, 28)
+    f = filter_desc("mitchell")
+    (tmp_path / "m.pbrt").write_text(P.write_text(desc, cam, spp=4, max_depth=5, pixel_filter=f, sampler=sampler))
+    _say("case pbrt main --wavefront", sampler)
+    P.main([str(tmp_path / "m.pbrt"), "-o", str(tmp_path / "mk.pfm")])
+    P.main([str(tmp_path / "m.pbrt"), "-o", str(tmp_path / "wf.pfm"), "--wavefront"])
+    a, b = read_pfm(str(tmp_path / "mk.pfm")), read_pfm(str(tmp_path / "wf.pfm"))
+    _say("max abs diff", np.abs(a - b).max())
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-5)
