@@ -1067,10 +1067,12 @@ def render(loaded, spp=None, wavefront=False, seed=0, builder="SAH"):
 
 def main(argv=None):
     import argparse
-    from .imageio import write_pfm
-    ap = argparse.ArgumentParser(description="render a pbrt-v3 scene file on the GPU and write a PFM image")
+    from .imageio import write_image
+    ap = argparse.ArgumentParser(description="render a pbrt-v3 scene file on the GPU and write the image "
+                                             "(.pfm, .png or .bmp by the output's suffix)")
     ap.add_argument("scene")
-    ap.add_argument("-o", "--output", help="output .pfm (default: the Film filename with a .pfm suffix)")
+    ap.add_argument("-o", "--output", help="output .pfm, .png or .bmp (default: the Film filename; a suffix "
+                                           "other than those becomes .pfm)")
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--wavefront", action="store_true")
     ap.add_argument("--builder", default="SAH", choices=("SAH", "LBVH"))
@@ -1079,8 +1081,13 @@ def main(argv=None):
     for w in ld.warnings:
         print("warning:", w)
     img = render(ld, a.spp, a.wavefront, builder=a.builder)
-    out = a.output or os.path.splitext(ld.filename or "pbrt")[0] + ".pfm"
-    write_pfm(out, img.cpu().numpy())
+    out = a.output
+    if out is None:
+        out = ld.filename or "pbrt.pfm"
+        if os.path.splitext(out)[1].lower() not in (".pfm", ".png", ".bmp"):
+            print("warning: no writer for %r; writing PFM" % out)
+            out = os.path.splitext(out)[0] + ".pfm"
+    write_image(out, img.cpu().numpy())
     print("wrote", out)
 
 

@@ -3,6 +3,7 @@ re-read scene traces bit-identical hits and renders the same image as the
 programmatic one, and a
 hand-written file using includes, transforms, instancing, named materials, a
 PLY mesh and the sky matches the oracle at equal sampler streams."""
+import os
 import sys
 
 import numpy as np
@@ -175,3 +176,25 @@ def test_command_line_wavefront_filtered(gpu, tmp_path, sampler):
     a, b = read_pfm(str(tmp_path / "mk.pfm")), read_pfm(str(tmp_path / "wf.pfm"))
     _say("max abs diff", np.abs(a - b).max())
     assert np.allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+def test_command_line_writes_png_and_bmp(gpu, tmp_path):
+    """The output suffix picks the format (imageio.rs:30-67): the PNG and BMP
+    of a render hold the sRGB bytes of the same render's PFM."""
+    from PIL import Image
+    from rs_illume_amd.imageio import read_pfm, to_bytes
+    desc, cam = S.cornell_box(0), S.cornell_camera(24, 20)
+    (tmp_path / "c.pbrt").write_text(P.write_text(desc, cam, spp=2, max_depth=5, filename="c.png"))
+    _say("case pbrt main png/bmp")
+    P.main([str(tmp_path / "c.pbrt"), "-o", str(tmp_path / "c.pfm")])
+    P.main([str(tmp_path / "c.pbrt"), "-o", str(tmp_path / "c.bmp")])
+    cwd = os.getcwd()
+    os.chdir(str(tmp_path))
+    try:
+        P.main([str(tmp_path / "c.pbrt")])  # default: the Film's own filename, c.png
+    finally:
+        os.chdir(cwd)
+    want = to_bytes(read_pfm(str(tmp_path / "c.pfm")))
+    for name in ("c.png", "c.bmp"):
+        got = np.asarray(Image.open(str(tmp_path / name)).convert("RGB"))
+        assert np.array_equal(got, want), name

@@ -554,3 +554,47 @@ def test_pfm_read_variants(tmp_path):
             parse_pfm(bad)
     with pytest.raises(ValueError):
         pfm_bytes(np.zeros((2, 2)))
+
+
+# ------------------------------------------------------------------------- 8-bit images
+def test_gamma_and_bytes_closed_forms():
+    """imageio.rs:42-55 with gamma_correct (pbrt.rs:176-183): the linear
+    segment below 0.0031308, the power segment above it, rounding by +0.5 and
+    truncation, clamping, and NaN -> 0 (Rust's saturating float cast)."""
+    from rs_illume_amd.imageio import gamma_correct, to_bytes
+    v = np.array([0.0, 0.001, 0.0031308, 0.18, 0.5, 1.0], np.float32)
+    v64 = v.astype(np.float64)
+    exact = np.where(v64 < 0.0031308, 12.92 * v64, 1.055 * v64 ** (1 / 2.4) - 0.055)
+    assert np.allclose(gamma_correct(v), exact, rtol=3e-7, atol=0)
+    # float32 as in the reference: 1.055f * 1 - 0.055f rounds to 1 - 2^-24
+    assert gamma_correct(np.float32(1.0)) == np.float32(1.055) - np.float32(0.055) == np.float32(1 - 2 ** -24)
+    # 0.5 -> 0.7354 -> 188.03; 0.18 -> 0.4614 -> 118.16; 0.001 -> 0.01292 -> 3.79
+    assert to_bytes(np.array([0, 0.001, 0.18, 0.5, 1, 1e6, -3, np.nan, np.inf, -np.inf],
+                             np.float32)).tolist() == [0, 3, 118, 188, 255, 255, 0, 0, 255, 0]
+    # every byte value is reached, and the map is monotone
+    ramp = np.linspace(0, 1, 4097, dtype=np.float32)
+    b = to_bytes(ramp)
+    assert set(b.tolist()) == set(range(256)) and np.all(np.diff(b.astype(int)) >= 0)
+
+
+def test_png_and_bmp_decode_to_the_bytes(tmp_path):
+    """An independent decoder (Pillow) reads back exactly the 8-bit values,
+    top row first, for both formats; write_image picks the format by suffix."""
+    import io
+    from PIL import Image
+    from rs_illume_amd.imageio import bmp_bytes, png_bytes, to_bytes, write_image
+    img = np.random.default_rng(3).random((9, 13, 3)).astype(np.float32) * 1.3
+    img[0, 0] = [np.nan, -1, 7]
+    for fn, fmt in ((png_bytes, "PNG"), (bmp_bytes, "BMP")):
+        im = Image.open(io.BytesIO(fn(img)))
+        assert im.format == fmt and im.size == (13, 9)
+        assert np.array_equal(np.asarray(im.convert("RGB")), to_bytes(img))
+    for ext in ("png", "bmp", "pfm"):
+        write_image(str(tmp_path / ("a." + ext)), img)
+    assert np.array_equal(np.asarray(Image.open(str(tmp_path / "a.png"))), to_bytes(img))
+    assert np.array_equal(read_pfm(str(tmp_path / "a.pfm"))[1:], img[1:])
+    for bad in ("a.tga", "a.exr", "a"):
+        with pytest.raises(ValueError):
+            write_image(str(tmp_path / bad), img)
+    with pytest.raises(ValueError):
+        png_bytes(np.zeros((0, 4, 3), np.float32))
