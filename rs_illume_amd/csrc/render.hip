@@ -9,9 +9,11 @@
 // extension rays, any-hit shadow rays and MIS rays share the traversal of
 // trace.h). Lanes of a wave hold neighbouring pixels of the same sample
 // index, so primary rays are coherent. Each finished path adds its radiance
-// to the caller's float RGB sum buffer with float atomics; ray counts are
-// reduced per block in LDS before one 64-bit atomic per block.
+// to the caller's float RGB sum buffer with float atomics, or, through a
+// reconstruction filter, writes its sample for the gather pass that films it;
+// ray counts are reduced per block in LDS before one 64-bit atomic per block.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "film.h"
 #include "kernels.h"
 #include "path.h"
@@ -25,8 +27,9 @@ namespace {
 constexpr int kRenderBlock = 128;
 
 // FILTERED: camera samples over the sample bounds of `fs` (stream = index in
-// those bounds) splatted through the filter into 4-float pixels; otherwise
-// one sample pixel per image pixel, added to 3-float pixels.
+// those bounds), each written to the sample buffer `film` (radiance and film
+// position at its path number) for gather_kernel; otherwise one sample pixel
+// per image pixel, added to 3-float pixels.
 template <int BLOCK, bool FILTERED>
 __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
                                                         uint64_t seed, uint32_t sample_offset, float *__restrict__ film,
@@ -61,7 +64,9 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
         }
         const f3 L = guard_radiance(ps.L);
         if (FILTERED) {
-            film_splat(fs, film, (int)cam.width, (int)cam.height, fx, fy, L.x, L.y, L.z);
+            // the gather pass films it (film = the sample buffer: [gid] (L, fx), then [n + gid] fy)
+            reinterpret_cast<float4 *>(film)[gid] = make_float4(L.x, L.y, L.z, fx);
+            film[4 * n_pix * spp + gid] = fy;
         } else {
             atomicAdd(film + 3 * pix, L.x);
             atomicAdd(film + 3 * pix + 1, L.y);
@@ -77,6 +82,52 @@ __global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera ca
             atomicAdd(counters + 1, (unsigned long long)blk_counts[1]);
         }
     }
+}
+
+// The filtered film of a chunk of spp samples per sample pixel, by gathering:
+// one lane per image pixel visits, in a fixed order (sample, then sample-pixel
+// rows and columns), every sample whose sample pixel lies within the filter
+// radius (plus one pixel of margin), applies film_splat's footprint test and
+// weight to it with the same float operations, and adds the sums to its RGBW
+// pixel once. No atomics: the film is the same bits on every run.
+__global__ __launch_bounds__(256) void gather_kernel(FilmSplat fs, const float4 *__restrict__ smp_l,
+                                                     const float *__restrict__ smp_y, uint32_t spp, uint32_t width,
+                                                     uint32_t height, float *__restrict__ rgbw) {
+    const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= width || y >= height) return;
+    const uint64_t n_pix = (uint64_t)fs.wext * fs.hext;
+    const int sx0 = max((int)floorf((float)x + 0.5f - fs.rx) - 1, fs.x0);
+    const int sx1 = min((int)floorf((float)x + 0.5f + fs.rx) + 1, fs.x0 + (int)fs.wext - 1);
+    const int sy0 = max((int)floorf((float)y + 0.5f - fs.ry) - 1, fs.y0);
+    const int sy1 = min((int)floorf((float)y + 0.5f + fs.ry) + 1, fs.y0 + (int)fs.hext - 1);
+    float r = 0.f, g = 0.f, b = 0.f, wsum = 0.f;
+    for (uint32_t s = 0; s < spp; ++s) {
+        for (int sy = sy0; sy <= sy1; ++sy) {
+            const uint64_t row = (uint64_t)s * n_pix + (uint64_t)(sy - fs.y0) * fs.wext;
+            for (int sx = sx0; sx <= sx1; ++sx) {
+                const uint64_t i = row + (uint64_t)(sx - fs.x0);
+                const float4 a = smp_l[i];
+                const float dx = a.w - 0.5f, dy = smp_y[i] - 0.5f;
+                const int x0 = max((int)ceilf(dx - fs.rx), 0), x1 = min((int)floorf(dx + fs.rx) + 1, (int)width);
+                const int y0 = max((int)ceilf(dy - fs.ry), 0), y1 = min((int)floorf(dy + fs.ry) + 1, (int)height);
+                if ((int)x < x0 || (int)x >= x1 || (int)y < y0 || (int)y >= y1) continue;
+                const float fy = fabsf(((float)y - dy) * fs.iry * (float)kFilterTable);
+                const int iy = min(kFilterTable - 1, (int)floorf(fy));
+                const float fx = fabsf(((float)x - dx) * fs.irx * (float)kFilterTable);
+                const int ix = min(kFilterTable - 1, (int)floorf(fx));
+                const float w = fs.table[iy * kFilterTable + ix];
+                r += a.x * w;
+                g += a.y * w;
+                b += a.z * w;
+                wsum += w;
+            }
+        }
+    }
+    float *p = rgbw + 4 * ((size_t)y * width + x);
+    p[0] += r;
+    p[1] += g;
+    p[2] += b;
+    p[3] += wsum;
 }
 
 // Film::write_image's per-pixel normalisation (film.rs:187-226).
@@ -114,13 +165,29 @@ hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp,
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,
                                   uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
                                   unsigned long long *counters, const SamplerLaunch &hl, hipStream_t stream) {
-    const uint64_t n = (uint64_t)fs.wext * fs.hext * spp;
-    if (n == 0) return hipSuccess;
-    const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
-    if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
-    render_kernel<kRenderBlock, true><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
-        sc, cam, spp, max_depth, seed, sample_offset, rgbw, counters, fs, hl);
-    return hipGetLastError();
+    const uint64_t n_pix = (uint64_t)fs.wext * fs.hext;
+    if (n_pix == 0 || spp == 0) return hipSuccess;
+    // chunks of at most 2^24 samples (20 bytes each) unless one sample per pixel is more
+    constexpr uint64_t kChunkSamples = 1ull << 24;
+    const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(spp, kChunkSamples / n_pix));
+    const uint64_t cap = n_pix * chunk;
+    if ((cap + kRenderBlock - 1) / kRenderBlock > 0x7fffffffULL) return hipErrorInvalidValue;
+    float *buf = nullptr;
+    hipError_t e = hipMallocAsync((void **)&buf, cap * 5 * sizeof(float), stream);
+    if (e != hipSuccess) return e;
+    const dim3 ggrid((cam.width + 15) / 16, (cam.height + 15) / 16);
+    for (uint32_t s0 = 0; s0 < spp && e == hipSuccess; s0 += chunk) {
+        const uint32_t k = std::min(chunk, spp - s0);
+        const uint64_t n = n_pix * k;
+        render_kernel<kRenderBlock, true><<<(uint32_t)((n + kRenderBlock - 1) / kRenderBlock), kRenderBlock, 0,
+                                           stream>>>(sc, cam, k, max_depth, seed, sample_offset + s0, buf, counters,
+                                                     fs, hl);
+        gather_kernel<<<ggrid, 256, 0, stream>>>(fs, reinterpret_cast<const float4 *>(buf), buf + 4 * n, k,
+                                                 cam.width, cam.height, rgbw);
+        e = hipGetLastError();
+    }
+    const hipError_t ef = hipFreeAsync(buf, stream);
+    return e != hipSuccess ? e : ef;
 }
 
 hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream) {
