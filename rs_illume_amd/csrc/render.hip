@@ -30,8 +30,12 @@ constexpr int kRenderBlock = 128;
 // those bounds), each written to the sample buffer `film` (radiance and film
 // position at its path number) for gather_kernel; otherwise one sample pixel
 // per image pixel, added to 3-float pixels.
+// At least 3 waves per SIMD (168 VGPRs, one or two spilled): the traversals
+// inside a bounce hide their node fetches better than at the 2 waves the
+// natural 182 VGPRs give (Cornell box 256^2 x 16 spp 1.31 -> 1.04 ms, a 256^2
+// terrain view 1.62 -> 1.45 ms); 4 waves spill 40+ VGPRs and lose on the terrain.
 template <int BLOCK, bool FILTERED>
-__global__ __launch_bounds__(BLOCK) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
+__global__ __launch_bounds__(BLOCK, 3) void render_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t max_depth,
                                                         uint64_t seed, uint32_t sample_offset, float *__restrict__ film,
                                                         unsigned long long *__restrict__ counters, FilmSplat fs,
                                                         SamplerLaunch hl) {
