@@ -13,17 +13,12 @@ namespace illume {
 
 namespace {
 
-// Launch shape of the one-ray-per-lane kernel per node width (measured on the
-// 2^20-triangle / 2^24-ray workload): two-child nodes run best in 128-lane
-// blocks at the natural register count; four-child nodes in 64-lane blocks
-// limited to 64 VGPRs (8 waves per SIMD), which keeps more node fetches in
-// flight.
+// Launch shape of the one-ray-per-lane kernel (measured on the 2^20-triangle
+// / 2^24-ray workload): 64-lane blocks limited to 64 VGPRs (8 waves per SIMD),
+// which keeps more node fetches in flight (two-child tree: any hit 9.9 ->
+// 9.3 ms against 128-lane blocks at its natural 72-80 VGPRs).
 template <int WIDTH>
 struct TraverseShape {
-    static constexpr int kBlock = 128, kWaves = 1;
-};
-template <>
-struct TraverseShape<4> {
     static constexpr int kBlock = 64, kWaves = 8;
 };
 
