@@ -17,8 +17,9 @@ Also measured (keys of the same JSON line):
              tracer vs the guided one (online training included in its time);
   mlp      - the proposal network's MFMA kernels: inference and training
              throughput as a fraction of the bf16 dense peak;
-  batched  - 64 procedural scenes of uneven size in one wavefront render_batch
-             call with the autotuned launch shape;
+  batched  - 64 procedural scenes of uneven size in one batched call: the
+             wavefront pool with the autotuned launch shape and the
+             megakernel batch, the faster one reported on top;
   combined - terrain + spheres under the analytic sky, 16 views batched,
              plain vs guided time-to-relMSE per view and samples/s.
 
@@ -210,9 +211,11 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
 
 
 def bench_batched(no_cpu, log=log, builder="SAH"):
-    """Config 2: 64 procedural scenes (seeds 1-64) through one wavefront pool
-    in one render_batch call; launch shape chosen by the autotuner on a 2-spp
-    probe of the same batch. The oracle renders scene 1 at 1 spp on host cores."""
+    """Config 2: 64 procedural scenes (seeds 1-64) in one batched call, twice:
+    through one wavefront pool (launch shape chosen by the autotuner on a 2-spp
+    probe of the same batch) and through the megakernel batch
+    (illume_render_batch); the faster is the config's figure. The oracle
+    renders scene 1 at 1 spp on host cores."""
     import torch
     from rs_illume_amd import autotune
     from rs_illume_amd import scenes as S
@@ -242,16 +245,35 @@ def bench_batched(no_cpu, log=log, builder="SAH"):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     rays = counts.cpu().numpy().astype(np.float64)
-    out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
-                       "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one render_batch call",
-           "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths, "bvh_builder": builder,
-           "scene_setup_s": round(setup_s, 2),
-           "seconds": round(dt, 4), "samples_per_s": round(n_paths / dt, 1),
-           "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1), "iterations": wf.last_iterations(),
-           "autotune": {"best": {k: best[k] for k in ("max_paths", "block", "regen_frac", "graph_iters")},
-                        "candidates": len(table), "seconds": round(tune_s, 1)},
-           "film_finite": bool(torch.isfinite(film).all())}
+    wave = {"seconds": round(dt, 4), "samples_per_s": round(n_paths / dt, 1),
+            "mrays_per_s": round(float(rays.sum()) / dt / 1e6, 1), "iterations": wf.last_iterations(),
+            "autotune": {"best": {k: best[k] for k in ("max_paths", "block", "regen_frac", "graph_iters")},
+                         "candidates": len(table), "seconds": round(tune_s, 1)},
+            "film_finite": bool(torch.isfinite(film).all())}
     wf.close()
+    del film
+    # the same jobs on the megakernel in one launch (illume_render_batch)
+    from rs_illume_amd.render import render_batch
+    render_batch(jobs[:1])
+    mk_counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    film, _ = render_batch(jobs, ray_counts=mk_counts)
+    torch.cuda.synchronize()
+    dt_mk = time.perf_counter() - t0
+    mk_rays = mk_counts.cpu().numpy().astype(np.float64)
+    mega = {"seconds": round(dt_mk, 4), "samples_per_s": round(n_paths / dt_mk, 1),
+            "mrays_per_s": round(float(mk_rays.sum()) / dt_mk / 1e6, 1),
+            "film_finite": bool(torch.isfinite(film).all())}
+    del film
+    fastest = "megakernel_batch" if dt_mk < dt else "wavefront"
+    top = mega if dt_mk < dt else wave
+    out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
+                       "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one batched render call",
+           "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths, "bvh_builder": builder,
+           "scene_setup_s": round(setup_s, 2), "renderer": fastest,
+           "seconds": top["seconds"], "samples_per_s": top["samples_per_s"], "mrays_per_s": top["mrays_per_s"],
+           "wavefront": wave, "megakernel_batch": mega}
     if not no_cpu:
         pyoracle = _oracle()
         d, cam, _ = descs[0]
@@ -273,15 +295,16 @@ def bench_batched(no_cpu, log=log, builder="SAH"):
 def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
                    builder="SAH"):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
-    sky, 16 views of uneven size in one render_batch per pass, max depth 10.
-    Plain and guided (one proposal network for the scene, trained online;
+    sky, 16 views of uneven size in one batched call per pass, max depth 10.
+    Plain (megakernel batch) and guided (wavefront pool; one proposal network
+    for the scene, trained online;
     the radiance field it learns is view-independent) run progressive passes
     of all views; a view's time-to-relMSE is the batch time at which its image
     first reaches the target against a ref_spp plain reference."""
     import torch
     from rs_illume_amd import scenes as S
     from rs_illume_amd.guide import Guide
-    from rs_illume_amd.render import GuideCfg, Scene, Wavefront
+    from rs_illume_amd.render import GuideCfg, Scene, Wavefront, render_batch
     t0 = time.time()
     desc = S.terrain_scene(0)
     t1 = time.time()
@@ -300,7 +323,7 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
     t0 = time.perf_counter()
     ref = torch.zeros(3 * n_px, dtype=torch.float32, device="cuda")
     for k in range(ref_spp // 32):
-        wf.render_batch(jobs(32, 32 * k, 5000), film=ref)
+        render_batch(jobs(32, 32 * k, 5000), film=ref)
     torch.cuda.synchronize()
     ref_s = time.perf_counter() - t0
     ref /= ref_spp
@@ -314,13 +337,20 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
         warm = guide is not None and warmup_steps > 0
         reached = [None] * len(cams)
         elapsed, spp = 0.0, 0
-        wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99) if guide is not None else None, gcfg=gc)
+        # plain passes on the megakernel batch (the faster plain renderer), guided ones on the wavefront pool
+        if guide is None:
+            render_batch(jobs(1, 1 << 23, 9000))
+        else:
+            wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99), gcfg=gc)
         while spp < max_spp and any(r is None for r in reached):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if warm:
                 gc.guide_frac = 0.0
-            wf.render_batch(jobs(spp_pass, spp, 100 if guide is None else 200), film=film, guide=guide, gcfg=gc)
+            if guide is None:
+                render_batch(jobs(spp_pass, spp, 100), film=film)
+            else:
+                wf.render_batch(jobs(spp_pass, spp, 200), film=film, guide=guide, gcfg=gc)
             if warm and guide.stats()[0] >= warmup_steps:
                 warm = False
             if guide is not None and not warm:
@@ -350,6 +380,7 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
                         "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
                         % (n_px / 1e6, spp_pass),
             "bvh_builder": builder, "scene_setup_s": round(setup_s, 2), "reference_spp": ref_spp,
+            "plain_renderer": "megakernel_batch", "guided_renderer": "wavefront",
             "reference_s": round(ref_s, 2),
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
             "plain": plain, "guided": guided,

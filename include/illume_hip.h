@@ -344,6 +344,13 @@ typedef struct {
     uint64_t film_offset;
 } illume_job;
 
+/* The jobs of a batch on the megakernel of illume_render, in one launch over
+ * all their paths (each thread finds its job): every job's image equals its
+ * own illume_render call (cfg.flags selects its sampler; no proposal). For
+ * many plain renders of uneven size; the wavefront pool below adds the
+ * learned proposal. Integrator::render (core/integrator.rs:17-29) per job. */
+int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream);
+
 int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavefront **out);
 int illume_wavefront_destroy(illume_wavefront *wf);
 /* Launch shape: threads per block of the extend/shade stages (64/128/256),

@@ -21,6 +21,22 @@ hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, cons
                                   uint32_t max_depth, uint64_t seed, uint32_t sample_offset, float *rgbw,
                                   unsigned long long *counters, const SamplerLaunch &hl, hipStream_t stream);
 
+// One image of a batched megakernel render (illume_render_batch).
+struct RenderJob {
+    const DevScene *scene;  // device copy of the scene record
+    DevCamera cam;
+    unsigned long long path_begin;   // first path number of the job in the batch
+    unsigned long long film_offset;  // float offset of its 3-float pixels
+    uint64_t seed;
+    uint32_t sample_offset, max_depth;
+    SamplerLaunch sl;
+};
+// Renders every job's width * height * spp paths (jobs ordered by path_begin,
+// total paths in all): one thread per path, each finding its job by binary
+// search.
+hipError_t launch_render_batch(const RenderJob *jobs, uint32_t n_jobs, unsigned long long total, float *film,
+                               unsigned long long *counters, hipStream_t stream);
+
 // The sampler parameters of kind kSamplerHalton or kSamplerSobol for the
 // sample bounds [x0, x0 + w) x [y0, y0 + h), over this device's table
 // (uploaded on first use); false if the table could not be made.

@@ -88,6 +88,62 @@ __global__ __launch_bounds__(BLOCK, 3) void render_kernel(DevScene sc, DevCamera
     }
 }
 
+// Batched megakernel: one thread per path of the whole batch; lanes find
+// their job (scene, camera, film offset, sampler) by binary search over the
+// jobs' first path numbers, so one launch renders images of any sizes.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 3) void render_batch_kernel(const RenderJob *__restrict__ jobs, uint32_t n_jobs,
+                                                                 unsigned long long total, float *__restrict__ film,
+                                                                 unsigned long long *__restrict__ counters) {
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    __shared__ uint32_t blk_counts[2];
+    if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned long long g = (unsigned long long)blockIdx.x * BLOCK + threadIdx.x;
+    PathCounters pc{0, 0};
+    if (g < total) {
+        uint32_t lo = 0, hi = n_jobs;  // last job with path_begin <= g
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].path_begin <= g) lo = mid;
+            else hi = mid;
+        }
+        const RenderJob &job = jobs[lo];
+        const DevScene &sc = *job.scene;
+        const uint64_t n_pix = (uint64_t)job.cam.width * job.cam.height;
+        const unsigned long long local = g - job.path_begin;
+        const uint64_t pix = local % n_pix;
+        int spill[kSpill];
+        TraceStack<BLOCK> st;
+        st.reset(lds_stack + threadIdx.x, spill);
+        PathState ps;
+        float fx, fy;
+        path_start_at(job.cam, job.seed, pix, (int)(pix % job.cam.width), (int)(pix / job.cam.width),
+                      job.sample_offset + (uint32_t)(local / n_pix), ps, fx, fy, job.sl);
+        GuideCfg gc{};
+        uint32_t n_recs = 0;
+        while (ps.active) {
+            TraceHit h;
+            const bool found = scene_closest<BLOCK>(sc, ps.o, ps.d, ps.t_max, h, st, pc);
+            path_shade<BLOCK>(sc, job.max_depth, ps, found, h, nullptr, gc, nullptr, n_recs, st, pc);
+        }
+        const f3 L = guard_radiance(ps.L);
+        float *px = film + job.film_offset + 3 * pix;
+        atomicAdd(px, L.x);
+        atomicAdd(px + 1, L.y);
+        atomicAdd(px + 2, L.z);
+    }
+    if (counters) {
+        atomicAdd(&blk_counts[0], pc.closest);
+        atomicAdd(&blk_counts[1], pc.any);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(counters, (unsigned long long)blk_counts[0]);
+            atomicAdd(counters + 1, (unsigned long long)blk_counts[1]);
+        }
+    }
+}
+
 // The filtered film of a chunk of spp samples per sample pixel, by gathering:
 // one lane per image pixel visits, in a fixed order (sample, then sample-pixel
 // rows and columns), every sample whose sample pixel lies within the filter
@@ -192,6 +248,16 @@ hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, cons
     }
     const hipError_t ef = hipFreeAsync(buf, stream);
     return e != hipSuccess ? e : ef;
+}
+
+hipError_t launch_render_batch(const RenderJob *jobs, uint32_t n_jobs, unsigned long long total, float *film,
+                               unsigned long long *counters, hipStream_t stream) {
+    if (total == 0 || n_jobs == 0) return hipSuccess;
+    const unsigned long long grid = (total + kRenderBlock - 1) / kRenderBlock;
+    if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
+    render_batch_kernel<kRenderBlock><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(jobs, n_jobs, total, film,
+                                                                                   counters);
+    return hipGetLastError();
 }
 
 hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb, hipStream_t stream) {

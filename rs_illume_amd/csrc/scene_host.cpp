@@ -427,6 +427,50 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
     return ILLUME_OK;
 }
 
+int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream) {
+    if (!film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    std::vector<RenderJob> hj;
+    unsigned long long total = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const illume_job &in = jobs[j];
+        if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
+        const uint32_t kind = sampler_kind(in.cfg.flags);
+        if (kind == ~0u) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
+        if ((uint64_t)in.cfg.sample_offset + in.cfg.spp > (1u << 24))
+            return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+        RenderJob job;
+        std::memset(&job, 0, sizeof(job));
+        if (!make_camera(in.camera, &job.cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
+        const unsigned long long n_pix = (unsigned long long)in.camera.width * in.camera.height;
+        if (n_pix >= (1ull << 40)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+        const unsigned long long n = in.scene->n_prims ? n_pix * in.cfg.spp : 0ull;
+        if (n == 0) continue;  // no samples, or nothing to hit and no lights: black
+        if (kind && !sampler_launch(kind, 0, 0, (int)in.camera.width, (int)in.camera.height, &job.sl))
+            return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
+        job.scene = in.scene->d_dev;
+        job.path_begin = total;
+        job.film_offset = in.film_offset;
+        job.seed = in.cfg.seed;
+        job.sample_offset = in.cfg.sample_offset;
+        job.max_depth = in.cfg.max_depth;
+        hj.push_back(job);
+        total += n;
+    }
+    if (total == 0) return ILLUME_OK;
+    hipStream_t st = (hipStream_t)stream;
+    RenderJob *d_jobs = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_jobs, hj.size() * sizeof(RenderJob), st);
+    if (e != hipSuccess) return abi_hip_fail(e, "job table allocation");
+    // from pageable memory: returns once the table has been staged, so hj may go
+    e = hipMemcpyAsync(d_jobs, hj.data(), hj.size() * sizeof(RenderJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_render_batch(d_jobs, (uint32_t)hj.size(), total, film, (unsigned long long *)ray_counts, st);
+    const hipError_t ef = hipFreeAsync(d_jobs, st);
+    if (e != hipSuccess) return abi_hip_fail(e, "batched render launch");
+    if (ef != hipSuccess) return abi_hip_fail(ef, "job table release");
+    return ILLUME_OK;
+}
+
 int illume_render_filtered(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                            const illume_filter *filter, float *rgbw_sum, uint64_t *ray_counts, void *stream) {
     if (!scene || !camera || !cfg || !filter || !rgbw_sum) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");

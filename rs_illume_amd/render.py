@@ -184,6 +184,43 @@ def _camera_c(cam):
     return c
 
 
+def _job_array(jobs, per_px):
+    """illume_job array of (Scene, camera, spp, max_depth, seed[, sample_offset[,
+    sampler]]) tuples, images laid out one after another (per_px floats per
+    pixel): (array, offsets, total floats)."""
+    arr = (JobC * len(jobs))()
+    offsets, off = [], 0
+    for i, jb in enumerate(jobs):
+        sc, cam, spp, depth, seed = jb[:5]
+        so = jb[5] if len(jb) > 5 else 0
+        smp = jb[6] if len(jb) > 6 else "random"
+        if smp not in SAMPLERS:
+            raise ValueError("sampler must be one of %s" % (tuple(SAMPLERS),))
+        c = _camera_c(cam)
+        arr[i].scene = sc.handle
+        arr[i].camera = c
+        arr[i].cfg = RenderCfg(int(spp), int(depth), int(so), SAMPLERS[smp], int(seed))
+        arr[i].film_offset = off
+        offsets.append(off)
+        off += c.width * c.height * per_px
+    return arr, offsets, off
+
+
+def render_batch(jobs, film=None, ray_counts=None, stream=None):
+    """Plain renders of many scenes and cameras on the megakernel in one
+    launch (illume_render_batch). jobs as for :meth:`Wavefront.render_batch`;
+    returns (film, offsets) with every job's radiance sums (H*W*3 floats) at
+    its offset."""
+    L = _declare_wf()
+    arr, offsets, off = _job_array(jobs, 3)
+    if film is None:
+        film = torch.zeros(off, dtype=torch.float32, device="cuda")
+    rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+    _lib.check(L.illume_render_batch(arr, len(jobs), ctypes.c_void_p(film.data_ptr()), rc, _stream_handle(stream)),
+               "illume_render_batch")
+    return film, offsets
+
+
 def _declare_wf():
     L = _declare()
     if getattr(L, "_wf_declared", False):
@@ -205,6 +242,8 @@ def _declare_wf():
     L.illume_wavefront_render_batch_filtered.restype = i32
     L.illume_wavefront_render_batch_filtered.argtypes = [vp, ctypes.POINTER(JobC), ctypes.POINTER(vp), u32, vp,
                                                          ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L.illume_render_batch.restype = i32
+    L.illume_render_batch.argtypes = [ctypes.POINTER(JobC), u32, vp, vp, vp]
     L.illume_wavefront_render_filtered.restype = i32
     L.illume_wavefront_render_filtered.argtypes = [vp, vp, vp, ctypes.POINTER(RenderCfg), vp, vp,
                                                    ctypes.POINTER(GuideCfg), vp, vp, vp]
@@ -261,22 +300,7 @@ class Wavefront:
         weight; :func:`rs_illume_amd.film.resolve` gives the pixels)."""
         if filters is not None and len(filters) != len(jobs):
             raise ValueError("one filter per job")
-        per_px = 3 if filters is None else 4
-        arr = (JobC * len(jobs))()
-        offsets, off = [], 0
-        for i, jb in enumerate(jobs):
-            sc, cam, spp, depth, seed = jb[:5]
-            so = jb[5] if len(jb) > 5 else 0
-            smp = jb[6] if len(jb) > 6 else "random"
-            if smp not in SAMPLERS:
-                raise ValueError("sampler must be one of %s" % (tuple(SAMPLERS),))
-            c = _camera_c(cam)
-            arr[i].scene = sc.handle
-            arr[i].camera = c
-            arr[i].cfg = RenderCfg(int(spp), int(depth), int(so), SAMPLERS[smp], int(seed))
-            arr[i].film_offset = off
-            offsets.append(off)
-            off += c.width * c.height * per_px
+        arr, offsets, off = _job_array(jobs, 3 if filters is None else 4)
         if film is None:
             film = torch.zeros(off, dtype=torch.float32, device="cuda")
         g = guide.handle if guide is not None else None

@@ -142,3 +142,50 @@ def test_invalid_scene_rejected(gpu):
     a["prim_emission"][-1] = (1, 1, 1)   # an emissive sphere
     with pytest.raises(IllumeError):
         Scene(a)
+
+
+def test_megakernel_batch_equals_individual_renders(gpu):
+    """illume_render_batch: one launch over every job's paths (lanes find their
+    job by binary search) gives each job the image and ray counts of its own
+    illume_render call; a job without samples adds nothing; flags that name no
+    sampler are refused."""
+    import ctypes
+    import torch
+    from rs_illume_amd import _lib
+    from rs_illume_amd._lib import IllumeError
+    from rs_illume_amd.render import JobC, PathIntegrator, RenderCfg, Scene, _camera_c, _declare_wf, render_batch
+    _say("case megakernel batch")
+    descs = [S.cornell_box(0), S.cornell_box(1, glossy=True, light_size=0.2, light_L=26.5625),
+             S.terrain_scene(0, n=24, n_spheres=40)]
+    scenes = [Scene(d) for d in descs]
+    cams = [S.cornell_camera(40, 40), S.cornell_camera(24, 56), S.cornell_camera(64, 32)]
+    jobs = [(scenes[0], cams[0], 6, 5, 1, 0, "random"), (scenes[1], cams[1], 3, 8, 2, 5, "halton"),
+            (scenes[2], cams[2], 9, 6, 3, 2, "sobol"), (scenes[0], cams[1], 0, 5, 4, 0, "random")]
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    film, offs = render_batch(jobs, ray_counts=counts)
+    torch.cuda.synchronize()
+    film = film.cpu().numpy()
+    want = np.zeros(2, np.int64)
+    for (sc, cam, spp, depth, seed, so, smp), off in zip(jobs, offs):
+        w, h = int(cam["width"][0]), int(cam["height"][0])
+        got = film[off:off + w * h * 3].reshape(h, w, 3)
+        if spp == 0:
+            assert not got.any()
+            continue
+        c = torch.zeros(2, dtype=torch.int64, device="cuda")
+        ref = PathIntegrator(cam, spp=spp, max_depth=depth, seed=seed, sampler=smp).render_sum(
+            sc, sample_offset=so, ray_counts=c)
+        torch.cuda.synchronize()
+        want += c.cpu().numpy()
+        ref = ref.cpu().numpy()
+        assert np.allclose(ref, got, rtol=1e-5, atol=1e-5), (smp, np.abs(ref - got).max())
+    assert np.array_equal(counts.cpu().numpy(), want), (counts, want)
+    bad = (JobC * 1)()
+    bad[0].scene = scenes[0].handle
+    bad[0].camera = _camera_c(cams[0])
+    bad[0].cfg = RenderCfg(1, 5, 0, 3, 0)  # ILLUME_RENDER_HALTON | ILLUME_RENDER_SOBOL: not a sampler
+    bad[0].film_offset = 0
+    buf = torch.zeros(40 * 40 * 3, dtype=torch.float32, device="cuda")
+    with pytest.raises(IllumeError):
+        _lib.check(_declare_wf().illume_render_batch(bad, 1, ctypes.c_void_p(buf.data_ptr()), None, None),
+                   "illume_render_batch")
