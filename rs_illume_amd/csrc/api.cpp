@@ -153,11 +153,6 @@ int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *
     hipError_t e = illume::lbvh_build(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node,
                                       (hipStream_t)stream, &r);
     if (e != hipSuccess) return hip_fail(e, "device BVH build");
-    if ((int)r.max_depth * (width - 1) >= illume::max_supported_depth()) {
-        hipFree(r.nodes);
-        hipFree(r.prims);
-        return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    }
     illume_bvh *h = new (std::nothrow) illume_bvh();
     if (!h) {
         hipFree(r.nodes);
@@ -169,19 +164,36 @@ int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *
     hipGetDevice(&h->device);
     h->d_prims = r.prims;
     size_t nb = (size_t)r.n_nodes * sizeof(illume::GpuNode);
-    uint32_t depth = r.max_depth;
+    // the levels above the Morton treelets rebuilt by SAH on the host
+    std::vector<illume::GpuNode> n2(r.n_nodes);
+    e = hipMemcpy(n2.data(), r.nodes, nb, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        hipFree(r.nodes);
+        illume_bvh_destroy(h);
+        return hip_fail(e, "hipMemcpy(binary nodes)");
+    }
+    uint32_t depth = illume::sah_top(n2, r.upper);
+    // stack entries: up to width-1 pending siblings per level of the traced tree
+    if (width == 2 && (int)depth >= illume::max_supported_depth()) {
+        hipFree(r.nodes);
+        illume_bvh_destroy(h);
+        return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
     if (width == 2) {
         h->d_nodes = r.nodes;
-    } else {
-        std::vector<illume::GpuNode> n2(r.n_nodes);
-        std::vector<illume::GpuNode4> n4;
-        e = hipMemcpy(n2.data(), r.nodes, nb, hipMemcpyDeviceToHost);
-        hipFree(r.nodes);
+        e = hipMemcpy(h->d_nodes, n2.data(), nb, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             illume_bvh_destroy(h);
             return hip_fail(e, "hipMemcpy(binary nodes)");
         }
+    } else {
+        std::vector<illume::GpuNode4> n4;
+        hipFree(r.nodes);
         depth = illume::collapse_to_width4(n2, &n4);
+        if ((int)depth * 3 >= illume::max_supported_depth()) {
+            illume_bvh_destroy(h);
+            return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+        }
         std::vector<illume::GpuNode4Q> nq;
         if (!illume::quantize_width4(n4, &nq)) {
             illume_bvh_destroy(h);

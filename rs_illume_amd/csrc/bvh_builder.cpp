@@ -414,6 +414,169 @@ std::string validate_prims(const float *verts, uint32_t n_verts, const uint32_t 
     return "";
 }
 
+namespace {
+struct TopItem {
+    int32_t ref;  // treelet root: node index or leaf reference
+    float b[6];   // its box (lo xyz, hi xyz)
+    float c[3];   // box centre
+};
+void box_union(float *acc, const float *b) {
+    for (int a = 0; a < 3; ++a) {
+        acc[a] = std::min(acc[a], b[a]);
+        acc[3 + a] = std::max(acc[3 + a], b[3 + a]);
+    }
+}
+float box_area(const float *b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+void child_of(const GpuNode &g, int c, int32_t *ref, float *b) {
+    if (c == 0) {
+        *ref = g.child0;
+        b[0] = g.c0_lo_x; b[1] = g.c0_lo_y; b[2] = g.c0_lo_z; b[3] = g.c0_hi_x; b[4] = g.c0_hi_y; b[5] = g.c0_hi_z;
+    } else {
+        *ref = g.child1;
+        b[0] = g.c1_lo_x; b[1] = g.c1_lo_y; b[2] = g.c1_lo_z; b[3] = g.c1_hi_x; b[4] = g.c1_hi_y; b[5] = g.c1_hi_z;
+    }
+}
+void set_child(GpuNode &g, int c, int32_t ref, const float *b) {
+    if (c == 0) {
+        g.child0 = ref;
+        g.c0_lo_x = b[0]; g.c0_lo_y = b[1]; g.c0_lo_z = b[2]; g.c0_hi_x = b[3]; g.c0_hi_y = b[4]; g.c0_hi_z = b[5];
+    } else {
+        g.child1 = ref;
+        g.c1_lo_x = b[0]; g.c1_lo_y = b[1]; g.c1_lo_z = b[2]; g.c1_hi_x = b[3]; g.c1_hi_y = b[4]; g.c1_hi_z = b[5];
+    }
+}
+
+// Binary SAH tree over items[begin, end) written into the next free slots;
+// returns the subtree's reference and box. One item is the item itself.
+struct TopBuilder {
+    std::vector<TopItem> &items;
+    std::vector<GpuNode> &nodes;
+    const std::vector<int32_t> &slots;
+    size_t next = 0;
+    int32_t build(size_t begin, size_t end, float *box) {
+        if (end - begin == 1) {
+            std::memcpy(box, items[begin].b, sizeof(items[begin].b));
+            return items[begin].ref;
+        }
+        const int32_t slot = slots[next++];
+        float cb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = begin; i < end; ++i) {
+            const float c6[6] = {items[i].c[0], items[i].c[1], items[i].c[2], items[i].c[0], items[i].c[1],
+                                 items[i].c[2]};
+            box_union(cb, c6);
+        }
+        int dim = 0;
+        for (int a = 1; a < 3; ++a)
+            if (cb[3 + a] - cb[a] > cb[3 + dim] - cb[dim]) dim = a;
+        size_t mid = begin + (end - begin) / 2;
+        const float ext = cb[3 + dim] - cb[dim];
+        if (ext > 0.f) {
+            // 12 buckets along dim; the split with the least count x area on both sides
+            constexpr int kB = 12;
+            int cnt[kB] = {0};
+            float bb[kB][6];
+            for (int k = 0; k < kB; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    bb[k][a] = INFINITY;
+                    bb[k][3 + a] = -INFINITY;
+                }
+            auto bucket = [&](const TopItem &it) {
+                return std::min(kB - 1, (int)((float)kB * ((it.c[dim] - cb[dim]) / ext)));
+            };
+            for (size_t i = begin; i < end; ++i) {
+                const int k = bucket(items[i]);
+                ++cnt[k];
+                box_union(bb[k], items[i].b);
+            }
+            float best = INFINITY;
+            int best_k = -1;
+            for (int k = 0; k < kB - 1; ++k) {
+                float l[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}, r[6];
+                std::memcpy(r, l, sizeof(l));
+                int nl = 0, nr = 0;
+                for (int j = 0; j <= k; ++j) {
+                    nl += cnt[j];
+                    if (cnt[j]) box_union(l, bb[j]);
+                }
+                for (int j = k + 1; j < kB; ++j) {
+                    nr += cnt[j];
+                    if (cnt[j]) box_union(r, bb[j]);
+                }
+                if (nl == 0 || nr == 0) continue;
+                const float cost = (float)nl * box_area(l) + (float)nr * box_area(r);
+                if (cost < best) {
+                    best = cost;
+                    best_k = k;
+                }
+            }
+            if (best_k >= 0) {
+                auto it = std::partition(items.begin() + begin, items.begin() + end,
+                                         [&](const TopItem &x) { return bucket(x) <= best_k; });
+                mid = (size_t)(it - items.begin());
+            }
+        }
+        if (mid == begin || mid == end) mid = begin + (end - begin) / 2;
+        float bl[6], br[6];
+        const int32_t rl = build(begin, mid, bl);
+        const int32_t rr = build(mid, end, br);
+        GpuNode g;
+        std::memset(&g, 0, sizeof(g));
+        set_child(g, 0, rl, bl);
+        set_child(g, 1, rr, br);
+        nodes[slot] = g;
+        std::memcpy(box, bl, sizeof(bl));
+        box_union(box, br);
+        return slot;
+    }
+};
+}  // namespace
+
+uint32_t binary_depth(const std::vector<GpuNode> &n2) {
+    if (n2.empty()) return 0;
+    uint32_t best = 0;
+    std::vector<std::pair<int32_t, uint32_t>> st{{0, 1}};
+    while (!st.empty()) {
+        auto [i, d] = st.back();
+        st.pop_back();
+        best = std::max(best, d);
+        if (n2[i].child0 >= 0) st.push_back({n2[i].child0, d + 1});
+        if (n2[i].child1 >= 0) st.push_back({n2[i].child1, d + 1});
+    }
+    return best;
+}
+
+uint32_t sah_top(std::vector<GpuNode> &n2, const std::vector<uint8_t> &upper) {
+    if (n2.empty() || upper.size() < n2.size() || !upper[0]) return binary_depth(n2);
+    // the upper nodes (their slots, root first) and the treelet roots below them
+    std::vector<int32_t> slots;
+    std::vector<TopItem> items;
+    std::vector<int32_t> st{0};
+    while (!st.empty()) {
+        const int32_t i = st.back();
+        st.pop_back();
+        slots.push_back(i);
+        for (int c = 0; c < 2; ++c) {
+            TopItem it;
+            child_of(n2[i], c, &it.ref, it.b);
+            if (it.ref >= 0 && upper[it.ref]) {
+                st.push_back(it.ref);
+            } else {
+                for (int a = 0; a < 3; ++a) it.c[a] = 0.5f * (it.b[a] + it.b[3 + a]);
+                items.push_back(it);
+            }
+        }
+    }
+    // a binary tree over T treelet roots has T - 1 inner nodes: the same count
+    if (items.size() != slots.size() + 1) return binary_depth(n2);
+    TopBuilder tb{items, n2, slots};
+    float box[6];
+    tb.build(0, items.size(), box);
+    return binary_depth(n2);
+}
+
 uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4> *n4) {
     struct Kid {
         float b[6];

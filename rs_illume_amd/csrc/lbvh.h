@@ -13,7 +13,23 @@ struct LbvhResult {
     PrimRecord *prims = nullptr;   // device, leaf (Morton) order
     uint32_t n_nodes = 0, n_leaves = 0, max_depth = 0;
     float bounds[6] = {0, 0, 0, 0, 0, 0};
+    // host: per node, 1 if its primitives span more than one treelet (their
+    // Morton codes differ within the top kTreeletBits bits)
+    std::vector<uint8_t> upper;
 };
+
+// Treelets of the SAH top: primitives whose Morton codes share the top 12 bits
+// (4 per axis), as the reference's HLBVH clusters them (bvh.rs:276-300).
+constexpr int kTreeletBits = 12;
+
+// The SAH top (the reference's build_upper_sah, bvh.rs:406-502, over pbrt's
+// 12 buckets): the nodes above the treelets (upper[i] != 0, reachable from the
+// root through upper nodes only) are rebuilt by a binned SAH over the treelet
+// roots' boxes, in the same node slots (root at 0; T treelets need T - 1
+// nodes either way). Returns the binary tree's depth after the rebuild.
+uint32_t sah_top(std::vector<GpuNode> &n2, const std::vector<uint8_t> &upper);
+// Inner-node depth of a two-child BVH (root at 0).
+uint32_t binary_depth(const std::vector<GpuNode> &n2);
 
 // Host input arrays (as for build_bvh), already validated; n_tris + n_spheres
 // >= 2. The caller owns res->nodes and res->prims (hipFree).

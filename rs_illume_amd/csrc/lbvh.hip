@@ -244,6 +244,15 @@ __global__ void emit_kernel(int n, int max_prims, const uint32_t *__restrict__ l
     if (leaves && (i == 0 || last[i] - first[i] + 1 > (uint32_t)max_prims)) atomicAdd(n_leaves, leaves);
 }
 
+// Node i spans more than one treelet: the codes at both ends of its sorted
+// range differ within the top kTreeletBits of the 63-bit code.
+__global__ void upper_kernel(int n, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ first,
+                             const uint32_t *__restrict__ last, uint8_t *__restrict__ upper) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    upper[i] = ((keys[first[i]] ^ keys[last[i]]) >> (63 - kTreeletBits)) != 0 ? 1 : 0;
+}
+
 __global__ void records_kernel(const float *__restrict__ verts, const uint32_t *__restrict__ idx, uint32_t n_tris,
                                const float *__restrict__ spheres, uint32_t n, const uint32_t *__restrict__ ids,
                                PrimRecord *__restrict__ out) {
@@ -293,6 +302,7 @@ hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx,
     DevBuf tmp;
     float *d_verts = nullptr, *d_sph = nullptr, *pbox, *cen, *partial, *cb, *sb, *nbox;
     uint32_t *d_idx = nullptr, *ids_in, *ids, *left, *right, *first, *last, *visits, *counters;
+    uint8_t *upper;
     uint64_t *keys_in, *keys;
     int32_t *parent_inner, *parent_leaf;
     hipError_t e = hipSuccess;
@@ -316,7 +326,8 @@ hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx,
         (e = tmp.alloc(&right, n)) != hipSuccess || (e = tmp.alloc(&first, n)) != hipSuccess ||
         (e = tmp.alloc(&last, n)) != hipSuccess || (e = tmp.alloc(&parent_inner, n)) != hipSuccess ||
         (e = tmp.alloc(&parent_leaf, n)) != hipSuccess || (e = tmp.alloc(&nbox, 6 * (size_t)n)) != hipSuccess ||
-        (e = tmp.alloc(&visits, n)) != hipSuccess || (e = tmp.alloc(&counters, 2)) != hipSuccess)
+        (e = tmp.alloc(&visits, n)) != hipSuccess || (e = tmp.alloc(&counters, 2)) != hipSuccess ||
+        (e = tmp.alloc(&upper, n)) != hipSuccess)
         return e;
     prim_box_kernel<<<blocks, 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, pbox, cen);
     bounds_partial_kernel<<<rblocks, 256, 0, st>>>(cen, 3, n, partial);
@@ -348,8 +359,11 @@ hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx,
     emit_kernel<<<blocks, 256, 0, st>>>((int)n, max_prims, left, right, first, last, pbox, ids, nbox, nodes,
                                         counters + 1);
     records_kernel<<<blocks, 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, ids, prims);
+    upper_kernel<<<blocks, 256, 0, st>>>((int)n, keys, first, last, upper);
     uint32_t hc[2];
+    res->upper.assign(n - 1, 0);
     if ((e = hipGetLastError()) == hipSuccess) e = hipMemcpyAsync(hc, counters, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res->upper.data(), upper, n - 1, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(res->bounds, sb, 24, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {

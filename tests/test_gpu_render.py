@@ -189,3 +189,25 @@ def test_megakernel_batch_equals_individual_renders(gpu):
     with pytest.raises(IllumeError):
         _lib.check(_declare_wf().illume_render_batch(bad, 1, ctypes.c_void_p(buf.data_ptr()), None, None),
                    "illume_render_batch")
+
+
+def test_device_built_terrain_four_child(gpu):
+    """The 2M-triangle terrain built on the device (LBVH with the SAH top) as a
+    four-child tree: its depth is checked against the traversal stack per
+    four-child level (3 pending entries each), and it renders the image the
+    host SAH tree renders (hits do not depend on the tree)."""
+    import torch
+    from rs_illume_amd.render import PathIntegrator, Scene
+    _say("case terrain, device builder")
+    desc = S.terrain_scene(0)
+    cam = S.terrain_views(0)[0]
+    cam = cam.copy()
+    cam["width"], cam["height"] = 64, 48
+    imgs = {}
+    for builder in ("LBVH", "SAH"):
+        sc = Scene(desc, builder=builder)
+        _say(builder, sc.bvh_info() if hasattr(sc, "bvh_info") else "")
+        imgs[builder] = PathIntegrator(cam, spp=2, max_depth=6, seed=3).render_sum(sc).cpu().numpy()
+        torch.cuda.synchronize()
+        sc.close()
+    assert np.allclose(imgs["LBVH"], imgs["SAH"], rtol=1e-5, atol=1e-5), np.abs(imgs["LBVH"] - imgs["SAH"]).max()
