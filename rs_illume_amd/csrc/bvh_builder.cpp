@@ -3,8 +3,13 @@
 //
 // Differences from the reference's build that do not change hit results:
 //  * 16 bins on each of the 3 axes (the reference: 12 buckets on the longest
-//    centroid axis); the cost model is the reference's: split cost
-//    1 + (n0*SA0 + n1*SA1)/SA against leaf cost n (bvh.rs:203-226).
+//    centroid axis); split cost kTraversalCost + (n0*SA0 + n1*SA1)/SA against
+//    leaf cost n, where the reference prices the traversal step at 1
+//    (bvh.rs:199-200). On the GPU traversal a node step costs about as much as
+//    a primitive test and steps of the two kinds run side by side, so smaller
+//    leaves pay: kTraversalCost 0.25 gives closest hit +3-4% on the
+//    2^20-triangle soup and +4% on a terrain view against 1 (0 to 0.5 within
+//    2%; 1.5 and 2 lose 11% and 21%).
 //  * A node whose primitives all share one centroid is split by count rather
 //    than made into an oversized leaf (bvh.rs:134-142), so every leaf holds at
 //    most max_prims_in_node primitives.
@@ -12,8 +17,10 @@
 //    tree depth and so the traversal stack.
 //  * Sphere boxes are widened by one ulp outwards so box culling stays
 //    conservative for hits computed with the sphere's interval arithmetic.
-// Closest-hit ids do not depend on the tree shape (the closest hit is unique
-// up to exact ties), which is why the tree may differ from the reference's.
+// Closest-hit ids do not depend on the tree shape up to ties (two primitives
+// hit within an ulp or two: which one a traversal reports depends on the
+// order it reaches them), which is why the tree may differ from the
+// reference's.
 #include "bvh_builder.h"
 #include "lbvh.h"
 #include <algorithm>
@@ -27,6 +34,7 @@ namespace {
 
 constexpr int kBins = 16;
 constexpr int kMedianDepth = 40;
+constexpr float kTraversalCost = 0.25f;  // a node step against one primitive test (see the header)
 
 struct Box {
     float lo[3], hi[3];
@@ -146,7 +154,7 @@ struct Builder {
                     lacc.grow(bb[i]);
                     lcnt += bc[i];
                     if (lcnt == 0 || right_cnt[i + 1] == 0) continue;
-                    float cost = 1.f + ((float)lcnt * lacc.area() + (float)right_cnt[i + 1] * right_area[i + 1]) /
+                    float cost = kTraversalCost + ((float)lcnt * lacc.area() + (float)right_cnt[i + 1] * right_area[i + 1]) /
                                            std::max(parent_area, 1e-30f);
                     if (cost < best_cost) {
                         best_cost = cost;
