@@ -91,8 +91,11 @@ __global__ __launch_bounds__(BLOCK, 3) void render_kernel(DevScene sc, DevCamera
 // Batched megakernel: one thread per path of the whole batch; lanes find
 // their job (scene, camera, film offset, sampler) by binary search over the
 // jobs' first path numbers, so one launch renders images of any sizes.
+// Held to at least 5 waves per SIMD (96 VGPRs; the spills stay in cache):
+// against 3 waves, 32 procedural scenes render 18% and 16 terrain views 17%
+// faster (6 waves equal, 8 slower).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK, 3) void render_batch_kernel(const RenderJob *__restrict__ jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(BLOCK, 5) void render_batch_kernel(const RenderJob *__restrict__ jobs, uint32_t n_jobs,
                                                                  unsigned long long total, float *__restrict__ film,
                                                                  unsigned long long *__restrict__ counters) {
     __shared__ int lds_stack[kStackLds * BLOCK];
