@@ -51,31 +51,43 @@ __global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse
     }
 }
 
-// Persistent batch traversal: each wave (one 64-lane block) owns a contiguous
-// chunk of rays. A lane that finishes its ray keeps its result until at least
-// kRefillMin lanes of the wave are idle; then those lanes write their results
-// and take the chunk's next rays (in lane order), so a wave's trip count
-// follows its chunk's total work instead of each group of 64 rays' slowest
-// ray. The grid is one wave per wave slot of the device. Waiting for a
-// quarter of the wave to be idle amortises the refill (ray load, reciprocal
-// direction, shear, result write) over many lanes: refilling at 4 idle lanes
-// or at 32 was slower on the 2^20-triangle / 2^24-ray workload.
+// Persistent batch traversal: each wave (one 64-lane block) starts on a
+// contiguous run of rays. A lane that finishes its ray keeps its result until
+// at least kRefillMin lanes of the wave are idle; then those lanes write their
+// results and take the run's next rays (in lane order), so a wave's trip count
+// follows its rays' total work instead of each group of 64 rays' slowest ray.
+// The grid is one wave per wave slot of the device. Waiting for a quarter of
+// the wave to be idle amortises the refill (ray load, reciprocal direction,
+// shear, result write) over many lanes: refilling at 4 idle lanes or at 32 was
+// slower on the 2^20-triangle / 2^24-ray workload.
 constexpr int kRefillMin = 16;
+
+// Load balance: the static runs cover 1/kStaticDiv of the batch (an equal run
+// per wave); the rest is handed out kQueueRun rays at a time from a counter in
+// device memory, taken by a wave whose run cannot feed its idle lanes. Waves
+// end within one run of each other instead of with their static shares'
+// spread: on the 2^20-triangle / 2^24-ray workload closest hit 8.65 -> 7.39 ms
+// per launch and any hit 5.23 -> 4.56 ms (static fractions 0.4-0.55 within 1%
+// of each other, 0.7 at 7.98 ms, all-static at 8.65; runs of 128 or 256 rays
+// 1-2% slower than 64).
+constexpr uint32_t kStaticDiv = 2;
+constexpr uint32_t kQueueRun = 64;  // at least one wave, so one run feeds every idle lane
 
 constexpr int kRefillWaves = 8;  // waves per SIMD: at most 64 VGPRs, no spills
 
 template <bool ANY, int WIDTH>
 __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
     const void *__restrict__ nodes, const PrimRecord *__restrict__ prims, const illume_ray *__restrict__ rays,
-    uint32_t n, uint32_t per_wave, illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded) {
+    uint32_t n, uint32_t per_wave, uint32_t grab, unsigned long long *__restrict__ queue,
+    illume_hit *__restrict__ hits, uint8_t *__restrict__ occluded) {
     constexpr int kBlock = 64;
     constexpr uint32_t kNoRay = 0xFFFFFFFFu;
     __shared__ int lds_stack[kStackLds * kBlock];
     const uint32_t lane = threadIdx.x;
-    const uint32_t begin = blockIdx.x * per_wave;
-    if (begin >= n) return;
-    const uint32_t end = min(n, begin + per_wave);
-    uint32_t next = begin;  // the chunk's next ray (wave-uniform)
+    const uint32_t begin = min(n, blockIdx.x * per_wave);
+    uint32_t end = min(n, begin + per_wave);
+    uint32_t next = begin;  // the wave's next ray and the end of its run (wave-uniform)
+    bool more = true;       // the shared queue may still hold rays
     int spill[kSpill];
     TraceStack<kBlock> st;
     st.reset(lds_stack + lane, spill);
@@ -89,6 +101,17 @@ __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
         const uint64_t idle = __ballot(!busy);
         const int n_idle = __popcll(idle);
         if (n_idle >= kRefillMin || n_idle == kBlock) {
+            // The run cannot feed every idle lane: take the queue's next run.
+            const uint32_t left = end - next;
+            uint32_t run = kNoRay;
+            if (left < (uint32_t)n_idle && more) {
+                unsigned long long v = 0;
+                if (lane == 0) v = atomicAdd(queue, (unsigned long long)grab);
+                v = __shfl(v, 0);
+                if (v >= n) more = false;
+                else run = (uint32_t)v;
+            }
+            const uint32_t run_end = run == kNoRay ? 0u : min(n, run + grab);
             if (!busy) {
                 if (ray != kNoRay) {
                     if (ANY) {
@@ -102,8 +125,11 @@ __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
                     }
                     ray = kNoRay;
                 }
-                const uint32_t r = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (r < end) {
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                uint32_t r = kNoRay;
+                if (rank < left) r = next + rank;
+                else if (run != kNoRay && run + (rank - left) < run_end) r = run + (rank - left);
+                if (r != kNoRay) {
                     const float4 *rp = reinterpret_cast<const float4 *>(rays + r);
                     const float4 r0 = rp[0], r1 = rp[1];
                     o = mk(r0.x, r0.y, r0.z);
@@ -119,8 +145,13 @@ __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
                     busy = true;
                 }
             }
-            next = min(next + (uint32_t)n_idle, end);
-            if (__ballot(busy) == 0) break;  // chunk done, every result written
+            if (run != kNoRay) {
+                next = min(run + ((uint32_t)n_idle - left), run_end);
+                end = run_end;
+            } else {
+                next = min(next + (uint32_t)n_idle, end);
+            }
+            if (__ballot(busy) == 0) break;  // queue drained, every result written
         }
         if (busy) {
             if (pn != 0) {
@@ -146,6 +177,8 @@ __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
     }
 }
 
+__global__ void queue_start_kernel(unsigned long long *queue, unsigned long long start) { *queue = start; }
+
 __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -162,7 +195,11 @@ __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded
 int max_supported_depth() { return kMaxTraceDepth; }
 
 // Persistent launch on the four-child tree: one 64-lane wave per wave slot
-// (compute units x 4 SIMDs x waves per SIMD), each owning a chunk of the batch.
+// (compute units x 4 SIMDs x waves per SIMD), each starting on an equal static
+// run; the queue counter (stream-ordered allocation, set by a one-thread
+// kernel so that the launch sequence can be captured in a graph) starts past
+// the static runs.
+template <int W>
 hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
                          illume_hit *hits, uint8_t *occluded, bool any, hipStream_t stream) {
     int dev = 0, cus = 0;
@@ -171,17 +208,27 @@ hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illum
     if (e != hipSuccess) return e;
     const uint32_t slots = (uint32_t)std::max(cus, 1) * 4u * (uint32_t)kRefillWaves;
     const uint32_t waves = std::min((n + 63u) / 64u, slots);
-    const uint32_t per_wave = (n + waves - 1) / waves;
+    const uint32_t per_wave = (n + waves - 1) / waves / kStaticDiv;
+    const uint32_t grab = kQueueRun;
+    unsigned long long *queue = nullptr;
+    e = hipMallocAsync((void **)&queue, sizeof(unsigned long long), stream);
+    if (e != hipSuccess) return e;
+    queue_start_kernel<<<1, 1, 0, stream>>>(queue, (unsigned long long)waves * per_wave);
     if (any)
-        traverse_refill_kernel<true, 4><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, nullptr, occluded);
+        traverse_refill_kernel<true, W><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, grab, queue,
+                                                                  nullptr, occluded);
     else
-        traverse_refill_kernel<false, 4><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, hits, nullptr);
-    return hipGetLastError();
+        traverse_refill_kernel<false, W><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, grab, queue,
+                                                                   hits, nullptr);
+    e = hipGetLastError();
+    const hipError_t f = hipFreeAsync(queue, stream);
+    return e != hipSuccess ? e : f;
 }
 
 // One ray per lane: counting queries, and every query on the two-child tree
-// (whose cheaper steps do not repay the refill: 14.4 against 12.8 ms per
-// launch on the 2^20-triangle / 2^24-ray workload).
+// (whose cheaper steps do not repay the refill: closest hit 13.5 against
+// 12.6 ms per launch on the 2^20-triangle / 2^24-ray workload, with the
+// queue; any hit 9.08 against 9.28).
 template <int W>
 hipError_t launch_per_ray(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
                           illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {
@@ -209,7 +256,7 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
                                                               counts ? nullptr : (any ? occluded : nullptr), counts);
         return hipGetLastError();
     }
-    if (width == 4 && !counts) return launch_refill(nodes, prims, rays, n, hits, occluded, any, stream);
+    if (width == 4 && !counts) return launch_refill<4>(nodes, prims, rays, n, hits, occluded, any, stream);
     if (width == 4) return launch_per_ray<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
     return launch_per_ray<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
 }
