@@ -56,11 +56,12 @@ __global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse
 // at least kRefillMin lanes of the wave are idle; then those lanes write their
 // results and take the run's next rays (in lane order), so a wave's trip count
 // follows its rays' total work instead of each group of 64 rays' slowest ray.
-// The grid is one wave per wave slot of the device. Waiting for a quarter of
+// The grid is one wave per wave slot of the device. Waiting for a third of
 // the wave to be idle amortises the refill (ray load, reciprocal direction,
-// shear, result write) over many lanes: refilling at 4 idle lanes or at 32 was
-// slower on the 2^20-triangle / 2^24-ray workload.
-constexpr int kRefillMin = 16;
+// shear, result write) over many lanes: on the 2^20-triangle / 2^24-ray
+// workload 20 idle lanes gave 7.31 ms per launch, 16 7.35, 24 7.33, 12 7.45
+// and 8 7.61 (closest hit; any hit 4.50, 4.54, 4.49, 4.65, 4.78).
+constexpr int kRefillMin = 20;
 
 // Load balance: the static runs cover 1/kStaticDiv of the batch (an equal run
 // per wave); the rest is handed out kQueueRun rays at a time from a counter in
