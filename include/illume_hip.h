@@ -101,7 +101,9 @@ int illume_bvh_info(const illume_bvh *bvh, illume_bvh_info_t *out);
 /* Closest hit for n rays: batched Primitive::intersect on the aggregate
  * (core/primitive/mod.rs:11, accelerators/bvh.rs:563-612, reached through
  * Scene::intersect core/scene.rs:43-47). Each ray's t_max bounds the search.
- * rays, hits: device pointers. stream: hipStream_t (null = default stream). */
+ * rays, hits: device pointers. stream: hipStream_t (null = default stream).
+ * On a four-child tree the call (and illume_intersect_any) allocates 8 bytes
+ * stream-ordered on `stream` for its ray queue and frees them the same way. */
 int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
                              void *stream);
 

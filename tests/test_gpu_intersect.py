@@ -184,12 +184,12 @@ def test_full_size_1m_tris_16m_rays(gpu, oracle, width, builder):
 @pytest.mark.parametrize("width", [2, 4])
 def test_batch_sizes_across_wave_chunks(gpu, oracle, width):
     """Batch queries on the four-child tree run as persistent waves, each
-    owning a chunk of the batch and refilling its lanes from it; the chunk size
-    depends on the batch size and the device's wave slots (64 lanes x 4 SIMDs
-    x 8 waves per compute unit). Every batch size — below one wave, exact
-    multiples of 64, one ray either side of filling every wave slot — gives each
-    ray exactly the hit and occlusion it gets in one large batch, and the large
-    batch matches the oracle."""
+    starting on a static run of the batch and taking further 64-ray runs from a
+    device queue; the run sizes depend on the batch size and the device's wave
+    slots (64 lanes x 4 SIMDs x 8 waves per compute unit). Every batch size —
+    below one wave, exact multiples of 64, one ray either side of filling every
+    wave slot — gives each ray exactly the hit and occlusion it gets in one
+    large batch, and the large batch matches the oracle."""
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, hits_to_numpy, rays_to_device
@@ -216,3 +216,42 @@ def test_batch_sizes_across_wave_chunks(gpu, oracle, width):
             o = bvh.intersect_p(sub)
             assert torch.equal(h, h_all[off:off + n]), (n, off)
             assert torch.equal(o, o_all[off:off + n]), (n, off)
+
+
+def test_intersect_in_captured_graph(gpu):
+    """The batch calls (their stream-ordered queue allocation included) can be
+    captured in a graph: every replay, with eager work and synchronisations
+    between replays, gives the eager call's hits and occlusion."""
+    import torch
+    from rs_illume_amd import workloads as W
+    from rs_illume_amd.accel import BVHAccel, rays_to_device
+    v, idx, sph = small_scene(16384, 64, 121, median_sigma=0.02)
+    _say("case graph capture")
+    for width in (2, 4):
+        bvh = BVHAccel(v, idx, sph, 4, width=width)
+        rd = rays_to_device(W.sphere_to_cube_rays(300000, seed=122))
+        h_ref, o_ref = bvh.intersect(rd), bvh.intersect_p(rd)
+        hits = torch.empty_like(h_ref)
+        occ = torch.empty_like(o_ref)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            bvh.intersect(rd, hits)
+            bvh.intersect_p(rd, occ)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            bvh.intersect(rd, hits)
+            bvh.intersect_p(rd, occ)
+        for k in range(3):
+            hits.fill_(-1)
+            occ.fill_(7)
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(hits, h_ref), (width, k)
+            assert torch.equal(occ, o_ref), (width, k)
+            bvh.intersect(rd[:1000].contiguous())
+            torch.cuda.synchronize()
+        del g
+        bvh.close()
