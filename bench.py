@@ -490,6 +490,20 @@ def main():
     mrays = n / (ms_per_step * 1e-3) / 1e6
 
     hit_frac = float(((hits[:, 0].to(torch.int64) & 0xFFFFFFFF) != 0xFFFFFFFF).float().mean())
+
+    # any hit (intersect_p) on the same tree and rays, outside the headline's timed region
+    occluded = torch.empty(n, dtype=torch.uint8, device="cuda")
+    for _ in range(max(args.warmup, 1)):
+        bvh.intersect_p(rd, occluded)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        bvh.intersect_p(rd, occluded)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    any_ms = ev0.elapsed_time(ev1) / args.steps
+    any_hit = {"ms_per_launch": round(any_ms, 4), "mrays_per_s": round(n / (any_ms * 1e-3) / 1e6, 2),
+               "occluded_fraction": round(float(occluded.float().mean()), 4)}
     cpu, oracle_counts = None, None
     if not args.no_cpu:
         log("cpu baseline")
@@ -554,6 +568,7 @@ def main():
                      "gpu_box_tests_per_ray": round(box_tests, 2), "gpu_prim_tests_per_ray": round(prim_tests, 2)},
         "cpu_baseline": cpu,
         "hit_fraction": round(hit_frac, 4),
+        "any_hit": any_hit,
         "bvh": {"inner_nodes": info["n_inner_nodes"], "max_depth": info["max_depth"],
                 "device_mb": round(info["device_bytes"] / 2 ** 20, 1), "build_s": round(info["build_seconds"], 2)},
         "wall_s_timed": round(wall, 4),
