@@ -1,7 +1,7 @@
 // Batched ray queries over the traversal steps of trace.h: closest and any
 // hit on the four-child tree by persistent waves that refill their lanes from
-// a chunk of the batch; the two-child tree and the instrumented (counting)
-// traversal one ray per lane.
+// a static run of the batch, then from runs handed out by a device queue; the
+// two-child tree and the instrumented (counting) traversal one ray per lane.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "kernels.h"
