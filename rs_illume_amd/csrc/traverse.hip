@@ -1,7 +1,8 @@
 // Batched ray queries over the traversal steps of trace.h: closest and any
-// hit on the four-child tree by persistent waves that refill their lanes from
-// a static run of the batch, then from runs handed out by a device queue; the
-// two-child tree and the instrumented (counting) traversal one ray per lane.
+// hit on the four-child tree, and any hit on the two-child tree, by persistent
+// waves that refill their lanes from a static run of the batch, then from runs
+// handed out by a device queue; closest hit on the two-child tree and the
+// instrumented (counting) traversal one ray per lane.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "kernels.h"
@@ -195,7 +196,7 @@ __global__ void fill_miss_kernel(uint32_t n, illume_hit *hits, uint8_t *occluded
 
 int max_supported_depth() { return kMaxTraceDepth; }
 
-// Persistent launch on the four-child tree: one 64-lane wave per wave slot
+// Persistent launch (either tree width): one 64-lane wave per wave slot
 // (compute units x 4 SIMDs x waves per SIMD), each starting on an equal static
 // run; the queue counter (stream-ordered allocation, set by a one-thread
 // kernel so that the launch sequence can be captured in a graph) starts past
@@ -226,10 +227,10 @@ hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illum
     return e != hipSuccess ? e : f;
 }
 
-// One ray per lane: counting queries, and every query on the two-child tree
-// (whose cheaper steps do not repay the refill: closest hit 13.5 against
-// 12.6 ms per launch on the 2^20-triangle / 2^24-ray workload, with the
-// queue; any hit 9.08 against 9.28).
+// One ray per lane: counting queries, and closest hit on the two-child tree
+// (whose cheaper steps do not repay the refill: 13.5 against 12.6 ms per
+// launch on the 2^20-triangle / 2^24-ray workload, with the queue; any hit
+// there takes the refill kernel, 9.08 against 9.28 ms).
 template <int W>
 hipError_t launch_per_ray(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
                           illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {
@@ -258,6 +259,7 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
         return hipGetLastError();
     }
     if (width == 4 && !counts) return launch_refill<4>(nodes, prims, rays, n, hits, occluded, any, stream);
+    if (any && !counts) return launch_refill<2>(nodes, prims, rays, n, hits, occluded, any, stream);
     if (width == 4) return launch_per_ray<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
     return launch_per_ray<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
 }
