@@ -5,9 +5,16 @@
  * the reference interface it replaces (file:line in mlunnay/rs_illume). No
  * C++ or torch types cross it: plain pointers, sizes and an opaque hipStream_t
  * passed as void*. Every call returns an illume_status; nothing unwinds across
- * the ABI. Ray/hit buffers are caller-owned DEVICE pointers; the BVH handle
- * owns its device memory and is immutable after creation, so concurrent
- * queries on different streams are safe.
+ * the ABI. Ray/hit buffers are caller-owned DEVICE pointers.
+ *
+ * Threading (the reference's traits are Send + Sync and are queried from rayon
+ * workers, core/primitive/mod.rs:6, core/integrator.rs:59): BVH, scene, filter
+ * handles are immutable after creation and own their device memory, so any
+ * number of host threads may query or render one handle at once, each on its
+ * own stream. A guide handle is read-only for rendering and inference but
+ * written by its training calls; a wavefront handle holds the per-render path
+ * pool and is used by one call at a time (one stream, one host thread). The
+ * last-error message is kept per host thread.
  */
 #ifndef ILLUME_HIP_H
 #define ILLUME_HIP_H
@@ -55,7 +62,7 @@ typedef struct illume_bvh illume_bvh;
 
 typedef struct {
     uint64_t n_prims;
-    uint64_t n_inner_nodes;  /* 64-byte two-child nodes */
+    uint64_t n_inner_nodes;  /* 64-byte nodes: two-child, or quantized four-child (width 4) */
     uint64_t n_leaves;
     uint64_t max_depth;
     uint64_t device_bytes;   /* nodes + primitive records */
@@ -73,9 +80,11 @@ typedef struct {
 int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                       const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out);
 
-/* Same build with a chosen node width: 2 = 64-byte two-child nodes, 4 =
- * 128-byte four-child nodes (the binary SAH tree collapsed, half the depth).
- * Hit results do not depend on the width. illume_bvh_create uses width 2. */
+/* Same build with a chosen node width: 2 = 64-byte two-child nodes (float
+ * boxes), 4 = 64-byte four-child nodes (the binary SAH tree collapsed to half
+ * the depth, child boxes quantized to one byte per plane and rounded outwards,
+ * so every box the float test enters is entered). Hit results do not depend
+ * on the width. illume_bvh_create uses width 2. */
 int illume_bvh_create_ex(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                          const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
                          illume_bvh **out);
@@ -102,14 +111,18 @@ int illume_bvh_info(const illume_bvh *bvh, illume_bvh_info_t *out);
  * (core/primitive/mod.rs:11, accelerators/bvh.rs:563-612, reached through
  * Scene::intersect core/scene.rs:43-47). Each ray's t_max bounds the search.
  * rays, hits: device pointers. stream: hipStream_t (null = default stream).
- * On a four-child tree the call (and illume_intersect_any) allocates 8 bytes
- * stream-ordered on `stream` for its ray queue and frees them the same way. */
+ * n <= 2^31 (larger batches: ILLUME_ERR_UNSUPPORTED; split them).
+ * The persistent launches — closest and any hit on a four-child tree, any
+ * hit on a two-child tree — allocate 8 bytes stream-ordered on `stream` for
+ * their ray queue and free them the same way (hipMallocAsync/hipFreeAsync);
+ * closest hit on a two-child tree allocates nothing. */
 int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
                              void *stream);
 
 /* Any hit for n rays: batched Primitive::intersect_p (core/primitive/mod.rs:12,
  * accelerators/bvh.rs:614-662, via Scene::intersect_p core/scene.rs:49-53).
- * occluded[i] = 1 if any primitive is hit in (0, t_max], else 0. */
+ * occluded[i] = 1 if any primitive is hit in (0, t_max], else 0. n <= 2^31;
+ * allocates its queue counter as illume_intersect_closest does. */
 int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
                          void *stream);
 
@@ -238,7 +251,9 @@ int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out);
  * radiance of cfg->spp camera paths to rgb_sum[3*(y*width+x) + c] (a device
  * float buffer of width*height*3 the caller zeroes and divides by the total
  * sample count; box filter of radius 0.5). ray_counts (device, 2 x uint64, may
- * be null) accumulates the number of closest-hit and any-hit rays traced. */
+ * be null) accumulates the number of closest-hit and any-hit rays traced.
+ * Scenes of 2^20 or more primitives run through illume_render_batch and, like
+ * it, refuse stream capture (ILLUME_ERR_UNSUPPORTED). */
 int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                   float *rgb_sum, uint64_t *ray_counts, void *stream);
 
@@ -350,7 +365,10 @@ typedef struct {
  * all their paths (each thread finds its job): every job's image equals its
  * own illume_render call (cfg.flags selects its sampler; no proposal). For
  * many plain renders of uneven size; the wavefront pool below adds the
- * learned proposal. Integrator::render (core/integrator.rs:17-29) per job. */
+ * learned proposal. Integrator::render (core/integrator.rs:17-29) per job.
+ * The job table is copied from a host buffer released by a stream-ordered host
+ * callback, so the call cannot be captured in a graph: on a capturing stream it
+ * returns ILLUME_ERR_UNSUPPORTED before any device work. */
 int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream);
 
 int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavefront **out);
@@ -381,15 +399,6 @@ int illume_wavefront_render_filtered(illume_wavefront *wf, const illume_scene *s
                                      const illume_render_cfg *cfg, const illume_filter *filter, illume_guide *guide,
                                      const illume_guide_cfg *gcfg, float *rgbw_sum, uint64_t *ray_counts,
                                      void *stream);
-
-/* ------------------------------------------------------------------ measurement
- * No counterpart in the reference: a calibration probe for hardware read
- * counters. n_lanes lanes each read `steps` random 64-byte lines of buf (device,
- * buf_bytes bytes, 16-byte aligned) with four float4 loads per line, the way
- * the traversal fetches nodes; out: device [n_lanes] floats (sums, so the
- * loads are kept). Bytes read = n_lanes * steps * 64. */
-int illume_probe_random_reads(const void *buf, uint64_t buf_bytes, uint32_t n_lanes, uint32_t steps, float *out,
-                              void *stream);
 
 /* Human-readable name of a status code. */
 const char *illume_status_string(int status);

@@ -8,7 +8,8 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librs_illume_amd.so")
-HEADER = os.path.join(os.path.dirname(HERE), "include", "illume_hip.h")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HEADER = os.path.join(INCLUDE, "illume_hip.h")
 
 _lib = None
 
@@ -52,6 +53,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("librs_illume_amd.so is not built: run `python -m rs_illume_amd.build_lib` "
                                "(or __graft_entry__.build())")
+        # torch's HIP runtime first: the library then binds to the runtime
+        # already in the process (the same soname) instead of loading a second
+        # one that does not see the devices torch has opened
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
         _lib = L
@@ -67,8 +72,13 @@ def check(status, fn):
         raise IllumeError(status, fn, lib().illume_last_error().decode())
 
 
-def header_functions(path=HEADER):
-    """Names of the functions declared in include/illume_hip.h."""
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(illume_[a-z_0-9]+)\s*\(", text)))
+def header_functions(path=None):
+    """Names of the functions declared in one header, or (path None) in every
+    header under include/ (the drop-in boundary illume_hip.h and the
+    measurement-only illume_probe.h)."""
+    paths = [path] if path else sorted(os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h"))
+    names = set()
+    for p in paths:
+        text = re.sub(r"/\*.*?\*/", "", open(p).read(), flags=re.S)
+        names.update(re.findall(r"\b(illume_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)

@@ -239,8 +239,15 @@ int illume_bvh_info(const illume_bvh *bvh, illume_bvh_info_t *out) {
     return ILLUME_OK;
 }
 
+static int check_ray_count(uint32_t n) {
+    if (n > illume::kMaxRaysPerCall)
+        return fail(ILLUME_ERR_UNSUPPORTED, "at most 2^31 rays per traversal call (split the batch)");
+    return ILLUME_OK;
+}
+
 static int run_traverse(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
                         uint8_t *occluded, uint32_t *counts, bool any, void *stream) {
+    if (int st = check_ray_count(n)) return st;
     if (!bvh) return fail(ILLUME_ERR_INVALID_ARG, "bvh is null");
     if (n == 0) return ILLUME_OK;
     if (!rays) return fail(ILLUME_ERR_INVALID_ARG, "rays is null");
@@ -253,18 +260,21 @@ static int run_traverse(const illume_bvh *bvh, const illume_ray *rays, uint32_t 
 
 int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
                              void *stream) {
+    if (int st = check_ray_count(n)) return st;
     if (n && !hits) return fail(ILLUME_ERR_INVALID_ARG, "hits is null");
     return run_traverse(bvh, rays, n, hits, nullptr, nullptr, false, stream);
 }
 
 int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
                          void *stream) {
+    if (int st = check_ray_count(n)) return st;
     if (n && !occluded) return fail(ILLUME_ERR_INVALID_ARG, "occluded is null");
     return run_traverse(bvh, rays, n, nullptr, occluded, nullptr, true, stream);
 }
 
 int illume_intersect_counts(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, int any, uint32_t *counts,
                             void *stream) {
+    if (int st = check_ray_count(n)) return st;
     if (n && !counts) return fail(ILLUME_ERR_INVALID_ARG, "counts is null");
     return run_traverse(bvh, rays, n, nullptr, nullptr, counts, any != 0, stream);
 }

@@ -51,9 +51,15 @@ hipError_t launch_resolve(const float *rgbw, uint64_t n, float scale, float *rgb
 // Deepest tree the traversal stack supports (LDS ring + private spill).
 int max_supported_depth();
 
+// Largest ray count of one traversal call: grid sizes and the queue's run
+// ends are computed in 32 bits and stay exact up to here.
+constexpr uint32_t kMaxRaysPerCall = 1u << 31;
+
 // Closest hit (hits), any hit (occluded) or counting traversal (counts != null).
 // `empty` = the BVH has no primitives: outputs are filled with misses.
-// nodes: GpuNode (width 2) or GpuNode4Q (width 4) array.
+// nodes: GpuNode (width 2) or GpuNode4Q (width 4) array. n <= kMaxRaysPerCall.
+// The persistent launches (closest and any hit on four-child trees, any hit
+// on two-child trees) allocate their 8-byte queue counter stream-ordered.
 hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims, bool empty, const illume_ray *rays,
                            uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
                            hipStream_t stream);

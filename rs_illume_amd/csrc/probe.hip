@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <string>
 #include "../../include/illume_hip.h"
+#include "../../include/illume_probe.h"
 
 namespace illume {
 int abi_fail(int status, const std::string &msg);

@@ -472,11 +472,37 @@ int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, ui
     }
     if (total == 0) return ILLUME_OK;
     hipStream_t st = (hipStream_t)stream;
+    // The job table is copied from a host buffer that must outlive the copy.
+    // A captured graph would replay the copy from a buffer released after the
+    // first replay, so capture is refused.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(st, &cap);
+    if (e != hipSuccess) return abi_hip_fail(e, "stream capture query");
+    if (cap != hipStreamCaptureStatusNone)
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "illume_render_batch (and illume_render of scenes with 2^20 or more "
+                                                "primitives) cannot be captured in a graph");
+    const size_t bytes = hj.size() * sizeof(RenderJob);
+    // host staging owned by the stream: released by a host callback once the
+    // stream has passed the copy, whatever the driver does with pageable memory
+    RenderJob *staging = new (std::nothrow) RenderJob[hj.size()];
+    if (!staging) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "job table staging");
+    std::memcpy(staging, hj.data(), bytes);
     RenderJob *d_jobs = nullptr;
-    hipError_t e = hipMallocAsync((void **)&d_jobs, hj.size() * sizeof(RenderJob), st);
-    if (e != hipSuccess) return abi_hip_fail(e, "job table allocation");
-    // from pageable memory: returns once the table has been staged, so hj may go
-    e = hipMemcpyAsync(d_jobs, hj.data(), hj.size() * sizeof(RenderJob), hipMemcpyHostToDevice, st);
+    e = hipMallocAsync((void **)&d_jobs, bytes, st);
+    if (e != hipSuccess) {
+        delete[] staging;
+        return abi_hip_fail(e, "job table allocation");
+    }
+    e = hipMemcpyAsync(d_jobs, staging, bytes, hipMemcpyHostToDevice, st);
+    const hipError_t eh = e == hipSuccess
+                              ? hipLaunchHostFunc(st, [](void *p) { delete[] static_cast<RenderJob *>(p); }, staging)
+                              : e;
+    if (eh != hipSuccess) {
+        // no callback queued: wait for the copy (if it was queued) before the staging goes
+        (void)hipStreamSynchronize(st);
+        delete[] staging;
+    }
+    if (e == hipSuccess) e = eh;
     if (e == hipSuccess)
         e = launch_render_batch(d_jobs, (uint32_t)hj.size(), total, film, (unsigned long long *)ray_counts, st);
     const hipError_t ef = hipFreeAsync(d_jobs, st);

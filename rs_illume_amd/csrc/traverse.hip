@@ -230,7 +230,7 @@ hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illum
 // One ray per lane: counting queries, and closest hit on the two-child tree
 // (whose cheaper steps do not repay the refill: 13.5 against 12.6 ms per
 // launch on the 2^20-triangle / 2^24-ray workload, with the queue; any hit
-// there takes the refill kernel, 9.08 against 9.28 ms).
+// there takes the refill kernel, 8.84 against 9.3 ms).
 template <int W>
 hipError_t launch_per_ray(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
                           illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any, hipStream_t stream) {

@@ -63,6 +63,19 @@ def test_null_arguments():
     assert L.illume_bvh_destroy(None) == 0
 
 
+def test_ray_count_limit_checked_first():
+    """Grid sizes and queue run ends are 32-bit: batches above 2^31 rays are
+    refused (ILLUME_ERR_UNSUPPORTED) before any other check or device work."""
+    from rs_illume_amd import _lib
+    L = _lib.lib()
+    big = (1 << 31) + 1
+    assert L.illume_intersect_closest(None, None, big, None, None) == 4
+    assert "2^31" in L.illume_last_error().decode()
+    assert L.illume_intersect_any(None, None, big, None, None) == 4
+    assert L.illume_intersect_counts(None, None, big, 0, None, None) == 4
+    assert L.illume_intersect_closest(None, None, 1 << 31, None, None) == 1  # at the limit: next check fails
+
+
 def test_kernels_only_in_hip_sources():
     """Every kernel is compiled for gfx950: kernels may only be defined in .hip
     files, and every source is compiled with the gfx950 device flags."""

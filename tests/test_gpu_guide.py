@@ -61,7 +61,8 @@ def test_grad_matches_oracle(gpu, oracle):
     rel64 = np.linalg.norm(gg - g64) / np.linalg.norm(g64)
     cos = gg @ g16 / (np.linalg.norm(gg) * np.linalg.norm(g16))
     _say("grad rel err vs oracle bf16", rel16, "vs f64", rel64, "cos", cos)
-    assert cos > 0.995 and rel16 < 0.1 and rel64 < 0.15
+    # same bf16 roundings as the oracle's network: 3.6e-4 observed
+    assert cos > 0.9999 and rel16 < 1e-2 and rel64 < 0.15
     # every layer's block of the gradient agrees in direction
     from rs_illume_amd.guide import offsets
     for w_off, b_off in offsets():
