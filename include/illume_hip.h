@@ -126,6 +126,20 @@ int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint
 int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
                          void *stream);
 
+/* No counterpart in the reference: illume_intersect_closest for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results, bit for
+ * bit; it calls no hipMalloc/hipFree (stream-ordered or not). The persistent
+ * launch's ray queue counter lives in a 256-byte array that the library keeps
+ * on each device until the process ends (its address is looked up once per
+ * device, on the first call with that device current); the counter is reset
+ * by a one-thread kernel at the start of every call, so any ray count up to
+ * the limit (n <= 2^31, else ILLUME_ERR_UNSUPPORTED) uses the same 8 bytes.
+ * Calls that overlap on one device (two streams or two host threads at once)
+ * share that counter and race: serialise them. */
+int illume_intersect_closest_noalloc(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
+                                     void *stream);
+
 /* Instrumented closest/any traversal that also counts, per ray, node-box tests
  * and primitive tests (the reference's "BVH/..." and "Intersections/..."
  * counters, core/stats_accumulator.rs). counts: device uint32[2*n] (node

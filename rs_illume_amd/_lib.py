@@ -36,6 +36,7 @@ def _declare(L):
         "illume_bvh_info": (i32, [vp, ctypes.POINTER(BvhInfo)]),
         "illume_intersect_closest": (i32, [vp, vp, u32, vp, vp]),
         "illume_intersect_any": (i32, [vp, vp, u32, vp, vp]),
+        "illume_intersect_closest_noalloc": (i32, [vp, vp, u32, vp, vp]),
         "illume_intersect_counts": (i32, [vp, vp, u32, i32, vp, vp]),
         "illume_status_string": (ctypes.c_char_p, [i32]),
         "illume_last_error": (ctypes.c_char_p, []),

@@ -265,6 +265,19 @@ int illume_intersect_closest(const illume_bvh *bvh, const illume_ray *rays, uint
     return run_traverse(bvh, rays, n, hits, nullptr, nullptr, false, stream);
 }
 
+int illume_intersect_closest_noalloc(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
+                                     void *stream) {
+    if (int st = check_ray_count(n)) return st;
+    if (!bvh) return fail(ILLUME_ERR_INVALID_ARG, "bvh is null");
+    if (n == 0) return ILLUME_OK;
+    if (!rays) return fail(ILLUME_ERR_INVALID_ARG, "rays is null");
+    if (!hits) return fail(ILLUME_ERR_INVALID_ARG, "hits is null");
+    hipError_t e = illume::launch_closest_noalloc(bvh->d_nodes, bvh->width, bvh->d_prims, bvh->info.n_prims == 0,
+                                                  rays, n, hits, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "traversal launch");
+    return ILLUME_OK;
+}
+
 int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, uint8_t *occluded,
                          void *stream) {
     if (int st = check_ray_count(n)) return st;

@@ -64,4 +64,15 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
                            uint32_t n, illume_hit *hits, uint8_t *occluded, uint32_t *counts, bool any,
                            hipStream_t stream);
 
+// Words of the device-resident traversal scratch (256 bytes; word 0 is the
+// persistent launch's queue counter).
+constexpr int kTraverseScratchWords = 32;
+
+// Closest hit like launch_traverse, allocating nothing: the persistent
+// launch's queue counter is word 0 of the library's per-device scratch array
+// (address looked up once per device). Concurrent calls on one device race on
+// that word. n <= kMaxRaysPerCall.
+hipError_t launch_closest_noalloc(const void *nodes, int width, const PrimRecord *prims, bool empty,
+                                  const illume_ray *rays, uint32_t n, illume_hit *hits, hipStream_t stream);
+
 }  // namespace illume

@@ -5,10 +5,18 @@
 // instrumented (counting) traversal one ray per lane.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include "kernels.h"
 #include "trace.h"
 
 #pragma clang fp contract(off)
+
+// Device-resident scratch of the non-allocating closest-hit entry point
+// (illume_intersect_closest_noalloc): word 0 is the persistent launch's ray
+// queue counter. Each device holds its own copy until the process ends; host
+// code uses the address hipGetSymbolAddress returns for the current device,
+// never this name. Calls that overlap on one device share it and race.
+static __device__ __align__(256) unsigned long long scratch[illume::kTraverseScratchWords];
 
 namespace illume {
 
@@ -201,9 +209,14 @@ int max_supported_depth() { return kMaxTraceDepth; }
 // run; the queue counter (stream-ordered allocation, set by a one-thread
 // kernel so that the launch sequence can be captured in a graph) starts past
 // the static runs.
+//
+// The counter lives in an 8-byte stream-ordered allocation made and freed by
+// the call, or, when `static_queue` is given, in that device word (the
+// non-allocating entry point's scratch).
 template <int W>
 hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illume_ray *rays, uint32_t n,
-                         illume_hit *hits, uint8_t *occluded, bool any, hipStream_t stream) {
+                         illume_hit *hits, uint8_t *occluded, bool any, unsigned long long *static_queue,
+                         hipStream_t stream) {
     int dev = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -212,9 +225,11 @@ hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illum
     const uint32_t waves = std::min((n + 63u) / 64u, slots);
     const uint32_t per_wave = (n + waves - 1) / waves / kStaticDiv;
     const uint32_t grab = kQueueRun;
-    unsigned long long *queue = nullptr;
-    e = hipMallocAsync((void **)&queue, sizeof(unsigned long long), stream);
-    if (e != hipSuccess) return e;
+    unsigned long long *queue = static_queue;
+    if (!queue) {
+        e = hipMallocAsync((void **)&queue, sizeof(unsigned long long), stream);
+        if (e != hipSuccess) return e;
+    }
     queue_start_kernel<<<1, 1, 0, stream>>>(queue, (unsigned long long)waves * per_wave);
     if (any)
         traverse_refill_kernel<true, W><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, grab, queue,
@@ -223,8 +238,31 @@ hipError_t launch_refill(const void *nodes, const PrimRecord *prims, const illum
         traverse_refill_kernel<false, W><<<waves, 64, 0, stream>>>(nodes, prims, rays, n, per_wave, grab, queue,
                                                                    hits, nullptr);
     e = hipGetLastError();
+    if (static_queue) return e;
     const hipError_t f = hipFreeAsync(queue, stream);
     return e != hipSuccess ? e : f;
+}
+
+// Address of this device's `scratch` array, looked up once per device (with
+// the device current) and kept; hipGetSymbolAddress allocates nothing.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[kMaxScratchDevices] = {};
+
+hipError_t scratch_address(unsigned long long **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 // One ray per lane: counting queries, and closest hit on the two-child tree
@@ -258,10 +296,25 @@ hipError_t launch_traverse(const void *nodes, int width, const PrimRecord *prims
                                                               counts ? nullptr : (any ? occluded : nullptr), counts);
         return hipGetLastError();
     }
-    if (width == 4 && !counts) return launch_refill<4>(nodes, prims, rays, n, hits, occluded, any, stream);
-    if (any && !counts) return launch_refill<2>(nodes, prims, rays, n, hits, occluded, any, stream);
+    if (width == 4 && !counts) return launch_refill<4>(nodes, prims, rays, n, hits, occluded, any, nullptr, stream);
+    if (any && !counts) return launch_refill<2>(nodes, prims, rays, n, hits, occluded, any, nullptr, stream);
     if (width == 4) return launch_per_ray<4>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
     return launch_per_ray<2>(nodes, prims, rays, n, hits, occluded, counts, any, stream);
+}
+
+hipError_t launch_closest_noalloc(const void *nodes, int width, const PrimRecord *prims, bool empty,
+                                  const illume_ray *rays, uint32_t n, illume_hit *hits, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    if (empty) {
+        fill_miss_kernel<<<(n + 255) / 256, 256, 0, stream>>>(n, hits, nullptr, nullptr);
+        return hipGetLastError();
+    }
+    // the two-child tree's closest hit runs one ray per lane and needs no queue
+    if (width != 4) return launch_per_ray<2>(nodes, prims, rays, n, hits, nullptr, nullptr, false, stream);
+    unsigned long long *queue = nullptr;
+    const hipError_t e = scratch_address(&queue);
+    if (e != hipSuccess) return e;
+    return launch_refill<4>(nodes, prims, rays, n, hits, nullptr, false, queue, stream);
 }
 
 }  // namespace illume
