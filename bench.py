@@ -23,7 +23,14 @@ Also measured (keys of the same JSON line):
   combined - terrain + spheres under the analytic sky, 16 views batched,
              plain vs guided time-to-relMSE per view and samples/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-render]
+Roofline of the headline kernel: the HBM-side bytes per closest-hit launch
+(FETCH_SIZE + WRITE_SIZE of the traversal kernel, measured in this run by
+child rocprofv3 --pmc passes before the GPU is touched, with the L2 hit rate)
+over the launch time from HIP events; the algorithmic bytes per ray (the
+kernel's own 64-byte nodes and 48-byte primitive records, and the reference's
+pbrt layout) are reported beside it under their own names.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-render] [--no-counters]
 """
 import argparse
 import json
@@ -48,29 +55,59 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(width):
-    """HBM bytes per closest-hit launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
-    path = os.path.join(REPO, "profiles", "traverse_pmc_1M_tris_16M_rays.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    w = d["width"].get(str(width))
-    if not w:
-        return None
-    cal = d["calibration"]
-    return (w["fetch_kib"] * cal["FETCH_SIZE"] + w["write_kib"] * cal["WRITE_SIZE"]) * 1024.0
-
-
 def _oracle():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
     return pyoracle
 
 
+def host_cpu():
+    """(threads, model): every host thread this process may run on, and the
+    CPU model, for the CPU-baseline legs."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, n), model
+
+
 def cpu_threads():
-    return max(1, min(16, os.cpu_count() or 1))
+    return host_cpu()[0]
+
+
+def cpu_progressive(osc, cam, ref, max_depth, seed, spp_pass, budget_s, target=0.01):
+    """The oracle's path tracer on host cores in progressive passes of
+    spp_pass samples per pixel until the wall budget is spent: the time and spp
+    at which relMSE against `ref` (host (H, W, 3)) first reaches `target`, and
+    the relMSE at the budget."""
+    from rs_illume_amd.scenes import rel_mse
+    threads = cpu_threads()
+    acc = None
+    spp, elapsed, reached = 0, 0.0, None
+    while elapsed < budget_s:
+        t0 = time.time()
+        s, _ = osc.render_sum(cam, spp_pass, max_depth, seed=seed, sample_offset=spp, n_threads=threads)
+        elapsed += time.time() - t0
+        acc = s.astype(np.float64) if acc is None else acc + s
+        spp += spp_pass
+        err = rel_mse(acc / spp, ref)
+        if reached is None and err <= target:
+            reached = (elapsed, spp)
+    w, h = int(cam["width"][0]), int(cam["height"][0])
+    return {"seconds": round(elapsed, 2), "spp": spp, "rel_mse": round(float(err), 6),
+            "samples_per_s": round(w * h * spp / elapsed, 1),
+            "time_to_target_s": round(reached[0], 3) if reached else None,
+            "spp_to_target": reached[1] if reached else None, "cores": threads, "cpu_model": host_cpu()[1],
+            "kind": "port"}
 
 
 def cpu_baseline(v, idx, rays, budget_s=15.0):
@@ -97,7 +134,8 @@ def cpu_baseline(v, idx, rays, budget_s=15.0):
         _, st = sc.intersect(sample, n_threads=threads, with_stats=True)
         dt = time.time() - t0
     counts = (float(st[0]) / len(sample), float(st[1]) / len(sample))
-    return {"value": round(len(sample) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return {"value": round(len(sample) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "cpu_model": host_cpu()[1], "kind": "port",
             "sample": "%d of the %d rays (strided), closest hit, oracle pbrt-v3 SAH BVH (build %.1f s excluded)"
                       % (len(sample), len(rays), build_s)}, counts
 
@@ -171,18 +209,24 @@ def bench_cornell(no_cpu):
         dtc = time.time() - t0
         out["cpu_baseline"] = {"samples_per_s": round(256 * 256 * cspp / dtc, 1),
                                "mrays_per_s": round(float(st.sum()) / dtc / 1e6, 3), "cores": threads,
+                               "cpu_model": host_cpu()[1],
                                "kind": "port", "sample": "%d spp of the 256x256 frame" % cspp}
     sc.close()
     return out
 
 
-def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32, plain="megakernel"):
-    """Config-3 equal-quality comparison (rs_illume_amd.converge)."""
+def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32, plain="megakernel", no_cpu=False,
+                 cpu_budget_s=20.0):
+    """Config-3 equal-quality comparison (rs_illume_amd.converge); the CPU
+    leg runs the oracle's path tracer on host cores in 2-spp passes for a fixed
+    wall budget against the same reference (time to relMSE 0.01 and the relMSE
+    at the budget)."""
     import torch
     from rs_illume_amd import converge as C
     from rs_illume_amd import scenes as S
     from rs_illume_amd.render import PathIntegrator, Scene
-    sc = Scene(S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625))
+    desc = S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625)
+    sc = Scene(desc)
     cam = S.cornell_camera(res, res)
     pi = PathIntegrator(cam, spp=256, max_depth=8, seed=987654321)
     ref = torch.zeros((res, res, 3), dtype=torch.float32, device="cuda")
@@ -206,6 +250,12 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
         out[k] = {"seconds": round(rr["seconds"], 5), "spp": rr["spp"], "rel_mse": round(rr["rel_mse"], 5)} \
             if rr else {"seconds": None, "spp": r[k]["spp"], "rel_mse": round(r[k]["curve"][-1][2], 5)}
     out["time_ratio_guided_over_plain"] = round(r["time_ratio"], 3) if "time_ratio" in r else None
+    if not no_cpu:
+        log("guided: CPU leg (oracle, %.0f s budget)" % cpu_budget_s)
+        osc = _oracle().OracleRenderScene(desc.arrays())
+        cpu = cpu_progressive(osc, cam, ref.cpu().numpy(), 8, 1, 2, cpu_budget_s)
+        cpu["sample"] = "the whole %dx%d frame in 2-spp passes for a %.0f s wall budget" % (res, res, cpu_budget_s)
+        out["cpu_baseline"] = cpu
     sc.close()
     return out
 
@@ -275,32 +325,50 @@ def bench_batched(no_cpu, log=log, builder="SAH"):
            "seconds": top["seconds"], "samples_per_s": top["samples_per_s"], "mrays_per_s": top["mrays_per_s"],
            "wavefront": wave, "megakernel_batch": mega}
     if not no_cpu:
+        # the oracle renders whole scenes (their resolution and spp) on host
+        # cores in seed order, scenes 1-4 at least, until a 15 s render budget
+        # is spent; the rest of the batch's CPU time is extrapolated at that rate
         pyoracle = _oracle()
-        d, cam, _ = descs[0]
-        osc = pyoracle.OracleRenderScene(d.arrays())
         threads = cpu_threads()
-        t0 = time.time()
-        _, st = osc.render_sum(cam, 1, 8, seed=1, n_threads=threads)
-        dtc = time.time() - t0
-        px = int(cam["width"][0]) * int(cam["height"][0])
-        out["cpu_baseline"] = {"samples_per_s": round(px / dtc, 1), "mrays_per_s": round(float(st.sum()) / dtc / 1e6, 3),
-                               "cores": threads, "kind": "port",
-                               "sample": "scene 1 (%d tris, %dx%d) at 1 spp; not the whole batch"
-                                         % (len(d.idx), int(cam["width"][0]), int(cam["height"][0]))}
+        paths, secs, rays, build, done = 0, 0.0, 0.0, 0.0, 0
+        for seed in range(1, len(descs) + 1):
+            if seed > 4 and secs >= 15.0:
+                break
+            d, cam, spp = descs[seed - 1]
+            t0 = time.time()
+            osc = pyoracle.OracleRenderScene(d.arrays())
+            build += time.time() - t0
+            t0 = time.time()
+            _, st = osc.render_sum(cam, spp, 8, seed=seed, n_threads=threads)
+            secs += time.time() - t0
+            paths += int(cam["width"][0]) * int(cam["height"][0]) * spp
+            rays += float(st.sum())
+            done = seed
+            del osc
+        sps = paths / secs
+        out["cpu_baseline"] = {"samples_per_s": round(sps, 1), "mrays_per_s": round(rays / secs / 1e6, 3),
+                               "cores": threads, "cpu_model": host_cpu()[1], "kind": "port",
+                               "sample": "scenes 1-%d in full (%d of the batch's %d paths, %.1f s; oracle BVH "
+                                         "builds %.1f s excluded)" % (done, paths, n_paths, secs, build),
+                               "batch_seconds": round(secs + (n_paths - paths) / sps, 1),
+                               "batch_seconds_basis": "measured" if done == len(descs) else
+                               "scenes 1-%d measured, the rest extrapolated at their rate" % done}
     for sc in scenes:
         sc.close()
     return out
 
 
-def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
-                   builder="SAH"):
+def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
+                   builder="SAH", no_cpu=False, cpu_budget_s=20.0):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
     Plain (megakernel batch) and guided (wavefront pool; one proposal network
     for the scene, trained online;
     the radiance field it learns is view-independent) run progressive passes
     of all views; a view's time-to-relMSE is the batch time at which its image
-    first reaches the target against a ref_spp plain reference."""
+    first reaches the target against a ref_spp plain reference. The CPU leg
+    renders view 0 at its native resolution with the oracle on host cores in
+    1-spp passes for a fixed wall budget (samples/s, relMSE reached)."""
     import torch
     from rs_illume_amd import scenes as S
     from rs_illume_amd.guide import Guide
@@ -373,6 +441,18 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
     guided = run(Guide(seed=7))
     wf.close()
     sc.close()
+    cpu = None
+    if not no_cpu:
+        log("combined: CPU leg (oracle, view 0, %.0f s budget)" % cpu_budget_s)
+        t1 = time.time()
+        osc = _oracle().OracleRenderScene(desc.arrays())
+        build_s = time.time() - t1
+        w0, h0 = int(cams[0]["width"][0]), int(cams[0]["height"][0])
+        ref0 = ref[int(offs[0]):int(offs[1])].reshape(h0, w0, 3).cpu().numpy()
+        cpu = cpu_progressive(osc, cams[0], ref0, depth, 11, 1, cpu_budget_s, target)
+        cpu["sample"] = ("view 0 (%dx%d) in 1-spp passes for a %.0f s wall budget (oracle BVH build %.1f s excluded)"
+                         % (w0, h0, cpu_budget_s, build_s))
+        del osc
     both = [(p, g) for p, g in zip(plain["time_to_target_per_view"], guided["time_to_target_per_view"])
             if p is not None and g is not None]
     return {"workload": "fBm terrain 1024^2 (2.09M tris, 6 octaves, lacunarity 2, gain 0.5) + 10k spheres "
@@ -385,7 +465,7 @@ def bench_combined(ref_spp=256, spp_pass=2, max_spp=256, target=0.01, log=log, t
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
             "plain": plain, "guided": guided,
             "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both
-            else None}
+            else None, "cpu_baseline": cpu}
 
 
 def bench_mlp():
@@ -434,6 +514,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing measurements")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (roofline from algorithmic bytes only)")
     ap.add_argument("--only", choices=("cornell", "mlp", "guided", "batched", "combined"),
                     help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
@@ -445,13 +527,27 @@ def main():
               "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # hardware counters of the headline kernel, from child rocprofv3 passes
+    # over the same workload, before this process touches the GPU
+    pmc, pmc_error = None, None
+    if not args.no_counters and not args.only and args.rays == N_RAYS and args.builder == "SAH":
+        from rs_illume_amd import counters
+        try:
+            t0 = time.time()
+            pmc = counters.collect(args.width, log=log)
+            log("counter passes %.1f s" % (time.time() - t0))
+        except Exception as e:  # reported in the JSON line; the timing still runs
+            pmc_error = str(e)[-300:]
+            log("counter passes failed:", pmc_error)
+
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, rays_to_device
 
     if args.only:
-        fn = {"cornell": lambda: bench_cornell(args.no_cpu), "mlp": bench_mlp, "guided": bench_guided,
-              "batched": lambda: bench_batched(args.no_cpu), "combined": bench_combined}[args.only]
+        fn = {"cornell": lambda: bench_cornell(args.no_cpu), "mlp": bench_mlp,
+              "guided": lambda: bench_guided(no_cpu=args.no_cpu), "batched": lambda: bench_batched(args.no_cpu),
+              "combined": lambda: bench_combined(no_cpu=args.no_cpu)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
 
@@ -508,19 +604,55 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu, oracle_counts = cpu_baseline(v, idx, rays)
-    # algorithmic bytes per ray: the oracle's pbrt-layout BVH counts (32-byte
-    # nodes, 36-byte triangles) when the CPU leg ran, else the GPU's own counts
+    # algorithmic bytes per ray. Own layout: what this kernel's data
+    # structures hold for the work it does (ray 32 B in, hit 16 B out, one
+    # 64-byte node per visit, one 48-byte primitive record per test; counts
+    # from the counting traversal). pbrt layout: the reference's 32-byte
+    # binary nodes and 36-byte triangles at the oracle's counts (or, without
+    # the CPU leg, the GPU's counts).
+    visits = box_tests / float(args.width)
+    own_bytes = 32 + 16 + 64 * visits + 48 * prim_tests
     if oracle_counts:
         n_node, n_tri = oracle_counts
         src = "oracle pbrt BVH (node tests, triangle tests) on the cpu_baseline sample"
     else:
         n_node, n_tri = box_tests / 2.0, prim_tests
         src = "GPU counting kernel (box tests / 2, primitive tests)"
-    alg_bytes = 32 + 8 + 32 * n_node + 36 * n_tri
-    traffic = pmc_traffic(args.width) if n == N_RAYS else None
-    if traffic:
-        traffic = round(traffic)
-    achieved_gbs = alg_bytes * n / (ms_per_step * 1e-3) / 1e9
+    pbrt_bytes = 32 + 8 + 32 * n_node + 36 * n_tri
+    sec = ms_per_step * 1e-3
+    own_gbs = own_bytes * n / sec / 1e9
+    pbrt_gbs = pbrt_bytes * n / sec / 1e9
+    algorithmic = {
+        "own_layout": {"bytes_per_ray": round(own_bytes, 1), "gbs": round(own_gbs, 1),
+                       "frac": round(own_gbs / HBM_PEAK_GBS, 4), "node_visits_per_ray": round(visits, 2),
+                       "prim_tests_per_ray": round(prim_tests, 2),
+                       "formula": "32 (ray) + 16 (hit) + 64 x node visits + 48 x primitive tests"},
+        "pbrt_layout": {"bytes_per_ray": round(pbrt_bytes, 1), "gbs": round(pbrt_gbs, 1),
+                        "frac": round(pbrt_gbs / HBM_PEAK_GBS, 4), "node_tests_per_ray": round(n_node, 2),
+                        "tri_tests_per_ray": round(n_tri, 2), "source": src,
+                        "formula": "32 + 8 + 32 x node tests + 36 x triangle tests (the reference's layout; "
+                                   "the caches serve much of it, so this is not a bound)"},
+        "gpu_box_tests_per_ray": round(box_tests, 2), "gpu_prim_tests_per_ray": round(prim_tests, 2)}
+    if pmc and n == N_RAYS:
+        traffic = pmc["traffic_bytes"]
+        achieved = traffic / sec / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(traffic),
+                    "basis": "measured: FETCH_SIZE (calibrated) + WRITE_SIZE per launch of the traversal kernel from "
+                             "rocprofv3 --pmc passes in this run, over the HIP-event launch time",
+                    "kernel": pmc["kernel"], "fetch_bytes": round(pmc["fetch_bytes"]),
+                    "write_bytes": round(pmc["write_bytes"]),
+                    "fetch_calibration": {k: (round(v, 4) if isinstance(v, float) else v)
+                                          for k, v in pmc["fetch_calibration"].items()},
+                    "l2_hit_rate": round(pmc["l2_hit_rate"], 4),
+                    "memory_share_of_l2_reads": round(pmc["memory_share_of_l2_reads"], 4),
+                    "algorithmic": algorithmic}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(own_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(own_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                    "basis": "algorithmic bytes of the kernel's own layout (no counter pass in this run%s)"
+                             % ((": " + pmc_error) if pmc_error else ""),
+                    "algorithmic": algorithmic}
     del rd, hits
     bvh.close()
     torch.cuda.empty_cache()
@@ -532,11 +664,11 @@ def main():
         log("mlp")
         extra["mlp"] = bench_mlp()
         log("guided vs plain")
-        extra["guided"] = bench_guided()
+        extra["guided"] = bench_guided(no_cpu=args.no_cpu)
         log("batched")
         extra["batched"] = bench_batched(args.no_cpu)
         log("combined")
-        extra["combined"] = bench_combined()
+        extra["combined"] = bench_combined(no_cpu=args.no_cpu)
 
     out = {
         "metric": "Mrays/s closest-hit; time-to-relMSE 0.01, guided vs plain GPU path tracer",
@@ -556,16 +688,7 @@ def main():
                    "n_tris": N_TRIS, "n_rays": n, "max_prims_in_node": 4, "bvh_width": args.width,
                    "bvh_builder": args.builder,
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": "profiles/traverse_pmc_1M_tris_16M_rays.json: rocprofv3 FETCH_SIZE + "
-                                       "WRITE_SIZE bytes per launch (calibrated)" if traffic else None,
-                     "traffic_gbs": round(traffic / (ms_per_step * 1e-3) / 1e9, 1) if traffic else None,
-                     "traffic_frac": round(traffic / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-                     "algorithmic_bytes_per_launch": round(alg_bytes * n),
-                     "bytes_per_ray": round(alg_bytes, 1), "bytes_source": src,
-                     "node_tests_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2),
-                     "gpu_box_tests_per_ray": round(box_tests, 2), "gpu_prim_tests_per_ray": round(prim_tests, 2)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "hit_fraction": round(hit_frac, 4),
         "any_hit": any_hit,
