@@ -451,3 +451,110 @@ def guide_sample(outs, u):
 def adam(p, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     """In-place Adam step on float32 arrays p, m, v."""
     _guide_decl().orc_adam(_p(p), _p(m), _p(v), _p(_f32(g)), len(p), int(t), lr, b1, b2, eps)
+
+
+# ---------------------------------------------------------------- BSDF lobes (local frame, normal = +z)
+MAT_DTYPE = np.dtype([("type", "<i4"), ("kd", "<f4", 3), ("eta", "<f4", 3), ("k", "<f4", 3), ("alpha", "<f4"),
+                      ("ior", "<f4")])
+MATTE, METAL, GLASS = 0, 1, 2
+BSDF_REFLECTION, BSDF_TRANSMISSION, BSDF_DIFFUSE, BSDF_GLOSSY, BSDF_SPECULAR = 1, 2, 4, 8, 16
+
+
+def material(type, kd=(0, 0, 0), eta=(1, 1, 1), k=(0, 0, 0), alpha=0.1, ior=1.5):
+    m = np.zeros(1, MAT_DTYPE)
+    m["type"], m["kd"], m["eta"], m["k"], m["alpha"], m["ior"] = type, kd, eta, k, alpha, ior
+    return m
+
+
+def _bsdf_decl():
+    L = lib()
+    if getattr(L, "_bdecl", False):
+        return L
+    vp, u32, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float
+    for fn, at in (("orc_tr_eval", [f32, vp, vp, u32, vp, vp, vp, vp]), ("orc_tr_g", [f32, vp, vp, u32, vp]),
+                   ("orc_tr_sample_wh", [f32, vp, vp, u32, vp]), ("orc_bxdf_f", [vp, vp, vp, u32, vp]),
+                   ("orc_bxdf_pdf", [vp, vp, vp, u32, vp]), ("orc_bxdf_sample_f", [vp, vp, vp, u32, vp, vp, vp, vp]),
+                   ("orc_fr_dielectric", [vp, u32, f32, f32, vp]), ("orc_fr_conductor", [vp, u32, vp, vp, vp]),
+                   ("orc_refract", [vp, vp, u32, f32, vp, vp])):
+        getattr(L, fn).argtypes = at
+        getattr(L, fn).restype = None
+    L._bdecl = True
+    return L
+
+
+def _v3(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, 3))
+
+
+def tr_eval(alpha, wo, wh):
+    """Trowbridge-Reitz D(wh), Lambda(wo), G1(wo) and the visible-normal pdf of
+    wh given wo (arrays of n)."""
+    wo, wh = _v3(wo), _v3(wh)
+    n = len(wo)
+    d, lam, g1, pdf = (np.zeros(n, np.float32) for _ in range(4))
+    _bsdf_decl().orc_tr_eval(alpha, _p(wo), _p(wh), n, _p(d), _p(lam), _p(g1), _p(pdf))
+    return {"D": d, "lambda": lam, "G1": g1, "pdf": pdf}
+
+
+def tr_g(alpha, wo, wi):
+    wo, wi = _v3(wo), _v3(wi)
+    g = np.zeros(len(wo), np.float32)
+    _bsdf_decl().orc_tr_g(alpha, _p(wo), _p(wi), len(wo), _p(g))
+    return g
+
+
+def tr_sample_wh(alpha, wo, u):
+    wo = _v3(wo)
+    u = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 2))
+    wh = np.zeros_like(wo)
+    _bsdf_decl().orc_tr_sample_wh(alpha, _p(wo), _p(u), len(wo), _p(wh))
+    return wh
+
+
+def bxdf_f(mat, wo, wi):
+    wo, wi = _v3(wo), _v3(wi)
+    out = np.zeros_like(wo)
+    _bsdf_decl().orc_bxdf_f(_p(mat), _p(wo), _p(wi), len(wo), _p(out))
+    return out
+
+
+def bxdf_pdf(mat, wo, wi):
+    wo, wi = _v3(wo), _v3(wi)
+    out = np.zeros(len(wo), np.float32)
+    _bsdf_decl().orc_bxdf_pdf(_p(mat), _p(wo), _p(wi), len(wo), _p(out))
+    return out
+
+
+def bxdf_sample_f(mat, wo, u):
+    """(wi, f rgb, pdf, sampled flags) per sample; pdf 0 = no sample."""
+    wo = _v3(wo)
+    u = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 2))
+    n = len(wo)
+    wi, f = np.zeros_like(wo), np.zeros_like(wo)
+    pdf, sampled = np.zeros(n, np.float32), np.zeros(n, np.int32)
+    _bsdf_decl().orc_bxdf_sample_f(_p(mat), _p(wo), _p(u), n, _p(wi), _p(f), _p(pdf), _p(sampled))
+    return wi, f, pdf, sampled
+
+
+def fr_dielectric(cos_i, eta_i, eta_t):
+    c = np.ascontiguousarray(np.asarray(cos_i, np.float32).reshape(-1))
+    out = np.zeros_like(c)
+    _bsdf_decl().orc_fr_dielectric(_p(c), len(c), eta_i, eta_t, _p(out))
+    return out
+
+
+def fr_conductor(cos_i, eta, k):
+    c = np.ascontiguousarray(np.asarray(cos_i, np.float32).reshape(-1))
+    out = np.zeros((len(c), 3), np.float32)
+    _bsdf_decl().orc_fr_conductor(_p(c), len(c), _p(np.asarray(eta, np.float32)), _p(np.asarray(k, np.float32)),
+                                  _p(out))
+    return out
+
+
+def refract(wi, nrm, eta):
+    """(wt, ok) for each wi about normal nrm at relative index eta = eta_i / eta_t."""
+    wi, nrm = _v3(wi), _v3(nrm)
+    wt = np.zeros_like(wi)
+    ok = np.zeros(len(wi), np.int32)
+    _bsdf_decl().orc_refract(_p(wi), _p(nrm), len(wi), eta, _p(wt), _p(ok))
+    return wt, ok.astype(bool)

@@ -188,49 +188,30 @@ struct TR {
     float G1(const V3 &w) const { return 1 / (1 + lambda(w)); }
     float G(const V3 &wo, const V3 &wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
     float pdf(const V3 &wo, const V3 &wh) const { return D(wh) * G1(wo) * std::fabs(dot(wo, wh)) / abs_cos_theta(wo); }
-    static void sample11(float cos_t, float u1, float u2, float *sx, float *sy) {
-        if (cos_t > .9999) {
-            float r = std::sqrt(u1 / (1 - u1));
-            float phi = 6.28318530718f * u2;
-            *sx = r * std::cos(phi);
-            *sy = r * std::sin(phi);
-            return;
-        }
-        float sin_t = std::sqrt(std::max(0.f, 1.f - cos_t * cos_t));
-        float tan_t = sin_t / cos_t;
-        float a = 1 / tan_t;
-        float g1 = 2 / (1 + std::sqrt(1.f + 1.f / (a * a)));
-        float A = 2 * u1 / g1 - 1;
-        float tmp = 1.f / (A * A - 1.f);
-        if (tmp > 1e10) tmp = 1e10;
-        float B = tan_t;
-        float D = std::sqrt(std::max(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
-        float sx1 = B * tmp - D, sx2 = B * tmp + D;
-        *sx = (A < 0 || sx2 > 1.f / tan_t) ? sx1 : sx2;
-        float S;
-        if (u2 > 0.5f) {
-            S = 1.f;
-            u2 = 2.f * (u2 - .5f);
-        } else {
-            S = -1.f;
-            u2 = 2.f * (.5f - u2);
-        }
-        float z = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f)) /
-                  (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
-        *sy = S * z * std::sqrt(1.f + *sx * *sx);
-    }
+    // Visible-normal sample (divergence 32: the reference's sample_visible_area
+    // branch runs the classic sampler). The reference's visible sampler
+    // (trowbridge_reitz_distribution.rs:111-182, pbrt-v3's sample11) inverts
+    // the y-slope CDF with a rational fit that never reaches |slope| > 7.3 and
+    // misplaces tail mass while the estimator divides by the exact pdf; this
+    // samples the distribution of visible normals exactly (Heitz 2018,
+    // "Sampling the GGX Distribution of Visible Normals", JCGT 7(4)): stretch
+    // wo to the unit-roughness configuration, sample the projected area of the
+    // hemisphere as a disk warped towards the visible half, lift it onto the
+    // hemisphere and unstretch.
     V3 sample_wh(const V3 &wo, const float *u) const {
         bool flip = wo.z < 0;
-        V3 wi = flip ? -wo : wo;
-        V3 ws = normalize(V3(ax * wi.x, ay * wi.y, wi.z));
-        float sx, sy;
-        sample11(cos_theta(ws), u[0], u[1], &sx, &sy);
-        float tmp = cos_phi(ws) * sx - sin_phi(ws) * sy;
-        sy = sin_phi(ws) * sx + cos_phi(ws) * sy;
-        sx = tmp;
-        sx = ax * sx;
-        sy = ay * sy;
-        V3 wh = normalize(V3(-sx, -sy, 1.));
+        V3 w = flip ? -wo : wo;
+        V3 vh = normalize(V3(ax * w.x, ay * w.y, w.z));
+        float lensq = vh.x * vh.x + vh.y * vh.y;
+        V3 t1 = lensq > 0 ? V3(-vh.y, vh.x, 0) * (1 / std::sqrt(lensq)) : V3(1, 0, 0);
+        V3 t2 = cross(vh, t1);
+        float r = std::sqrt(u[0]);
+        float phi = 2 * Pi * u[1];
+        float p1 = r * std::cos(phi), p2 = r * std::sin(phi);
+        float s = 0.5f * (1 + vh.z);
+        p2 = (1 - s) * std::sqrt(std::max(0.f, 1 - p1 * p1)) + s * p2;
+        V3 nh = p1 * t1 + p2 * t2 + std::sqrt(std::max(0.f, 1 - p1 * p1 - p2 * p2)) * vh;
+        V3 wh = normalize(V3(ax * nh.x, ay * nh.y, std::max(0.f, nh.z)));
         return flip ? -wh : wh;
     }
 };
@@ -633,3 +614,114 @@ RGB path_li(const RenderScene &sc, Ray ray, Sampler &sampler, int max_depth, Ren
 }
 
 }  // namespace orc
+
+// ---------------------------------------------------------------- BSDF-level entry points
+// Local-frame (shading normal = +z) evaluations of the BxDFs above, one call
+// per batch of directions (3 floats each), for the closed-form and
+// statistical tests of the lobes. Material record: type, kd[3], eta[3], k[3],
+// alpha, ior (the layout of illume_material without its padding).
+namespace {
+struct MatIn {
+    int32_t type;
+    float kd[3], eta[3], k[3], alpha, ior;
+};
+orc::Material mat_of(const MatIn *in) {
+    orc::Material m;
+    m.type = in->type;
+    m.kd = orc::RGB(in->kd[0], in->kd[1], in->kd[2]);
+    m.eta = orc::RGB(in->eta[0], in->eta[1], in->eta[2]);
+    m.k = orc::RGB(in->k[0], in->k[1], in->k[2]);
+    m.alpha = in->alpha;
+    m.ior = in->ior;
+    return m;
+}
+orc::V3 v3(const float *p) { return orc::V3(p[0], p[1], p[2]); }
+void put(float *p, const orc::V3 &v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+orc::BSDF local_bsdf(const orc::Material *m) {
+    orc::BSDF b;
+    b.m = m;
+    if (m->type == orc::MAT_MATTE) b.type = orc::BSDF_REFLECTION | orc::BSDF_DIFFUSE;
+    else if (m->type == orc::MAT_METAL) b.type = orc::BSDF_REFLECTION | orc::BSDF_GLOSSY;
+    else b.type = orc::BSDF_REFLECTION | orc::BSDF_TRANSMISSION | orc::BSDF_SPECULAR;
+    return b;
+}
+}  // namespace
+
+extern "C" {
+
+// Trowbridge-Reitz (isotropic alpha): D(wh), Lambda(w), G1(w), and the
+// visible-normal pdf of wh given wo (D G1 |wo.wh| / |cos wo|).
+void orc_tr_eval(float alpha, const float *wo, const float *wh, uint32_t n, float *d, float *lambda_wo,
+                 float *g1_wo, float *pdf) {
+    orc::TR tr{alpha, alpha};
+    for (uint32_t i = 0; i < n; ++i) {
+        const orc::V3 o = v3(wo + 3 * i), h = v3(wh + 3 * i);
+        d[i] = tr.D(h);
+        lambda_wo[i] = tr.lambda(o);
+        g1_wo[i] = tr.G1(o);
+        pdf[i] = tr.pdf(o, h);
+    }
+}
+
+// TR height-correlated masking-shadowing G(wo, wi).
+void orc_tr_g(float alpha, const float *wo, const float *wi, uint32_t n, float *g) {
+    orc::TR tr{alpha, alpha};
+    for (uint32_t i = 0; i < n; ++i) g[i] = tr.G(v3(wo + 3 * i), v3(wi + 3 * i));
+}
+
+// Visible-normal sample of the TR distribution: wh for wo and u (2 floats each).
+void orc_tr_sample_wh(float alpha, const float *wo, const float *u, uint32_t n, float *wh) {
+    orc::TR tr{alpha, alpha};
+    for (uint32_t i = 0; i < n; ++i) put(wh + 3 * i, tr.sample_wh(v3(wo + 3 * i), u + 2 * i));
+}
+
+// BxDF::f (rgb), BxDF::pdf and BxDF::sample_f of one material in the local frame.
+void orc_bxdf_f(const void *mat, const float *wo, const float *wi, uint32_t n, float *rgb) {
+    const orc::Material m = mat_of((const MatIn *)mat);
+    const orc::BSDF b = local_bsdf(&m);
+    for (uint32_t i = 0; i < n; ++i) put(rgb + 3 * i, b.bxdf_f(v3(wo + 3 * i), v3(wi + 3 * i)));
+}
+void orc_bxdf_pdf(const void *mat, const float *wo, const float *wi, uint32_t n, float *pdf) {
+    const orc::Material m = mat_of((const MatIn *)mat);
+    const orc::BSDF b = local_bsdf(&m);
+    for (uint32_t i = 0; i < n; ++i) pdf[i] = b.bxdf_pdf(v3(wo + 3 * i), v3(wi + 3 * i));
+}
+// wi (3), f (3), pdf and the sampled lobe's flags per sample; pdf 0 = no sample.
+void orc_bxdf_sample_f(const void *mat, const float *wo, const float *u, uint32_t n, float *wi, float *rgb,
+                       float *pdf, int32_t *sampled) {
+    const orc::Material m = mat_of((const MatIn *)mat);
+    const orc::BSDF b = local_bsdf(&m);
+    for (uint32_t i = 0; i < n; ++i) {
+        orc::V3 w;
+        float p = 0.f;
+        int s = b.type;
+        const orc::RGB f = b.bxdf_sample_f(v3(wo + 3 * i), &w, u + 2 * i, &p, &s);
+        put(wi + 3 * i, w);
+        put(rgb + 3 * i, f);
+        pdf[i] = p;
+        sampled[i] = p > 0.f ? s : 0;
+    }
+}
+
+// Fresnel terms and refraction (reflection/mod.rs:420-475, 547-560).
+void orc_fr_dielectric(const float *cos_i, uint32_t n, float eta_i, float eta_t, float *out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = orc::fr_dielectric(cos_i[i], eta_i, eta_t);
+}
+void orc_fr_conductor(const float *cos_i, uint32_t n, const float *eta, const float *k, float *rgb) {
+    for (uint32_t i = 0; i < n; ++i)
+        put(rgb + 3 * i, orc::fr_conductor(cos_i[i], orc::RGB(eta[0], eta[1], eta[2]), orc::RGB(k[0], k[1], k[2])));
+}
+// wt and a success flag for each (wi, normal) pair at relative index eta = eta_i / eta_t.
+void orc_refract(const float *wi, const float *nrm, uint32_t n, float eta, float *wt, int32_t *ok) {
+    for (uint32_t i = 0; i < n; ++i) {
+        orc::V3 t;
+        ok[i] = orc::refract(v3(wi + 3 * i), v3(nrm + 3 * i), eta, &t) ? 1 : 0;
+        put(wt + 3 * i, ok[i] ? t : orc::V3());
+    }
+}
+
+}  // extern "C"

@@ -251,47 +251,27 @@ __device__ __forceinline__ float tr_lambda(float a, f3 w) {
 __device__ __forceinline__ float tr_pdf(float a, f3 wo, f3 wh) {
     return tr_D(a, wh) * (1 / (1 + tr_lambda(a, wo))) * fabsf(dot(wo, wh)) / abs_cos_theta(wo);
 }
+// Visible-normal sample of the distribution, exactly (Heitz 2018, "Sampling
+// the GGX Distribution of Visible Normals", JCGT 7(4)): wo stretched to the
+// unit-roughness configuration, a disk sample warped towards the visible half
+// of the projected hemisphere, lifted onto it and unstretched. (The
+// reference's sampler, trowbridge_reitz_distribution.rs:111-182, inverts the
+// y slope with a rational fit whose tails miss the distribution the pdf
+// describes.) The same operations in the same order as the oracle.
 __device__ __forceinline__ f3 tr_sample_wh(float a, f3 wo, float u1, float u2) {
     bool flip = wo.z < 0;
-    f3 wi = flip ? neg(wo) : wo;
-    f3 ws = normalize(mk(a * wi.x, a * wi.y, wi.z));
-    float cos_t = ws.z, sx, sy;
-    if (cos_t > .9999f) {
-        float r = sqrtf(u1 / (1 - u1));
-        float phi = 6.28318530718f * u2;
-        sx = r * cosf(phi);
-        sy = r * sinf(phi);
-    } else {
-        float sin_t = sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
-        float tan_t = sin_t / cos_t;
-        float aa = 1 / tan_t;
-        float g1 = 2 / (1 + sqrtf(1.f + 1.f / (aa * aa)));
-        float A = 2 * u1 / g1 - 1;
-        float tmp = 1.f / (A * A - 1.f);
-        if (tmp > 1e10f) tmp = 1e10f;
-        float B = tan_t;
-        float D = sqrtf(fmaxf(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
-        float sx1 = B * tmp - D, sx2 = B * tmp + D;
-        sx = (A < 0 || sx2 > 1.f / tan_t) ? sx1 : sx2;
-        float S;
-        if (u2 > 0.5f) {
-            S = 1.f;
-            u2 = 2.f * (u2 - .5f);
-        } else {
-            S = -1.f;
-            u2 = 2.f * (.5f - u2);
-        }
-        float z = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f)) /
-                  (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f);
-        sy = S * z * sqrtf(1.f + sx * sx);
-    }
-    float cp = cos_phi(ws), sp = sin_phi(ws);
-    float tmp = cp * sx - sp * sy;
-    sy = sp * sx + cp * sy;
-    sx = tmp;
-    sx = a * sx;
-    sy = a * sy;
-    f3 wh = normalize(mk(-sx, -sy, 1.f));
+    f3 w = flip ? neg(wo) : wo;
+    f3 vh = normalize(mk(a * w.x, a * w.y, w.z));
+    float lensq = vh.x * vh.x + vh.y * vh.y;
+    f3 t1 = lensq > 0 ? mul(mk(-vh.y, vh.x, 0.f), 1 / sqrtf(lensq)) : mk(1.f, 0.f, 0.f);
+    f3 t2 = cross(vh, t1);
+    float r = sqrtf(u1);
+    float phi = 2 * kPi * u2;
+    float p1 = r * cosf(phi), p2 = r * sinf(phi);
+    float s = 0.5f * (1 + vh.z);
+    p2 = (1 - s) * sqrtf(fmaxf(0.f, 1 - p1 * p1)) + s * p2;
+    f3 nh = add(add(smul(p1, t1), smul(p2, t2)), smul(sqrtf(fmaxf(0.f, 1 - p1 * p1 - p2 * p2)), vh));
+    f3 wh = normalize(mk(a * nh.x, a * nh.y, fmaxf(0.f, nh.z)));
     return flip ? neg(wh) : wh;
 }
 
