@@ -196,6 +196,21 @@ def bench_cornell(no_cpu):
     torch.cuda.synchronize()
     dts = (time.perf_counter() - t0) / reps
     out["sobol_sampler"] = {"samples_per_s": round(256 * 256 * spp / dts, 1), "ms_per_frame": round(dts * 1e3, 3)}
+    # the reference's AOIntegrator on the same frame: 16 cosine-distributed
+    # directions per camera sample
+    from rs_illume_amd.render import AOIntegrator
+    ao = AOIntegrator(cam, spp=spp, n_samples=16, cos_sample=True, seed=0)
+    acc = ao.render_sum(sc)
+    ao_counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ao.render_sum(sc, rgb_sum=acc, ray_counts=ao_counts)
+    torch.cuda.synchronize()
+    dta = (time.perf_counter() - t0) / reps
+    ao_rays = float(ao_counts.sum().item()) / reps
+    out["ao_integrator"] = {"samples_per_s": round(256 * 256 * spp / dta, 1), "ms_per_frame": round(dta * 1e3, 3),
+                            "mrays_per_s": round(ao_rays / dta / 1e6, 1), "ao_samples": 16}
     if not no_cpu:
         pyoracle = _oracle()
         osc = pyoracle.OracleRenderScene(desc.arrays())

@@ -271,6 +271,19 @@ int illume_scene_bvh(const illume_scene *scene, const illume_bvh **out);
 int illume_render(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                   float *rgb_sum, uint64_t *ray_counts, void *stream);
 
+/* Integrator::render with AOIntegrator::li (integrators/ao.rs:16-112; the
+ * reference's only implemented Li): for every pixel adds, per camera sample,
+ * the ambient occlusion at the camera ray's closest hit to all three channels
+ * of rgb_sum[3*(y*width+x) + c] (device floats the caller zeroes). Each of
+ * n_ao_samples hemisphere directions about the geometric normal facing the
+ * ray (cosine-distributed if cos_sample != 0, else uniform; two sampler
+ * dimensions each, after the camera sample) adds (wi . n) / (pdf n_ao_samples)
+ * when its shadow ray is unoccluded: an unoccluded point gives pi, a miss 0.
+ * cfg->max_depth is unused; cfg->flags selects the sampler as in
+ * illume_render. n_ao_samples <= 65536. ray_counts as in illume_render. */
+int illume_render_ao(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                     uint32_t n_ao_samples, int32_t cos_sample, float *rgb_sum, uint64_t *ray_counts, void *stream);
+
 /* ------------------------------------------------------------------ reconstruction filters
  * The Filter trait (core/filter.rs) and its implementations: BoxFilter
  * (filters/box_filter.rs), TriangleFilter (triangle.rs), GaussianFilter

@@ -207,6 +207,8 @@ def _render_decl():
     L.orc_sobol_value.argtypes = [ctypes.c_int64, i32]
     L.orc_sobol_value.restype = ctypes.c_float
     L.orc_render_filtered.restype = i32
+    L.orc_render_ao.argtypes = [vp, vp, u32, u32, i32, u64, u32, u32, vp, vp, i32]
+    L.orc_render_ao.restype = None
     for fn, at in (("orc_filter_table", [vp, vp]), ("orc_filter_eval", [vp, vp, u32, vp]),
                    ("orc_film_sample_bounds", [vp, i32, i32, vp]),
                    ("orc_film_add_samples", [vp, vp, u32, i32, i32, vp])):
@@ -244,6 +246,19 @@ class OracleRenderScene:
         st = np.zeros(2, np.uint64)
         _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _sampler_flags(sampler),
                                   _p(out), _p(st), n_threads or n_threads_default())
+        return out, st
+
+    def render_ao(self, camera, spp, n_samples, cos_sample=True, seed=0, sample_offset=0, n_threads=None,
+                  sampler="random"):
+        """AOIntegrator (integrators/ao.rs): per-pixel sums (H, W, 3) of the
+        ambient occlusion of spp camera samples, n_samples hemisphere
+        directions each, and (closest, any) ray counts."""
+        cam = np.ascontiguousarray(camera).reshape(1)
+        w, h = int(cam["width"][0]), int(cam["height"][0])
+        out = np.zeros((h, w, 3), np.float32)
+        st = np.zeros(2, np.uint64)
+        _render_decl().orc_render_ao(self.h, _p(cam), spp, n_samples, 1 if cos_sample else 0, seed, sample_offset,
+                                     _sampler_flags(sampler), _p(out), _p(st), n_threads or n_threads_default())
         return out, st
 
     def render_filtered(self, camera, filter_desc, spp, max_depth, seed=0, sample_offset=0, n_threads=None,

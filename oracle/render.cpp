@@ -90,6 +90,14 @@ static V3 cosine_sample_hemisphere(const float *u) {
     float z = std::sqrt(std::max(0.f, 1 - dx * dx - dy * dy));
     return V3(dx, dy, z);
 }
+// sampling.rs:168-181 with pbrt-v3's y = r sin(phi) (divergence 22) and
+// pdf 1 / (2 pi) (divergence 23)
+static V3 uniform_sample_hemisphere(const float *u) {
+    float z = u[0];
+    float r = std::sqrt(std::max(0.f, 1 - z * z));
+    float phi = 2 * Pi * u[1];
+    return V3(r * std::cos(phi), r * std::sin(phi), z);
+}
 // sampling.rs:333-337
 static float power_heuristic(int nf, float f_pdf, int ng, float g_pdf) {
     float f = nf * f_pdf, g = ng * g_pdf;
@@ -608,6 +616,42 @@ RGB path_li(const RenderScene &sc, Ray ray, Sampler &sampler, int max_depth, Ren
             float q = std::max(0.05f, 1 - max_comp(rr_beta));
             if (sampler.get_1d() < q) break;
             beta = beta / (1 - q);
+        }
+    }
+    return L;
+}
+
+// AOIntegrator::li (integrators/ao.rs:45-93): at the camera ray's closest
+// hit, n = the geometric normal facing the ray's origin, s = normalize(dpdu),
+// t = n_geometric x s; each of n_samples hemisphere directions (cosine- or
+// uniform-distributed, 2 sampler dimensions each, in order) adds
+// (wi . n) / (pdf * n_samples) when its shadow ray is unoccluded. Every
+// primitive here has a material, so the skip of BSDF-less surfaces (58-62)
+// never applies.
+RGB ao_li(const RenderScene &sc, Ray ray, Sampler &sampler, int n_samples, bool cos_sample, RenderStats *st) {
+    SurfaceInteraction isect;
+    if (!scene_intersect(sc, ray, &isect, st)) return RGB();
+    const V3 n = faceforward(isect.n, -ray.d);
+    const V3 s = normalize(isect.dpdu);
+    const V3 t = cross(isect.n, s);
+    RGB L;
+    for (int i = 0; i < n_samples; ++i) {
+        float u[2];
+        sampler.get_2d(u);
+        V3 wi;
+        float pdf;
+        if (cos_sample) {
+            wi = cosine_sample_hemisphere(u);
+            pdf = std::fabs(wi.z) * InvPi;  // cosine_hemisphere_pdf
+        } else {
+            wi = uniform_sample_hemisphere(u);
+            pdf = Inv2Pi;  // uniform_hemisphere_pdf
+        }
+        wi = V3(s.x * wi.x + t.x * wi.y + n.x * wi.z, s.y * wi.x + t.y * wi.y + n.y * wi.z,
+                s.z * wi.x + t.z * wi.y + n.z * wi.z);
+        if (!scene_intersect_p(sc, isect.spawn_ray(wi), st)) {
+            const float v = dot(wi, n) / (pdf * (float)n_samples);
+            L += RGB(v, v, v);
         }
     }
     return L;

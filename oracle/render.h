@@ -118,5 +118,6 @@ struct RenderStats {
 
 // PathIntegrator::li (PBR book 3e 14.5.4)
 RGB path_li(const RenderScene &sc, Ray ray, Sampler &sampler, int max_depth, RenderStats *st);
+RGB ao_li(const RenderScene &sc, Ray ray, Sampler &sampler, int n_samples, bool cos_sample, RenderStats *st);
 
 }  // namespace orc

@@ -22,6 +22,72 @@ struct OrcCamera {
 };
 }  // namespace
 
+namespace {
+// The pixel / sample loop of orc_render and orc_render_ao; li(ray, sampler,
+// stats) is the integrator's Li.
+template <class Li>
+void render_loop(const void *camera, uint32_t spp, uint64_t seed, uint32_t sample_offset, uint32_t flags,
+                 float *rgb_sum, uint64_t *stats, int n_threads, const Li &li) {
+    const OrcCamera *c = (const OrcCamera *)camera;
+    Camera cam;
+    cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
+             V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
+    uint64_t n_pix = (uint64_t)c->width * c->height;
+    HaltonPixelMap hmap;
+    hmap.init((int)c->width, (int)c->height);
+    SobolPixelMap smap;
+    smap.init(0, 0, (int)c->width, (int)c->height);
+    const bool halton = (flags & 1u) != 0, sobol = (flags & 2u) != 0;
+    int nt = std::max(1, n_threads);
+    std::vector<RenderStats> st(nt);
+    std::atomic<uint64_t> next(0);
+    auto work = [&](int t) {
+        while (true) {
+            uint64_t b = next.fetch_add(64);
+            if (b >= n_pix) break;
+            uint64_t e = std::min(n_pix, b + 64);
+            for (uint64_t pix = b; pix < e; ++pix) {
+                int x = (int)(pix % c->width), y = (int)(pix / c->width);
+                RGB sum;
+                for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
+                    Sampler sampler;
+                    if (halton) sampler.start_halton(&hmap, x, y, si);
+                    else if (sobol) sampler.start_sobol(&smap, x, y, si);
+                    else sampler.start(seed, pix, si);
+                    // sampler.rs:78-84 get_camera_sample: p_film, time, p_lens
+                    float uf[2], ul[2];
+                    sampler.get_2d(uf);
+                    float time = sampler.get_1d();
+                    sampler.get_2d(ul);
+                    Ray ray = cam.generate_ray((float)x + uf[0], (float)y + uf[1], time);
+                    RGB L = li(ray, sampler, &st[t]);
+                    if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = RGB();
+                    else if (luminance(L) < -1e-5f) L = RGB();
+                    else if (std::isinf(luminance(L))) L = RGB();
+                    sum += L;
+                }
+                rgb_sum[3 * pix] += sum.x;
+                rgb_sum[3 * pix + 1] += sum.y;
+                rgb_sum[3 * pix + 2] += sum.z;
+            }
+        }
+    };
+    if (nt == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    if (stats) {
+        stats[0] = stats[1] = 0;
+        for (auto &x : st) {
+            stats[0] += x.closest_rays;
+            stats[1] += x.any_rays;
+        }
+    }
+}
+}  // namespace
+
 extern "C" {
 
 // prim_emission: 3 floats per primitive (zero = not emissive; only triangles
@@ -110,63 +176,21 @@ void orc_render_scene_destroy(void *h) { delete (RenderScene *)h; }
 void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t sample_offset,
                 uint32_t flags, float *rgb_sum, uint64_t *stats, int n_threads) {
     RenderScene *s = (RenderScene *)h;
-    const OrcCamera *c = (const OrcCamera *)camera;
-    Camera cam;
-    cam.init(V3(c->pos[0], c->pos[1], c->pos[2]), V3(c->look[0], c->look[1], c->look[2]),
-             V3(c->up[0], c->up[1], c->up[2]), c->fov, (int)c->width, (int)c->height);
-    uint64_t n_pix = (uint64_t)c->width * c->height;
-    HaltonPixelMap hmap;
-    hmap.init((int)c->width, (int)c->height);
-    SobolPixelMap smap;
-    smap.init(0, 0, (int)c->width, (int)c->height);
-    const bool halton = (flags & 1u) != 0, sobol = (flags & 2u) != 0;
-    int nt = std::max(1, n_threads);
-    std::vector<RenderStats> st(nt);
-    std::atomic<uint64_t> next(0);
-    auto work = [&](int t) {
-        while (true) {
-            uint64_t b = next.fetch_add(64);
-            if (b >= n_pix) break;
-            uint64_t e = std::min(n_pix, b + 64);
-            for (uint64_t pix = b; pix < e; ++pix) {
-                int x = (int)(pix % c->width), y = (int)(pix / c->width);
-                RGB sum;
-                for (uint32_t si = sample_offset; si < sample_offset + spp; ++si) {
-                    Sampler sampler;
-                    if (halton) sampler.start_halton(&hmap, x, y, si);
-                    else if (sobol) sampler.start_sobol(&smap, x, y, si);
-                    else sampler.start(seed, pix, si);
-                    // sampler.rs:78-84 get_camera_sample: p_film, time, p_lens
-                    float uf[2], ul[2];
-                    sampler.get_2d(uf);
-                    float time = sampler.get_1d();
-                    sampler.get_2d(ul);
-                    Ray ray = cam.generate_ray((float)x + uf[0], (float)y + uf[1], time);
-                    RGB L = path_li(*s, ray, sampler, (int)max_depth, &st[t]);
-                    if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = RGB();
-                    else if (luminance(L) < -1e-5f) L = RGB();
-                    else if (std::isinf(luminance(L))) L = RGB();
-                    sum += L;
-                }
-                rgb_sum[3 * pix] += sum.x;
-                rgb_sum[3 * pix + 1] += sum.y;
-                rgb_sum[3 * pix + 2] += sum.z;
-            }
-        }
-    };
-    if (nt == 1) work(0);
-    else {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
-        for (auto &x : th) x.join();
-    }
-    if (stats) {
-        stats[0] = stats[1] = 0;
-        for (auto &x : st) {
-            stats[0] += x.closest_rays;
-            stats[1] += x.any_rays;
-        }
-    }
+    render_loop(camera, spp, seed, sample_offset, flags, rgb_sum, stats, n_threads,
+                [&](const Ray &ray, Sampler &sampler, RenderStats *st) {
+                    return path_li(*s, ray, sampler, (int)max_depth, st);
+                });
+}
+
+// SamplerIntegratorBase::render with AOIntegrator::li (integrators/ao.rs:16-112):
+// n_ao hemisphere samples per camera sample, cosine-distributed if cos_sample.
+void orc_render_ao(void *h, const void *camera, uint32_t spp, uint32_t n_ao, int cos_sample, uint64_t seed,
+                   uint32_t sample_offset, uint32_t flags, float *rgb_sum, uint64_t *stats, int n_threads) {
+    RenderScene *s = (RenderScene *)h;
+    render_loop(camera, spp, seed, sample_offset, flags, rgb_sum, stats, n_threads,
+                [&](const Ray &ray, Sampler &sampler, RenderStats *st) {
+                    return ao_li(*s, ray, sampler, (int)n_ao, cos_sample != 0, st);
+                });
 }
 
 // SamplerIntegratorBase::render through a reconstruction filter: camera

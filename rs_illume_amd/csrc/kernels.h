@@ -15,6 +15,12 @@ hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp,
                          uint32_t sample_offset, float *rgb, unsigned long long *counters, const SamplerLaunch &hl,
                          hipStream_t stream);
 
+// Ambient occlusion (AOIntegrator::li): adds the AO sums of spp camera
+// samples per pixel, n_ao hemisphere directions each, to rgb (grey).
+hipError_t launch_render_ao(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t n_ao, bool cos_sample,
+                            uint64_t seed, uint32_t sample_offset, float *rgb, unsigned long long *counters,
+                            const SamplerLaunch &hl, hipStream_t stream);
+
 // Path tracer through a reconstruction filter: samples over fs's sample
 // bounds, splatted into rgbw (4 floats per image pixel).
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,

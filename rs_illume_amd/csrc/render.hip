@@ -88,6 +88,77 @@ __global__ __launch_bounds__(BLOCK, 3) void render_kernel(DevScene sc, DevCamera
     }
 }
 
+// SamplerIntegratorBase::render with AOIntegrator::li (integrators/ao.rs:45-93):
+// one thread per camera sample. At the closest hit the frame is n = the
+// geometric normal facing the ray's origin, s = normalize(dpdu), t = n_g x s;
+// each of n_ao hemisphere directions (cosine or uniform, two sampler
+// dimensions each after the camera sample) adds (wi . n) / (pdf n_ao) when its
+// shadow ray (an any-hit query, t_max infinite) is unoccluded. The grey value
+// goes to all three channels of the pixel's sum.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 4) void render_ao_kernel(DevScene sc, DevCamera cam, uint32_t spp, uint32_t n_ao,
+                                                           bool cos_sample, uint64_t seed, uint32_t sample_offset,
+                                                           float *__restrict__ film,
+                                                           unsigned long long *__restrict__ counters,
+                                                           SamplerLaunch hl) {
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    __shared__ uint32_t blk_counts[2];
+    if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t n_pix = (uint64_t)cam.width * cam.height;
+    const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    PathCounters pc{0, 0};
+    if (gid < n_pix * spp) {
+        const uint64_t pix = gid % n_pix;
+        const uint32_t s = (uint32_t)(gid / n_pix) + sample_offset;
+        int spill[kSpill];
+        TraceStack<BLOCK> st;
+        st.reset(lds_stack + threadIdx.x, spill);
+        PathState ps;
+        float fx, fy;
+        path_start_at(cam, seed, pix, (int)(pix % cam.width), (int)(pix / cam.width), s, ps, fx, fy, hl);
+        TraceHit h;
+        float ao = 0.f;
+        if (scene_closest<BLOCK>(sc, ps.o, ps.d, ps.t_max, h, st, pc)) {
+            Isect it;
+            hit_isect(sc, h, ps.o, ps.d, it);
+            const f3 n = faceforward(it.n, neg(ps.d));
+            const f3 sv = normalize(it.dpdu);
+            const f3 tv = cross(it.n, sv);
+            for (uint32_t i = 0; i < n_ao; ++i) {
+                const float u0 = ps.rng.get_1d(), u1 = ps.rng.get_1d();
+                f3 w;
+                float pdf;
+                if (cos_sample) {
+                    w = cosine_sample_hemisphere(u0, u1);
+                    pdf = fabsf(w.z) * kInvPi;
+                } else {
+                    w = uniform_sample_hemisphere(u0, u1);
+                    pdf = kInv2Pi;
+                }
+                const f3 wi = mk(sv.x * w.x + tv.x * w.y + n.x * w.z, sv.y * w.x + tv.y * w.y + n.y * w.z,
+                                 sv.z * w.x + tv.z * w.y + n.z * w.z);
+                f3 so;
+                spawn_ray(it, wi, so);
+                if (!scene_any<BLOCK>(sc, so, wi, __builtin_inff(), st, pc)) ao += dot(wi, n) / (pdf * (float)n_ao);
+            }
+        }
+        const f3 L = guard_radiance(mk(ao, ao, ao));
+        atomicAdd(film + 3 * pix, L.x);
+        atomicAdd(film + 3 * pix + 1, L.y);
+        atomicAdd(film + 3 * pix + 2, L.z);
+    }
+    if (counters) {
+        atomicAdd(&blk_counts[0], pc.closest);
+        atomicAdd(&blk_counts[1], pc.any);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(counters, (unsigned long long)blk_counts[0]);
+            atomicAdd(counters + 1, (unsigned long long)blk_counts[1]);
+        }
+    }
+}
+
 // Batched megakernel: one thread per path of the whole batch; lanes find
 // their job (scene, camera, film offset, sampler) by binary search over the
 // jobs' first path numbers, so one launch renders images of any sizes.
@@ -222,6 +293,18 @@ hipError_t launch_render(const DevScene &sc, const DevCamera &cam, uint32_t spp,
     if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
     render_kernel<kRenderBlock, false><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(
         sc, cam, spp, max_depth, seed, sample_offset, rgb, counters, FilmSplat{}, hl);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_ao(const DevScene &sc, const DevCamera &cam, uint32_t spp, uint32_t n_ao, bool cos_sample,
+                            uint64_t seed, uint32_t sample_offset, float *rgb, unsigned long long *counters,
+                            const SamplerLaunch &hl, hipStream_t stream) {
+    const uint64_t n = (uint64_t)cam.width * cam.height * spp;
+    if (n == 0) return hipSuccess;
+    const uint64_t grid = (n + kRenderBlock - 1) / kRenderBlock;
+    if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
+    render_ao_kernel<kRenderBlock><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(sc, cam, spp, n_ao, cos_sample, seed,
+                                                                                sample_offset, rgb, counters, hl);
     return hipGetLastError();
 }
 

@@ -441,6 +441,30 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
     return ILLUME_OK;
 }
 
+int illume_render_ao(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
+                     uint32_t n_ao_samples, int32_t cos_sample, float *rgb_sum, uint64_t *ray_counts, void *stream) {
+    if (!scene || !camera || !cfg || !rgb_sum) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    DevCamera cam;
+    if (!make_camera(*camera, &cam)) return abi_fail(ILLUME_ERR_INVALID_ARG, "invalid camera");
+    if (cfg->spp == 0) return ILLUME_OK;
+    if ((uint64_t)cfg->sample_offset + cfg->spp > (1u << 24))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+    if ((uint64_t)camera->width * camera->height >= (1ull << 40))
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "image too large");
+    if (n_ao_samples > (1u << 16)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "more than 65536 AO samples");
+    const uint32_t kind = sampler_kind(cfg->flags);
+    if (kind == ~0u) return abi_fail(ILLUME_ERR_INVALID_ARG, "unknown render flags");
+    if (scene->n_prims == 0 || n_ao_samples == 0) return ILLUME_OK;  // nothing hit, or no directions: 0
+    SamplerLaunch hl{};
+    if (kind && !sampler_launch(kind, 0, 0, (int)camera->width, (int)camera->height, &hl))
+        return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
+    hipError_t e = launch_render_ao(scene->dev, cam, cfg->spp, n_ao_samples, cos_sample != 0, cfg->seed,
+                                    cfg->sample_offset, rgb_sum, (unsigned long long *)ray_counts, hl,
+                                    (hipStream_t)stream);
+    if (e != hipSuccess) return abi_hip_fail(e, "AO render launch");
+    return ILLUME_OK;
+}
+
 int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream) {
     if (!film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     std::vector<RenderJob> hj;

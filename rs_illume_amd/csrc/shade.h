@@ -16,6 +16,7 @@ namespace illume {
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kInv2Pi = 0.15915494309189533577f;  // 1 / (2 pi) (pbrt.rs:35 has 2 / pi)
 constexpr float kOneMinusEpsilon = 0x1.fffffep-1f;
 
 // ---------------------------------------------------------------- sampler
@@ -171,6 +172,15 @@ __device__ __forceinline__ f3 cosine_sample_hemisphere(float u0, float u1) {
     }
     float z = sqrtf(fmaxf(0.f, 1 - dx * dx - dy * dy));
     return mk(dx, dy, z);
+}
+
+// sampling.rs:168-181 with y = r sin(phi) (the reference writes r cos(phi));
+// pdf 1 / (2 pi)
+__device__ __forceinline__ f3 uniform_sample_hemisphere(float u0, float u1) {
+    float z = u0;
+    float r = sqrtf(fmaxf(0.f, 1 - z * z));
+    float phi = 2 * kPi * u1;
+    return mk(r * cosf(phi), r * sinf(phi), z);
 }
 // sampling.rs:333-337
 __device__ __forceinline__ float power_heuristic(float f_pdf, float g_pdf) {

@@ -45,6 +45,8 @@ def _declare():
     L.illume_scene_bvh.argtypes = [vp, ctypes.POINTER(vp)]
     L.illume_render.restype = i32
     L.illume_render.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), vp, vp, vp]
+    L.illume_render_ao.restype = i32
+    L.illume_render_ao.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), u32, ctypes.c_int32, vp, vp, vp]
     L._render_declared = True
     return L
 
@@ -156,6 +158,30 @@ class PathIntegrator:
         """The image (H, W, 3) float32 device tensor: mean radiance per pixel."""
         s = self.render_sum(scene, stream=stream)
         return s / float(self.spp)
+
+
+class AOIntegrator(PathIntegrator):
+    """AOIntegrator (integrators/ao.rs:16-112): ``AOIntegrator::new(cos_sample,
+    ns, camera, sampler, pixel_bounds)`` then ``render``. Each camera sample's
+    value is the ambient occlusion at its closest hit, estimated with
+    ``n_samples`` hemisphere directions (cosine-distributed if ``cos_sample``):
+    pi for an unoccluded point, 0 for a miss; grey in all three channels."""
+
+    def __init__(self, camera, spp=16, n_samples=64, cos_sample=True, seed=0, sampler="random"):
+        super().__init__(camera, spp, 0, seed, sampler)
+        self.n_samples = int(n_samples)
+        self.cos_sample = bool(cos_sample)
+
+    def render_sum(self, scene, rgb_sum=None, spp=None, sample_offset=0, ray_counts=None, stream=None):
+        w, h = self.resolution
+        if rgb_sum is None:
+            rgb_sum = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
+        cfg = RenderCfg(self.spp if spp is None else int(spp), 0, int(sample_offset), self.flags, self.seed)
+        rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+        _lib.check(_declare().illume_render_ao(scene.handle, _p(self.camera), ctypes.byref(cfg), self.n_samples,
+                                               1 if self.cos_sample else 0, ctypes.c_void_p(rgb_sum.data_ptr()), rc,
+                                               _stream_handle(stream)), "illume_render_ao")
+        return rgb_sum
 
 
 class GuideCfg(ctypes.Structure):
