@@ -231,7 +231,7 @@ def bench_cornell(no_cpu):
 
 
 def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32, plain="megakernel", no_cpu=False,
-                 cpu_budget_s=20.0):
+                 cpu_budget_s=20.0, guide_frac=0.25, product=True):
     """Config-3 equal-quality comparison (rs_illume_amd.converge); the CPU
     leg runs the oracle's path tracer on host cores in 2-spp passes for a fixed
     wall budget against the same reference (time to relMSE 0.01 and the relMSE
@@ -254,17 +254,21 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
     ref /= ref_spp
     log("reference %d spp in %.1f s" % (ref_spp, ref_s))
     r = C.guided_vs_plain(sc, cam, ref, target=0.01, spp_per_pass=4, max_spp=4096, max_depth=8, seed=1,
-                          train_steps=train_steps, warmup_steps=warmup_steps, plain=plain)
+                          train_steps=train_steps, warmup_steps=warmup_steps, plain=plain, guide_frac=guide_frac,
+                          product=product)
     out = {"workload": "glossy small-light Cornell box %dx%d, max depth 8, TR copper walls alpha~U[0.05,0.5], "
                        "light 1%% of the ceiling; relMSE vs %d-spp plain render" % (res, res, ref_spp),
            "target_rel_mse": 0.01, "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
            "spp_per_pass": r["spp_per_pass"], "plain_renderer": r["plain_renderer"], "train_steps": train_steps,
-           "warmup_steps": warmup_steps}
+           "warmup_steps": warmup_steps, "guide_frac": guide_frac, "product_with_bsdf_lobe": product}
     for k in ("plain", "guided"):
         rr = r[k]["reached"]
         out[k] = {"seconds": round(rr["seconds"], 5), "spp": rr["spp"], "rel_mse": round(rr["rel_mse"], 5)} \
             if rr else {"seconds": None, "spp": r[k]["spp"], "rel_mse": round(r[k]["curve"][-1][2], 5)}
     out["time_ratio_guided_over_plain"] = round(r["time_ratio"], 3) if "time_ratio" in r else None
+    # the measured bound: a proposal trained long, then frozen, at equal spp
+    log("guided: frozen-proposal ceiling")
+    out["frozen_ceiling"] = C.frozen_ceiling(sc, cam, ref, guide_frac=guide_frac, product=product)
     if not no_cpu:
         log("guided: CPU leg (oracle, %.0f s budget)" % cpu_budget_s)
         osc = _oracle().OracleRenderScene(desc.arrays())
@@ -374,7 +378,7 @@ def bench_batched(no_cpu, log=log, builder="SAH"):
 
 
 def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
-                   builder="SAH", no_cpu=False, cpu_budget_s=20.0):
+                   builder="SAH", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
     Plain (megakernel batch) and guided (wavefront pool; one proposal network
@@ -387,7 +391,7 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
     import torch
     from rs_illume_amd import scenes as S
     from rs_illume_amd.guide import Guide
-    from rs_illume_amd.render import GuideCfg, Scene, Wavefront, render_batch
+    from rs_illume_amd.render import GUIDE_PRODUCT, GuideCfg, Scene, Wavefront, render_batch
     t0 = time.time()
     desc = S.terrain_scene(0)
     t1 = time.time()
@@ -416,7 +420,7 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
 
     def run(guide):
         film = torch.zeros_like(ref)
-        gc = GuideCfg(0.5, train_steps, 65536, 0.0) if guide is not None else None
+        gc = GuideCfg(guide_frac, train_steps, 65536, 0.0, GUIDE_PRODUCT) if guide is not None else None
         warm = guide is not None and warmup_steps > 0
         reached = [None] * len(cams)
         elapsed, spp = 0.0, 0
@@ -437,7 +441,7 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
             if warm and guide.stats()[0] >= warmup_steps:
                 warm = False
             if guide is not None and not warm:
-                gc.guide_frac = 0.5
+                gc.guide_frac = guide_frac
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t1
             spp += spp_pass
@@ -478,6 +482,7 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
             "plain_renderer": "megakernel_batch", "guided_renderer": "wavefront",
             "reference_s": round(ref_s, 2),
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
+            "guide_frac": guide_frac, "product_with_bsdf_lobe": True,
             "plain": plain, "guided": guided,
             "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both
             else None, "cpu_baseline": cpu}

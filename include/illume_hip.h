@@ -377,7 +377,16 @@ typedef struct {
     float min_alpha;       /* the proposal is used (and trained) only at diffuse vertices and at
                               microfacet vertices with alpha >= min_alpha; near-specular lobes
                               keep plain BSDF sampling. 0 = every non-specular vertex */
+    uint32_t flags;        /* ILLUME_GUIDE_PRODUCT or 0 */
 } illume_guide_cfg;
+
+/* illume_guide_cfg.flags: sample the proposal multiplied by a von Mises-Fisher
+ * fit of the vertex's BSDF lobe (about the mirror direction for microfacet
+ * reflection, about the normal for Lambertian) instead of the proposal alone.
+ * The product of two vMF lobes is a vMF lobe, so the product mixture is
+ * sampled and its pdf evaluated exactly; the one-sample MIS with the BSDF
+ * stays unbiased. */
+#define ILLUME_GUIDE_PRODUCT 1u
 
 /* One render of a batch: its image occupies width*height*3 floats of the
  * film buffer starting at film_offset. */

@@ -138,6 +138,38 @@ T mix_pdf(const Mix<T> &m, const float *w) {
     return q;
 }
 
+// The mixture times a vMF lobe (mu_b, k_b) fitted to a vertex's BSDF: lobe j
+// becomes the vMF with k mu = k_j mu_j + k_b mu_b, its weight scaled by the
+// integral of the two densities' product, c(k_j) c(k_b) exp(k - k_j - k_b) /
+// c(k) with c(k) = k / (2 pi (1 - e^{-2k})), then renormalised. False if the
+// weights degenerate.
+template <typename T>
+T vmf_norm(T k) {
+    return k / ((T)(2 * 3.14159265358979323846) * (-std::expm1(-2 * k)));
+}
+template <typename T>
+bool product(Mix<T> &m, const float *mu_b, T k_b) {
+    T s = 0;
+    const T cb = vmf_norm(k_b);
+    for (int j = 0; j < kLobes; ++j) {
+        const T kj = m.kappa[j];
+        const T vx = kj * m.mu[j][0] + k_b * (T)mu_b[0], vy = kj * m.mu[j][1] + k_b * (T)mu_b[1],
+                vz = kj * m.mu[j][2] + k_b * (T)mu_b[2];
+        const T k = std::max(std::sqrt(vx * vx + vy * vy + vz * vz), (T)1e-4);
+        const T scale = vmf_norm(kj) * cb * std::exp(std::min(k - kj - k_b, (T)0)) / vmf_norm(k);
+        m.mu[j][0] = vx / k;
+        m.mu[j][1] = vy / k;
+        m.mu[j][2] = vz / k;
+        m.kappa[j] = k;
+        m.pi[j] *= scale;
+        s += m.pi[j];
+    }
+    if (!(s > 0) || !std::isfinite(s)) return false;
+    const T inv = 1 / s;
+    for (int j = 0; j < kLobes; ++j) m.pi[j] *= inv;
+    return true;
+}
+
 // d(-log q)/d out, for one direction w.
 template <typename T>
 void nll_grad(const T *out, const Mix<T> &m, const float *w, T *g) {
@@ -276,11 +308,44 @@ void orc_guide_pdf(const float *outs, int n, const float *wi, float *pdf) {
     }
 }
 
+// Pdf of the mixture times the vMF lobe (lobe_mu[3r..], lobe_k[r]) per row;
+// NaN where the product degenerates.
+void orc_guide_product_pdf(const float *outs, int n, const float *lobe_mu, const float *lobe_k, const float *wi,
+                           float *pdf) {
+    for (int r = 0; r < n; ++r) {
+        Mix<double> m;
+        double o[kOut];
+        for (int k = 0; k < kOut; ++k) o[k] = outs[kOut * r + k];
+        decode<double>(o, m);
+        pdf[r] = product<double>(m, lobe_mu + 3 * r, (double)lobe_k[r]) ? (float)mix_pdf(m, wi + 3 * r) : NAN;
+    }
+}
+
+static void sample_rows(const float *outs, int n, const float *u, float *wi, float *pdf, const float *lobe_mu,
+                        const float *lobe_k);
+
 // Sample the mixture: u = (lobe, u1, u2) per row -> direction and its mixture pdf.
 void orc_guide_sample(const float *outs, int n, const float *u, float *wi, float *pdf) {
+    sample_rows(outs, n, u, wi, pdf, nullptr, nullptr);
+}
+// Sample the mixture times the vMF lobe of each row, with the product's pdf.
+void orc_guide_product_sample(const float *outs, int n, const float *lobe_mu, const float *lobe_k, const float *u,
+                              float *wi, float *pdf) {
+    sample_rows(outs, n, u, wi, pdf, lobe_mu, lobe_k);
+}
+
+}  // extern "C"
+
+static void sample_rows(const float *outs, int n, const float *u, float *wi, float *pdf, const float *lobe_mu,
+                        const float *lobe_k) {
     for (int r = 0; r < n; ++r) {
         Mix<float> m;
         decode<float>(outs + kOut * r, m);
+        if (lobe_mu && !product<float>(m, lobe_mu + 3 * r, lobe_k[r])) {
+            wi[3 * r] = wi[3 * r + 1] = wi[3 * r + 2] = 0.f;
+            pdf[r] = NAN;
+            continue;
+        }
         float ul = u[3 * r], u1 = u[3 * r + 1], u2 = u[3 * r + 2];
         int j = 0;
         float c = 0;
@@ -301,6 +366,8 @@ void orc_guide_sample(const float *outs, int n, const float *u, float *wi, float
         pdf[r] = (float)mix_pdf(m, wi + 3 * r);
     }
 }
+
+extern "C" {
 
 // Adam (Kingma & Ba 2015), bias-corrected, step t >= 1.
 void orc_adam(float *p, float *m, float *v, const float *g, int n, int t, float lr, float b1, float b2, float eps) {

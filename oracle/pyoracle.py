@@ -409,6 +409,8 @@ def _guide_decl():
     L.orc_guide_loss_f64.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.orc_guide_pdf.argtypes = [vp, i32, vp, vp]
     L.orc_guide_sample.argtypes = [vp, i32, vp, vp, vp]
+    L.orc_guide_product_pdf.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.orc_guide_product_sample.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.orc_adam.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
     L._gdecl = True
     return L
@@ -460,6 +462,25 @@ def guide_sample(outs, u):
     wi = np.zeros((len(outs), 3), np.float32)
     pdf = np.zeros(len(outs), np.float32)
     _guide_decl().orc_guide_sample(_p(outs), len(outs), _p(u), _p(wi), _p(pdf))
+    return wi, pdf
+
+
+def guide_product_pdf(outs, lobe_mu, lobe_k, wi):
+    """Pdf of each row's mixture times the vMF lobe (lobe_mu, lobe_k)."""
+    outs, wi = _f32(outs).reshape(-1, GUIDE_OUT), _f32(wi).reshape(-1, 3)
+    mu, k = _f32(lobe_mu).reshape(-1, 3), _f32(lobe_k).reshape(-1)
+    pdf = np.zeros(len(outs), np.float32)
+    L = _guide_decl()
+    L.orc_guide_product_pdf(_p(outs), len(outs), _p(mu), _p(k), _p(wi), _p(pdf))
+    return pdf
+
+
+def guide_product_sample(outs, lobe_mu, lobe_k, u):
+    outs, u = _f32(outs).reshape(-1, GUIDE_OUT), _f32(u).reshape(-1, 3)
+    mu, k = _f32(lobe_mu).reshape(-1, 3), _f32(lobe_k).reshape(-1)
+    wi = np.zeros((len(outs), 3), np.float32)
+    pdf = np.zeros(len(outs), np.float32)
+    _guide_decl().orc_guide_product_sample(_p(outs), len(outs), _p(mu), _p(k), _p(u), _p(wi), _p(pdf))
     return wi, pdf
 
 

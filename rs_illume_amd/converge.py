@@ -49,12 +49,13 @@ def progressive(render_pass, ref, target, spp_per_pass, max_spp, eps=0.01):
 
 def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=4096, max_depth=8, guide=None,
                     guide_frac=0.5, train_batch=65536, plain="megakernel", seed=1, max_paths=1 << 21, log=None,
-                    train_steps=16, warmup_steps=0):
+                    train_steps=16, warmup_steps=0, product=False):
     """Time-to-relMSE of the plain and the guided tracer on one scene.
     ``plain`` selects the plain renderer ("megakernel" or "wavefront"). The
     guided tracer samples the BSDF alone (training on those paths) until the
     proposal has taken ``warmup_steps`` Adam steps, then uses one-sample MIS
-    with probability ``guide_frac``; every pass is an unbiased estimate."""
+    with probability ``guide_frac`` (``product``: the proposal times a vMF
+    fit of each vertex's BSDF lobe); every pass is an unbiased estimate."""
     from .guide import Guide
     from .render import GuideCfg, PathIntegrator, Wavefront, WavefrontPathIntegrator
     w, h = int(np.asarray(camera)["width"].reshape(-1)[0]), int(np.asarray(camera)["height"].reshape(-1)[0])
@@ -72,10 +73,12 @@ def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=409
     g = guide if guide is not None else Guide(seed=seed)
     wf = Wavefront(max_paths=max_paths, max_depth=max_depth)
     gpi = WavefrontPathIntegrator(scene, camera, spp=spp_per_pass, max_depth=max_depth, seed=seed + 1000, guide=g,
-                                  guide_frac=guide_frac, train=train_steps, train_batch=train_batch, wavefront=wf)
+                                  guide_frac=guide_frac, train=train_steps, train_batch=train_batch, wavefront=wf,
+                                  product=product)
     # warm-up on a throw-away guide so graph capture is not timed
     wg = WavefrontPathIntegrator(scene, camera, spp=1, max_depth=max_depth, seed=seed + 2000, guide=Guide(seed=seed + 1),
-                                 guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf)
+                                 guide_frac=guide_frac, train=True, train_batch=train_batch, wavefront=wf,
+                                 product=product)
     wg.render_sum(rgb_sum=torch.zeros((h, w, 3), dtype=torch.float32, device="cuda"))
     state = {"warm": warmup_steps > 0}
 
@@ -93,10 +96,59 @@ def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=409
     if log:
         log("guided", rg["reached"], rg["curve"][-1])
     out = {"plain": rp, "guided": rg, "target": target, "spp_per_pass": spp_per_pass, "plain_renderer": plain,
-           "train_steps": train_steps, "warmup_steps": warmup_steps}
+           "train_steps": train_steps, "warmup_steps": warmup_steps, "guide_frac": guide_frac, "product": product}
     if rp["reached"] and rg["reached"]:
         out["time_ratio"] = rg["reached"]["seconds"] / rp["reached"]["seconds"]
     return out
+
+
+def _timed_pass(fn, reps=5):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def frozen_ceiling(scene, camera, ref, spp=16, seeds=(11, 12, 13, 14), train_passes=48, train_spp=4,
+                   guide_frac=0.25, product=True, max_depth=8, max_paths=1 << 21):
+    """What the proposal can buy at best: trained for train_passes guided
+    passes (untimed), then frozen; relMSE at equal spp of the guided and the
+    plain (megakernel) tracer, averaged over seeds, and the cost of a pass of
+    each. ratio_equal_spp x cost_ratio is the equal-quality time ratio the
+    guided tracer would reach if training cost nothing."""
+    from .guide import Guide
+    from .render import PathIntegrator, Wavefront, WavefrontPathIntegrator
+    g = Guide(seed=1, lr=1e-3)
+    wf = Wavefront(max_paths=max_paths, max_depth=max_depth)
+    tr = WavefrontPathIntegrator(scene, camera, spp=train_spp, max_depth=max_depth, seed=50, guide=g,
+                                 guide_frac=guide_frac, train=16, wavefront=wf, product=product)
+    for k in range(train_passes):
+        tr.render_sum(sample_offset=train_spp * k)
+    steps, loss = g.stats()
+    ep, eg = [], []
+    for s in seeds:
+        p = PathIntegrator(camera, spp=spp, max_depth=max_depth, seed=s)
+        ep.append(rel_mse_dev(p.render_sum(scene) / spp, ref))
+        gp = WavefrontPathIntegrator(scene, camera, spp=spp, max_depth=max_depth, seed=s + 100, guide=g,
+                                     guide_frac=guide_frac, train=0, wavefront=wf, product=product)
+        eg.append(rel_mse_dev(gp.render_sum() / spp, ref))
+    film = torch.zeros_like(ref)
+    p4 = PathIntegrator(camera, spp=4, max_depth=max_depth, seed=7)
+    g4 = WavefrontPathIntegrator(scene, camera, spp=4, max_depth=max_depth, seed=8, guide=g, guide_frac=guide_frac,
+                                 train=0, wavefront=wf, product=product)
+    tp = _timed_pass(lambda: p4.render_sum(scene, rgb_sum=film))
+    tg = _timed_pass(lambda: g4.render_sum(rgb_sum=film))
+    wf.close()
+    ratio = float(np.mean(eg)) / float(np.mean(ep))
+    return {"adam_steps": steps, "last_loss": round(loss, 4), "spp": spp, "seeds": len(seeds),
+            "rel_mse_plain": round(float(np.mean(ep)), 6), "rel_mse_guided": round(float(np.mean(eg)), 6),
+            "ratio_equal_spp": round(ratio, 4), "plain_4spp_pass_ms": round(tp * 1e3, 3),
+            "guided_4spp_pass_ms": round(tg * 1e3, 3), "cost_ratio": round(tg / tp, 3),
+            "time_ratio_bound_free_training": round(ratio * tg / tp, 3), "guide_frac": guide_frac,
+            "product": product}
 
 
 def _train_passes(res=512, passes=3, spp=16, train=4, batch=65536):

@@ -322,12 +322,42 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture &m, float ul, float u
     return normalize(add(add(smul(lx, t1), smul(ly, t2)), smul(w, mu)));
 }
 
+// Product of the mixture with a vMF fit of the vertex's cosine-weighted BSDF
+// lobe (mean mu_b, concentration k_b). Each lobe's product is a vMF with
+// k mu = k_j mu_j + k_b mu_b; its weight is scaled by the product's integral
+// c(k_j) c(k_b) exp(k - k_j - k_b) / c(k), c(k) = k / (2 pi (1 - exp(-2k))).
+// The result is again a normalised mixture, so its pdf is exact. False if the
+// weights degenerate.
+__device__ __forceinline__ float vmf_norm(float k) { return k / (6.28318530717958647692f * (-expm1f(-2.f * k))); }
+__device__ __forceinline__ bool mixture_times_lobe(Mixture &m, f3 mu_b, float k_b) {
+    float s = 0.f;
+    const float cb = vmf_norm(k_b);
+    for (int j = 0; j < 8; ++j) {
+        const float kj = m.kappa[j];
+        const float vx = kj * m.mu[j][0] + k_b * mu_b.x, vy = kj * m.mu[j][1] + k_b * mu_b.y,
+                    vz = kj * m.mu[j][2] + k_b * mu_b.z;
+        const float k = fmaxf(sqrtf(vx * vx + vy * vy + vz * vz), 1e-4f);
+        const float scale = vmf_norm(kj) * cb * expf(fminf(k - kj - k_b, 0.f)) / vmf_norm(k);
+        m.mu[j][0] = vx / k;
+        m.mu[j][1] = vy / k;
+        m.mu[j][2] = vz / k;
+        m.kappa[j] = k;
+        m.pi[j] *= scale;
+        s += m.pi[j];
+    }
+    if (!(s > 0.f) || !__builtin_isfinite(s)) return false;
+    const float inv = 1.f / s;
+    for (int j = 0; j < 8; ++j) m.pi[j] *= inv;
+    return true;
+}
+
 // Guide usage for one render pass.
 struct GuideCfg {
     const float *out;         // [paths][40] network outputs, null = plain path tracing
     float frac;               // probability of sampling the proposal (one-sample MIS)
     float bounds_lo[3], inv_ext[3];
     float min_alpha;          // microfacet vertices below this roughness are not guided
+    uint32_t product;         // sample the proposal times a vMF fit of the BSDF lobe
 };
 
 // Vertices where the proposal is queried and trained: diffuse, or microfacet
@@ -456,7 +486,17 @@ __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_dept
     int flags = 0;
     f3 f;
     Mixture mix;
-    const bool guided = gout && nonspec && decode_mixture(gout, mix);
+    bool guided = gout && nonspec && decode_mixture(gout, mix);
+    if (guided && gc.product) {
+        // the lobe about the mirror direction (microfacet: concentration 0.8 /
+        // alpha^2, the half-maximum width of the reflected Trowbridge-Reitz
+        // lobe) or about the normal (Lambertian: 1.4, the cosine's half maximum)
+        const f3 ns = faceforward(it.shading_n, wo);
+        const bool metal = bsdf.m.type == kMetal;
+        const f3 mu_b = metal ? sub(smul(2.f * dot(wo, ns), ns), wo) : ns;
+        const float k_b = metal ? fminf(0.8f / fmaxf(bsdf.m.alpha * bsdf.m.alpha, 1e-6f), 1000.f) : 1.4f;
+        guided = mixture_times_lobe(mix, mu_b, k_b);
+    }
     if (guided) {
         const float usel = ps.rng.get_1d();
         float p_bsdf = 0.f;

@@ -201,6 +201,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         }
         job.gc.frac = guided ? gcfg->guide_frac : 0.f;
         job.gc.min_alpha = gcfg ? gcfg->min_alpha : 0.f;
+        job.gc.product = gcfg ? (gcfg->flags & ILLUME_GUIDE_PRODUCT) : 0u;
         job.gc.out = guided ? wf->w.guide_out : nullptr;
         hj.push_back(job);
         total += n;

@@ -184,9 +184,12 @@ class AOIntegrator(PathIntegrator):
         return rgb_sum
 
 
+GUIDE_PRODUCT = 1  # illume_guide_cfg.flags: proposal x vMF fit of the BSDF lobe
+
+
 class GuideCfg(ctypes.Structure):
     _fields_ = [("guide_frac", ctypes.c_float), ("train", ctypes.c_uint32), ("train_batch", ctypes.c_uint32),
-                ("min_alpha", ctypes.c_float)]
+                ("min_alpha", ctypes.c_float), ("flags", ctypes.c_uint32)]
 
 
 class CameraC(ctypes.Structure):
@@ -355,13 +358,13 @@ class WavefrontPathIntegrator(PathIntegrator):
     ``min_alpha``."""
 
     def __init__(self, scene, camera, spp=16, max_depth=5, seed=0, max_paths=1 << 20, guide=None, guide_frac=0.5,
-                 train=False, train_batch=65536, wavefront=None, min_alpha=0.0, sampler="random"):
+                 train=False, train_batch=65536, wavefront=None, min_alpha=0.0, sampler="random", product=False):
         super().__init__(camera, spp, max_depth, seed, sampler)
         self.scene = scene
         self.guide = guide
         steps = 16 if train is True else int(train)
         self.gcfg = GuideCfg(float(guide_frac) if guide is not None else 0.0, steps, int(train_batch),
-                             float(min_alpha))
+                             float(min_alpha), GUIDE_PRODUCT if product else 0)
         self.wf = wavefront if wavefront is not None else Wavefront(max_paths, max_depth)
 
     def render_sum(self, scene=None, rgb_sum=None, spp=None, sample_offset=0, ray_counts=None, stream=None):

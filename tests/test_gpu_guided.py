@@ -67,29 +67,37 @@ def test_batched_jobs_equal_individual_renders(gpu):
             assert np.allclose(ref, got, rtol=1e-5, atol=1e-5), np.abs(ref - got).max()
 
 
-def test_nan_proposal_falls_back_to_plain(gpu):
+@pytest.mark.parametrize("product", [False, True])
+def test_nan_proposal_falls_back_to_plain(gpu, product):
     from rs_illume_amd.guide import Guide, N_PARAMS
     from rs_illume_amd.render import Scene, WavefrontPathIntegrator
-    _say("case NaN proposal")
+    _say("case NaN proposal, product", product)
     sc = Scene(_glossy())
     cam = S.cornell_camera(32, 32)
     plain = WavefrontPathIntegrator(sc, cam, spp=4, max_depth=8, seed=2).render_sum().cpu().numpy()
     g = Guide(np.full(N_PARAMS, np.nan, np.float32))
-    guided = WavefrontPathIntegrator(sc, cam, spp=4, max_depth=8, seed=2, guide=g, guide_frac=0.5).render_sum()
+    guided = WavefrontPathIntegrator(sc, cam, spp=4, max_depth=8, seed=2, guide=g, guide_frac=0.5,
+                                     product=product).render_sum()
     guided = guided.cpu().numpy()
     assert np.allclose(plain, guided, rtol=1e-5, atol=1e-5)
 
 
-def test_guided_render_is_unbiased(gpu):
+@pytest.mark.parametrize("product", [False, True])
+def test_guided_render_is_unbiased(gpu, product):
+    """An untrained proposal (an arbitrary mixture per vertex), with or
+    without the product with the BSDF lobe, at fraction 0.5: the image mean
+    equals the plain tracer's, so the pdf the estimator divides by is the
+    density the directions were drawn from."""
     from rs_illume_amd.guide import Guide
     from rs_illume_amd.render import Scene, WavefrontPathIntegrator
-    _say("case guided unbiased")
+    _say("case guided unbiased, product", product)
     sc = Scene(_glossy())
     cam = S.cornell_camera(32, 32)
     spp = 1024
     plain = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=11).render().cpu().numpy()
     g = Guide(seed=4)
-    guided = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=12, guide=g, guide_frac=0.5).render()
+    guided = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=12, guide=g, guide_frac=0.5,
+                                     product=product).render()
     guided = guided.cpu().numpy()
     m_p, m_g = plain.mean(), guided.mean()
     _say("mean plain", m_p, "mean guided (untrained proposal)", m_g, "relMSE between", S.rel_mse(guided, plain))
@@ -131,3 +139,30 @@ def test_training_gains_over_plain_at_equal_spp(gpu):
     # updates is not deterministic).
     assert abs(guided.mean() - ref.mean()) < 0.05 * ref.mean()
     assert e_g < 0.5 * e_p
+
+
+def test_trained_product_proposal_unbiased(gpu):
+    """A proposal trained online on the glossy small-light box with the
+    BSDF-lobe product, then frozen: at high spp its image mean equals the
+    plain tracer's within 1%."""
+    from rs_illume_amd.guide import Guide
+    from rs_illume_amd.render import Scene, WavefrontPathIntegrator
+    _say("case trained product proposal unbiased")
+    sc = Scene(_glossy())
+    cam = S.cornell_camera(48, 48)
+    g = Guide(seed=3, lr=1e-3)
+    trainer = WavefrontPathIntegrator(sc, cam, spp=16, max_depth=8, seed=60, max_paths=1 << 16, guide=g,
+                                      guide_frac=0.5, train=16, train_batch=16384, product=True)
+    for k in range(48):
+        trainer.render_sum(sample_offset=16 * k)
+    steps, _ = g.stats()
+    assert steps > 100
+    spp = 2048
+    plain = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=21, max_paths=1 << 20).render()
+    guided = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=22, max_paths=1 << 20, guide=g,
+                                     guide_frac=0.5, product=True).render()
+    plain, guided = plain.cpu().numpy(), guided.cpu().numpy()
+    _say("adam steps", steps, "mean plain", plain.mean(), "guided", guided.mean(), "relMSE between",
+         S.rel_mse(guided, plain))
+    assert np.isfinite(guided).all()
+    assert abs(guided.mean() - plain.mean()) < 0.01 * plain.mean()

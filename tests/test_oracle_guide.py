@@ -138,3 +138,44 @@ def test_training_reduces_loss_on_a_fixed_distribution(oracle):
         oracle.adam(p, m, v, g, t, lr=5e-3)
     l1, _ = oracle.guide_loss_grad(p, x, nrm, wi, w, BOUNDS)
     assert l1 < l0 - 1.0, (l0, l1)
+
+
+def _sphere_grid(n=400):
+    th = (np.arange(n) + 0.5) / n * np.pi
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dw = (np.sin(T) * (np.pi / n) * (2 * np.pi / (2 * n))).reshape(-1)
+    return d, dw
+
+
+def test_product_with_a_bsdf_lobe(oracle):
+    """The mixture times a vMF lobe (the BSDF-aware proposal): its pdf
+    integrates to 1, is proportional to mixture pdf x lobe density at every
+    direction (a wrong product scale breaks the proportionality, a wrong
+    product mean the normalisation), and its sampler follows it."""
+    rng = np.random.default_rng(17)
+    outs = (rng.normal(size=(1, 40)) * 1.5).astype(np.float32)
+    outs[0, 4::5] = rng.uniform(0, 4, 8)
+    d, dw = _sphere_grid()
+    for mu_b, k_b in (((0.0, 0.0, 1.0), 1.4), ((0.6, 0.0, 0.8), 9.0), ((-0.3, 0.9, 0.316), 40.0)):
+        mu_b = np.array(mu_b, np.float32) / np.linalg.norm(mu_b)
+        rows = np.repeat(outs, len(d), 0)
+        mus, ks = np.repeat(mu_b[None], len(d), 0), np.full(len(d), k_b, np.float32)
+        prod = oracle.guide_product_pdf(rows, mus, ks, d).astype(np.float64)
+        assert abs(np.sum(prod * dw) - 1) < 3e-3, (k_b, np.sum(prod * dw))
+        lobe = k_b / (2 * np.pi * (1 - np.exp(-2 * k_b))) * np.exp(k_b * (d @ mu_b - 1))
+        base = oracle.guide_pdf(rows, d).astype(np.float64) * lobe
+        big = base > 1e-3 * base.max()
+        ratio = prod[big] / base[big]
+        assert np.allclose(ratio, ratio.mean(), rtol=1e-4), (k_b, ratio.min(), ratio.max())
+        m = 200000
+        wi, spdf = oracle.guide_product_sample(np.repeat(outs, m, 0), np.repeat(mu_b[None], m, 0),
+                                               np.full(m, k_b, np.float32), rng.random((m, 3)).astype(np.float32))
+        assert np.allclose(np.linalg.norm(wi, axis=1), 1, atol=1e-5)
+        assert np.allclose(spdf, oracle.guide_product_pdf(np.repeat(outs, m, 0), np.repeat(mu_b[None], m, 0),
+                                                          np.full(m, k_b, np.float32), wi), rtol=1e-3)
+        mean_q = (prod[:, None] * d * dw[:, None]).sum(0)
+        assert np.allclose(wi.mean(0), mean_q, atol=1e-2), (wi.mean(0), mean_q)
+        cap = d @ mu_b > 0.9
+        assert abs((wi @ mu_b > 0.9).mean() - np.sum(prod[cap] * dw[cap])) < 5e-3
