@@ -279,7 +279,7 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
     return out
 
 
-def bench_batched(no_cpu, log=log, builder="SAH"):
+def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
     """Config 2: 64 procedural scenes (seeds 1-64) in one batched call, twice:
     through one wavefront pool (launch shape chosen by the autotuner on a 2-spp
     probe of the same batch) and through the megakernel batch
@@ -335,6 +335,17 @@ def bench_batched(no_cpu, log=log, builder="SAH"):
             "mrays_per_s": round(float(mk_rays.sum()) / dt_mk / 1e6, 1),
             "film_finite": bool(torch.isfinite(film).all())}
     del film
+    if pmc and builder == "SAH":
+        # HBM-side bytes of the batched render kernel from the counter passes
+        # (same scenes, trees and launch), over this run's launch time
+        gbs = pmc["traffic_bytes"] / dt_mk / 1e9
+        mega["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": round(pmc["traffic_bytes"]),
+                            "fetch_bytes": round(pmc["fetch_bytes"]), "write_bytes": round(pmc["write_bytes"]),
+                            "l2_hit_rate": round(pmc["l2_hit_rate"], 4), "kernel": pmc["kernel"],
+                            "bytes_per_ray": round(pmc["traffic_bytes"] / max(float(mk_rays.sum()), 1.0), 1),
+                            "basis": "FETCH_SIZE (raw) + WRITE_SIZE of one render_batch_kernel launch from "
+                                     "rocprofv3 --pmc passes in this run, over the timed launch"}
     fastest = "megakernel_batch" if dt_mk < dt else "wavefront"
     top = mega if dt_mk < dt else wave
     out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
@@ -559,6 +570,15 @@ def main():
         except Exception as e:  # reported in the JSON line; the timing still runs
             pmc_error = str(e)[-300:]
             log("counter passes failed:", pmc_error)
+    pmc_batched = None
+    if not args.no_counters and not args.only and not args.no_render:
+        from rs_illume_amd import counters
+        try:
+            t0 = time.time()
+            pmc_batched = counters.collect_batched(log=log)
+            log("batched counter passes %.1f s" % (time.time() - t0))
+        except Exception as e:
+            log("batched counter passes failed:", str(e)[-300:])
 
     import torch
     from rs_illume_amd import workloads as W
@@ -686,7 +706,7 @@ def main():
         log("guided vs plain")
         extra["guided"] = bench_guided(no_cpu=args.no_cpu)
         log("batched")
-        extra["batched"] = bench_batched(args.no_cpu)
+        extra["batched"] = bench_batched(args.no_cpu, pmc=pmc_batched)
         log("combined")
         extra["combined"] = bench_combined(no_cpu=args.no_cpu)
 

@@ -140,6 +140,19 @@ int illume_intersect_any(const illume_bvh *bvh, const illume_ray *rays, uint32_t
 int illume_intersect_closest_noalloc(const illume_bvh *bvh, const illume_ray *rays, uint32_t n, illume_hit *hits,
                                      void *stream);
 
+/* One shape against a batch of rays, for parity tests of the shapes alone:
+ * Shape::intersect (any == 0; core/shape.rs:8-112, shapes/triangle.rs:142-419,
+ * shapes/sphere.rs:105-236) or Shape::intersect_p (any != 0; triangle.rs:421-607,
+ * sphere.rs:238-323) of a single Triangle (shape: 9 floats, vertices p0 p1 p2)
+ * or full Sphere (shape: 4 floats, centre and radius). shape is a host
+ * pointer; rays, hits device pointers. A hit writes prim_id 0, t and (triangles,
+ * closest hit) the barycentrics b1, b2; a miss prim_id ILLUME_MISS. The closest
+ * hit rejects degenerate triangles as Triangle::intersect does. n <= 2^31. */
+#define ILLUME_SHAPE_TRIANGLE 0
+#define ILLUME_SHAPE_SPHERE 1
+int illume_shape_intersect_batch(int32_t kind, const float *shape, const illume_ray *rays, uint32_t n, int32_t any,
+                                 illume_hit *hits, void *stream);
+
 /* Instrumented closest/any traversal that also counts, per ray, node-box tests
  * and primitive tests (the reference's "BVH/..." and "Intersections/..."
  * counters, core/stats_accumulator.rs). counts: device uint32[2*n] (node
@@ -283,6 +296,17 @@ int illume_render(const illume_scene *scene, const illume_camera *camera, const 
  * illume_render. n_ao_samples <= 65536. ray_counts as in illume_render. */
 int illume_render_ao(const illume_scene *scene, const illume_camera *camera, const illume_render_cfg *cfg,
                      uint32_t n_ao_samples, int32_t cos_sample, float *rgb_sum, uint64_t *ray_counts, void *stream);
+
+/* SamplerIntegrator::li (core/integrator.rs:31-35) for caller-given rays, for
+ * parity tests: ray i takes the random sampler stream (cfg->seed, i,
+ * cfg->sample_offset), skips the five dimensions of a camera sample and runs
+ * PathIntegrator::li (cfg->max_depth) from the ray, so the camera ray of pixel
+ * p for sample s gives illume_render's sample s of pixel p. L_out: device
+ * floats, 3 per ray, the radiance after SamplerIntegratorBase::render's guard.
+ * cfg->flags must be 0; cfg->spp is unused. Latency-bound by design: renders
+ * go through illume_render. ray_counts as in illume_render. */
+int illume_li_batch(const illume_scene *scene, const illume_ray *rays, uint32_t n, const illume_render_cfg *cfg,
+                    float *L_out, uint64_t *ray_counts, void *stream);
 
 /* ------------------------------------------------------------------ reconstruction filters
  * The Filter trait (core/filter.rs) and its implementations: BoxFilter

@@ -209,6 +209,8 @@ def _render_decl():
     L.orc_render_filtered.restype = i32
     L.orc_render_ao.argtypes = [vp, vp, u32, u32, i32, u64, u32, u32, vp, vp, i32]
     L.orc_render_ao.restype = None
+    L.orc_li_batch.argtypes = [vp, vp, u32, u32, u64, u32, vp, vp, i32]
+    L.orc_li_batch.restype = None
     for fn, at in (("orc_filter_table", [vp, vp]), ("orc_filter_eval", [vp, vp, u32, vp]),
                    ("orc_film_sample_bounds", [vp, i32, i32, vp]),
                    ("orc_film_add_samples", [vp, vp, u32, i32, i32, vp])):
@@ -246,6 +248,17 @@ class OracleRenderScene:
         st = np.zeros(2, np.uint64)
         _render_decl().orc_render(self.h, _p(cam), spp, max_depth, seed, sample_offset, _sampler_flags(sampler),
                                   _p(out), _p(st), n_threads or n_threads_default())
+        return out, st
+
+    def li_batch(self, rays, max_depth, seed=0, sample=0, n_threads=None):
+        """SamplerIntegrator::li of each ray (RAY_DTYPE): (n, 3) radiance and
+        (closest, any) ray counts; ray i uses sampler stream (seed, i, sample)
+        after the camera sample's five dimensions."""
+        r = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros((len(r), 3), np.float32)
+        st = np.zeros(2, np.uint64)
+        _render_decl().orc_li_batch(self.h, _p(r.view(np.float32)), len(r), max_depth, seed, sample, _p(out), _p(st),
+                                    n_threads or n_threads_default())
         return out, st
 
     def render_ao(self, camera, spp, n_samples, cos_sample=True, seed=0, sample_offset=0, n_threads=None,

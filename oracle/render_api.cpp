@@ -182,6 +182,55 @@ void orc_render(void *h, const void *camera, uint32_t spp, uint32_t max_depth, u
                 });
 }
 
+// SamplerIntegrator::li (integrator.rs:31-35) for caller-given rays (32 bytes
+// each: o, d, t_max, time): ray i takes the sampler stream (seed, i, sample),
+// skips the five dimensions of a camera sample and runs PathIntegrator::li;
+// L_out gets the guarded radiance, 3 floats per ray.
+void orc_li_batch(void *h, const float *rays, uint32_t n, uint32_t max_depth, uint64_t seed, uint32_t sample,
+                  float *L_out, uint64_t *stats, int n_threads) {
+    RenderScene *s = (RenderScene *)h;
+    int nt = std::max(1, n_threads);
+    std::vector<RenderStats> st(nt);
+    std::atomic<uint64_t> next(0);
+    auto work = [&](int t) {
+        while (true) {
+            uint64_t b = next.fetch_add(64);
+            if (b >= n) break;
+            uint64_t e = std::min<uint64_t>(n, b + 64);
+            for (uint64_t i = b; i < e; ++i) {
+                const float *r = rays + 8 * i;
+                Sampler sampler;
+                sampler.start(seed, i, sample);
+                float u[2];
+                sampler.get_2d(u);
+                sampler.get_1d();
+                sampler.get_2d(u);
+                Ray ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), r[6], r[7]);
+                RGB L = path_li(*s, ray, sampler, (int)max_depth, &st[t]);
+                if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = RGB();
+                else if (luminance(L) < -1e-5f) L = RGB();
+                else if (std::isinf(luminance(L))) L = RGB();
+                L_out[3 * i] = L.x;
+                L_out[3 * i + 1] = L.y;
+                L_out[3 * i + 2] = L.z;
+            }
+        }
+    };
+    if (nt == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    if (stats) {
+        stats[0] = stats[1] = 0;
+        for (auto &x : st) {
+            stats[0] += x.closest_rays;
+            stats[1] += x.any_rays;
+        }
+    }
+}
+
 // SamplerIntegratorBase::render with AOIntegrator::li (integrators/ao.rs:16-112):
 // n_ao hemisphere samples per camera sample, cosine-distributed if cos_sample.
 void orc_render_ao(void *h, const void *camera, uint32_t spp, uint32_t n_ao, int cos_sample, uint64_t seed,

@@ -151,6 +151,32 @@ class BVHAccel:
         return self.intersect_p(rays_to_device(rays_np)).cpu().numpy()
 
 
+SHAPES = {"triangle": 0, "sphere": 1}
+
+
+def shape_intersect(kind, shape, rays, any_hit=False, hits=None, stream=None):
+    """Shape::intersect (or, any_hit, Shape::intersect_p) of one shape
+    (core/shape.rs:8-112) for a batch of rays: ``kind`` "triangle" (shape =
+    9 floats, p0 p1 p2) or "sphere" (centre, radius). -> int32 (n, 4) hits as
+    :meth:`BVHAccel.intersect` gives them, prim id 0 on a hit."""
+    BVHAccel._check_rays(rays)
+    n = rays.shape[0]
+    sh = np.ascontiguousarray(shape, np.float32).reshape(-1)
+    if hits is None:
+        hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+    L = _lib.lib()
+    if not getattr(L, "_shape_declared", False):
+        L.illume_shape_intersect_batch.restype = ctypes.c_int
+        L.illume_shape_intersect_batch.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                   ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+        L._shape_declared = True
+    _lib.check(L.illume_shape_intersect_batch(SHAPES[kind], sh.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.c_void_p(rays.data_ptr()), n, 1 if any_hit else 0,
+                                              ctypes.c_void_p(hits.data_ptr()), _stream_handle(stream)),
+               "illume_shape_intersect_batch")
+    return hits
+
+
 def split_hits(hits_dev):
     """int32 (n,4) device hits -> (prim int64 with MISS as 0xFFFFFFFF, t, b1, b2) tensors."""
     prim = hits_dev[:, 0].to(torch.int64) & 0xFFFFFFFF
@@ -158,4 +184,5 @@ def split_hits(hits_dev):
     return prim, f[:, 1], f[:, 2], f[:, 3]
 
 
-__all__ = ["BVHAccel", "rays_to_device", "hits_to_numpy", "split_hits", "MISS", "RAY_DTYPE", "HIT_DTYPE"]
+__all__ = ["BVHAccel", "rays_to_device", "hits_to_numpy", "split_hits", "shape_intersect", "MISS", "RAY_DTYPE",
+           "HIT_DTYPE"]

@@ -80,16 +80,20 @@ def _mean_for(values, pattern, counter):
     return (sum(v) / len(v), len(v)) if v else (None, 0)
 
 
-def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None):
-    """One rocprofv3 --pmc pass over the traversal workload; returns the
-    per-dispatch counter values."""
+BATCH = r"render_batch_kernel"
+
+
+def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None, batched=False):
+    """One rocprofv3 --pmc pass over the traversal workload (or, batched, the
+    64-scene batched render); returns the per-dispatch counter values."""
     out_dir = keep_dir or tempfile.mkdtemp(prefix="illume_pmc_", dir="/tmp")
     env = dict(os.environ)
     env["TMPDIR"] = "/tmp"
     env["PYTHONPATH"] = REPO + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
     cmd = ["timeout", "-s", "KILL", str(timeout), "rocprofv3", "--pmc"] + list(counters) + \
           ["--output-format", "csv", "-d", out_dir, "-o", "run", "--", sys.executable, "-m",
-           "rs_illume_amd.workloads", "--width", str(width), "--launches", str(launches)]
+           "rs_illume_amd.workloads", "--width", str(width), "--launches", str(launches)] + \
+          (["--batched"] if batched else [])
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
     vals = counter_values(out_dir)
     if not vals:
@@ -139,6 +143,29 @@ def collect(width=4, launches=2, log=None):
         "memory_share_of_l2_reads": got["TCC_EA0_RDREQ_DRAM_sum"] / got["TCC_READ_sum"] if got["TCC_READ_sum"]
         else None,
     }
+
+
+def collect_batched(log=None):
+    """HBM-side traffic and L2 hit rate of one batched render launch (the
+    64 procedural scenes): two passes, FETCH_SIZE + TCC_HIT_sum, then
+    WRITE_SIZE + TCC_MISS_sum. FETCH_SIZE is reported raw (no calibration:
+    the batched render's reads mix node lines, primitive records and
+    per-pixel film atomics)."""
+    got = {}
+    for counters in (("FETCH_SIZE", "TCC_HIT_sum"), ("WRITE_SIZE", "TCC_MISS_sum")):
+        if log:
+            log("rocprofv3 --pmc %s (batched render)" % " ".join(counters))
+        vals = run_pass(counters, launches=1, timeout=240, batched=True)
+        for c in counters:
+            got[c], _ = _mean_for(vals, BATCH, c)
+            if got[c] is None:
+                raise RuntimeError("no %s value for the batched render kernel" % c)
+        names = sorted({k for (_, k) in vals if re.search(BATCH, k)})
+        got["kernel"] = names[0] if names else None
+    hit, miss = got["TCC_HIT_sum"], got["TCC_MISS_sum"]
+    fetch, write = got["FETCH_SIZE"] * 1024.0, got["WRITE_SIZE"] * 1024.0
+    return {"kernel": got["kernel"], "fetch_bytes": fetch, "write_bytes": write, "traffic_bytes": fetch + write,
+            "l2_hit_rate": hit / (hit + miss) if hit + miss else None}
 
 
 def kernel_stats(out_dir, pattern, skip=0):

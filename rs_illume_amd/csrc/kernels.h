@@ -21,6 +21,11 @@ hipError_t launch_render_ao(const DevScene &sc, const DevCamera &cam, uint32_t s
                             uint64_t seed, uint32_t sample_offset, float *rgb, unsigned long long *counters,
                             const SamplerLaunch &hl, hipStream_t stream);
 
+// SamplerIntegrator::li for n caller-given rays (stream i, sample `sample`,
+// the camera sample's five dimensions skipped): 3 floats per ray in L_out.
+hipError_t launch_li_batch(const DevScene &sc, const illume_ray *rays, uint32_t n, uint32_t max_depth, uint64_t seed,
+                           uint32_t sample, float *L_out, unsigned long long *counters, hipStream_t stream);
+
 // Path tracer through a reconstruction filter: samples over fs's sample
 // bounds, splatted into rgbw (4 floats per image pixel).
 hipError_t launch_render_filtered(const DevScene &sc, const DevCamera &cam, const FilmSplat &fs, uint32_t spp,

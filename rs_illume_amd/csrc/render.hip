@@ -159,6 +159,63 @@ __global__ __launch_bounds__(BLOCK, 4) void render_ao_kernel(DevScene sc, DevCam
     }
 }
 
+// SamplerIntegrator::li for caller-given rays (one thread per ray): ray i
+// takes the sampler stream (seed, i, sample), skips the five dimensions of a
+// camera sample and runs PathIntegrator::li from the ray, so the camera ray of
+// pixel p for sample s gives illume_render's sample s of pixel p. L_out gets
+// the guarded radiance (3 floats per ray).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 3) void li_batch_kernel(DevScene sc, const illume_ray *__restrict__ rays, uint32_t n,
+                                                          uint32_t max_depth, uint64_t seed, uint32_t sample,
+                                                          float *__restrict__ L_out,
+                                                          unsigned long long *__restrict__ counters) {
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    __shared__ uint32_t blk_counts[2];
+    if (threadIdx.x < 2) blk_counts[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    PathCounters pc{0, 0};
+    if (i < n) {
+        int spill[kSpill];
+        TraceStack<BLOCK> st;
+        st.reset(lds_stack + threadIdx.x, spill);
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        PathState ps;
+        ps.rng.seed(seed, i, sample);
+        for (int k = 0; k < 5; ++k) ps.rng.get_1d();  // film 2D, time 1D, lens 2D
+        ps.o = mk(r0.x, r0.y, r0.z);
+        ps.d = mk(r0.w, r1.x, r1.y);
+        ps.t_max = r1.z;
+        ps.beta = mk(1.f, 1.f, 1.f);
+        ps.L = mk(0.f, 0.f, 0.f);
+        ps.eta_scale = 1.f;
+        ps.bounces = 0;
+        ps.specular = false;
+        ps.active = true;
+        GuideCfg gc{};
+        uint32_t n_recs = 0;
+        while (ps.active) {
+            TraceHit h;
+            const bool found = scene_closest<BLOCK>(sc, ps.o, ps.d, ps.t_max, h, st, pc);
+            path_shade<BLOCK>(sc, max_depth, ps, found, h, nullptr, gc, nullptr, n_recs, st, pc);
+        }
+        const f3 L = guard_radiance(ps.L);
+        L_out[3 * (size_t)i] = L.x;
+        L_out[3 * (size_t)i + 1] = L.y;
+        L_out[3 * (size_t)i + 2] = L.z;
+    }
+    if (counters) {
+        atomicAdd(&blk_counts[0], pc.closest);
+        atomicAdd(&blk_counts[1], pc.any);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(counters, (unsigned long long)blk_counts[0]);
+            atomicAdd(counters + 1, (unsigned long long)blk_counts[1]);
+        }
+    }
+}
+
 // Batched megakernel: one thread per path of the whole batch; lanes find
 // their job (scene, camera, film offset, sampler) by binary search over the
 // jobs' first path numbers, so one launch renders images of any sizes.
@@ -305,6 +362,24 @@ hipError_t launch_render_ao(const DevScene &sc, const DevCamera &cam, uint32_t s
     if (grid > 0x7fffffffULL) return hipErrorInvalidValue;
     render_ao_kernel<kRenderBlock><<<(uint32_t)grid, kRenderBlock, 0, stream>>>(sc, cam, spp, n_ao, cos_sample, seed,
                                                                                 sample_offset, rgb, counters, hl);
+    return hipGetLastError();
+}
+
+__global__ void zero_floats_kernel(float *__restrict__ p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+hipError_t launch_li_batch(const DevScene &sc, const illume_ray *rays, uint32_t n, uint32_t max_depth, uint64_t seed,
+                           uint32_t sample, float *L_out, unsigned long long *counters, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    if (!sc.nodes) {  // nothing to hit, no lights: black
+        const uint64_t m = 3ull * n;
+        zero_floats_kernel<<<(uint32_t)((m + 255) / 256), 256, 0, stream>>>(L_out, m);
+        return hipGetLastError();
+    }
+    li_batch_kernel<kRenderBlock><<<(n + kRenderBlock - 1) / kRenderBlock, kRenderBlock, 0, stream>>>(
+        sc, rays, n, max_depth, seed, sample, L_out, counters);
     return hipGetLastError();
 }
 

@@ -465,6 +465,23 @@ int illume_render_ao(const illume_scene *scene, const illume_camera *camera, con
     return ILLUME_OK;
 }
 
+int illume_li_batch(const illume_scene *scene, const illume_ray *rays, uint32_t n, const illume_render_cfg *cfg,
+                    float *L_out, uint64_t *ray_counts, void *stream) {
+    if (!scene || !cfg) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
+    if (cfg->flags != 0)
+        return abi_fail(ILLUME_ERR_UNSUPPORTED, "illume_li_batch takes the random sampler streams only (flags 0)");
+    if (cfg->sample_offset >= (1u << 24)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "sample index beyond 2^24");
+    if (n > (1u << 31)) return abi_fail(ILLUME_ERR_UNSUPPORTED, "at most 2^31 rays per call");
+    if (n == 0) return ILLUME_OK;
+    if (!rays || !L_out) return abi_fail(ILLUME_ERR_INVALID_ARG, "rays or L_out is null");
+    DevScene dev = scene->dev;
+    if (scene->n_prims == 0) dev.nodes = nullptr;  // the launcher writes black
+    hipError_t e = launch_li_batch(dev, rays, n, cfg->max_depth, cfg->seed, cfg->sample_offset, L_out,
+                                   (unsigned long long *)ray_counts, (hipStream_t)stream);
+    if (e != hipSuccess) return abi_hip_fail(e, "li batch launch");
+    return ILLUME_OK;
+}
+
 int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream) {
     if (!film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     std::vector<RenderJob> hj;

@@ -45,6 +45,8 @@ def _declare():
     L.illume_scene_bvh.argtypes = [vp, ctypes.POINTER(vp)]
     L.illume_render.restype = i32
     L.illume_render.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), vp, vp, vp]
+    L.illume_li_batch.restype = i32
+    L.illume_li_batch.argtypes = [vp, vp, u32, ctypes.POINTER(RenderCfg), vp, vp, vp]
     L.illume_render_ao.restype = i32
     L.illume_render_ao.argtypes = [vp, vp, ctypes.POINTER(RenderCfg), u32, ctypes.c_int32, vp, vp, vp]
     L._render_declared = True
@@ -158,6 +160,20 @@ class PathIntegrator:
         """The image (H, W, 3) float32 device tensor: mean radiance per pixel."""
         s = self.render_sum(scene, stream=stream)
         return s / float(self.spp)
+
+    def li(self, scene, rays, sample=0, ray_counts=None, stream=None):
+        """SamplerIntegrator::li (core/integrator.rs:31-35) of each ray of a
+        device float32 (n, 8) ray tensor -> (n, 3) radiance; ray i uses the
+        sampler stream (seed, i, sample) after a camera sample's five
+        dimensions (illume_li_batch). For parity tests; renders use render()."""
+        n = rays.shape[0]
+        out = torch.empty((n, 3), dtype=torch.float32, device=rays.device)
+        cfg = RenderCfg(1, self.max_depth, int(sample), 0, self.seed)
+        rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+        _lib.check(_declare().illume_li_batch(scene.handle, ctypes.c_void_p(rays.data_ptr()), n, ctypes.byref(cfg),
+                                              ctypes.c_void_p(out.data_ptr()), rc, _stream_handle(stream)),
+                   "illume_li_batch")
+        return out
 
 
 class AOIntegrator(PathIntegrator):

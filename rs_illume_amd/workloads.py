@@ -119,14 +119,34 @@ def _probe_ceiling(sizes_mib=(16, 100, 1024, 8192), lanes=1 << 22, steps=64):
         del buf
 
 
+def _profile_batched(launches=1):
+    """The profiling workload of the batched render: the 64 procedural scenes
+    (seeds 1-64, host SAH trees) in `launches` calls of illume_render_batch
+    at their own resolution and spp, max depth 8."""
+    import torch
+    from . import scenes as S
+    from .render import Scene, render_batch
+    descs = [S.procedural_scene(seed) for seed in range(1, 65)]
+    scenes = [Scene(d) for d, _, _ in descs]
+    jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
+    film = None
+    for _ in range(launches):
+        film, _ = render_batch(jobs, film=film)
+    torch.cuda.synchronize()
+    print("%d batched render launches done" % launches, flush=True)
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="closest-hit profiling workload")
     ap.add_argument("--width", type=int, default=2, choices=(2, 4))
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--probe", action="store_true", help="time the random-read probe over several buffer sizes")
+    ap.add_argument("--batched", action="store_true", help="the 64-scene batched render instead")
     a = ap.parse_args()
     if a.probe:
         _probe_ceiling()
+    elif a.batched:
+        _profile_batched(a.launches)
     else:
         _profile_launches(a.width, a.launches)
