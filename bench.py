@@ -392,9 +392,9 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
                    builder="SAH", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
-    Plain (megakernel batch) and guided (wavefront pool; one proposal network
-    for the scene, trained online;
-    the radiance field it learns is view-independent) run progressive passes
+    Plain (megakernel batch) and guided (wavefront pool; a proposal network
+    per view, trained online on its view's samples, and one network shared by
+    all views reported beside it) run progressive passes
     of all views; a view's time-to-relMSE is the batch time at which its image
     first reaches the target against a ref_spp plain reference. The CPU leg
     renders view 0 at its native resolution with the oracle on host cores in
@@ -429,29 +429,38 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
     log("combined: %.1fM tris, %d spheres, %d views, %.2f Mpx; reference %d spp in %.1f s"
         % (len(desc.idx) / 1e6, len(desc.spheres), len(cams), n_px / 1e6, ref_spp, ref_s))
 
-    def run(guide):
+    def run(guide=None, guides=None):
+        # guide: one network for every view; guides: one network per view
+        guided_run = guide is not None or guides is not None
         film = torch.zeros_like(ref)
-        gc = GuideCfg(guide_frac, train_steps, 65536, 0.0, GUIDE_PRODUCT) if guide is not None else None
-        warm = guide is not None and warmup_steps > 0
+        gc = GuideCfg(guide_frac, train_steps, 65536, 0.0, GUIDE_PRODUCT) if guided_run else None
+        warm = guided_run and warmup_steps > 0
         reached = [None] * len(cams)
         elapsed, spp = 0.0, 0
         # plain passes on the megakernel batch (the faster plain renderer), guided ones on the wavefront pool
-        if guide is None:
+        if not guided_run:
             render_batch(jobs(1, 1 << 23, 9000))
         else:
             wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99), gcfg=gc)
+
+        def steps():
+            # Adam steps of the network (the mean over the per-view networks)
+            return guide.stats()[0] if guide is not None else float(np.mean([g.stats()[0] for g in guides]))
+
         while spp < max_spp and any(r is None for r in reached):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             if warm:
                 gc.guide_frac = 0.0
-            if guide is None:
+            if not guided_run:
                 render_batch(jobs(spp_pass, spp, 100), film=film)
-            else:
+            elif guide is not None:
                 wf.render_batch(jobs(spp_pass, spp, 200), film=film, guide=guide, gcfg=gc)
-            if warm and guide.stats()[0] >= warmup_steps:
+            else:
+                wf.render_batch(jobs(spp_pass, spp, 200), film=film, gcfg=gc, guides=guides)
+            if warm and steps() >= warmup_steps:
                 warm = False
-            if guide is not None and not warm:
+            if guided_run and not warm:
                 gc.guide_frac = guide_frac
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t1
@@ -467,8 +476,11 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
                 "time_to_target_per_view": [r[0] if r else None for r in reached],
                 "spp_to_target_per_view": [r[1] if r else None for r in reached]}
 
-    plain = run(None)
-    guided = run(Guide(seed=7))
+    plain = run()
+    # a proposal network per view, as configured (each trained on its view's
+    # samples), and one network shared by all views beside it
+    guided = run(guides=[Guide(seed=70 + v) for v in range(len(cams))])
+    guided_one = run(guide=Guide(seed=7))
     wf.close()
     sc.close()
     cpu = None
@@ -483,8 +495,11 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
         cpu["sample"] = ("view 0 (%dx%d) in 1-spp passes for a %.0f s wall budget (oracle BVH build %.1f s excluded)"
                          % (w0, h0, cpu_budget_s, build_s))
         del osc
-    both = [(p, g) for p, g in zip(plain["time_to_target_per_view"], guided["time_to_target_per_view"])
-            if p is not None and g is not None]
+    def median_ratio(g_run):
+        both = [(p, g) for p, g in zip(plain["time_to_target_per_view"], g_run["time_to_target_per_view"])
+                if p is not None and g is not None]
+        return round(float(np.median([g / p for p, g in both])), 3) if both else None
+
     return {"workload": "fBm terrain 1024^2 (2.09M tris, 6 octaves, lacunarity 2, gain 0.5) + 10k spheres "
                         "(radius lognormal, median 0.02), Lambertian/microfacet, analytic sky; 16 views "
                         "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
@@ -494,9 +509,10 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
             "reference_s": round(ref_s, 2),
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
             "guide_frac": guide_frac, "product_with_bsdf_lobe": True,
-            "plain": plain, "guided": guided,
-            "median_time_ratio_guided_over_plain": round(float(np.median([g / p for p, g in both])), 3) if both
-            else None, "cpu_baseline": cpu}
+            "plain": plain, "guided": guided, "guided_networks": "one per view (16), trained on its view's samples",
+            "guided_one_network": guided_one,
+            "median_time_ratio_guided_over_plain": median_ratio(guided),
+            "median_time_ratio_one_network_over_plain": median_ratio(guided_one), "cpu_baseline": cpu}
 
 
 def bench_mlp():

@@ -445,6 +445,16 @@ int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, con
                             float *rgb_sum, uint64_t *ray_counts, void *stream);
 int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
                                   const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream);
+/* illume_wavefront_render_batch with a proposal network per job: guides[j]
+ * (null: no proposal for job j) guides job j's paths and, with gcfg->train,
+ * is trained on job j's radiance samples only (each network on the samples of
+ * the jobs that name it). Jobs may share a network; at most 16 distinct
+ * networks per call (ILLUME_ERR_UNSUPPORTED beyond). One launch sequence for
+ * the whole batch: one proposal stage infers every network's rows. With every
+ * guides[j] equal to one guide this is illume_wavefront_render_batch. */
+int illume_wavefront_render_batch_guides(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs,
+                                         illume_guide *const *guides, const illume_guide_cfg *gcfg, float *film,
+                                         uint64_t *ray_counts, void *stream);
 /* The filtered film of illume_render_filtered on the wavefront pool: job j's
  * camera samples cover filters[j]'s sample bounds and are splatted into its
  * image of width*height*4 floats (r, g, b, weight sums) at film_offset;

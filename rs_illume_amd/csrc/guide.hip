@@ -116,11 +116,11 @@ __device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*
     }
 }
 
-// n rows of in [n][6] -> out [n][40]; n from n_ptr (at most max_n) when
-// n_ptr is non-null.
-__global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in,
-                                                                       const uint32_t *__restrict__ n_ptr,
-                                                                       uint32_t n_arg, float *__restrict__ out) {
+// n rows of in [n][6] -> out [n][40] by the blocks blockIdx.x of one grid
+// row; n from n_ptr (at most max_n) when n_ptr is non-null.
+__device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__restrict__ in,
+                                           const uint32_t *__restrict__ n_ptr, uint32_t n_arg,
+                                           float *__restrict__ out) {
     __shared__ InferWeights W;
     const uint32_t n = n_ptr ? min(*n_ptr, n_arg) : n_arg;
     const uint32_t n_iter = (n + 31) / 32;
@@ -174,6 +174,23 @@ __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev 
             }
         }
     }
+}
+
+__global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in,
+                                                                       const uint32_t *__restrict__ n_ptr,
+                                                                       uint32_t n_arg, float *__restrict__ out) {
+    infer_rows(g, in, n_ptr, n_arg, out);
+}
+
+// One grid row per network k (blockIdx.y): its rows start at row k * stride
+// of in and out, their count is counts[k].
+__global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_nets_kernel(const GuideDev *__restrict__ nets,
+                                                                            const float *__restrict__ in,
+                                                                            const uint32_t *__restrict__ counts,
+                                                                            uint32_t stride, float *__restrict__ out) {
+    const uint32_t k = blockIdx.y;
+    const GuideDev g = nets[k];
+    infer_rows(g, in + 6 * (size_t)k * stride, counts + k, stride, out + (size_t)kOut * k * stride);
 }
 
 // d(-log q)/d outputs for one row (mixture decode as in the sampler).
@@ -581,6 +598,14 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
     if (max_n == 0) return hipSuccess;
     const uint32_t blocks = min((max_n + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
     guide_infer_kernel<<<blocks, kInfWaves * kWave, 0, st>>>(g, in, n_ptr, max_n, out);
+    return hipGetLastError();
+}
+
+hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
+                                   uint32_t stride, float *out, hipStream_t st) {
+    if (n_nets == 0 || stride == 0) return hipSuccess;
+    const uint32_t blocks = min((stride + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
+    guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, in, counts, stride, out);
     return hipGetLastError();
 }
 

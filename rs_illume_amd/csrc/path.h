@@ -358,6 +358,7 @@ struct GuideCfg {
     float bounds_lo[3], inv_ext[3];
     float min_alpha;          // microfacet vertices below this roughness are not guided
     uint32_t product;         // sample the proposal times a vMF fit of the BSDF lobe
+    int32_t net;              // the job's proposal network in a batch, -1 none
 };
 
 // Vertices where the proposal is queried and trained: diffuse, or microfacet

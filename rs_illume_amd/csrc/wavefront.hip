@@ -70,6 +70,31 @@ __device__ __forceinline__ uint32_t find_job(const WfState &w, unsigned long lon
     return lo;
 }
 
+// A slot in a queue for every lane of the wave (among the active lanes) whose
+// `pred` holds, with one atomicAdd per wave instead of one per lane (millions
+// of lanes adding 1 to one address serialise at the L2): the first such lane
+// reserves them all, each takes base + its rank. Every active lane of the wave
+// must call it (divergent branches call it with their own active lanes).
+__device__ __forceinline__ uint32_t wave_slot(uint32_t *ctr, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const int leader = m ? __ffsll(m) - 1 : 0;
+    uint32_t base = 0;
+    if (m && (int)lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ unsigned long long wave_slot64(unsigned long long *ctr, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const int leader = m ? __ffsll(m) - 1 : 0;
+    unsigned long long base = 0;
+    if (m && (int)lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 __global__ void wf_reset_kernel(WfState w, unsigned long long total) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < w.capacity) w.free_q[i] = i;
@@ -78,7 +103,7 @@ __global__ void wf_reset_kernel(WfState w, unsigned long long total) {
         c.n_q[0] = c.n_q[1] = 0;
         c.n_free = w.capacity;
         c.parity = 0;
-        c.guide_count = 0;
+        for (int k = 0; k < kMaxNets; ++k) c.guide_count[k] = 0;
         c.take = 0;
         c.next_path = 0;
         c.total_paths = total;
@@ -128,7 +153,7 @@ __global__ void wf_book_end_kernel(WfState w) {
     WfCounters &c = *w.ctr;
     c.n_q[c.parity] = 0;
     c.parity ^= 1u;
-    c.guide_count = 0;
+    for (int k = 0; k < kMaxNets; ++k) c.guide_count[k] = 0;
 }
 
 template <int BLOCK>
@@ -155,12 +180,24 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
         w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
         w.hit1[p] = make_float4(h.b1, h.b2, 0.f, 0.f);
         int gslot = -1;
-        if (guided && found && job.gc.out) {
+        int net = -1;  // the network whose queue segment takes this vertex
+        if (guided && found && job.gc.out && job.gc.net >= 0) {
             const DevMaterial &m = sc.materials[sc.prim_material[h.prim]];
-            if (guidable(m, job.gc.min_alpha) && w.flags[p].x < job.max_depth) {
+            if (guidable(m, job.gc.min_alpha) && w.flags[p].x < job.max_depth) net = job.gc.net;
+        }
+        // rows in each network's segment, one reservation per wave and network
+        unsigned long long left = __ballot(net >= 0);
+        while (left) {
+            const int lnet = __shfl(net, __ffsll(left) - 1);
+            const bool mine = net == lnet;
+            const uint32_t r = wave_slot(&w.ctr->guide_count[lnet], mine);
+            if (mine) gslot = lnet * (int)w.capacity + (int)r;
+            left &= ~__ballot(mine);
+        }
+        if (gslot >= 0) {
+            {
                 Isect it;
                 hit_isect(sc, h, o, d, it);
-                gslot = (int)atomicAdd(&w.ctr->guide_count, 1u);
                 float *in = w.guide_in + 6 * (size_t)gslot;
                 in[0] = (it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0];
                 in[1] = (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1];
@@ -181,23 +218,35 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
 // Training samples of a finished path: for vertex k with a snapshot, the
 // radiance that arrived along its sampled direction is
 // (L_final - L_snap) / beta_after; the weight is its luminance over the pdf.
-__device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf) {
+// They go to the region of the job's network `net` (-1: none), reserved per
+// wave and network. Every active lane of the wave calls it.
+__device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf, int net) {
     const float lf[3] = {Lf.x, Lf.y, Lf.z};
-    for (uint32_t k = 0; k < n; ++k) {
-        const VertexRec &r = w.recs[(size_t)p * w.max_depth + k];
-        if (r.lsnap[0] < 0.f) continue;
-        float li[3];
-        for (int c = 0; c < 3; ++c) li[c] = r.beta[c] > 0.f ? fmaxf(0.f, lf[c] - r.lsnap[c]) / r.beta[c] : 0.f;
-        const float wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r.pdf;
-        if (!(wgt > 0.f) || !__builtin_isfinite(wgt)) continue;
-        const unsigned long long s = atomicAdd(&w.ctr->rec_count, 1ull);
-        if (s >= w.train_capacity) continue;
-        float *o = w.train_rec + 10 * (size_t)s;
-        for (int c = 0; c < 6; ++c) o[c] = r.in[c];
-        o[6] = r.wi[0];
-        o[7] = r.wi[1];
-        o[8] = r.wi[2];
-        o[9] = wgt;
+    if (net < 0) n = 0;
+    for (uint32_t k = 0; __ballot(k < n); ++k) {
+        float li[3], wgt = 0.f;
+        const VertexRec *r = k < n ? &w.recs[(size_t)p * w.max_depth + k] : nullptr;
+        if (r && r->lsnap[0] >= 0.f) {
+            for (int c = 0; c < 3; ++c) li[c] = r->beta[c] > 0.f ? fmaxf(0.f, lf[c] - r->lsnap[c]) / r->beta[c] : 0.f;
+            wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r->pdf;
+        }
+        const bool valid = r && wgt > 0.f && __builtin_isfinite(wgt);
+        const int vnet = valid ? net : -1;
+        unsigned long long left = __ballot(vnet >= 0);
+        while (left) {
+            const int lnet = __shfl(vnet, __ffsll(left) - 1);
+            const bool mine = vnet == lnet;
+            const unsigned long long s = wave_slot64(&w.ctr->rec_count[lnet], mine);
+            if (mine && s < w.rec_cap_net) {
+                float *o = w.train_rec + 10 * ((size_t)lnet * w.rec_cap_net + s);
+                for (int c = 0; c < 6; ++c) o[c] = r->in[c];
+                o[6] = r->wi[0];
+                o[7] = r->wi[1];
+                o[8] = r->wi[2];
+                o[9] = wgt;
+            }
+            left &= ~__ballot(mine);
+        }
     }
 }
 
@@ -234,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
         w.n_recs[p] = n_recs;
         if (ps.active) {
             store_state(w, p, ps);
-            w.q[parity ^ 1u][atomicAdd(&w.ctr->n_q[parity ^ 1u], 1u)] = p;
+            w.q[parity ^ 1u][wave_slot(&w.ctr->n_q[parity ^ 1u], true)] = p;
         } else {
             const f3 L = guard_radiance(ps.L);
             if (job.filtered) {
@@ -247,10 +296,13 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
                 atomicAdd(px + 1, L.y);
                 atomicAdd(px + 2, L.z);
             }
-            if (recs) emit_records(w, p, n_recs, L);
+            if (recs) emit_records(w, p, n_recs, L, job.gc.net);
             w.flags[p] = make_uint2(ps.bounces, 0u);
-            w.free_q[atomicAdd(&w.ctr->n_free, 1u)] = p;
-            atomicAdd(&w.ctr->done_paths, 1ull);
+            const uint32_t f = wave_slot(&w.ctr->n_free, true);
+            w.free_q[f] = p;
+            // the wave's finished paths counted once
+            const unsigned long long done = __ballot(true);
+            if ((int)__lane_id() == __ffsll(done) - 1) atomicAdd(&w.ctr->done_paths, (unsigned long long)__popcll(done));
         }
     }
     if (counters) {
@@ -267,14 +319,14 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__res
 }
 
 template <int BLOCK>
-hipError_t iteration(const WfState &w, bool guided, const GuideDev *g, float *film, unsigned long long *cnt,
-                     hipStream_t st) {
+hipError_t iteration(const WfState &w, bool guided, float *film, unsigned long long *cnt, hipStream_t st) {
     const uint32_t grid = (w.capacity + BLOCK - 1) / BLOCK;
     wf_regen_kernel<<<(w.capacity + 255) / 256, 256, 0, st>>>(w);
     wf_book_regen_kernel<<<1, 1, 0, st>>>(w);
     wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, guided ? 1 : 0, cnt);
     if (guided) {
-        hipError_t e = guide_launch_infer_count(*g, w.guide_in, &w.ctr->guide_count, w.capacity, w.guide_out, st);
+        hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.guide_in, w.ctr->guide_count, w.capacity,
+                                               w.guide_out, st);
         if (e != hipSuccess) return e;
     }
     wf_shade_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, film, cnt);
@@ -290,21 +342,23 @@ hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hip
 }
 
 namespace {
-__global__ void wf_clear_records_kernel(WfCounters *c) { c->rec_count = 0ull; }
+__global__ void wf_clear_records_kernel(WfCounters *c, int net) {
+    for (int k = 0; k < kMaxNets; ++k)
+        if (net < 0 || k == net) c->rec_count[k] = 0ull;
+}
 }  // namespace
 
-hipError_t wf_launch_clear_records(const WfState &w, hipStream_t st) {
-    wf_clear_records_kernel<<<1, 1, 0, st>>>(w.ctr);
+hipError_t wf_launch_clear_records(const WfState &w, int net, hipStream_t st) {
+    wf_clear_records_kernel<<<1, 1, 0, st>>>(w.ctr, net);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_iteration(const WfState &w, bool guided, const void *guide_dev, float *film,
-                               unsigned long long *ray_counts, int block, hipStream_t st) {
-    const GuideDev *g = (const GuideDev *)guide_dev;
+hipError_t wf_launch_iteration(const WfState &w, bool guided, float *film, unsigned long long *ray_counts, int block,
+                               hipStream_t st) {
     switch (block) {
-        case 64: return iteration<64>(w, guided, g, film, ray_counts, st);
-        case 256: return iteration<256>(w, guided, g, film, ray_counts, st);
-        default: return iteration<128>(w, guided, g, film, ray_counts, st);
+        case 64: return iteration<64>(w, guided, film, ray_counts, st);
+        case 256: return iteration<256>(w, guided, film, ray_counts, st);
+        default: return iteration<128>(w, guided, film, ray_counts, st);
     }
 }
 

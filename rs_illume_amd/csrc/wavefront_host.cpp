@@ -43,6 +43,8 @@ struct illume_wavefront {
     WfState w;
     WfJob *d_jobs = nullptr;
     uint32_t jobs_capacity = 0;
+    GuideDev *d_nets = nullptr;  // kMaxNets entries: the networks of the current batch
+    uint32_t nets_alloc = 1;     // networks the proposal queue (guide_in, guide_out) has room for
     std::vector<void *> allocs;
     std::map<GraphKey, hipGraphExec_t> graphs;
     hipStream_t own_stream = nullptr;  // used when the caller passes the null stream (not capturable)
@@ -107,6 +109,7 @@ int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavef
     if (e == hipSuccess) e = walloc(wf, &w.n_recs, P);
     w.train_capacity = (uint32_t)std::min<size_t>(P * max_depth, 1u << 25);
     if (e == hipSuccess) e = walloc(wf, &w.train_rec, 10 * (size_t)w.train_capacity);
+    if (e == hipSuccess) e = walloc(wf, &wf->d_nets, kMaxNets);
     if (e == hipSuccess) e = walloc(wf, &w.ctr, 1);
     if (e == hipSuccess) e = hipMemset(w.ctr, 0, sizeof(WfCounters));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&wf->own_stream, hipStreamNonBlocking);
@@ -137,17 +140,52 @@ uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf) { return w
 
 namespace {
 
+// Grows the proposal queue to n_nets networks' segments (capacity rows each).
+hipError_t ensure_net_rows(illume_wavefront *wf, uint32_t n_nets) {
+    if (n_nets <= wf->nets_alloc) return hipSuccess;
+    WfState &w = wf->w;
+    for (void *p : {(void *)w.guide_in, (void *)w.guide_out}) {
+        hipFree(p);
+        wf->allocs.erase(std::remove(wf->allocs.begin(), wf->allocs.end(), p), wf->allocs.end());
+    }
+    w.guide_in = w.guide_out = nullptr;
+    wf->nets_alloc = 0;
+    drop_graphs(wf);
+    hipError_t e = walloc(wf, &w.guide_in, 6 * (size_t)w.capacity * n_nets);
+    if (e == hipSuccess) e = walloc(wf, &w.guide_out, 40 * (size_t)w.capacity * n_nets);
+    if (e == hipSuccess) wf->nets_alloc = n_nets;
+    return e;
+}
+
 // The batch render behind the entry points; filters == null: every job
 // unfiltered (3 floats per pixel), else filters[j] (non-null) for job j
-// (4 floats per pixel, splatted).
+// (4 floats per pixel, splatted). guides == null: `guide` (may be null) for
+// every job; else guides[j] (null: no proposal) for job j.
 int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filter *const *filters, uint32_t n_jobs,
-                 illume_guide *guide, const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
+                 illume_guide *guide, illume_guide *const *guides, const illume_guide_cfg *gcfg, float *film,
+                 uint64_t *ray_counts, void *stream) {
     if (!wf || !film || (n_jobs && !jobs)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (gcfg && !(gcfg->guide_frac >= 0.f && gcfg->guide_frac <= 1.f))
         return abi_fail(ILLUME_ERR_INVALID_ARG, "guide_frac must be in [0, 1]");
     if (gcfg && !(gcfg->min_alpha >= 0.f)) return abi_fail(ILLUME_ERR_INVALID_ARG, "min_alpha must be >= 0");
-    const bool guided = guide && gcfg && gcfg->guide_frac > 0.f;
-    const bool train = guide && gcfg && gcfg->train;
+    // the batch's distinct networks in job order; job j uses net_of[j] (-1: none)
+    std::vector<illume_guide *> nets;
+    std::vector<int> net_of(n_jobs, -1);
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        illume_guide *g = guides ? guides[j] : guide;
+        if (!g) continue;
+        auto it = std::find(nets.begin(), nets.end(), g);
+        if (it == nets.end()) {
+            if (nets.size() == (size_t)kMaxNets)
+                return abi_fail(ILLUME_ERR_UNSUPPORTED, "at most 16 distinct proposal networks per batch");
+            nets.push_back(g);
+            net_of[j] = (int)nets.size() - 1;
+        } else {
+            net_of[j] = (int)(it - nets.begin());
+        }
+    }
+    const bool guided = !nets.empty() && gcfg && gcfg->guide_frac > 0.f;
+    const bool train = !nets.empty() && gcfg && gcfg->train;
     const uint32_t batch = (gcfg && gcfg->train_batch) ? gcfg->train_batch : 65536u;
     // Work already queued by the caller (e.g. zeroing the film) completes
     // first; the pipeline then runs on the caller's stream, or on the pool's
@@ -202,7 +240,9 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         job.gc.frac = guided ? gcfg->guide_frac : 0.f;
         job.gc.min_alpha = gcfg ? gcfg->min_alpha : 0.f;
         job.gc.product = gcfg ? (gcfg->flags & ILLUME_GUIDE_PRODUCT) : 0u;
-        job.gc.out = guided ? wf->w.guide_out : nullptr;
+        job.gc.net = guided || train ? net_of[j] : -1;
+        job.gc.out = guided && net_of[j] >= 0 ? wf->w.guide_out : nullptr;
+        if (!(guided && net_of[j] >= 0)) job.gc.frac = 0.f;
         hj.push_back(job);
         total += n;
     }
@@ -220,23 +260,38 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         wf->jobs_capacity = (uint32_t)hj.size();
         drop_graphs(wf);
     }
+    if (guided && (e = ensure_net_rows(wf, (uint32_t)nets.size())) != hipSuccess)
+        return abi_hip_fail(e, "proposal queue allocation");
+    // guide_out moved if the queue grew: the jobs point into it
+    for (WfJob &job : hj)
+        if (job.gc.out) job.gc.out = wf->w.guide_out;
+    std::vector<GuideDev> hnets;
+    for (illume_guide *g : nets) hnets.push_back(*guide_dev(g));
     e = caller ? hipStreamSynchronize(caller) : hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(wf->d_jobs, hj.data(), hj.size() * sizeof(WfJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess && guided)
+        e = hipMemcpy(wf->d_nets, hnets.data(), hnets.size() * sizeof(GuideDev), hipMemcpyHostToDevice);
     if (e != hipSuccess) return abi_hip_fail(e, "job table upload");
     WfState w = wf->w;
     w.jobs = wf->d_jobs;
     w.n_jobs = (uint32_t)hj.size();
     w.regen_min = wf->regen_min;
     if (!train) w.recs = nullptr;
+    w.nets = wf->d_nets;
+    w.n_nets = (uint32_t)nets.size();
+    w.rec_cap_net = w.n_nets ? w.train_capacity / w.n_nets : w.train_capacity;
     unsigned long long *cnt = (unsigned long long *)ray_counts;
-    const void *gdev = guided ? (const void *)guide_dev(guide) : nullptr;
     e = wf_launch_reset(w, total, st);
-    if (e == hipSuccess && train) e = wf_launch_clear_records(w, st);
+    if (e == hipSuccess && train) e = wf_launch_clear_records(w, -1, st);
     // Every path finishes within max_depth + 1 iterations of its start; the
     // pool restarts at least one path per iteration whenever it is idle.
     const uint64_t max_iters = (total / wf->capacity + 2) * (uint64_t)(wf->max_depth + 2) + total + 16;
     const uint32_t k = std::max(1u, wf->graph_iters);
-    const std::string gbytes = guided ? std::string((const char *)gdev, sizeof(GuideDev)) : std::string();
+    // the networks (their count sets the record regions) are part of the key
+    // whenever they are used, for inference or for training records
+    const std::string gbytes = (guided || train)
+                                   ? std::string((const char *)hnets.data(), hnets.size() * sizeof(GuideDev))
+                                   : std::string();
     GraphKey key(guided, train, gbytes, film, cnt, wf->block, w.n_jobs);
     hipGraphExec_t exec = nullptr;
     if (wf->graph_iters > 0 && e == hipSuccess) {
@@ -246,7 +301,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         } else {
             hipGraph_t graph;
             e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, gdev, film, cnt, wf->block, st);
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, film, cnt, wf->block, st);
             hipError_t e2 = hipStreamEndCapture(st, &graph);
             if (e == hipSuccess) e = e2;
             if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -259,25 +314,29 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     while (e == hipSuccess) {
         if (exec) e = hipGraphLaunch(exec, st);
         else
-            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, gdev, film, cnt, wf->block, st);
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, film, cnt, wf->block, st);
         iters += k;
         if (e == hipSuccess) e = hipMemcpyAsync(&c, w.ctr, sizeof(c), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) break;
         const bool finished = c.done_paths >= c.total_paths;
-        if (train && (c.rec_count >= batch || finished) && c.rec_count > 0) {
-            // up to gcfg->train Adam steps per replay; step s reads the
-            // records s, s + S*q, s + 2*S*q, ... (S steps, q = spacing), so
-            // the steps see disjoint, evenly spread subsets
-            const uint32_t n_rec = (uint32_t)std::min<unsigned long long>(c.rec_count, w.train_capacity);
+        for (uint32_t nk = 0; train && nk < w.n_nets && e == hipSuccess; ++nk) {
+            const unsigned long long rc = c.rec_count[nk];
+            if (!((rc >= batch || finished) && rc > 0)) continue;
+            // up to gcfg->train Adam steps per replay for network nk, on its
+            // own records; step s reads the records s, s + S*q, s + 2*S*q,
+            // ... (S steps, q = spacing), so the steps see disjoint, evenly
+            // spread subsets
+            const uint32_t n_rec = (uint32_t)std::min<unsigned long long>(rc, w.rec_cap_net);
             const uint32_t rows = std::min(batch, n_rec);
             const uint32_t steps = std::max(1u, std::min(gcfg->train, n_rec / rows));
             const uint32_t q = n_rec / (rows * steps);
+            const float *region = w.train_rec + 10 * (size_t)nk * w.rec_cap_net;
             for (uint32_t s = 0; s < steps; ++s) {
-                int r = guide_train_strided(guide, w.train_rec + 10 * (size_t)s * q, rows, steps * q, st);
+                int r = guide_train_strided(nets[nk], region + 10 * (size_t)s * q, rows, steps * q, st);
                 if (r != ILLUME_OK) return r;
             }
-            e = wf_launch_clear_records(w, st);
+            e = wf_launch_clear_records(w, (int)nk, st);
         }
         if (finished) break;
         if (iters > max_iters) {
@@ -294,9 +353,16 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
 
 extern "C" {
 
+int illume_wavefront_render_batch_guides(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs,
+                                         illume_guide *const *guides, const illume_guide_cfg *gcfg, float *film,
+                                         uint64_t *ray_counts, void *stream) {
+    if (n_jobs && !guides) return abi_fail(ILLUME_ERR_INVALID_ARG, "guides is null");
+    return render_batch(wf, jobs, nullptr, n_jobs, nullptr, guides, gcfg, film, ray_counts, stream);
+}
+
 int illume_wavefront_render_batch(illume_wavefront *wf, const illume_job *jobs, uint32_t n_jobs, illume_guide *guide,
                                   const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts, void *stream) {
-    return render_batch(wf, jobs, nullptr, n_jobs, guide, gcfg, film, ray_counts, stream);
+    return render_batch(wf, jobs, nullptr, n_jobs, guide, nullptr, gcfg, film, ray_counts, stream);
 }
 
 int illume_wavefront_render_batch_filtered(illume_wavefront *wf, const illume_job *jobs,
@@ -304,7 +370,7 @@ int illume_wavefront_render_batch_filtered(illume_wavefront *wf, const illume_jo
                                            const illume_guide_cfg *gcfg, float *film, uint64_t *ray_counts,
                                            void *stream) {
     if (n_jobs && !filters) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
-    return render_batch(wf, jobs, filters, n_jobs, guide, gcfg, film, ray_counts, stream);
+    return render_batch(wf, jobs, filters, n_jobs, guide, nullptr, gcfg, film, ray_counts, stream);
 }
 
 int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
@@ -317,7 +383,7 @@ int illume_wavefront_render(illume_wavefront *wf, const illume_scene *scene, con
     job.camera = *camera;
     job.cfg = *cfg;
     job.film_offset = 0;
-    return render_batch(wf, &job, nullptr, 1, guide, gcfg, rgb_sum, ray_counts, stream);
+    return render_batch(wf, &job, nullptr, 1, guide, nullptr, gcfg, rgb_sum, ray_counts, stream);
 }
 
 int illume_wavefront_render_filtered(illume_wavefront *wf, const illume_scene *scene, const illume_camera *camera,
@@ -331,7 +397,7 @@ int illume_wavefront_render_filtered(illume_wavefront *wf, const illume_scene *s
     job.camera = *camera;
     job.cfg = *cfg;
     job.film_offset = 0;
-    return render_batch(wf, &job, &filter, 1, guide, gcfg, rgbw_sum, ray_counts, stream);
+    return render_batch(wf, &job, &filter, 1, guide, nullptr, gcfg, rgbw_sum, ray_counts, stream);
 }
 
 }  // extern "C"

@@ -284,6 +284,9 @@ def _declare_wf():
                                           vp]
     L.illume_wavefront_render_batch.restype = i32
     L.illume_wavefront_render_batch.argtypes = [vp, ctypes.POINTER(JobC), u32, vp, ctypes.POINTER(GuideCfg), vp, vp, vp]
+    L.illume_wavefront_render_batch_guides.restype = i32
+    L.illume_wavefront_render_batch_guides.argtypes = [vp, ctypes.POINTER(JobC), u32, ctypes.POINTER(vp),
+                                                       ctypes.POINTER(GuideCfg), vp, vp, vp]
     L.illume_wavefront_render_batch_filtered.restype = i32
     L.illume_wavefront_render_batch_filtered.argtypes = [vp, ctypes.POINTER(JobC), ctypes.POINTER(vp), u32, vp,
                                                          ctypes.POINTER(GuideCfg), vp, vp, vp]
@@ -335,16 +338,33 @@ class Wavefront:
         except Exception:
             pass
 
-    def render_batch(self, jobs, film=None, guide=None, gcfg=None, ray_counts=None, stream=None, filters=None):
+    def render_batch(self, jobs, film=None, guide=None, gcfg=None, ray_counts=None, stream=None, filters=None,
+                     guides=None):
         """jobs: list of (Scene, camera, spp, max_depth, seed[, sample_offset[,
         sampler]]), sampler one of ``SAMPLERS`` (default "random").
         Returns (film, offsets): film is a flat float32 device tensor holding
         every job's radiance sums (H*W*3 each) at its offset. With
         ``filters`` (one :class:`rs_illume_amd.film.Filter` per job) each
         job's image is its filtered film, H*W*4 floats (weighted RGB and
-        weight; :func:`rs_illume_amd.film.resolve` gives the pixels)."""
+        weight; :func:`rs_illume_amd.film.resolve` gives the pixels). With
+        ``guides`` (one :class:`rs_illume_amd.guide.Guide` or None per job)
+        each job has its own proposal network, trained on its own samples
+        (illume_wavefront_render_batch_guides)."""
         if filters is not None and len(filters) != len(jobs):
             raise ValueError("one filter per job")
+        if guides is not None:
+            if len(guides) != len(jobs) or filters is not None or guide is not None:
+                raise ValueError("guides: one per job, without filters or a shared guide")
+            arr, offsets, off = _job_array(jobs, 3)
+            if film is None:
+                film = torch.zeros(off, dtype=torch.float32, device="cuda")
+            gh = (ctypes.c_void_p * len(jobs))(*[(g.handle.value if g is not None else None) for g in guides])
+            gc = gcfg if gcfg is not None else GuideCfg(0.0, 0, 0, 0.0)
+            rc = ctypes.c_void_p(ray_counts.data_ptr()) if ray_counts is not None else None
+            _lib.check(_declare_wf().illume_wavefront_render_batch_guides(
+                self._h, arr, len(jobs), gh, ctypes.byref(gc), ctypes.c_void_p(film.data_ptr()), rc,
+                _stream_handle(stream)), "illume_wavefront_render_batch_guides")
+            return film, offsets
         arr, offsets, off = _job_array(jobs, 3 if filters is None else 4)
         if film is None:
             film = torch.zeros(off, dtype=torch.float32, device="cuda")

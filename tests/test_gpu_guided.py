@@ -166,3 +166,62 @@ def test_trained_product_proposal_unbiased(gpu):
          S.rel_mse(guided, plain))
     assert np.isfinite(guided).all()
     assert abs(guided.mean() - plain.mean()) < 0.01 * plain.mean()
+
+
+def _three_jobs():
+    from rs_illume_amd.render import Scene
+    descs = [S.cornell_box(0), S.cornell_box(1, glossy=True, light_size=0.2, light_L=26.5625), S.cornell_box(2)]
+    scenes = [Scene(d) for d in descs]
+    cams = [S.cornell_camera(40, 40), S.cornell_camera(24, 56), S.cornell_camera(64, 32)]
+    return scenes, [(scenes[0], cams[0], 6, 8, 1), (scenes[1], cams[1], 3, 8, 2), (scenes[2], cams[2], 9, 8, 3)]
+
+
+def test_per_job_networks_route_each_job_to_its_network(gpu):
+    """Frozen networks A and B on jobs 0 and 1, none on job 2, in one batch:
+    each job's image equals its render alone with its own network (or plain);
+    every job naming one network equals the shared-network call."""
+    from rs_illume_amd.guide import Guide
+    from rs_illume_amd.render import GuideCfg, Wavefront
+    _say("case per-job networks")
+    scenes, jobs = _three_jobs()
+    a, b = Guide(seed=4), Guide(seed=5)
+    frozen = GuideCfg(0.5, 0, 0, 0.0, 1)
+    wf = Wavefront(max_paths=4096, max_depth=8, block=128, graph_iters=4)
+    film, offs = wf.render_batch(jobs, gcfg=frozen, guides=[a, b, None])
+    film = film.cpu().numpy()
+    for j, g in enumerate((a, b, None)):
+        one, _ = wf.render_batch([jobs[j]], guide=g, gcfg=frozen)
+        one = one.cpu().numpy()
+        got = film[offs[j]:offs[j] + one.size]
+        assert np.allclose(got, one, rtol=1e-5, atol=1e-5), (j, np.abs(got - one).max())
+    shared, _ = wf.render_batch(jobs, guide=a, gcfg=frozen)
+    same, _ = wf.render_batch(jobs, gcfg=frozen, guides=[a, a, a])
+    assert np.allclose(shared.cpu().numpy(), same.cpu().numpy(), rtol=1e-5, atol=1e-5)
+    with pytest.raises(Exception):
+        many = [Guide(seed=100 + k) for k in range(17)]
+        wf.render_batch([jobs[0]] * 17, gcfg=frozen, guides=many)
+    wf.close()
+
+
+def test_per_job_networks_train_on_their_own_samples(gpu):
+    """Training with a network per job: every network takes Adam steps from
+    its job's samples; a job without a network trains nothing; the trained
+    networks differ from each other and from their initial weights."""
+    from rs_illume_amd.guide import Guide
+    from rs_illume_amd.render import GuideCfg, Wavefront
+    _say("case per-job training")
+    scenes, jobs = _three_jobs()
+    a, b = Guide(seed=6), Guide(seed=6)  # equal initial weights
+    p0 = a.params()
+    gc = GuideCfg(0.5, 8, 4096, 0.0, 1)
+    wf = Wavefront(max_paths=1 << 14, max_depth=8, block=128, graph_iters=4)
+    for k in range(6):
+        wf.render_batch([(sc, cam, spp, d, seed, spp * k) for sc, cam, spp, d, seed in jobs], gcfg=gc,
+                        guides=[a, b, None])
+    sa, sb = a.stats()[0], b.stats()[0]
+    _say("adam steps", sa, sb)
+    assert sa > 0 and sb > 0
+    pa, pb = a.params(), b.params()
+    assert np.isfinite(pa).all() and np.isfinite(pb).all()
+    assert not np.array_equal(pa, p0) and not np.array_equal(pa, pb)
+    wf.close()
