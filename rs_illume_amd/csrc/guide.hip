@@ -187,10 +187,12 @@ __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev 
 __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_nets_kernel(const GuideDev *__restrict__ nets,
                                                                             const float *__restrict__ in,
                                                                             const uint32_t *__restrict__ counts,
-                                                                            uint32_t stride, float *__restrict__ out) {
+                                                                            uint32_t count_stride, uint32_t stride,
+                                                                            float *__restrict__ out) {
     const uint32_t k = blockIdx.y;
     const GuideDev g = nets[k];
-    infer_rows(g, in + 6 * (size_t)k * stride, counts + k, stride, out + (size_t)kOut * k * stride);
+    infer_rows(g, in + 6 * (size_t)k * stride, counts + (size_t)k * count_stride, stride,
+               out + (size_t)kOut * k * stride);
 }
 
 // d(-log q)/d outputs for one row (mixture decode as in the sampler).
@@ -602,10 +604,11 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
 }
 
 hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t stride, float *out, hipStream_t st) {
+                                   uint32_t count_stride, uint32_t stride, float *out, hipStream_t st) {
     if (n_nets == 0 || stride == 0) return hipSuccess;
     const uint32_t blocks = min((stride + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
-    guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, in, counts, stride, out);
+    guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, in, counts, count_stride, stride,
+                                                                                 out);
     return hipGetLastError();
 }
 

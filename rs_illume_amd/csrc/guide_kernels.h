@@ -59,9 +59,10 @@ hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, fl
 hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
                                     float *out, hipStream_t st);
 // Several networks at once: network k (of n_nets, device array nets) infers
-// the counts[k] (device) rows starting at row k * stride of in and out.
+// the counts[k * count_stride] (device) rows starting at row k * stride of in
+// and out.
 hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t stride, float *out, hipStream_t st);
+                                   uint32_t count_stride, uint32_t stride, float *out, hipStream_t st);
 // Gradient of the weighted NLL over n records into grad (zeroed first); wsum
 // (device float) receives the weight sum, loss (optional) the loss. Record r
 // is read at rec + 10 * stride * r (stride >= 1 trains on an even subset).

@@ -77,72 +77,82 @@ __device__ __forceinline__ f3 sky_L(const DevScene &sc, f3 w_in) {
 }
 __device__ __forceinline__ float sky_pdf(f3 wi) { return wi.y > 0.f ? wi.y * kInvPi : 0.f; }
 
-// estimate_direct for the sky: the light sample's shadow ray leaves the scene
-// (any hit over (0, inf)); the BSDF sample counts only if its ray escapes.
-template <int BLOCK>
-__device__ f3 estimate_direct_sky(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1,
-                                  float ul0, float ul1, TraceStack<BLOCK> &st, PathCounters &pc) {
+// The two rays of estimate_direct (integrator.rs:353-494), found before either
+// is traced: the light sample's shadow ray with the term it adds when
+// unoccluded, and the BSDF sample's ray with its factor f |cos| and MIS
+// weight; the radiance that ray finds is looked up once it is traced. The
+// megakernel traces both at once (direct_resolve); the wavefront renderer
+// queues them for a traversal kernel of their own.
+struct DirectRays {
+    f3 so, sd;       // shadow ray: any hit over (0, s_tmax)
+    float s_tmax;
+    f3 cs;           // the light sample's term when the shadow ray is unoccluded
+    f3 mo, md;       // BSDF-sample ray: closest hit over (0, inf)
+    f3 mf;           // f |cos| of the BSDF sample
+    float mw, mpdf;  // its MIS weight and pdf
+    int li;          // the sampled light (sc.sky_index for the sky)
+    bool shadow, mis;
+};
+
+// The sky: the light sample's shadow ray leaves the scene (any hit over
+// (0, inf)); the BSDF sample counts only if its ray escapes.
+__device__ inline void direct_rays_sky(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1,
+                                       float ul0, float ul1, DirectRays &r) {
     const int flags = kAll & ~kSpecular;
-    f3 ld = mk(0.f, 0.f, 0.f);
+    r.li = sc.sky_index;
+    r.shadow = r.mis = false;
     const f3 dl = cosine_sample_hemisphere(ul0, ul1);
     f3 wi = mk(dl.x, dl.z, dl.y);
     float light_pdf = sky_pdf(wi);
     float scattering_pdf = 0.f;
-    f3 li_v = light_pdf > 0.f ? sky_L(sc, wi) : mk(0.f, 0.f, 0.f);
+    const f3 li_v = light_pdf > 0.f ? sky_L(sc, wi) : mk(0.f, 0.f, 0.f);
     if (light_pdf > 0 && !is_black(li_v)) {
         const f3 fv = bsdf.f(it.wo, wi, flags);
         const float ad = abs_dot(wi, it.shading_n);
-        f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
+        const f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
         scattering_pdf = bsdf.pdf(it.wo, wi, flags);
         if (!is_black(f)) {
-            f3 so;
-            spawn_ray(it, wi, so);
-            if (scene_any<BLOCK>(sc, so, wi, __builtin_inff(), st, pc)) li_v = mk(0.f, 0.f, 0.f);
-            if (!is_black(li_v)) {
-                float weight = power_heuristic(light_pdf, scattering_pdf);
-                f3 c = cmul(f, li_v);
-                ld = add(ld, mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf));
-            }
+            spawn_ray(it, wi, r.so);
+            r.sd = wi;
+            r.s_tmax = __builtin_inff();
+            const float weight = power_heuristic(light_pdf, scattering_pdf);
+            const f3 c = cmul(f, li_v);
+            r.cs = mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf);
+            r.shadow = true;
         }
     }
-    {
-        int sampled = 0;
-        f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
-        float ad = abs_dot(wi, it.shading_n);
-        f = mk(f.x * ad, f.y * ad, f.z * ad);
-        bool sampled_specular = (sampled & kSpecular) != 0;
-        if (!is_black(f) && scattering_pdf > 0) {
-            float weight = 1;
-            if (!sampled_specular) {
-                light_pdf = sky_pdf(wi);
-                if (light_pdf == 0) return ld;
-                weight = power_heuristic(scattering_pdf, light_pdf);
-            }
-            f3 so;
-            spawn_ray(it, wi, so);
-            TraceHit h;
-            if (!scene_closest<BLOCK>(sc, so, wi, __builtin_inff(), h, st, pc)) {
-                const f3 li2 = sky_L(sc, wi);
-                if (!is_black(li2)) {
-                    f3 c = cmul(f, li2);
-                    ld = add(ld, mk(c.x * weight / scattering_pdf, c.y * weight / scattering_pdf,
-                                    c.z * weight / scattering_pdf));
-                }
-            }
+    int sampled = 0;
+    f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
+    const float ad = abs_dot(wi, it.shading_n);
+    f = mk(f.x * ad, f.y * ad, f.z * ad);
+    if (!is_black(f) && scattering_pdf > 0) {
+        float weight = 1;
+        if (!(sampled & kSpecular)) {
+            light_pdf = sky_pdf(wi);
+            if (light_pdf == 0) return;
+            weight = power_heuristic(scattering_pdf, light_pdf);
         }
+        spawn_ray(it, wi, r.mo);
+        r.md = wi;
+        r.mf = f;
+        r.mw = weight;
+        r.mpdf = scattering_pdf;
+        r.mis = true;
     }
-    return ld;
 }
 
-// estimate_direct (integrator.rs:353-494) for a one-sided triangle area light,
-// with both MIS strategies (the BSDF-sampled term per pbrt-v3).
-template <int BLOCK>
-__device__ f3 estimate_direct(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1, int li,
-                              float ul0, float ul1, TraceStack<BLOCK> &st, PathCounters &pc) {
-    if (li == sc.sky_index) return estimate_direct_sky<BLOCK>(sc, it, bsdf, us0, us1, ul0, ul1, st, pc);
+// A one-sided triangle area light, with both MIS strategies (the
+// BSDF-sampled term per pbrt-v3).
+__device__ inline void direct_rays(const DevScene &sc, const Isect &it, const Bsdf &bsdf, float us0, float us1, int li,
+                                   float ul0, float ul1, DirectRays &r) {
+    if (li == sc.sky_index) {
+        direct_rays_sky(sc, it, bsdf, us0, us1, ul0, ul1, r);
+        return;
+    }
+    r.li = li;
+    r.shadow = r.mis = false;
     const DevLight &light = sc.lights[li];
     const int flags = kAll & ~kSpecular;
-    f3 ld = mk(0.f, 0.f, 0.f);
     const f3 p0 = ld3(light.p0), p1 = ld3(light.p1), p2 = ld3(light.p2);
     // Light::sample_li -> Triangle::sample (triangle.rs:616-642) and the
     // area-to-solid-angle conversion of shape.rs:43-57.
@@ -172,62 +182,89 @@ __device__ f3 estimate_direct(const DevScene &sc, const Isect &it, const Bsdf &b
     if (light_pdf > 0 && !is_black(li_v)) {
         const f3 fv = bsdf.f(it.wo, wi, flags);
         const float ad = abs_dot(wi, it.shading_n);
-        f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
+        const f3 f = mk(fv.x * ad, fv.y * ad, fv.z * ad);
         scattering_pdf = bsdf.pdf(it.wo, wi, flags);
         if (!is_black(f)) {
-            f3 so, sd;
-            spawn_ray_to(it, lp, lpe, ln, so, sd);
-            if (scene_any<BLOCK>(sc, so, sd, 1.f - 0.0001f, st, pc)) li_v = mk(0.f, 0.f, 0.f);
-            if (!is_black(li_v)) {
-                float weight = power_heuristic(light_pdf, scattering_pdf);
-                f3 c = cmul(f, li_v);
-                ld = add(ld, mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf));
-            }
+            spawn_ray_to(it, lp, lpe, ln, r.so, r.sd);
+            r.s_tmax = 1.f - 0.0001f;
+            const float weight = power_heuristic(light_pdf, scattering_pdf);
+            const f3 c = cmul(f, li_v);
+            r.cs = mk(c.x * weight / light_pdf, c.y * weight / light_pdf, c.z * weight / light_pdf);
+            r.shadow = true;
         }
     }
-    {
-        int sampled = 0;
-        f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
-        float ad = abs_dot(wi, it.shading_n);
-        f = mk(f.x * ad, f.y * ad, f.z * ad);
-        bool sampled_specular = (sampled & kSpecular) != 0;
-        if (!is_black(f) && scattering_pdf > 0) {
-            float weight = 1;
-            if (!sampled_specular) {
-                // Shape::pdf_with_point (shape.rs:60-81): the light's own triangle only
-                f3 so;
-                spawn_ray(it, wi, so);
-                float t, c0, c1, c2;
-                if (!triangle_test(p0, p1, p2, so, wi, __builtin_inff(), t, c0, c1, c2) ||
-                    triangle_degenerate(p0, p1, p2))
-                    return ld;
-                Isect il;
-                triangle_isect(p0, p1, p2, c0, c1, c2, wi, il);
-                float lpdf = length_squared(sub(it.p, il.p)) / (abs_dot(il.n, neg(wi)) * light.area);
-                if (__builtin_isinf(lpdf)) lpdf = 0.f;
-                light_pdf = lpdf;
-                if (light_pdf == 0) return ld;
-                weight = power_heuristic(scattering_pdf, light_pdf);
-            }
+    int sampled = 0;
+    f3 f = bsdf.sample_f(it.wo, wi, us0, us1, scattering_pdf, flags, sampled);
+    const float ad = abs_dot(wi, it.shading_n);
+    f = mk(f.x * ad, f.y * ad, f.z * ad);
+    if (!is_black(f) && scattering_pdf > 0) {
+        float weight = 1;
+        if (!(sampled & kSpecular)) {
+            // Shape::pdf_with_point (shape.rs:60-81): the light's own triangle only
             f3 so;
             spawn_ray(it, wi, so);
-            TraceHit h;
-            bool found = scene_closest<BLOCK>(sc, so, wi, __builtin_inff(), h, st, pc);
-            f3 li2 = mk(0.f, 0.f, 0.f);
-            if (found && sc.prim_light[h.prim] == li) {
-                Isect lh;
-                hit_isect(sc, h, so, wi, lh);
-                li2 = area_L(light, lh.n, neg(wi));
-            }
-            if (!is_black(li2)) {
-                f3 c = cmul(f, li2);
-                ld = add(ld, mk(c.x * weight / scattering_pdf, c.y * weight / scattering_pdf,
-                                c.z * weight / scattering_pdf));
-            }
+            float t, c0, c1, c2;
+            if (!triangle_test(p0, p1, p2, so, wi, __builtin_inff(), t, c0, c1, c2) ||
+                triangle_degenerate(p0, p1, p2))
+                return;
+            Isect il;
+            triangle_isect(p0, p1, p2, c0, c1, c2, wi, il);
+            float lpdf = length_squared(sub(it.p, il.p)) / (abs_dot(il.n, neg(wi)) * light.area);
+            if (__builtin_isinf(lpdf)) lpdf = 0.f;
+            if (lpdf == 0) return;
+            weight = power_heuristic(scattering_pdf, lpdf);
+        }
+        spawn_ray(it, wi, r.mo);
+        r.md = wi;
+        r.mf = f;
+        r.mw = weight;
+        r.mpdf = scattering_pdf;
+        r.mis = true;
+    }
+}
+
+// Traces the rays of direct_rays and returns estimate_direct's value: the
+// light sample's term if its shadow ray is unoccluded, plus the BSDF sample's
+// if its ray reaches the sampled light first (the sky: escapes the scene).
+template <int BLOCK>
+__device__ inline f3 direct_resolve(const DevScene &sc, const DirectRays &r, TraceStack<BLOCK> &st,
+                                    PathCounters &pc) {
+    f3 ld = mk(0.f, 0.f, 0.f);
+    if (r.shadow && !scene_any<BLOCK>(sc, r.so, r.sd, r.s_tmax, st, pc)) ld = add(ld, r.cs);
+    if (r.mis) {
+        TraceHit h;
+        const bool found = scene_closest<BLOCK>(sc, r.mo, r.md, __builtin_inff(), h, st, pc);
+        f3 li2 = mk(0.f, 0.f, 0.f);
+        if (r.li == sc.sky_index) {
+            if (!found) li2 = sky_L(sc, r.md);
+        } else if (found && sc.prim_light[h.prim] == r.li) {
+            Isect lh;
+            hit_isect(sc, h, r.mo, r.md, lh);
+            li2 = area_L(sc.lights[r.li], lh.n, neg(r.md));
+        }
+        if (!is_black(li2)) {
+            const f3 c = cmul(r.mf, li2);
+            ld = add(ld, mk(c.x * r.mw / r.mpdf, c.y * r.mw / r.mpdf, c.z * r.mw / r.mpdf));
         }
     }
     return ld;
 }
+
+// uniform_sample_one_light's term added to the path radiance:
+// L + beta (ld / lpdf).
+__device__ __forceinline__ f3 add_direct(f3 L, f3 beta, f3 ld, float lpdf) {
+    const f3 ldir = mk(ld.x / lpdf, ld.y / lpdf, ld.z / lpdf);
+    return add(L, cmul(beta, ldir));
+}
+
+// A vertex's direct-lighting rays left for a later traversal stage, with what
+// add_direct needs once they are traced.
+struct DeferredDirect {
+    DirectRays r;
+    f3 beta;
+    float lpdf;
+    bool pending;
+};
 
 
 // PerspectiveCamera::generate_ray (perspective.rs:103-126, lens radius 0) for
@@ -440,10 +477,14 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
 // continuation is one-sample MIS: the mixture with probability frac, else the
 // BSDF, weighted by frac * q + (1 - frac) * p_bsdf. recs (may be null) gets a
 // training vertex per sampled continuation; n_recs counts them.
-template <int BLOCK>
+// DEFER: the direct-lighting rays are not traced here but returned in *dd
+// (dd->pending), for the caller to trace and add with add_direct; st and pc
+// are then unused.
+template <int BLOCK, bool DEFER = false>
 __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
                            const float *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
-                           TraceStack<BLOCK> &st, PathCounters &pc) {
+                           TraceStack<BLOCK> &st, PathCounters &pc, DeferredDirect *dd = nullptr) {
+    if (DEFER) dd->pending = false;
     if (recs && n_recs > 0 && recs[n_recs - 1].lsnap[0] < 0.f) {
         // the radiance gathered so far excludes everything arriving along the
         // last recorded direction
@@ -477,9 +518,16 @@ __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_dept
         if (lpdf != 0) {
             const float ul0 = ps.rng.get_1d(), ul1 = ps.rng.get_1d();
             const float us0 = ps.rng.get_1d(), us1 = ps.rng.get_1d();
-            f3 ldir = estimate_direct<BLOCK>(sc, it, bsdf, us0, us1, ln, ul0, ul1, st, pc);
-            ldir = mk(ldir.x / lpdf, ldir.y / lpdf, ldir.z / lpdf);
-            ps.L = add(ps.L, cmul(ps.beta, ldir));
+            if (DEFER) {
+                direct_rays(sc, it, bsdf, us0, us1, ln, ul0, ul1, dd->r);
+                dd->beta = ps.beta;
+                dd->lpdf = lpdf;
+                dd->pending = true;
+            } else {
+                DirectRays r;
+                direct_rays(sc, it, bsdf, us0, us1, ln, ul0, ul1, r);
+                ps.L = add_direct(ps.L, ps.beta, direct_resolve<BLOCK>(sc, r, st, pc), lpdf);
+            }
         }
     }
     f3 wo = neg(ps.d), wi = mk(0.f, 0.f, 0.f);

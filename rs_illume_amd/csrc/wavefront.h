@@ -27,49 +27,76 @@ struct WfJob {
     FilmSplat fs;
 };
 
+// Batch-wide counters, each on a 256-byte line of its own (the per-tile
+// counts live in WfState's tile arrays).
+constexpr int kCountStride = 64;  // uint32 stride between the networks' proposal-row counts
 struct WfCounters {
-    uint32_t n_q[2];      // lengths of the two slot queues
-    uint32_t n_free;      // free slots
-    uint32_t parity;      // queue read this iteration
-    uint32_t take;        // slots (re)started by the last regenerate stage
-    uint32_t guide_count[kMaxNets];  // rows in each network's segment of the proposal queue
-    unsigned long long next_path, total_paths, done_paths;
-    unsigned long long rec_count[kMaxNets];  // training samples written for each network
+    unsigned long long next_path, total_paths;
+    alignas(256) unsigned long long done_paths;
+    alignas(256) uint32_t guide_count[kMaxNets][kCountStride];  // rows of network k: guide_count[k][0]
+    alignas(256) unsigned long long rec_count[kMaxNets][kCountStride / 2];  // training samples of network k
+};
+
+// Live paths, structure of arrays. The pool is cut into tiles of `tile`
+// slots; tile t (slots t * tile ..) belongs to block t of every stage kernel
+// for the whole batch, holds its live paths at its first positions, and the
+// shade stage writes the survivors compacted into the same tile of the other
+// buffer. No stage uses a batch-wide queue, so no counter is shared by more
+// than one block per iteration, and a tile's state is touched by the same
+// XCD in every stage.
+struct WfPaths {
+    float4 *o_tmax;   // ray origin, t_max
+    float4 *d_eta;    // ray direction, eta scale
+    float4 *beta;     // throughput; w (bits): bounces | specular << 8 | training vertices << 16
+    float4 *L;        // radiance; w (bits): job index
+    ulonglong2 *rng;  // sampler state, increment
+    uint4 *meta;      // sample pixel, film x, film y (bits), training-record slot
 };
 
 struct WfState {
-    float4 *o_tmax, *d_eta, *beta, *L;
-    ulonglong2 *rng;   // PCG32 state, increment
-    uint2 *flags;      // bounces, bit0 active | bit1 specular bounce
-    uint32_t *pix, *job;
-    float2 *film_xy;   // film position of the path's camera sample (filtered jobs)
-    float4 *hit0, *hit1;
-    int32_t *guide_slot;     // row of the proposal queue, -1 if none
+    WfPaths s[2];
+    float4 *hit0, *hit1;     // closest hit of slot p: prim, slot, t, b0 / b1, b2, proposal row (bits)
+    // paths that ended this iteration, from each tile's first slot
+    float4 *fin_L;           // radiance; w (bits): job index
+    uint4 *fin_meta;         // as WfPaths::meta
+    uint32_t *fin_nrec;      // training vertices
+    // direct-lighting requests (DeferredDirect), from each tile's first slot:
+    // [0] shadow origin, t_max  [1] shadow direction, light-selection pdf
+    // [2] shadow term, flags (bit 0 shadow ray, bit 1 BSDF-sample ray) | light << 2
+    // [3] path throughput, destination (bits: slot in the next state buffer,
+    //     or bit 31 | slot in the finish buffer)
+    // [4] BSDF-sample origin, MIS weight  [5] its direction, pdf  [6] f |cos|
+    float4 *nee[7];
+    uint32_t *tile_live[2];  // per tile: live paths in buffer b
+    uint32_t *tile_nee;      // per tile: direct-lighting requests of this iteration
+    uint32_t *tile_fin;      // per tile: paths that ended this iteration
+    uint32_t *tile_free;     // per tile: free training-record slots
+    uint32_t *rec_free;      // per tile: its free record slots (a stack from the tile's first slot)
     float *guide_in;         // [n_nets][capacity][6]: network k's rows start at row k * capacity
     float *guide_out;        // [n_nets][capacity][40]
     const GuideDev *nets;    // device array of the batch's networks
     uint32_t n_nets;
-    uint32_t *q[2];          // slot queues
-    uint32_t *free_q;        // free slots
-    VertexRec *recs;         // [capacity][max_depth], null when not training
-    uint32_t *n_recs;
+    VertexRec *recs;         // [capacity][max_depth] by record slot, null when not training
     float *train_rec;        // [train_capacity][10]: network k's records from k * rec_cap_net
     uint32_t train_capacity;
     uint32_t rec_cap_net;    // record capacity of each network's region
     uint32_t capacity, max_depth;
-    uint32_t regen_min;      // regenerate only when fewer than this many slots are active
+    uint32_t tile, n_tiles;  // slots per tile (4 x block), tiles
+    uint32_t regen_thr;      // a tile starts new paths when fewer than this many are live
     const WfJob *jobs;
     uint32_t n_jobs;
     WfCounters *ctr;
 };
 
-// Resets the counters for a new batch of total_paths paths: all slots free.
-hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hipStream_t st);
+// Resets the counters and tiles for a new batch of total_paths paths (every
+// tile empty); w.tile and w.n_tiles set for `block`.
+hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, int block, hipStream_t st);
 // Clears the training-record counter of network `net`, or of every network
 // for net < 0 (a kernel, so it may sit between graph replays).
 hipError_t wf_launch_clear_records(const WfState &w, int net, hipStream_t st);
-// One iteration: regenerate, extend, [infer], shade, bookkeeping.
-hipError_t wf_launch_iteration(const WfState &w, bool guided, float *film, unsigned long long *ray_counts, int block,
-                               hipStream_t st);
+// One iteration reading state buffer `parity`: regenerate, extend, [infer],
+// shade, direct lighting, finish.
+hipError_t wf_launch_iteration(const WfState &w, uint32_t parity, bool guided, float *film,
+                               unsigned long long *ray_counts, int block, hipStream_t st);
 
 }  // namespace illume

@@ -1,17 +1,30 @@
 // Wavefront path tracer: the bounce loop of PathIntegrator::li split into
 // per-stage kernels over a pool of path slots whose state lives in HBM as
-// structure-of-arrays, with compacted slot queues and path regeneration
-// (Laine, Karras, Aila 2013), so that
-//   * lanes only ever hold live paths (the shade stage compacts survivors
-//     into the next queue),
+// structure-of-arrays, with path regeneration (Laine, Karras, Aila 2013), so
+// that
+//   * lanes only ever hold live paths (the shade stage compacts survivors),
 //   * finished slots are refilled with new camera paths from any job of the
 //     batch (scenes and cameras of very different sizes share one pool),
 //   * a batched MFMA stage (the learned proposal's inference) runs between
 //     the closest-hit query and shading.
 //
-// One iteration = regenerate -> extend -> [infer] -> shade -> bookkeeping.
-// Every kernel reads its work counts from device memory and exits early, so
-// the host launches fixed grids and can capture iterations in a hipGraph.
+// One iteration = regenerate -> extend -> [infer] -> shade -> direct
+// lighting -> finish. Shading traces nothing: the continuation ray is the next
+// iteration's extend stage, and the two rays of the direct-lighting estimate
+// (the light sample's shadow ray, the BSDF sample's ray toward the light) are
+// traced by a kernel that holds only traversal state; paths that ended are
+// filmed after it, once their last vertex's direct light is in.
+//
+// The pool is cut into tiles of 4 x block slots, and block t of every stage
+// kernel owns tile t for the whole batch: it reads its tile's counts, keeps
+// its live paths at the tile's first slots (survivors compacted with a
+// block-wide scan into the other state buffer), and writes its requests and
+// finished paths from the tile's first slot. Only starting new paths, counting
+// finished ones and reserving proposal rows touch batch-wide counters, once
+// per block. Every kernel launches one block per tile and reads its work
+// counts from device memory, so the host can capture iterations in a
+// hipGraph (parity, which buffer holds the live paths, alternates with the
+// iteration and is a launch argument).
 #include <hip/hip_runtime.h>
 #include "guide_kernels.h"
 #include "kernels.h"
@@ -24,20 +37,24 @@ namespace illume {
 
 namespace {
 
-__device__ __forceinline__ void store_state(const WfState &w, uint32_t p, const PathState &ps) {
-    w.o_tmax[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.t_max);
-    w.d_eta[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.eta_scale);
-    w.beta[p] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, 0.f);
-    w.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.f);
-    w.rng[p] = make_ulonglong2(ps.rng.state, ps.rng.inc);
-    w.flags[p] = make_uint2(ps.bounces, (ps.active ? 1u : 0u) | (ps.specular ? 2u : 0u));
+constexpr uint32_t kSpecularBit = 1u << 8;
+constexpr uint32_t kFinishDest = 1u << 31;  // direct-lighting destination in the finish buffer
+constexpr int kTileRounds = 4;             // slots per tile = kTileRounds x block
+
+__device__ __forceinline__ uint32_t path_bits(const PathState &ps, uint32_t n_recs) {
+    return ps.bounces | (ps.specular ? kSpecularBit : 0u) | (n_recs << 16);
 }
-// table: the job's low-discrepancy table (null for the random streams); the
-// sampler's index and next dimension are the stored state and increment.
-__device__ __forceinline__ void load_state(const WfState &w, uint32_t p, const uint32_t *table, PathState &ps) {
-    const float4 a = w.o_tmax[p], b = w.d_eta[p], c = w.beta[p], l = w.L[p];
-    const ulonglong2 r = w.rng[p];
-    const uint2 f = w.flags[p];
+
+// Path at slot p of buffer s (the sampler's low-discrepancy table comes from
+// the path's job; null for the random streams, whose index and next dimension
+// are the stored state and increment).
+__device__ __forceinline__ void load_path(const WfPaths &s, uint32_t p, const WfJob *jobs, PathState &ps,
+                                          uint32_t &job, uint32_t &n_recs, uint4 &meta) {
+    const float4 a = s.o_tmax[p], b = s.d_eta[p], c = s.beta[p], l = s.L[p];
+    const ulonglong2 r = s.rng[p];
+    meta = s.meta[p];
+    const uint32_t bits = __float_as_uint(c.w);
+    job = __float_as_uint(l.w);
     ps.o = mk(a.x, a.y, a.z);
     ps.t_max = a.w;
     ps.d = mk(b.x, b.y, b.z);
@@ -46,18 +63,21 @@ __device__ __forceinline__ void load_state(const WfState &w, uint32_t p, const u
     ps.L = mk(l.x, l.y, l.z);
     ps.rng.state = r.x;
     ps.rng.inc = r.y;
-    ps.rng.lds = table;
-    ps.bounces = f.x;
-    ps.active = (f.y & 1u) != 0;
-    ps.specular = (f.y & 2u) != 0;
+    ps.rng.lds = jobs[job].sl.table;
+    ps.bounces = bits & 0xFFu;
+    ps.specular = (bits & kSpecularBit) != 0;
+    ps.active = true;
+    n_recs = bits >> 16;
 }
 
-// Slots to (re)start this iteration: all free slots, up to the paths left,
-// once fewer than regen_min slots are live.
-__device__ __forceinline__ uint32_t regen_take(const WfState &w, const WfCounters &c) {
-    if (c.n_q[c.parity] >= w.regen_min) return 0;
-    const unsigned long long rem = c.total_paths - c.next_path;
-    return (uint32_t)min((unsigned long long)c.n_free, rem);
+__device__ __forceinline__ void store_path(const WfPaths &s, uint32_t p, const PathState &ps, uint32_t job,
+                                           uint32_t n_recs, uint4 meta) {
+    s.o_tmax[p] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.t_max);
+    s.d_eta[p] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.eta_scale);
+    s.beta[p] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, __uint_as_float(path_bits(ps, n_recs)));
+    s.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, __uint_as_float(job));
+    s.rng[p] = make_ulonglong2(ps.rng.state, ps.rng.inc);
+    s.meta[p] = meta;
 }
 
 __device__ __forceinline__ uint32_t find_job(const WfState &w, unsigned long long g) {
@@ -70,11 +90,9 @@ __device__ __forceinline__ uint32_t find_job(const WfState &w, unsigned long lon
     return lo;
 }
 
-// A slot in a queue for every lane of the wave (among the active lanes) whose
-// `pred` holds, with one atomicAdd per wave instead of one per lane (millions
-// of lanes adding 1 to one address serialise at the L2): the first such lane
-// reserves them all, each takes base + its rank. Every active lane of the wave
-// must call it (divergent branches call it with their own active lanes).
+// A slot for every lane of the wave (among the active lanes) whose `pred`
+// holds, with one atomicAdd per wave: the first such lane reserves them all,
+// each takes base + its rank. Every active lane of the wave must call it.
 __device__ __forceinline__ uint32_t wave_slot(uint32_t *ctr, bool pred) {
     const unsigned long long m = __ballot(pred);
     const uint32_t lane = __lane_id();
@@ -95,80 +113,146 @@ __device__ __forceinline__ unsigned long long wave_slot64(unsigned long long *ct
     return base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
 }
 
-__global__ void wf_reset_kernel(WfState w, unsigned long long total) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < w.capacity) w.free_q[i] = i;
-    if (i == 0) {
+// Block-wide append to N lists: for each list q, the threads whose f[q]
+// holds get consecutive positions run[q] + (their rank in the block), and
+// run[q] advances by the block's count. Every thread of the block calls it
+// (two barriers); run is the same in every thread.
+template <int BLOCK, int N>
+__device__ __forceinline__ void block_append(const bool (&f)[N], uint32_t (&pos)[N], uint32_t (&run)[N],
+                                             uint32_t *lds) {
+    constexpr int W = BLOCK / 64;
+    const int wave = threadIdx.x / 64;
+    const uint32_t lane = __lane_id();
+    uint32_t rank[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const unsigned long long m = __ballot(f[q]);
+        rank[q] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) lds[q * W + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        uint32_t off = run[q], tot = 0;
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+            const uint32_t c = lds[q * W + v];
+            if (v < wave) off += c;
+            tot += c;
+        }
+        pos[q] = off + rank[q];
+        run[q] += tot;
+    }
+    __syncthreads();
+}
+
+// Adds the block's sum of v (one value per thread) to *dst with one atomic.
+// Every thread of the block calls it.
+template <int BLOCK>
+__device__ __forceinline__ void block_add(unsigned long long *dst, uint32_t v, uint32_t *lds) {
+    constexpr int W = BLOCK / 64;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0) lds[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < W; ++k) t += lds[k];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_reset_kernel(WfState w, unsigned long long total) {
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    if (w.recs)
+        for (uint32_t j = threadIdx.x; j < S; j += BLOCK) w.rec_free[t * S + j] = t * S + j;
+    if (threadIdx.x == 0) {
+        w.tile_live[0][t] = w.tile_live[1][t] = 0;
+        w.tile_free[t] = S;
+    }
+    if (t == 0 && threadIdx.x == 0) {
         WfCounters &c = *w.ctr;
-        c.n_q[0] = c.n_q[1] = 0;
-        c.n_free = w.capacity;
-        c.parity = 0;
-        for (int k = 0; k < kMaxNets; ++k) c.guide_count[k] = 0;
-        c.take = 0;
         c.next_path = 0;
         c.total_paths = total;
         c.done_paths = 0;
     }
 }
 
-__global__ void wf_regen_kernel(WfState w) {
-    const WfCounters c = *w.ctr;
-    const uint32_t take = regen_take(w, c);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= take) return;
-    const uint32_t slot = w.free_q[c.n_free - take + i];
-    const unsigned long long g = c.next_path + i;
-    const uint32_t j = find_job(w, g);
-    const WfJob &job = w.jobs[j];
-    // filtered jobs sample the filter's sample bounds (stream = index in them),
-    // the others one sample pixel per image pixel
-    const bool filt = job.filtered != 0;
-    const unsigned long long n_pix =
-        filt ? (unsigned long long)job.fs.wext * job.fs.hext : (unsigned long long)job.cam.width * job.cam.height;
-    const unsigned long long local = g - job.path_begin;
-    const uint64_t pix = local % n_pix;
-    const int x = filt ? job.fs.x0 + (int)(pix % job.fs.wext) : (int)(pix % job.cam.width);
-    const int y = filt ? job.fs.y0 + (int)(pix / job.fs.wext) : (int)(pix / job.cam.width);
-    PathState ps;
-    float fx, fy;
-    path_start_at(job.cam, job.seed, pix, x, y, job.sample_offset + (uint32_t)(local / n_pix), ps, fx, fy, job.sl);
-    store_state(w, slot, ps);
-    w.film_xy[slot] = make_float2(fx, fy);
-    w.pix[slot] = (uint32_t)pix;
-    w.job[slot] = j;
-    w.n_recs[slot] = 0;
-    w.q[c.parity][c.n_q[c.parity] + i] = slot;
-}
-
-__global__ void wf_book_regen_kernel(WfState w) {
-    WfCounters &c = *w.ctr;
-    const uint32_t take = regen_take(w, c);
-    c.n_free -= take;
-    c.next_path += take;
-    c.n_q[c.parity] += take;
-    c.take = take;
-}
-
-__global__ void wf_book_end_kernel(WfState w) {
-    WfCounters &c = *w.ctr;
-    c.n_q[c.parity] = 0;
-    c.parity ^= 1u;
-    for (int k = 0; k < kMaxNets; ++k) c.guide_count[k] = 0;
-}
-
+// New camera paths after tile t's live ones in buffer `parity`, once fewer
+// than regen_thr are live: a run of path indices from the batch-wide counter
+// (one atomic per block). Block 0 also clears the proposal-row counts.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
+__global__ __launch_bounds__(BLOCK) void wf_regen_kernel(WfState w, uint32_t parity) {
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_take;
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    const uint32_t live = w.tile_live[parity][t];
+    if (threadIdx.x == 0) {
+        uint32_t take = 0;
+        unsigned long long base = 0;
+        WfCounters &c = *w.ctr;
+        const unsigned long long total = c.total_paths;
+        if (live < w.regen_thr && c.next_path < total) {
+            const uint32_t want = S - live;
+            base = atomicAdd(&c.next_path, (unsigned long long)want);
+            if (base < total) take = (uint32_t)min((unsigned long long)want, total - base);
+        }
+        s_base = base;
+        s_take = take;
+        if (t == 0)
+            for (int k = 0; k < kMaxNets; ++k) c.guide_count[k][0] = 0;
+    }
+    __syncthreads();
+    const uint32_t take = s_take;
+    if (take == 0) return;
+    const unsigned long long base = s_base;
+    const uint32_t free0 = w.recs ? w.tile_free[t] : 0u;
+    for (uint32_t i = threadIdx.x; i < take; i += BLOCK) {
+        const unsigned long long g = base + i;
+        const uint32_t j = find_job(w, g);
+        const WfJob &job = w.jobs[j];
+        // filtered jobs sample the filter's sample bounds (stream = index in
+        // them), the others one sample pixel per image pixel
+        const bool filt = job.filtered != 0;
+        const unsigned long long n_pix =
+            filt ? (unsigned long long)job.fs.wext * job.fs.hext : (unsigned long long)job.cam.width * job.cam.height;
+        const unsigned long long local = g - job.path_begin;
+        const uint64_t pix = local % n_pix;
+        const int x = filt ? job.fs.x0 + (int)(pix % job.fs.wext) : (int)(pix % job.cam.width);
+        const int y = filt ? job.fs.y0 + (int)(pix / job.fs.wext) : (int)(pix / job.cam.width);
+        PathState ps;
+        float fx, fy;
+        path_start_at(job.cam, job.seed, pix, x, y, job.sample_offset + (uint32_t)(local / n_pix), ps, fx, fy,
+                      job.sl);
+        const uint32_t rec = w.recs ? w.rec_free[t * S + free0 - take + i] : 0u;
+        store_path(w.s[parity], t * S + live + i, ps, j, 0u,
+                   make_uint4((uint32_t)pix, __float_as_uint(fx), __float_as_uint(fy), rec));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        w.tile_live[parity][t] = live + take;
+        if (w.recs) w.tile_free[t] = free0 - take;
+    }
+}
+
+// Closest hit of every live path's ray; guided vertices get a row in their
+// network's segment of the proposal queue (one reservation per wave and
+// network).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t parity, int guided,
                                                           unsigned long long *__restrict__ counters) {
     __shared__ int lds_stack[kStackLds * BLOCK];
-    const uint32_t parity = w.ctr->parity;
-    const uint32_t n_in = w.ctr->n_q[parity];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ uint32_t lds_red[BLOCK / 64];
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    const uint32_t n_in = w.tile_live[parity][t];
+    const WfPaths &s = w.s[parity];
     uint32_t n_closest = 0;
-    if (i < n_in) {
-        const uint32_t p = w.q[parity][i];
-        const WfJob &job = w.jobs[w.job[p]];
+    for (uint32_t i = threadIdx.x; i < n_in; i += BLOCK) {
+        const uint32_t p = t * S + i;
+        const float4 a = s.o_tmax[p], b = s.d_eta[p];
+        const WfJob &job = w.jobs[__float_as_uint(s.L[p].w)];
         const DevScene &sc = *job.scene;
-        const float4 a = w.o_tmax[p], b = w.d_eta[p];
         int spill[kSpill];
         TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x, spill);
@@ -177,55 +261,50 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, int guided,
         const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
         const bool found = trace_ray4<false, false, BLOCK>(sc.nodes, sc.prims, o, d, a.w, h, st, nb, np);
         ++n_closest;
-        w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
-        w.hit1[p] = make_float4(h.b1, h.b2, 0.f, 0.f);
         int gslot = -1;
         int net = -1;  // the network whose queue segment takes this vertex
         if (guided && found && job.gc.out && job.gc.net >= 0) {
             const DevMaterial &m = sc.materials[sc.prim_material[h.prim]];
-            if (guidable(m, job.gc.min_alpha) && w.flags[p].x < job.max_depth) net = job.gc.net;
+            const uint32_t bounces = __float_as_uint(s.beta[p].w) & 0xFFu;
+            if (guidable(m, job.gc.min_alpha) && bounces < job.max_depth) net = job.gc.net;
         }
-        // rows in each network's segment, one reservation per wave and network
         unsigned long long left = __ballot(net >= 0);
         while (left) {
             const int lnet = __shfl(net, __ffsll(left) - 1);
             const bool mine = net == lnet;
-            const uint32_t r = wave_slot(&w.ctr->guide_count[lnet], mine);
+            const uint32_t r = wave_slot(&w.ctr->guide_count[lnet][0], mine);
             if (mine) gslot = lnet * (int)w.capacity + (int)r;
             left &= ~__ballot(mine);
         }
         if (gslot >= 0) {
-            {
-                Isect it;
-                hit_isect(sc, h, o, d, it);
-                float *in = w.guide_in + 6 * (size_t)gslot;
-                in[0] = (it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0];
-                in[1] = (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1];
-                in[2] = (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2];
-                in[3] = it.n.x;
-                in[4] = it.n.y;
-                in[5] = it.n.z;
-            }
+            Isect it;
+            hit_isect(sc, h, o, d, it);
+            float *in = w.guide_in + 6 * (size_t)gslot;
+            in[0] = (it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0];
+            in[1] = (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1];
+            in[2] = (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2];
+            in[3] = it.n.x;
+            in[4] = it.n.y;
+            in[5] = it.n.z;
         }
-        w.guide_slot[p] = gslot;
+        w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
+        w.hit1[p] = make_float4(h.b1, h.b2, __int_as_float(gslot), 0.f);
     }
-    if (counters) {
-        for (int off = 32; off > 0; off >>= 1) n_closest += __shfl_xor(n_closest, off);
-        if ((threadIdx.x & 63) == 0 && n_closest) atomicAdd(counters, (unsigned long long)n_closest);
-    }
+    if (counters) block_add<BLOCK>(counters, n_closest, lds_red);
 }
 
-// Training samples of a finished path: for vertex k with a snapshot, the
-// radiance that arrived along its sampled direction is
-// (L_final - L_snap) / beta_after; the weight is its luminance over the pdf.
-// They go to the region of the job's network `net` (-1: none), reserved per
-// wave and network. Every active lane of the wave calls it.
-__device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf, int net) {
+// Training samples of a finished path whose vertices are in record slot
+// `rec`: for vertex k with a snapshot, the radiance that arrived along its
+// sampled direction is (L_final - L_snap) / beta_after; the weight is its
+// luminance over the pdf. They go to the region of the job's network `net`
+// (-1: none), reserved per wave and network. Every active lane of the wave
+// calls it.
+__device__ void emit_records(const WfState &w, uint32_t rec, uint32_t n, f3 Lf, int net) {
     const float lf[3] = {Lf.x, Lf.y, Lf.z};
     if (net < 0) n = 0;
     for (uint32_t k = 0; __ballot(k < n); ++k) {
         float li[3], wgt = 0.f;
-        const VertexRec *r = k < n ? &w.recs[(size_t)p * w.max_depth + k] : nullptr;
+        const VertexRec *r = k < n ? &w.recs[(size_t)rec * w.max_depth + k] : nullptr;
         if (r && r->lsnap[0] >= 0.f) {
             for (int c = 0; c < 3; ++c) li[c] = r->beta[c] > 0.f ? fmaxf(0.f, lf[c] - r->lsnap[c]) / r->beta[c] : 0.f;
             wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r->pdf;
@@ -236,7 +315,7 @@ __device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf, in
         while (left) {
             const int lnet = __shfl(vnet, __ffsll(left) - 1);
             const bool mine = vnet == lnet;
-            const unsigned long long s = wave_slot64(&w.ctr->rec_count[lnet], mine);
+            const unsigned long long s = wave_slot64(&w.ctr->rec_count[lnet][0], mine);
             if (mine && s < w.rec_cap_net) {
                 float *o = w.train_rec + 10 * ((size_t)lnet * w.rec_cap_net + s);
                 for (int c = 0; c < 6; ++c) o[c] = r->in[c];
@@ -250,115 +329,221 @@ __device__ void emit_records(const WfState &w, uint32_t p, uint32_t n, f3 Lf, in
     }
 }
 
+// One bounce of tile t's live paths, tracing nothing. GUIDED (a proposal or
+// training records in use) reads the proposal rows and keeps training
+// vertices; the plain kernel compiles without them. Survivors go compacted to
+// tile t of the other state buffer, ended paths to the finish buffer, and a
+// vertex's direct-lighting rays to the request buffer, each from the tile's
+// first slot.
+template <int BLOCK, bool GUIDED>
+__global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, uint32_t parity) {
+    __shared__ uint32_t lds[3 * (BLOCK / 64)];
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    const uint32_t n_in = w.tile_live[parity][t];
+    const uint32_t nxt = parity ^ 1u;
+    uint32_t run[3] = {0u, 0u, 0u};  // survivors, ended, requests
+    for (uint32_t r0 = 0; r0 < n_in; r0 += BLOCK) {
+        const uint32_t i = r0 + threadIdx.x;
+        bool f[3] = {false, false, false};
+        PathState ps;
+        uint32_t j = 0, n_recs = 0;
+        uint4 meta;
+        DeferredDirect dd;
+        if (i < n_in) {
+            const uint32_t p = t * S + i;
+            load_path(w.s[parity], p, w.jobs, ps, j, n_recs, meta);
+            const WfJob &job = w.jobs[j];
+            const float4 h0 = w.hit0[p], h1 = w.hit1[p];
+            TraceHit h;
+            h.prim = __float_as_uint(h0.x);
+            h.slot = __float_as_uint(h0.y);
+            h.t = h0.z;
+            h.b0 = h0.w;
+            h.b1 = h1.x;
+            h.b2 = h1.y;
+            const bool found = h.prim != 0xFFFFFFFFu;
+            const float *gout = nullptr;
+            VertexRec *recs = nullptr;
+            if (GUIDED) {
+                const int gslot = __float_as_int(h1.z);
+                gout = (job.gc.out && gslot >= 0) ? job.gc.out + 40 * (size_t)gslot : nullptr;
+                if (w.recs) recs = w.recs + (size_t)meta.w * w.max_depth;
+            }
+            TraceStack<BLOCK> st;  // unused: shading traces nothing
+            PathCounters pc{0, 0};
+            path_shade<BLOCK, true>(*job.scene, job.max_depth, ps, found, h, gout, job.gc, recs, n_recs, st, pc,
+                                    &dd);
+            f[0] = ps.active;
+            f[1] = !ps.active;
+            // a non-finite throughput turns even a zero direct term into NaN,
+            // as in the megakernel, so such a vertex is queued as well
+            f[2] = dd.pending &&
+                   (dd.r.shadow || dd.r.mis || !__builtin_isfinite(dd.beta.x + dd.beta.y + dd.beta.z));
+        }
+        uint32_t pos[3];
+        block_append<BLOCK, 3>(f, pos, run, lds);
+        if (f[0]) store_path(w.s[nxt], t * S + pos[0], ps, j, n_recs, meta);
+        if (f[1]) {
+            w.fin_L[t * S + pos[1]] = make_float4(ps.L.x, ps.L.y, ps.L.z, __uint_as_float(j));
+            w.fin_meta[t * S + pos[1]] = meta;
+            w.fin_nrec[t * S + pos[1]] = n_recs;
+        }
+        if (f[2]) {
+            const DirectRays &r = dd.r;
+            const uint32_t q = t * S + pos[2];
+            const uint32_t bits = (r.shadow ? 1u : 0u) | (r.mis ? 2u : 0u) | ((uint32_t)r.li << 2);
+            const uint32_t dest = f[0] ? t * S + pos[0] : (kFinishDest | (t * S + pos[1]));
+            if (r.shadow) w.nee[0][q] = make_float4(r.so.x, r.so.y, r.so.z, r.s_tmax);
+            w.nee[1][q] = make_float4(r.sd.x, r.sd.y, r.sd.z, dd.lpdf);
+            w.nee[2][q] = make_float4(r.cs.x, r.cs.y, r.cs.z, __uint_as_float(bits));
+            w.nee[3][q] = make_float4(dd.beta.x, dd.beta.y, dd.beta.z, __uint_as_float(dest));
+            if (r.mis) {
+                w.nee[4][q] = make_float4(r.mo.x, r.mo.y, r.mo.z, r.mw);
+                w.nee[5][q] = make_float4(r.md.x, r.md.y, r.md.z, r.mpdf);
+                w.nee[6][q] = make_float4(r.mf.x, r.mf.y, r.mf.z, 0.f);
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        w.tile_live[nxt][t] = run[0];
+        w.tile_fin[t] = run[1];
+        w.tile_nee[t] = run[2];
+    }
+}
+
+// Tile t's direct-lighting requests: the shadow ray (any hit) and the BSDF
+// sample's ray (closest hit), then L += beta (ld / lpdf) into the radiance of
+// the request's path (in the next state buffer or the finish buffer).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, float *__restrict__ film,
-                                                         unsigned long long *__restrict__ counters) {
+__global__ __launch_bounds__(BLOCK) void wf_nee_kernel(WfState w, uint32_t parity,
+                                                       unsigned long long *__restrict__ counters) {
     __shared__ int lds_stack[kStackLds * BLOCK];
-    const uint32_t parity = w.ctr->parity;
-    const uint32_t n_in = w.ctr->n_q[parity];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ uint32_t lds_red[BLOCK / 64];
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    const uint32_t n_in = w.tile_nee[t];
+    float4 *next_L = w.s[parity ^ 1u].L;
     PathCounters pc{0, 0};
-    if (i < n_in) {
-        const uint32_t p = w.q[parity][i];
-        const WfJob &job = w.jobs[w.job[p]];
+    for (uint32_t i = threadIdx.x; i < n_in; i += BLOCK) {
+        const uint32_t q = t * S + i;
+        const float4 n1 = w.nee[1][q], n2 = w.nee[2][q], n3 = w.nee[3][q];
+        const uint32_t dest = __float_as_uint(n3.w);
+        float4 *lp = (dest & kFinishDest) ? w.fin_L + (dest & ~kFinishDest) : next_L + dest;
+        const float4 l = *lp;
+        const DevScene &sc = *w.jobs[__float_as_uint(l.w)].scene;
         int spill[kSpill];
         TraceStack<BLOCK> st;
         st.reset(lds_stack + threadIdx.x, spill);
-        PathState ps;
-        load_state(w, p, job.sl.table, ps);
-        const float4 h0 = w.hit0[p], h1 = w.hit1[p];
-        TraceHit h;
-        h.prim = __float_as_uint(h0.x);
-        h.slot = __float_as_uint(h0.y);
-        h.t = h0.z;
-        h.b0 = h0.w;
-        h.b1 = h1.x;
-        h.b2 = h1.y;
-        const bool found = h.prim != 0xFFFFFFFFu;
-        const int gslot = w.guide_slot[p];
-        const float *gout = (job.gc.out && gslot >= 0) ? job.gc.out + 40 * (size_t)gslot : nullptr;
-        uint32_t n_recs = w.n_recs[p];
-        VertexRec *recs = w.recs ? w.recs + (size_t)p * w.max_depth : nullptr;
-        path_shade<BLOCK>(*job.scene, job.max_depth, ps, found, h, gout, job.gc, recs, n_recs, st, pc);
-        w.n_recs[p] = n_recs;
-        if (ps.active) {
-            store_state(w, p, ps);
-            w.q[parity ^ 1u][wave_slot(&w.ctr->n_q[parity ^ 1u], true)] = p;
-        } else {
-            const f3 L = guard_radiance(ps.L);
-            if (job.filtered) {
-                const float2 fxy = w.film_xy[p];
-                film_splat(job.fs, film + job.film_offset, (int)job.cam.width, (int)job.cam.height, fxy.x, fxy.y, L.x,
-                           L.y, L.z);
-            } else {
-                float *px = film + job.film_offset + 3 * (size_t)w.pix[p];
-                atomicAdd(px, L.x);
-                atomicAdd(px + 1, L.y);
-                atomicAdd(px + 2, L.z);
-            }
-            if (recs) emit_records(w, p, n_recs, L, job.gc.net);
-            w.flags[p] = make_uint2(ps.bounces, 0u);
-            const uint32_t f = wave_slot(&w.ctr->n_free, true);
-            w.free_q[f] = p;
-            // the wave's finished paths counted once
-            const unsigned long long done = __ballot(true);
-            if ((int)__lane_id() == __ffsll(done) - 1) atomicAdd(&w.ctr->done_paths, (unsigned long long)__popcll(done));
+        DirectRays r;
+        const uint32_t bits = __float_as_uint(n2.w);
+        r.shadow = (bits & 1u) != 0;
+        r.mis = (bits & 2u) != 0;
+        r.li = (int)(bits >> 2);
+        if (r.shadow) {
+            const float4 n0 = w.nee[0][q];
+            r.so = mk(n0.x, n0.y, n0.z);
+            r.s_tmax = n0.w;
+            r.sd = mk(n1.x, n1.y, n1.z);
+            r.cs = mk(n2.x, n2.y, n2.z);
         }
+        if (r.mis) {
+            const float4 n4 = w.nee[4][q], n5 = w.nee[5][q], n6 = w.nee[6][q];
+            r.mo = mk(n4.x, n4.y, n4.z);
+            r.mw = n4.w;
+            r.md = mk(n5.x, n5.y, n5.z);
+            r.mpdf = n5.w;
+            r.mf = mk(n6.x, n6.y, n6.z);
+        }
+        const f3 ld = direct_resolve<BLOCK>(sc, r, st, pc);
+        const f3 L = add_direct(mk(l.x, l.y, l.z), mk(n3.x, n3.y, n3.z), ld, n1.w);
+        *lp = make_float4(L.x, L.y, L.z, l.w);
     }
     if (counters) {
-        uint32_t c = pc.closest, a = pc.any;
-        for (int off = 32; off > 0; off >>= 1) {
-            c += __shfl_xor(c, off);
-            a += __shfl_xor(a, off);
+        block_add<BLOCK>(counters, pc.closest, lds_red);
+        block_add<BLOCK>(counters + 1, pc.any, lds_red);
+    }
+}
+
+// Tile t's paths that ended this iteration: the guarded radiance into the
+// film (or its splat over the filter footprint), training records, record
+// slots back to the tile's free stack, the block's count added to done_paths.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_finish_kernel(WfState w, float *__restrict__ film) {
+    const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
+    const uint32_t n_in = w.tile_fin[t];
+    const uint32_t free0 = w.recs ? w.tile_free[t] : 0u;
+    for (uint32_t i = threadIdx.x; i < n_in; i += BLOCK) {
+        const uint32_t q = t * S + i;
+        const float4 l = w.fin_L[q];
+        const uint4 meta = w.fin_meta[q];
+        const WfJob &job = w.jobs[__float_as_uint(l.w)];
+        const f3 L = guard_radiance(mk(l.x, l.y, l.z));
+        if (job.filtered) {
+            film_splat(job.fs, film + job.film_offset, (int)job.cam.width, (int)job.cam.height,
+                       __uint_as_float(meta.y), __uint_as_float(meta.z), L.x, L.y, L.z);
+        } else {
+            float *px = film + job.film_offset + 3 * (size_t)meta.x;
+            atomicAdd(px, L.x);
+            atomicAdd(px + 1, L.y);
+            atomicAdd(px + 2, L.z);
         }
-        if ((threadIdx.x & 63) == 0) {
-            if (c) atomicAdd(counters, (unsigned long long)c);
-            if (a) atomicAdd(counters + 1, (unsigned long long)a);
+        if (w.recs) {
+            emit_records(w, meta.w, w.fin_nrec[q], L, job.gc.net);
+            w.rec_free[t * S + free0 + i] = meta.w;
         }
+    }
+    if (threadIdx.x == 0 && n_in) {
+        atomicAdd(&w.ctr->done_paths, (unsigned long long)n_in);
+        if (w.recs) w.tile_free[t] = free0 + n_in;
     }
 }
 
 template <int BLOCK>
-hipError_t iteration(const WfState &w, bool guided, float *film, unsigned long long *cnt, hipStream_t st) {
-    const uint32_t grid = (w.capacity + BLOCK - 1) / BLOCK;
-    wf_regen_kernel<<<(w.capacity + 255) / 256, 256, 0, st>>>(w);
-    wf_book_regen_kernel<<<1, 1, 0, st>>>(w);
-    wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, guided ? 1 : 0, cnt);
+hipError_t iteration(const WfState &w, uint32_t parity, bool guided, float *film, unsigned long long *cnt,
+                     hipStream_t st) {
+    const uint32_t grid = w.n_tiles;
+    wf_regen_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity);
+    wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, guided ? 1 : 0, cnt);
     if (guided) {
-        hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.guide_in, w.ctr->guide_count, w.capacity,
-                                               w.guide_out, st);
+        hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.guide_in, &w.ctr->guide_count[0][0], kCountStride,
+                                               w.capacity, w.guide_out, st);
         if (e != hipSuccess) return e;
     }
-    wf_shade_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, film, cnt);
-    wf_book_end_kernel<<<1, 1, 0, st>>>(w);
+    if (guided || w.recs)
+        wf_shade_kernel<BLOCK, true><<<grid, BLOCK, 0, st>>>(w, parity);
+    else
+        wf_shade_kernel<BLOCK, false><<<grid, BLOCK, 0, st>>>(w, parity);
+    wf_nee_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, cnt);
+    wf_finish_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, film);
     return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, hipStream_t st) {
-    wf_reset_kernel<<<(w.capacity + 255) / 256, 256, 0, st>>>(w, total_paths);
-    return hipGetLastError();
-}
-
-namespace {
 __global__ void wf_clear_records_kernel(WfCounters *c, int net) {
     for (int k = 0; k < kMaxNets; ++k)
-        if (net < 0 || k == net) c->rec_count[k] = 0ull;
+        if (net < 0 || k == net) c->rec_count[k][0] = 0ull;
 }
+
 }  // namespace
+
+hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, int block, hipStream_t st) {
+    switch (block) {
+        case 64: wf_reset_kernel<64><<<w.n_tiles, 64, 0, st>>>(w, total_paths); break;
+        case 256: wf_reset_kernel<256><<<w.n_tiles, 256, 0, st>>>(w, total_paths); break;
+        default: wf_reset_kernel<128><<<w.n_tiles, 128, 0, st>>>(w, total_paths); break;
+    }
+    return hipGetLastError();
+}
 
 hipError_t wf_launch_clear_records(const WfState &w, int net, hipStream_t st) {
     wf_clear_records_kernel<<<1, 1, 0, st>>>(w.ctr, net);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_iteration(const WfState &w, bool guided, float *film, unsigned long long *ray_counts, int block,
-                               hipStream_t st) {
+hipError_t wf_launch_iteration(const WfState &w, uint32_t parity, bool guided, float *film,
+                               unsigned long long *ray_counts, int block, hipStream_t st) {
     switch (block) {
-        case 64: return iteration<64>(w, guided, film, ray_counts, st);
-        case 256: return iteration<256>(w, guided, film, ray_counts, st);
-        default: return iteration<128>(w, guided, film, ray_counts, st);
+        case 64: return iteration<64>(w, parity, guided, film, ray_counts, st);
+        case 256: return iteration<256>(w, parity, guided, film, ray_counts, st);
+        default: return iteration<128>(w, parity, guided, film, ray_counts, st);
     }
 }
 

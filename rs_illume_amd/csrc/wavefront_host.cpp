@@ -52,6 +52,10 @@ struct illume_wavefront {
 };
 
 namespace {
+// Pool sizes are whole multiples of the largest tile (4 x 256 slots); the
+// per-tile count arrays have room for the smallest tiles (4 x 64: 16 per
+// quantum).
+constexpr uint32_t kTileQuantum = 1024;
 template <typename T>
 hipError_t walloc(illume_wavefront *wf, T **p, size_t count) {
     hipError_t e = hipMalloc(p, count * sizeof(T));
@@ -82,31 +86,40 @@ int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavef
     if (max_depth == 0 || max_depth > 64) return abi_fail(ILLUME_ERR_UNSUPPORTED, "max_depth in 1..64");
     illume_wavefront *wf = new (std::nothrow) illume_wavefront();
     if (!wf) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
+    // a whole number of tiles for every block size (tiles of 4 x block slots)
+    max_paths = (max_paths + kTileQuantum - 1) / kTileQuantum * kTileQuantum;
     wf->capacity = max_paths;
     wf->max_depth = max_depth;
     wf->regen_min = max_paths + 1;  // regenerate every iteration
     WfState &w = wf->w;
     std::memset(&w, 0, sizeof(w));
     const size_t P = max_paths;
-    hipError_t e = walloc(wf, &w.o_tmax, P);
-    if (e == hipSuccess) e = walloc(wf, &w.d_eta, P);
-    if (e == hipSuccess) e = walloc(wf, &w.beta, P);
-    if (e == hipSuccess) e = walloc(wf, &w.L, P);
-    if (e == hipSuccess) e = walloc(wf, &w.rng, P);
-    if (e == hipSuccess) e = walloc(wf, &w.flags, P);
-    if (e == hipSuccess) e = walloc(wf, &w.pix, P);
-    if (e == hipSuccess) e = walloc(wf, &w.job, P);
-    if (e == hipSuccess) e = walloc(wf, &w.film_xy, P);
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2; ++b) {
+        WfPaths &s = w.s[b];
+        if (e == hipSuccess) e = walloc(wf, &s.o_tmax, P);
+        if (e == hipSuccess) e = walloc(wf, &s.d_eta, P);
+        if (e == hipSuccess) e = walloc(wf, &s.beta, P);
+        if (e == hipSuccess) e = walloc(wf, &s.L, P);
+        if (e == hipSuccess) e = walloc(wf, &s.rng, P);
+        if (e == hipSuccess) e = walloc(wf, &s.meta, P);
+    }
     if (e == hipSuccess) e = walloc(wf, &w.hit0, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit1, P);
-    if (e == hipSuccess) e = walloc(wf, &w.guide_slot, P);
+    if (e == hipSuccess) e = walloc(wf, &w.fin_L, P);
+    if (e == hipSuccess) e = walloc(wf, &w.fin_meta, P);
+    if (e == hipSuccess) e = walloc(wf, &w.fin_nrec, P);
+    for (int k = 0; k < 7; ++k)
+        if (e == hipSuccess) e = walloc(wf, &w.nee[k], P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_in, 6 * P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_out, 40 * P);
-    if (e == hipSuccess) e = walloc(wf, &w.q[0], P);
-    if (e == hipSuccess) e = walloc(wf, &w.q[1], P);
-    if (e == hipSuccess) e = walloc(wf, &w.free_q, P);
     if (e == hipSuccess) e = walloc(wf, &w.recs, P * max_depth);
-    if (e == hipSuccess) e = walloc(wf, &w.n_recs, P);
+    if (e == hipSuccess) e = walloc(wf, &w.rec_free, P);
+    for (int b = 0; b < 2; ++b)
+        if (e == hipSuccess) e = walloc(wf, &w.tile_live[b], P / kTileQuantum * 16);
+    if (e == hipSuccess) e = walloc(wf, &w.tile_nee, P / kTileQuantum * 16);
+    if (e == hipSuccess) e = walloc(wf, &w.tile_fin, P / kTileQuantum * 16);
+    if (e == hipSuccess) e = walloc(wf, &w.tile_free, P / kTileQuantum * 16);
     w.train_capacity = (uint32_t)std::min<size_t>(P * max_depth, 1u << 25);
     if (e == hipSuccess) e = walloc(wf, &w.train_rec, 10 * (size_t)w.train_capacity);
     if (e == hipSuccess) e = walloc(wf, &wf->d_nets, kMaxNets);
@@ -275,18 +288,24 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     WfState w = wf->w;
     w.jobs = wf->d_jobs;
     w.n_jobs = (uint32_t)hj.size();
-    w.regen_min = wf->regen_min;
     if (!train) w.recs = nullptr;
     w.nets = wf->d_nets;
     w.n_nets = (uint32_t)nets.size();
     w.rec_cap_net = w.n_nets ? w.train_capacity / w.n_nets : w.train_capacity;
     unsigned long long *cnt = (unsigned long long *)ray_counts;
-    e = wf_launch_reset(w, total, st);
+    w.tile = 4 * wf->block;
+    w.n_tiles = w.capacity / w.tile;
+    // a tile regenerates while fewer than its share of regen_min are live
+    w.regen_thr = (uint32_t)std::min<uint64_t>(((uint64_t)wf->regen_min * w.tile + w.capacity - 1) / w.capacity,
+                                               (uint64_t)w.tile + 1);
+    if (e == hipSuccess) e = wf_launch_reset(w, total, wf->block, st);
     if (e == hipSuccess && train) e = wf_launch_clear_records(w, -1, st);
     // Every path finishes within max_depth + 1 iterations of its start; the
     // pool restarts at least one path per iteration whenever it is idle.
     const uint64_t max_iters = (total / wf->capacity + 2) * (uint64_t)(wf->max_depth + 2) + total + 16;
-    const uint32_t k = std::max(1u, wf->graph_iters);
+    // iterations alternate the state buffer: a replay of an even number of
+    // them starts and ends on buffer 0
+    const uint32_t k = wf->graph_iters > 0 ? (wf->graph_iters + 1) & ~1u : 2u;
     // the networks (their count sets the record regions) are part of the key
     // whenever they are used, for inference or for training records
     const std::string gbytes = (guided || train)
@@ -301,7 +320,8 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         } else {
             hipGraph_t graph;
             e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, film, cnt, wf->block, st);
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i)
+                e = wf_launch_iteration(w, i & 1u, guided, film, cnt, wf->block, st);
             hipError_t e2 = hipStreamEndCapture(st, &graph);
             if (e == hipSuccess) e = e2;
             if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -314,14 +334,15 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     while (e == hipSuccess) {
         if (exec) e = hipGraphLaunch(exec, st);
         else
-            for (uint32_t i = 0; i < k && e == hipSuccess; ++i) e = wf_launch_iteration(w, guided, film, cnt, wf->block, st);
+            for (uint32_t i = 0; i < k && e == hipSuccess; ++i)
+                e = wf_launch_iteration(w, i & 1u, guided, film, cnt, wf->block, st);
         iters += k;
         if (e == hipSuccess) e = hipMemcpyAsync(&c, w.ctr, sizeof(c), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) break;
         const bool finished = c.done_paths >= c.total_paths;
         for (uint32_t nk = 0; train && nk < w.n_nets && e == hipSuccess; ++nk) {
-            const unsigned long long rc = c.rec_count[nk];
+            const unsigned long long rc = c.rec_count[nk][0];
             if (!((rc >= batch || finished) && rc > 0)) continue;
             // up to gcfg->train Adam steps per replay for network nk, on its
             // own records; step s reads the records s, s + S*q, s + 2*S*q,
