@@ -412,7 +412,7 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
     depth = 10
     sizes = [int(c["width"][0]) * int(c["height"][0]) for c in cams]
     n_px = sum(sizes)
-    wf = Wavefront(max_paths=1 << 22, max_depth=depth, block=64, graph_iters=4)
+    wf = Wavefront(max_paths=1 << 22, max_depth=depth, block=256, graph_iters=4)
 
     def jobs(spp, off, seed):
         return [(sc, c, spp, depth, seed + v, off) for v, c in enumerate(cams)]

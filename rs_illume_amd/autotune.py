@@ -21,10 +21,10 @@ import os
 import time
 
 DEFAULT_SPACE = {
-    "max_paths": (1 << 20, 1 << 21, 1 << 22),
-    "block": (64, 128, 256),
-    "regen_frac": (1.01, 0.5, 0.25),
-    "graph_iters": (4, 16),
+    "max_paths": (1 << 21, 1 << 22, 1 << 23),
+    "block": (128, 256),
+    "regen_frac": (1.01, 0.5),
+    "graph_iters": (4, 8),
 }
 
 

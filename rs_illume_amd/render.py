@@ -303,7 +303,7 @@ class Wavefront:
     """Pool of ``max_paths`` path slots in HBM with the wavefront pipeline
     (regenerate, extend, infer, shade) replayed from hipGraphs."""
 
-    def __init__(self, max_paths=1 << 20, max_depth=8, block=128, regen_min=None, graph_iters=8):
+    def __init__(self, max_paths=1 << 20, max_depth=8, block=256, regen_min=None, graph_iters=8):
         L = _declare_wf()
         torch.cuda.init()
         h = ctypes.c_void_p()
@@ -312,10 +312,11 @@ class Wavefront:
         self._h = h
         self.max_paths = int(max_paths)
         self.max_depth = int(max_depth)
-        self.tune(block, self.max_paths + 1 if regen_min is None else regen_min, graph_iters)
+        self.tune(block, regen_min, graph_iters)
 
-    def tune(self, block=128, regen_min=None, graph_iters=8):
-        rm = self.max_paths + 1 if regen_min is None else int(regen_min)
+    def tune(self, block=256, regen_min=None, graph_iters=8):
+        # None: regenerate every iteration (the pool clamps to its capacity + 1)
+        rm = 0xFFFFFFFF if regen_min is None else int(regen_min)
         _lib.check(_declare_wf().illume_wavefront_set_tuning(self._h, int(block), rm, int(graph_iters)),
                    "illume_wavefront_set_tuning")
         self.params = {"block": int(block), "regen_min": rm, "graph_iters": int(graph_iters)}
