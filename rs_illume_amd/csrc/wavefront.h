@@ -19,6 +19,7 @@ struct WfJob {
     DevCamera cam;
     unsigned long long path_begin;
     unsigned long long film_offset;  // float offset of the job's image in the film buffer
+    unsigned long long pix_begin;    // first pixel of the job among the batch's unfiltered pixels
     uint64_t seed;
     uint32_t spp, sample_offset, max_depth;
     uint32_t filtered;  // 1: camera samples over fs's sample bounds, splatted into 4-float pixels
@@ -50,7 +51,7 @@ struct WfPaths {
     float4 *beta;     // throughput; w (bits): bounces | specular << 8 | training vertices << 16
     float4 *L;        // radiance; w (bits): job index
     ulonglong2 *rng;  // sampler state, increment
-    uint4 *meta;      // sample pixel, film x, film y (bits), training-record slot
+    uint4 *meta;      // sample pixel, film x, film y (bits), training-record slot (bit 31: records off)
 };
 
 struct WfState {
@@ -83,6 +84,9 @@ struct WfState {
     uint32_t capacity, max_depth;
     uint32_t tile, n_tiles;  // slots per tile (4 x block), tiles
     uint32_t regen_thr;      // a tile starts new paths when fewer than this many are live
+    uint32_t rec_thresh;     // with records on, path g records training vertices iff hash(g) <= rec_thresh
+    float4 *smp;             // radiance of path g of an unfiltered job at smp[g] (null: added to the film
+                             // with atomics when the path ends)
     const WfJob *jobs;
     uint32_t n_jobs;
     WfCounters *ctr;
@@ -94,6 +98,9 @@ hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, int
 // Clears the training-record counter of network `net`, or of every network
 // for net < 0 (a kernel, so it may sit between graph replays).
 hipError_t wf_launch_clear_records(const WfState &w, int net, hipStream_t st);
+// Sums the sample buffer into the film: every pixel of every unfiltered job
+// (n_pix of them in all) gets its samples in sample order.
+hipError_t wf_launch_film_reduce(const WfState &w, unsigned long long n_pix, float *film, hipStream_t st);
 // One iteration reading state buffer `parity`: regenerate, extend, [infer],
 // shade, direct lighting, finish.
 hipError_t wf_launch_iteration(const WfState &w, uint32_t parity, bool guided, float *film,

@@ -40,6 +40,17 @@ namespace {
 constexpr uint32_t kSpecularBit = 1u << 8;
 constexpr uint32_t kFinishDest = 1u << 31;  // direct-lighting destination in the finish buffer
 constexpr int kTileRounds = 4;             // slots per tile = kTileRounds x block
+constexpr uint32_t kNoRecords = 1u << 31;  // record-slot bit: this path keeps no training vertices
+
+// Integer hash of a path index (the 64-bit finaliser of MurmurHash3, folded).
+__device__ __forceinline__ uint32_t path_hash(unsigned long long g) {
+    g ^= g >> 33;
+    g *= 0xff51afd7ed558ccdull;
+    g ^= g >> 33;
+    g *= 0xc4ceb9fe1a85ec53ull;
+    g ^= g >> 33;
+    return (uint32_t)g;
+}
 
 __device__ __forceinline__ uint32_t path_bits(const PathState &ps, uint32_t n_recs) {
     return ps.bounces | (ps.specular ? kSpecularBit : 0u) | (n_recs << 16);
@@ -225,9 +236,16 @@ __global__ __launch_bounds__(BLOCK) void wf_regen_kernel(WfState w, uint32_t par
         float fx, fy;
         path_start_at(job.cam, job.seed, pix, x, y, job.sample_offset + (uint32_t)(local / n_pix), ps, fx, fy,
                       job.sl);
-        const uint32_t rec = w.recs ? w.rec_free[t * S + free0 - take + i] : 0u;
+        uint32_t rec = 0u;
+        if (w.recs) {
+            rec = w.rec_free[t * S + free0 - take + i];
+            if (path_hash(g) > w.rec_thresh) rec |= kNoRecords;
+        }
+        // unfiltered paths with a sample buffer keep their path index in place
+        // of the film position
+        const uint32_t my = (!filt && w.smp) ? (uint32_t)g : __float_as_uint(fy);
         store_path(w.s[parity], t * S + live + i, ps, j, 0u,
-                   make_uint4((uint32_t)pix, __float_as_uint(fx), __float_as_uint(fy), rec));
+                   make_uint4((uint32_t)pix, __float_as_uint(fx), my, rec));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -367,7 +385,7 @@ __global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, uint32_t par
             if (GUIDED) {
                 const int gslot = __float_as_int(h1.z);
                 gout = (job.gc.out && gslot >= 0) ? job.gc.out + 40 * (size_t)gslot : nullptr;
-                if (w.recs) recs = w.recs + (size_t)meta.w * w.max_depth;
+                if (w.recs && !(meta.w & kNoRecords)) recs = w.recs + (size_t)meta.w * w.max_depth;
             }
             TraceStack<BLOCK> st;  // unused: shading traces nothing
             PathCounters pc{0, 0};
@@ -480,6 +498,8 @@ __global__ __launch_bounds__(BLOCK) void wf_finish_kernel(WfState w, float *__re
         if (job.filtered) {
             film_splat(job.fs, film + job.film_offset, (int)job.cam.width, (int)job.cam.height,
                        __uint_as_float(meta.y), __uint_as_float(meta.z), L.x, L.y, L.z);
+        } else if (w.smp) {
+            w.smp[meta.z] = make_float4(L.x, L.y, L.z, 0.f);
         } else {
             float *px = film + job.film_offset + 3 * (size_t)meta.x;
             atomicAdd(px, L.x);
@@ -487,8 +507,9 @@ __global__ __launch_bounds__(BLOCK) void wf_finish_kernel(WfState w, float *__re
             atomicAdd(px + 2, L.z);
         }
         if (w.recs) {
-            emit_records(w, meta.w, w.fin_nrec[q], L, job.gc.net);
-            w.rec_free[t * S + free0 + i] = meta.w;
+            const uint32_t rec = meta.w & ~kNoRecords;
+            emit_records(w, rec, (meta.w & kNoRecords) ? 0u : w.fin_nrec[q], L, job.gc.net);
+            w.rec_free[t * S + free0 + i] = rec;
         }
     }
     if (threadIdx.x == 0 && n_in) {
@@ -517,6 +538,35 @@ hipError_t iteration(const WfState &w, uint32_t parity, bool guided, float *film
     return hipGetLastError();
 }
 
+// Pixel i of the batch's unfiltered pixels: its job (the last with
+// pix_begin <= i; filtered jobs span no pixels) and its samples
+// path_begin + s * n_pix + p, s = 0 .. spp - 1, summed in that order and added
+// to the film.
+__global__ void wf_film_reduce_kernel(WfState w, unsigned long long n_pix_all, float *__restrict__ film) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pix_all) return;
+    uint32_t lo = 0, hi = w.n_jobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (w.jobs[mid].pix_begin <= i) lo = mid;
+        else hi = mid;
+    }
+    const WfJob &job = w.jobs[lo];
+    const unsigned long long p = i - job.pix_begin;
+    const unsigned long long n_pix = (unsigned long long)job.cam.width * job.cam.height;
+    float r = 0.f, g = 0.f, b = 0.f;
+    for (uint32_t s = 0; s < job.spp; ++s) {
+        const float4 v = w.smp[job.path_begin + s * n_pix + p];
+        r += v.x;
+        g += v.y;
+        b += v.z;
+    }
+    float *px = film + job.film_offset + 3 * p;
+    px[0] += r;
+    px[1] += g;
+    px[2] += b;
+}
+
 __global__ void wf_clear_records_kernel(WfCounters *c, int net) {
     for (int k = 0; k < kMaxNets; ++k)
         if (net < 0 || k == net) c->rec_count[k][0] = 0ull;
@@ -530,6 +580,12 @@ hipError_t wf_launch_reset(const WfState &w, unsigned long long total_paths, int
         case 256: wf_reset_kernel<256><<<w.n_tiles, 256, 0, st>>>(w, total_paths); break;
         default: wf_reset_kernel<128><<<w.n_tiles, 128, 0, st>>>(w, total_paths); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_film_reduce(const WfState &w, unsigned long long n_pix, float *film, hipStream_t st) {
+    if (n_pix == 0) return hipSuccess;
+    wf_film_reduce_kernel<<<(unsigned)((n_pix + 255) / 256), 256, 0, st>>>(w, n_pix, film);
     return hipGetLastError();
 }
 

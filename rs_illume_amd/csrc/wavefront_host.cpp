@@ -49,6 +49,8 @@ struct illume_wavefront {
     std::map<GraphKey, hipGraphExec_t> graphs;
     hipStream_t own_stream = nullptr;  // used when the caller passes the null stream (not capturable)
     uint64_t iterations = 0;  // of the last render call
+    float4 *smp = nullptr;    // per-path radiance of unfiltered batches
+    unsigned long long smp_capacity = 0;
 };
 
 namespace {
@@ -56,6 +58,9 @@ namespace {
 // per-tile count arrays have room for the smallest tiles (4 x 64: 16 per
 // quantum).
 constexpr uint32_t kTileQuantum = 1024;
+// Largest batch with a sample buffer: 2^29 paths, 8 GiB (path indices fit the
+// 32-bit field they travel in).
+constexpr unsigned long long kSampleBufferPaths = 1ull << 29;
 template <typename T>
 hipError_t walloc(illume_wavefront *wf, T **p, size_t count) {
     hipError_t e = hipMalloc(p, count * sizeof(T));
@@ -153,6 +158,12 @@ uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf) { return w
 
 namespace {
 
+// Iterations per replay: they alternate the state buffer, so a replay of an
+// even number of them starts and ends on buffer 0.
+uint32_t k_iters_per_replay(const illume_wavefront *wf) {
+    return wf->graph_iters > 0 ? (wf->graph_iters + 1) & ~1u : 2u;
+}
+
 // Grows the proposal queue to n_nets networks' segments (capacity rows each).
 hipError_t ensure_net_rows(illume_wavefront *wf, uint32_t n_nets) {
     if (n_nets <= wf->nets_alloc) return hipSuccess;
@@ -206,7 +217,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     hipStream_t caller = (hipStream_t)stream;
     hipStream_t st = caller ? caller : wf->own_stream;
     std::vector<WfJob> hj;
-    unsigned long long total = 0;
+    unsigned long long total = 0, n_pix_all = 0;
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const illume_job &in = jobs[j];
         if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
@@ -238,6 +249,8 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
             return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
         job.scene = scene_dev_device(in.scene);
         job.path_begin = total;
+        job.pix_begin = n_pix_all;
+        if (!filters) n_pix_all += n_pix;
         job.film_offset = in.film_offset;
         job.seed = in.cfg.seed;
         job.spp = in.cfg.spp;
@@ -273,6 +286,22 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         wf->jobs_capacity = (uint32_t)hj.size();
         drop_graphs(wf);
     }
+    // Unfiltered batches of up to kSampleBufferPaths paths write each path's
+    // radiance to a sample buffer (one 16-byte store) that one kernel sums into
+    // the film at the end, instead of three float atomics per path.
+    const bool use_smp = !filters && total <= kSampleBufferPaths;
+    if (use_smp && total > wf->smp_capacity) {
+        if (wf->smp) {
+            hipFree(wf->smp);
+            wf->allocs.erase(std::remove(wf->allocs.begin(), wf->allocs.end(), (void *)wf->smp), wf->allocs.end());
+        }
+        wf->smp = nullptr;
+        wf->smp_capacity = 0;
+        drop_graphs(wf);
+        e = walloc(wf, &wf->smp, total);
+        if (e != hipSuccess) return abi_hip_fail(e, "sample buffer allocation");
+        wf->smp_capacity = total;
+    }
     if (guided && (e = ensure_net_rows(wf, (uint32_t)nets.size())) != hipSuccess)
         return abi_hip_fail(e, "proposal queue allocation");
     // guide_out moved if the queue grew: the jobs point into it
@@ -291,6 +320,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     if (!train) w.recs = nullptr;
     w.nets = wf->d_nets;
     w.n_nets = (uint32_t)nets.size();
+    w.smp = use_smp ? wf->smp : nullptr;
     w.rec_cap_net = w.n_nets ? w.train_capacity / w.n_nets : w.train_capacity;
     unsigned long long *cnt = (unsigned long long *)ray_counts;
     w.tile = 4 * wf->block;
@@ -298,22 +328,31 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     // a tile regenerates while fewer than its share of regen_min are live
     w.regen_thr = (uint32_t)std::min<uint64_t>(((uint64_t)wf->regen_min * w.tile + w.capacity - 1) / w.capacity,
                                                (uint64_t)w.tile + 1);
+    // Training uses at most gcfg->train steps of `batch` records per network
+    // and replay of k iterations, while every live path may record a vertex
+    // per iteration: only a hash-chosen share of the paths records, sized for
+    // eight times the records the steps take (all paths for small batches).
+    w.rec_thresh = 0xFFFFFFFFu;
+    if (train) {
+        const double need = 8.0 * std::max(1u, gcfg->train) * batch * std::max<uint32_t>(1u, (uint32_t)w.n_nets);
+        const double made = (double)k_iters_per_replay(wf) * (double)std::min<unsigned long long>(w.capacity, total);
+        if (need < made) w.rec_thresh = (uint32_t)(need / made * 4294967295.0);
+    }
     if (e == hipSuccess) e = wf_launch_reset(w, total, wf->block, st);
     if (e == hipSuccess && train) e = wf_launch_clear_records(w, -1, st);
     // Every path finishes within max_depth + 1 iterations of its start; the
     // pool restarts at least one path per iteration whenever it is idle.
     const uint64_t max_iters = (total / wf->capacity + 2) * (uint64_t)(wf->max_depth + 2) + total + 16;
-    // iterations alternate the state buffer: a replay of an even number of
-    // them starts and ends on buffer 0
-    const uint32_t k = wf->graph_iters > 0 ? (wf->graph_iters + 1) & ~1u : 2u;
-    // the networks (their count sets the record regions) are part of the key
-    // whenever they are used, for inference or for training records
-    const std::string gbytes = (guided || train)
-                                   ? std::string((const char *)hnets.data(), hnets.size() * sizeof(GuideDev))
-                                   : std::string();
+    const uint32_t k = k_iters_per_replay(wf);
+    // A graph holds the launch arguments it was captured with: the state
+    // (pointers, tile shape, record share, sample buffer) and, whenever they are
+    // used for inference or training records, the networks.
+    std::string gbytes((const char *)&w, sizeof(w));
+    if (guided || train) gbytes.append((const char *)hnets.data(), hnets.size() * sizeof(GuideDev));
     GraphKey key(guided, train, gbytes, film, cnt, wf->block, w.n_jobs);
     hipGraphExec_t exec = nullptr;
     if (wf->graph_iters > 0 && e == hipSuccess) {
+        if (wf->graphs.size() >= 64) drop_graphs(wf);
         auto it = wf->graphs.find(key);
         if (it != wf->graphs.end()) {
             exec = it->second;
@@ -366,6 +405,10 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         }
     }
     wf->iterations = iters;
+    if (e == hipSuccess && use_smp) e = wf_launch_film_reduce(w, n_pix_all, film, st);
+    // on the pool's own stream (the caller passed the null stream) the film is
+    // complete when the call returns
+    if (e == hipSuccess && !caller) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return abi_hip_fail(e, "wavefront render");
     return ILLUME_OK;
 }
