@@ -296,76 +296,104 @@ __device__ __forceinline__ void camera_ray(const DevCamera &cam, float fx, float
 }
 
 // ---------------------------------------------------------------- learned proposal
+// The proposal's arithmetic uses the hardware exp2 / log2 / reciprocal /
+// sin / cos (sin and cos take revolutions): its sampler and its pdf are the
+// same formulas evaluated from the same decoded lobes, so the one-sample MIS
+// estimator divides by the density the directions are drawn from up to float
+// rounding, whatever the precision of the single operations.
+constexpr float kLog2e = 1.44269504088896340736f;
+constexpr float kLn2 = 0.69314718055994530942f;
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * kLog2e); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1 - exp(-2k) for k >= 0, with the series where the difference cancels
+__device__ __forceinline__ float one_minus_exp_m2k(float k) {
+    return k < 1e-3f ? 2.f * k * (1.f - k) : 1.f - fast_exp(-2.f * k);
+}
+// vMF normalisation on the sphere: k / (2 pi (1 - exp(-2k))), 1 / (4 pi) at k = 0
+__device__ __forceinline__ float vmf_norm(float k) {
+    return k > 0.f ? k * fast_rcp(6.28318530717958647692f * one_minus_exp_m2k(k)) : 0.0795774715459476679f;
+}
+
 // 8-lobe vMF mixture decoded from the network's 40 outputs (softmax weights,
-// normalised means, concentrations exp(clamp(r, -4.6, 6.9))).
+// normalised means, concentrations exp(clamp(r, -4.6, 6.9))); c[j] =
+// pi[j] * vmf_norm(kappa[j]), the lobe's density factor.
 struct Mixture {
-    float pi[8], mu[8][3], kappa[8];
+    float pi[8], mu[8][3], kappa[8], c[8];
 };
 __device__ __forceinline__ bool decode_mixture(const float *__restrict__ y, Mixture &m) {
     float mx = y[0];
     for (int j = 1; j < 8; ++j) mx = fmaxf(mx, y[5 * j]);
     float s = 0.f;
     for (int j = 0; j < 8; ++j) {
-        m.pi[j] = expf(y[5 * j] - mx);
+        m.pi[j] = fast_exp(y[5 * j] - mx);
         s += m.pi[j];
     }
     bool ok = s > 0.f && __builtin_isfinite(s);
+    const float inv_s = fast_rcp(s);
     for (int j = 0; j < 8; ++j) {
-        m.pi[j] /= s;
+        m.pi[j] *= inv_s;
         const float a = y[5 * j + 1], b = y[5 * j + 2], c = y[5 * j + 3];
-        const float len = sqrtf(a * a + b * b + c * c);
-        const float inv = len > 0.f ? 1.f / len : 0.f;
+        const float len2 = a * a + b * b + c * c;
+        const float inv = len2 > 0.f ? __builtin_amdgcn_rsqf(len2) : 0.f;
         m.mu[j][0] = a * inv;
         m.mu[j][1] = b * inv;
-        m.mu[j][2] = len > 0.f ? c * inv : 1.f;
-        m.kappa[j] = expf(fminf(fmaxf(y[5 * j + 4], -4.6f), 6.9f));
-        ok = ok && __builtin_isfinite(m.pi[j]) && __builtin_isfinite(len) && __builtin_isfinite(m.kappa[j]);
+        m.mu[j][2] = len2 > 0.f ? c * inv : 1.f;
+        m.kappa[j] = fast_exp(fminf(fmaxf(y[5 * j + 4], -4.6f), 6.9f));
+        m.c[j] = m.pi[j] * vmf_norm(m.kappa[j]);
+        ok = ok && __builtin_isfinite(m.pi[j]) && __builtin_isfinite(len2) && __builtin_isfinite(m.kappa[j]);
     }
     return ok;
 }
 __device__ __forceinline__ float mixture_pdf(const Mixture &m, f3 w) {
     float q = 0.f;
     for (int j = 0; j < 8; ++j) {
-        const float k = m.kappa[j];
         const float c = m.mu[j][0] * w.x + m.mu[j][1] * w.y + m.mu[j][2] * w.z;
-        q += m.pi[j] * (k / (6.28318530717958647692f * (-expm1f(-2.f * k))) * expf(k * (c - 1.f)));
+        q += m.c[j] * fast_exp(m.kappa[j] * (c - 1.f));
     }
     return q;
 }
 // Lobe by ul, then the vMF sample of Jakob (2012) around the lobe mean.
 __device__ __forceinline__ f3 mixture_sample(const Mixture &m, float ul, float u1, float u2) {
-    int j = 0;
+    // the first lobe whose cumulative weight exceeds ul (the last if none);
+    // selected with constant indices so that the mixture stays in registers
+    int sel = 7;
     float c = 0.f;
-    for (; j < 7; ++j) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
         c += m.pi[j];
-        if (ul < c) break;
+        if (sel == 7 && ul < c) sel = j;
     }
-    const float k = m.kappa[j];
-    float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
+    float k = m.kappa[0];
+    f3 mu = mk(m.mu[0][0], m.mu[0][1], m.mu[0][2]);
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (sel == j) {
+            k = m.kappa[j];
+            mu = mk(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
+        }
+    float w = 1.f + __builtin_amdgcn_logf(u1 + (1.f - u1) * fast_exp(-2.f * k)) * kLn2 * fast_rcp(k);
     w = fminf(1.f, fmaxf(-1.f, w));
-    const float s = sqrtf(fmaxf(0.f, 1.f - w * w));
-    const float phi = 6.28318530717958647692f * u2;
-    const f3 mu = mk(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
+    const float s = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - w * w));
     f3 t1, t2;
     if (fabsf(mu.x) > fabsf(mu.y)) {
-        const float l = sqrtf(mu.x * mu.x + mu.z * mu.z);
-        t1 = mk(-mu.z / l, 0.f, mu.x / l);
+        const float il = __builtin_amdgcn_rsqf(mu.x * mu.x + mu.z * mu.z);
+        t1 = mk(-mu.z * il, 0.f, mu.x * il);
     } else {
-        const float l = sqrtf(mu.y * mu.y + mu.z * mu.z);
-        t1 = mk(0.f, mu.z / l, -mu.y / l);
+        const float il = __builtin_amdgcn_rsqf(mu.y * mu.y + mu.z * mu.z);
+        t1 = mk(0.f, mu.z * il, -mu.y * il);
     }
     t2 = mk(mu.y * t1.z - mu.z * t1.y, mu.z * t1.x - mu.x * t1.z, mu.x * t1.y - mu.y * t1.x);
-    const float lx = s * cosf(phi), ly = s * sinf(phi);
+    // sin / cos of 2 pi u2 (the hardware takes revolutions)
+    const float lx = s * __builtin_amdgcn_cosf(u2), ly = s * __builtin_amdgcn_sinf(u2);
     return normalize(add(add(smul(lx, t1), smul(ly, t2)), smul(w, mu)));
 }
 
 // Product of the mixture with a vMF fit of the vertex's cosine-weighted BSDF
 // lobe (mean mu_b, concentration k_b). Each lobe's product is a vMF with
 // k mu = k_j mu_j + k_b mu_b; its weight is scaled by the product's integral
-// c(k_j) c(k_b) exp(k - k_j - k_b) / c(k), c(k) = k / (2 pi (1 - exp(-2k))).
-// The result is again a normalised mixture, so its pdf is exact. False if the
-// weights degenerate.
-__device__ __forceinline__ float vmf_norm(float k) { return k / (6.28318530717958647692f * (-expm1f(-2.f * k))); }
+// c(k_j) c(k_b) exp(k - k_j - k_b) / c(k), c(k) = vmf_norm(k). The result is
+// again a normalised mixture, so its pdf is exact. False if the weights
+// degenerate.
 __device__ __forceinline__ bool mixture_times_lobe(Mixture &m, f3 mu_b, float k_b) {
     float s = 0.f;
     const float cb = vmf_norm(k_b);
@@ -373,22 +401,26 @@ __device__ __forceinline__ bool mixture_times_lobe(Mixture &m, f3 mu_b, float k_
         const float kj = m.kappa[j];
         const float vx = kj * m.mu[j][0] + k_b * mu_b.x, vy = kj * m.mu[j][1] + k_b * mu_b.y,
                     vz = kj * m.mu[j][2] + k_b * mu_b.z;
-        const float k = fmaxf(sqrtf(vx * vx + vy * vy + vz * vz), 1e-4f);
-        const float scale = vmf_norm(kj) * cb * expf(fminf(k - kj - k_b, 0.f)) / vmf_norm(k);
-        m.mu[j][0] = vx / k;
-        m.mu[j][1] = vy / k;
-        m.mu[j][2] = vz / k;
+        const float k = fmaxf(__builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz), 1e-4f);
+        const float ck = vmf_norm(k);
+        const float ik = fast_rcp(k);
+        m.pi[j] *= vmf_norm(kj) * cb * fast_exp(fminf(k - kj - k_b, 0.f)) * fast_rcp(ck);
+        m.mu[j][0] = vx * ik;
+        m.mu[j][1] = vy * ik;
+        m.mu[j][2] = vz * ik;
         m.kappa[j] = k;
-        m.pi[j] *= scale;
+        m.c[j] = ck;  // times the normalised weight below
         s += m.pi[j];
     }
     if (!(s > 0.f) || !__builtin_isfinite(s)) return false;
-    const float inv = 1.f / s;
-    for (int j = 0; j < 8; ++j) m.pi[j] *= inv;
+    const float inv = fast_rcp(s);
+    for (int j = 0; j < 8; ++j) {
+        m.pi[j] *= inv;
+        m.c[j] *= m.pi[j];
+    }
     return true;
 }
 
-// Guide usage for one render pass.
 struct GuideCfg {
     const float *out;         // [paths][40] network outputs, null = plain path tracing
     float frac;               // probability of sampling the proposal (one-sample MIS)

@@ -353,8 +353,11 @@ __device__ void emit_records(const WfState &w, uint32_t rec, uint32_t n, f3 Lf, 
 // tile t of the other state buffer, ended paths to the finish buffer, and a
 // vertex's direct-lighting rays to the request buffer, each from the tile's
 // first slot.
+// At least 3 waves per SIMD (168 VGPRs; the natural 182 / 203 give 2): the
+// Cornell box's 4-spp pass 2.73 -> 2.55 ms plain, 4.93 -> 4.80 ms guided;
+// 4 waves spill too much in the guided kernel (5.43 ms).
 template <int BLOCK, bool GUIDED>
-__global__ __launch_bounds__(BLOCK) void wf_shade_kernel(WfState w, uint32_t parity) {
+__global__ __launch_bounds__(BLOCK, 3) void wf_shade_kernel(WfState w, uint32_t parity) {
     __shared__ uint32_t lds[3 * (BLOCK / 64)];
     const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
     const uint32_t n_in = w.tile_live[parity][t];
