@@ -1,17 +1,17 @@
 // Learned sampling proposal: 144-64-64-64-40 ReLU MLP from the frequency
 // encoding of (normalised hit position, normal) to an 8-lobe vMF mixture.
-// Inference and training (forward, backward, Adam) run on MFMA with bf16
-// weights and activations and fp32 accumulation (v_mfma_f32_16x16x32_bf16).
+// Inference and training (forward, backward, weight gradients, Adam) run on
+// MFMA with bf16 weights and activations and fp32 accumulation
+// (v_mfma_f32_16x16x32_bf16).
 //
 // Every product is computed transposed (weights as the A operand, rows of the
 // batch as the B operand), with the hidden units ordered so that an output
 // tile's registers are already the next product's B operand: activations and
 // their gradients stay in registers through all layers. Weights are staged in
-// LDS once per block. Training runs in two passes: per 32-row tile the
-// forward and backward products (storing activations and gradients
-// transposed), then the weight gradients as MFMA products with K = rows over
-// chunks of tiles, then one reduction; Adam updates the fp32 master weights
-// and rewrites the bf16 copies.
+// LDS once per block. A training step is one fused pass (forward, backward and
+// the weight-gradient products per 32-row tile, the weight gradients kept in
+// registers per block) and one reduction that applies Adam to the fp32 master
+// weights and rewrites the bf16 copies.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include "guide_kernels.h"
@@ -25,7 +25,6 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kWave = 64;
 constexpr int kEnc = 144, kOut = 40, kLobes = 8;
 constexpr uint32_t kInferGrid = 768;   // inference blocks at most (3 per CU: 47 KB of LDS each)
-static_assert(kGradPartialFloats >= (size_t)128 * kGuideParams, "one partial row per gradient split");
 
 __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -195,63 +194,76 @@ __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_nets_kernel(cons
                out + (size_t)kOut * k * stride);
 }
 
-// d(-log q)/d outputs for one row (mixture decode as in the sampler).
+// d(-log q)/d outputs for one row (mixture decode as in the sampler), on the
+// hardware exp2 / log2 / reciprocal / square root: the loss's gradient needs
+// no more than float rounding, and the oracle's gradient agrees within its
+// bf16 tolerance.
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896340736f); }
+__device__ __forceinline__ float hrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1 - exp(-2k) for k >= 0, with the series where the difference cancels
+__device__ __forceinline__ float hom2k(float k) { return k < 1e-3f ? 2.f * k * (1.f - k) : 1.f - hexp(-2.f * k); }
 __device__ void nll_grad_row(const float *y, const float *w, float scale, float *gy, float *loss) {
     float mx = y[0];
     for (int j = 1; j < kLobes; ++j) mx = fmaxf(mx, y[5 * j]);
     float pi[kLobes], s = 0.f;
     for (int j = 0; j < kLobes; ++j) {
-        pi[j] = expf(y[5 * j] - mx);
+        pi[j] = hexp(y[5 * j] - mx);
         s += pi[j];
     }
-    float v[kLobes], mu[kLobes][3], kap[kLobes], mn[kLobes], q = 0.f;
+    const float inv_s = hrcp(s);
+    float v[kLobes], mu[kLobes][3], kap[kLobes], imn[kLobes], q = 0.f;
     bool clamped[kLobes];
     for (int j = 0; j < kLobes; ++j) {
-        pi[j] /= s;
+        pi[j] *= inv_s;
         const float a = y[5 * j + 1], b = y[5 * j + 2], c = y[5 * j + 3];
-        const float len = sqrtf(a * a + b * b + c * c);
-        mn[j] = len;
-        const float inv = len > 0.f ? 1.f / len : 0.f;
+        const float len2 = a * a + b * b + c * c;
+        const float inv = len2 > 0.f ? __builtin_amdgcn_rsqf(len2) : 0.f;
+        imn[j] = inv;
         mu[j][0] = a * inv;
         mu[j][1] = b * inv;
-        mu[j][2] = len > 0.f ? c * inv : 1.f;
+        mu[j][2] = len2 > 0.f ? c * inv : 1.f;
         const float r = y[5 * j + 4];
         clamped[j] = r < -4.6f || r > 6.9f;
-        kap[j] = expf(fminf(fmaxf(r, -4.6f), 6.9f));
+        kap[j] = hexp(fminf(fmaxf(r, -4.6f), 6.9f));
         const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
-        v[j] = kap[j] / (6.28318530717958647692f * (-expm1f(-2.f * kap[j]))) * expf(kap[j] * (cdot - 1.f));
+        v[j] = kap[j] * hrcp(6.28318530717958647692f * hom2k(kap[j])) * hexp(kap[j] * (cdot - 1.f));
         q += pi[j] * v[j];
     }
-    *loss = -scale * logf(q);
+    *loss = -scale * __builtin_amdgcn_logf(q) * 0.69314718055994530942f;
+    const float iq = hrcp(q);
     for (int j = 0; j < kLobes; ++j) {
-        const float gam = pi[j] * v[j] / q;
+        const float gam = pi[j] * v[j] * iq;
         gy[5 * j] = scale * (pi[j] - gam);
         const float k = kap[j];
         const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
-        const float dlk = 1.f / k - 2.f * expf(-2.f * k) / (-expm1f(-2.f * k)) + (cdot - 1.f);
+        const float dlk = hrcp(k) - 2.f * hexp(-2.f * k) * hrcp(hom2k(k)) + (cdot - 1.f);
         gy[5 * j + 4] = clamped[j] ? 0.f : scale * (-gam * dlk * k);
-        const float inv = mn[j] > 0.f ? 1.f / mn[j] : 0.f;
-        for (int a = 0; a < 3; ++a) gy[5 * j + 1 + a] = scale * (-gam * k * (w[a] - cdot * mu[j][a]) * inv);
+        for (int a = 0; a < 3; ++a) gy[5 * j + 1 + a] = scale * (-gam * k * (w[a] - cdot * mu[j][a]) * imn[j]);
     }
 }
 
 // ------------------------------------------------------------------ training
-// Phase A (guide_train_tile_kernel), per 32-row tile and per wave: the
-// forward pass as in inference, keeping the activations in registers; the
-// outputs go to the wave's LDS, where lanes 0..31 turn their row into the
-// NLL gradient; the backward products dX^T = W G^T then run with the weights
-// as A operands (wb copies, K in register order) so that every gradient again
-// lands in the registers that the next backward product reads. Activations
-// X_1..X_3 and pre-activation gradients G_0..G_3 are stored transposed,
-// [unit][32 rows] bf16, for phase B.
-// Phase B (guide_train_dw_kernel): dW_l = X_l^T G_l as MFMA products with
-// K = rows, split over chunks of tiles (one partial row per chunk), X_0 (the
-// encoding) recomputed from the records; the bias gradients are the column
-// sums of G_l. guide_reduce_kernel adds the partial rows.
-constexpr int kTrainWaves = 6;
-constexpr uint32_t kTrainGrid = 256;   // phase A blocks at most (one per CU: 140 KB of LDS each)
-constexpr uint32_t kTrainSplits = 128;  // phase B row chunks
-constexpr int kDwItems = 9 + 4 + 4 + 4;  // (layer, 16-unit input tile) pairs
+// One fused pass per Adam step (guide_train_kernel), then one reduction that
+// also applies Adam (guide_reduce_adam_kernel).
+//
+// A block of kTrainWaves waves takes a contiguous run of 32-row tiles, one
+// tile per wave per round. Each wave runs its tile's forward pass as in
+// inference (activations in registers), the NLL gradient of its rows (lanes
+// 0..31, one row each, through the wave's LDS) and the backward products
+// dX^T = W G^T with the weights as A operands (wb copies) so that every
+// gradient lands in the registers the next backward product reads. The
+// activations X_1..X_3 and the pre-activation gradients G_3..G_0 are staged
+// transposed ([unit][32 rows] bf16) in the wave's LDS, one G at a time; after
+// each G the block synchronises and every wave adds the weight-gradient
+// products dW_l = X_l^T G_l (MFMA with K = the 32 rows of a tile) of all the
+// round's tiles into the dW tiles it owns, which stay in registers for the
+// whole run (X_0, the encoding, is recomputed from the rows' staged inputs).
+// Nothing per tile goes to HBM: a block writes one partial gradient row, its
+// weight sum and its loss; the reduction adds the rows, normalises by the
+// total weight and applies Adam to the fp32 master weights and their bf16
+// copies.
+constexpr int kTrainWaves = 4;
+constexpr int kTrainThreads = kTrainWaves * kWave;
 
 struct alignas(16) TrainWeights {
     __bf16 w0[64 * 168];
@@ -263,15 +275,23 @@ struct alignas(16) TrainWeights {
     __bf16 b3[64 * 72];
     float bias[4][64];
 };
+// Staged tensors are [unit][row], kStageLd rows per unit.
+constexpr int kStageLd = 32;
 struct alignas(16) TrainWaveLds {
-    float y[32][48];     // outputs of the tile, then their gradients
-    __bf16 tr[64 * 40];  // transpose staging, [unit][rows] with rows padded to 40
+    __bf16 x[3][64 * kStageLd];  // X_1..X_3 of the wave's tile
+    union {
+        float y[32][48];           // outputs of the tile, then their gradients
+        __bf16 g[64 * kStageLd];   // the current G_l
+    } yg;
+    float in[32][6];               // the rows' network inputs (zero past n)
 };
 struct TrainLds {
     TrainWeights w;
     TrainWaveLds wave[kTrainWaves];
 };
 static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS within one CU's 160 KB");
+static_assert(kGradPartialFloats >= (size_t)kTrainBlocksMax * kGuideParams + 2 * kTrainBlocksMax,
+              "one partial row per block and its weight sum and loss");
 
 // LDS written by some lanes of a wave and read by others of the same wave.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -305,40 +325,92 @@ __device__ __forceinline__ void mask_to_b(const floatx4 *acc, uint32_t bits, int
         }
 }
 
-// Stores a 64-unit tensor held in register order (v[n] for the two 16-row
-// columns) as [unit][32 rows] bf16 at dst, through the wave's LDS.
-__device__ __forceinline__ void store_units(TrainWaveLds &S, const bf16x8 (*v)[2], __bf16 *dst, int lane) {
+// A 64-unit tensor held in register order (v[nn] for the two 16-row columns)
+// -> dst [unit][row] in LDS.
+__device__ __forceinline__ void stage_units(const bf16x8 (*v)[2], __bf16 *dst, int lane) {
     const int g = lane >> 4, r = lane & 15;
 #pragma unroll
     for (int nn = 0; nn < 2; ++nn)
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) S.tr[(16 * m + 4 * g + e) * 40 + 16 * nn + r] = v[nn][m / 2][4 * (m % 2) + e];
-    wave_lds_sync();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = lane + 64 * q, unit = c / 4, part = c % 4;
-        reinterpret_cast<uint4 *>(dst)[c] = *reinterpret_cast<const uint4 *>(S.tr + unit * 40 + part * 8);
-    }
-    wave_lds_sync();
+            for (int e = 0; e < 4; ++e) dst[(16 * m + 4 * g + e) * kStageLd + 16 * nn + r] = v[nn][m / 2][4 * (m % 2) + e];
 }
 
-__global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
-    GuideDev g, const float *__restrict__ rec, uint32_t n, uint32_t stride_f, float *__restrict__ wsum,
-    __bf16 *__restrict__ tiles, float *__restrict__ loss) {
+// dW tile (16 inputs it x 16 outputs u) of layer l >= 1 += X_l^T G_l over the
+// 32 rows of the round's tiles; colsum += the rows' G_l of output column u
+// (the bias gradient) when wanted.
+__device__ __forceinline__ floatx4 dw_tile(const TrainLds &L, int l, int it, int u, floatx4 acc, float *colsum,
+                                           int lane) {
+    const int grp = lane >> 4, r16 = lane & 15;
+#pragma unroll
+    for (int ti = 0; ti < kTrainWaves; ++ti) {
+        const TrainWaveLds &T = L.wave[ti];
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(T.x[l - 1] + (16 * it + r16) * kStageLd + 8 * grp);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(T.yg.g + (16 * u + r16) * kStageLd + 8 * grp);
+        acc = mfma(a, b, acc);
+        if (colsum)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) *colsum += (float)b[e];
+    }
+    return acc;
+}
+
+// Layer 0's A operand: encoding feature 16 it + r16 of rows 8 grp .. 8 grp + 7
+// of a tile, from its staged inputs tin[32][6].
+__device__ __forceinline__ bf16x8 enc_a(const float (*tin)[6], int it, int lane) {
+    const int grp = lane >> 4, r16 = lane & 15;
+    const int f = 16 * it + r16, i = f / 24, o = (f % 24) / 2;
+    bf16x8 a;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float r = ldexpf(tin[8 * grp + e][i], o - 1);
+        const float rev = r - floorf(r);
+        a[e] = to_bf16((f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev));
+    }
+    return a;
+}
+
+// Writes a dW tile (and, with want_bias, the column sums as bias gradients)
+// of layer l to the partial row.
+__device__ __forceinline__ void put_dw(const GuideDev &g, float *prow, int l, int it, int u, floatx4 acc,
+                                       float colsum, bool want_bias, int lane) {
+    const int grp = lane >> 4, r16 = lane & 15;
+    const int din = g.din[l], dout = g.dout[l], out = 16 * u + r16;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int in = 16 * it + 4 * grp + e;
+        if (in < din && out < dout) prow[g.w_off[l] + in * dout + out] = acc[e];
+    }
+    if (want_bias) {
+        colsum += __shfl_xor(colsum, 16);
+        colsum += __shfl_xor(colsum, 32);
+        if (grp == 0 && out < dout) prow[g.b_off[l] + out] = colsum;
+    }
+}
+
+// Block b trains on tiles [b * per_block, (b + 1) * per_block) of the n
+// records (record r at rec + stride_f * r floats); partial row b gets its
+// gradient (rows weighted by w, not yet normalised), wl[2 b], wl[2 b + 1] its
+// weight sum and loss. Wave w owns the dW tiles of input tile it = w of
+// layers 1..3 and it = w, w + 4, w + 8 (< 9) of layer 0, every output tile;
+// wave 0 also sums the bias gradients.
+__global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev g, const float *__restrict__ rec,
+                                                                      uint32_t n, uint32_t stride_f,
+                                                                      uint32_t per_block, float *__restrict__ partial,
+                                                                      float *__restrict__ wl) {
     __shared__ TrainLds L;
-    const uint32_t n_tiles = (n + 31) / 32;
-    if (blockIdx.x * kTrainWaves >= n_tiles) return;  // the whole block has no rows
+    __shared__ float red[2][kTrainWaves];
+    const uint32_t t_begin = blockIdx.x * per_block, t_end = t_begin + per_block;
     {
         const __bf16 *src[7] = {g.wa[0], g.wa[1], g.wa[2], g.wa[3], g.wb[1], g.wb[2], g.wb[3]};
         __bf16 *dst[7] = {L.w.w0, L.w.w1, L.w.w2, L.w.w3, L.w.b1, L.w.b2, L.w.b3};
         const int n16[7] = {64 * 168 / 8, 64 * 72 / 8, 64 * 72 / 8, 48 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8};
 #pragma unroll
         for (int l = 0; l < 7; ++l)
-            for (int i = threadIdx.x; i < n16[l]; i += kTrainWaves * kWave)
+            for (int i = threadIdx.x; i < n16[l]; i += kTrainThreads)
                 reinterpret_cast<uint4 *>(dst[l])[i] = reinterpret_cast<const uint4 *>(src[l])[i];
-        for (int i = threadIdx.x; i < 4 * 64; i += kTrainWaves * kWave) {
+        for (int i = threadIdx.x; i < 4 * 64; i += kTrainThreads) {
             const int l = i / 64, o = i % 64;
             L.w.bias[l][o] = o < kGuideWaRows[l] ? g.bias[l][o] : 0.f;
         }
@@ -346,11 +418,39 @@ __global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, grp = lane >> 4, r16 = lane & 15;
     TrainWaveLds &S = L.wave[wave];
+    const bool bias_owner = wave == 0;
+    const int n_it0 = wave == 0 ? 3 : 2;  // layer-0 input tiles w, w + 4 (, w + 8)
+    floatx4 d3[3], d2[4], d1[4], d0[3][4];
+    float c3[3], c2[4], c1[4], c0[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (u < 3) {
+            d3[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+            c3[u] = 0.f;
+        }
+        d2[u] = d1[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        c2[u] = c1[u] = c0[u] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d0[k][u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
     float wave_loss = 0.f, wave_wsum = 0.f;
-    for (uint32_t t = blockIdx.x * kTrainWaves + wave; t < n_tiles; t += gridDim.x * kTrainWaves) {
-        __asm__ volatile("" ::: "memory");
-        const uint32_t base = t * 32;
-        __bf16 *tile = tiles + (size_t)t * (kGuideTileBytes / 2);
+    for (uint32_t t0 = t_begin; t0 < t_end; t0 += kTrainWaves) {
+        const uint32_t base = (t0 + wave) * 32;
+        // the rows' sampled directions and weights, needed after the forward pass
+        float w_row[4] = {0.f, 0.f, 0.f, 0.f};
+        if (lane < 32 && base + lane < n) {
+            const float *r = rec + (size_t)(base + lane) * stride_f;
+            w_row[0] = r[6];
+            w_row[1] = r[7];
+            w_row[2] = r[8];
+            w_row[3] = r[9];
+        }
+        // ---- forward (rows past n encode zero inputs; their gradient is zero)
+        if (lane < 32) {
+            const uint32_t row = base + lane;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) S.in[lane][i] = row < n ? rec[(size_t)row * stride_f + i] : 0.f;
+        }
         uint32_t mask1, mask2, mask3;
         {
             bf16x8 x0[2][5], h[2][2];
@@ -361,49 +461,45 @@ __global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
             relu_to_b<4>(acc[0], L.w.bias[0], grp, h[0]);
             relu_to_b<4>(acc[1], L.w.bias[0], grp, h[1]);
             mask1 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            store_units(S, h, tile + 0 * 64 * 32, lane);
+            stage_units(h, S.x[0], lane);
             layer_t<2, 4>(L.w.w1, 72, h, acc, lane);
             relu_to_b<4>(acc[0], L.w.bias[1], grp, h[0]);
             relu_to_b<4>(acc[1], L.w.bias[1], grp, h[1]);
             mask2 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            store_units(S, h, tile + 1 * 64 * 32, lane);
+            stage_units(h, S.x[1], lane);
             layer_t<2, 4>(L.w.w2, 72, h, acc, lane);
             relu_to_b<4>(acc[0], L.w.bias[2], grp, h[0]);
             relu_to_b<4>(acc[1], L.w.bias[2], grp, h[1]);
             mask3 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            store_units(S, h, tile + 2 * 64 * 32, lane);
+            stage_units(h, S.x[2], lane);
             layer_t<2, 3>(L.w.w3, 72, h, acc, lane);
 #pragma unroll
             for (int nn = 0; nn < 2; ++nn)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int o = 16 * m + 4 * grp;
-                    *reinterpret_cast<float4 *>(&S.y[16 * nn + r16][o]) =
+                    *reinterpret_cast<float4 *>(&S.yg.y[16 * nn + r16][o]) =
                         make_float4(acc[nn][m][0] + L.w.bias[3][o], acc[nn][m][1] + L.w.bias[3][o + 1],
                                     acc[nn][m][2] + L.w.bias[3][o + 2], acc[nn][m][3] + L.w.bias[3][o + 3]);
                 }
         }
         wave_lds_sync();
-        // output gradient: lane r < 32 owns row base + r
+        // ---- output gradient: lane r < 32 owns row base + r
         if (lane < 32) {
             float gy[kOut], row_loss = 0.f;
             for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
-            const uint32_t row = base + lane;
-            if (row < n) {
-                const float *r = rec + (size_t)row * stride_f;
-                const float wgt = r[9];
-                if (wgt > 0.f) {
-                    const float w3[3] = {r[6], r[7], r[8]};
-                    nll_grad_row(S.y[lane], w3, wgt, gy, &row_loss);
-                    wave_wsum += wgt;
-                }
+            const float wgt = w_row[3];
+            if (wgt > 0.f) {
+                nll_grad_row(S.yg.y[lane], w_row, wgt, gy, &row_loss);
+                wave_wsum += wgt;
             }
             wave_loss += row_loss;
-            for (int o = 0; o < 48; ++o) S.y[lane][o] = o < kOut ? gy[o] : 0.f;
+            for (int o = 0; o < 48; ++o) S.yg.y[lane][o] = o < kOut ? gy[o] : 0.f;
         }
         wave_lds_sync();
-        // G_3: B operand (natural output order, K padded to 64) and its tile
-        bf16x8 g3[2][2], gz[2][2];
+        // G_3 as the first backward product's B operand (natural output order,
+        // K padded to 64) and transposed for the weight gradients
+        bf16x8 g3[2][2], gz[2][2], gt[4];
         floatx4 acc[2][4];
 #pragma unroll
         for (int nn = 0; nn < 2; ++nn)
@@ -412,29 +508,80 @@ __global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = 32 * j + 8 * grp + e;
-                    g3[nn][j][e] = to_bf16(k < 48 ? S.y[16 * nn + r16][k] : 0.f);
+                    g3[nn][j][e] = to_bf16(k < 48 ? S.yg.y[16 * nn + r16][k] : 0.f);
                 }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = lane + 64 * q, unit = c / 4, part = c % 4;
-            bf16x8 v;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = to_bf16(unit < 48 ? S.y[part * 8 + e][unit] : 0.f);
-            reinterpret_cast<bf16x8 *>(tile + 6 * 64 * 32)[c] = v;
+            for (int e = 0; e < 8; ++e) gt[q][e] = to_bf16(unit < 48 ? S.yg.y[part * 8 + e][unit] : 0.f);
         }
-        // backward: G_2 = (W_3 G_3^T) * relu'(h3), then G_1, G_0 likewise
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = lane + 64 * q, unit = c / 4, part = c % 4;
+            *reinterpret_cast<bf16x8 *>(S.yg.g + unit * kStageLd + part * 8) = gt[q];
+        }
+        __syncthreads();
+        // ---- dW_3 (input tile w, output tiles 0..2)
+#pragma unroll
+        for (int u = 0; u < 3; ++u) d3[u] = dw_tile(L, 3, wave, u, d3[u], bias_owner ? &c3[u] : nullptr, lane);
+        __syncthreads();
+        // ---- G_2 = (W_3 G_3^T) * relu'(h3), dW_2
         layer_t<2, 4>(L.w.b3, 72, g3, acc, lane);
         mask_to_b(acc[0], mask3, 0, gz[0]);
         mask_to_b(acc[1], mask3, 1, gz[1]);
-        store_units(S, gz, tile + 5 * 64 * 32, lane);
+        stage_units(gz, S.yg.g, lane);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d2[u] = dw_tile(L, 2, wave, u, d2[u], bias_owner ? &c2[u] : nullptr, lane);
+        __syncthreads();
+        // ---- G_1, dW_1
         layer_t<2, 4>(L.w.b2, 72, gz, acc, lane);
         mask_to_b(acc[0], mask2, 0, gz[0]);
         mask_to_b(acc[1], mask2, 1, gz[1]);
-        store_units(S, gz, tile + 4 * 64 * 32, lane);
+        stage_units(gz, S.yg.g, lane);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d1[u] = dw_tile(L, 1, wave, u, d1[u], bias_owner ? &c1[u] : nullptr, lane);
+        __syncthreads();
+        // ---- G_0, dW_0 (A = the encoding of the round's rows)
         layer_t<2, 4>(L.w.b1, 72, gz, acc, lane);
         mask_to_b(acc[0], mask1, 0, gz[0]);
         mask_to_b(acc[1], mask1, 1, gz[1]);
-        store_units(S, gz, tile + 3 * 64 * 32, lane);
+        stage_units(gz, S.yg.g, lane);
+        __syncthreads();
+#pragma unroll
+        for (int ti = 0; ti < kTrainWaves; ++ti) {
+            const TrainWaveLds &T = L.wave[ti];
+            bf16x8 b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                b[u] = *reinterpret_cast<const bf16x8 *>(T.yg.g + (16 * u + r16) * kStageLd + 8 * grp);
+                if (bias_owner)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) c0[u] += (float)b[u][e];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= n_it0) break;
+                const bf16x8 a = enc_a(T.in, wave + 4 * k, lane);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d0[k][u] = mfma(a, b[u], d0[k][u]);
+            }
+        }
+        __syncthreads();
+    }
+    // ---- the block's partial gradient row, weight sum and loss
+    float *prow = partial + (size_t)blockIdx.x * kGuideParams;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (u < 3) put_dw(g, prow, 3, wave, u, d3[u], c3[u], bias_owner, lane);
+        put_dw(g, prow, 2, wave, u, d2[u], c2[u], bias_owner, lane);
+        put_dw(g, prow, 1, wave, u, d1[u], c1[u], bias_owner, lane);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (k < n_it0) put_dw(g, prow, 0, wave + 4 * k, u, d0[k][u], c0[u], bias_owner && k == 0, lane);
     }
     float l = wave_loss, w = wave_wsum;
     for (int off = 32; off > 0; off >>= 1) {
@@ -442,124 +589,83 @@ __global__ __launch_bounds__(kTrainWaves *kWave) void guide_train_tile_kernel(
         w += __shfl_xor(w, off);
     }
     if (lane == 0) {
-        if (loss && l != 0.f) atomicAdd(loss, l);
-        if (w != 0.f) atomicAdd(wsum, w);
+        red[0][wave] = w;
+        red[1][wave] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float ws = 0.f, ls = 0.f;
+        for (int k = 0; k < kTrainWaves; ++k) {
+            ws += red[0][k];
+            ls += red[1][k];
+        }
+        wl[2 * blockIdx.x] = ws;
+        wl[2 * blockIdx.x + 1] = ls;
     }
 }
 
-// One wave per (split, layer, input tile): dW over the split's tiles for all
-// of the layer's output tiles, written to partial row `split`; input tile 0
-// also writes the bias gradients (column sums of G).
-__global__ __launch_bounds__(kWave) void guide_train_dw_kernel(GuideDev g, const float *__restrict__ rec, uint32_t n,
-                                                               uint32_t stride_f, const __bf16 *__restrict__ tiles,
-                                                               uint32_t n_tiles, uint32_t chunk,
-                                                               float *__restrict__ partial) {
-    const int item = blockIdx.x % kDwItems;
-    const uint32_t split = blockIdx.x / kDwItems;
-    const int l = item < 9 ? 0 : 1 + (item - 9) / 4;
-    const int it = item < 9 ? item : (item - 9) % 4;
-    const int lane = threadIdx.x, grp = lane >> 4, r16 = lane & 15;
-    const int n_out = l == 3 ? 3 : 4;
-    const uint32_t t0 = split * chunk, t1 = min(n_tiles, t0 + chunk);
-    floatx4 acc[4];
-    float colsum[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-        colsum[u] = 0.f;
+// Parameter i's gradient: the sum of the blocks' partial rows over the total
+// weight (64 consecutive parameters per block, 4 threads each summing every
+// fourth row, so that a wave reads 256 contiguous bytes of a row, then an LDS
+// reduction). With adam, one Adam step (Kingma & Ba 2015) on the fp32 master
+// weight and its bf16 copies; grad (optional) keeps the gradient. Block 0
+// writes the total weight to *wsum and the normalised loss to *loss (each
+// optional).
+__global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, const float *__restrict__ partial,
+                                                                const float *__restrict__ wl, uint32_t rows,
+                                                                float *__restrict__ grad, float *__restrict__ wsum,
+                                                                float *__restrict__ loss, int adam, float lr, float c1,
+                                                                float c2) {
+    __shared__ float part[4][64];
+    __shared__ float tot[2];
+    const int p = threadIdx.x % 64, q = threadIdx.x / 64;
+    if (threadIdx.x < 64) {
+        // total weight (lanes 0..31) and loss (lanes 32..63) over the rows
+        const int which = threadIdx.x / 32;
+        float v = 0.f;
+        for (uint32_t r = threadIdx.x % 32; r < rows; r += 32) v += wl[2 * r + which];
+        for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+        if (threadIdx.x % 32 == 0) tot[which] = v;
     }
-#pragma unroll 4
-    for (uint32_t t = t0; t < t1; ++t) {
-        const __bf16 *tile = tiles + (size_t)t * (kGuideTileBytes / 2);
-        bf16x8 a;
-        if (l == 0) {
-            // encoding feature f of rows 8 grp .. 8 grp + 7 of the tile
-            const int f = 16 * it + r16, i = f / 24, o = (f % 24) / 2;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint32_t row = t * 32 + 8 * grp + e;
-                const float v = row < n ? rec[(size_t)row * stride_f + i] : 0.f;
-                const float r = ldexpf(v, o - 1);
-                const float rev = r - floorf(r);
-                a[e] = to_bf16((f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev));
-            }
-        } else {
-            a = reinterpret_cast<const bf16x8 *>(tile + ((l - 1) * 64 + 16 * it + r16) * 32)[grp];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (u >= n_out) break;
-            const bf16x8 b = reinterpret_cast<const bf16x8 *>(tile + ((3 + l) * 64 + 16 * u + r16) * 32)[grp];
-            acc[u] = mfma(a, b, acc[u]);
-            if (it == 0)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) colsum[u] += (float)b[e];
-        }
-    }
-    float *prow = partial + (size_t)split * kGuideParams;
-    const int din = g.din[l], dout = g.dout[l];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        if (u >= n_out) break;
-        const int out = 16 * u + r16;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int in = 16 * it + 4 * grp + e;
-            if (in < din && out < dout) prow[g.w_off[l] + in * dout + out] = acc[u][e];
-        }
-        if (it == 0) {
-            float c = colsum[u];
-            c += __shfl_xor(c, 16);
-            c += __shfl_xor(c, 32);
-            if (grp == 0 && out < dout) prow[g.b_off[l] + out] = c;
-        }
-    }
-}
-
-// grad[i] = (sum over the splits' rows of partial[s][i]) / wsum: 32
-// parameters per block, 8 threads per parameter each summing every eighth
-// row, then an LDS reduction. The tile pass weighted rows by w, not w / sum w;
-// the normalisation (of the loss too) happens here.
-__global__ __launch_bounds__(256) void guide_reduce_kernel(const float *__restrict__ partial, uint32_t rows,
-                                                           const float *__restrict__ wsum, float *__restrict__ grad,
-                                                           float *__restrict__ loss) {
-    __shared__ float part[8][32];
-    const int p = threadIdx.x % 32, q = threadIdx.x / 32;
-    const uint32_t i = blockIdx.x * 32 + p;
+    const uint32_t i = blockIdx.x * 64 + p;
     float s = 0.f;
-    if (i < (uint32_t)kGuideParams)
-        for (uint32_t r = q; r < rows; r += 8) s += partial[(size_t)r * kGuideParams + i];
+    if (i < (uint32_t)kGuideParams) {
+        float s1 = 0.f;
+        uint32_t r = q;
+        for (; r + 4 < rows; r += 8) {
+            s += partial[(size_t)r * kGuideParams + i];
+            s1 += partial[(size_t)(r + 4) * kGuideParams + i];
+        }
+        if (r < rows) s += partial[(size_t)r * kGuideParams + i];
+        s += s1;
+    }
     part[q][p] = s;
     __syncthreads();
-    const float ws = *wsum;
+    const float ws = tot[0];
     const float inv = ws > 0.f ? 1.f / ws : 0.f;
-    if (q == 0 && i < (uint32_t)kGuideParams) {
-        float t = 0.f;
-        for (int k = 0; k < 8; ++k) t += part[k][p];
-        grad[i] = t * inv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (wsum) *wsum = ws;
+        if (loss) *loss = tot[1] * inv;
     }
-    if (loss && blockIdx.x == 0 && threadIdx.x == 0) *loss *= inv;
+    if (q == 0 && i < (uint32_t)kGuideParams) {
+        const float gi = (part[0][p] + part[1][p] + part[2][p] + part[3][p]) * inv;
+        if (grad) grad[i] = gi;
+        if (adam) {
+            const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+            const float m = b1 * g.m[i] + (1.f - b1) * gi;
+            const float v = b2 * g.v[i] + (1.f - b2) * gi * gi;
+            g.m[i] = m;
+            g.v[i] = v;
+            const float pnew = g.params[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+            g.params[i] = pnew;
+            write_bf16_copy(g, i, pnew);
+        }
+    }
 }
-
 
 __global__ void zero_kernel(float *__restrict__ p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.f;
-}
-
-// Adam (Kingma & Ba 2015) on the fp32 master weights, then the bf16 copies.
-__global__ void guide_adam_kernel(GuideDev g, const float *__restrict__ grad, float lr, float c1, float c2) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)kGuideParams) return;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float gi = grad[i];
-    const float m = b1 * g.m[i] + (1.f - b1) * gi;
-    const float v = b2 * g.v[i] + (1.f - b2) * gi * gi;
-    g.m[i] = m;
-    g.v[i] = v;
-    const float p = g.params[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
-    g.params[i] = p;
-    write_bf16_copy(g, i, p);
 }
 
 __global__ void guide_sync_kernel(GuideDev g) {
@@ -612,31 +718,41 @@ hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const 
     return hipGetLastError();
 }
 
-hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
-                             float *wsum, float *loss, float *partial, void *tiles, hipStream_t st) {
-    zero_kernel<<<1, 64, 0, st>>>(wsum, 1);
-    if (loss) zero_kernel<<<1, 64, 0, st>>>(loss, 1);
-    if (n == 0) {
-        zero_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(grad, kGuideParams);
-        return hipGetLastError();
-    }
+namespace {
+// The fused pass over n records and the reduction (with or without Adam).
+hipError_t train_pass(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad, float *wsum,
+                      float *loss, float *partial, int adam, float lr, int step, hipStream_t st) {
     const uint32_t stride_f = 10u * (stride ? stride : 1u);
     const uint32_t n_tiles = (n + 31) / 32;
-    const uint32_t blocks = min((n_tiles + kTrainWaves - 1) / kTrainWaves, kTrainGrid);
-    guide_train_tile_kernel<<<blocks, kTrainWaves * kWave, 0, st>>>(g, rec, n, stride_f, wsum, (__bf16 *)tiles,
-                                                                    loss);
-    const uint32_t chunk = (n_tiles + kTrainSplits - 1) / kTrainSplits;
-    const uint32_t splits = (n_tiles + chunk - 1) / chunk;
-    guide_train_dw_kernel<<<splits * kDwItems, kWave, 0, st>>>(g, rec, n, stride_f, (const __bf16 *)tiles, n_tiles,
-                                                                chunk, partial);
-    guide_reduce_kernel<<<(kGuideParams + 31) / 32, 256, 0, st>>>(partial, splits, wsum, grad, loss);
+    // per block: a whole number of rounds (one tile per wave), at most
+    // kTrainBlocksMax blocks
+    uint32_t per_block = (n_tiles + kTrainBlocksMax - 1) / kTrainBlocksMax;
+    per_block = (per_block + kTrainWaves - 1) / kTrainWaves * kTrainWaves;
+    const uint32_t blocks = (n_tiles + per_block - 1) / per_block;
+    float *wl = partial + (size_t)kTrainBlocksMax * kGuideParams;
+    guide_train_kernel<<<blocks, kTrainThreads, 0, st>>>(g, rec, n, stride_f, per_block, partial, wl);
+    const float c1 = 1.f - powf(0.9f, (float)step), c2 = 1.f - powf(0.999f, (float)step);
+    guide_reduce_adam_kernel<<<(kGuideParams + 63) / 64, 256, 0, st>>>(g, partial, wl, blocks, grad, wsum, loss, adam,
+                                                                        lr, c1, c2);
     return hipGetLastError();
 }
+}  // namespace
 
-hipError_t guide_launch_adam(const GuideDev &g, const float *grad, float lr, int step, hipStream_t st) {
-    const float c1 = 1.f - powf(0.9f, (float)step), c2 = 1.f - powf(0.999f, (float)step);
-    guide_adam_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(g, grad, lr, c1, c2);
-    return hipGetLastError();
+hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
+                             float *wsum, float *loss, float *partial, hipStream_t st) {
+    if (n == 0) {
+        zero_kernel<<<(kGuideParams + 255) / 256, 256, 0, st>>>(grad, kGuideParams);
+        if (wsum) zero_kernel<<<1, 64, 0, st>>>(wsum, 1);
+        if (loss) zero_kernel<<<1, 64, 0, st>>>(loss, 1);
+        return hipGetLastError();
+    }
+    return train_pass(g, rec, n, stride, grad, wsum, loss, partial, 0, 0.f, 1, st);
+}
+
+hipError_t guide_launch_train_step(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float lr,
+                                   int step, float *loss, float *partial, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    return train_pass(g, rec, n, stride, nullptr, nullptr, loss, partial, 1, lr, step, st);
 }
 
 hipError_t guide_launch_sync(const GuideDev &g, hipStream_t st) {

@@ -20,9 +20,7 @@ struct illume_guide {
     GuideDev dev;
     std::vector<void *> allocs;
     float *grad = nullptr, *wsum = nullptr, *loss = nullptr;
-    float *partial = nullptr;  // per-split gradients of a training step
-    void *tiles = nullptr;     // per-tile activations and gradients of a training step
-    size_t tiles_bytes = 0;
+    float *partial = nullptr;  // per-block gradients, weight sums and losses of a training step
     float lr = 1e-3f;
     int step = 0;
 };
@@ -46,26 +44,10 @@ hipError_t dalloc(illume_guide *g, T **p, size_t count) {
 }
 }  // namespace
 
-namespace {
-// The training pass's tile scratch for n records, grown on demand (never
-// inside graph capture: training runs between replays).
-hipError_t ensure_tiles(illume_guide *g, uint32_t n) {
-    const size_t need = (size_t)((n + 31) / 32) * kGuideTileBytes;
-    if (need <= g->tiles_bytes) return hipSuccess;
-    if (g->tiles) hipFree(g->tiles);
-    g->tiles = nullptr;
-    g->tiles_bytes = 0;
-    hipError_t e = hipMalloc(&g->tiles, need);
-    if (e == hipSuccess) g->tiles_bytes = need;
-    return e;
-}
-}  // namespace
-
 extern "C" {
 
 int illume_guide_destroy(illume_guide *g) {
     if (!g) return ILLUME_OK;
-    if (g->tiles) hipFree(g->tiles);
     for (void *p : g->allocs) hipFree(p);
     delete g;
     return ILLUME_OK;
@@ -130,9 +112,7 @@ int illume_guide_infer(const illume_guide *g, const float *inputs, uint32_t n, f
 
 int illume_guide_grad(const illume_guide *g, const float *records, uint32_t n, float *grad_out, void *stream) {
     if (!g || !grad_out || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
-    hipError_t e = ensure_tiles((illume_guide *)g, n);
-    if (e == hipSuccess)
-        e = guide_launch_grad(g->dev, records, n, 1, grad_out, g->wsum, nullptr, g->partial, g->tiles, (hipStream_t)stream);
+    hipError_t e = guide_launch_grad(g->dev, records, n, 1, grad_out, g->wsum, nullptr, g->partial, (hipStream_t)stream);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide grad launch");
 }
 
@@ -142,9 +122,7 @@ namespace illume {
 // One Adam step on n records read with the given stride (in records).
 int guide_train_strided(illume_guide *g, const float *records, uint32_t n, uint32_t stride, hipStream_t st) {
     if (n == 0) return ILLUME_OK;
-    hipError_t e = ensure_tiles(g, n);
-    if (e == hipSuccess) e = guide_launch_grad(g->dev, records, n, stride, g->grad, g->wsum, g->loss, g->partial, g->tiles, st);
-    if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
+    const hipError_t e = guide_launch_train_step(g->dev, records, n, stride, g->lr, ++g->step, g->loss, g->partial, st);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
 }
 }  // namespace illume
@@ -154,10 +132,8 @@ extern "C" {
 int illume_guide_train_step(illume_guide *g, const float *records, uint32_t n, void *stream) {
     if (!g || (n && !records)) return abi_fail(ILLUME_ERR_INVALID_ARG, "null argument");
     if (n == 0) return ILLUME_OK;
-    hipStream_t st = (hipStream_t)stream;
-    hipError_t e = ensure_tiles(g, n);
-    if (e == hipSuccess) e = guide_launch_grad(g->dev, records, n, 1, g->grad, g->wsum, g->loss, g->partial, g->tiles, st);
-    if (e == hipSuccess) e = guide_launch_adam(g->dev, g->grad, g->lr, ++g->step, st);
+    const hipError_t e = guide_launch_train_step(g->dev, records, n, 1, g->lr, ++g->step, g->loss, g->partial,
+                                                 (hipStream_t)stream);
     return e == hipSuccess ? ILLUME_OK : abi_hip_fail(e, "guide train launch");
 }
 

@@ -47,11 +47,6 @@ __host__ __device__ inline int guide_wa_pos(int l, int k) {
 // arrive in the register order of guide_wa_pos.
 __host__ __device__ inline int guide_wb_pos(int l, int o) { return l == 3 ? o : guide_wa_pos(l + 1, o); }
 
-// Per 32-row tile, the training pass stores the activations X_1..X_3 and the
-// pre-activation gradients G_0..G_3 transposed ([unit][32 rows] bf16) for
-// the weight-gradient products: 3 x 64 + 4 x 64 units (G_3 padded to 64).
-constexpr size_t kGuideTileBytes = (size_t)(3 * 64 + 4 * 64) * 32 * 2;
-
 __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p);
 
 hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, float *out, hipStream_t st);
@@ -63,15 +58,20 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
 // and out.
 hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
                                    uint32_t count_stride, uint32_t stride, float *out, hipStream_t st);
-// Gradient of the weighted NLL over n records into grad (zeroed first); wsum
-// (device float) receives the weight sum, loss (optional) the loss. Record r
-// is read at rec + 10 * stride * r (stride >= 1 trains on an even subset).
-// partial: scratch of kGradPartialFloats floats (per-split gradients);
-// tiles: scratch of ceil(n / 32) * kGuideTileBytes bytes.
-constexpr size_t kGradPartialFloats = (size_t)256 * kGuideParams;
+// Training scratch (partial): one gradient row per block of the fused pass
+// (at most kTrainBlocksMax blocks), then each block's weight sum and loss.
+constexpr uint32_t kTrainBlocksMax = 256;
+constexpr size_t kGradPartialFloats = (size_t)kTrainBlocksMax * kGuideParams + 2 * kTrainBlocksMax;
+// Gradient of the weighted NLL over n records, normalised by their total
+// weight, into grad; wsum (optional, device float) receives the total weight,
+// loss (optional) the normalised loss. Record r is read at rec + 10 * stride
+// * r (stride >= 1 trains on an even subset).
 hipError_t guide_launch_grad(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float *grad,
-                             float *wsum, float *loss, float *partial, void *tiles, hipStream_t st);
-hipError_t guide_launch_adam(const GuideDev &g, const float *grad, float lr, int step, hipStream_t st);
+                             float *wsum, float *loss, float *partial, hipStream_t st);
+// One Adam step (learning rate lr, step number step >= 1) on the gradient of
+// those n records; loss (optional) as above.
+hipError_t guide_launch_train_step(const GuideDev &g, const float *rec, uint32_t n, uint32_t stride, float lr,
+                                   int step, float *loss, float *partial, hipStream_t st);
 // Rewrites the bf16 copies and biases from the fp32 master weights.
 hipError_t guide_launch_sync(const GuideDev &g, hipStream_t st);
 
