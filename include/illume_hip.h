@@ -431,12 +431,20 @@ typedef struct {
  * returns ILLUME_ERR_UNSUPPORTED before any device work. */
 int illume_render_batch(const illume_job *jobs, uint32_t n_jobs, float *film, uint64_t *ray_counts, void *stream);
 
+/* A pool of max_paths path slots (64 .. 2^26, rounded up to a multiple of
+ * 1,024) for paths of up to max_depth (1 .. 64) bounces. An unfiltered batch
+ * of up to 2^29 paths also takes a per-path sample buffer (16 bytes per path,
+ * grown on demand, kept by the pool) that one kernel sums into the film at
+ * the end of the call; larger or filtered batches add to the film with
+ * atomics. With the null stream a render runs on the pool's own stream and
+ * returns once the film is complete. */
 int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavefront **out);
 int illume_wavefront_destroy(illume_wavefront *wf);
-/* Launch shape: threads per block of the extend/shade stages (64/128/256),
- * regenerate only while fewer than regen_min slots are live (max_paths+1 =
- * always), iterations per captured graph (0 = no graph). Chosen by the
- * autotuner (rs_illume_amd/autotune.py). */
+/* Launch shape: threads per block of the stage kernels (64/128/256; each
+ * block owns a tile of 4 x block slots), a tile regenerates only while fewer
+ * than its share of regen_min slots are live (max_paths+1 or more = always),
+ * iterations per captured graph (0 = no graph; rounded up to an even count).
+ * Chosen by the autotuner (rs_illume_amd/autotune.py). */
 int illume_wavefront_set_tuning(illume_wavefront *wf, uint32_t block, uint32_t regen_min, uint32_t graph_iters);
 /* Iterations the last render call ran. */
 uint64_t illume_wavefront_last_iterations(const illume_wavefront *wf);
