@@ -100,6 +100,21 @@ int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *
                           const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width, void *stream,
                           illume_bvh **out);
 
+/* The binned SAH build of illume_bvh_create_ex (BVHAccel::new with
+ * SplitMethod::SAH, accelerators/bvh.rs:32-247) run on the device: the same
+ * split rule evaluated on the same exact bin boxes and counts, so the tree is
+ * the host build's (the same inner nodes and boxes, the same leaves; only the
+ * order of primitives inside a leaf may differ, and a node of more than 128
+ * primitives below depth 40 is split by count where the host takes an object
+ * median). Nodes of more than 128 primitives are split level by level over
+ * the whole device, smaller subtrees by one wave each; width 4 is collapsed
+ * and quantized on the device too. Traverses exactly as fast as the host
+ * tree and builds in milliseconds. Host input arrays; the build runs on
+ * `stream` and is complete on return. */
+int illume_bvh_create_gpu_sah(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                              const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
+                              void *stream, illume_bvh **out);
+
 /* Frees the handle and its device memory (Drop of BVHAccel). */
 int illume_bvh_destroy(illume_bvh *bvh);
 
@@ -256,7 +271,8 @@ typedef struct illume_sky {
 typedef struct illume_scene_opts {
     const illume_sky *sky; /* NULL: no sky */
     int32_t bvh_builder;   /* 0: binned SAH on the host (illume_bvh_create), 1: linear BVH on the device
-                              (illume_bvh_create_gpu) */
+                              (illume_bvh_create_gpu), 2: binned SAH on the device
+                              (illume_bvh_create_gpu_sah) */
     int32_t reserved;
 } illume_scene_opts;
 

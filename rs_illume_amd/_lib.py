@@ -32,6 +32,7 @@ def _declare(L):
         "illume_bvh_create": (i32, [vp, u32, vp, u32, vp, u32, i32, ctypes.POINTER(vp)]),
         "illume_bvh_create_ex": (i32, [vp, u32, vp, u32, vp, u32, i32, i32, ctypes.POINTER(vp)]),
         "illume_bvh_create_gpu": (i32, [vp, u32, vp, u32, vp, u32, i32, i32, vp, ctypes.POINTER(vp)]),
+        "illume_bvh_create_gpu_sah": (i32, [vp, u32, vp, u32, vp, u32, i32, i32, vp, ctypes.POINTER(vp)]),
         "illume_bvh_destroy": (i32, [vp]),
         "illume_bvh_info": (i32, [vp, ctypes.POINTER(BvhInfo)]),
         "illume_intersect_closest": (i32, [vp, vp, u32, vp, vp]),

@@ -15,7 +15,7 @@ import torch
 from . import _lib
 from .workloads import HIT_DTYPE, MISS, RAY_DTYPE
 
-SPLIT_METHODS = ("SAH", "LBVH")
+SPLIT_METHODS = ("SAH", "LBVH", "SAH_GPU")
 
 
 def _stream_handle(stream):
@@ -43,9 +43,10 @@ class BVHAccel:
     ``idx`` uint32 [T,3] and optional ``spheres`` float32 [S,4] (cx, cy, cz, r).
     Primitive ids are triangles 0..T-1 then spheres T..T+S-1. ``width``
     selects 64-byte two-child (2) or 128-byte four-child (4) nodes.
-    ``split_method`` "SAH" builds the binned-SAH tree on the host, "LBVH" a
-    linear BVH over Morton codes on the device (as the reference's HLBVH
-    option, bvh.rs:249-502); hits are identical.
+    ``split_method`` "SAH" builds the binned-SAH tree on the host, "SAH_GPU"
+    the same tree on the device, "LBVH" a linear BVH over Morton codes on the
+    device (as the reference's HLBVH option, bvh.rs:249-502); hits are
+    identical.
     """
 
     def __init__(self, verts, idx, spheres=None, max_prims_in_node=4, split_method="SAH", width=2):
@@ -63,7 +64,12 @@ class BVHAccel:
         torch.cuda.init()
         h = ctypes.c_void_p()
         p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a.size else None
-        if split_method == "LBVH":
+        if split_method == "SAH_GPU":
+            _lib.check(L.illume_bvh_create_gpu_sah(p(self.verts), len(self.verts), p(self.idx), self.n_tris,
+                                                   p(self.spheres), self.n_spheres, int(max_prims_in_node),
+                                                   int(width), _stream_handle(None), ctypes.byref(h)),
+                       "illume_bvh_create_gpu_sah")
+        elif split_method == "LBVH":
             _lib.check(L.illume_bvh_create_gpu(p(self.verts), len(self.verts), p(self.idx), self.n_tris,
                                                p(self.spheres), self.n_spheres, int(max_prims_in_node), int(width),
                                                _stream_handle(None), ctypes.byref(h)), "illume_bvh_create_gpu")

@@ -220,6 +220,59 @@ int illume_bvh_create_gpu(const float *verts, uint32_t n_verts, const uint32_t *
     return ILLUME_OK;
 }
 
+int illume_bvh_create_gpu_sah(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
+                              const float *spheres, uint32_t n_spheres, int max_prims_in_node, int width,
+                              void *stream, illume_bvh **out) {
+    if (!out) return fail(ILLUME_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (max_prims_in_node < 1 || max_prims_in_node > illume::kMaxLeafPrims)
+        return fail(ILLUME_ERR_UNSUPPORTED, "max_prims_in_node must be in 1..15");
+    if (width != 2 && width != 4) return fail(ILLUME_ERR_UNSUPPORTED, "width must be 2 or 4");
+    const uint64_t n = (uint64_t)n_tris + n_spheres;
+    if (n >= illume::kMaxPrimSlots) return fail(ILLUME_ERR_INVALID_ARG, "too many primitives (limit 2^27)");
+    // a handful of primitives: the host build (the same tree) handles a root leaf
+    if (n < 2 * (uint64_t)max_prims_in_node)
+        return illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node, width,
+                                    out);
+    auto t0 = std::chrono::steady_clock::now();
+    std::string err = illume::validate_prims(verts, n_verts, tri_idx, n_tris, spheres, n_spheres);
+    if (!err.empty()) return fail(ILLUME_ERR_INVALID_ARG, err);
+    illume::SahGpuResult r;
+    hipError_t e = illume::sah_gpu_build(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node,
+                                         width, (hipStream_t)stream, &r);
+    if (e == hipErrorInvalidValue) return fail(ILLUME_ERR_INVALID_ARG, "node boxes cannot be quantized (non-finite bounds)");
+    if (e != hipSuccess) return hip_fail(e, "device SAH build");
+    void *nodes = width == 4 ? (void *)r.nodes4 : (void *)r.nodes;
+    // stack entries: up to width-1 pending siblings per level of the traced tree
+    if ((int)r.depth * (width - 1) >= illume::max_supported_depth()) {
+        (void)hipFree(nodes);
+        (void)hipFree(r.prims);
+        return fail(ILLUME_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    illume_bvh *h = new (std::nothrow) illume_bvh();
+    if (!h) {
+        (void)hipFree(nodes);
+        (void)hipFree(r.prims);
+        return fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    std::memset(&h->info, 0, sizeof(h->info));
+    h->width = width;
+    hipGetDevice(&h->device);
+    h->d_nodes = nodes;
+    h->d_prims = r.prims;
+    const size_t nb = (size_t)r.n_nodes * (width == 4 ? sizeof(illume::GpuNode4Q) : sizeof(illume::GpuNode));
+    auto t1 = std::chrono::steady_clock::now();
+    h->info.n_prims = n;
+    h->info.n_inner_nodes = r.n_nodes;
+    h->info.n_leaves = r.n_leaves;
+    h->info.max_depth = r.depth;
+    h->info.device_bytes = nb + n * sizeof(illume::PrimRecord);
+    h->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    std::memcpy(h->info.bounds, r.bounds, sizeof(r.bounds));
+    *out = h;
+    return ILLUME_OK;
+}
+
 int illume_bvh_create(const float *verts, uint32_t n_verts, const uint32_t *tri_idx, uint32_t n_tris,
                       const float *spheres, uint32_t n_spheres, int max_prims_in_node, illume_bvh **out) {
     return illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, max_prims_in_node, 2, out);

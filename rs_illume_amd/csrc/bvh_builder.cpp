@@ -17,12 +17,19 @@
 //    tree depth and so the traversal stack.
 //  * Sphere boxes are widened by one ulp outwards so box culling stays
 //    conservative for hits computed with the sphere's interval arithmetic.
+//  * Partitions are stable (each side keeps its order) and object medians
+//    take the n/2 smallest (centroid, position) pairs, so the device build
+//    (sah_gpu.hip) reproduces the leaves with their primitives in the same
+//    order: the sphere test's interval check makes the reported hit depend on
+//    the order in which a leaf's primitives are tested when a sphere and
+//    another primitive are hit at nearly the same distance.
 // Closest-hit ids do not depend on the tree shape up to ties (two primitives
 // hit within an ulp or two: which one a traversal reports depends on the
 // order it reaches them), which is why the tree may differ from the
 // reference's.
 #include "bvh_builder.h"
 #include "lbvh.h"
+#include "quantize.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,9 +39,9 @@
 namespace illume {
 namespace {
 
-constexpr int kBins = 16;
-constexpr int kMedianDepth = 40;
-constexpr float kTraversalCost = 0.25f;  // a node step against one primitive test (see the header)
+constexpr int kBins = kSahBins;
+constexpr int kMedianDepth = kSahMedianDepth;
+constexpr float kTraversalCost = kSahTraversalCost;
 
 struct Box {
     float lo[3], hi[3];
@@ -57,9 +64,7 @@ struct Box {
         }
     }
     float area() const {
-        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.f;
-        return 2.f * (dx * dy + dx * dz + dy * dz);
+        return sah_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
     }
 };
 
@@ -74,6 +79,7 @@ struct Builder {
     std::vector<Box> pbox;
     std::vector<float> cen;  // 3 per prim
     std::vector<uint32_t> ids;
+    std::vector<uint32_t> scratch;
     std::vector<TmpNode> tmp;
     uint32_t max_depth = 0;
     int max_prims;
@@ -89,7 +95,22 @@ struct Builder {
         return (int)tmp.size() - 1;
     }
 
-    // Object median along the longest centroid axis.
+    // Reorders ids[s, e): the entries with left[i] first, then the others,
+    // each side in its current order (a stable partition); returns the split.
+    uint32_t stable_split(uint32_t s, uint32_t e, const std::vector<uint8_t> &left) {
+        std::vector<uint32_t> &t = scratch;
+        t.clear();
+        for (uint32_t i = s; i < e; ++i)
+            if (left[i - s]) t.push_back(ids[i]);
+        const uint32_t mid = s + (uint32_t)t.size();
+        for (uint32_t i = s; i < e; ++i)
+            if (!left[i - s]) t.push_back(ids[i]);
+        std::copy(t.begin(), t.end(), ids.begin() + s);
+        return mid;
+    }
+
+    // Object median along the longest centroid axis: the n/2 smallest
+    // (centroid, position) pairs go left.
     uint32_t median_split(uint32_t s, uint32_t e, const Box &cb) {
         int axis = 0;
         float ext = -1;
@@ -100,10 +121,15 @@ struct Builder {
                 axis = a;
             }
         }
-        uint32_t mid = (s + e) / 2;
-        std::nth_element(ids.begin() + s, ids.begin() + mid, ids.begin() + e,
-                         [&](uint32_t a, uint32_t b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
-        return mid;
+        const uint32_t n = e - s, half = n / 2;
+        std::vector<std::pair<float, uint32_t>> key(n);
+        for (uint32_t i = 0; i < n; ++i) key[i] = {cen[3 * ids[s + i] + axis], i};
+        std::vector<std::pair<float, uint32_t>> sel(key);
+        std::nth_element(sel.begin(), sel.begin() + half, sel.end());
+        const std::pair<float, uint32_t> pivot = sel[half];
+        std::vector<uint8_t> left(n);
+        for (uint32_t i = 0; i < n; ++i) left[i] = key[i] < pivot ? 1 : 0;
+        return stable_split(s, e, left);
     }
 
     int build(uint32_t s, uint32_t e, uint32_t depth) {
@@ -173,12 +199,13 @@ struct Builder {
                 int a = best_axis;
                 float k = kBins / (cb.hi[a] - cb.lo[a]);
                 float lo = cb.lo[a];
-                auto it = std::partition(ids.begin() + s, ids.begin() + e, [&](uint32_t p) {
-                    int bi = (int)((cen[3 * p + a] - lo) * k);
+                std::vector<uint8_t> left(n);
+                for (uint32_t i = s; i < e; ++i) {
+                    int bi = (int)((cen[3 * ids[i] + a] - lo) * k);
                     bi = std::min(std::max(bi, 0), kBins - 1);
-                    return bi <= best_bin;
-                });
-                mid = (uint32_t)(it - ids.begin());
+                    left[i - s] = bi <= best_bin ? 1 : 0;
+                }
+                mid = stable_split(s, e, left);
                 if (mid == s || mid == e) mid = median_split(s, e, cb);
                 split = true;
             }
@@ -603,10 +630,7 @@ uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4
         }
         return k;
     };
-    auto area = [](const Kid &k) {
-        const float dx = k.b[3] - k.b[0], dy = k.b[4] - k.b[1], dz = k.b[5] - k.b[2];
-        return 2.f * (dx * dy + dy * dz + dz * dx);
-    };
+    auto area = [](const Kid &k) { return collapse_area(k.b); };
     const float inf = std::numeric_limits<float>::infinity();
     n4->clear();
     n4->reserve(n2.size() / 2 + 1);
@@ -660,70 +684,10 @@ uint32_t collapse_to_width4(const std::vector<GpuNode> &n2, std::vector<GpuNode4
     return max_depth;
 }
 
-namespace {
-// plane of byte q: origin + q * step, float32, product exact, sum rounded
-// the device evaluates a plane with the same fused multiply-add
-inline float q_plane(float origin, float step, int q) { return std::fma((float)q, step, origin); }
-inline float pow2f(int e) { return std::ldexp(1.f, e); }
-}  // namespace
-
 bool quantize_width4(const std::vector<GpuNode4> &n4, std::vector<GpuNode4Q> *out) {
     out->resize(n4.size());
-    for (size_t i = 0; i < n4.size(); ++i) {
-        const GpuNode4 &g = n4[i];
-        GpuNode4Q q;
-        std::memset(&q, 0, sizeof(q));
-        const float *lo[3] = {g.lo_x, g.lo_y, g.lo_z}, *hi[3] = {g.hi_x, g.hi_y, g.hi_z};
-        uint8_t *qlo[3] = {q.lo_x, q.lo_y, q.lo_z}, *qhi[3] = {q.hi_x, q.hi_y, q.hi_z};
-        float *origin[3] = {&q.ox, &q.oy, &q.oz};
-        uint8_t *expo[3] = {&q.ex, &q.ey, &q.ez};
-        bool used[4];
-        for (int c = 0; c < 4; ++c) {
-            used[c] = g.child[c] != kEmptyChild;
-            q.child[c] = g.child[c];
-        }
-        for (int a = 0; a < 3; ++a) {
-            float mn = std::numeric_limits<float>::infinity(), mx = -mn;
-            for (int c = 0; c < 4; ++c)
-                if (used[c]) {
-                    if (!std::isfinite(lo[a][c]) || !std::isfinite(hi[a][c])) return false;
-                    mn = std::min(mn, lo[a][c]);
-                    mx = std::max(mx, hi[a][c]);
-                }
-            if (!(mn <= mx)) {  // no used child: any valid encoding
-                mn = mx = 0.f;
-            }
-            *origin[a] = mn;
-            // smallest power-of-two step whose 255th plane reaches mx
-            int e = -126;
-            const double ext = (double)mx - (double)mn;
-            if (ext > 0) e = std::max(-126, (int)std::ceil(std::log2(ext / 255.0)));
-            while (e < 127 && q_plane(mn, pow2f(e), 255) < mx) ++e;
-            if (q_plane(mn, pow2f(e), 255) < mx) return false;
-            *expo[a] = (uint8_t)(e + 127);
-            const float step = pow2f(e);
-            for (int c = 0; c < 4; ++c) {
-                if (!used[c]) {
-                    qlo[a][c] = 255;
-                    qhi[a][c] = 0;
-                    continue;
-                }
-                // largest q with plane(q) <= lo, smallest q with plane(q) >= hi
-                int ql = (int)std::floor(((double)lo[a][c] - (double)mn) / (double)step);
-                ql = std::min(255, std::max(0, ql));
-                while (ql > 0 && q_plane(mn, step, ql) > lo[a][c]) --ql;
-                while (ql < 255 && q_plane(mn, step, ql + 1) <= lo[a][c]) ++ql;
-                int qh = (int)std::ceil(((double)hi[a][c] - (double)mn) / (double)step);
-                qh = std::min(255, std::max(0, qh));
-                while (qh < 255 && q_plane(mn, step, qh) < hi[a][c]) ++qh;
-                while (qh > 0 && q_plane(mn, step, qh - 1) >= hi[a][c]) --qh;
-                if (q_plane(mn, step, ql) > lo[a][c] || q_plane(mn, step, qh) < hi[a][c]) return false;
-                qlo[a][c] = (uint8_t)ql;
-                qhi[a][c] = (uint8_t)qh;
-            }
-        }
-        (*out)[i] = q;
-    }
+    for (size_t i = 0; i < n4.size(); ++i)
+        if (!quantize_node(n4[i], &(*out)[i])) return false;
     return true;
 }
 

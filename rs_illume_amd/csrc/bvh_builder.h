@@ -7,6 +7,22 @@
 
 namespace illume {
 
+// Split rule of the binned SAH build, shared by the host builder and the
+// device builder (sah_gpu.hip), which make the same tree: kSahBins bins on
+// each axis over the node's centroid bounds, split cost kSahTraversalCost +
+// (n0 A0 + n1 A1) / A against leaf cost n, object medians from depth
+// kSahMedianDepth on.
+constexpr int kSahBins = 16;
+constexpr int kSahMedianDepth = 40;
+constexpr float kSahTraversalCost = 0.25f;  // a node step against one primitive test
+
+// Surface area of a box of extents dx, dy, dz (0 for an empty box), in the
+// operation order both builders use.
+__host__ __device__ inline float sah_area(float dx, float dy, float dz) {
+    if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.f;
+    return 2.f * (dx * dy + dx * dz + dy * dz);
+}
+
 struct BuildInput {
     const float *verts = nullptr;  // 3 floats per vertex
     uint32_t n_verts = 0;

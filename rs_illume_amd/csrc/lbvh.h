@@ -36,6 +36,24 @@ uint32_t binary_depth(const std::vector<GpuNode> &n2);
 hipError_t lbvh_build(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
                       const float *spheres, uint32_t n_spheres, int max_prims, hipStream_t st, LbvhResult *res);
 
+// Binned SAH build on the device (sah_gpu.hip), the host builder's split rule.
+struct SahGpuResult {
+    GpuNode *nodes = nullptr;      // width 2: device, n_nodes two-child nodes, root at 0
+    GpuNode4Q *nodes4 = nullptr;   // width 4: device, n_nodes quantized four-child nodes, root at 0
+    PrimRecord *prims = nullptr;   // device, leaf order
+    uint32_t n_nodes = 0, n_leaves = 0;
+    uint32_t depth = 0;            // levels of the emitted tree (two- or four-child)
+    uint32_t binary_depth = 0;     // inner nodes on the longest root-to-leaf path of the binary tree
+    uint32_t top_levels = 0;       // level passes of the top phase
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// Host input arrays, already validated; n_tris + n_spheres >= 2. The caller
+// owns res->nodes / res->nodes4 and res->prims (hipFree). Complete on return.
+hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *idx, uint32_t n_tris,
+                         const float *spheres, uint32_t n_spheres, int max_prims, int width, hipStream_t st,
+                         SahGpuResult *res);
+
 // Four-child nodes from a two-child BVH (any node order, root at 0): each
 // four-child node takes its binary node's two children and opens the inner
 // child of largest surface area until it has four. Depth-first order; leaf

@@ -235,7 +235,7 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     if (!out) return abi_fail(ILLUME_ERR_INVALID_ARG, "out is null");
     const illume_sky *sky = opts ? opts->sky : nullptr;
     const int builder = opts ? opts->bvh_builder : 0;
-    if (builder != 0 && builder != 1) return abi_fail(ILLUME_ERR_UNSUPPORTED, "bvh_builder must be 0 or 1");
+    if (builder < 0 || builder > 2) return abi_fail(ILLUME_ERR_UNSUPPORTED, "bvh_builder must be 0, 1 or 2");
     if (sky)
         for (int c = 0; c < 3; ++c)
             if (!(sky->zenith[c] >= 0.f && sky->horizon[c] >= 0.f && std::isfinite(sky->zenith[c]) &&
@@ -263,9 +263,11 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     illume_scene *s = new (std::nothrow) illume_scene();
     if (!s) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(&s->dev, 0, sizeof(s->dev));
-    int st = builder == 1
-                 ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, nullptr, &s->bvh)
-                 : illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, &s->bvh);
+    int st = builder == 1   ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, nullptr,
+                                                    &s->bvh)
+             : builder == 2 ? illume_bvh_create_gpu_sah(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4,
+                                                        nullptr, &s->bvh)
+                            : illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, &s->bvh);
     if (st != ILLUME_OK) {
         delete s;
         return st;

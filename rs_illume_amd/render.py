@@ -53,13 +53,18 @@ def _declare():
     return L
 
 
+# illume_scene_opts.bvh_builder codes
+BUILDERS = {"SAH": 0, "LBVH": 1, "SAH_GPU": 2}
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a.size else None
 
 
 class Scene:
     """Scene::new(aggregate, lights) over a :class:`scenes.SceneDesc` (with its
-    analytic sky, if it has one). ``builder``: "SAH" (host) or "LBVH" (device)."""
+    analytic sky, if it has one). ``builder``: "SAH" (binned SAH on the host),
+    "SAH_GPU" (the same tree built on the device) or "LBVH" (device)."""
 
     def __init__(self, desc, builder="SAH"):
         if not torch.cuda.is_available():
@@ -69,11 +74,11 @@ class Scene:
         L = _declare()
         torch.cuda.init()
         h = ctypes.c_void_p()
-        if builder not in ("SAH", "LBVH"):
-            raise ValueError("builder must be 'SAH' or 'LBVH'")
+        if builder not in BUILDERS:
+            raise ValueError("builder must be one of %s" % (tuple(BUILDERS),))
         sky = a.get("sky")
         self._sky = None if sky is None else np.ascontiguousarray(sky, np.float32)
-        opts = SceneOpts(self._sky.ctypes.data if self._sky is not None else None, 1 if builder == "LBVH" else 0, 0)
+        opts = SceneOpts(self._sky.ctypes.data if self._sky is not None else None, BUILDERS[builder], 0)
         _lib.check(L.illume_scene_create_ex(_p(a["verts"]), len(a["verts"]), _p(a["idx"]), len(a["idx"]),
                                             _p(a["spheres"]), len(a["spheres"]), _p(a["prim_material"]),
                                             _p(a["prim_emission"]), _p(a["materials"]), len(a["materials"]),
