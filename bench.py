@@ -279,7 +279,7 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
     return out
 
 
-def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
+def bench_batched(no_cpu, log=log, builder="SAH_GPU", pmc=None):
     """Config 2: 64 procedural scenes (seeds 1-64) in one batched call, twice:
     through one wavefront pool (launch shape chosen by the autotuner on a 2-spp
     probe of the same batch) and through the megakernel batch
@@ -295,6 +295,7 @@ def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
     t0 = time.time()
     scenes = [Scene(d, builder=builder) for d, _, _ in descs]
     setup_s = time.time() - t0
+    bvh_s = sum(sc.bvh_info()["build_seconds"] for sc in scenes)
     jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
     n_tris = sum(len(d.idx) for d, _, _ in descs)
     n_px = sum(int(c["width"][0]) * int(c["height"][0]) for _, c, _ in descs)
@@ -335,7 +336,7 @@ def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
             "mrays_per_s": round(float(mk_rays.sum()) / dt_mk / 1e6, 1),
             "film_finite": bool(torch.isfinite(film).all())}
     del film
-    if pmc and builder == "SAH":
+    if pmc:
         # HBM-side bytes of the batched render kernel from the counter passes
         # (same scenes, trees and launch), over this run's launch time
         gbs = pmc["traffic_bytes"] / dt_mk / 1e9
@@ -351,7 +352,10 @@ def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
     out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
                        "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one batched render call",
            "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths, "bvh_builder": builder,
-           "scene_setup_s": round(setup_s, 2), "renderer": fastest,
+           "scene_setup_s": round(setup_s, 2), "bvh_build_s": round(bvh_s, 3),
+           "end_to_end_s": round(setup_s + top["seconds"], 3),
+           "end_to_end_basis": "scene set-up (scene tables, device BVH builds) + the batched render; the numpy "
+                               "scene generation and the autotuner's probe excluded", "renderer": fastest,
            "seconds": top["seconds"], "samples_per_s": top["samples_per_s"], "mrays_per_s": top["mrays_per_s"],
            "wavefront": wave, "megakernel_batch": mega}
     if not no_cpu:
@@ -389,7 +393,7 @@ def bench_batched(no_cpu, log=log, builder="SAH", pmc=None):
 
 
 def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
-                   builder="SAH", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
+                   builder="SAH_GPU", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
     Plain (megakernel batch) and guided (wavefront pool; a proposal network
@@ -567,7 +571,9 @@ def main():
                     help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
     ap.add_argument("--width", type=int, default=4, choices=(2, 4), help="BVH node width")
-    ap.add_argument("--builder", default="SAH", choices=("SAH", "LBVH"), help="BVH builder (host SAH or device LBVH)")
+    ap.add_argument("--builder", default="SAH_GPU", choices=("SAH", "SAH_GPU", "LBVH"),
+                    help="BVH builder: binned SAH on the device (default) or the host (the same tree), or the "
+                         "device LBVH")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard with no exchange; "
@@ -577,11 +583,11 @@ def main():
     # hardware counters of the headline kernel, from child rocprofv3 passes
     # over the same workload, before this process touches the GPU
     pmc, pmc_error = None, None
-    if not args.no_counters and not args.only and args.rays == N_RAYS and args.builder == "SAH":
+    if not args.no_counters and not args.only and args.rays == N_RAYS:
         from rs_illume_amd import counters
         try:
             t0 = time.time()
-            pmc = counters.collect(args.width, log=log)
+            pmc = counters.collect(args.width, log=log, builder=args.builder)
             log("counter passes %.1f s" % (time.time() - t0))
         except Exception as e:  # reported in the JSON line; the timing still runs
             pmc_error = str(e)[-300:]
@@ -591,7 +597,7 @@ def main():
         from rs_illume_amd import counters
         try:
             t0 = time.time()
-            pmc_batched = counters.collect_batched(log=log)
+            pmc_batched = counters.collect_batched(log=log, builder=args.builder)
             log("batched counter passes %.1f s" % (time.time() - t0))
         except Exception as e:
             log("batched counter passes failed:", str(e)[-300:])
@@ -722,9 +728,9 @@ def main():
         log("guided vs plain")
         extra["guided"] = bench_guided(no_cpu=args.no_cpu)
         log("batched")
-        extra["batched"] = bench_batched(args.no_cpu, pmc=pmc_batched)
+        extra["batched"] = bench_batched(args.no_cpu, pmc=pmc_batched, builder=args.builder)
         log("combined")
-        extra["combined"] = bench_combined(no_cpu=args.no_cpu)
+        extra["combined"] = bench_combined(no_cpu=args.no_cpu, builder=args.builder)
 
     out = {
         "metric": "Mrays/s closest-hit; time-to-relMSE 0.01, guided vs plain GPU path tracer",

@@ -83,7 +83,7 @@ def _mean_for(values, pattern, counter):
 BATCH = r"render_batch_kernel"
 
 
-def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None, batched=False):
+def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None, batched=False, builder="SAH_GPU"):
     """One rocprofv3 --pmc pass over the traversal workload (or, batched, the
     64-scene batched render); returns the per-dispatch counter values."""
     out_dir = keep_dir or tempfile.mkdtemp(prefix="illume_pmc_", dir="/tmp")
@@ -92,7 +92,7 @@ def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None, batched=
     env["PYTHONPATH"] = REPO + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
     cmd = ["timeout", "-s", "KILL", str(timeout), "rocprofv3", "--pmc"] + list(counters) + \
           ["--output-format", "csv", "-d", out_dir, "-o", "run", "--", sys.executable, "-m",
-           "rs_illume_amd.workloads", "--width", str(width), "--launches", str(launches)] + \
+           "rs_illume_amd.workloads", "--width", str(width), "--launches", str(launches), "--builder", builder] + \
           (["--batched"] if batched else [])
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
     vals = counter_values(out_dir)
@@ -102,7 +102,7 @@ def run_pass(counters, width=4, launches=2, timeout=150, keep_dir=None, batched=
     return vals
 
 
-def collect(width=4, launches=2, log=None):
+def collect(width=4, launches=2, log=None, builder="SAH_GPU"):
     """Per-launch counter figures of the closest-hit traversal kernel on the
     2^20-triangle / 2^24-ray workload (see the module docstring)."""
     pattern = CLOSEST4 if width == 4 else CLOSEST2
@@ -110,7 +110,7 @@ def collect(width=4, launches=2, log=None):
     for counters in PASSES:
         if log:
             log("rocprofv3 --pmc %s" % " ".join(counters))
-        vals = run_pass(counters, width=width, launches=launches)
+        vals = run_pass(counters, width=width, launches=launches, builder=builder)
         for c in counters:
             got[c], n = _mean_for(vals, pattern, c)
             if got[c] is None:
@@ -145,7 +145,7 @@ def collect(width=4, launches=2, log=None):
     }
 
 
-def collect_batched(log=None):
+def collect_batched(log=None, builder="SAH_GPU"):
     """HBM-side traffic and L2 hit rate of one batched render launch (the
     64 procedural scenes): two passes, FETCH_SIZE + TCC_HIT_sum, then
     WRITE_SIZE + TCC_MISS_sum. FETCH_SIZE is reported raw (no calibration:
@@ -155,7 +155,7 @@ def collect_batched(log=None):
     for counters in (("FETCH_SIZE", "TCC_HIT_sum"), ("WRITE_SIZE", "TCC_MISS_sum")):
         if log:
             log("rocprofv3 --pmc %s (batched render)" % " ".join(counters))
-        vals = run_pass(counters, launches=1, timeout=240, batched=True)
+        vals = run_pass(counters, launches=1, timeout=240, batched=True, builder=builder)
         for c in counters:
             got[c], _ = _mean_for(vals, BATCH, c)
             if got[c] is None:

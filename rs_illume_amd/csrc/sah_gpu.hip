@@ -2,25 +2,28 @@
 // (accelerators/bvh.rs:32-247, recursive_build 95-247) under the host
 // builder's split rule (bvh_builder.h: kSahBins bins on each axis, split cost
 // kSahTraversalCost + (n0 A0 + n1 A1) / A against leaf cost n, count splits
-// for coincident centroids, object medians from depth kSahMedianDepth on).
-// Bins hold exact min/max boxes and integer counts, so every split decision is
-// the host's, bit for bit, and the tree is the host tree: the same inner
-// nodes with the same boxes, the same leaves (only the order of primitives
-// inside a leaf may differ). The one exception is an object median of a node
-// of more than kSmall primitives below depth kSahMedianDepth, split here by
-// count in the current order (the host selects by centroid).
+// for coincident centroids, object medians from depth kSahMedianDepth on,
+// stable partitions). Bins hold exact min/max boxes and integer counts, so
+// every split decision is the host's, bit for bit, and the tree is the host
+// tree: the same inner nodes with the same boxes, the same leaves with their
+// primitives in the same order. The one exception is an object median of a
+// node of more than kSmall primitives below depth kSahMedianDepth, split here
+// by count in the current order (the host selects by centroid).
 //
 //   1. primitive boxes (one thread per primitive), the root's box and
 //      centroid bounds;
-//   2. top phase, one pass per level over nodes of more than kSmall
-//      primitives: binning (block-level bins in LDS for the block's first
-//      node, global atomics otherwise), one wave per node evaluating the 3 x 15
-//      candidate splits, a flag per primitive, a device-wide exclusive scan,
-//      and a stable scatter into the children's ranges that also reduces the
-//      children's boxes and centroid bounds;
+//   2. top phase, one pass per level over the nodes of more than kSmall
+//      primitives. Every node's range is cut into chunks of at least kChunk
+//      primitives (at most kMaxChunks per node), one block per chunk: binning
+//      into LDS bins (each bin: count, box, centroid bounds), flushed once per
+//      block with global atomics; one wave per node evaluates the 3 x 15
+//      candidate splits and takes both children's boxes and centroid bounds
+//      from the bins; a flag per primitive, a device-wide exclusive scan, and a
+//      stable scatter of each node's range into the other buffer;
 //   3. bottom phase: one wave per node of at most kSmall primitives builds
 //      its whole subtree with the primitives in LDS (binning with LDS
-//      atomics, partition by ballot), depth first;
+//      atomics, partition by ballot), depth first, children's boxes from the
+//      bins as above;
 //   4. leaf-ordered primitive records; for width 4, the collapse (open the
 //      inner child of largest area until four) and quantization of each
 //      four-child node on the device, one launch per four-child level.
@@ -39,12 +42,13 @@
 namespace illume {
 namespace {
 
-constexpr uint32_t kNone = 0xffffffffu;
-constexpr int kSmall = 128;  // nodes of at most this many primitives: one wave builds the subtree
-constexpr int kStack = 64;   // pending nodes of a bottom-phase subtree (depth bound: 40 + 7 + 1)
+constexpr int kSmall = 128;        // nodes of at most this many primitives: one wave builds the subtree
+constexpr int kStack = 48;         // pending nodes of a bottom-phase subtree (depth bound: 40 + 7 + 1)
+constexpr uint32_t kChunk = 2048;  // primitives per block of a top-phase pass, at least
+constexpr uint32_t kMaxChunks = 256;  // blocks per node, at most (bounds the atomics on one bin)
 constexpr int kBinSlots = 3 * kSahBins;
-constexpr int kBinWords = 7;  // count, lo xyz, hi xyz (order-preserving encodings)
-constexpr int kGBinWords = 8;
+constexpr int kBinWords = 13;   // count, box lo xyz, box hi xyz, centroid lo xyz, centroid hi xyz (encoded)
+constexpr int kGBinWords = 16;  // the same, padded, in global memory
 constexpr int kMaxTopLevels = 256;
 
 // Order-preserving float <-> uint32 encoding (min/max on the encodings are
@@ -63,12 +67,13 @@ struct PBox {
     float4 lo, hi;  // lo.w: the primitive's id (raw bits)
 };
 
-struct BNode {          // a node of the top phase or the root of a bottom-phase subtree
-    uint32_t box[6];    // encoded lo xyz, hi xyz
-    uint32_t cbox[6];   // encoded centroid bounds
+// A node of the top phase or the root of a bottom-phase subtree.
+struct BNode {
+    uint32_t box[6];     // encoded lo xyz, hi xyz
+    uint32_t cbox[6];    // encoded centroid bounds
     uint32_t start, count;
-    int32_t parent;     // two-child node slot of the parent, -1 for the root
-    uint32_t depth_side;  // depth << 1 | which child of the parent
+    int32_t parent;      // two-child node slot of the parent, -1 for the root
+    uint32_t depth_side; // depth << 2 | buffer holding the range << 1 | which child of the parent
 };
 static_assert(sizeof(BNode) == 64, "BNode is 64 bytes");
 
@@ -77,13 +82,16 @@ struct Split {          // decision for a top-phase node
     uint32_t axis, bin;
     float k, lo;        // bin = clamp((int)((c - lo) * k), 0, kSahBins - 1)
     uint32_t start, lcnt;
-    uint32_t child[2];        // BNode ids
-    uint32_t child_front[2];  // next frontier index of a child of more than kSmall primitives, else kNone
-    uint32_t pad;
+    uint32_t child[2];  // BNode ids
+    uint32_t pad[3];
 };
 
+// Device counters. Two-child node slots are handed out in blocks (`alloc`):
+// one per top-phase node, m - 1 per bottom-phase subtree of m primitives (its
+// inner nodes at most), n - 1 in all; `inner` counts the slots used, `used`
+// marks them.
 struct Ctr {
-    uint32_t inner, bnodes, next_front, small, leaves, depth, overflow, n4;
+    uint32_t inner, bnodes, next_front, small, leaves, depth, overflow, n4, alloc, pad[3];
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -105,52 +113,104 @@ __device__ __forceinline__ float wave_max(float v) {
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m, int lane) {
     return (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
+__device__ __forceinline__ float pick3(int a, float x, float y, float z) { return a == 0 ? x : (a == 1 ? y : z); }
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
 __device__ __forceinline__ float centroid(const PBox &p, int a) {
-    const float lo = a == 0 ? p.lo.x : (a == 1 ? p.lo.y : p.lo.z);
-    const float hi = a == 0 ? p.hi.x : (a == 1 ? p.hi.y : p.hi.z);
-    return lo * 0.5f + hi * 0.5f;
+    return pick3(a, p.lo.x, p.lo.y, p.lo.z) * 0.5f + pick3(a, p.hi.x, p.hi.y, p.hi.z) * 0.5f;
 }
 __device__ __forceinline__ int bin_of(float c, float lo, float k) {
     const int b = (int)((c - lo) * k);
     return min(max(b, 0), kSahBins - 1);
 }
+// Number of blocks a top-phase pass gives a node of `count` primitives.
+__host__ __device__ __forceinline__ uint32_t chunks_of(uint32_t count) {
+    const uint32_t c = (count + kChunk - 1) / kChunk;
+    return c < 1 ? 1 : (c > kMaxChunks ? kMaxChunks : c);
+}
 
-// Best split over bins (kBinSlots entries of `stride` words: count, lo xyz,
-// hi xyz): lane 15 a + i evaluates the split after bin i on axis a, as the
-// host loop does; the wave's minimum cost, ties to the lowest lane (the host
-// keeps the first strictly smaller cost in axis-then-bin order).
+// Adds primitive p to the bins of axis a (bin b) at `bins` (kBinWords
+// encoded words per bin, `stride` apart).
+__device__ __forceinline__ void bin_add(uint32_t *dst, const uint32_t *e) {
+    atomicAdd(dst, 1u);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        atomicMin(dst + 1 + x, e[x]);
+        atomicMax(dst + 4 + x, e[3 + x]);
+        atomicMin(dst + 7 + x, e[6 + x]);
+        atomicMax(dst + 10 + x, e[9 + x]);
+    }
+}
+// The 12 encoded bounds of a primitive: box lo, box hi, centroid (twice).
+__device__ __forceinline__ void prim_words(const PBox &p, uint32_t *e) {
+    const float lo[3] = {p.lo.x, p.lo.y, p.lo.z}, hi[3] = {p.hi.x, p.hi.y, p.hi.z};
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const uint32_t c = enc(lo[x] * 0.5f + hi[x] * 0.5f);
+        e[x] = enc(lo[x]);
+        e[3 + x] = enc(hi[x]);
+        e[6 + x] = c;
+        e[9 + x] = c;
+    }
+}
+__device__ __forceinline__ void clear_bin(uint32_t *e) {
+    e[0] = 0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        e[1 + x] = e[7 + x] = kEncPosInf;
+        e[4 + x] = e[10 + x] = kEncNegInf;
+    }
+}
+
+// Best split over bins (kBinSlots bins of `stride` words): lane 15 a + i
+// evaluates the split after bin i on axis a, as the host loop does, and keeps
+// its two sides' unions (box lo hi, centroid lo hi: 12 floats each); the
+// wave's minimum cost, ties to the lowest lane (the host keeps the first
+// strictly smaller cost in axis-then-bin order). cost == INFINITY: no
+// candidate.
 struct Best {
     float cost;
     int lane;
     uint32_t lcnt;
 };
-__device__ __forceinline__ Best sah_best(const uint32_t *bins, int stride, const float *cb_lo, const float *cb_hi,
-                                         float parent_area, int lane) {
+__device__ __forceinline__ Best sah_best(const uint32_t *bins, int stride, const float *ext3, float parent_area,
+                                         int lane, float *lb, float *rb) {
     float cost = INFINITY;
     uint32_t lcnt = 0;
+#pragma unroll
+    for (int x = 0; x < 12; ++x) {
+        const bool lo = x < 3 || (x >= 6 && x < 9);
+        lb[x] = rb[x] = lo ? INFINITY : -INFINITY;
+    }
     if (lane < 3 * (kSahBins - 1)) {
         const int a = lane / (kSahBins - 1), i = lane % (kSahBins - 1);
-        const float ext = cb_hi[a] - cb_lo[a];
+        const float ext = pick3(a, ext3[0], ext3[1], ext3[2]);
         if (ext > 0) {
-            float l[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
             uint32_t rcnt = 0;
             for (int b = 0; b < kSahBins; ++b) {
                 const uint32_t *e = bins + (a * kSahBins + b) * stride;
                 const uint32_t cnt = e[0];
                 if (cnt == 0) continue;
-                float *acc = b <= i ? l : r;
-                if (b <= i) lcnt += cnt;
+                float v[12];
+#pragma unroll
+                for (int x = 0; x < 12; ++x) v[x] = dec(e[1 + x]);
+                const bool left = b <= i;
+                if (left) lcnt += cnt;
                 else rcnt += cnt;
-                for (int x = 0; x < 3; ++x) {
-                    acc[x] = fminf(acc[x], dec(e[1 + x]));
-                    acc[3 + x] = fmaxf(acc[3 + x], dec(e[4 + x]));
+#pragma unroll
+                for (int x = 0; x < 12; ++x) {
+                    const bool lo = x < 3 || (x >= 6 && x < 9);
+                    const float l2 = lo ? fminf(lb[x], v[x]) : fmaxf(lb[x], v[x]);
+                    const float r2 = lo ? fminf(rb[x], v[x]) : fmaxf(rb[x], v[x]);
+                    lb[x] = left ? l2 : lb[x];
+                    rb[x] = left ? rb[x] : r2;
                 }
             }
             if (lcnt != 0 && rcnt != 0)
-                cost = kSahTraversalCost + ((float)lcnt * sah_area(l[3] - l[0], l[4] - l[1], l[5] - l[2]) +
-                                            (float)rcnt * sah_area(r[3] - r[0], r[4] - r[1], r[5] - r[2])) /
+                cost = kSahTraversalCost + ((float)lcnt * sah_area(lb[3] - lb[0], lb[4] - lb[1], lb[5] - lb[2]) +
+                                            (float)rcnt * sah_area(rb[3] - rb[0], rb[4] - rb[1], rb[5] - rb[2])) /
                                                fmaxf(parent_area, 1e-30f);
         }
     }
@@ -166,7 +226,9 @@ __device__ __forceinline__ Best sah_best(const uint32_t *bins, int stride, const
             b.lcnt = n2;
         }
     }
-    return b;  // b.cost == INFINITY: no candidate
+    b.lane = __builtin_amdgcn_readfirstlane(b.lane);
+    b.lcnt = __builtin_amdgcn_readfirstlane(b.lcnt);
+    return b;
 }
 
 // Child `side` of two-child node `slot`: its box and reference.
@@ -186,8 +248,7 @@ __device__ __forceinline__ void set_child(GpuNode *nodes, int32_t slot, uint32_t
 // ---- 1. primitive boxes, the root's box and centroid bounds
 __global__ __launch_bounds__(256) void prim_kernel(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
                                                    uint32_t n_tris, const float *__restrict__ spheres, uint32_t n,
-                                                   PBox *__restrict__ pb, uint32_t *__restrict__ node_of,
-                                                   uint32_t root_front, BNode *__restrict__ root) {
+                                                   PBox *__restrict__ pb, BNode *__restrict__ root) {
     __shared__ float red[4][12];
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     float v[12];  // lo xyz, clo xyz (minimised); hi xyz, chi xyz (maximised)
@@ -214,7 +275,6 @@ __global__ __launch_bounds__(256) void prim_kernel(const float *__restrict__ ver
         b.lo = make_float4(lo[0], lo[1], lo[2], __uint_as_float(p));
         b.hi = make_float4(hi[0], hi[1], hi[2], 0.f);
         pb[p] = b;
-        node_of[p] = root_front;
         for (int a = 0; a < 3; ++a) {
             const float c = lo[a] * 0.5f + hi[a] * 0.5f;
             v[a] = lo[a];
@@ -242,7 +302,8 @@ __global__ __launch_bounds__(256) void prim_kernel(const float *__restrict__ ver
     }
 }
 
-__global__ void init_root_kernel(BNode *root, uint32_t n, uint32_t *front, uint32_t *small_list, Ctr *ctr) {
+__global__ void init_root_kernel(BNode *root, uint32_t n, uint32_t *front, uint32_t *chunk_end, uint32_t *small_list,
+                                 Ctr *ctr) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int a = 0; a < 3; ++a) {
         root->box[a] = root->cbox[a] = kEncPosInf;
@@ -252,9 +313,11 @@ __global__ void init_root_kernel(BNode *root, uint32_t n, uint32_t *front, uint3
     root->count = n;
     root->parent = -1;
     root->depth_side = 0;
-    Ctr c{0, 1, 0, 0, 0, 0, 0, 0};
+    Ctr c{};
+    c.bnodes = 1;
     if (n > (uint32_t)kSmall) {
         front[0] = 0;
+        chunk_end[0] = chunks_of(n);
         c.next_front = 1;
     } else {
         small_list[0] = 0;
@@ -266,88 +329,121 @@ __global__ void init_root_kernel(BNode *root, uint32_t n, uint32_t *front, uint3
 __global__ void clear_bins_kernel(uint32_t *__restrict__ bins, uint32_t n_entries) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_entries) return;
-    uint32_t *e = bins + (size_t)i * kGBinWords;
-    e[0] = 0;
-    e[1] = e[2] = e[3] = kEncPosInf;
-    e[4] = e[5] = e[6] = kEncNegInf;
-    e[7] = 0;
+    clear_bin(bins + (size_t)i * kGBinWords);
 }
 
 // ---- 2. top phase
-__global__ __launch_bounds__(256) void bin_kernel(const PBox *__restrict__ pb, const uint32_t *__restrict__ node_of,
-                                                  uint32_t n, const uint32_t *__restrict__ front,
+// The chunk of block b: frontier node f (chunk_end is the inclusive scan of
+// the nodes' chunk counts) and its position range [p0, p1). False past the
+// last chunk.
+struct Chunk {
+    uint32_t f, p0, p1;
+};
+__device__ __forceinline__ bool find_chunk(const uint32_t *__restrict__ chunk_end, uint32_t nf,
+                                           const uint32_t *__restrict__ front, const BNode *__restrict__ bnodes,
+                                           uint32_t b, Chunk *c) {
+    if (b >= chunk_end[nf - 1]) return false;
+    uint32_t lo = 0, hi = nf - 1;  // first f with chunk_end[f] > b
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (chunk_end[mid] > b) hi = mid;
+        else lo = mid + 1;
+    }
+    const uint32_t first = lo ? chunk_end[lo - 1] : 0u, nch = chunk_end[lo] - first;
+    const BNode &B = bnodes[front[lo]];
+    const uint32_t per = (B.count + nch - 1) / nch;
+    c->f = lo;
+    c->p0 = B.start + min(B.count, (b - first) * per);
+    c->p1 = B.start + min(B.count, (b - first + 1) * per);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void bin_kernel(const PBox *__restrict__ pb, const uint32_t *__restrict__ chunk_end,
+                                                  uint32_t nf, const uint32_t *__restrict__ front,
                                                   const BNode *__restrict__ bnodes, uint32_t *__restrict__ bins) {
     __shared__ uint32_t lb[kBinSlots * kBinWords];
-    const uint32_t first = blockIdx.x * 256;
-    const uint32_t f0 = node_of[first];
-    for (int i = threadIdx.x; i < kBinSlots; i += 256) {
-        lb[i * kBinWords] = 0;
-        for (int x = 0; x < 3; ++x) {
-            lb[i * kBinWords + 1 + x] = kEncPosInf;
-            lb[i * kBinWords + 4 + x] = kEncNegInf;
-        }
+    Chunk c;
+    if (!find_chunk(chunk_end, nf, front, bnodes, blockIdx.x, &c)) return;
+    for (int i = threadIdx.x; i < kBinSlots; i += 256) clear_bin(lb + i * kBinWords);
+    const BNode &B = bnodes[front[c.f]];
+    float lo[3], k[3];
+    bool ok[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = dec(B.cbox[a]);
+        const float ext = dec(B.cbox[3 + a]) - lo[a];
+        ok[a] = ext > 0;
+        k[a] = ok[a] ? (float)kSahBins / ext : 0.f;
     }
     __syncthreads();
-    const uint32_t pos = first + threadIdx.x;
-    const uint32_t f = pos < n ? node_of[pos] : kNone;
-    if (f != kNone) {
-        const BNode &B = bnodes[front[f]];
+    for (uint32_t pos = c.p0 + threadIdx.x; pos < c.p1; pos += 256) {
         const PBox p = pb[pos];
-        const uint32_t e[6] = {enc(p.lo.x), enc(p.lo.y), enc(p.lo.z), enc(p.hi.x), enc(p.hi.y), enc(p.hi.z)};
-        for (int a = 0; a < 3; ++a) {
-            const float lo = dec(B.cbox[a]), ext = dec(B.cbox[3 + a]) - lo;
-            if (!(ext > 0)) continue;
-            const float k = (float)kSahBins / ext;
-            const int b = bin_of(centroid(p, a), lo, k);
-            uint32_t *dst = f == f0 ? lb + (a * kSahBins + b) * kBinWords
-                                    : bins + ((size_t)f * kBinSlots + a * kSahBins + b) * kGBinWords;
-            atomicAdd(dst, 1u);
-            for (int x = 0; x < 3; ++x) {
-                atomicMin(dst + 1 + x, e[x]);
-                atomicMax(dst + 4 + x, e[3 + x]);
-            }
-        }
+        uint32_t e[12];
+        prim_words(p, e);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (ok[a]) bin_add(lb + (a * kSahBins + bin_of(centroid(p, a), lo[a], k[a])) * kBinWords, e);
     }
     __syncthreads();
-    if (f0 != kNone && threadIdx.x < kBinSlots) {
+    if (threadIdx.x < kBinSlots) {
         const uint32_t *src = lb + threadIdx.x * kBinWords;
         if (src[0]) {
-            uint32_t *dst = bins + ((size_t)f0 * kBinSlots + threadIdx.x) * kGBinWords;
+            uint32_t *dst = bins + ((size_t)c.f * kBinSlots + threadIdx.x) * kGBinWords;
             atomicAdd(dst, src[0]);
             for (int x = 0; x < 3; ++x) {
                 atomicMin(dst + 1 + x, src[1 + x]);
                 atomicMax(dst + 4 + x, src[4 + x]);
+                atomicMin(dst + 7 + x, src[7 + x]);
+                atomicMax(dst + 10 + x, src[10 + x]);
             }
         }
     }
 }
 
+__device__ __forceinline__ void enc_bounds(uint32_t *box, uint32_t *cbox, const float *v) {
+    for (int x = 0; x < 6; ++x) {
+        box[x] = enc(v[x]);
+        cbox[x] = enc(v[6 + x]);
+    }
+}
+
 // One wave per top-phase node: the split, the node's slot and its entry in
-// the parent, the two children's records.
+// the parent, the two children's records (boxes and centroid bounds from the
+// bins; a count split leaves them to the scatter), the next frontier with each
+// large child's chunk count.
 __global__ __launch_bounds__(256) void split_kernel(const uint32_t *__restrict__ front, uint32_t nf,
                                                     BNode *__restrict__ bnodes, const uint32_t *__restrict__ bins,
                                                     Split *__restrict__ split, uint32_t *__restrict__ next_front,
+                                                    uint32_t *__restrict__ next_chunks,
                                                     uint32_t *__restrict__ small_list, GpuNode *__restrict__ nodes,
-                                                    Ctr *__restrict__ ctr) {
+                                                    uint8_t *__restrict__ used, Ctr *__restrict__ ctr,
+                                                    uint32_t next_buf) {
     const uint32_t f = blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
     if (f >= nf) return;
     const uint32_t id = front[f];
     const BNode B = bnodes[id];
-    float box[6], cb_lo[3], cb_hi[3];
+    float box[6], cb_lo[3], ext3[3];
     for (int a = 0; a < 3; ++a) {
         box[a] = dec(B.box[a]);
         box[3 + a] = dec(B.box[3 + a]);
         cb_lo[a] = dec(B.cbox[a]);
-        cb_hi[a] = dec(B.cbox[3 + a]);
+        ext3[a] = dec(B.cbox[3 + a]) - cb_lo[a];
     }
-    const uint32_t n = B.count, depth = B.depth_side >> 1;
+    const uint32_t n = B.count, depth = B.depth_side >> 2;
     const float parent_area = sah_area(box[3] - box[0], box[4] - box[1], box[5] - box[2]);
-    const Best best = sah_best(bins + (size_t)f * kBinSlots * kGBinWords, kGBinWords, cb_lo, cb_hi, parent_area, lane);
+    float lb[12], rb[12];
+    const Best best = sah_best(bins + (size_t)f * kBinSlots * kGBinWords, kGBinWords, ext3, parent_area, lane, lb, rb);
+    float cv[2][12];
+#pragma unroll
+    for (int x = 0; x < 12; ++x) {
+        cv[0][x] = bcast(lb[x], best.lane);
+        cv[1][x] = bcast(rb[x], best.lane);
+    }
     if (lane != 0) return;
     Split s;
     s.start = B.start;
-    s.pad = 0;
+    s.pad[0] = s.pad[1] = s.pad[2] = 0;
     if (depth >= (uint32_t)kSahMedianDepth || best.cost == INFINITY) {
         // coincident centroids (the host's count split) or, below the median
         // depth, a count split standing in for the host's object median
@@ -359,32 +455,37 @@ __global__ __launch_bounds__(256) void split_kernel(const uint32_t *__restrict__
         s.mode = 0;
         s.axis = best.lane / (kSahBins - 1);
         s.bin = best.lane % (kSahBins - 1);
-        s.lo = cb_lo[s.axis];
-        s.k = (float)kSahBins / (cb_hi[s.axis] - cb_lo[s.axis]);
+        s.lo = pick3((int)s.axis, cb_lo[0], cb_lo[1], cb_lo[2]);
+        s.k = (float)kSahBins / pick3((int)s.axis, ext3[0], ext3[1], ext3[2]);
         s.lcnt = best.lcnt;
     }
-    const int32_t slot = (int32_t)atomicAdd(&ctr->inner, 1u);
+    const int32_t slot = (int32_t)atomicAdd(&ctr->alloc, 1u);
+    atomicAdd(&ctr->inner, 1u);
+    used[slot] = 1;
     if (B.parent >= 0) set_child(nodes, B.parent, B.depth_side & 1u, box, slot);
     const uint32_t c0 = atomicAdd(&ctr->bnodes, 2u);
     for (uint32_t k = 0; k < 2; ++k) {
         BNode c;
-        for (int a = 0; a < 3; ++a) {
-            c.box[a] = c.cbox[a] = kEncPosInf;
-            c.box[3 + a] = c.cbox[3 + a] = kEncNegInf;
+        if (s.mode == 0) {
+            enc_bounds(c.box, c.cbox, cv[k]);
+        } else {
+            for (int a = 0; a < 3; ++a) {
+                c.box[a] = c.cbox[a] = kEncPosInf;
+                c.box[3 + a] = c.cbox[3 + a] = kEncNegInf;
+            }
         }
         c.start = k ? s.start + s.lcnt : s.start;
         c.count = k ? n - s.lcnt : s.lcnt;
         c.parent = slot;
-        c.depth_side = (depth + 1) << 1 | k;
+        c.depth_side = (depth + 1) << 2 | next_buf << 1 | k;
         bnodes[c0 + k] = c;
         s.child[k] = c0 + k;
         if (c.count > (uint32_t)kSmall) {
             const uint32_t cf = atomicAdd(&ctr->next_front, 1u);
             next_front[cf] = c0 + k;
-            s.child_front[k] = cf;
+            next_chunks[cf] = chunks_of(c.count);
         } else {
             small_list[atomicAdd(&ctr->small, 1u)] = c0 + k;
-            s.child_front[k] = kNone;
         }
     }
     split[f] = s;
@@ -395,73 +496,70 @@ __device__ __forceinline__ uint32_t goes_left(const Split &s, const PBox &p, uin
     return pos - s.start < s.lcnt ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(256) void flag_kernel(const PBox *__restrict__ pb, const uint32_t *__restrict__ node_of,
-                                                   uint32_t n, const Split *__restrict__ split,
+__global__ __launch_bounds__(256) void flag_kernel(const PBox *__restrict__ pb, const uint32_t *__restrict__ chunk_end,
+                                                   uint32_t nf, const uint32_t *__restrict__ front,
+                                                   const BNode *__restrict__ bnodes, const Split *__restrict__ split,
                                                    uint32_t *__restrict__ flags) {
-    const uint32_t pos = blockIdx.x * 256 + threadIdx.x;
-    if (pos >= n) return;
-    const uint32_t f = node_of[pos];
-    flags[pos] = f == kNone ? 0u : goes_left(split[f], pb[pos], pos);
+    Chunk c;
+    if (!find_chunk(chunk_end, nf, front, bnodes, blockIdx.x, &c)) return;
+    const Split s = split[c.f];
+    for (uint32_t pos = c.p0 + threadIdx.x; pos < c.p1; pos += 256) flags[pos] = goes_left(s, pb[pos], pos);
 }
 
-// Stable partition of every top-phase node's range by its flags; primitives of
-// other nodes keep their positions. Children of more than kSmall primitives
-// get their boxes and centroid bounds here (wave-reduced atomics).
-__global__ __launch_bounds__(256) void scatter_kernel(const PBox *__restrict__ pb, const uint32_t *__restrict__ node_of,
-                                                      uint32_t n, const Split *__restrict__ split,
+// Stable partition of every top-phase node's range by its flags into the
+// other buffer. Children of count splits get their boxes and centroid bounds
+// here (wave-reduced atomics).
+__global__ __launch_bounds__(256) void scatter_kernel(const PBox *__restrict__ pb,
+                                                      const uint32_t *__restrict__ chunk_end, uint32_t nf,
+                                                      const uint32_t *__restrict__ front,
+                                                      BNode *__restrict__ bnodes, const Split *__restrict__ split,
                                                       const uint32_t *__restrict__ flags,
-                                                      const uint32_t *__restrict__ excl, PBox *__restrict__ pb2,
-                                                      uint32_t *__restrict__ node_of2, BNode *__restrict__ bnodes) {
-    const uint32_t pos = blockIdx.x * 256 + threadIdx.x;
+                                                      const uint32_t *__restrict__ excl, PBox *__restrict__ pb2) {
+    Chunk c;
+    if (!find_chunk(chunk_end, nf, front, bnodes, blockIdx.x, &c)) return;
+    const Split s = split[c.f];
     const int lane = threadIdx.x % 64;
-    uint32_t child = kNone;
-    PBox p;
-    p.lo = p.hi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (pos < n) {
-        const uint32_t f = node_of[pos];
-        p = pb[pos];
-        uint32_t np = pos, cf = kNone;
-        if (f != kNone) {
-            const Split s = split[f];
-            const uint32_t before = excl[pos] - excl[s.start];
-            const uint32_t k = flags[pos] ? 0u : 1u;
-            np = k == 0 ? s.start + before : s.start + s.lcnt + (pos - s.start - before);
-            cf = s.child_front[k];
-            if (cf != kNone) child = s.child[k];
+    const uint32_t base = excl[s.start];
+    for (uint32_t p0 = c.p0; p0 < c.p1; p0 += 256) {
+        const uint32_t pos = p0 + threadIdx.x;
+        uint32_t k = 2;  // no primitive
+        PBox p;
+        p.lo = p.hi = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (pos < c.p1) {
+            p = pb[pos];
+            const uint32_t before = excl[pos] - base;
+            k = flags[pos] ? 0u : 1u;
+            pb2[k == 0 ? s.start + before : s.start + s.lcnt + (pos - s.start - before)] = p;
         }
-        pb2[np] = p;
-        node_of2[np] = cf;
-    }
-    uint64_t pending = __ballot(child != kNone);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t c = __shfl(child, leader);
-        const uint64_t mask = __ballot(child == c);
-        const bool in = child == c;
-        float v[12];
-        for (int a = 0; a < 3; ++a) {
-            const float lo = a == 0 ? p.lo.x : (a == 1 ? p.lo.y : p.lo.z);
-            const float hi = a == 0 ? p.hi.x : (a == 1 ? p.hi.y : p.hi.z);
-            const float ce = lo * 0.5f + hi * 0.5f;
-            v[a] = in ? lo : INFINITY;
-            v[3 + a] = in ? ce : INFINITY;
-            v[6 + a] = in ? hi : -INFINITY;
-            v[9 + a] = in ? ce : -INFINITY;
-        }
-        for (int a = 0; a < 6; ++a) {
-            v[a] = wave_min(v[a]);
-            v[6 + a] = wave_max(v[6 + a]);
-        }
-        if (lane == leader) {
-            BNode &B = bnodes[c];
+        if (s.mode == 0) continue;
+        // count split: the children's bounds
+#pragma unroll
+        for (uint32_t side = 0; side < 2; ++side) {
+            const bool in = k == side;
+            if (!__ballot(in)) continue;
+            float v[12];
+            const float lo[3] = {p.lo.x, p.lo.y, p.lo.z}, hi[3] = {p.hi.x, p.hi.y, p.hi.z};
             for (int a = 0; a < 3; ++a) {
-                atomicMin(&B.box[a], enc(v[a]));
-                atomicMin(&B.cbox[a], enc(v[3 + a]));
-                atomicMax(&B.box[3 + a], enc(v[6 + a]));
-                atomicMax(&B.cbox[3 + a], enc(v[9 + a]));
+                const float ce = lo[a] * 0.5f + hi[a] * 0.5f;
+                v[a] = in ? lo[a] : INFINITY;
+                v[3 + a] = in ? ce : INFINITY;
+                v[6 + a] = in ? hi[a] : -INFINITY;
+                v[9 + a] = in ? ce : -INFINITY;
+            }
+            for (int a = 0; a < 6; ++a) {
+                v[a] = wave_min(v[a]);
+                v[6 + a] = wave_max(v[6 + a]);
+            }
+            if (lane == 0) {
+                BNode &C = bnodes[s.child[side]];
+                for (int a = 0; a < 3; ++a) {
+                    atomicMin(&C.box[a], enc(v[a]));
+                    atomicMin(&C.cbox[a], enc(v[3 + a]));
+                    atomicMax(&C.box[3 + a], enc(v[6 + a]));
+                    atomicMax(&C.cbox[3 + a], enc(v[9 + a]));
+                }
             }
         }
-        pending &= ~mask;
     }
 }
 
@@ -471,13 +569,14 @@ struct SmallLds {
     uint32_t bins[kBinSlots * kBinWords];
     uint32_t st_range[kStack];  // local start << 16 | count
     int32_t st_parent[kStack];
-    uint32_t st_ds[kStack];     // depth << 1 | side
+    uint32_t st_ds[kStack];     // depth << 2 | bounds known << 1 | side
+    float st_b[kStack][12];     // box lo hi, centroid lo hi (when known)
 };
 
-__global__ __launch_bounds__(256) void small_kernel(PBox *__restrict__ pb, const uint32_t *__restrict__ small_list,
-                                                    uint32_t n_small, const BNode *__restrict__ bnodes,
-                                                    GpuNode *__restrict__ nodes, Ctr *__restrict__ ctr,
-                                                    int max_prims) {
+__global__ __launch_bounds__(256) void small_kernel(PBox *__restrict__ pb0, const PBox *__restrict__ pb1,
+                                                    const uint32_t *__restrict__ small_list, uint32_t n_small,
+                                                    const BNode *__restrict__ bnodes, GpuNode *__restrict__ nodes,
+                                                    uint8_t *__restrict__ used, Ctr *__restrict__ ctr, int max_prims) {
     __shared__ SmallLds lds[4];
     const uint32_t item = blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
@@ -485,133 +584,140 @@ __global__ __launch_bounds__(256) void small_kernel(PBox *__restrict__ pb, const
     SmallLds &L = lds[threadIdx.x / 64];
     const BNode R = bnodes[small_list[item]];
     const uint32_t s0 = R.start, m0 = R.count;
+    const PBox *src = (R.depth_side >> 1) & 1u ? pb1 : pb0;
     for (uint32_t j = lane; j < m0; j += 64) {
-        const PBox p = pb[s0 + j];
+        const PBox p = src[s0 + j];
         L.lo[j] = p.lo;
         L.hi[j] = p.hi;
     }
-    int sp = 0;
-    if (lane == 0) {
-        L.st_range[0] = m0;
-        L.st_parent[0] = R.parent;
-        L.st_ds[0] = R.depth_side;
+    // the root's bounds: from its parent's bins, the scatter's reduction or
+    // the primitive pass (every top-phase child has them)
+    float nb[12];
+    for (int x = 0; x < 6; ++x) {
+        nb[x] = dec(R.box[x]);
+        nb[6 + x] = dec(R.cbox[x]);
     }
-    sp = 1;
+    uint32_t s = 0, m = m0, ds = (R.depth_side >> 2) << 2 | 2u | (R.depth_side & 1u);
+    int32_t parent = R.parent;
+    int sp = 0;
+    // the subtree's node slots: at most m0 - 1 inner nodes
+    uint32_t base = 0, n_inner = 0;
+    if (lane == 0 && m0 > 1) base = atomicAdd(&ctr->alloc, m0 - 1);
+    base = __shfl(base, 0);
     wave_lds_sync();
     uint32_t leaves = 0, max_depth = 0;
     bool overflow = false;
-    while (sp > 0 && !overflow) {
-        --sp;
-        uint32_t s = L.st_range[sp] >> 16, m = L.st_range[sp] & 0xffffu;
-        int32_t parent = L.st_parent[sp];
-        uint32_t ds = L.st_ds[sp];
-        wave_lds_sync();
-        for (;;) {
-            const uint32_t depth = ds >> 1;
-            // this lane's primitives: local s + lane and s + 64 + lane
-            PBox p[2];
-            bool valid[2];
+    for (;;) {
+        const uint32_t depth = ds >> 2;
+        // this lane's primitives: local s + lane and s + 64 + lane
+        PBox p[2];
+        bool valid[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t j = (uint32_t)lane + 64u * h;
+            valid[h] = j < m;
+            p[h].lo = valid[h] ? L.lo[s + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            p[h].hi = valid[h] ? L.hi[s + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (!((ds >> 1) & 1u)) {
+            // bounds not known (a count split or an object median made the
+            // node): reduce them over its primitives
             float v[12];
-            for (int a = 0; a < 6; ++a) v[a] = INFINITY, v[6 + a] = -INFINITY;
+            for (int x = 0; x < 6; ++x) v[x] = INFINITY, v[6 + x] = -INFINITY;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const uint32_t j = (uint32_t)lane + 64u * h;
-                valid[h] = j < m;
-                if (valid[h]) {
-                    p[h].lo = L.lo[s + j];
-                    p[h].hi = L.hi[s + j];
-                    const float lo[3] = {p[h].lo.x, p[h].lo.y, p[h].lo.z}, hi[3] = {p[h].hi.x, p[h].hi.y, p[h].hi.z};
-                    for (int a = 0; a < 3; ++a) {
-                        const float c = lo[a] * 0.5f + hi[a] * 0.5f;
-                        v[a] = fminf(v[a], lo[a]);
-                        v[3 + a] = fminf(v[3 + a], c);
-                        v[6 + a] = fmaxf(v[6 + a], hi[a]);
-                        v[9 + a] = fmaxf(v[9 + a], c);
-                    }
-                } else {
-                    p[h].lo = p[h].hi = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (!valid[h]) continue;
+                const float lo[3] = {p[h].lo.x, p[h].lo.y, p[h].lo.z}, hi[3] = {p[h].hi.x, p[h].hi.y, p[h].hi.z};
+                for (int a = 0; a < 3; ++a) {
+                    const float c = lo[a] * 0.5f + hi[a] * 0.5f;
+                    v[a] = fminf(v[a], lo[a]);
+                    v[3 + a] = fminf(v[3 + a], c);
+                    v[6 + a] = fmaxf(v[6 + a], hi[a]);
+                    v[9 + a] = fmaxf(v[9 + a], c);
                 }
             }
-            for (int a = 0; a < 6; ++a) {
-                v[a] = wave_min(v[a]);
-                v[6 + a] = wave_max(v[6 + a]);
+            for (int x = 0; x < 6; ++x) {
+                v[x] = wave_min(v[x]);
+                v[6 + x] = wave_max(v[6 + x]);
             }
-            const float box[6] = {v[0], v[1], v[2], v[6], v[7], v[8]};
-            const float cb_lo[3] = {v[3], v[4], v[5]}, cb_hi[3] = {v[9], v[10], v[11]};
-            // decision, in the host builder's order
-            int mode = -1;  // -1 leaf, 0 bins, 1 first half, 2 object median
-            int axis = 0, bin = 0;
-            float k = 0.f, lo_ax = 0.f;
-            if (m == 1) {
-                mode = -1;
-            } else if (depth >= (uint32_t)kSahMedianDepth) {
-                mode = m <= (uint32_t)max_prims ? -1 : 2;
-            } else {
-                for (int i = lane; i < kBinSlots; i += 64) {
-                    L.bins[i * kBinWords] = 0;
-                    for (int x = 0; x < 3; ++x) {
-                        L.bins[i * kBinWords + 1 + x] = kEncPosInf;
-                        L.bins[i * kBinWords + 4 + x] = kEncNegInf;
-                    }
-                }
-                wave_lds_sync();
+            // -> box lo hi, centroid lo hi
+            for (int a = 0; a < 3; ++a) {
+                nb[a] = v[a];
+                nb[3 + a] = v[6 + a];
+                nb[6 + a] = v[3 + a];
+                nb[9 + a] = v[9 + a];
+            }
+        }
+        const float *box = nb;
+        const float ext3[3] = {nb[9] - nb[6], nb[10] - nb[7], nb[11] - nb[8]};
+        // decision, in the host builder's order
+        int mode = -1;  // -1 leaf, 0 bins, 1 first half, 2 object median
+        int axis = 0, bin = 0;
+        float k = 0.f, lo_ax = 0.f;
+        float cv[2][12];
+        if (m == 1) {
+            mode = -1;
+        } else if (depth >= (uint32_t)kSahMedianDepth) {
+            mode = m <= (uint32_t)max_prims ? -1 : 2;
+        } else {
+            for (int i = lane; i < kBinSlots; i += 64) clear_bin(L.bins + i * kBinWords);
+            wave_lds_sync();
+            float kk[3];
+            for (int a = 0; a < 3; ++a) kk[a] = ext3[a] > 0 ? (float)kSahBins / ext3[a] : 0.f;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (!valid[h]) continue;
-                    const uint32_t e[6] = {enc(p[h].lo.x), enc(p[h].lo.y), enc(p[h].lo.z),
-                                           enc(p[h].hi.x), enc(p[h].hi.y), enc(p[h].hi.z)};
-                    for (int a = 0; a < 3; ++a) {
-                        const float ext = cb_hi[a] - cb_lo[a];
-                        if (!(ext > 0)) continue;
-                        const int b = bin_of(centroid(p[h], a), cb_lo[a], (float)kSahBins / ext);
-                        uint32_t *dst = L.bins + (a * kSahBins + b) * kBinWords;
-                        atomicAdd(dst, 1u);
-                        for (int x = 0; x < 3; ++x) {
-                            atomicMin(dst + 1 + x, e[x]);
-                            atomicMax(dst + 4 + x, e[3 + x]);
-                        }
-                    }
-                }
-                wave_lds_sync();
-                const float parent_area = sah_area(box[3] - box[0], box[4] - box[1], box[5] - box[2]);
-                const Best best = sah_best(L.bins, kBinWords, cb_lo, cb_hi, parent_area, lane);
-                wave_lds_sync();
-                if (best.cost == INFINITY) {
-                    mode = m <= (uint32_t)max_prims ? -1 : 1;
-                } else if (m <= (uint32_t)max_prims && best.cost >= (float)m) {
-                    mode = -1;
-                } else {
-                    mode = 0;
-                    axis = best.lane / (kSahBins - 1);
-                    bin = best.lane % (kSahBins - 1);
-                    lo_ax = cb_lo[axis];
-                    k = (float)kSahBins / (cb_hi[axis] - cb_lo[axis]);
+            for (int h = 0; h < 2; ++h) {
+                if (!valid[h]) continue;
+                uint32_t e[12];
+                prim_words(p[h], e);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (ext3[a] > 0)
+                        bin_add(L.bins + (a * kSahBins + bin_of(centroid(p[h], a), nb[6 + a], kk[a])) * kBinWords, e);
+            }
+            wave_lds_sync();
+            const float parent_area = sah_area(box[3] - box[0], box[4] - box[1], box[5] - box[2]);
+            float lb[12], rb[12];
+            const Best best = sah_best(L.bins, kBinWords, ext3, parent_area, lane, lb, rb);
+            wave_lds_sync();
+            if (best.cost == INFINITY) {
+                mode = m <= (uint32_t)max_prims ? -1 : 1;
+            } else if (m <= (uint32_t)max_prims && best.cost >= (float)m) {
+                mode = -1;
+            } else {
+                mode = 0;
+                axis = best.lane / (kSahBins - 1);
+                bin = best.lane % (kSahBins - 1);
+                lo_ax = pick3(axis, nb[6], nb[7], nb[8]);
+                k = (float)kSahBins / pick3(axis, ext3[0], ext3[1], ext3[2]);
+#pragma unroll
+                for (int x = 0; x < 12; ++x) {
+                    cv[0][x] = bcast(lb[x], best.lane);
+                    cv[1][x] = bcast(rb[x], best.lane);
                 }
             }
-            if (mode < 0) {
-                if (lane == 0 && parent >= 0) set_child(nodes, parent, ds & 1u, box, make_leaf_ref(s0 + s, m));
-                ++leaves;
-                max_depth = max(max_depth, depth);
-                break;
-            }
-            int32_t slot = 0;
+        }
+        bool pop = false;
+        if (mode < 0) {
+            if (lane == 0 && parent >= 0) set_child(nodes, parent, ds & 1u, box, make_leaf_ref(s0 + s, m));
+            ++leaves;
+            max_depth = max(max_depth, depth);
+            pop = true;
+        } else {
+            const int32_t slot = (int32_t)(base + n_inner++);
             if (lane == 0) {
-                slot = (int32_t)atomicAdd(&ctr->inner, 1u);
+                used[slot] = 1;
                 if (parent >= 0) set_child(nodes, parent, ds & 1u, box, slot);
             }
-            slot = __shfl(slot, 0);
             // flags
             bool left[2] = {false, false};
             int med_axis = 0;
             if (mode == 2) {
                 float ext = -1.f;
-                for (int a = 0; a < 3; ++a) {
-                    const float x = cb_hi[a] - cb_lo[a];
-                    if (x > ext) {
-                        ext = x;
+                for (int a = 0; a < 3; ++a)
+                    if (ext3[a] > ext) {
+                        ext = ext3[a];
                         med_axis = a;
                     }
-                }
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -622,7 +728,7 @@ __global__ __launch_bounds__(256) void small_kernel(PBox *__restrict__ pb, const
                 } else if (mode == 1) {
                     left[h] = j < m / 2;
                 } else {
-                    // object median: rank by centroid, ties by position
+                    // object median: rank by (centroid, position)
                     const float cj = centroid(p[h], med_axis);
                     uint32_t rank = 0;
                     for (uint32_t t = 0; t < m; ++t) {
@@ -655,25 +761,45 @@ __global__ __launch_bounds__(256) void small_kernel(PBox *__restrict__ pb, const
                 overflow = true;
                 break;
             }
+            const uint32_t known = mode == 0 ? 2u : 0u;
             if (lane == 0) {
                 L.st_range[sp] = (s + nl) << 16 | (m - nl);
                 L.st_parent[sp] = slot;
-                L.st_ds[sp] = (depth + 1) << 1 | 1u;
+                L.st_ds[sp] = (depth + 1) << 2 | known | 1u;
+            }
+            if (mode == 0) {
+#pragma unroll
+                for (int x = 0; x < 12; ++x) {
+                    if (lane == 0) L.st_b[sp][x] = cv[1][x];
+                    nb[x] = cv[0][x];
+                }
             }
             ++sp;
             wave_lds_sync();
             m = nl;
             parent = slot;
-            ds = (depth + 1) << 1;
+            ds = (depth + 1) << 2 | known;
+        }
+        if (pop) {
+            if (sp == 0) break;
+            --sp;
+            s = L.st_range[sp] >> 16;
+            m = L.st_range[sp] & 0xffffu;
+            parent = L.st_parent[sp];
+            ds = L.st_ds[sp];
+#pragma unroll
+            for (int x = 0; x < 12; ++x) nb[x] = L.st_b[sp][x];
+            wave_lds_sync();
         }
     }
     for (uint32_t j = lane; j < m0; j += 64) {
         PBox p;
         p.lo = L.lo[j];
         p.hi = L.hi[j];
-        pb[s0 + j] = p;
+        pb0[s0 + j] = p;
     }
     if (lane == 0) {
+        atomicAdd(&ctr->inner, n_inner);
         atomicAdd(&ctr->leaves, leaves);
         atomicMax(&ctr->depth, max_depth);
         if (overflow) atomicOr(&ctr->overflow, 1u);
@@ -728,45 +854,68 @@ __device__ __forceinline__ Kid kid_of(const GpuNode &g, int c) {
 // One thread per four-child node of this level (item = binary node, slot):
 // the children by the host collapse's rule, slots for the inner ones
 // (consecutive), the quantized node.
-// cap bounds the items, the next items and the four-child slots (the binary
-// tree's inner-node count); a malformed tree sets overflow bit 4 instead of
-// writing past it.
+// k slots from *ctr for every lane of the wave, one atomic per wave: the
+// lane's first slot (its k may be 0). Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t *ctr, uint32_t k, int lane) {
+    uint32_t x = k;  // inclusive prefix sum over the lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(ctr, total);
+    return __shfl(base, 0) + x - k;
+}
+
+// cap2 bounds the binary node indices (the slots handed out), cap4 the items,
+// the next items and the four-child slots (the binary tree's inner-node
+// count); a malformed tree sets overflow bit 4 instead of reading or writing
+// past them.
 __global__ __launch_bounds__(256) void collapse_kernel(const GpuNode *__restrict__ n2, const uint2 *__restrict__ items,
                                                        uint32_t n_items, uint2 *__restrict__ next,
-                                                       GpuNode4Q *__restrict__ out, uint32_t cap,
+                                                       GpuNode4Q *__restrict__ out, uint32_t cap2, uint32_t cap4,
                                                        Ctr *__restrict__ ctr) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_items) return;
-    const uint2 it = items[i];
-    Kid kids[4];
-    int nk = 2;
-    if (it.x >= cap || it.y >= cap) {
+    const int lane = threadIdx.x % 64;
+    bool ok = i < n_items;
+    const uint2 it = ok ? items[i] : make_uint2(0u, 0u);
+    if (ok && (it.x >= cap2 || it.y >= cap4)) {
         atomicOr(&ctr->overflow, 4u);
-        return;
+        ok = false;
     }
-    kids[0] = kid_of(n2[it.x], 0);
-    kids[1] = kid_of(n2[it.x], 1);
-    while (nk < 4) {
-        int best = -1;
-        float best_area = -1.f;
-        for (int c = 0; c < nk; ++c)
-            if (kids[c].ref >= 0 && collapse_area(kids[c].b) > best_area) {
-                best_area = collapse_area(kids[c].b);
-                best = c;
+    Kid kids[4];
+    int nk = 0;
+    if (ok) {
+        nk = 2;
+        kids[0] = kid_of(n2[it.x], 0);
+        kids[1] = kid_of(n2[it.x], 1);
+        while (nk < 4) {
+            int best = -1;
+            float best_area = -1.f;
+            for (int c = 0; c < nk; ++c)
+                if (kids[c].ref >= 0 && collapse_area(kids[c].b) > best_area) {
+                    best_area = collapse_area(kids[c].b);
+                    best = c;
+                }
+            if (best < 0) break;
+            if ((uint32_t)kids[best].ref >= cap2) {
+                atomicOr(&ctr->overflow, 4u);
+                ok = false;
+                break;
             }
-        if (best < 0) break;
-        if ((uint32_t)kids[best].ref >= cap) {
-            atomicOr(&ctr->overflow, 4u);
-            return;
+            const GpuNode g = n2[kids[best].ref];
+            kids[best] = kid_of(g, 0);
+            kids[nk++] = kid_of(g, 1);
         }
-        const GpuNode g = n2[kids[best].ref];
-        kids[best] = kid_of(g, 0);
-        kids[nk++] = kid_of(g, 1);
     }
     uint32_t n_inner = 0;
-    for (int c = 0; c < nk; ++c) n_inner += kids[c].ref >= 0 ? 1u : 0u;
-    const uint32_t base = n_inner ? atomicAdd(&ctr->n4, n_inner) : 0u;
-    const uint32_t nbase = n_inner ? atomicAdd(&ctr->next_front, n_inner) : 0u;
+    if (ok)
+        for (int c = 0; c < nk; ++c) n_inner += kids[c].ref >= 0 ? 1u : 0u;
+    const uint32_t base = wave_alloc(&ctr->n4, n_inner, lane);
+    const uint32_t nbase = wave_alloc(&ctr->next_front, n_inner, lane);
+    if (!ok) return;
     GpuNode4 g4;
     uint32_t used = 0;
     for (int c = 0; c < 4; ++c) {
@@ -783,7 +932,7 @@ __global__ __launch_bounds__(256) void collapse_kernel(const GpuNode *__restrict
             g4.child[c] = k.ref;
         } else {
             g4.child[c] = (int32_t)(base + used);
-            if (nbase + used < cap) next[nbase + used] = make_uint2((uint32_t)k.ref, base + used);
+            if (nbase + used < cap4 && base + used < cap4) next[nbase + used] = make_uint2((uint32_t)k.ref, base + used);
             else atomicOr(&ctr->overflow, 4u);
             ++used;
         }
@@ -791,6 +940,18 @@ __global__ __launch_bounds__(256) void collapse_kernel(const GpuNode *__restrict
     GpuNode4Q q;
     if (!quantize_node(g4, &q)) atomicOr(&ctr->overflow, 2u);
     out[it.y] = q;
+}
+
+// Width 2: the used slots, in slot order, renumbered by the exclusive scan
+// `idx` of the used marks (child references follow).
+__global__ void compact_kernel(const GpuNode *__restrict__ n2, const uint8_t *__restrict__ used,
+                               const uint32_t *__restrict__ idx, uint32_t n_slots, GpuNode *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots || !used[i]) return;
+    GpuNode g = n2[i];
+    if (g.child0 >= 0) g.child0 = (int32_t)idx[g.child0];
+    if (g.child1 >= 0) g.child1 = (int32_t)idx[g.child1];
+    out[idx[i]] = g;
 }
 
 __global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
@@ -801,7 +962,8 @@ __global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
 __global__ void collapse_init_kernel(uint2 *items, Ctr *ctr) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     items[0] = make_uint2(0u, 0u);
-    Ctr c{0, 0, 0, 0, 0, 0, 0, 1};
+    Ctr c{};
+    c.n4 = 1;
     *ctr = c;
 }
 
@@ -845,18 +1007,21 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
     // frontier nodes of one level hold more than kSmall primitives each
     const uint32_t front_cap = n / (kSmall + 1) + 1;
     PBox *pb[2];
-    uint32_t *node_of[2], *flags, *excl, *front[2], *small_list, *bins;
+    uint32_t *flags, *excl, *front[2], *next_chunks, *chunk_end, *small_list, *bins;
+    uint8_t *used;
     Split *split;
     Ctr *ctr;
     GpuNode *n2;
     if ((e = tmp.alloc(&pb[0], n)) != hipSuccess || (e = tmp.alloc(&pb[1], n)) != hipSuccess ||
-        (e = tmp.alloc(&node_of[0], n)) != hipSuccess || (e = tmp.alloc(&node_of[1], n)) != hipSuccess ||
         (e = tmp.alloc(&flags, n)) != hipSuccess || (e = tmp.alloc(&excl, n)) != hipSuccess ||
         (e = tmp.alloc(&front[0], front_cap)) != hipSuccess || (e = tmp.alloc(&front[1], front_cap)) != hipSuccess ||
+        (e = tmp.alloc(&next_chunks, front_cap)) != hipSuccess || (e = tmp.alloc(&chunk_end, front_cap)) != hipSuccess ||
         (e = tmp.alloc(&small_list, n)) != hipSuccess ||
         (e = tmp.alloc(&bins, (size_t)front_cap * kBinSlots * kGBinWords)) != hipSuccess ||
-        (e = tmp.alloc(&split, front_cap)) != hipSuccess || (e = tmp.alloc(&ctr, 1)) != hipSuccess)
+        (e = tmp.alloc(&split, front_cap)) != hipSuccess || (e = tmp.alloc(&ctr, 1)) != hipSuccess ||
+        (e = tmp.alloc(&used, n)) != hipSuccess)
         return e;
+    if ((e = hipMemsetAsync(used, 0, n, st)) != hipSuccess) return e;
     // BNodes: the root, then two per top-phase split; grown on demand
     size_t bnode_cap = 1 + 2 * (size_t)front_cap * 4;
     BNode *bnodes = nullptr;
@@ -874,21 +1039,23 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
         }
     } nguard{&n2};
     if ((e = hipMemsetAsync(n2, 0, (size_t)std::max<uint32_t>(n - 1, 1) * sizeof(GpuNode), st)) != hipSuccess) return e;
-    size_t scan_bytes = 0;
+    size_t scan_bytes = 0, cscan_bytes = 0;
     if ((e = rocprim::exclusive_scan(nullptr, scan_bytes, flags, excl, 0u, n, rocprim::plus<uint32_t>(), st)) !=
-        hipSuccess)
+            hipSuccess ||
+        (e = rocprim::inclusive_scan(nullptr, cscan_bytes, next_chunks, chunk_end, front_cap,
+                                     rocprim::plus<uint32_t>(), st)) != hipSuccess)
         return e;
+    scan_bytes = std::max(scan_bytes, cscan_bytes);
     void *scan_tmp = nullptr;
     if ((e = tmp.alloc((char **)&scan_tmp, scan_bytes)) != hipSuccess) return e;
 
-    init_root_kernel<<<1, 64, 0, st>>>(bnodes, n, front[0], small_list, ctr);
-    prim_kernel<<<blocks_for(n, 256), 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, pb[0], node_of[0],
-                                                   n > (uint32_t)kSmall ? 0u : kNone, bnodes);
+    init_root_kernel<<<1, 64, 0, st>>>(bnodes, n, front[0], chunk_end, small_list, ctr);
+    prim_kernel<<<blocks_for(n, 256), 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, pb[0], bnodes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     Ctr hc;
     if ((e = hipMemcpyAsync(&hc, ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    int cur = 0;
+    uint32_t cur = 0;
     uint32_t nf = hc.next_front;
     for (int level = 0; nf > 0; ++level) {
         if (level >= kMaxTopLevels) return hipErrorUnknown;
@@ -906,34 +1073,41 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
             bnodes = b2;
             bnode_cap = cap2;
         }
+        const uint32_t nxt = cur ^ 1u;
+        // chunks: at most ceil(count / kChunk) per node, so this many blocks cover them
+        const uint32_t grid = n / kChunk + nf + 1;
         set_u32_kernel<<<1, 64, 0, st>>>(&ctr->next_front, 0u);
         clear_bins_kernel<<<blocks_for((uint64_t)nf * kBinSlots, 256), 256, 0, st>>>(bins, nf * kBinSlots);
-        bin_kernel<<<blocks_for(n, 256), 256, 0, st>>>(pb[cur], node_of[cur], n, front[cur], bnodes, bins);
-        split_kernel<<<blocks_for(nf, 4), 256, 0, st>>>(front[cur], nf, bnodes, bins, split, front[cur ^ 1],
-                                                       small_list, n2, ctr);
-        flag_kernel<<<blocks_for(n, 256), 256, 0, st>>>(pb[cur], node_of[cur], n, split, flags);
+        bin_kernel<<<grid, 256, 0, st>>>(pb[cur], chunk_end, nf, front[cur], bnodes, bins);
+        split_kernel<<<blocks_for(nf, 4), 256, 0, st>>>(front[cur], nf, bnodes, bins, split, front[nxt], next_chunks,
+                                                       small_list, n2, used, ctr, nxt);
+        flag_kernel<<<grid, 256, 0, st>>>(pb[cur], chunk_end, nf, front[cur], bnodes, split, flags);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = rocprim::exclusive_scan(scan_tmp, scan_bytes, flags, excl, 0u, n, rocprim::plus<uint32_t>(), st)) !=
             hipSuccess)
             return e;
-        scatter_kernel<<<blocks_for(n, 256), 256, 0, st>>>(pb[cur], node_of[cur], n, split, flags, excl,
-                                                          pb[cur ^ 1], node_of[cur ^ 1], bnodes);
+        scatter_kernel<<<grid, 256, 0, st>>>(pb[cur], chunk_end, nf, front[cur], bnodes, split, flags, excl, pb[nxt]);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        // the next level's chunk table (entries past its frontier are never read)
+        if ((e = rocprim::inclusive_scan(scan_tmp, scan_bytes, next_chunks, chunk_end, front_cap,
+                                         rocprim::plus<uint32_t>(), st)) != hipSuccess)
+            return e;
         if ((e = hipMemcpyAsync(&hc, ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-        cur ^= 1;
+        cur = nxt;
         nf = hc.next_front;
         res->top_levels = level + 1;
     }
     if (hc.small > 0)
-        small_kernel<<<blocks_for(hc.small, 4), 256, 0, st>>>(pb[cur], small_list, hc.small, bnodes, n2, ctr,
-                                                              max_prims);
+        small_kernel<<<blocks_for(hc.small, 4), 256, 0, st>>>(pb[0], pb[1], small_list, hc.small, bnodes, n2, used,
+                                                              ctr, max_prims);
     PrimRecord *prims = nullptr;
     if ((e = hipMalloc(&prims, (size_t)n * sizeof(PrimRecord))) != hipSuccess) return e;
-    records_kernel<<<blocks_for(n, 256), 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, pb[cur], prims);
+    records_kernel<<<blocks_for(n, 256), 256, 0, st>>>(d_verts, d_idx, n_tris, d_sph, n, pb[0], prims);
     if ((e = hipGetLastError()) == hipSuccess) e = hipMemcpyAsync(&hc, ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess && (hc.overflow || hc.inner != hc.leaves - 1)) e = hipErrorUnknown;
+    if (e == hipSuccess && (hc.overflow || hc.inner != hc.leaves - 1 || hc.alloc > std::max<uint32_t>(n - 1, 1)))
+        e = hipErrorUnknown;
     // the root's box
     if (e == hipSuccess) {
         uint32_t rb[6];
@@ -953,8 +1127,30 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
     res->binary_depth = hc.depth;
     res->n_nodes = hc.inner;
     if (width == 2) {
-        nguard.keep = true;
-        res->nodes = n2;
+        // the used slots, compacted in slot order
+        uint32_t *cidx = nullptr;
+        GpuNode *out = nullptr;
+        size_t cbytes = 0;
+        void *ctmp = nullptr;
+        e = tmp.alloc(&cidx, hc.alloc);
+        if (e == hipSuccess)
+            e = rocprim::exclusive_scan(nullptr, cbytes, used, cidx, 0u, hc.alloc, rocprim::plus<uint32_t>(), st);
+        if (e == hipSuccess) e = tmp.alloc((char **)&ctmp, cbytes);
+        if (e == hipSuccess)
+            e = rocprim::exclusive_scan(ctmp, cbytes, used, cidx, 0u, hc.alloc, rocprim::plus<uint32_t>(), st);
+        if (e == hipSuccess) e = hipMalloc(&out, (size_t)hc.inner * sizeof(GpuNode));
+        if (e == hipSuccess) {
+            compact_kernel<<<blocks_for(hc.alloc, 256), 256, 0, st>>>(n2, used, cidx, hc.alloc, out);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            (void)hipFree(out);
+            (void)hipFree(prims);
+            res->prims = nullptr;
+            return e;
+        }
+        res->nodes = out;
         res->depth = hc.depth;
         return hipSuccess;
     }
@@ -974,7 +1170,7 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
     while (e == hipSuccess && n_items > 0) {
         set_u32_kernel<<<1, 64, 0, st>>>(&ctr->next_front, 0u);
         collapse_kernel<<<blocks_for(n_items, 256), 256, 0, st>>>(n2, items[ic], n_items, items[ic ^ 1], n4,
-                                                                 hc.inner, ctr);
+                                                                 hc.alloc, hc.inner, ctr);
         if ((e = hipGetLastError()) != hipSuccess) break;
         if ((e = hipMemcpyAsync(&c4, ctr, sizeof(Ctr), hipMemcpyDeviceToHost, st)) != hipSuccess) break;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) break;

@@ -105,6 +105,14 @@ class Scene:
         _lib.check(_declare().illume_scene_bvh(self._h, ctypes.byref(b)), "illume_scene_bvh")
         return b
 
+    def bvh_info(self):
+        """illume_bvh_info of the scene's aggregate (node counts, depth, build
+        time)."""
+        inf = _lib.BvhInfo()
+        _lib.check(_lib.lib().illume_bvh_info(self._bvh(), ctypes.byref(inf)), "illume_bvh_info")
+        return {"n_prims": inf.n_prims, "n_inner_nodes": inf.n_inner_nodes, "n_leaves": inf.n_leaves,
+                "max_depth": inf.max_depth, "device_bytes": inf.device_bytes, "build_seconds": inf.build_seconds}
+
     def intersect(self, rays, hits=None, stream=None):
         """Scene::intersect (scene.rs:43-47), batched: see BVHAccel.intersect."""
         n = rays.shape[0]

@@ -77,8 +77,12 @@ class SceneDesc:
         base = len(self.verts)
         self.verts = np.concatenate([self.verts, np.stack([a, b, c])]).astype(np.float32)
         self.idx = np.concatenate([self.idx, np.array([[base, base + 1, base + 2]], np.uint32)])
-        self.tri_material.append(mat)
-        self.tri_emission.append(emission)
+        if isinstance(self.tri_material, np.ndarray):  # per-triangle arrays of a generated mesh
+            self.tri_material = np.append(self.tri_material, np.uint32(mat))
+            self.tri_emission = np.concatenate([self.tri_emission, np.float32(emission).reshape(1, 3)])
+        else:
+            self.tri_material.append(mat)
+            self.tri_emission.append(emission)
 
     def add_quad(self, p0, p1, p2, p3, mat, emission=(0, 0, 0), normal=None):
         """Planar quad p0 p1 p2 p3 (in order around the boundary) as two triangles."""
@@ -91,7 +95,8 @@ class SceneDesc:
 
     def arrays(self):
         n_t, n_s = len(self.idx), len(self.spheres)
-        prim_material = np.array(list(self.tri_material) + list(self.sphere_material), np.uint32).reshape(n_t + n_s)
+        prim_material = np.concatenate([np.asarray(self.tri_material, np.uint32).reshape(n_t),
+                                        np.asarray(self.sphere_material, np.uint32).reshape(n_s)])
         prim_emission = np.zeros((n_t + n_s, 3), np.float32)
         if n_t:
             prim_emission[:n_t] = np.asarray(self.tri_emission, np.float32).reshape(n_t, 3)
@@ -215,7 +220,7 @@ def procedural_scene(seed):
     ground = s.add_material(matte((0.6, 0.6, 0.6)))
     s.add_quad((-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, 4), ground, normal=(0, 1, 0))
     verts, idx = [s.verts], [s.idx]
-    tri_mat = list(s.tri_material)
+    tri_mat = [np.asarray(s.tri_material, np.uint32)]
     n = 2
     base_v = len(s.verts)
     while n < target:
@@ -234,13 +239,13 @@ def procedural_scene(seed):
             m = s.add_material(matte(tuple(rng.uniform(0.1, 0.8, 3))))
         verts.append(vv)
         idx.append((f + base_v).astype(np.uint32))
-        tri_mat.extend([m] * len(f))
+        tri_mat.append(np.full(len(f), m, np.uint32))
         base_v += len(vv)
         n += len(f)
     s.verts = np.concatenate(verts).astype(np.float32)
     s.idx = np.concatenate(idx).astype(np.uint32)
-    s.tri_material = tri_mat
-    s.tri_emission = [(0, 0, 0)] * len(s.idx)
+    s.tri_material = np.concatenate(tri_mat)
+    s.tri_emission = np.zeros((len(s.idx), 3), np.float32)
     n_lights = int(rng.integers(1, 5))
     lm = s.add_material(matte((0, 0, 0)))
     for _ in range(n_lights):

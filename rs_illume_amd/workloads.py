@@ -53,7 +53,7 @@ def spheres_in_cube(n, seed=4, median_radius=0.02, log_sigma=0.5):
     return np.concatenate([c, r[:, None]], axis=1).astype(np.float32)
 
 
-def _profile_launches(width=2, launches=2, copy_mib=1024):
+def _profile_launches(width=2, launches=2, copy_mib=1024, builder="SAH_GPU"):
     """The profiling workload for hardware counters: the calibrations of
     FETCH_SIZE / WRITE_SIZE against known bytes (a plain device copy of
     copy_mib MiB; a torch gather of 2^22 random 64-byte rows; and
@@ -82,7 +82,7 @@ def _profile_launches(width=2, launches=2, copy_mib=1024):
     del gathered, probe_out
     v, idx = triangle_soup(1 << 20, seed=1)
     rays = sphere_to_cube_rays(1 << 24, seed=2)
-    bvh = BVHAccel(v, idx, None, 4, width=width)
+    bvh = BVHAccel(v, idx, None, 4, split_method=builder, width=width)
     rd = rays_to_device(rays)
     hits = torch.empty((rd.shape[0], 4), dtype=torch.int32, device="cuda")
     for _ in range(launches):
@@ -119,21 +119,35 @@ def _probe_ceiling(sizes_mib=(16, 100, 1024, 8192), lanes=1 << 22, steps=64):
         del buf
 
 
-def _profile_batched(launches=1):
+def _profile_batched(launches=1, builder="SAH_GPU"):
     """The profiling workload of the batched render: the 64 procedural scenes
-    (seeds 1-64, host SAH trees) in `launches` calls of illume_render_batch
-    at their own resolution and spp, max depth 8."""
+    (seeds 1-64, SAH trees from `builder`) in `launches` calls of
+    illume_render_batch at their own resolution and spp, max depth 8."""
     import torch
     from . import scenes as S
     from .render import Scene, render_batch
     descs = [S.procedural_scene(seed) for seed in range(1, 65)]
-    scenes = [Scene(d) for d, _, _ in descs]
+    scenes = [Scene(d, builder=builder) for d, _, _ in descs]
     jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
     film = None
     for _ in range(launches):
         film, _ = render_batch(jobs, film=film)
     torch.cuda.synchronize()
     print("%d batched render launches done" % launches, flush=True)
+
+
+def _profile_build(builder="SAH_GPU", width=4, repeats=3):
+    """The profiling workload of the device BVH builders: `repeats` builds of
+    the 2^20-triangle soup's tree."""
+    from .accel import BVHAccel
+    v, idx = triangle_soup(1 << 20, seed=1)
+    for _ in range(repeats):
+        b = BVHAccel(v, idx, None, 4, split_method=builder, width=width)
+        info = b.info()
+        b.close()
+        print("%s build (width %d): %.4f s, %d nodes, depth %d" % (builder, width, info["build_seconds"],
+                                                                  info["n_inner_nodes"], info["max_depth"]),
+              flush=True)
 
 
 if __name__ == "__main__":
@@ -143,10 +157,14 @@ if __name__ == "__main__":
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--probe", action="store_true", help="time the random-read probe over several buffer sizes")
     ap.add_argument("--batched", action="store_true", help="the 64-scene batched render instead")
+    ap.add_argument("--builder", default="SAH_GPU", choices=("SAH", "SAH_GPU", "LBVH"))
+    ap.add_argument("--build", action="store_true", help="BVH builds of the 2^20-triangle soup instead")
     a = ap.parse_args()
     if a.probe:
         _probe_ceiling()
+    elif a.build:
+        _profile_build(a.builder, a.width)
     elif a.batched:
-        _profile_batched(a.launches)
+        _profile_batched(a.launches, a.builder)
     else:
-        _profile_launches(a.width, a.launches)
+        _profile_launches(a.width, a.launches, builder=a.builder)
