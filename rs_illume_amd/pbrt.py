@@ -1075,7 +1075,7 @@ def main(argv=None):
                                            "other than those becomes .pfm)")
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--wavefront", action="store_true")
-    ap.add_argument("--builder", default="SAH", choices=("SAH", "LBVH"))
+    ap.add_argument("--builder", default="SAH", choices=("SAH", "SAH_GPU", "LBVH"))
     a = ap.parse_args(argv)
     ld = load_file(a.scene)
     for w in ld.warnings:
