@@ -26,6 +26,29 @@ constexpr int kWave = 64;
 constexpr int kEnc = 144, kOut = 40, kLobes = 8;
 constexpr uint32_t kInferGrid = 768;   // inference blocks at most (3 per CU: 47 KB of LDS each)
 
+// Copies BYTES (a multiple of 16) from global memory to LDS, THREADS threads,
+// each issuing its loads eight at a time before the matching stores.
+template <int THREADS, int BYTES>
+__device__ __forceinline__ void stage_to_lds(void *dst, const void *src) {
+    constexpr int N = BYTES / 16, PER = (N + THREADS - 1) / THREADS;
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int k0 = 0; k0 < PER; k0 += 8) {
+        uint4 r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = (int)threadIdx.x + THREADS * (k0 + k);
+            if (k0 + k < PER) r[k] = s[i < N ? i : N - 1];  // in bounds either way; stored only if i < N
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = (int)threadIdx.x + THREADS * (k0 + k);
+            if (k0 + k < PER && i < N) d[i] = r[k];
+        }
+    }
+}
+
 __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -57,6 +80,9 @@ struct alignas(16) InferWeights {
     float bias[4][64];
 };
 static_assert(sizeof(InferWeights) % 16 == 0, "16-byte staging");
+// the image's forward weights are its first bytes, in InferWeights' order
+constexpr int kInferWBytes = (int)offsetof(InferWeights, bias);
+static_assert(offsetof(GuideWeightImage, b1) == kInferWBytes, "forward weights first in the image");
 
 __device__ __forceinline__ bf16x8 lds_wa(const __bf16 *w, int ld, int m, int j, int lane) {
     return *reinterpret_cast<const bf16x8 *>(w + (16 * m + (lane & 15)) * ld + 32 * j + (lane >> 4) * 8);
@@ -124,20 +150,8 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__res
     const uint32_t n = n_ptr ? min(*n_ptr, n_arg) : n_arg;
     const uint32_t n_iter = (n + 31) / 32;
     if (blockIdx.x * kInfWaves >= n_iter) return;  // the whole block has no rows
-    {
-        const __bf16 *src[4] = {g.wa[0], g.wa[1], g.wa[2], g.wa[3]};
-        __bf16 *dst[4] = {W.w0, W.w1, W.w2, W.w3};
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int n16 = kGuideWaRows[l] * kGuideWaLd[l] / 8;
-            for (int i = threadIdx.x; i < n16; i += kInfWaves * kWave)
-                reinterpret_cast<uint4 *>(dst[l])[i] = reinterpret_cast<const uint4 *>(src[l])[i];
-        }
-        for (int i = threadIdx.x; i < 4 * 64; i += kInfWaves * kWave) {
-            const int l = i / 64, o = i % 64;
-            W.bias[l][o] = o < kGuideWaRows[l] ? g.bias[l][o] : 0.f;
-        }
-    }
+    stage_to_lds<kInfWaves * kWave, kInferWBytes>(W.w0, g.img->w0);
+    stage_to_lds<kInfWaves * kWave, (int)sizeof(W.bias)>(W.bias, g.img->bias);
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, grp = lane >> 4;
     for (uint32_t it = blockIdx.x * kInfWaves + wave; it < n_iter; it += gridDim.x * kInfWaves) {
@@ -242,6 +256,64 @@ __device__ void nll_grad_row(const float *y, const float *w, float scale, float 
     }
 }
 
+// The same gradient with a row split over two lanes of a wave: lane r and lane
+// r + 32 take lobes 4 h .. 4 h + 3 (h = lane / 32) of the row whose outputs
+// are y, combine the mixture's maximum, normaliser and density with one
+// exchange each, and overwrite their lobes' outputs in y with the gradient
+// (each lane reads and writes only its own lobes). Returns the row's loss
+// (both lanes). Every lane of the wave calls it.
+__device__ __forceinline__ float nll_grad_half(float *y, const float *w, float scale, int lane) {
+    const int h = lane >> 5;
+    float *yh = y + 20 * h;
+    float mx = yh[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) mx = fmaxf(mx, yh[5 * j]);
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    float pi[4], s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        pi[j] = hexp(yh[5 * j] - mx);
+        s += pi[j];
+    }
+    s += __shfl_xor(s, 32);
+    const float inv_s = hrcp(s);
+    float v[4], mu[4][3], kap[4], imn[4], q = 0.f;
+    bool clamped[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        pi[j] *= inv_s;
+        const float a = yh[5 * j + 1], b = yh[5 * j + 2], c = yh[5 * j + 3];
+        const float len2 = a * a + b * b + c * c;
+        const float inv = len2 > 0.f ? __builtin_amdgcn_rsqf(len2) : 0.f;
+        imn[j] = inv;
+        mu[j][0] = a * inv;
+        mu[j][1] = b * inv;
+        mu[j][2] = len2 > 0.f ? c * inv : 1.f;
+        const float r = yh[5 * j + 4];
+        clamped[j] = r < -4.6f || r > 6.9f;
+        kap[j] = hexp(fminf(fmaxf(r, -4.6f), 6.9f));
+        const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        v[j] = kap[j] * hrcp(6.28318530717958647692f * hom2k(kap[j])) * hexp(kap[j] * (cdot - 1.f));
+        q += pi[j] * v[j];
+    }
+    q += __shfl_xor(q, 32);
+    const float iq = hrcp(q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float gam = pi[j] * v[j] * iq;
+        const float k = kap[j];
+        const float cdot = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        const float dlk = hrcp(k) - 2.f * hexp(-2.f * k) * hrcp(hom2k(k)) + (cdot - 1.f);
+        const bool live = scale > 0.f;  // a row of weight 0 contributes exactly nothing
+        yh[5 * j] = live ? scale * (pi[j] - gam) : 0.f;
+        yh[5 * j + 4] = (clamped[j] || !live) ? 0.f : scale * (-gam * dlk * k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            yh[5 * j + 1 + a] = live ? scale * (-gam * k * (w[a] - cdot * mu[j][a]) * imn[j]) : 0.f;
+    }
+    return -scale * __builtin_amdgcn_logf(q) * 0.69314718055994530942f;
+}
+
 // ------------------------------------------------------------------ training
 // One fused pass per Adam step (guide_train_kernel), then one reduction that
 // also applies Adam (guide_reduce_adam_kernel).
@@ -262,23 +334,13 @@ __device__ void nll_grad_row(const float *y, const float *w, float scale, float 
 // weight sum and its loss; the reduction adds the rows, normalises by the
 // total weight and applies Adam to the fp32 master weights and their bf16
 // copies.
-constexpr int kTrainWaves = 4;
+constexpr int kTrainWaves = 8;  // two per SIMD
 constexpr int kTrainThreads = kTrainWaves * kWave;
 
-struct alignas(16) TrainWeights {
-    __bf16 w0[64 * 168];
-    __bf16 w1[64 * 72];
-    __bf16 w2[64 * 72];
-    __bf16 w3[48 * 72];
-    __bf16 b1[64 * 72];
-    __bf16 b2[64 * 72];
-    __bf16 b3[64 * 72];
-    float bias[4][64];
-};
 // Staged tensors are [unit][row], kStageLd rows per unit.
 constexpr int kStageLd = 32;
 struct alignas(16) TrainWaveLds {
-    __bf16 x[3][64 * kStageLd];  // X_1..X_3 of the wave's tile
+    __bf16 x[64 * kStageLd];     // the wave's X_l while dW_l is formed (X_3, then X_2, then X_1)
     union {
         float y[32][48];           // outputs of the tile, then their gradients
         __bf16 g[64 * kStageLd];   // the current G_l
@@ -286,7 +348,7 @@ struct alignas(16) TrainWaveLds {
     float in[32][6];               // the rows' network inputs (zero past n)
 };
 struct TrainLds {
-    TrainWeights w;
+    GuideWeightImage w;
     TrainWaveLds wave[kTrainWaves];
 };
 static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS within one CU's 160 KB");
@@ -337,21 +399,30 @@ __device__ __forceinline__ void stage_units(const bf16x8 (*v)[2], __bf16 *dst, i
             for (int e = 0; e < 4; ++e) dst[(16 * m + 4 * g + e) * kStageLd + 16 * nn + r] = v[nn][m / 2][4 * (m % 2) + e];
 }
 
+// The lane's element offset in a staged fragment, (r16 row, 8 grp column):
+// an opaque value per call, so that the fragments' addresses are formed where
+// they are read instead of being kept live (and spilled) across the round.
+__device__ __forceinline__ int frag_off(int lane) {
+    int o = (lane & 15) * kStageLd + 8 * (lane >> 4);
+    __asm__ volatile("" : "+v"(o));
+    return o;
+}
+
 // dW tile (16 inputs it x 16 outputs u) of layer l >= 1 += X_l^T G_l over the
 // 32 rows of the round's tiles; colsum += the rows' G_l of output column u
-// (the bias gradient) when wanted.
-__device__ __forceinline__ floatx4 dw_tile(const TrainLds &L, int l, int it, int u, floatx4 acc, float *colsum,
-                                           int lane) {
-    const int grp = lane >> 4, r16 = lane & 15;
+// (the bias gradient) when want_bias.
+__device__ __forceinline__ floatx4 dw_tile(const TrainLds &L, int it, int u, floatx4 acc, float &colsum,
+                                           bool want_bias, int lane) {
+    const int fo = frag_off(lane);
 #pragma unroll
     for (int ti = 0; ti < kTrainWaves; ++ti) {
         const TrainWaveLds &T = L.wave[ti];
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(T.x[l - 1] + (16 * it + r16) * kStageLd + 8 * grp);
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(T.yg.g + (16 * u + r16) * kStageLd + 8 * grp);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(T.x + 16 * it * kStageLd + fo);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(T.yg.g + 16 * u * kStageLd + fo);
         acc = mfma(a, b, acc);
-        if (colsum)
+        if (want_bias)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) *colsum += (float)b[e];
+            for (int e = 0; e < 8; ++e) colsum += (float)b[e];
     }
     return acc;
 }
@@ -392,9 +463,12 @@ __device__ __forceinline__ void put_dw(const GuideDev &g, float *prow, int l, in
 // Block b trains on tiles [b * per_block, (b + 1) * per_block) of the n
 // records (record r at rec + stride_f * r floats); partial row b gets its
 // gradient (rows weighted by w, not yet normalised), wl[2 b], wl[2 b + 1] its
-// weight sum and loss. Wave w owns the dW tiles of input tile it = w of
-// layers 1..3 and it = w, w + 4, w + 8 (< 9) of layer 0, every output tile;
-// wave 0 also sums the bias gradients.
+// weight sum and loss. dW tiles of wave w (11, 44 accumulator registers, so
+// that two waves share a SIMD): layers 1..3, input tile w % 4 and output tiles
+// w / 4 and w / 4 + 2 (< 3 for layer 3); layer 0, input tile w with every
+// output tile, and input tile 8 (features 128..143; tile 9 is padding) with
+// output tile w % 4 over the rows of waves 4 (w / 4) .. 4 (w / 4) + 3, the two
+// halves added at the end. The owners of input tile 0 sum the bias gradients.
 __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev g, const float *__restrict__ rec,
                                                                       uint32_t n, uint32_t stride_f,
                                                                       uint32_t per_block, float *__restrict__ partial,
@@ -402,44 +476,40 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
     __shared__ TrainLds L;
     __shared__ float red[2][kTrainWaves];
     const uint32_t t_begin = blockIdx.x * per_block, t_end = t_begin + per_block;
-    {
-        const __bf16 *src[7] = {g.wa[0], g.wa[1], g.wa[2], g.wa[3], g.wb[1], g.wb[2], g.wb[3]};
-        __bf16 *dst[7] = {L.w.w0, L.w.w1, L.w.w2, L.w.w3, L.w.b1, L.w.b2, L.w.b3};
-        const int n16[7] = {64 * 168 / 8, 64 * 72 / 8, 64 * 72 / 8, 48 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8, 64 * 72 / 8};
-#pragma unroll
-        for (int l = 0; l < 7; ++l)
-            for (int i = threadIdx.x; i < n16[l]; i += kTrainThreads)
-                reinterpret_cast<uint4 *>(dst[l])[i] = reinterpret_cast<const uint4 *>(src[l])[i];
-        for (int i = threadIdx.x; i < 4 * 64; i += kTrainThreads) {
-            const int l = i / 64, o = i % 64;
-            L.w.bias[l][o] = o < kGuideWaRows[l] ? g.bias[l][o] : 0.f;
-        }
-    }
+    stage_to_lds<kTrainThreads, (int)sizeof(GuideWeightImage)>(&L.w, g.img);
     __syncthreads();
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, grp = lane >> 4, r16 = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);  // uniform: scalar registers
     TrainWaveLds &S = L.wave[wave];
-    const bool bias_owner = wave == 0;
-    const int n_it0 = wave == 0 ? 3 : 2;  // layer-0 input tiles w, w + 4 (, w + 8)
-    floatx4 d3[3], d2[4], d1[4], d0[3][4];
-    float c3[3], c2[4], c1[4], c0[4];
+    const int itw = wave % 4, uo = wave / 4;  // layers 1..3: input tile, first output tile
+    const bool bias_hi = itw == 0;            // sums the bias gradients of its output tiles (layers 1..3)
+    const bool bias_0 = wave == 0;            // ... and of layer 0
+    floatx4 d3[2], d2[2], d1[2], d0[4], d8;
+    float c3[2], c2[2], c1[2], c0[4];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        d3[j] = d2[j] = d1[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        c3[j] = c2[j] = c1[j] = 0.f;
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        if (u < 3) {
-            d3[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-            c3[u] = 0.f;
-        }
-        d2[u] = d1[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-        c2[u] = c1[u] = c0[u] = 0.f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) d0[k][u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        d0[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        c0[u] = 0.f;
     }
+    d8 = floatx4{0.f, 0.f, 0.f, 0.f};
     float wave_loss = 0.f, wave_wsum = 0.f;
     for (uint32_t t0 = t_begin; t0 < t_end; t0 += kTrainWaves) {
+        // lane-derived values (the encoding's feature choices, fragment
+        // addresses) are formed inside the round: hoisted out of it they
+        // would be kept live through the whole round and spilled
+        int lane = (int)threadIdx.x % kWave;
+        __asm__ volatile("" : "+v"(lane));
+        const int grp = lane >> 4, r16 = lane & 15;
         const uint32_t base = (t0 + wave) * 32;
-        // the rows' sampled directions and weights, needed after the forward pass
+        // the rows' sampled directions and weights, needed after the forward
+        // pass (lanes r and r + 32: row r)
         float w_row[4] = {0.f, 0.f, 0.f, 0.f};
-        if (lane < 32 && base + lane < n) {
-            const float *r = rec + (size_t)(base + lane) * stride_f;
+        if (base + (lane & 31) < n) {
+            const float *r = rec + (size_t)(base + (lane & 31)) * stride_f;
             w_row[0] = r[6];
             w_row[1] = r[7];
             w_row[2] = r[8];
@@ -452,26 +522,25 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             for (int i = 0; i < 6; ++i) S.in[lane][i] = row < n ? rec[(size_t)row * stride_f + i] : 0.f;
         }
         uint32_t mask1, mask2, mask3;
+        bf16x8 h1[2][2], h2[2][2];  // X_1, X_2 until their weight gradients
         {
             bf16x8 x0[2][5], h[2][2];
             floatx4 acc[2][4];
             encode_b(rec, stride_f, base + r16, n, grp, x0[0]);
             encode_b(rec, stride_f, base + 16 + r16, n, grp, x0[1]);
             layer_t<5, 4>(L.w.w0, 168, x0, acc, lane);
-            relu_to_b<4>(acc[0], L.w.bias[0], grp, h[0]);
-            relu_to_b<4>(acc[1], L.w.bias[0], grp, h[1]);
-            mask1 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            stage_units(h, S.x[0], lane);
-            layer_t<2, 4>(L.w.w1, 72, h, acc, lane);
-            relu_to_b<4>(acc[0], L.w.bias[1], grp, h[0]);
-            relu_to_b<4>(acc[1], L.w.bias[1], grp, h[1]);
-            mask2 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            stage_units(h, S.x[1], lane);
-            layer_t<2, 4>(L.w.w2, 72, h, acc, lane);
+            relu_to_b<4>(acc[0], L.w.bias[0], grp, h1[0]);
+            relu_to_b<4>(acc[1], L.w.bias[0], grp, h1[1]);
+            mask1 = relu_bits(h1[0], 0) | relu_bits(h1[1], 1);
+            layer_t<2, 4>(L.w.w1, 72, h1, acc, lane);
+            relu_to_b<4>(acc[0], L.w.bias[1], grp, h2[0]);
+            relu_to_b<4>(acc[1], L.w.bias[1], grp, h2[1]);
+            mask2 = relu_bits(h2[0], 0) | relu_bits(h2[1], 1);
+            layer_t<2, 4>(L.w.w2, 72, h2, acc, lane);
             relu_to_b<4>(acc[0], L.w.bias[2], grp, h[0]);
             relu_to_b<4>(acc[1], L.w.bias[2], grp, h[1]);
             mask3 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
-            stage_units(h, S.x[2], lane);
+            stage_units(h, S.x, lane);  // X_3
             layer_t<2, 3>(L.w.w3, 72, h, acc, lane);
 #pragma unroll
             for (int nn = 0; nn < 2; ++nn)
@@ -484,17 +553,19 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
                 }
         }
         wave_lds_sync();
-        // ---- output gradient: lane r < 32 owns row base + r
-        if (lane < 32) {
-            float gy[kOut], row_loss = 0.f;
-            for (int o = 0; o < kOut; ++o) gy[o] = 0.f;
+        // ---- output gradient: lanes r and r + 32 share row base + r (a row of
+        // weight 0, or past n, gets a zero gradient)
+        {
+            float *yr = S.yg.y[lane & 31];
             const float wgt = w_row[3];
-            if (wgt > 0.f) {
-                nll_grad_row(S.yg.y[lane], w_row, wgt, gy, &row_loss);
+            const float row_loss = nll_grad_half(yr, w_row, wgt > 0.f ? wgt : 0.f, lane);
+            if (lane < 32 && wgt > 0.f) {
                 wave_wsum += wgt;
+                wave_loss += row_loss;
             }
-            wave_loss += row_loss;
-            for (int o = 0; o < 48; ++o) S.yg.y[lane][o] = o < kOut ? gy[o] : 0.f;
+            if (lane >= 32)
+#pragma unroll
+                for (int o = kOut; o < 48; ++o) yr[o] = 0.f;
         }
         wave_lds_sync();
         // G_3 as the first backward product's B operand (natural output order,
@@ -523,27 +594,29 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             *reinterpret_cast<bf16x8 *>(S.yg.g + unit * kStageLd + part * 8) = gt[q];
         }
         __syncthreads();
-        // ---- dW_3 (input tile w, output tiles 0..2)
-#pragma unroll
-        for (int u = 0; u < 3; ++u) d3[u] = dw_tile(L, 3, wave, u, d3[u], bias_owner ? &c3[u] : nullptr, lane);
+        // ---- dW_3
+        d3[0] = dw_tile(L, itw, uo, d3[0], c3[0], bias_hi, lane);
+        if (uo == 0) d3[1] = dw_tile(L, itw, 2, d3[1], c3[1], bias_hi, lane);
         __syncthreads();
         // ---- G_2 = (W_3 G_3^T) * relu'(h3), dW_2
         layer_t<2, 4>(L.w.b3, 72, g3, acc, lane);
         mask_to_b(acc[0], mask3, 0, gz[0]);
         mask_to_b(acc[1], mask3, 1, gz[1]);
         stage_units(gz, S.yg.g, lane);
+        stage_units(h2, S.x, lane);
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; ++u) d2[u] = dw_tile(L, 2, wave, u, d2[u], bias_owner ? &c2[u] : nullptr, lane);
+        for (int j = 0; j < 2; ++j) d2[j] = dw_tile(L, itw, uo + 2 * j, d2[j], c2[j], bias_hi, lane);
         __syncthreads();
         // ---- G_1, dW_1
         layer_t<2, 4>(L.w.b2, 72, gz, acc, lane);
         mask_to_b(acc[0], mask2, 0, gz[0]);
         mask_to_b(acc[1], mask2, 1, gz[1]);
         stage_units(gz, S.yg.g, lane);
+        stage_units(h1, S.x, lane);
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; ++u) d1[u] = dw_tile(L, 1, wave, u, d1[u], bias_owner ? &c1[u] : nullptr, lane);
+        for (int j = 0; j < 2; ++j) d1[j] = dw_tile(L, itw, uo + 2 * j, d1[j], c1[j], bias_hi, lane);
         __syncthreads();
         // ---- G_0, dW_0 (A = the encoding of the round's rows)
         layer_t<2, 4>(L.w.b1, 72, gz, acc, lane);
@@ -551,37 +624,48 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         mask_to_b(acc[1], mask1, 1, gz[1]);
         stage_units(gz, S.yg.g, lane);
         __syncthreads();
+        const int fo = frag_off(lane);
 #pragma unroll
         for (int ti = 0; ti < kTrainWaves; ++ti) {
             const TrainWaveLds &T = L.wave[ti];
             bf16x8 b[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                b[u] = *reinterpret_cast<const bf16x8 *>(T.yg.g + (16 * u + r16) * kStageLd + 8 * grp);
-                if (bias_owner)
+                b[u] = *reinterpret_cast<const bf16x8 *>(T.yg.g + 16 * u * kStageLd + fo);
+                if (bias_0)
 #pragma unroll
                     for (int e = 0; e < 8; ++e) c0[u] += (float)b[u][e];
             }
+            const bf16x8 a = enc_a(T.in, wave, lane);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k >= n_it0) break;
-                const bf16x8 a = enc_a(T.in, wave + 4 * k, lane);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) d0[k][u] = mfma(a, b[u], d0[k][u]);
+            for (int u = 0; u < 4; ++u) d0[u] = mfma(a, b[u], d0[u]);
+            if (ti / 4 == uo) {
+                const bf16x8 a8 = enc_a(T.in, 8, lane);
+                const int u8 = itw;
+                d8 = mfma(a8, u8 == 0 ? b[0] : u8 == 1 ? b[1] : u8 == 2 ? b[2] : b[3], d8);
             }
         }
         __syncthreads();
     }
     // ---- the block's partial gradient row, weight sum and loss
+    const int lane = (int)threadIdx.x % kWave;
     float *prow = partial + (size_t)blockIdx.x * kGuideParams;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        if (u < 3) put_dw(g, prow, 3, wave, u, d3[u], c3[u], bias_owner, lane);
-        put_dw(g, prow, 2, wave, u, d2[u], c2[u], bias_owner, lane);
-        put_dw(g, prow, 1, wave, u, d1[u], c1[u], bias_owner, lane);
+    for (int j = 0; j < 2; ++j) {
+        const int u = uo + 2 * j;
+        if (u < 3) put_dw(g, prow, 3, itw, u, d3[j], c3[j], bias_hi, lane);
+        put_dw(g, prow, 2, itw, u, d2[j], c2[j], bias_hi, lane);
+        put_dw(g, prow, 1, itw, u, d1[j], c1[j], bias_hi, lane);
+    }
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (k < n_it0) put_dw(g, prow, 0, wave + 4 * k, u, d0[k][u], c0[u], bias_owner && k == 0, lane);
+    for (int u = 0; u < 4; ++u) put_dw(g, prow, 0, wave, u, d0[u], c0[u], bias_0, lane);
+    // input tile 8 of layer 0: waves 4..7 hand their halves to waves 0..3
+    floatx4 *half8 = reinterpret_cast<floatx4 *>(S.x);
+    if (uo == 1) half8[lane] = d8;
+    __syncthreads();
+    if (uo == 0) {
+        const floatx4 o = reinterpret_cast<const floatx4 *>(L.wave[wave + 4].x)[lane];
+        put_dw(g, prow, 0, 8, itw, d8 + o, 0.f, false, lane);
     }
     float l = wave_loss, w = wave_wsum;
     for (int off = 32; off > 0; off >>= 1) {
@@ -605,20 +689,24 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
 }
 
 // Parameter i's gradient: the sum of the blocks' partial rows over the total
-// weight (64 consecutive parameters per block, 4 threads each summing every
-// fourth row, so that a wave reads 256 contiguous bytes of a row, then an LDS
-// reduction). With adam, one Adam step (Kingma & Ba 2015) on the fp32 master
-// weight and its bf16 copies; grad (optional) keeps the gradient. Block 0
-// writes the total weight to *wsum and the normalised loss to *loss (each
-// optional).
+// weight. A block takes 64 consecutive parameters as 16 float4 columns; its
+// thread (column c, row group r) loads rows r, r + 16, ... of column c (all
+// loads in flight at once, 256 contiguous bytes of a row per 16 threads), and
+// an LDS reduction adds the row groups. With adam, one Adam step (Kingma &
+// Ba 2015) on the fp32 master weight and its bf16 copies; grad (optional)
+// keeps the gradient. Block 0 writes the total weight to *wsum and the
+// normalised loss to *loss (each optional). rows <= kTrainBlocksMax.
+constexpr int kRedGroups = 16;
+static_assert(kTrainBlocksMax <= 16 * kRedGroups, "16 rows per thread cover every partial row");
+static_assert(kGuideParams % 4 == 0, "float4 columns");
 __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, const float *__restrict__ partial,
                                                                 const float *__restrict__ wl, uint32_t rows,
                                                                 float *__restrict__ grad, float *__restrict__ wsum,
                                                                 float *__restrict__ loss, int adam, float lr, float c1,
                                                                 float c2) {
-    __shared__ float part[4][64];
+    __shared__ float4 part[kRedGroups][16];
     __shared__ float tot[2];
-    const int p = threadIdx.x % 64, q = threadIdx.x / 64;
+    constexpr uint32_t kCols = kGuideParams / 4;
     if (threadIdx.x < 64) {
         // total weight (lanes 0..31) and loss (lanes 32..63) over the rows
         const int which = threadIdx.x / 32;
@@ -627,19 +715,26 @@ __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, cons
         for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
         if (threadIdx.x % 32 == 0) tot[which] = v;
     }
-    const uint32_t i = blockIdx.x * 64 + p;
-    float s = 0.f;
-    if (i < (uint32_t)kGuideParams) {
-        float s1 = 0.f;
-        uint32_t r = q;
-        for (; r + 4 < rows; r += 8) {
-            s += partial[(size_t)r * kGuideParams + i];
-            s1 += partial[(size_t)(r + 4) * kGuideParams + i];
+    const int c = threadIdx.x % 16, rg = threadIdx.x / 16;
+    const uint32_t col = blockIdx.x * 16 + c;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (col < kCols) {
+        const float4 *src = reinterpret_cast<const float4 *>(partial) + col;
+        float4 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t r = rg + kRedGroups * k;
+            v[k] = r < rows ? src[(size_t)r * kCols] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if (r < rows) s += partial[(size_t)r * kGuideParams + i];
-        s += s1;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            s.x += v[k].x;
+            s.y += v[k].y;
+            s.z += v[k].z;
+            s.w += v[k].w;
+        }
     }
-    part[q][p] = s;
+    part[rg][c] = s;
     __syncthreads();
     const float ws = tot[0];
     const float inv = ws > 0.f ? 1.f / ws : 0.f;
@@ -647,8 +742,16 @@ __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, cons
         if (wsum) *wsum = ws;
         if (loss) *loss = tot[1] * inv;
     }
-    if (q == 0 && i < (uint32_t)kGuideParams) {
-        const float gi = (part[0][p] + part[1][p] + part[2][p] + part[3][p]) * inv;
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (threadIdx.x < 64 && i < (uint32_t)kGuideParams) {
+        const int cc = threadIdx.x / 4, e = threadIdx.x % 4;
+        float sum = 0.f;
+#pragma unroll
+        for (int r = 0; r < kRedGroups; ++r) {
+            const float4 q = part[r][cc];
+            sum += e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
+        }
+        const float gi = sum * inv;
         if (grad) grad[i] = gi;
         if (adam) {
             const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -676,19 +779,23 @@ __global__ void guide_sync_kernel(GuideDev g) {
 }  // namespace
 
 __device__ void write_bf16_copy(const GuideDev &g, uint32_t i, float p) {
+    GuideWeightImage &w = *g.img;
+#pragma unroll
     for (int l = 0; l < 4; ++l) {
         const uint32_t wo = g.w_off[l], bo = g.b_off[l];
-        const int din = g.din[l], dout = g.dout[l];
+        const int dout = g.dout[l];
         if (i >= wo && i < bo) {
             const uint32_t k = (i - wo) / dout, o = (i - wo) % dout;
-            g.wt[l][o * g.wt_ld[l] + k] = (__bf16)p;   // n-major [out][in_pad]
-            g.wk[l][k * g.wk_ld[l] + o] = (__bf16)p;   // k-major [in][out_pad]
-            g.wa[l][o * kGuideWaLd[l] + guide_wa_pos(l, (int)k)] = (__bf16)p;
-            if (l > 0) g.wb[l][k * 72 + guide_wb_pos(l, (int)o)] = (__bf16)p;
+            __bf16 *wa = l == 0 ? w.w0 : l == 1 ? w.w1 : l == 2 ? w.w2 : w.w3;
+            wa[o * kGuideWaLd[l] + guide_wa_pos(l, (int)k)] = (__bf16)p;
+            if (l > 0) {
+                __bf16 *wb = l == 1 ? w.b1 : l == 2 ? w.b2 : w.b3;
+                wb[k * 72 + guide_wb_pos(l, (int)o)] = (__bf16)p;
+            }
             return;
         }
         if (i >= bo && i < bo + (uint32_t)dout) {
-            g.bias[l][i - bo] = p;
+            w.bias[l][i - bo] = p;
             return;
         }
     }

@@ -31,7 +31,6 @@ const GuideDev *guide_dev(const illume_guide *g) { return &g->dev; }
 
 namespace {
 const int kDin[4] = {144, 64, 64, 64}, kDout[4] = {64, 64, 64, 40};
-const int kInPad[4] = {160, 64, 64, 64}, kOutPadN[4] = {64, 64, 64, 48}, kOutPadK[4] = {64, 64, 64, 64};
 
 template <typename T>
 hipError_t dalloc(illume_guide *g, T **p, size_t count) {
@@ -68,18 +67,12 @@ int illume_guide_create(const float *init_params, float lr, illume_guide **out) 
     for (int l = 0; l < 4 && e == hipSuccess; ++l) {
         d.din[l] = kDin[l];
         d.dout[l] = kDout[l];
-        d.wt_ld[l] = kInPad[l];
-        d.wk_ld[l] = kOutPadK[l];
         d.w_off[l] = off;
         off += kDin[l] * kDout[l];
         d.b_off[l] = off;
         off += kDout[l];
-        e = dalloc(g, &d.wt[l], (size_t)kOutPadN[l] * kInPad[l]);
-        if (e == hipSuccess) e = dalloc(g, &d.wk[l], (size_t)kDin[l] * kOutPadK[l]);
-        if (e == hipSuccess) e = dalloc(g, &d.wa[l], (size_t)kGuideWaRows[l] * kGuideWaLd[l]);
-        if (e == hipSuccess && l > 0) e = dalloc(g, &d.wb[l], (size_t)64 * 72);
-        if (e == hipSuccess) e = dalloc(g, &d.bias[l], (size_t)kOutPadN[l]);
     }
+    if (e == hipSuccess) e = dalloc(g, &d.img, 1);
     if (e == hipSuccess) e = dalloc(g, &d.params, kGuideParams);
     if (e == hipSuccess) e = dalloc(g, &d.m, kGuideParams);
     if (e == hipSuccess) e = dalloc(g, &d.v, kGuideParams);

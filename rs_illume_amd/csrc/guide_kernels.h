@@ -8,16 +8,29 @@ namespace illume {
 
 constexpr int kGuideParams = 20200;
 
+// The bf16 weights and fp32 biases in the layout the kernels stage in LDS, so
+// that a block stages them with one contiguous copy: forward A operands
+// w0..w3 ([out_pad][kGuideWaLd[l]], inputs in register order, guide_wa_pos),
+// backward A operands b1..b3 (W_l as [in][72], outputs in register order,
+// guide_wb_pos), biases (64 per layer, zero padded).
+struct alignas(16) GuideWeightImage {
+    __bf16 w0[64 * 168];
+    __bf16 w1[64 * 72];
+    __bf16 w2[64 * 72];
+    __bf16 w3[48 * 72];
+    __bf16 b1[64 * 72];
+    __bf16 b2[64 * 72];
+    __bf16 b3[64 * 72];
+    float bias[4][64];
+};
+static_assert(sizeof(GuideWeightImage) % 16 == 0, "16-byte staging");
+
 struct GuideDev {
     float *params;  // fp32 master weights, flat [W1 b1 W2 b2 W3 b3 W4 b4], W_l as [in][out]
     float *m, *v;   // Adam moments
-    __bf16 *wt[4];  // n-major copies [out_pad][in_pad] (forward B operands)
-    __bf16 *wk[4];  // k-major copies [in][out_pad] (backward B operands)
-    __bf16 *wa[4];  // inference A operands [out_pad][kGuideWaLd[l]], inputs in register order (guide_wa_pos)
-    __bf16 *wb[4];  // backward A operands, l = 1..3: W_l [in][72], outputs in register order (guide_wb_pos)
-    float *bias[4]; // biases, fp32, out_pad entries (zero padded)
+    GuideWeightImage *img;  // bf16 copies and biases
     uint32_t w_off[4], b_off[4];
-    int din[4], dout[4], wt_ld[4], wk_ld[4];
+    int din[4], dout[4];
 };
 
 // Inference copies: row strides (input slots padded by 8 so that the 16 rows
