@@ -593,35 +593,38 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             const int c = lane + 64 * q, unit = c / 4, part = c % 4;
             *reinterpret_cast<bf16x8 *>(S.yg.g + unit * kStageLd + part * 8) = gt[q];
         }
-        __syncthreads();
-        // ---- dW_3
-        d3[0] = dw_tile(L, itw, uo, d3[0], c3[0], bias_hi, lane);
-        if (uo == 0) d3[1] = dw_tile(L, itw, 2, d3[1], c3[1], bias_hi, lane);
-        __syncthreads();
-        // ---- G_2 = (W_3 G_3^T) * relu'(h3), dW_2
+        // Each G_{l-1} = (W_l G_l^T) * relu'(h_l) needs only this wave's G_l,
+        // so it is formed before the barrier that opens dW_l and its products
+        // run while the other waves arrive.
+        // ---- G_2 (registers), dW_3
         layer_t<2, 4>(L.w.b3, 72, g3, acc, lane);
         mask_to_b(acc[0], mask3, 0, gz[0]);
         mask_to_b(acc[1], mask3, 1, gz[1]);
+        __syncthreads();
+        d3[0] = dw_tile(L, itw, uo, d3[0], c3[0], bias_hi, lane);
+        if (uo == 0) d3[1] = dw_tile(L, itw, 2, d3[1], c3[1], bias_hi, lane);
+        __syncthreads();
+        // ---- G_2 and X_2 staged, G_1 (registers), dW_2
         stage_units(gz, S.yg.g, lane);
         stage_units(h2, S.x, lane);
+        layer_t<2, 4>(L.w.b2, 72, gz, acc, lane);
+        mask_to_b(acc[0], mask2, 0, gz[0]);
+        mask_to_b(acc[1], mask2, 1, gz[1]);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 2; ++j) d2[j] = dw_tile(L, itw, uo + 2 * j, d2[j], c2[j], bias_hi, lane);
         __syncthreads();
-        // ---- G_1, dW_1
-        layer_t<2, 4>(L.w.b2, 72, gz, acc, lane);
-        mask_to_b(acc[0], mask2, 0, gz[0]);
-        mask_to_b(acc[1], mask2, 1, gz[1]);
+        // ---- G_1 and X_1 staged, G_0 (registers), dW_1
         stage_units(gz, S.yg.g, lane);
         stage_units(h1, S.x, lane);
+        layer_t<2, 4>(L.w.b1, 72, gz, acc, lane);
+        mask_to_b(acc[0], mask1, 0, gz[0]);
+        mask_to_b(acc[1], mask1, 1, gz[1]);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 2; ++j) d1[j] = dw_tile(L, itw, uo + 2 * j, d1[j], c1[j], bias_hi, lane);
         __syncthreads();
-        // ---- G_0, dW_0 (A = the encoding of the round's rows)
-        layer_t<2, 4>(L.w.b1, 72, gz, acc, lane);
-        mask_to_b(acc[0], mask1, 0, gz[0]);
-        mask_to_b(acc[1], mask1, 1, gz[1]);
+        // ---- G_0 staged, dW_0 (A = the encoding of the round's rows)
         stage_units(gz, S.yg.g, lane);
         __syncthreads();
         const int fo = frag_off(lane);
