@@ -57,6 +57,7 @@ struct WfPaths {
 struct WfState {
     WfPaths s[2];
     float4 *hit0, *hit1;     // closest hit of slot p: prim, slot, t, b0 / b1, b2, proposal row (bits)
+    float4 *gstage;          // [2 p], [2 p + 1]: slot p's proposal inputs until its row is reserved
     // paths that ended this iteration, from each tile's first slot
     float4 *fin_L;           // radiance; w (bits): job index
     uint4 *fin_meta;         // as WfPaths::meta

@@ -20,8 +20,8 @@
 // its live paths at the tile's first slots (survivors compacted with a
 // block-wide scan into the other state buffer), and writes its requests and
 // finished paths from the tile's first slot. Only starting new paths, counting
-// finished ones and reserving proposal rows touch batch-wide counters, once
-// per block. Every kernel launches one block per tile and reads its work
+// finished ones and reserving proposal rows and training samples touch
+// batch-wide counters, once per block (and network). Every kernel launches one block per tile and reads its work
 // counts from device memory, so the host can capture iterations in a
 // hipGraph (parity, which buffer holds the live paths, alternates with the
 // iteration and is a launch argument).
@@ -101,29 +101,6 @@ __device__ __forceinline__ uint32_t find_job(const WfState &w, unsigned long lon
     return lo;
 }
 
-// A slot for every lane of the wave (among the active lanes) whose `pred`
-// holds, with one atomicAdd per wave: the first such lane reserves them all,
-// each takes base + its rank. Every active lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_slot(uint32_t *ctr, bool pred) {
-    const unsigned long long m = __ballot(pred);
-    const uint32_t lane = __lane_id();
-    const int leader = m ? __ffsll(m) - 1 : 0;
-    uint32_t base = 0;
-    if (m && (int)lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
-__device__ __forceinline__ unsigned long long wave_slot64(unsigned long long *ctr, bool pred) {
-    const unsigned long long m = __ballot(pred);
-    const uint32_t lane = __lane_id();
-    const int leader = m ? __ffsll(m) - 1 : 0;
-    unsigned long long base = 0;
-    if (m && (int)lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // Block-wide append to N lists: for each list q, the threads whose f[q]
 // holds get consecutive positions run[q] + (their rank in the block), and
 // run[q] advances by the block's count. Every thread of the block calls it
@@ -155,6 +132,72 @@ __device__ __forceinline__ void block_append(const bool (&f)[N], uint32_t (&pos)
         run[q] += tot;
     }
     __syncthreads();
+}
+
+// Block-wide reservation of slots from per-network counters, one atomic per
+// (block, network): a thread's round k (of kTileRounds) names network
+// key(k) - 1 (key 0: none) and needs count(k) consecutive slots. Networks are
+// served in increasing order; for each, emit(k, first) is called for every
+// round of that network with the first of its slots, counter(key, total)
+// returning the first of the block's total slots. Every thread of the block
+// calls it (four barriers per network, two more at the end).
+template <int BLOCK, typename KeyF, typename CountF, typename CtrF, typename EmitF>
+__device__ __forceinline__ void block_reserve(uint32_t *lds, KeyF key, CountF count, CtrF counter, EmitF emit) {
+    constexpr int W = BLOCK / 64;
+    const int wave = threadIdx.x / 64;
+    const int lane = (int)__lane_id();
+    uint32_t done = 0;  // rounds already served (bit k)
+    for (int guard = 0; guard <= kMaxNets; ++guard) {
+        uint32_t mine = 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < kTileRounds; ++k) {
+            const uint32_t v = key(k);
+            if (v && !((done >> k) & 1u)) mine = min(mine, v);
+        }
+        for (int o = 32; o > 0; o >>= 1) mine = min(mine, (uint32_t)__shfl_xor((int)mine, o));
+        if (lane == 0) lds[wave] = mine;
+        __syncthreads();
+        uint32_t cur = 0xffffffffu;
+#pragma unroll
+        for (int v = 0; v < W; ++v) cur = min(cur, lds[v]);
+        __syncthreads();
+        if (cur == 0xffffffffu) return;  // the same in every thread
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kTileRounds; ++k)
+            if (key(k) == cur && !((done >> k) & 1u)) cnt += count(k);
+        uint32_t x = cnt;  // inclusive prefix sum over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) lds[wave] = x;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+            const uint32_t c = lds[v];
+            if (v < wave) before += c;
+            total += c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long b = total ? counter(cur, total) : 0ull;
+            lds[W] = (uint32_t)b;
+            lds[W + 1] = (uint32_t)(b >> 32);
+        }
+        __syncthreads();
+        unsigned long long first = ((unsigned long long)lds[W + 1] << 32 | lds[W]) + before + x - cnt;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTileRounds; ++k)
+            if (key(k) == cur && !((done >> k) & 1u)) {
+                emit(k, first);
+                first += count(k);
+                done |= 1u << k;
+            }
+    }
 }
 
 // Adds the block's sum of v (one value per thread) to *dst with one atomic.
@@ -261,12 +304,13 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t parity, int guided,
                                                           unsigned long long *__restrict__ counters) {
     __shared__ int lds_stack[kStackLds * BLOCK];
-    __shared__ uint32_t lds_red[BLOCK / 64];
+    __shared__ uint32_t lds_red[BLOCK / 64 + 2];
     const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
     const uint32_t n_in = w.tile_live[parity][t];
     const WfPaths &s = w.s[parity];
     uint32_t n_closest = 0;
-    for (uint32_t i = threadIdx.x; i < n_in; i += BLOCK) {
+    uint32_t pend = 0;  // round k: 1 + the network that takes the vertex's proposal row (5 bits), 0: none
+    for (uint32_t k = 0, i = threadIdx.x; i < n_in; ++k, i += BLOCK) {
         const uint32_t p = t * S + i;
         const float4 a = s.o_tmax[p], b = s.d_eta[p];
         const WfJob &job = w.jobs[__float_as_uint(s.L[p].w)];
@@ -286,65 +330,43 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t pa
             const uint32_t bounces = __float_as_uint(s.beta[p].w) & 0xFFu;
             if (guidable(m, job.gc.min_alpha) && bounces < job.max_depth) net = job.gc.net;
         }
-        unsigned long long left = __ballot(net >= 0);
-        while (left) {
-            const int lnet = __shfl(net, __ffsll(left) - 1);
-            const bool mine = net == lnet;
-            const uint32_t r = wave_slot(&w.ctr->guide_count[lnet][0], mine);
-            if (mine) gslot = lnet * (int)w.capacity + (int)r;
-            left &= ~__ballot(mine);
-        }
-        if (gslot >= 0) {
+        if (net >= 0) {
+            // the proposal's inputs, staged at the path's slot until the
+            // block has reserved its rows
             Isect it;
             hit_isect(sc, h, o, d, it);
-            float *in = w.guide_in + 6 * (size_t)gslot;
-            in[0] = (it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0];
-            in[1] = (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1];
-            in[2] = (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2];
-            in[3] = it.n.x;
-            in[4] = it.n.y;
-            in[5] = it.n.z;
+            w.gstage[2 * (size_t)p] = make_float4((it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0],
+                                                  (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1],
+                                                  (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2], 0.f);
+            w.gstage[2 * (size_t)p + 1] = make_float4(it.n.x, it.n.y, it.n.z, 0.f);
+            pend |= (uint32_t)(net + 1) << (5 * k);
         }
         w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
         w.hit1[p] = make_float4(h.b1, h.b2, __int_as_float(gslot), 0.f);
     }
-    if (counters) block_add<BLOCK>(counters, n_closest, lds_red);
-}
-
-// Training samples of a finished path whose vertices are in record slot
-// `rec`: for vertex k with a snapshot, the radiance that arrived along its
-// sampled direction is (L_final - L_snap) / beta_after; the weight is its
-// luminance over the pdf. They go to the region of the job's network `net`
-// (-1: none), reserved per wave and network. Every active lane of the wave
-// calls it.
-__device__ void emit_records(const WfState &w, uint32_t rec, uint32_t n, f3 Lf, int net) {
-    const float lf[3] = {Lf.x, Lf.y, Lf.z};
-    if (net < 0) n = 0;
-    for (uint32_t k = 0; __ballot(k < n); ++k) {
-        float li[3], wgt = 0.f;
-        const VertexRec *r = k < n ? &w.recs[(size_t)rec * w.max_depth + k] : nullptr;
-        if (r && r->lsnap[0] >= 0.f) {
-            for (int c = 0; c < 3; ++c) li[c] = r->beta[c] > 0.f ? fmaxf(0.f, lf[c] - r->lsnap[c]) / r->beta[c] : 0.f;
-            wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r->pdf;
-        }
-        const bool valid = r && wgt > 0.f && __builtin_isfinite(wgt);
-        const int vnet = valid ? net : -1;
-        unsigned long long left = __ballot(vnet >= 0);
-        while (left) {
-            const int lnet = __shfl(vnet, __ffsll(left) - 1);
-            const bool mine = vnet == lnet;
-            const unsigned long long s = wave_slot64(&w.ctr->rec_count[lnet][0], mine);
-            if (mine && s < w.rec_cap_net) {
-                float *o = w.train_rec + 10 * ((size_t)lnet * w.rec_cap_net + s);
-                for (int c = 0; c < 6; ++c) o[c] = r->in[c];
-                o[6] = r->wi[0];
-                o[7] = r->wi[1];
-                o[8] = r->wi[2];
-                o[9] = wgt;
-            }
-            left &= ~__ballot(mine);
-        }
+    if (guided) {
+        // proposal rows: one reservation per network for the whole block
+        // (per-wave atomics on the batch-wide row counter serialised)
+        block_reserve<BLOCK>(
+            lds_red, [&](int k) { return (pend >> (5 * k)) & 31u; }, [&](int) { return 1u; },
+            [&](uint32_t key, uint32_t total) {
+                return (unsigned long long)atomicAdd(&w.ctr->guide_count[key - 1][0], total);
+            },
+            [&](int k, unsigned long long first) {
+                const uint32_t p = t * S + (uint32_t)k * BLOCK + threadIdx.x;
+                const int gslot = (int)(((pend >> (5 * k)) & 31u) - 1u) * (int)w.capacity + (int)first;
+                const float4 a = w.gstage[2 * (size_t)p], b = w.gstage[2 * (size_t)p + 1];
+                float *in = w.guide_in + 6 * (size_t)gslot;
+                in[0] = a.x;
+                in[1] = a.y;
+                in[2] = a.z;
+                in[3] = b.x;
+                in[4] = b.y;
+                in[5] = b.z;
+                reinterpret_cast<float *>(&w.hit1[p])[2] = __int_as_float(gslot);
+            });
     }
+    if (counters) block_add<BLOCK>(counters, n_closest, lds_red);
 }
 
 // One bounce of tile t's live paths, tracing nothing. GUIDED (a proposal or
@@ -487,12 +509,44 @@ __global__ __launch_bounds__(BLOCK) void wf_nee_kernel(WfState w, uint32_t parit
 // Tile t's paths that ended this iteration: the guarded radiance into the
 // film (or its splat over the filter footprint), training records, record
 // slots back to the tile's free stack, the block's count added to done_paths.
+// Training sample of vertex k of record slot `rec` for a path that ended with
+// radiance lf: the radiance that arrived along its sampled direction is
+// (L_final - L_snap) / beta_after, its weight the luminance of that over the
+// pdf. False if the vertex has no snapshot or the weight is not positive and
+// finite; else the 10 floats (inputs, direction, weight) go to o (if not null).
+__device__ __forceinline__ bool vertex_record(const WfState &w, uint32_t rec, uint32_t k, const float *lf, float *o) {
+    const VertexRec &r = w.recs[(size_t)rec * w.max_depth + k];
+    if (!(r.lsnap[0] >= 0.f)) return false;
+    float li[3];
+    for (int c = 0; c < 3; ++c) li[c] = r.beta[c] > 0.f ? fmaxf(0.f, lf[c] - r.lsnap[c]) / r.beta[c] : 0.f;
+    const float wgt = (0.212671f * li[0] + 0.715160f * li[1] + 0.072169f * li[2]) / r.pdf;
+    if (!(wgt > 0.f && __builtin_isfinite(wgt))) return false;
+    if (o) {
+        for (int c = 0; c < 6; ++c) o[c] = r.in[c];
+        o[6] = r.wi[0];
+        o[7] = r.wi[1];
+        o[8] = r.wi[2];
+        o[9] = wgt;
+    }
+    return true;
+}
+
+// Ended paths of tile t: film (or sample buffer), then their training
+// samples, reserved once per block and network (per-wave atomics on the
+// batch-wide record counter serialised).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void wf_finish_kernel(WfState w, float *__restrict__ film) {
+    __shared__ uint32_t lds[BLOCK / 64 + 2];
     const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
     const uint32_t n_in = w.tile_fin[t];
     const uint32_t free0 = w.recs ? w.tile_free[t] : 0u;
-    for (uint32_t i = threadIdx.x; i < n_in; i += BLOCK) {
+    uint32_t pend = 0;  // round k: 1 + the network of its training samples (5 bits), 0: none
+    unsigned long long vmask[kTileRounds];  // round k: its vertices with a training sample
+#pragma unroll
+    for (int k = 0; k < kTileRounds; ++k) {
+        vmask[k] = 0ull;
+        const uint32_t i = threadIdx.x + (uint32_t)k * BLOCK;
+        if (i >= n_in) continue;
         const uint32_t q = t * S + i;
         const float4 l = w.fin_L[q];
         const uint4 meta = w.fin_meta[q];
@@ -511,10 +565,41 @@ __global__ __launch_bounds__(BLOCK) void wf_finish_kernel(WfState w, float *__re
         }
         if (w.recs) {
             const uint32_t rec = meta.w & ~kNoRecords;
-            emit_records(w, rec, (meta.w & kNoRecords) ? 0u : w.fin_nrec[q], L, job.gc.net);
+            const uint32_t n = (meta.w & kNoRecords) ? 0u : w.fin_nrec[q];
+            const int net = job.gc.net;
+            if (net >= 0) {
+                const float lf[3] = {L.x, L.y, L.z};
+                unsigned long long m = 0ull;
+                for (uint32_t v = 0; v < n; ++v)
+                    if (vertex_record(w, rec, v, lf, nullptr)) m |= 1ull << v;
+                vmask[k] = m;
+                if (m) pend |= (uint32_t)(net + 1) << (5 * k);
+            }
             w.rec_free[t * S + free0 + i] = rec;
         }
     }
+    if (w.recs)
+        block_reserve<BLOCK>(
+            lds, [&](int k) { return (pend >> (5 * k)) & 31u; },
+            [&](int k) { return (uint32_t)__popcll(vmask[k]); },
+            [&](uint32_t key, uint32_t total) {
+                return atomicAdd(&w.ctr->rec_count[key - 1][0], (unsigned long long)total);
+            },
+            [&](int k, unsigned long long first) {
+                const uint32_t q = t * S + threadIdx.x + (uint32_t)k * BLOCK;
+                const float4 l = w.fin_L[q];
+                const f3 L = guard_radiance(mk(l.x, l.y, l.z));
+                const float lf[3] = {L.x, L.y, L.z};
+                const uint32_t rec = w.fin_meta[q].w & ~kNoRecords;
+                const size_t region = (size_t)(((pend >> (5 * k)) & 31u) - 1u) * w.rec_cap_net;
+                unsigned long long m = vmask[k], slot = first;
+                while (m) {
+                    const uint32_t v = (uint32_t)(__ffsll(m) - 1);
+                    m &= m - 1ull;
+                    if (slot < w.rec_cap_net) vertex_record(w, rec, v, lf, w.train_rec + 10 * (region + slot));
+                    ++slot;
+                }
+            });
     if (threadIdx.x == 0 && n_in) {
         atomicAdd(&w.ctr->done_paths, (unsigned long long)n_in);
         if (w.recs) w.tile_free[t] = free0 + n_in;

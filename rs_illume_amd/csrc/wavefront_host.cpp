@@ -111,6 +111,7 @@ int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavef
     }
     if (e == hipSuccess) e = walloc(wf, &w.hit0, P);
     if (e == hipSuccess) e = walloc(wf, &w.hit1, P);
+    if (e == hipSuccess) e = walloc(wf, &w.gstage, 2 * P);
     if (e == hipSuccess) e = walloc(wf, &w.fin_L, P);
     if (e == hipSuccess) e = walloc(wf, &w.fin_meta, P);
     if (e == hipSuccess) e = walloc(wf, &w.fin_nrec, P);

@@ -141,13 +141,13 @@ def test_invalid_arguments_gpu(gpu):
         BVHAccel(np.array([[0, 0, np.nan], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
 
 
-@pytest.mark.parametrize("builder", ["SAH", "LBVH"])
+@pytest.mark.parametrize("builder", ["SAH", "SAH_GPU", "LBVH"])
 @pytest.mark.parametrize("width", [2, 4])
 def test_full_size_1m_tris_16m_rays(gpu, oracle, width, builder):
     """The 1M-triangle, 16M-incoherent-ray workload at full size: oracle parity
-    on every ray for the measured configuration (four-child SAH tree) and on a
-    strided 1/64 sample for the others, plus size-independent properties on
-    all rays."""
+    on every ray for the four-child SAH trees (the measured configuration
+    builds it on the device) and on a strided 1/64 sample for the others, plus
+    size-independent properties on all rays."""
     import torch
     from rs_illume_amd import workloads as W
     from rs_illume_amd.accel import BVHAccel, hits_to_numpy, rays_to_device, split_hits
@@ -171,7 +171,7 @@ def test_full_size_1m_tris_16m_rays(gpu, oracle, width, builder):
     hd2 = bvh.intersect(rd)
     assert torch.equal(hd, hd2)
     _say("gpu hit fraction", float(hit.float().mean()))
-    sample = np.arange(0, n_rays, 1 if (width == 4 and builder == "SAH") else 64)
+    sample = np.arange(0, n_rays, 1 if (width == 4 and builder in ("SAH", "SAH_GPU")) else 64)
     _say("oracle rays", len(sample))
     osc = oracle.OracleScene(v, idx, None, 4)
     rs = rays[sample]
