@@ -49,13 +49,16 @@ def progressive(render_pass, ref, target, spp_per_pass, max_spp, eps=0.01):
 
 def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=4096, max_depth=8, guide=None,
                     guide_frac=0.5, train_batch=65536, plain="megakernel", seed=1, max_paths=1 << 21, log=None,
-                    train_steps=16, warmup_steps=0, product=False):
+                    train_steps=16, warmup_steps=0, product=False, train_budget=None):
     """Time-to-relMSE of the plain and the guided tracer on one scene.
     ``plain`` selects the plain renderer ("megakernel" or "wavefront"). The
     guided tracer samples the BSDF alone (training on those paths) until the
     proposal has taken ``warmup_steps`` Adam steps, then uses one-sample MIS
     with probability ``guide_frac`` (``product``: the proposal times a vMF
-    fit of each vertex's BSDF lobe); every pass is an unbiased estimate."""
+    fit of each vertex's BSDF lobe); every pass is an unbiased estimate.
+    With ``train_budget`` the proposal trains online until it has taken that
+    many Adam steps, then stays frozen (its passes then keep no training
+    vertices)."""
     from .guide import Guide
     from .render import GuideCfg, PathIntegrator, Wavefront, WavefrontPathIntegrator
     w, h = int(np.asarray(camera)["width"].reshape(-1)[0]), int(np.asarray(camera)["height"].reshape(-1)[0])
@@ -91,12 +94,15 @@ def guided_vs_plain(scene, camera, ref, target=0.01, spp_per_pass=8, max_spp=409
                 gpi.gcfg.guide_frac = guide_frac
         else:
             gpi.render_sum(rgb_sum=film, spp=spp_per_pass, sample_offset=off)
+        if train_budget is not None and gpi.gcfg.train and g.stats()[0] >= train_budget:
+            gpi.gcfg.train = 0
 
     rg = progressive(guided_pass, ref, target, spp_per_pass, max_spp)
     if log:
         log("guided", rg["reached"], rg["curve"][-1])
     out = {"plain": rp, "guided": rg, "target": target, "spp_per_pass": spp_per_pass, "plain_renderer": plain,
-           "train_steps": train_steps, "warmup_steps": warmup_steps, "guide_frac": guide_frac, "product": product}
+           "train_steps": train_steps, "warmup_steps": warmup_steps, "guide_frac": guide_frac, "product": product,
+           "train_budget": train_budget, "adam_steps": g.stats()[0]}
     if rp["reached"] and rg["reached"]:
         out["time_ratio"] = rg["reached"]["seconds"] / rp["reached"]["seconds"]
     return out

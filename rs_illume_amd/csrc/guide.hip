@@ -141,11 +141,20 @@ __device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*
     }
 }
 
-// n rows of in [n][6] -> out [n][40] by the blocks blockIdx.x of one grid
-// row; n from n_ptr (at most max_n) when n_ptr is non-null.
+// Four consecutive outputs of a row.
+__device__ __forceinline__ void put4(float *o, float4 y) { *reinterpret_cast<float4 *>(o) = y; }
+__device__ __forceinline__ void put4(__half *o, float4 y) {
+    const uint32_t a = (uint32_t)__half_as_ushort(__float2half(y.x)) | (uint32_t)__half_as_ushort(__float2half(y.y)) << 16;
+    const uint32_t b = (uint32_t)__half_as_ushort(__float2half(y.z)) | (uint32_t)__half_as_ushort(__float2half(y.w)) << 16;
+    *reinterpret_cast<uint2 *>(o) = make_uint2(a, b);
+}
+
+// n rows of in [n][6] -> out [n][40] (float or fp16) by the blocks blockIdx.x
+// of one grid row; n from n_ptr (at most max_n) when n_ptr is non-null.
+template <typename OutT>
 __device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__restrict__ in,
                                            const uint32_t *__restrict__ n_ptr, uint32_t n_arg,
-                                           float *__restrict__ out) {
+                                           OutT *__restrict__ out) {
     __shared__ InferWeights W;
     const uint32_t n = n_ptr ? min(*n_ptr, n_arg) : n_arg;
     const uint32_t n_iter = (n + 31) / 32;
@@ -183,7 +192,7 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__res
                 if (o >= kOut) continue;
                 const float4 y = make_float4(acc[nn][m][0] + W.bias[3][o], acc[nn][m][1] + W.bias[3][o + 1],
                                              acc[nn][m][2] + W.bias[3][o + 2], acc[nn][m][3] + W.bias[3][o + 3]);
-                *reinterpret_cast<float4 *>(out + (size_t)row * kOut + o) = y;
+                put4(out + (size_t)row * kOut + o, y);
             }
         }
     }
@@ -201,7 +210,7 @@ __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_nets_kernel(cons
                                                                             const float *__restrict__ in,
                                                                             const uint32_t *__restrict__ counts,
                                                                             uint32_t count_stride, uint32_t stride,
-                                                                            float *__restrict__ out) {
+                                                                            __half *__restrict__ out) {
     const uint32_t k = blockIdx.y;
     const GuideDev g = nets[k];
     infer_rows(g, in + 6 * (size_t)k * stride, counts + (size_t)k * count_stride, stride,
@@ -820,7 +829,7 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
 }
 
 hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t count_stride, uint32_t stride, float *out, hipStream_t st) {
+                                   uint32_t count_stride, uint32_t stride, __half *out, hipStream_t st) {
     if (n_nets == 0 || stride == 0) return hipSuccess;
     const uint32_t blocks = min((stride + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
     guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, in, counts, count_stride, stride,

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <stdint.h>
 
 namespace illume {
@@ -68,9 +69,9 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
                                     float *out, hipStream_t st);
 // Several networks at once: network k (of n_nets, device array nets) infers
 // the counts[k * count_stride] (device) rows starting at row k * stride of in
-// and out.
+// and out; the outputs are stored as fp16 (what the renderer's shading reads).
 hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t count_stride, uint32_t stride, float *out, hipStream_t st);
+                                   uint32_t count_stride, uint32_t stride, __half *out, hipStream_t st);
 // Training scratch (partial): one gradient row per block of the fused pass
 // (at most kTrainBlocksMax blocks), then each block's weight sum and loss.
 constexpr uint32_t kTrainBlocksMax = 256;

@@ -4,6 +4,7 @@
 // optional learned proposal.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include "shade.h"
 #include "trace.h"
 
@@ -314,13 +315,25 @@ __device__ __forceinline__ float vmf_norm(float k) {
     return k > 0.f ? k * fast_rcp(6.28318530717958647692f * one_minus_exp_m2k(k)) : 0.0795774715459476679f;
 }
 
-// 8-lobe vMF mixture decoded from the network's 40 outputs (softmax weights,
-// normalised means, concentrations exp(clamp(r, -4.6, 6.9))); c[j] =
-// pi[j] * vmf_norm(kappa[j]), the lobe's density factor.
+// 8-lobe vMF mixture decoded from the network's 40 outputs, stored as fp16
+// (softmax weights, normalised means, concentrations exp(clamp(r, -4.6,
+// 6.9))); c[j] = pi[j] * vmf_norm(kappa[j]), the lobe's density factor.
 struct Mixture {
     float pi[8], mu[8][3], kappa[8], c[8];
 };
-__device__ __forceinline__ bool decode_mixture(const float *__restrict__ y, Mixture &m) {
+__device__ __forceinline__ bool decode_mixture(const __half *__restrict__ yh, Mixture &m) {
+    float y[40];
+    const uint4 *src = reinterpret_cast<const uint4 *>(yh);  // 80 bytes, 16-byte aligned rows
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const uint4 v = src[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            y[8 * q + 2 * e] = __half2float(__ushort_as_half((unsigned short)(w[e] & 0xffffu)));
+            y[8 * q + 2 * e + 1] = __half2float(__ushort_as_half((unsigned short)(w[e] >> 16)));
+        }
+    }
     float mx = y[0];
     for (int j = 1; j < 8; ++j) mx = fmaxf(mx, y[5 * j]);
     float s = 0.f;
@@ -422,7 +435,7 @@ __device__ __forceinline__ bool mixture_times_lobe(Mixture &m, f3 mu_b, float k_
 }
 
 struct GuideCfg {
-    const float *out;         // [paths][40] network outputs, null = plain path tracing
+    const __half *out;        // [paths][40] network outputs (fp16), null = plain path tracing
     float frac;               // probability of sampling the proposal (one-sample MIS)
     float bounds_lo[3], inv_ext[3];
     float min_alpha;          // microfacet vertices below this roughness are not guided
@@ -514,7 +527,7 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
 // are then unused.
 template <int BLOCK, bool DEFER = false>
 __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
-                           const float *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
+                           const __half *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
                            TraceStack<BLOCK> &st, PathCounters &pc, DeferredDirect *dd = nullptr) {
     if (DEFER) dd->pending = false;
     if (recs && n_recs > 0 && recs[n_recs - 1].lsnap[0] < 0.f) {

@@ -75,7 +75,7 @@ struct WfState {
     uint32_t *tile_free;     // per tile: free training-record slots
     uint32_t *rec_free;      // per tile: its free record slots (a stack from the tile's first slot)
     float *guide_in;         // [n_nets][capacity][6]: network k's rows start at row k * capacity
-    float *guide_out;        // [n_nets][capacity][40]
+    __half *guide_out;       // [n_nets][capacity][40], fp16
     const GuideDev *nets;    // device array of the batch's networks
     uint32_t n_nets;
     VertexRec *recs;         // [capacity][max_depth] by record slot, null when not training

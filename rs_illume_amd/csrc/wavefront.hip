@@ -405,7 +405,7 @@ __global__ __launch_bounds__(BLOCK, 3) void wf_shade_kernel(WfState w, uint32_t 
             h.b1 = h1.x;
             h.b2 = h1.y;
             const bool found = h.prim != 0xFFFFFFFFu;
-            const float *gout = nullptr;
+            const __half *gout = nullptr;
             VertexRec *recs = nullptr;
             if (GUIDED) {
                 const int gslot = __float_as_int(h1.z);

@@ -173,7 +173,8 @@ hipError_t ensure_net_rows(illume_wavefront *wf, uint32_t n_nets) {
         hipFree(p);
         wf->allocs.erase(std::remove(wf->allocs.begin(), wf->allocs.end(), p), wf->allocs.end());
     }
-    w.guide_in = w.guide_out = nullptr;
+    w.guide_in = nullptr;
+    w.guide_out = nullptr;
     wf->nets_alloc = 0;
     drop_graphs(wf);
     hipError_t e = walloc(wf, &w.guide_in, 6 * (size_t)w.capacity * n_nets);
