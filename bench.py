@@ -15,6 +15,8 @@ Also measured (keys of the same JSON line):
   guided   - glossy small-light Cornell box 512x512, depth 8: wall time until
              relMSE <= 0.01 against a 16384-spp plain render, plain GPU path
              tracer vs the guided one (online training included in its time);
+  guided_indirect - the same comparison in a box lit only indirectly (its
+             light facing the ceiling), where the proposal pays;
   mlp      - the proposal network's MFMA kernels: inference and training
              throughput as a fraction of the bf16 dense peak;
   batched  - 64 procedural scenes of uneven size in one batched call: the
@@ -275,6 +277,52 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
         cpu = cpu_progressive(osc, cam, ref.cpu().numpy(), 8, 1, 2, cpu_budget_s)
         cpu["sample"] = "the whole %dx%d frame in 2-spp passes for a %.0f s wall budget" % (res, res, cpu_budget_s)
         out["cpu_baseline"] = cpu
+    sc.close()
+    return out
+
+
+def bench_guided_indirect(ref_spp=16384, res=512, log=log, spp_per_pass=16, warmup_steps=16, train_steps=8,
+                          train_budget=256, guide_frac=0.5):
+    """The proposal where guiding pays: config 0's box with its light turned
+    over (facing the ceiling, L = 60), so every surface but the ceiling is lit
+    only indirectly, 512x512, max depth 8. Time to relMSE 0.01 against a
+    16384-spp plain render, plain megakernel vs the guided wavefront tracer
+    (trained online from scratch: BSDF-only passes until 16 Adam steps, then
+    one-sample MIS at fraction 0.5; training stops after train_budget steps and
+    its time is included)."""
+    import torch
+    from rs_illume_amd import converge as C
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.render import PathIntegrator, Scene
+    sc = Scene(S.cornell_box(0, light_up=True, light_L=60.0))
+    cam = S.cornell_camera(res, res)
+    pi = PathIntegrator(cam, spp=256, max_depth=8, seed=987654321)
+    ref = torch.zeros((res, res, 3), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(ref_spp // 256):
+        pi.render_sum(sc, rgb_sum=ref, spp=256, sample_offset=256 * k)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    ref /= ref_spp
+    log("indirect room: reference %d spp in %.1f s" % (ref_spp, ref_s))
+    r = C.guided_vs_plain(sc, cam, ref, target=0.01, spp_per_pass=spp_per_pass, max_spp=16384, max_depth=8, seed=1,
+                          train_steps=train_steps, warmup_steps=warmup_steps, plain="megakernel",
+                          guide_frac=guide_frac, product=False, train_budget=train_budget)
+    out = {"workload": "Cornell box %dx%d lit only indirectly (the 0.25^2 light at y = 0.9 facing the ceiling, "
+                       "L = 60), Lambertian, max depth 8; relMSE vs %d-spp plain render" % (res, res, ref_spp),
+           "target_rel_mse": 0.01, "reference_spp": ref_spp, "reference_s": round(ref_s, 2),
+           "spp_per_pass": spp_per_pass, "plain_renderer": "megakernel", "warmup_steps": warmup_steps,
+           "train_steps_per_replay": train_steps, "train_budget": train_budget, "adam_steps": r["adam_steps"],
+           "guide_frac": guide_frac}
+    for k in ("plain", "guided"):
+        rr = r[k]["reached"]
+        out[k] = {"seconds": round(rr["seconds"], 4), "spp": rr["spp"], "rel_mse": round(rr["rel_mse"], 5)} \
+            if rr else {"seconds": None, "spp": r[k]["spp"], "rel_mse": round(r[k]["curve"][-1][2], 5)}
+        out[k]["seconds_per_spp_ms"] = round(1e3 * r[k]["seconds"] / max(r[k]["spp"], 1), 4)
+    out["time_ratio_guided_over_plain"] = round(r["time_ratio"], 3) if "time_ratio" in r else None
+    out["spp_ratio_guided_over_plain"] = round(out["guided"]["spp"] / out["plain"]["spp"], 3) \
+        if r["plain"]["reached"] and r["guided"]["reached"] else None
     sc.close()
     return out
 
@@ -567,7 +615,7 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing measurements")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (roofline from algorithmic bytes only)")
-    ap.add_argument("--only", choices=("cornell", "mlp", "guided", "batched", "combined"),
+    ap.add_argument("--only", choices=("cornell", "mlp", "guided", "guided_indirect", "batched", "combined"),
                     help="run one path-tracing measurement alone and print its JSON")
     ap.add_argument("--rays", type=int, default=N_RAYS)
     ap.add_argument("--width", type=int, default=4, choices=(2, 4), help="BVH node width")
@@ -608,7 +656,8 @@ def main():
 
     if args.only:
         fn = {"cornell": lambda: bench_cornell(args.no_cpu), "mlp": bench_mlp,
-              "guided": lambda: bench_guided(no_cpu=args.no_cpu), "batched": lambda: bench_batched(args.no_cpu),
+              "guided": lambda: bench_guided(no_cpu=args.no_cpu), "guided_indirect": bench_guided_indirect,
+              "batched": lambda: bench_batched(args.no_cpu),
               "combined": lambda: bench_combined(no_cpu=args.no_cpu)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
@@ -727,6 +776,8 @@ def main():
         extra["mlp"] = bench_mlp()
         log("guided vs plain")
         extra["guided"] = bench_guided(no_cpu=args.no_cpu)
+        log("guided vs plain, indirectly lit room")
+        extra["guided_indirect"] = bench_guided_indirect()
         log("batched")
         extra["batched"] = bench_batched(args.no_cpu, pmc=pmc_batched, builder=args.builder)
         log("combined")
