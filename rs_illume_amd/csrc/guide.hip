@@ -55,8 +55,8 @@ __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
 // Frequency encoding: input i (position xyz in [0,1], normal xyz), octave j:
-// sin(2^j pi v) at feature 24 i + 2 j, cos at 24 i + 2 j + 1; features
-// 144..159 are zero. The angle is reduced exactly (2^j pi v =
+// sin(2^j pi v) at feature 24 i + 2 j, cos at 24 i + 2 j + 1 (layer 0's
+// 160 inputs hold 16 zeros besides). The angle is reduced exactly (2^j pi v =
 // 2 pi fract(2^(j-1) v): scaling by a power of two and taking the fraction
 // are exact) and the hardware v_sin/v_cos, which take their argument in
 // revolutions, do the rest.
@@ -89,28 +89,37 @@ __device__ __forceinline__ bf16x8 lds_wa(const __bf16 *w, int ld, int m, int j, 
 }
 
 // Encoding of one row into layer 0's B operand: lane group g computes
-// features 40 g .. 40 g + 39 (input i = f / 24, octave o = (f % 24) / 2,
-// sin at even f, cos at odd f); features 144.. are zero.
+// octaves 3g .. 3g + 2 of all six inputs, octave 3g + d of input i at slot
+// s = 12 d + 2 i (sine) and s + 1 (cosine); slots 36..39 are zero. Octave 3g
+// comes from the hardware sin / cos of the exactly reduced angle, the next two
+// from the double-angle formulas sin 2a = 2 sin a cos a and cos 2a =
+// (cos a - sin a)(cos a + sin a): their error, a few float ulps, is far below
+// the bf16 rounding of the operand, and a row costs 12 transcendentals per
+// lane instead of 40.
 __device__ __forceinline__ void encode_b(const float *__restrict__ in, uint32_t stride, uint32_t row, uint32_t n, int g,
                                          bf16x8 *b) {
-    float v[6];
+    float e[40];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) v[i] = row < n ? in[(size_t)row * stride + i] : 0.f;
+    for (int i = 0; i < 6; ++i) {
+        const float x = row < n ? in[(size_t)row * stride + i] : 0.f;
+        const float r = ldexpf(x, 3 * g - 1);
+        const float rev = r - floorf(r);
+        float sv = __builtin_amdgcn_sinf(rev), cv = __builtin_amdgcn_cosf(rev);
 #pragma unroll
-    for (int s = 0; s < 40; s += 2) {
-        const int f = 40 * g + s;
-        float sv = 0.f, cv = 0.f;
-        if (f < kEnc) {
-            const int i = f / 24, o = (f % 24) / 2;
-            const float x = i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : i == 3 ? v[3] : i == 4 ? v[4] : v[5];
-            const float r = ldexpf(x, o - 1);
-            const float rev = r - floorf(r);
-            sv = __builtin_amdgcn_sinf(rev);
-            cv = __builtin_amdgcn_cosf(rev);
+        for (int d = 0; d < 3; ++d) {
+            if (d > 0) {
+                const float s2 = 2.f * sv * cv;
+                cv = (cv - sv) * (cv + sv);
+                sv = s2;
+            }
+            e[12 * d + 2 * i] = sv;
+            e[12 * d + 2 * i + 1] = cv;
         }
-        b[s / 8][s % 8] = to_bf16(sv);
-        b[s / 8][s % 8 + 1] = to_bf16(cv);
     }
+#pragma unroll
+    for (int q = 36; q < 40; ++q) e[q] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 40; ++q) b[q / 8][q % 8] = to_bf16(e[q]);
 }
 
 // acc (output tiles of one 16-row column) -> bias, ReLU, bf16 -> the next

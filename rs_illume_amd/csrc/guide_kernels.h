@@ -41,15 +41,18 @@ constexpr int kGuideWaLd[4] = {168, 72, 72, 72};
 constexpr int kGuideWaRows[4] = {64, 64, 64, 48};
 
 // Slot of input k of layer l in the inference kernel's register order.
-// Layer 0 (encoding feature f, 0..143): lane group g = f / 40 computes
-// features 40g..40g+39 of its row; slot s = f % 40 lands in k-block s / 8,
-// element s % 8, i.e. k = 32 (s / 8) + 8 g + s % 8. Layers 1..3 (hidden unit
+// Layer 0 (encoding feature f = 24 i + 2 o + c, 0..143: input i, octave o,
+// c = 1 for the cosine): lane group g = o / 3 computes octaves 3g..3g+2 of
+// every input of its row, feature f at slot s = 12 (o % 3) + 2 i + c, which
+// lands in k-block s / 8, element s % 8, i.e. k = 32 (s / 8) + 8 g + s % 8
+// (slots 36..39 of each group are zero). Layers 1..3 (hidden unit
 // d, the output row of the previous layer's MFMA tile m = d / 16, row
 // r = d % 16): the lane that holds it is in group r / 4, element r % 4, and
 // tiles 2j, 2j+1 fill k-block j, so k = 32 (m / 2) + 8 (r / 4) + 4 (m % 2) + r % 4.
 __host__ __device__ inline int guide_wa_pos(int l, int k) {
     if (l == 0) {
-        const int g = k / 40, s = k % 40;
+        const int i = k / 24, o = (k % 24) / 2, c = k % 2;
+        const int g = o / 3, s = 12 * (o % 3) + 2 * i + c;
         return 32 * (s / 8) + 8 * g + s % 8;
     }
     const int m = k / 16, r = k % 16;

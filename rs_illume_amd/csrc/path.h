@@ -318,10 +318,24 @@ __device__ __forceinline__ float vmf_norm(float k) {
 // 8-lobe vMF mixture decoded from the network's 40 outputs, stored as fp16
 // (softmax weights, normalised means, concentrations exp(clamp(r, -4.6,
 // 6.9))); c[j] = pi[j] * vmf_norm(kappa[j]), the lobe's density factor.
+// The decoded lobes live in LDS, not registers: the shading code around them
+// needs every register it has at three waves per SIMD. A block keeps
+// [8][STRIDE] float4 (mu, kappa) and [8][STRIDE] float2 (pi, c), lobe-major
+// and thread-minor, so a wave's accesses to one lobe are contiguous; each
+// thread reaches only its own column.
+template <int STRIDE>
 struct Mixture {
-    float pi[8], mu[8][3], kappa[8], c[8];
+    float4 *mk;  // this thread's column of the (mu, kappa) array
+    float2 *pc;  // this thread's column of the (pi, c) array
+    __device__ float4 &lobe(int j) const { return mk[j * STRIDE]; }
+    __device__ float2 &weight(int j) const { return pc[j * STRIDE]; }
 };
-__device__ __forceinline__ bool decode_mixture(const __half *__restrict__ yh, Mixture &m) {
+// LDS bytes of a block's mixtures
+template <int STRIDE>
+constexpr uint32_t kMixtureLds = 8 * STRIDE * (sizeof(float4) + sizeof(float2));
+
+template <int STRIDE>
+__device__ __forceinline__ bool decode_mixture(const __half *__restrict__ yh, const Mixture<STRIDE> &m) {
     float y[40];
     const uint4 *src = reinterpret_cast<const uint4 *>(yh);  // 80 bytes, 16-byte aligned rows
 #pragma unroll
@@ -335,55 +349,54 @@ __device__ __forceinline__ bool decode_mixture(const __half *__restrict__ yh, Mi
         }
     }
     float mx = y[0];
+#pragma unroll
     for (int j = 1; j < 8; ++j) mx = fmaxf(mx, y[5 * j]);
-    float s = 0.f;
+    float pi[8], s = 0.f;
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-        m.pi[j] = fast_exp(y[5 * j] - mx);
-        s += m.pi[j];
+        pi[j] = fast_exp(y[5 * j] - mx);
+        s += pi[j];
     }
     bool ok = s > 0.f && __builtin_isfinite(s);
     const float inv_s = fast_rcp(s);
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-        m.pi[j] *= inv_s;
+        const float p = pi[j] * inv_s;
         const float a = y[5 * j + 1], b = y[5 * j + 2], c = y[5 * j + 3];
         const float len2 = a * a + b * b + c * c;
         const float inv = len2 > 0.f ? __builtin_amdgcn_rsqf(len2) : 0.f;
-        m.mu[j][0] = a * inv;
-        m.mu[j][1] = b * inv;
-        m.mu[j][2] = len2 > 0.f ? c * inv : 1.f;
-        m.kappa[j] = fast_exp(fminf(fmaxf(y[5 * j + 4], -4.6f), 6.9f));
-        m.c[j] = m.pi[j] * vmf_norm(m.kappa[j]);
-        ok = ok && __builtin_isfinite(m.pi[j]) && __builtin_isfinite(len2) && __builtin_isfinite(m.kappa[j]);
+        const float kappa = fast_exp(fminf(fmaxf(y[5 * j + 4], -4.6f), 6.9f));
+        m.lobe(j) = make_float4(a * inv, b * inv, len2 > 0.f ? c * inv : 1.f, kappa);
+        m.weight(j) = make_float2(p, p * vmf_norm(kappa));
+        ok = ok && __builtin_isfinite(p) && __builtin_isfinite(len2) && __builtin_isfinite(kappa);
     }
     return ok;
 }
-__device__ __forceinline__ float mixture_pdf(const Mixture &m, f3 w) {
+template <int STRIDE>
+__device__ __forceinline__ float mixture_pdf(const Mixture<STRIDE> &m, f3 w) {
     float q = 0.f;
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const float c = m.mu[j][0] * w.x + m.mu[j][1] * w.y + m.mu[j][2] * w.z;
-        q += m.c[j] * fast_exp(m.kappa[j] * (c - 1.f));
+        const float4 l = m.lobe(j);
+        const float c = l.x * w.x + l.y * w.y + l.z * w.z;
+        q += m.weight(j).y * fast_exp(l.w * (c - 1.f));
     }
     return q;
 }
 // Lobe by ul, then the vMF sample of Jakob (2012) around the lobe mean.
-__device__ __forceinline__ f3 mixture_sample(const Mixture &m, float ul, float u1, float u2) {
-    // the first lobe whose cumulative weight exceeds ul (the last if none);
-    // selected with constant indices so that the mixture stays in registers
+template <int STRIDE>
+__device__ __forceinline__ f3 mixture_sample(const Mixture<STRIDE> &m, float ul, float u1, float u2) {
+    // the first lobe whose cumulative weight exceeds ul (the last if none)
     int sel = 7;
     float c = 0.f;
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-        c += m.pi[j];
+        c += m.weight(j).x;
         if (sel == 7 && ul < c) sel = j;
     }
-    float k = m.kappa[0];
-    f3 mu = mk(m.mu[0][0], m.mu[0][1], m.mu[0][2]);
-#pragma unroll
-    for (int j = 1; j < 8; ++j)
-        if (sel == j) {
-            k = m.kappa[j];
-            mu = mk(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
-        }
+    const float4 l = m.lobe(sel);
+    const float k = l.w;
+    const f3 mu = mk(l.x, l.y, l.z);
     float w = 1.f + __builtin_amdgcn_logf(u1 + (1.f - u1) * fast_exp(-2.f * k)) * kLn2 * fast_rcp(k);
     w = fminf(1.f, fmaxf(-1.f, w));
     const float s = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - w * w));
@@ -407,29 +420,30 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture &m, float ul, float u
 // c(k_j) c(k_b) exp(k - k_j - k_b) / c(k), c(k) = vmf_norm(k). The result is
 // again a normalised mixture, so its pdf is exact. False if the weights
 // degenerate.
-__device__ __forceinline__ bool mixture_times_lobe(Mixture &m, f3 mu_b, float k_b) {
+template <int STRIDE>
+__device__ __forceinline__ bool mixture_times_lobe(const Mixture<STRIDE> &m, f3 mu_b, float k_b) {
     float s = 0.f;
     const float cb = vmf_norm(k_b);
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const float kj = m.kappa[j];
-        const float vx = kj * m.mu[j][0] + k_b * mu_b.x, vy = kj * m.mu[j][1] + k_b * mu_b.y,
-                    vz = kj * m.mu[j][2] + k_b * mu_b.z;
+        const float4 l = m.lobe(j);
+        const float kj = l.w;
+        const float vx = kj * l.x + k_b * mu_b.x, vy = kj * l.y + k_b * mu_b.y, vz = kj * l.z + k_b * mu_b.z;
         const float k = fmaxf(__builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz), 1e-4f);
         const float ck = vmf_norm(k);
         const float ik = fast_rcp(k);
-        m.pi[j] *= vmf_norm(kj) * cb * fast_exp(fminf(k - kj - k_b, 0.f)) * fast_rcp(ck);
-        m.mu[j][0] = vx * ik;
-        m.mu[j][1] = vy * ik;
-        m.mu[j][2] = vz * ik;
-        m.kappa[j] = k;
-        m.c[j] = ck;  // times the normalised weight below
-        s += m.pi[j];
+        const float p = m.weight(j).x * (vmf_norm(kj) * cb * fast_exp(fminf(k - kj - k_b, 0.f)) * fast_rcp(ck));
+        m.lobe(j) = make_float4(vx * ik, vy * ik, vz * ik, k);
+        m.weight(j) = make_float2(p, ck);  // c times the normalised weight below
+        s += p;
     }
     if (!(s > 0.f) || !__builtin_isfinite(s)) return false;
     const float inv = fast_rcp(s);
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-        m.pi[j] *= inv;
-        m.c[j] *= m.pi[j];
+        const float2 wc = m.weight(j);
+        const float p = wc.x * inv;
+        m.weight(j) = make_float2(p, wc.y * p);
     }
     return true;
 }
@@ -521,14 +535,16 @@ __device__ __forceinline__ void path_start_at(const DevCamera &cam, uint64_t see
 // With a proposal (gout != null, valid mixture, non-specular BSDF) the
 // continuation is one-sample MIS: the mixture with probability frac, else the
 // BSDF, weighted by frac * q + (1 - frac) * p_bsdf. recs (may be null) gets a
-// training vertex per sampled continuation; n_recs counts them.
+// training vertex per sampled continuation; n_recs counts them. mix holds
+// this thread's mixture columns in LDS, required when gout is not null.
 // DEFER: the direct-lighting rays are not traced here but returned in *dd
 // (dd->pending), for the caller to trace and add with add_direct; st and pc
 // are then unused.
 template <int BLOCK, bool DEFER = false>
 __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_depth, PathState &ps, bool found, const TraceHit &h,
                            const __half *__restrict__ gout, const GuideCfg &gc, VertexRec *recs, uint32_t &n_recs,
-                           TraceStack<BLOCK> &st, PathCounters &pc, DeferredDirect *dd = nullptr) {
+                           TraceStack<BLOCK> &st, PathCounters &pc, DeferredDirect *dd = nullptr,
+                           Mixture<BLOCK> mix = {nullptr, nullptr}) {
     if (DEFER) dd->pending = false;
     if (recs && n_recs > 0 && recs[n_recs - 1].lsnap[0] < 0.f) {
         // the radiance gathered so far excludes everything arriving along the
@@ -579,7 +595,6 @@ __device__ __forceinline__ void path_shade(const DevScene &sc, uint32_t max_dept
     float pdf = 0;
     int flags = 0;
     f3 f;
-    Mixture mix;
     bool guided = gout && nonspec && decode_mixture(gout, mix);
     if (guided && gc.product) {
         // the lobe about the mirror direction (microfacet: concentration 0.8 /

@@ -381,6 +381,11 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t pa
 template <int BLOCK, bool GUIDED>
 __global__ __launch_bounds__(BLOCK, 3) void wf_shade_kernel(WfState w, uint32_t parity) {
     __shared__ uint32_t lds[3 * (BLOCK / 64)];
+    // the decoded proposal of each thread's vertex (48 KB at 256 threads:
+    // three blocks per CU still fit)
+    __shared__ float4 mix_mk[GUIDED ? 8 * BLOCK : 1];
+    __shared__ float2 mix_pc[GUIDED ? 8 * BLOCK : 1];
+    static_assert(!GUIDED || sizeof(mix_mk) + sizeof(mix_pc) == kMixtureLds<BLOCK>, "mixture LDS layout");
     const uint32_t t = blockIdx.x, S = kTileRounds * BLOCK;
     const uint32_t n_in = w.tile_live[parity][t];
     const uint32_t nxt = parity ^ 1u;
@@ -414,8 +419,10 @@ __global__ __launch_bounds__(BLOCK, 3) void wf_shade_kernel(WfState w, uint32_t 
             }
             TraceStack<BLOCK> st;  // unused: shading traces nothing
             PathCounters pc{0, 0};
+            Mixture<BLOCK> mix{nullptr, nullptr};
+            if (GUIDED) mix = Mixture<BLOCK>{mix_mk + threadIdx.x, mix_pc + threadIdx.x};
             path_shade<BLOCK, true>(*job.scene, job.max_depth, ps, found, h, gout, job.gc, recs, n_recs, st, pc,
-                                    &dd);
+                                    &dd, mix);
             f[0] = ps.active;
             f[1] = !ps.active;
             // a non-finite throughput turns even a zero direct term into NaN,
