@@ -444,18 +444,41 @@ __device__ __forceinline__ floatx4 dw_tile(const TrainLds &L, int it, int u, flo
     }
     return acc;
 }
+// Two dW tiles of one input tile (it, u0) and (it, u1): each A fragment read
+// once for both products.
+__device__ __forceinline__ void dw_tile2(const TrainLds &L, int it, int u0, int u1, floatx4 &acc0, floatx4 &acc1,
+                                         float &colsum0, float &colsum1, bool want_bias, int lane) {
+    const int fo = frag_off(lane);
+#pragma unroll
+    for (int ti = 0; ti < kTrainWaves; ++ti) {
+        const TrainWaveLds &T = L.wave[ti];
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(T.x + 16 * it * kStageLd + fo);
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(T.yg.g + 16 * u0 * kStageLd + fo);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(T.yg.g + 16 * u1 * kStageLd + fo);
+        acc0 = mfma(a, b0, acc0);
+        acc1 = mfma(a, b1, acc1);
+        if (want_bias)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                colsum0 += (float)b0[e];
+                colsum1 += (float)b1[e];
+            }
+    }
+}
 
 // Layer 0's A operand: encoding feature 16 it + r16 of rows 8 grp .. 8 grp + 7
-// of a tile, from its staged inputs tin[32][6].
+// of a tile, from its staged inputs tin[32][6]. A cosine is the sine a quarter
+// revolution on (one transcendental per element, not both and a select).
 __device__ __forceinline__ bf16x8 enc_a(const float (*tin)[6], int it, int lane) {
     const int grp = lane >> 4, r16 = lane & 15;
     const int f = 16 * it + r16, i = f / 24, o = (f % 24) / 2;
+    const float quarter = (f & 1) ? 0.25f : 0.f;
     bf16x8 a;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const float r = ldexpf(tin[8 * grp + e][i], o - 1);
         const float rev = r - floorf(r);
-        a[e] = to_bf16((f & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev));
+        a[e] = to_bf16(__builtin_amdgcn_sinf(rev + quarter));
     }
     return a;
 }
@@ -619,8 +642,10 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         mask_to_b(acc[0], mask3, 0, gz[0]);
         mask_to_b(acc[1], mask3, 1, gz[1]);
         __syncthreads();
-        d3[0] = dw_tile(L, itw, uo, d3[0], c3[0], bias_hi, lane);
-        if (uo == 0) d3[1] = dw_tile(L, itw, 2, d3[1], c3[1], bias_hi, lane);
+        if (uo == 0)
+            dw_tile2(L, itw, 0, 2, d3[0], d3[1], c3[0], c3[1], bias_hi, lane);
+        else
+            d3[0] = dw_tile(L, itw, 1, d3[0], c3[0], bias_hi, lane);
         __syncthreads();
         // ---- G_2 and X_2 staged, G_1 (registers), dW_2
         stage_units(gz, S.yg.g, lane);
@@ -629,8 +654,7 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         mask_to_b(acc[0], mask2, 0, gz[0]);
         mask_to_b(acc[1], mask2, 1, gz[1]);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 2; ++j) d2[j] = dw_tile(L, itw, uo + 2 * j, d2[j], c2[j], bias_hi, lane);
+        dw_tile2(L, itw, uo, uo + 2, d2[0], d2[1], c2[0], c2[1], bias_hi, lane);
         __syncthreads();
         // ---- G_1 and X_1 staged, G_0 (registers), dW_1
         stage_units(gz, S.yg.g, lane);
@@ -639,8 +663,7 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         mask_to_b(acc[0], mask1, 0, gz[0]);
         mask_to_b(acc[1], mask1, 1, gz[1]);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 2; ++j) d1[j] = dw_tile(L, itw, uo + 2 * j, d1[j], c1[j], bias_hi, lane);
+        dw_tile2(L, itw, uo, uo + 2, d1[0], d1[1], c1[0], c1[1], bias_hi, lane);
         __syncthreads();
         // ---- G_0 staged, dW_0 (A = the encoding of the round's rows)
         stage_units(gz, S.yg.g, lane);
@@ -728,11 +751,28 @@ __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, cons
     __shared__ float4 part[kRedGroups][16];
     __shared__ float tot[2];
     constexpr uint32_t kCols = kGuideParams / 4;
+    // every load is issued before the first result is needed: this thread's
+    // Adam state, the weight sums and losses, the partial rows
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    const bool owner = threadIdx.x < 64 && i < (uint32_t)kGuideParams;
+    float m0 = 0.f, v0 = 0.f, p0 = 0.f;
+    if (owner && adam) {
+        m0 = g.m[i];
+        v0 = g.v[i];
+        p0 = g.params[i];
+    }
     if (threadIdx.x < 64) {
         // total weight (lanes 0..31) and loss (lanes 32..63) over the rows
         const int which = threadIdx.x / 32;
+        float w[kTrainBlocksMax / 32];
+#pragma unroll
+        for (int k = 0; k < (int)kTrainBlocksMax / 32; ++k) {
+            const uint32_t r = threadIdx.x % 32 + 32 * k;
+            w[k] = r < rows ? wl[2 * r + which] : 0.f;
+        }
         float v = 0.f;
-        for (uint32_t r = threadIdx.x % 32; r < rows; r += 32) v += wl[2 * r + which];
+#pragma unroll
+        for (int k = 0; k < (int)kTrainBlocksMax / 32; ++k) v += w[k];
         for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
         if (threadIdx.x % 32 == 0) tot[which] = v;
     }
@@ -763,8 +803,7 @@ __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, cons
         if (wsum) *wsum = ws;
         if (loss) *loss = tot[1] * inv;
     }
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-    if (threadIdx.x < 64 && i < (uint32_t)kGuideParams) {
+    if (owner) {
         const int cc = threadIdx.x / 4, e = threadIdx.x % 4;
         float sum = 0.f;
 #pragma unroll
@@ -776,11 +815,11 @@ __global__ __launch_bounds__(256) void guide_reduce_adam_kernel(GuideDev g, cons
         if (grad) grad[i] = gi;
         if (adam) {
             const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-            const float m = b1 * g.m[i] + (1.f - b1) * gi;
-            const float v = b2 * g.v[i] + (1.f - b2) * gi * gi;
+            const float m = b1 * m0 + (1.f - b1) * gi;
+            const float v = b2 * v0 + (1.f - b2) * gi * gi;
             g.m[i] = m;
             g.v[i] = v;
-            const float pnew = g.params[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+            const float pnew = p0 - lr * (m / c1) / (sqrtf(v / c2) + eps);
             g.params[i] = pnew;
             write_bf16_copy(g, i, pnew);
         }
