@@ -96,12 +96,39 @@ __device__ __forceinline__ bf16x8 lds_wa(const __bf16 *w, int ld, int m, int j, 
 // (cos a - sin a)(cos a + sin a): their error, a few float ulps, is far below
 // the bf16 rounding of the operand, and a row costs 12 transcendentals per
 // lane instead of 40.
-__device__ __forceinline__ void encode_b(const float *__restrict__ in, uint32_t stride, uint32_t row, uint32_t n, int g,
-                                         bf16x8 *b) {
+// The inputs of a row come from a RowSource: rows of a float array, or the
+// proposal inputs of path slots gathered through a row -> slot list.
+struct RowsDirect {
+    const float *__restrict__ in;
+    uint32_t stride;  // floats per row; the inputs are its first six
+    __device__ void load(uint32_t row, float *v) const {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) v[i] = in[(size_t)row * stride + i];
+    }
+};
+struct RowsGather {
+    const uint32_t *__restrict__ slots;  // row -> path slot
+    const float4 *__restrict__ stage;    // [2 p]: position xyz, [2 p + 1]: normal xyz
+    __device__ void load(uint32_t row, float *v) const {
+        const uint32_t p = slots[row];
+        const float4 a = stage[2 * (size_t)p], c = stage[2 * (size_t)p + 1];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = c.x;
+        v[4] = c.y;
+        v[5] = c.z;
+    }
+};
+
+template <typename Src>
+__device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t n, int g, bf16x8 *b) {
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (row < n) src.load(row, v);
     float e[40];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        const float x = row < n ? in[(size_t)row * stride + i] : 0.f;
+        const float x = v[i];
         const float r = ldexpf(x, 3 * g - 1);
         const float rev = r - floorf(r);
         float sv = __builtin_amdgcn_sinf(rev), cv = __builtin_amdgcn_cosf(rev);
@@ -158,12 +185,11 @@ __device__ __forceinline__ void put4(__half *o, float4 y) {
     *reinterpret_cast<uint2 *>(o) = make_uint2(a, b);
 }
 
-// n rows of in [n][6] -> out [n][40] (float or fp16) by the blocks blockIdx.x
-// of one grid row; n from n_ptr (at most max_n) when n_ptr is non-null.
-template <typename OutT>
-__device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__restrict__ in,
-                                           const uint32_t *__restrict__ n_ptr, uint32_t n_arg,
-                                           OutT *__restrict__ out) {
+// n rows of src -> out [n][40] (float or fp16) by the blocks blockIdx.x of
+// one grid row; n from n_ptr (at most max_n) when n_ptr is non-null.
+template <typename OutT, typename Src>
+__device__ __forceinline__ void infer_rows(const GuideDev &g, const Src &src, const uint32_t *__restrict__ n_ptr,
+                                           uint32_t n_arg, OutT *__restrict__ out) {
     __shared__ InferWeights W;
     const uint32_t n = n_ptr ? min(*n_ptr, n_arg) : n_arg;
     const uint32_t n_iter = (n + 31) / 32;
@@ -179,8 +205,8 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__res
         const uint32_t base = it * 32;
         bf16x8 b0[2][5], b1[2][2];
         floatx4 acc[2][4];
-        encode_b(in, 6, base + (lane & 15), n, grp, b0[0]);
-        encode_b(in, 6, base + 16 + (lane & 15), n, grp, b0[1]);
+        encode_b(src, base + (lane & 15), n, grp, b0[0]);
+        encode_b(src, base + 16 + (lane & 15), n, grp, b0[1]);
         layer_t<5, 4>(W.w0, 168, b0, acc, lane);
         relu_to_b<4>(acc[0], W.bias[0], grp, b1[0]);
         relu_to_b<4>(acc[1], W.bias[0], grp, b1[1]);
@@ -210,19 +236,21 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const float *__res
 __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_kernel(GuideDev g, const float *__restrict__ in,
                                                                        const uint32_t *__restrict__ n_ptr,
                                                                        uint32_t n_arg, float *__restrict__ out) {
-    infer_rows(g, in, n_ptr, n_arg, out);
+    infer_rows(g, RowsDirect{in, 6}, n_ptr, n_arg, out);
 }
 
 // One grid row per network k (blockIdx.y): its rows start at row k * stride
-// of in and out, their count is counts[k].
+// of slots and out, their count is counts[k]; row r's inputs are those of
+// path slot slots[r] in stage.
 __global__ __launch_bounds__(kInfWaves *kWave) void guide_infer_nets_kernel(const GuideDev *__restrict__ nets,
-                                                                            const float *__restrict__ in,
+                                                                            const float4 *__restrict__ stage,
+                                                                            const uint32_t *__restrict__ slots,
                                                                             const uint32_t *__restrict__ counts,
                                                                             uint32_t count_stride, uint32_t stride,
                                                                             __half *__restrict__ out) {
     const uint32_t k = blockIdx.y;
     const GuideDev g = nets[k];
-    infer_rows(g, in + 6 * (size_t)k * stride, counts + (size_t)k * count_stride, stride,
+    infer_rows(g, RowsGather{slots + (size_t)k * stride, stage}, counts + (size_t)k * count_stride, stride,
                out + (size_t)kOut * k * stride);
 }
 
@@ -567,8 +595,8 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         {
             bf16x8 x0[2][5], h[2][2];
             floatx4 acc[2][4];
-            encode_b(rec, stride_f, base + r16, n, grp, x0[0]);
-            encode_b(rec, stride_f, base + 16 + r16, n, grp, x0[1]);
+            encode_b(RowsDirect{rec, stride_f}, base + r16, n, grp, x0[0]);
+            encode_b(RowsDirect{rec, stride_f}, base + 16 + r16, n, grp, x0[1]);
             layer_t<5, 4>(L.w.w0, 168, x0, acc, lane);
             relu_to_b<4>(acc[0], L.w.bias[0], grp, h1[0]);
             relu_to_b<4>(acc[1], L.w.bias[0], grp, h1[1]);
@@ -876,12 +904,13 @@ hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const ui
     return hipGetLastError();
 }
 
-hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t count_stride, uint32_t stride, __half *out, hipStream_t st) {
+hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float4 *stage, const uint32_t *slots,
+                                   const uint32_t *counts, uint32_t count_stride, uint32_t stride, __half *out,
+                                   hipStream_t st) {
     if (n_nets == 0 || stride == 0) return hipSuccess;
     const uint32_t blocks = min((stride + 32 * kInfWaves - 1) / (32 * kInfWaves), kInferGrid);
-    guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, in, counts, count_stride, stride,
-                                                                                 out);
+    guide_infer_nets_kernel<<<dim3(blocks, n_nets), kInfWaves * kWave, 0, st>>>(nets, stage, slots, counts,
+                                                                                 count_stride, stride, out);
     return hipGetLastError();
 }
 

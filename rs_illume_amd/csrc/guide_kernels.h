@@ -71,10 +71,13 @@ hipError_t guide_launch_infer(const GuideDev &g, const float *in, uint32_t n, fl
 hipError_t guide_launch_infer_count(const GuideDev &g, const float *in, const uint32_t *n_ptr, uint32_t max_n,
                                     float *out, hipStream_t st);
 // Several networks at once: network k (of n_nets, device array nets) infers
-// the counts[k * count_stride] (device) rows starting at row k * stride of in
-// and out; the outputs are stored as fp16 (what the renderer's shading reads).
-hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float *in, const uint32_t *counts,
-                                   uint32_t count_stride, uint32_t stride, __half *out, hipStream_t st);
+// the counts[k * count_stride] (device) rows starting at row k * stride of
+// slots and out; row r's inputs are path slot slots[r]'s in stage (position
+// xyz in stage[2 p], normal xyz in stage[2 p + 1]). The outputs are stored as
+// fp16 (what the renderer's shading reads).
+hipError_t guide_launch_infer_nets(const GuideDev *nets, uint32_t n_nets, const float4 *stage, const uint32_t *slots,
+                                   const uint32_t *counts, uint32_t count_stride, uint32_t stride, __half *out,
+                                   hipStream_t st);
 // Training scratch (partial): one gradient row per block of the fused pass
 // (at most kTrainBlocksMax blocks), then each block's weight sum and loss.
 constexpr uint32_t kTrainBlocksMax = 256;

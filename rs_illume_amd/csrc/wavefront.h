@@ -57,7 +57,7 @@ struct WfPaths {
 struct WfState {
     WfPaths s[2];
     float4 *hit0, *hit1;     // closest hit of slot p: prim, slot, t, b0 / b1, b2, proposal row (bits)
-    float4 *gstage;          // [2 p], [2 p + 1]: slot p's proposal inputs until its row is reserved
+    float4 *gstage;          // [2 p], [2 p + 1]: slot p's proposal inputs (position, normal)
     // paths that ended this iteration, from each tile's first slot
     float4 *fin_L;           // radiance; w (bits): job index
     uint4 *fin_meta;         // as WfPaths::meta
@@ -74,7 +74,7 @@ struct WfState {
     uint32_t *tile_fin;      // per tile: paths that ended this iteration
     uint32_t *tile_free;     // per tile: free training-record slots
     uint32_t *rec_free;      // per tile: its free record slots (a stack from the tile's first slot)
-    float *guide_in;         // [n_nets][capacity][6]: network k's rows start at row k * capacity
+    uint32_t *guide_slot;    // [n_nets][capacity]: path slot of each proposal row; network k's rows start at row k * capacity
     __half *guide_out;       // [n_nets][capacity][40], fp16
     const GuideDev *nets;    // device array of the batch's networks
     uint32_t n_nets;

@@ -331,8 +331,8 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t pa
             if (guidable(m, job.gc.min_alpha) && bounces < job.max_depth) net = job.gc.net;
         }
         if (net >= 0) {
-            // the proposal's inputs, staged at the path's slot until the
-            // block has reserved its rows
+            // the proposal's inputs stay at the path's slot: its row, once
+            // the block has reserved the rows, records the slot
             Isect it;
             hit_isect(sc, h, o, d, it);
             w.gstage[2 * (size_t)p] = make_float4((it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0],
@@ -355,14 +355,7 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t pa
             [&](int k, unsigned long long first) {
                 const uint32_t p = t * S + (uint32_t)k * BLOCK + threadIdx.x;
                 const int gslot = (int)(((pend >> (5 * k)) & 31u) - 1u) * (int)w.capacity + (int)first;
-                const float4 a = w.gstage[2 * (size_t)p], b = w.gstage[2 * (size_t)p + 1];
-                float *in = w.guide_in + 6 * (size_t)gslot;
-                in[0] = a.x;
-                in[1] = a.y;
-                in[2] = a.z;
-                in[3] = b.x;
-                in[4] = b.y;
-                in[5] = b.z;
+                w.guide_slot[gslot] = p;
                 reinterpret_cast<float *>(&w.hit1[p])[2] = __int_as_float(gslot);
             });
     }
@@ -620,8 +613,8 @@ hipError_t iteration(const WfState &w, uint32_t parity, bool guided, float *film
     wf_regen_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity);
     wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, guided ? 1 : 0, cnt);
     if (guided) {
-        hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.guide_in, &w.ctr->guide_count[0][0], kCountStride,
-                                               w.capacity, w.guide_out, st);
+        hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.gstage, w.guide_slot, &w.ctr->guide_count[0][0],
+                                               kCountStride, w.capacity, w.guide_out, st);
         if (e != hipSuccess) return e;
     }
     if (guided || w.recs)

@@ -44,7 +44,7 @@ struct illume_wavefront {
     WfJob *d_jobs = nullptr;
     uint32_t jobs_capacity = 0;
     GuideDev *d_nets = nullptr;  // kMaxNets entries: the networks of the current batch
-    uint32_t nets_alloc = 1;     // networks the proposal queue (guide_in, guide_out) has room for
+    uint32_t nets_alloc = 1;     // networks the proposal queue (guide_slot, guide_out) has room for
     std::vector<void *> allocs;
     std::map<GraphKey, hipGraphExec_t> graphs;
     hipStream_t own_stream = nullptr;  // used when the caller passes the null stream (not capturable)
@@ -117,7 +117,7 @@ int illume_wavefront_create(uint32_t max_paths, uint32_t max_depth, illume_wavef
     if (e == hipSuccess) e = walloc(wf, &w.fin_nrec, P);
     for (int k = 0; k < 7; ++k)
         if (e == hipSuccess) e = walloc(wf, &w.nee[k], P);
-    if (e == hipSuccess) e = walloc(wf, &w.guide_in, 6 * P);
+    if (e == hipSuccess) e = walloc(wf, &w.guide_slot, P);
     if (e == hipSuccess) e = walloc(wf, &w.guide_out, 40 * P);
     if (e == hipSuccess) e = walloc(wf, &w.recs, P * max_depth);
     if (e == hipSuccess) e = walloc(wf, &w.rec_free, P);
@@ -169,15 +169,15 @@ uint32_t k_iters_per_replay(const illume_wavefront *wf) {
 hipError_t ensure_net_rows(illume_wavefront *wf, uint32_t n_nets) {
     if (n_nets <= wf->nets_alloc) return hipSuccess;
     WfState &w = wf->w;
-    for (void *p : {(void *)w.guide_in, (void *)w.guide_out}) {
+    for (void *p : {(void *)w.guide_slot, (void *)w.guide_out}) {
         hipFree(p);
         wf->allocs.erase(std::remove(wf->allocs.begin(), wf->allocs.end(), p), wf->allocs.end());
     }
-    w.guide_in = nullptr;
+    w.guide_slot = nullptr;
     w.guide_out = nullptr;
     wf->nets_alloc = 0;
     drop_graphs(wf);
-    hipError_t e = walloc(wf, &w.guide_in, 6 * (size_t)w.capacity * n_nets);
+    hipError_t e = walloc(wf, &w.guide_slot, (size_t)w.capacity * n_nets);
     if (e == hipSuccess) e = walloc(wf, &w.guide_out, 40 * (size_t)w.capacity * n_nets);
     if (e == hipSuccess) wf->nets_alloc = n_nets;
     return e;
