@@ -24,6 +24,12 @@ DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-pa
           "-munsafe-fp-atomics"]
 
 
+# Per-file device flags. guide.hip: MFMA results in ordinary VGPRs (the
+# heuristic put them in AGPRs, and every bias, ReLU and conversion then paid
+# an accumulator read or write).
+FILE_FLAGS = {"guide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def sources():
     out = []
     for f in sorted(os.listdir(CSRC)):
@@ -38,7 +44,7 @@ def _compile(src):
     # .cpp file its default device target (not gfx950), so a kernel defined
     # there would have no code object on the device. Host-side files must not
     # define kernels at all; check_host_sources enforces that.
-    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + ["-c", src, "-o", obj]
+    cmd = [HIPCC, "-x", "hip"] + DEVICE + FILE_FLAGS.get(os.path.basename(src), []) + COMMON + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -149,25 +149,45 @@ __device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t 
     for (int q = 0; q < 40; ++q) b[q / 8][q % 8] = to_bf16(e[q]);
 }
 
-// acc (output tiles of one 16-row column) -> bias, ReLU, bf16 -> the next
-// layer's B operand (tile m fills k-block m / 2, elements 4 (m % 2) + e).
+// acc (output tiles of one 16-row column, bias included) -> ReLU, bf16 -> the
+// next layer's B operand (tile m fills k-block m / 2, elements 4 (m % 2) + e).
+// The ReLU acts on the bf16 values as 16-bit integers (v_pk_max_i16 against
+// zero, two values per instruction): rounding keeps the sign, so it equals the
+// ReLU before rounding except that a NaN with its sign bit clear passes through.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int MT>
-__device__ __forceinline__ void relu_to_b(const floatx4 *acc, const float *bias, int g, bf16x8 *b) {
+__device__ __forceinline__ void relu_to_b(const floatx4 *acc, bf16x8 *b) {
+    static_assert(MT % 2 == 0, "whole k-blocks");
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+    for (int j = 0; j < MT / 2; ++j) {
+        u32x4 w;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const float h = acc[m][e] + bias[16 * m + 4 * g + e];
-            b[m / 2][4 * (m % 2) + e] = to_bf16(h > 0.f ? h : 0.f);
+        for (int q = 0; q < 4; ++q) {
+            // dword q of k-block j: tile 2 j + q / 2, elements 2 (q % 2) and the next
+            const int m = 2 * j + q / 2, e = 2 * (q % 2);
+            const bf16x2 p = __builtin_convertvector((float2v){acc[m][e], acc[m][e + 1]}, bf16x2);
+            const short2v r = __builtin_elementwise_max(__builtin_bit_cast(short2v, p), (short2v)0);
+            w[q] = __builtin_bit_cast(unsigned, r);
         }
+        b[j] = __builtin_bit_cast(bf16x8, w);
+    }
 }
 
+// acc[nn][m] = W^T tile m (16 outputs) x the two 16-row columns nn of b, the
+// accumulation started from the layer's bias (bias[16 m + 4 g + e] is this
+// lane's element e of tile m) or from zero when bias is null.
 template <int KB, int MT>
-__device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*b)[KB], floatx4 (*acc)[4], int lane) {
+__device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*b)[KB], floatx4 (*acc)[4], int lane,
+                                        const float *bias = nullptr) {
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-        acc[0][m] = floatx4{0.f, 0.f, 0.f, 0.f};
-        acc[1][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const floatx4 c0 = bias ? *reinterpret_cast<const floatx4 *>(bias + 16 * m + 4 * (lane >> 4))
+                                : floatx4{0.f, 0.f, 0.f, 0.f};
+        acc[0][m] = c0;
+        acc[1][m] = c0;
 #pragma unroll
         for (int j = 0; j < KB; ++j) {
             const bf16x8 a = lds_wa(w, ld, m, j, lane);
@@ -207,16 +227,16 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const Src &src, co
         floatx4 acc[2][4];
         encode_b(src, base + (lane & 15), n, grp, b0[0]);
         encode_b(src, base + 16 + (lane & 15), n, grp, b0[1]);
-        layer_t<5, 4>(W.w0, 168, b0, acc, lane);
-        relu_to_b<4>(acc[0], W.bias[0], grp, b1[0]);
-        relu_to_b<4>(acc[1], W.bias[0], grp, b1[1]);
-        layer_t<2, 4>(W.w1, 72, b1, acc, lane);
-        relu_to_b<4>(acc[0], W.bias[1], grp, b1[0]);
-        relu_to_b<4>(acc[1], W.bias[1], grp, b1[1]);
-        layer_t<2, 4>(W.w2, 72, b1, acc, lane);
-        relu_to_b<4>(acc[0], W.bias[2], grp, b1[0]);
-        relu_to_b<4>(acc[1], W.bias[2], grp, b1[1]);
-        layer_t<2, 3>(W.w3, 72, b1, acc, lane);
+        layer_t<5, 4>(W.w0, 168, b0, acc, lane, W.bias[0]);
+        relu_to_b<4>(acc[0], b1[0]);
+        relu_to_b<4>(acc[1], b1[1]);
+        layer_t<2, 4>(W.w1, 72, b1, acc, lane, W.bias[1]);
+        relu_to_b<4>(acc[0], b1[0]);
+        relu_to_b<4>(acc[1], b1[1]);
+        layer_t<2, 4>(W.w2, 72, b1, acc, lane, W.bias[2]);
+        relu_to_b<4>(acc[0], b1[0]);
+        relu_to_b<4>(acc[1], b1[1]);
+        layer_t<2, 3>(W.w3, 72, b1, acc, lane, W.bias[3]);
 #pragma unroll
         for (int nn = 0; nn < 2; ++nn) {
             const uint32_t row = base + 16 * nn + (lane & 15);
@@ -225,8 +245,7 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const Src &src, co
             for (int m = 0; m < 3; ++m) {
                 const int o = 16 * m + 4 * grp;
                 if (o >= kOut) continue;
-                const float4 y = make_float4(acc[nn][m][0] + W.bias[3][o], acc[nn][m][1] + W.bias[3][o + 1],
-                                             acc[nn][m][2] + W.bias[3][o + 2], acc[nn][m][3] + W.bias[3][o + 3]);
+                const float4 y = make_float4(acc[nn][m][0], acc[nn][m][1], acc[nn][m][2], acc[nn][m][3]);
                 put4(out + (size_t)row * kOut + o, y);
             }
         }
@@ -597,28 +616,27 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             floatx4 acc[2][4];
             encode_b(RowsDirect{rec, stride_f}, base + r16, n, grp, x0[0]);
             encode_b(RowsDirect{rec, stride_f}, base + 16 + r16, n, grp, x0[1]);
-            layer_t<5, 4>(L.w.w0, 168, x0, acc, lane);
-            relu_to_b<4>(acc[0], L.w.bias[0], grp, h1[0]);
-            relu_to_b<4>(acc[1], L.w.bias[0], grp, h1[1]);
+            layer_t<5, 4>(L.w.w0, 168, x0, acc, lane, L.w.bias[0]);
+            relu_to_b<4>(acc[0], h1[0]);
+            relu_to_b<4>(acc[1], h1[1]);
             mask1 = relu_bits(h1[0], 0) | relu_bits(h1[1], 1);
-            layer_t<2, 4>(L.w.w1, 72, h1, acc, lane);
-            relu_to_b<4>(acc[0], L.w.bias[1], grp, h2[0]);
-            relu_to_b<4>(acc[1], L.w.bias[1], grp, h2[1]);
+            layer_t<2, 4>(L.w.w1, 72, h1, acc, lane, L.w.bias[1]);
+            relu_to_b<4>(acc[0], h2[0]);
+            relu_to_b<4>(acc[1], h2[1]);
             mask2 = relu_bits(h2[0], 0) | relu_bits(h2[1], 1);
-            layer_t<2, 4>(L.w.w2, 72, h2, acc, lane);
-            relu_to_b<4>(acc[0], L.w.bias[2], grp, h[0]);
-            relu_to_b<4>(acc[1], L.w.bias[2], grp, h[1]);
+            layer_t<2, 4>(L.w.w2, 72, h2, acc, lane, L.w.bias[2]);
+            relu_to_b<4>(acc[0], h[0]);
+            relu_to_b<4>(acc[1], h[1]);
             mask3 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
             stage_units(h, S.x, lane);  // X_3
-            layer_t<2, 3>(L.w.w3, 72, h, acc, lane);
+            layer_t<2, 3>(L.w.w3, 72, h, acc, lane, L.w.bias[3]);
 #pragma unroll
             for (int nn = 0; nn < 2; ++nn)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int o = 16 * m + 4 * grp;
                     *reinterpret_cast<float4 *>(&S.yg.y[16 * nn + r16][o]) =
-                        make_float4(acc[nn][m][0] + L.w.bias[3][o], acc[nn][m][1] + L.w.bias[3][o + 1],
-                                    acc[nn][m][2] + L.w.bias[3][o + 2], acc[nn][m][3] + L.w.bias[3][o + 3]);
+                        make_float4(acc[nn][m][0], acc[nn][m][1], acc[nn][m][2], acc[nn][m][3]);
                 }
         }
         wave_lds_sync();
