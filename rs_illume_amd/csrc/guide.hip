@@ -121,10 +121,12 @@ struct RowsGather {
     }
 };
 
+// Rows past n (n >= 1) encode row n - 1's inputs: inference stores no output
+// for them and training gives them weight 0, so their gradient is zero.
 template <typename Src>
 __device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t n, int g, bf16x8 *b) {
-    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (row < n) src.load(row, v);
+    float v[6];
+    src.load(row < n ? row : n - 1, v);
     float e[40];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -177,14 +179,14 @@ __device__ __forceinline__ void relu_to_b(const floatx4 *acc, bf16x8 *b) {
 }
 
 // acc[nn][m] = W^T tile m (16 outputs) x the two 16-row columns nn of b, the
-// accumulation started from the layer's bias (bias[16 m + 4 g + e] is this
-// lane's element e of tile m) or from zero when bias is null.
-template <int KB, int MT>
+// accumulation started from the layer's bias (BIAS: bias[16 m + 4 g + e] is
+// this lane's element e of tile m) or from zero.
+template <int KB, int MT, bool BIAS = false>
 __device__ __forceinline__ void layer_t(const __bf16 *w, int ld, const bf16x8 (*b)[KB], floatx4 (*acc)[4], int lane,
                                         const float *bias = nullptr) {
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-        const floatx4 c0 = bias ? *reinterpret_cast<const floatx4 *>(bias + 16 * m + 4 * (lane >> 4))
+        const floatx4 c0 = BIAS ? *reinterpret_cast<const floatx4 *>(bias + 16 * m + 4 * (lane >> 4))
                                 : floatx4{0.f, 0.f, 0.f, 0.f};
         acc[0][m] = c0;
         acc[1][m] = c0;
@@ -227,16 +229,16 @@ __device__ __forceinline__ void infer_rows(const GuideDev &g, const Src &src, co
         floatx4 acc[2][4];
         encode_b(src, base + (lane & 15), n, grp, b0[0]);
         encode_b(src, base + 16 + (lane & 15), n, grp, b0[1]);
-        layer_t<5, 4>(W.w0, 168, b0, acc, lane, W.bias[0]);
+        layer_t<5, 4, true>(W.w0, 168, b0, acc, lane, W.bias[0]);
         relu_to_b<4>(acc[0], b1[0]);
         relu_to_b<4>(acc[1], b1[1]);
-        layer_t<2, 4>(W.w1, 72, b1, acc, lane, W.bias[1]);
+        layer_t<2, 4, true>(W.w1, 72, b1, acc, lane, W.bias[1]);
         relu_to_b<4>(acc[0], b1[0]);
         relu_to_b<4>(acc[1], b1[1]);
-        layer_t<2, 4>(W.w2, 72, b1, acc, lane, W.bias[2]);
+        layer_t<2, 4, true>(W.w2, 72, b1, acc, lane, W.bias[2]);
         relu_to_b<4>(acc[0], b1[0]);
         relu_to_b<4>(acc[1], b1[1]);
-        layer_t<2, 3>(W.w3, 72, b1, acc, lane, W.bias[3]);
+        layer_t<2, 3, true>(W.w3, 72, b1, acc, lane, W.bias[3]);
 #pragma unroll
         for (int nn = 0; nn < 2; ++nn) {
             const uint32_t row = base + 16 * nn + (lane & 15);
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             w_row[2] = r[8];
             w_row[3] = r[9];
         }
-        // ---- forward (rows past n encode zero inputs; their gradient is zero)
+        // ---- forward (rows past n repeat row n - 1 with weight 0: their gradient is zero)
         if (lane < 32) {
             const uint32_t row = base + lane;
 #pragma unroll
@@ -616,20 +618,20 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
             floatx4 acc[2][4];
             encode_b(RowsDirect{rec, stride_f}, base + r16, n, grp, x0[0]);
             encode_b(RowsDirect{rec, stride_f}, base + 16 + r16, n, grp, x0[1]);
-            layer_t<5, 4>(L.w.w0, 168, x0, acc, lane, L.w.bias[0]);
+            layer_t<5, 4, true>(L.w.w0, 168, x0, acc, lane, L.w.bias[0]);
             relu_to_b<4>(acc[0], h1[0]);
             relu_to_b<4>(acc[1], h1[1]);
             mask1 = relu_bits(h1[0], 0) | relu_bits(h1[1], 1);
-            layer_t<2, 4>(L.w.w1, 72, h1, acc, lane, L.w.bias[1]);
+            layer_t<2, 4, true>(L.w.w1, 72, h1, acc, lane, L.w.bias[1]);
             relu_to_b<4>(acc[0], h2[0]);
             relu_to_b<4>(acc[1], h2[1]);
             mask2 = relu_bits(h2[0], 0) | relu_bits(h2[1], 1);
-            layer_t<2, 4>(L.w.w2, 72, h2, acc, lane, L.w.bias[2]);
+            layer_t<2, 4, true>(L.w.w2, 72, h2, acc, lane, L.w.bias[2]);
             relu_to_b<4>(acc[0], h[0]);
             relu_to_b<4>(acc[1], h[1]);
             mask3 = relu_bits(h[0], 0) | relu_bits(h[1], 1);
             stage_units(h, S.x, lane);  // X_3
-            layer_t<2, 3>(L.w.w3, 72, h, acc, lane, L.w.bias[3]);
+            layer_t<2, 3, true>(L.w.w3, 72, h, acc, lane, L.w.bias[3]);
 #pragma unroll
             for (int nn = 0; nn < 2; ++nn)
 #pragma unroll
