@@ -90,8 +90,15 @@ struct WfState {
                              // with atomics when the path ends)
     const WfJob *jobs;
     uint32_t n_jobs;
+    uint32_t refill;         // extend refills its lanes (the batch holds a scene of >= kRefillPrims primitives)
     WfCounters *ctr;
 };
+
+// Scenes from this many primitives up trace the continuation rays with lane
+// refill: on the 64-scene batch (0.25 M triangles a scene) extend's lanes
+// were 19 of 64 active and the render 0.266 -> 0.237 s with refill, while the
+// Cornell box (12 primitives, rays of a few steps) lost 5-7% to it.
+constexpr uint32_t kRefillPrims = 1u << 12;
 
 // Resets the counters and tiles for a new batch of total_paths paths (every
 // tile empty); w.tile and w.n_tiles set for `block`.

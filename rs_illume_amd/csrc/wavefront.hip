@@ -297,9 +297,34 @@ __global__ __launch_bounds__(BLOCK) void wf_regen_kernel(WfState w, uint32_t par
     }
 }
 
-// Closest hit of every live path's ray; guided vertices get a row in their
-// network's segment of the proposal queue (one reservation per wave and
-// network).
+// The tail of both extend kernels: proposal rows for the guided vertices of
+// tile t (pend: round k's 5 bits hold 1 + the network, 0 none), one
+// reservation per network for the whole block (per-wave atomics on the
+// batch-wide row counter serialised), and the block's closest-hit count.
+template <int BLOCK>
+__device__ __forceinline__ void extend_tail(const WfState &w, uint32_t t, bool guided, uint32_t pend,
+                                            uint32_t n_closest, uint32_t *lds_red,
+                                            unsigned long long *__restrict__ counters) {
+    constexpr uint32_t S = kTileRounds * BLOCK;
+    if (guided) {
+        block_reserve<BLOCK>(
+            lds_red, [&](int k) { return (pend >> (5 * k)) & 31u; }, [&](int) { return 1u; },
+            [&](uint32_t key, uint32_t total) {
+                return (unsigned long long)atomicAdd(&w.ctr->guide_count[key - 1][0], total);
+            },
+            [&](int k, unsigned long long first) {
+                const uint32_t p = t * S + (uint32_t)k * BLOCK + threadIdx.x;
+                const int gslot = (int)(((pend >> (5 * k)) & 31u) - 1u) * (int)w.capacity + (int)first;
+                w.guide_slot[gslot] = p;
+                reinterpret_cast<float *>(&w.hit1[p])[2] = __int_as_float(gslot);
+            });
+    }
+    if (counters) block_add<BLOCK>(counters, n_closest, lds_red);
+}
+
+// Closest hit of every live path's ray, one path per thread and round;
+// guided vertices get a row in their network's segment of the proposal queue.
+// Small scenes (rays end after a few steps) take this kernel.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t parity, int guided,
                                                           unsigned long long *__restrict__ counters) {
@@ -344,22 +369,138 @@ __global__ __launch_bounds__(BLOCK) void wf_extend_kernel(WfState w, uint32_t pa
         w.hit0[p] = make_float4(__uint_as_float(found ? h.prim : 0xFFFFFFFFu), __uint_as_float(h.slot), h.t, h.b0);
         w.hit1[p] = make_float4(h.b1, h.b2, __int_as_float(gslot), 0.f);
     }
-    if (guided) {
-        // proposal rows: one reservation per network for the whole block
-        // (per-wave atomics on the batch-wide row counter serialised)
-        block_reserve<BLOCK>(
-            lds_red, [&](int k) { return (pend >> (5 * k)) & 31u; }, [&](int) { return 1u; },
-            [&](uint32_t key, uint32_t total) {
-                return (unsigned long long)atomicAdd(&w.ctr->guide_count[key - 1][0], total);
-            },
-            [&](int k, unsigned long long first) {
-                const uint32_t p = t * S + (uint32_t)k * BLOCK + threadIdx.x;
-                const int gslot = (int)(((pend >> (5 * k)) & 31u) - 1u) * (int)w.capacity + (int)first;
-                w.guide_slot[gslot] = p;
-                reinterpret_cast<float *>(&w.hit1[p])[2] = __int_as_float(gslot);
-            });
+    extend_tail<BLOCK>(w, t, guided != 0, pend, n_closest, lds_red, counters);
+}
+
+// The same for scenes large enough that rays take many traversal steps: the
+// traversal refills its lanes: a wave takes the tile's next paths (a
+// counter in LDS) into its idle lanes once at least kExtendRefill of them are
+// idle, so a lane whose ray ended early (a miss, a short path through the
+// tree) takes another instead of waiting for the wave's slowest ray. An ended
+// ray leaves only its primitive record and distance in LDS; a second pass
+// over the tile, one path per thread, forms the hit records and the guided
+// vertices' proposal inputs with every lane busy.
+constexpr int kExtendRefill = 20;
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void wf_extend_refill_kernel(WfState w, uint32_t parity, int guided,
+                                                                 unsigned long long *__restrict__ counters) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    constexpr uint32_t S = kTileRounds * BLOCK;
+    __shared__ int lds_stack[kStackLds * BLOCK];
+    __shared__ uint32_t lds_red[BLOCK / 64 + 2];
+    __shared__ uint32_t lds_slot[S];  // path i: primitive record of its closest hit, kNoSlot: none
+    __shared__ float lds_t[S];        // path i: its distance
+    __shared__ uint32_t lds_next;     // the tile's next path not yet taken
+    const uint32_t t = blockIdx.x;
+    const uint32_t n_in = w.tile_live[parity][t];
+    const WfPaths &s = w.s[parity];
+    if (threadIdx.x == 0) lds_next = 0;
+    __syncthreads();
+    const uint32_t lane = __lane_id();
+    uint32_t n_closest = 0;
+    {
+        int spill[kSpill];
+        TraceStack<BLOCK> st;
+        st.reset(lds_stack + threadIdx.x, spill);
+        uint32_t i = kNone;  // the lane's path (index in the tile), kNone: none
+        bool busy = false, more = n_in > 0;
+        const GpuNode4Q *nodes = nullptr;
+        const PrimRecord *prims = nullptr;
+        f3 o = mk(0.f, 0.f, 0.f), d = o, inv = o;
+        Shear sh{0, 0.f, 0.f, 0.f};
+        float t_max = 0.f;
+        int cur = kTraceDone;
+        uint32_t pf = 0, pn = 0, slot = kNoSlot, n_box = 0;
+        while (true) {
+            const uint64_t idle = __ballot(!busy);
+            const int n_idle = __popcll(idle);
+            if (n_idle >= kExtendRefill || n_idle == 64) {
+                if (!busy && i != kNone) {
+                    lds_slot[i] = slot;
+                    lds_t[i] = t_max;
+                    i = kNone;
+                }
+                if (more) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&lds_next, (uint32_t)n_idle);
+                    base = (uint32_t)__shfl((int)base, 0);
+                    if (base + (uint32_t)n_idle >= n_in) more = false;
+                    const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (!busy && r < n_in) {
+                        const uint32_t p = t * S + r;
+                        const float4 a = s.o_tmax[p], b = s.d_eta[p];
+                        const DevScene &sc = *w.jobs[__float_as_uint(s.L[p].w)].scene;
+                        nodes = sc.nodes;
+                        prims = sc.prims;
+                        o = mk(a.x, a.y, a.z);
+                        d = mk(b.x, b.y, b.z);
+                        t_max = a.w;
+                        inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+                        sh = make_shear(d);
+                        slot = kNoSlot;
+                        cur = 0;
+                        pf = pn = 0;
+                        st.sp = 0;
+                        i = r;
+                        busy = true;
+                        ++n_closest;
+                    }
+                }
+                if (__ballot(busy) == 0) break;  // no path left, every result in LDS
+            }
+            if (busy) {
+                if (pn != 0) {
+                    float th;
+                    if (prim_step<false>(prims, pf, o, d, sh, t_max, th)) {
+                        slot = pf;
+                        t_max = th;
+                    }
+                    ++pf;
+                    --pn;
+                } else {
+                    cur = node_step4<false, BLOCK>(nodes, cur, o, inv, t_max, st, n_box);
+                }
+                take_leaf(cur, pf, pn, st);
+                if (pn == 0 && cur == kTraceDone) busy = false;
+            }
+        }
     }
-    if (counters) block_add<BLOCK>(counters, n_closest, lds_red);
+    __syncthreads();  // every path's result in LDS
+    uint32_t pend = 0;  // round k: 1 + the network that takes the vertex's proposal row (5 bits), 0: none
+    for (uint32_t k = 0, i = threadIdx.x; i < n_in; ++k, i += BLOCK) {
+        const uint32_t p = t * S + i;
+        const uint32_t slot = lds_slot[i];
+        TraceHit h;
+        h.prim = 0xFFFFFFFFu;
+        h.slot = 0;
+        h.t = h.b0 = h.b1 = h.b2 = 0.f;
+        const bool found = slot != kNoSlot;
+        if (found) {
+            const float4 a = s.o_tmax[p], b = s.d_eta[p];
+            const WfJob &job = w.jobs[__float_as_uint(s.L[p].w)];
+            const DevScene &sc = *job.scene;
+            const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+            finish_hit(sc.prims, slot, o, make_shear(d), lds_t[i], h);
+            if (guided && job.gc.out && job.gc.net >= 0) {
+                const DevMaterial &m = sc.materials[sc.prim_material[h.prim]];
+                const uint32_t bounces = __float_as_uint(s.beta[p].w) & 0xFFu;
+                if (guidable(m, job.gc.min_alpha) && bounces < job.max_depth) {
+                    // the proposal's inputs stay at the path's slot: its row,
+                    // once the block has reserved the rows, records the slot
+                    Isect it;
+                    hit_isect(sc, h, o, d, it);
+                    w.gstage[2 * (size_t)p] = make_float4((it.p.x - job.gc.bounds_lo[0]) * job.gc.inv_ext[0],
+                                                          (it.p.y - job.gc.bounds_lo[1]) * job.gc.inv_ext[1],
+                                                          (it.p.z - job.gc.bounds_lo[2]) * job.gc.inv_ext[2], 0.f);
+                    w.gstage[2 * (size_t)p + 1] = make_float4(it.n.x, it.n.y, it.n.z, 0.f);
+                    pend |= (uint32_t)(job.gc.net + 1) << (5 * k);
+                }
+            }
+        }
+        w.hit0[p] = make_float4(__uint_as_float(h.prim), __uint_as_float(h.slot), h.t, h.b0);
+        w.hit1[p] = make_float4(h.b1, h.b2, __int_as_float(-1), 0.f);
+    }
+    extend_tail<BLOCK>(w, t, guided != 0, pend, n_closest, lds_red, counters);
 }
 
 // One bounce of tile t's live paths, tracing nothing. GUIDED (a proposal or
@@ -611,7 +752,10 @@ hipError_t iteration(const WfState &w, uint32_t parity, bool guided, float *film
                      hipStream_t st) {
     const uint32_t grid = w.n_tiles;
     wf_regen_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity);
-    wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, guided ? 1 : 0, cnt);
+    if (w.refill)
+        wf_extend_refill_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, guided ? 1 : 0, cnt);
+    else
+        wf_extend_kernel<BLOCK><<<grid, BLOCK, 0, st>>>(w, parity, guided ? 1 : 0, cnt);
     if (guided) {
         hipError_t e = guide_launch_infer_nets(w.nets, w.n_nets, w.gstage, w.guide_slot, &w.ctr->guide_count[0][0],
                                                kCountStride, w.capacity, w.guide_out, st);

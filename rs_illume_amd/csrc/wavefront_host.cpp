@@ -220,6 +220,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     hipStream_t st = caller ? caller : wf->own_stream;
     std::vector<WfJob> hj;
     unsigned long long total = 0, n_pix_all = 0;
+    uint32_t max_prims = 0;
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const illume_job &in = jobs[j];
         if (!in.scene) return abi_fail(ILLUME_ERR_INVALID_ARG, "job without a scene");
@@ -247,6 +248,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
         }
         const unsigned long long n = scene_prims(in.scene) ? n_pix * in.cfg.spp : 0ull;
         if (n == 0) continue;
+        max_prims = std::max(max_prims, scene_prims(in.scene));
         if (kind && !sampler_launch(kind, x0, y0, sw, sh, &job.sl))
             return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "low-discrepancy tables");
         job.scene = scene_dev_device(in.scene);
@@ -319,6 +321,7 @@ int render_batch(illume_wavefront *wf, const illume_job *jobs, const illume_filt
     WfState w = wf->w;
     w.jobs = wf->d_jobs;
     w.n_jobs = (uint32_t)hj.size();
+    w.refill = max_prims >= kRefillPrims ? 1u : 0u;
     if (!train) w.recs = nullptr;
     w.nets = wf->d_nets;
     w.n_nets = (uint32_t)nets.size();
