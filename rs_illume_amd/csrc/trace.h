@@ -202,6 +202,29 @@ __device__ __forceinline__ bool qslab(const uint4 &w1, const uint4 &w2, float ox
     return slab(lx, hx, ly, hy, lz, hz, o, inv, t_max, t);
 }
 
+// The same test with each axis's near and far plane already chosen (near.x,
+// far.x: the bytes of x's entry and exit planes, likewise y, z): an entry is
+// (near - o) inv and an exit (far - o) inv, the values the min and max of
+// qslab pick whenever neither is NaN. A NaN (0 * inf: the origin on a plane
+// the ray runs along) drops out of the entry maximum and the exit minimum,
+// so such a box is entered whenever qslab enters it.
+template <int C>
+__device__ __forceinline__ bool qslab_nf(const uint4 &near, const uint4 &far, float ox, float oy, float oz, float sx,
+                                         float sy, float sz, f3 o, f3 inv, float t_max, float &t) {
+    constexpr int sh = 8 * C;
+    constexpr float k = 1.0f + 2.0f * gamma_f(3);
+    const float ax = (fmaf((float)((near.x >> sh) & 0xffu), sx, ox) - o.x) * inv.x;
+    const float bx = (fmaf((float)((far.x >> sh) & 0xffu), sx, ox) - o.x) * inv.x;
+    const float ay = (fmaf((float)((near.y >> sh) & 0xffu), sy, oy) - o.y) * inv.y;
+    const float by = (fmaf((float)((far.y >> sh) & 0xffu), sy, oy) - o.y) * inv.y;
+    const float az = (fmaf((float)((near.z >> sh) & 0xffu), sz, oz) - o.z) * inv.z;
+    const float bz = (fmaf((float)((far.z >> sh) & 0xffu), sz, oz) - o.z) * inv.z;
+    const float t0 = fmaxf(fmaxf(ax, ay), az);
+    const float t1 = fminf(fminf(bx, by), bz) * k;
+    t = t0;
+    return t0 <= t1 && t0 < t_max && t1 > 0.f;
+}
+
 // One visit of a four-child node: tests the four dequantized child boxes,
 // sorts the entered children by entry distance, pushes all but the nearest
 // (farthest first) and returns the nearest; with none entered, returns the
@@ -217,11 +240,17 @@ __device__ __forceinline__ int node_step4(const GpuNode4Q *__restrict__ nodes, i
     const float sx = __uint_as_float((w0.w & 0xffu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
                 sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
     const int4 ch = make_int4((int)w2.z, (int)w2.w, (int)w3.x, (int)w3.y);
+    // the near and far plane bytes of each axis, chosen once per visit by the
+    // direction's sign (BVHAccel's dir_is_neg) instead of a min and a max per
+    // axis and child
+    const bool nx = __float_as_uint(inv.x) >> 31, ny = __float_as_uint(inv.y) >> 31, nz = __float_as_uint(inv.z) >> 31;
+    const uint4 near = make_uint4(nx ? w1.y : w1.x, ny ? w1.w : w1.z, nz ? w2.y : w2.x, 0u);
+    const uint4 far = make_uint4(nx ? w1.x : w1.y, ny ? w1.z : w1.w, nz ? w2.x : w2.y, 0u);
     float t0, t1, t2, t3;
-    const bool h0 = qslab<0>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t0) && ch.x != kEmptyChild;
-    const bool h1 = qslab<1>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t1) && ch.y != kEmptyChild;
-    const bool h2 = qslab<2>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t2) && ch.z != kEmptyChild;
-    const bool h3 = qslab<3>(w1, w2, ox, oy, oz, sx, sy, sz, o, inv, t_max, t3) && ch.w != kEmptyChild;
+    const bool h0 = qslab_nf<0>(near, far, ox, oy, oz, sx, sy, sz, o, inv, t_max, t0) && ch.x != kEmptyChild;
+    const bool h1 = qslab_nf<1>(near, far, ox, oy, oz, sx, sy, sz, o, inv, t_max, t1) && ch.y != kEmptyChild;
+    const bool h2 = qslab_nf<2>(near, far, ox, oy, oz, sx, sy, sz, o, inv, t_max, t2) && ch.z != kEmptyChild;
+    const bool h3 = qslab_nf<3>(near, far, ox, oy, oz, sx, sy, sz, o, inv, t_max, t3) && ch.w != kEmptyChild;
     float ta = h0 ? t0 : inf, tb = h1 ? t1 : inf, tc = h2 ? t2 : inf, td = h3 ? t3 : inf;
     int ra = ch.x, rb = ch.y, rc = ch.z, rd = ch.w;
     const int n_hit = (int)h0 + (int)h1 + (int)h2 + (int)h3;
