@@ -14,6 +14,7 @@ namespace illume {
 int abi_fail(int status, const std::string &msg);
 int abi_hip_fail(hipError_t e, const char *what);
 void bvh_device_arrays(const illume_bvh *bvh, const GpuNode4Q **nodes, const PrimRecord **prims);
+hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv);
 }  // namespace illume
 
 using namespace illume;
@@ -275,12 +276,6 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     s->n_prims = (uint32_t)np;
     bvh_device_arrays(s->bvh, &s->dev.nodes, &s->dev.prims);
 
-    std::vector<float4> tv(3 * (size_t)n_tris);
-    for (uint32_t t = 0; t < n_tris; ++t)
-        for (int k = 0; k < 3; ++k) {
-            const float *v = verts + 3 * (size_t)tri_idx[3 * (size_t)t + k];
-            tv[3 * (size_t)t + k] = make_float4(v[0], v[1], v[2], 0.f);
-        }
     std::vector<float4> sv(n_spheres);
     for (uint32_t i = 0; i < n_spheres; ++i)
         sv[i] = make_float4(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2], spheres[4 * i + 3]);
@@ -295,7 +290,10 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
         if (e[0] == 0.f && e[1] == 0.f && e[2] == 0.f) continue;
         DevLight l;
         std::memset(&l, 0, sizeof(l));
-        const float4 &a = tv[3 * (size_t)t], &b = tv[3 * (size_t)t + 1], &c = tv[3 * (size_t)t + 2];
+        const float *pa = verts + 3 * (size_t)tri_idx[3 * (size_t)t], *pb = verts + 3 * (size_t)tri_idx[3 * (size_t)t + 1],
+                    *pc = verts + 3 * (size_t)tri_idx[3 * (size_t)t + 2];
+        const float4 a = make_float4(pa[0], pa[1], pa[2], 0.f), b = make_float4(pb[0], pb[1], pb[2], 0.f),
+                     c = make_float4(pc[0], pc[1], pc[2], 0.f);
         l.p0[0] = a.x; l.p0[1] = a.y; l.p0[2] = a.z;
         l.p1[0] = b.x; l.p1[1] = b.y; l.p1[2] = b.z;
         l.p2[0] = c.x; l.p2[1] = c.y; l.p2[2] = c.z;
@@ -355,7 +353,14 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     DevMaterial *d_m = nullptr;
     DevLight *d_l = nullptr;
     float *d_f = nullptr, *d_c = nullptr;
-    if ((e = upload(&d_tv, tv)) == hipSuccess) s->allocs.push_back(d_tv);
+    // the triangles' vertex table, built on the device from the tree's
+    // primitive records (the same vertices, in mesh index order)
+    if (n_tris) {
+        if ((e = hipMalloc(&d_tv, 3 * (size_t)n_tris * sizeof(float4))) == hipSuccess) {
+            s->allocs.push_back(d_tv);
+            e = scene_tri_verts(s->dev.prims, (uint32_t)np, n_tris, d_tv);
+        }
+    }
     if (e == hipSuccess && (e = upload(&d_sv, sv)) == hipSuccess) s->allocs.push_back(d_sv);
     if (e == hipSuccess && (e = upload(&d_pm, pm)) == hipSuccess) s->allocs.push_back(d_pm);
     if (e == hipSuccess && (e = upload(&d_pl, pl)) == hipSuccess) s->allocs.push_back(d_pl);

@@ -49,6 +49,38 @@ __global__ __launch_bounds__(256) void shape_kernel(PrimRecord rec, const illume
 }  // namespace
 }  // namespace illume
 
+namespace illume {
+namespace {
+// Triangle t's vertices (tri_verts[3 t + k], w = 0) from the primitive
+// records, which hold them in mesh index order with t as the first record's
+// id; spheres' records are skipped.
+__global__ __launch_bounds__(256) void tri_verts_kernel(const PrimRecord *__restrict__ prims, uint32_t n_records,
+                                                        uint32_t n_tris, float4 *__restrict__ tv) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_records) return;
+    const float4 *pp = reinterpret_cast<const float4 *>(prims + s);
+    const float4 a = pp[0], b = pp[1], c = pp[2];
+    const uint32_t id = __float_as_uint(a.w);
+    if ((id & kSphereBit) || id >= n_tris) return;
+    tv[3 * (size_t)id] = make_float4(a.x, a.y, a.z, 0.f);
+    tv[3 * (size_t)id + 1] = make_float4(b.x, b.y, b.z, 0.f);
+    tv[3 * (size_t)id + 2] = make_float4(c.x, c.y, c.z, 0.f);
+}
+}  // namespace
+
+// A scene's per-triangle vertex table built on the device from its BVH's
+// primitive records (no host copy or upload of the 48 bytes per triangle);
+// entries of triangles without a record stay zero. Synchronous.
+hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv) {
+    hipError_t e = hipMemset(tv, 0, 3 * (size_t)n_tris * sizeof(float4));
+    if (e != hipSuccess || n_records == 0) return e;
+    tri_verts_kernel<<<(n_records + 255) / 256, 256>>>(prims, n_records, n_tris, tv);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
+}  // namespace illume
+
 extern "C" int illume_shape_intersect_batch(int32_t kind, const float *shape, const illume_ray *rays, uint32_t n,
                                             int32_t any, illume_hit *hits, void *stream) {
     using namespace illume;
