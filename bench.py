@@ -340,8 +340,13 @@ def bench_batched(no_cpu, log=log, builder="SAH_GPU", pmc=None):
     t0 = time.time()
     descs = [S.procedural_scene(seed) for seed in range(1, 65)]
     gen_s = time.time() - t0
+    # scene set-up from 8 host threads (each scene's tree is built on a stream
+    # of its own, so the device builds overlap; the reference builds its BVHs
+    # with rayon)
+    from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
-    scenes = [Scene(d, builder=builder) for d, _, _ in descs]
+    with ThreadPoolExecutor(8) as ex:
+        scenes = list(ex.map(lambda d: Scene(d[0], builder=builder), descs))
     setup_s = time.time() - t0
     bvh_s = sum(sc.bvh_info()["build_seconds"] for sc in scenes)
     jobs = [(sc, cam, spp, 8, seed) for seed, (sc, (_, cam, spp)) in enumerate(zip(scenes, descs), 1)]
@@ -400,10 +405,12 @@ def bench_batched(no_cpu, log=log, builder="SAH_GPU", pmc=None):
     out = {"workload": "64 procedural scenes (seeds 1-64): displaced icospheres + ground quad, Lambertian/dielectric, "
                        "1-4 quad lights, 64^2..1024^2, 8-64 spp, max depth 8, one batched render call",
            "scenes": len(jobs), "triangles": n_tris, "pixels": n_px, "paths": n_paths, "bvh_builder": builder,
-           "scene_setup_s": round(setup_s, 2), "bvh_build_s": round(bvh_s, 3),
+           "scene_setup_s": round(setup_s, 2), "bvh_build_sum_s": round(bvh_s, 3),
+           "bvh_build_sum_basis": "the 64 builds' own durations added up (they overlap on 8 host threads)",
            "end_to_end_s": round(setup_s + top["seconds"], 3),
-           "end_to_end_basis": "scene set-up (scene tables, device BVH builds) + the batched render; the numpy "
-                               "scene generation and the autotuner's probe excluded", "renderer": fastest,
+           "end_to_end_basis": "scene set-up from 8 host threads (scene tables, device BVH builds on a stream per "
+                               "scene) + the batched render; the numpy scene generation and the autotuner's probe "
+                               "excluded", "renderer": fastest,
            "seconds": top["seconds"], "samples_per_s": top["samples_per_s"], "mrays_per_s": top["mrays_per_s"],
            "wavefront": wave, "megakernel_batch": mega}
     if not no_cpu:

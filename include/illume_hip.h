@@ -14,7 +14,10 @@
  * own stream. A guide handle is read-only for rendering and inference but
  * written by its training calls; a wavefront handle holds the per-render path
  * pool and is used by one call at a time (one stream, one host thread). The
- * last-error message is kept per host thread.
+ * last-error message is kept per host thread. Scenes may be created from
+ * several host threads at once: each scene's set-up (its device tree build,
+ * vertex table and uploads) runs on a stream of its own, so builds of
+ * different scenes overlap on the device.
  */
 #ifndef ILLUME_HIP_H
 #define ILLUME_HIP_H

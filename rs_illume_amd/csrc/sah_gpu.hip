@@ -967,14 +967,19 @@ __global__ void collapse_init_kernel(uint2 *items, Ctr *ctr) {
     *ctr = c;
 }
 
+// The build's temporaries: stream-ordered allocations on the build's stream,
+// freed in stream order (hipFree would wait for the whole device, so builds
+// of several scenes on their own streams could not overlap).
 struct DevBuf {
+    hipStream_t st;
     std::vector<void *> ptrs;
+    explicit DevBuf(hipStream_t s) : st(s) {}
     ~DevBuf() {
-        for (void *p : ptrs) (void)hipFree(p);
+        for (void *p : ptrs) (void)hipFreeAsync(p, st);
     }
     template <typename T>
     hipError_t alloc(T **p, size_t count) {
-        hipError_t e = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(T));
+        hipError_t e = hipMallocAsync((void **)p, std::max<size_t>(count, 1) * sizeof(T), st);
         if (e == hipSuccess) ptrs.push_back(*p);
         return e;
     }
@@ -989,7 +994,7 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
                          SahGpuResult *res) {
     const uint32_t n = n_tris + n_spheres;
     *res = SahGpuResult();
-    DevBuf tmp;
+    DevBuf tmp(st);
     float *d_verts = nullptr, *d_sph = nullptr;
     uint32_t *d_idx = nullptr;
     hipError_t e = hipSuccess;
@@ -1025,19 +1030,13 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
     // BNodes: the root, then two per top-phase split; grown on demand
     size_t bnode_cap = 1 + 2 * (size_t)front_cap * 4;
     BNode *bnodes = nullptr;
-    if ((e = hipMalloc(&bnodes, bnode_cap * sizeof(BNode))) != hipSuccess) return e;
+    if ((e = hipMallocAsync((void **)&bnodes, bnode_cap * sizeof(BNode), st)) != hipSuccess) return e;
     struct BNodeGuard {
         BNode **p;
-        ~BNodeGuard() { (void)hipFree(*p); }
-    } bguard{&bnodes};
-    if ((e = hipMalloc(&n2, (size_t)std::max<uint32_t>(n - 1, 1) * sizeof(GpuNode))) != hipSuccess) return e;
-    struct NodeGuard {
-        GpuNode **p;
-        bool keep = false;
-        ~NodeGuard() {
-            if (!keep) (void)hipFree(*p);
-        }
-    } nguard{&n2};
+        hipStream_t st;
+        ~BNodeGuard() { (void)hipFreeAsync(*p, st); }
+    } bguard{&bnodes, st};
+    if ((e = tmp.alloc(&n2, (size_t)std::max<uint32_t>(n - 1, 1))) != hipSuccess) return e;
     if ((e = hipMemsetAsync(n2, 0, (size_t)std::max<uint32_t>(n - 1, 1) * sizeof(GpuNode), st)) != hipSuccess) return e;
     size_t scan_bytes = 0, cscan_bytes = 0;
     if ((e = rocprim::exclusive_scan(nullptr, scan_bytes, flags, excl, 0u, n, rocprim::plus<uint32_t>(), st)) !=
@@ -1063,13 +1062,12 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
         if (hc.bnodes + 2 * (size_t)nf > bnode_cap) {
             const size_t cap2 = std::max(bnode_cap * 2, hc.bnodes + 2 * (size_t)nf);
             BNode *b2 = nullptr;
-            if ((e = hipMalloc(&b2, cap2 * sizeof(BNode))) != hipSuccess) return e;
-            if ((e = hipMemcpyAsync(b2, bnodes, hc.bnodes * sizeof(BNode), hipMemcpyDeviceToDevice, st)) != hipSuccess ||
-                (e = hipStreamSynchronize(st)) != hipSuccess) {
-                (void)hipFree(b2);
+            if ((e = hipMallocAsync((void **)&b2, cap2 * sizeof(BNode), st)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(b2, bnodes, hc.bnodes * sizeof(BNode), hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+                (void)hipFreeAsync(b2, st);
                 return e;
             }
-            (void)hipFree(bnodes);
+            (void)hipFreeAsync(bnodes, st);  // after the copy, in stream order
             bnodes = b2;
             bnode_cap = cap2;
         }
@@ -1111,7 +1109,8 @@ hipError_t sah_gpu_build(const float *verts, uint32_t n_verts, const uint32_t *i
     // the root's box
     if (e == hipSuccess) {
         uint32_t rb[6];
-        e = hipMemcpy(rb, bnodes, sizeof(rb), hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(rb, bnodes, sizeof(rb), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         for (int a = 0; a < 6; ++a) {
             const uint32_t o = rb[a];
             const uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;

@@ -14,7 +14,7 @@ namespace illume {
 int abi_fail(int status, const std::string &msg);
 int abi_hip_fail(hipError_t e, const char *what);
 void bvh_device_arrays(const illume_bvh *bvh, const GpuNode4Q **nodes, const PrimRecord **prims);
-hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv);
+hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv, hipStream_t st);
 }  // namespace illume
 
 using namespace illume;
@@ -170,14 +170,31 @@ bool make_camera(const illume_camera &c, DevCamera *out) {
     return true;
 }
 
+// Queues the copy on st: src must outlive it (the caller synchronises st).
 template <typename T>
-hipError_t upload(T **dst, const std::vector<T> &src) {
+hipError_t upload(T **dst, const std::vector<T> &src, hipStream_t st) {
     *dst = nullptr;
     if (src.empty()) return hipSuccess;
     hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
     if (e != hipSuccess) return e;
-    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    return hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
 }
+
+// A stream of the caller's own for one scene's set-up (default priority, not
+// synchronising with the null stream), so that scenes created from several
+// host threads build their trees side by side.
+struct SetupStream {
+    hipStream_t st = nullptr;
+    SetupStream() {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+    }
+    ~SetupStream() {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+};
 
 }  // namespace
 
@@ -261,13 +278,15 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
         if (e[0] != 0.f || e[1] != 0.f || e[2] != 0.f)
             return abi_fail(ILLUME_ERR_UNSUPPORTED, "only triangles may emit");
     }
+    SetupStream ss;
+    if (!ss.st) return abi_fail(ILLUME_ERR_HIP, "stream creation failed");
     illume_scene *s = new (std::nothrow) illume_scene();
     if (!s) return abi_fail(ILLUME_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(&s->dev, 0, sizeof(s->dev));
-    int st = builder == 1   ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, nullptr,
+    int st = builder == 1   ? illume_bvh_create_gpu(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, ss.st,
                                                     &s->bvh)
              : builder == 2 ? illume_bvh_create_gpu_sah(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4,
-                                                        nullptr, &s->bvh)
+                                                        ss.st, &s->bvh)
                             : illume_bvh_create_ex(verts, n_verts, tri_idx, n_tris, spheres, n_spheres, 4, 4, &s->bvh);
     if (st != ILLUME_OK) {
         delete s;
@@ -358,16 +377,19 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     if (n_tris) {
         if ((e = hipMalloc(&d_tv, 3 * (size_t)n_tris * sizeof(float4))) == hipSuccess) {
             s->allocs.push_back(d_tv);
-            e = scene_tri_verts(s->dev.prims, (uint32_t)np, n_tris, d_tv);
+            e = scene_tri_verts(s->dev.prims, (uint32_t)np, n_tris, d_tv, ss.st);
         }
     }
-    if (e == hipSuccess && (e = upload(&d_sv, sv)) == hipSuccess) s->allocs.push_back(d_sv);
-    if (e == hipSuccess && (e = upload(&d_pm, pm)) == hipSuccess) s->allocs.push_back(d_pm);
-    if (e == hipSuccess && (e = upload(&d_pl, pl)) == hipSuccess) s->allocs.push_back(d_pl);
-    if (e == hipSuccess && (e = upload(&d_m, mats)) == hipSuccess) s->allocs.push_back(d_m);
-    if (e == hipSuccess && (e = upload(&d_l, lights)) == hipSuccess) s->allocs.push_back(d_l);
-    if (e == hipSuccess && (e = upload(&d_f, power)) == hipSuccess) s->allocs.push_back(d_f);
-    if (e == hipSuccess && (e = upload(&d_c, cdf)) == hipSuccess) s->allocs.push_back(d_c);
+    if (e == hipSuccess && (e = upload(&d_sv, sv, ss.st)) == hipSuccess) s->allocs.push_back(d_sv);
+    if (e == hipSuccess && (e = upload(&d_pm, pm, ss.st)) == hipSuccess) s->allocs.push_back(d_pm);
+    if (e == hipSuccess && (e = upload(&d_pl, pl, ss.st)) == hipSuccess) s->allocs.push_back(d_pl);
+    if (e == hipSuccess && (e = upload(&d_m, mats, ss.st)) == hipSuccess) s->allocs.push_back(d_m);
+    if (e == hipSuccess && (e = upload(&d_l, lights, ss.st)) == hipSuccess) s->allocs.push_back(d_l);
+    if (e == hipSuccess && (e = upload(&d_f, power, ss.st)) == hipSuccess) s->allocs.push_back(d_f);
+    if (e == hipSuccess && (e = upload(&d_c, cdf, ss.st)) == hipSuccess) s->allocs.push_back(d_c);
+    // the uploads read the host vectors above: wait for them here
+    const hipError_t es = hipStreamSynchronize(ss.st);
+    if (e == hipSuccess) e = es;
     if (e != hipSuccess) {
         illume_scene_destroy(s);
         return abi_hip_fail(e, "scene upload");
@@ -392,7 +414,8 @@ int illume_scene_create_ex(const float *verts, uint32_t n_verts, const uint32_t 
     e = hipMalloc(&s->d_dev, sizeof(DevScene));
     if (e == hipSuccess) {
         s->allocs.push_back(s->d_dev);
-        e = hipMemcpy(s->d_dev, &s->dev, sizeof(DevScene), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(s->d_dev, &s->dev, sizeof(DevScene), hipMemcpyHostToDevice, ss.st);
+        if (e == hipSuccess) e = hipStreamSynchronize(ss.st);
     }
     if (e != hipSuccess) {
         illume_scene_destroy(s);

@@ -70,14 +70,12 @@ __global__ __launch_bounds__(256) void tri_verts_kernel(const PrimRecord *__rest
 
 // A scene's per-triangle vertex table built on the device from its BVH's
 // primitive records (no host copy or upload of the 48 bytes per triangle);
-// entries of triangles without a record stay zero. Synchronous.
-hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv) {
-    hipError_t e = hipMemset(tv, 0, 3 * (size_t)n_tris * sizeof(float4));
+// entries of triangles without a record stay zero. Queued on st.
+hipError_t scene_tri_verts(const PrimRecord *prims, uint32_t n_records, uint32_t n_tris, float4 *tv, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(tv, 0, 3 * (size_t)n_tris * sizeof(float4), st);
     if (e != hipSuccess || n_records == 0) return e;
-    tri_verts_kernel<<<(n_records + 255) / 256, 256>>>(prims, n_records, n_tris, tv);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    return e;
+    tri_verts_kernel<<<(n_records + 255) / 256, 256, 0, st>>>(prims, n_records, n_tris, tv);
+    return hipGetLastError();
 }
 }  // namespace illume
 

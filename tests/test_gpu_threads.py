@@ -91,3 +91,41 @@ def test_two_threads_two_streams_one_scene(gpu):
     # the two seeds render different images: the threads did not share a buffer
     assert not np.allclose(refs[0].cpu().numpy(), refs[1].cpu().numpy())
     sc.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("builder", ["SAH_GPU", "LBVH", "SAH"])
+def test_scenes_created_from_four_threads(gpu, builder):
+    """Scenes created from several host threads at once (each set-up on a
+    stream of its own, device builds side by side) equal the same scenes
+    created one after another: tree statistics and renders."""
+    import torch
+    from rs_illume_amd import scenes as S
+    from rs_illume_amd.render import PathIntegrator, Scene
+    descs = [S.procedural_scene(seed) for seed in (3, 7, 11, 19, 23, 29, 31, 37)]
+    print("threads: 8 procedural scenes (%s) from 4 threads" % builder, flush=True)
+
+    def stats(sc):
+        i = sc.bvh_info()
+        return (i["n_prims"], i["n_inner_nodes"], i["n_leaves"], i["max_depth"])
+
+    seq = [Scene(d, builder=builder) for d, _, _ in descs]
+    seq_stats = [stats(sc) for sc in seq]
+    made = [None] * len(descs)
+
+    def work(k):
+        for q in range(k, len(descs), 4):
+            made[q] = Scene(descs[q][0], builder=builder)
+        return 0
+
+    assert _run_threads(work, n_threads=4) == [0, 0, 0, 0]
+    assert [stats(sc) for sc in made] == seq_stats
+    for q in (0, 5):
+        _, cam, _ = descs[q]
+        p = PathIntegrator(cam, spp=2, max_depth=5, seed=7)
+        a = p.render_sum(seq[q]).cpu().numpy()
+        b = p.render_sum(made[q]).cpu().numpy()
+        # float atomics sum each pixel's samples in any order
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-5)
+    for sc in seq + made:
+        sc.close()
