@@ -451,9 +451,10 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
                    builder="SAH_GPU", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
     """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
-    Plain (megakernel batch) and guided (wavefront pool; a proposal network
-    per view, trained online on its view's samples, and one network shared by
-    all views reported beside it) run progressive passes
+    Plain (on the megakernel batch and on the wavefront pool; the faster is
+    the baseline) and guided (wavefront pool; a proposal network per view,
+    trained online on its view's samples, and one network shared by all views
+    reported beside it) run progressive passes
     of all views; a view's time-to-relMSE is the batch time at which its image
     first reaches the target against a ref_spp plain reference. The CPU leg
     renders view 0 at its native resolution with the oracle on host cores in
@@ -488,17 +489,20 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
     log("combined: %.1fM tris, %d spheres, %d views, %.2f Mpx; reference %d spp in %.1f s"
         % (len(desc.idx) / 1e6, len(desc.spheres), len(cams), n_px / 1e6, ref_spp, ref_s))
 
-    def run(guide=None, guides=None):
-        # guide: one network for every view; guides: one network per view
+    def run(guide=None, guides=None, plain_wavefront=False):
+        # guide: one network for every view; guides: one network per view;
+        # neither: plain, on the megakernel batch or (plain_wavefront) the pool
         guided_run = guide is not None or guides is not None
+        on_pool = guided_run or plain_wavefront
         film = torch.zeros_like(ref)
         gc = GuideCfg(guide_frac, train_steps, 65536, 0.0, GUIDE_PRODUCT) if guided_run else None
         warm = guided_run and warmup_steps > 0
         reached = [None] * len(cams)
         elapsed, spp = 0.0, 0
-        # plain passes on the megakernel batch (the faster plain renderer), guided ones on the wavefront pool
-        if not guided_run:
+        if not on_pool:
             render_batch(jobs(1, 1 << 23, 9000))
+        elif not guided_run:
+            wf.render_batch(jobs(1, 1 << 23, 9000))
         else:
             wf.render_batch(jobs(1, 1 << 23, 9000), guide=Guide(seed=99), gcfg=gc)
 
@@ -511,8 +515,10 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
             t1 = time.perf_counter()
             if warm:
                 gc.guide_frac = 0.0
-            if not guided_run:
+            if not on_pool:
                 render_batch(jobs(spp_pass, spp, 100), film=film)
+            elif not guided_run:
+                wf.render_batch(jobs(spp_pass, spp, 100), film=film)
             elif guide is not None:
                 wf.render_batch(jobs(spp_pass, spp, 200), film=film, guide=guide, gcfg=gc)
             else:
@@ -535,7 +541,11 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
                 "time_to_target_per_view": [r[0] if r else None for r in reached],
                 "spp_to_target_per_view": [r[1] if r else None for r in reached]}
 
-    plain = run()
+    # plain on both renderers; the faster is the baseline
+    plain_mk = run()
+    plain_wf = run(plain_wavefront=True)
+    plain_on = "megakernel_batch" if plain_mk["seconds"] <= plain_wf["seconds"] else "wavefront"
+    plain = plain_mk if plain_on == "megakernel_batch" else plain_wf
     # a proposal network per view, as configured (each trained on its view's
     # samples), and one network shared by all views beside it
     guided = run(guides=[Guide(seed=70 + v) for v in range(len(cams))])
@@ -564,7 +574,8 @@ def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, 
                         "256^2..1920x1080 (%.2f Mpx), max depth 10, %d spp passes of all views per render_batch"
                         % (n_px / 1e6, spp_pass),
             "bvh_builder": builder, "scene_setup_s": round(setup_s, 2), "reference_spp": ref_spp,
-            "plain_renderer": "megakernel_batch", "guided_renderer": "wavefront",
+            "plain_renderer": plain_on, "guided_renderer": "wavefront",
+            "plain_megakernel_batch": plain_mk, "plain_wavefront": plain_wf,
             "reference_s": round(ref_s, 2),
             "target_rel_mse": target, "train_steps": train_steps, "warmup_steps": warmup_steps,
             "guide_frac": guide_frac, "product_with_bsdf_lobe": True,
