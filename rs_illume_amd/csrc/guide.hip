@@ -57,9 +57,10 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 // Frequency encoding: input i (position xyz in [0,1], normal xyz), octave j:
 // sin(2^j pi v) at feature 24 i + 2 j, cos at 24 i + 2 j + 1 (layer 0's
 // 160 inputs hold 16 zeros besides). The angle is reduced exactly (2^j pi v =
-// 2 pi fract(2^(j-1) v): scaling by a power of two and taking the fraction
-// are exact) and the hardware v_sin/v_cos, which take their argument in
-// revolutions, do the rest.
+// 2 pi fract(2^(j-1) v): scaling by a power of two and taking the fraction,
+// v_fract_f32, are exact, except that the fraction of a tiny negative number
+// is capped just below 1 rather than rounded to it) and the hardware
+// v_sin/v_cos, which take their argument in revolutions, do the rest.
 
 // Inference computes the transposed products Y^T = W^T X^T: the weights are
 // the MFMA A operand (16 output units x 32 inputs) and the rows of the batch
@@ -88,14 +89,6 @@ __device__ __forceinline__ bf16x8 lds_wa(const __bf16 *w, int ld, int m, int j, 
     return *reinterpret_cast<const bf16x8 *>(w + (16 * m + (lane & 15)) * ld + 32 * j + (lane >> 4) * 8);
 }
 
-// Encoding of one row into layer 0's B operand: lane group g computes
-// octaves 3g .. 3g + 2 of all six inputs, octave 3g + d of input i at slot
-// s = 12 d + 2 i (sine) and s + 1 (cosine); slots 36..39 are zero. Octave 3g
-// comes from the hardware sin / cos of the exactly reduced angle, the next two
-// from the double-angle formulas sin 2a = 2 sin a cos a and cos 2a =
-// (cos a - sin a)(cos a + sin a): their error, a few float ulps, is far below
-// the bf16 rounding of the operand, and a row costs 12 transcendentals per
-// lane instead of 40.
 // The inputs of a row come from a RowSource: rows of a float array, or the
 // proposal inputs of path slots gathered through a row -> slot list.
 struct RowsDirect {
@@ -121,8 +114,16 @@ struct RowsGather {
     }
 };
 
-// Rows past n (n >= 1) encode row n - 1's inputs: inference stores no output
-// for them and training gives them weight 0, so their gradient is zero.
+// Encoding of one row into layer 0's B operand: lane group g computes
+// octaves 3g .. 3g + 2 of all six inputs, octave 3g + d of input i at slot
+// s = 12 d + 2 i (sine) and s + 1 (cosine); slots 36..39 are zero. Octave 3g
+// comes from the hardware sin / cos of the exactly reduced angle, the next two
+// from the double-angle formulas sin 2a = 2 sin a cos a and cos 2a =
+// cos^2 a - sin^2 a (one fused multiply-add): their error, a few float ulps,
+// is far below the bf16 rounding of the operand, and a row costs 12
+// transcendentals per lane instead of 40. Rows past n (n >= 1) encode row
+// n - 1's inputs: inference stores no output for them and training gives them
+// weight 0, so their gradient is zero.
 template <typename Src>
 __device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t n, int g, bf16x8 *b) {
     float v[6];
@@ -132,14 +133,14 @@ __device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t 
     for (int i = 0; i < 6; ++i) {
         const float x = v[i];
         const float r = ldexpf(x, 3 * g - 1);
-        const float rev = r - floorf(r);
+        const float rev = __builtin_amdgcn_fractf(r);
         float sv = __builtin_amdgcn_sinf(rev), cv = __builtin_amdgcn_cosf(rev);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (d > 0) {
-                const float s2 = 2.f * sv * cv;
-                cv = (cv - sv) * (cv + sv);
-                sv = s2;
+                const float p = sv * cv;
+                cv = fmaf(cv, cv, -(sv * sv));
+                sv = p + p;
             }
             e[12 * d + 2 * i] = sv;
             e[12 * d + 2 * i + 1] = cv;
@@ -526,7 +527,7 @@ __device__ __forceinline__ bf16x8 enc_a(const float (*tin)[6], int it, int lane)
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const float r = ldexpf(tin[8 * grp + e][i], o - 1);
-        const float rev = r - floorf(r);
+        const float rev = __builtin_amdgcn_fractf(r);
         a[e] = to_bf16(__builtin_amdgcn_sinf(rev + quarter));
     }
     return a;
