@@ -662,25 +662,41 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         // K padded to 64) and transposed for the weight gradients
         bf16x8 g3[2][2], gz[2][2], gt[4];
         floatx4 acc[2][4];
+        // (the eight outputs of a k-block read as two 16-byte loads; outputs
+        // 48..63 are the padding)
 #pragma unroll
         for (int nn = 0; nn < 2; ++nn)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int k = 32 * j + 8 * grp + e;
-                    g3[nn][j][e] = to_bf16(k < 48 ? S.yg.y[16 * nn + r16][k] : 0.f);
+            for (int j = 0; j < 2; ++j) {
+                const int k0 = 32 * j + 8 * grp;
+                float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+                if (k0 < 48) {
+                    const float4 *src = reinterpret_cast<const float4 *>(&S.yg.y[16 * nn + r16][k0]);
+                    a = src[0];
+                    b = src[1];
                 }
+                g3[nn][j][0] = to_bf16(a.x);
+                g3[nn][j][1] = to_bf16(a.y);
+                g3[nn][j][2] = to_bf16(a.z);
+                g3[nn][j][3] = to_bf16(a.w);
+                g3[nn][j][4] = to_bf16(b.x);
+                g3[nn][j][5] = to_bf16(b.y);
+                g3[nn][j][6] = to_bf16(b.z);
+                g3[nn][j][7] = to_bf16(b.w);
+            }
+        // G_3^T: lane (part p, unit u) gathers rows 8 p .. 8 p + 7 of output u;
+        // a 16-lane group takes one p and 16 consecutive units, so its reads
+        // fall in 16 consecutive banks
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = lane + 64 * q, unit = c / 4, part = c % 4;
+            const int unit = (lane & 15) + 16 * q, part = (lane >> 4) & 3;
 #pragma unroll
             for (int e = 0; e < 8; ++e) gt[q][e] = to_bf16(unit < 48 ? S.yg.y[part * 8 + e][unit] : 0.f);
         }
         wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = lane + 64 * q, unit = c / 4, part = c % 4;
+            const int unit = (lane & 15) + 16 * q, part = (lane >> 4) & 3;
             *reinterpret_cast<bf16x8 *>(S.yg.g + unit * kStageLd + part * 8) = gt[q];
         }
         // Each G_{l-1} = (W_l G_l^T) * relu'(h_l) needs only this wave's G_l,
