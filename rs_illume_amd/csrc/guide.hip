@@ -123,11 +123,9 @@ struct RowsGather {
 // is far below the bf16 rounding of the operand, and a row costs 12
 // transcendentals per lane instead of 40. Rows past n (n >= 1) encode row
 // n - 1's inputs: inference stores no output for them and training gives them
-// weight 0, so their gradient is zero.
-template <typename Src>
-__device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t n, int g, bf16x8 *b) {
-    float v[6];
-    src.load(row < n ? row : n - 1, v);
+// weight 0, so their gradient is zero. encode_v encodes a row from its six
+// inputs v, encode_b loads them first.
+__device__ __forceinline__ void encode_v(const float *v, int g, bf16x8 *b) {
     float e[40];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -150,6 +148,13 @@ __device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t 
     for (int q = 36; q < 40; ++q) e[q] = 0.f;
 #pragma unroll
     for (int q = 0; q < 40; ++q) b[q / 8][q % 8] = to_bf16(e[q]);
+}
+
+template <typename Src>
+__device__ __forceinline__ void encode_b(const Src &src, uint32_t row, uint32_t n, int g, bf16x8 *b) {
+    float v[6];
+    src.load(row < n ? row : n - 1, v);
+    encode_v(v, g, b);
 }
 
 // acc (output tiles of one 16-row column, bias included) -> ReLU, bf16 -> the
@@ -551,6 +556,30 @@ __device__ __forceinline__ void put_dw(const GuideDev &g, float *prow, int l, in
     }
 }
 
+// A round's record data for one lane of a wave's 32-row tile at row base:
+// the sample weights of row base + (lane % 32) (zero past n), the inputs of
+// row base + lane for the staged copy (lanes < 32; zero past n), and the
+// encoding's inputs of rows base + r16 and base + 16 + r16 (past n: row n - 1).
+struct RoundIn {
+    float w[4];
+    float in[6];
+    float v[2][6];
+};
+__device__ __forceinline__ void load_round(const float *__restrict__ rec, uint32_t n, uint32_t stride_f, uint32_t base,
+                                           int lane, RoundIn &ri) {
+    const uint32_t rw = base + (uint32_t)(lane & 31);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ri.w[c] = rw < n ? rec[(size_t)rw * stride_f + 6 + c] : 0.f;
+    const uint32_t ri_row = base + (uint32_t)lane;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) ri.in[i] = (lane < 32 && ri_row < n) ? rec[(size_t)ri_row * stride_f + i] : 0.f;
+#pragma unroll
+    for (int nn = 0; nn < 2; ++nn) {
+        const uint32_t row = base + 16 * nn + (uint32_t)(lane & 15);
+        RowsDirect{rec, stride_f}.load(row < n ? row : n - 1, ri.v[nn]);
+    }
+}
+
 // Block b trains on tiles [b * per_block, (b + 1) * per_block) of the n
 // records (record r at rec + stride_f * r floats); partial row b gets its
 // gradient (rows weighted by w, not yet normalised), wl[2 b], wl[2 b + 1] its
@@ -567,9 +596,13 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
     __shared__ TrainLds L;
     __shared__ float red[2][kTrainWaves];
     const uint32_t t_begin = blockIdx.x * per_block, t_end = t_begin + per_block;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);  // uniform: scalar registers
+    // the first round's record data, loaded before the weights are staged so
+    // that its latency overlaps the staging
+    RoundIn first;
+    load_round(rec, n, stride_f, (t_begin + wave) * 32, (int)threadIdx.x % kWave, first);
     stage_to_lds<kTrainThreads, (int)sizeof(GuideWeightImage)>(&L.w, g.img);
     __syncthreads();
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);  // uniform: scalar registers
     TrainWaveLds &S = L.wave[wave];
     const int itw = wave % 4, uo = wave / 4;  // layers 1..3: input tile, first output tile
     const bool bias_hi = itw == 0;            // sums the bias gradients of its output tiles (layers 1..3)
@@ -596,29 +629,25 @@ __global__ __launch_bounds__(kTrainThreads, 1) void guide_train_kernel(GuideDev 
         __asm__ volatile("" : "+v"(lane));
         const int grp = lane >> 4, r16 = lane & 15;
         const uint32_t base = (t0 + wave) * 32;
+        RoundIn ri;
+        if (t0 == t_begin)
+            ri = first;
+        else
+            load_round(rec, n, stride_f, base, lane, ri);
         // the rows' sampled directions and weights, needed after the forward
         // pass (lanes r and r + 32: row r)
-        float w_row[4] = {0.f, 0.f, 0.f, 0.f};
-        if (base + (lane & 31) < n) {
-            const float *r = rec + (size_t)(base + (lane & 31)) * stride_f;
-            w_row[0] = r[6];
-            w_row[1] = r[7];
-            w_row[2] = r[8];
-            w_row[3] = r[9];
-        }
+        const float *w_row = ri.w;
         // ---- forward (rows past n repeat row n - 1 with weight 0: their gradient is zero)
-        if (lane < 32) {
-            const uint32_t row = base + lane;
+        if (lane < 32)
 #pragma unroll
-            for (int i = 0; i < 6; ++i) S.in[lane][i] = row < n ? rec[(size_t)row * stride_f + i] : 0.f;
-        }
+            for (int i = 0; i < 6; ++i) S.in[lane][i] = ri.in[i];
         uint32_t mask1, mask2, mask3;
         bf16x8 h1[2][2], h2[2][2];  // X_1, X_2 until their weight gradients
         {
             bf16x8 x0[2][5], h[2][2];
             floatx4 acc[2][4];
-            encode_b(RowsDirect{rec, stride_f}, base + r16, n, grp, x0[0]);
-            encode_b(RowsDirect{rec, stride_f}, base + 16 + r16, n, grp, x0[1]);
+            encode_v(ri.v[0], grp, x0[0]);
+            encode_v(ri.v[1], grp, x0[1]);
             layer_t<5, 4, true>(L.w.w0, 168, x0, acc, lane, L.w.bias[0]);
             relu_to_b<4>(acc[0], h1[0]);
             relu_to_b<4>(acc[1], h1[1]);
