@@ -65,12 +65,14 @@ __global__ __launch_bounds__(kBlock, TraverseShape<WIDTH>::kWaves) void traverse
 // at least kRefillMin lanes of the wave are idle; then those lanes write their
 // results and take the run's next rays (in lane order), so a wave's trip count
 // follows its rays' total work instead of each group of 64 rays' slowest ray.
-// The grid is one wave per wave slot of the device. Waiting for a third of
-// the wave to be idle amortises the refill (ray load, reciprocal direction,
+// The grid is one wave per wave slot of the device. Waiting for three eighths
+// of the wave to be idle amortises the refill (ray load, reciprocal direction,
 // shear, result write) over many lanes: on the 2^20-triangle / 2^24-ray
-// workload 20 idle lanes gave 7.31 ms per launch, 16 7.35, 24 7.33, 12 7.45
-// and 8 7.61 (closest hit; any hit 4.50, 4.54, 4.49, 4.65, 4.78).
-constexpr int kRefillMin = 20;
+// workload 24 idle lanes gave 7.08 ms per launch, 20 7.16, 28 7.11, 32 7.19
+// and 16 7.28 (closest hit; any hit 4.35, 4.39, 4.34, 4.36, 4.49). Before
+// the four-child visit chose its near and far planes by the direction's sign
+// (a cheaper node step) the best was 20.
+constexpr int kRefillMin = 24;
 
 // Load balance: the static runs cover 1/kStaticDiv of the batch (an equal run
 // per wave); the rest is handed out kQueueRun rays at a time from a counter in
