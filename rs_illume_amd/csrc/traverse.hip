@@ -81,11 +81,15 @@ constexpr int kRefillMin = 24;
 // spread: on the 2^20-triangle / 2^24-ray workload closest hit 8.65 -> 7.39 ms
 // per launch and any hit 5.23 -> 4.56 ms (static fractions 0.4-0.55 within 1%
 // of each other, 0.7 at 7.98 ms, all-static at 8.65; runs of 128 or 256 rays
-// 1-2% slower than 64).
+// 1-2% slower than 64; with the current node visit a static quarter times the
+// same as a static half, 7.155 against 7.150 ms).
 constexpr uint32_t kStaticDiv = 2;
 constexpr uint32_t kQueueRun = 64;  // at least one wave, so one run feeds every idle lane
 
-constexpr int kRefillWaves = 8;  // waves per SIMD: at most 64 VGPRs, no spills
+// Waves per SIMD: at most 64 VGPRs, no spills. Fewer waves with more registers
+// lose latency hiding: 7 waves (72 VGPRs) 7.21 ms and 6 waves (80) 7.60 ms per
+// closest-hit launch against 7.15 ms at 8 on the 2^24-ray workload.
+constexpr int kRefillWaves = 8;
 
 template <bool ANY, int WIDTH>
 __global__ __launch_bounds__(64, kRefillWaves) void traverse_refill_kernel(
