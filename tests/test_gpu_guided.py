@@ -26,13 +26,24 @@ def _glossy():
     return S.cornell_box(0, glossy=True, light_size=0.2, light_L=26.5625)
 
 
+def _terrain_camera(w, h):
+    """Looks down onto the terrain scenes' [-1, 1]^2 heightfield."""
+    return S.camera((1.6, 1.0, 1.2), (0, 0.1, 0), (0, 1, 0), 45.0, w, h)
+
+
 def test_wavefront_equals_megakernel(gpu):
     import torch
     from rs_illume_amd.render import PathIntegrator, Scene, WavefrontPathIntegrator
     _say("case wavefront == megakernel")
-    for desc, depth in ((S.cornell_box(0), 5), (_glossy(), 8), (S.terrain_scene(0, n=24, n_spheres=40), 6)):
+    # the last scene has 2 * 47^2 + 200 = 4,618 primitives: at 4,096 or more the
+    # extend stage runs its lane-refilling kernel instead of one path per thread
+    big = S.terrain_scene(1, n=48, n_spheres=200)
+    assert len(big.idx) + len(big.spheres) >= 4096
+    box_cam, terrain_cam = S.cornell_camera(48, 48), _terrain_camera(48, 48)
+    for desc, depth, cam in ((S.cornell_box(0), 5, box_cam), (_glossy(), 8, box_cam),
+                             (S.terrain_scene(0, n=24, n_spheres=40), 6, terrain_cam), (big, 6, terrain_cam)):
+        _say("scene", len(desc.idx), "triangles", len(desc.spheres), "spheres, depth", depth)
         sc = Scene(desc)
-        cam = S.cornell_camera(48, 48)
         mk_sum = PathIntegrator(cam, spp=8, max_depth=depth, seed=5).render_sum(sc)
         wf = WavefrontPathIntegrator(sc, cam, spp=8, max_depth=depth, seed=5, max_paths=5000)
         c = torch.zeros(2, dtype=torch.int64, device="cuda")
@@ -55,12 +66,17 @@ def test_batched_jobs_equal_individual_renders(gpu):
     scenes = [Scene(d) for d in descs]
     cams = [S.cornell_camera(40, 40), S.cornell_camera(24, 56), S.cornell_camera(64, 32)]
     jobs = [(scenes[0], cams[0], 6, 5, 1), (scenes[1], cams[1], 3, 8, 2), (scenes[2], cams[2], 9, 8, 3)]
-    for graph_iters, block, regen_min in ((8, 128, None), (0, 64, 700), (3, 256, 1)):
+    # with a job of 4,096 or more primitives the whole batch extends through
+    # the lane-refilling kernel
+    big = Scene(S.terrain_scene(1, n=48, n_spheres=200))
+    jobs_big = jobs + [(big, _terrain_camera(32, 40), 4, 6, 4)]
+    for graph_iters, block, regen_min, js in ((8, 128, None, jobs), (0, 64, 700, jobs), (3, 256, 1, jobs),
+                                              (8, 128, None, jobs_big), (0, 64, 1, jobs_big)):
         wf = Wavefront(max_paths=4096, max_depth=8, block=block, regen_min=regen_min, graph_iters=graph_iters)
-        film, offs = wf.render_batch(jobs)
+        film, offs = wf.render_batch(js)
         film = film.cpu().numpy()
-        _say("graph_iters", graph_iters, "block", block, "iterations", wf.last_iterations())
-        for (sc, cam, spp, depth, seed), off in zip(jobs, offs):
+        _say("graph_iters", graph_iters, "block", block, "jobs", len(js), "iterations", wf.last_iterations())
+        for (sc, cam, spp, depth, seed), off in zip(js, offs):
             w, h = int(cam["width"][0]), int(cam["height"][0])
             ref = PathIntegrator(cam, spp=spp, max_depth=depth, seed=seed).render_sum(sc).cpu().numpy()
             got = film[off:off + w * h * 3].reshape(h, w, 3)
