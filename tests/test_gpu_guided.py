@@ -84,12 +84,18 @@ def test_batched_jobs_equal_individual_renders(gpu):
 
 
 @pytest.mark.parametrize("product", [False, True])
-def test_nan_proposal_falls_back_to_plain(gpu, product):
+@pytest.mark.parametrize("scene", ["box", "terrain"])
+def test_nan_proposal_falls_back_to_plain(gpu, product, scene):
+    """The terrain scene (4,618 primitives) takes the lane-refilling extend
+    kernel, which forms the guided vertices' proposal inputs in its second
+    pass."""
     from rs_illume_amd.guide import Guide, N_PARAMS
     from rs_illume_amd.render import Scene, WavefrontPathIntegrator
-    _say("case NaN proposal, product", product)
-    sc = Scene(_glossy())
-    cam = S.cornell_camera(32, 32)
+    _say("case NaN proposal, product", product, "scene", scene)
+    if scene == "box":
+        sc, cam = Scene(_glossy()), S.cornell_camera(32, 32)
+    else:
+        sc, cam = Scene(S.terrain_scene(1, n=48, n_spheres=200)), _terrain_camera(32, 32)
     plain = WavefrontPathIntegrator(sc, cam, spp=4, max_depth=8, seed=2).render_sum().cpu().numpy()
     g = Guide(np.full(N_PARAMS, np.nan, np.float32))
     guided = WavefrontPathIntegrator(sc, cam, spp=4, max_depth=8, seed=2, guide=g, guide_frac=0.5,
@@ -99,16 +105,19 @@ def test_nan_proposal_falls_back_to_plain(gpu, product):
 
 
 @pytest.mark.parametrize("product", [False, True])
-def test_guided_render_is_unbiased(gpu, product):
+@pytest.mark.parametrize("scene", ["box", "terrain"])
+def test_guided_render_is_unbiased(gpu, product, scene):
     """An untrained proposal (an arbitrary mixture per vertex), with or
     without the product with the BSDF lobe, at fraction 0.5: the image mean
     equals the plain tracer's, so the pdf the estimator divides by is the
     density the directions were drawn from."""
     from rs_illume_amd.guide import Guide
     from rs_illume_amd.render import Scene, WavefrontPathIntegrator
-    _say("case guided unbiased, product", product)
-    sc = Scene(_glossy())
-    cam = S.cornell_camera(32, 32)
+    _say("case guided unbiased, product", product, "scene", scene)
+    if scene == "box":
+        sc, cam = Scene(_glossy()), S.cornell_camera(32, 32)
+    else:
+        sc, cam = Scene(S.terrain_scene(1, n=48, n_spheres=200)), _terrain_camera(32, 32)
     spp = 1024
     plain = WavefrontPathIntegrator(sc, cam, spp=spp, max_depth=8, seed=11).render().cpu().numpy()
     g = Guide(seed=4)
