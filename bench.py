@@ -642,8 +642,8 @@ def main():
                          "device LBVH")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rays shard with no exchange; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (rays would shard across GPUs with no "
+              "exchange, each GPU holding the whole tree). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     # hardware counters of the headline kernel, from child rocprofv3 passes

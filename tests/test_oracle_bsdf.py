@@ -2,7 +2,7 @@
 their formulas.
 
 The reference's Trowbridge-Reitz conductor and FresnelSpecular are wrong in
-ways only independent checks catch (SURVEY Appendix A): its microfacet pdf is
+ways only independent checks catch: its microfacet pdf is
 zero exactly where it should not be (microfacet_reflection.rs:79-82), its
 Lambda uses tan^2 (trowbridge_reitz_distribution.rs:58), its visible-normal
 and classic samplers are swapped (trowbridge_reitz_distribution.rs:71-101),
