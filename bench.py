@@ -143,7 +143,7 @@ def cpu_baseline(v, idx, rays, budget_s=15.0):
 
 
 def bench_cornell(no_cpu):
-    """Config-0 Cornell box on the megakernel path tracer; the oracle renders
+    """The Cornell box (256x256, 16 spp, depth 5) on the megakernel path tracer; the oracle renders
     a bounded number of samples per pixel on host cores."""
     import torch
     from rs_illume_amd import scenes as S
@@ -234,7 +234,8 @@ def bench_cornell(no_cpu):
 
 def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32, plain="megakernel", no_cpu=False,
                  cpu_budget_s=20.0, guide_frac=0.25, product=True):
-    """Config-3 equal-quality comparison (rs_illume_amd.converge); the CPU
+    """Glossy small-light Cornell box (512x512, depth 8): time to relMSE 0.01,
+    guided against plain (rs_illume_amd.converge); the CPU
     leg runs the oracle's path tracer on host cores in 2-spp passes for a fixed
     wall budget against the same reference (time to relMSE 0.01 and the relMSE
     at the budget)."""
@@ -283,7 +284,7 @@ def bench_guided(ref_spp=16384, res=512, log=log, train_steps=4, warmup_steps=32
 
 def bench_guided_indirect(ref_spp=16384, res=512, log=log, spp_per_pass=16, warmup_steps=16, train_steps=8,
                           train_budget=256, guide_frac=0.5):
-    """The proposal where guiding pays: config 0's box with its light turned
+    """The proposal where guiding pays: the Cornell box with its light turned
     over (facing the ceiling, L = 60), so every surface but the ceiling is lit
     only indirectly, 512x512, max depth 8. Time to relMSE 0.01 against a
     16384-spp plain render, plain megakernel vs the guided wavefront tracer
@@ -328,7 +329,7 @@ def bench_guided_indirect(ref_spp=16384, res=512, log=log, spp_per_pass=16, warm
 
 
 def bench_batched(no_cpu, log=log, builder="SAH_GPU", pmc=None):
-    """Config 2: 64 procedural scenes (seeds 1-64) in one batched call, twice:
+    """Batched workload: 64 procedural scenes (seeds 1-64) in one batched call, twice:
     through one wavefront pool (launch shape chosen by the autotuner on a 2-spp
     probe of the same batch) and through the megakernel batch
     (illume_render_batch); the faster is the config's figure. The oracle
@@ -449,7 +450,7 @@ def bench_batched(no_cpu, log=log, builder="SAH_GPU", pmc=None):
 
 def bench_combined(ref_spp=4096, spp_pass=2, max_spp=256, target=0.01, log=log, train_steps=4, warmup_steps=32,
                    builder="SAH_GPU", no_cpu=False, cpu_budget_s=20.0, guide_frac=0.25):
-    """Config 4: fBm terrain (2M triangles) + 10k spheres under the analytic
+    """Combined workload: fBm terrain (2M triangles) + 10k spheres under the analytic
     sky, 16 views of uneven size in one batched call per pass, max depth 10.
     Plain (on the megakernel batch and on the wavefront pool; the faster is
     the baseline) and guided (wavefront pool; a proposal network per view,
